@@ -1,0 +1,131 @@
+/*
+ * pe_amd.h -- C ABI of the MI355X (gfx950) rigid-body step.
+ *
+ * One world's step is the reference engine's World::step pipeline
+ * (src/intf/world.cpp:267-300): integrate + gravity, sweep-and-prune broadphase,
+ * narrowphase with the 8-slot contact cache, sequential-impulse friction-contact solve.
+ * Every entry point returns 0 on success or a negative PE_ERR_* code; the message of the
+ * last error on the calling thread is available from pe_last_error().  No C++ exception
+ * crosses this boundary.  Handles are opaque; a handle may be used by one host thread at
+ * a time ("NOT thread-safe", as src/intf/world.h:21 says of World).
+ *
+ * All device state is FP32 structure-of-arrays and stays resident in HBM between calls.
+ */
+#ifndef PE_AMD_H
+#define PE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes */
+#define PE_OK               0
+#define PE_ERR_INVALID    (-1)  /* bad argument (null pointer, size out of range) */
+#define PE_ERR_DEVICE     (-2)  /* HIP runtime error */
+#define PE_ERR_CAPACITY   (-3)  /* a fixed device capacity (pairs, contacts) overflowed */
+#define PE_ERR_UNSUPPORTED (-4) /* shape pair or feature not available */
+#define PE_ERR_BUSY       (-5)  /* server: no slot or HBM budget for this request now */
+#define PE_ERR_NOT_FOUND  (-6)  /* server: unknown request id */
+
+/* ------------------------------------------------------------------ shapes
+ * Values of pe_phys_shape::ShapeType (src/rigid/shape/shape.h:7-9). */
+#define PE_SHAPE_BOX       0
+#define PE_SHAPE_SPHERE    1
+#define PE_SHAPE_CYLINDER  2
+#define PE_SHAPE_CONVEX    3
+#define PE_SHAPE_CONCAVE   4
+
+/* Scene description (host memory, structure of arrays, FP64 so every consumer rounds the
+ * same value to its own precision).  Mirrors what the reference sets on a RigidBody before
+ * World::addRigidBody (src/rigid/object/rigidbody.h): setMass, setCollisionShape,
+ * setTransform, setLinearVelocity, setAngularVelocity, setFrictionCoeff,
+ * setRestitutionCoeff, setLinearDamping, setAngularDamping, setKinematic. */
+typedef struct pe_scene_desc {
+    int32_t n;                        /* number of bodies; body i is the i-th addRigidBody */
+    const int32_t* shape_type;        /* [n] PE_SHAPE_* */
+    const double* shape_param;        /* [4n] box: size x,y,z (BoxShape(size)); sphere: radius;
+                                         cylinder: radius, height */
+    const double* mass;               /* [n] */
+    const double* position;           /* [3n] transform origin */
+    const double* rotation;           /* [9n] transform basis, row-major */
+    const double* linear_velocity;    /* [3n] */
+    const double* angular_velocity;   /* [3n] */
+    const double* friction;           /* [n] */
+    const double* restitution;        /* [n] */
+    const double* linear_damping;     /* [n] or NULL (0) */
+    const double* angular_damping;    /* [n] or NULL (0) */
+    const int32_t* kinematic;         /* [n] 0/1 */
+} pe_scene_desc;
+
+/* World settings: World::setDt / setGravity (src/intf/world.h:24-25) and
+ * SequentialImpulseSolver::setIteration (sequential_impulse_solver.cpp:8). */
+typedef struct pe_world_params {
+    double dt;
+    double gravity[3];
+    int32_t iterations;
+} pe_world_params;
+
+/* Layout of one contact point as returned by pe_world_get_contacts (13 floats):
+ * world_pos[3] (on B), world_normal[3] (B -> A), local_pos_a[3], local_pos_b[3], distance
+ * -- the fields of pe_phys_collision::ContactPoint (contact_result.h:10-74). */
+#define PE_CONTACT_FLOATS 13
+#define PE_CONTACT_SLOTS  8   /* PE_CONTACT_CACHE_SIZE, contact_result.h:6 */
+
+const char* pe_last_error(void);
+int pe_device_count(int32_t* count);
+
+/* ------------------------------------------------------------------ plain single world */
+typedef struct pe_world pe_world;
+
+/* Creates a device-resident world.  max_pairs bounds the broadphase output (0 = 16 * n). */
+int pe_world_create(const pe_scene_desc* scene, const pe_world_params* params,
+                    int32_t max_pairs, pe_world** out);
+int pe_world_destroy(pe_world* w);
+
+/* World::step (src/intf/world.cpp:267-300), nsteps times, on the world's stream. */
+int pe_world_step(pe_world* w, int32_t nsteps);
+
+/* Stage entry points; one call of each, in this order, is one World::step. */
+/* World::updateObjectStatus + applyExternalForce (world.cpp:35-127) */
+int pe_world_integrate(pe_world* w);
+/* BroadPhaseBase::calcCollisionPairs, sweep-and-prune (broad_phase_sweep_and_prune.cpp:6-41) */
+int pe_world_broadphase(pe_world* w, int32_t* npairs);
+/* NarrowPhaseBase::calcContactResults (simple_narrow_phase.cpp:7-63) */
+int pe_world_narrowphase(pe_world* w);
+/* ConstraintSolver::setupSolver + solve (sequential_impulse_solver.cpp:11-81) */
+int pe_world_solve(pe_world* w);
+
+/* Synchronises the world's stream. */
+int pe_world_sync(pe_world* w);
+
+/* State access (FP32, device precision).  Any pointer may be NULL. */
+int pe_world_get_state(pe_world* w, float* position /*3n*/, float* rotation /*9n row-major*/,
+                       float* linear_velocity /*3n*/, float* angular_velocity /*3n*/);
+int pe_world_set_state(pe_world* w, const float* position, const float* rotation,
+                       const float* linear_velocity, const float* angular_velocity);
+int pe_world_get_aabbs(pe_world* w, float* aabb_min /*3n*/, float* aabb_max /*3n*/);
+/* Pairs of the last broadphase in emission order: (A, B) body indices. */
+int pe_world_get_pairs(pe_world* w, int32_t* pairs /*2*cap*/, int32_t cap, int32_t* npairs);
+/* Contact results of the last narrowphase, one per pair: point count, swap flag and the
+ * 8 slots in cache order (PE_CONTACT_FLOATS floats each). */
+int pe_world_get_contacts(pe_world* w, int32_t* npoints /*cap*/, float* points /*cap*8*13*/,
+                          int32_t cap);
+/* Solve order of the last solve: pair indices grouped by colour, colour start offsets. */
+int pe_world_get_solve_order(pe_world* w, int32_t* order /*cap*/, int32_t cap, int32_t* nordered,
+                             int32_t* color_start /*ccap+1*/, int32_t ccap, int32_t* ncolors);
+/* SAP axis that the next broadphase sorts on (BroadPhaseSweepAndPrune::_target_axis). */
+int pe_world_get_axis(pe_world* w, int32_t* axis);
+int pe_world_set_axis(pe_world* w, int32_t axis);
+/* Cumulative device time per stage in ms: integrate, broadphase, narrowphase, solver --
+ * the four timers of World (world.h:58-61).  enable = 1 records hipEvents around stages. */
+int pe_world_set_timing(pe_world* w, int32_t enable);
+int pe_world_get_stage_times(pe_world* w, double* ms /*4*/);
+/* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours. */
+int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PE_AMD_H */

@@ -1,0 +1,177 @@
+// CPU ORACLE -- test infrastructure only (see pe_oracle.hpp).
+// C entry points over the restatement, in FP32 (peo32_*) and FP64 (peo64_*).
+#include "pe_oracle.hpp"
+#include "../include/pe_amd.h"
+#include <cstdio>
+
+using namespace pe_oracle;
+
+namespace {
+template <class R> struct Handle {
+    World<R> w;
+    Solver<R> s;
+};
+
+template <class R> Handle<R>* create(const pe_scene_desc* sc, const pe_world_params* prm) {
+    auto* h = new Handle<R>();
+    World<R>& w = h->w;
+    w.dt = R(prm->dt);
+    w.gravity = V3<R>(R(prm->gravity[0]), R(prm->gravity[1]), R(prm->gravity[2]));
+    w.iterations = prm->iterations;
+    w.bodies.resize(sc->n);
+    for (int i = 0; i < sc->n; i++) {
+        Body<R>& b = w.bodies[i];
+        b.type = sc->shape_type[i];
+        const double* sp = sc->shape_param + 4 * i;
+        if (b.type == ST_BOX) {
+            for (int k = 0; k < 3; k++) { b.size[k] = R(sp[k]); b.half[k] = b.size[k] / R(2); }
+        } else if (b.type == ST_SPHERE) {
+            b.radius = R(sp[0]);
+        } else if (b.type == ST_CYLINDER) {
+            b.radius = R(sp[0]); b.height = R(sp[1]);
+        }
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) b.xf.basis(r, c) = R(sc->rotation[9 * i + 3 * r + c]);
+        for (int k = 0; k < 3; k++) {
+            b.xf.origin[k] = R(sc->position[3 * i + k]);
+            b.v[k] = R(sc->linear_velocity[3 * i + k]);
+            b.w[k] = R(sc->angular_velocity[3 * i + k]);
+        }
+        body_set_mass_props(b, R(sc->mass[i]));
+        body_update_world_inertia(b);
+        b.mu = R(sc->friction[i]);
+        b.e = R(sc->restitution[i]);
+        b.lin_damp = sc->linear_damping ? R(sc->linear_damping[i]) : R(0);
+        b.ang_damp = sc->angular_damping ? R(sc->angular_damping[i]) : R(0);
+        b.kinematic = sc->kinematic[i] != 0;
+    }
+    return h;
+}
+
+template <class R> void get_state(Handle<R>* h, double* pos, double* rot, double* v, double* w) {
+    for (size_t i = 0; i < h->w.bodies.size(); i++) {
+        const Body<R>& b = h->w.bodies[i];
+        for (int k = 0; k < 3; k++) {
+            if (pos) pos[3 * i + k] = b.xf.origin[k];
+            if (v) v[3 * i + k] = b.v[k];
+            if (w) w[3 * i + k] = b.w[k];
+        }
+        if (rot) for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) rot[9 * i + 3 * r + c] = b.xf.basis(r, c);
+    }
+}
+template <class R> void set_state(Handle<R>* h, const double* pos, const double* rot, const double* v, const double* w) {
+    for (size_t i = 0; i < h->w.bodies.size(); i++) {
+        Body<R>& b = h->w.bodies[i];
+        for (int k = 0; k < 3; k++) {
+            if (pos) b.xf.origin[k] = R(pos[3 * i + k]);
+            if (v) b.v[k] = R(v[3 * i + k]);
+            if (w) b.w[k] = R(w[3 * i + k]);
+        }
+        if (rot) for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) b.xf.basis(r, c) = R(rot[9 * i + 3 * r + c]);
+        body_update_world_inertia(b);
+    }
+}
+template <class R> void get_contacts(Handle<R>* h, int32_t* npts, int32_t* swap, double* pts) {
+    const auto& res = h->w.results;
+    for (size_t i = 0; i < res.size(); i++) {
+        if (npts) npts[i] = res[i].npts;
+        if (swap) swap[i] = res[i].swap;
+        if (!pts) continue;
+        for (int k = 0; k < 8; k++) {
+            const ContactPoint<R>& cp = res[i].pts[k];
+            double* o = pts + (i * 8 + k) * 13;
+            for (int c = 0; c < 3; c++) {
+                o[c] = cp.wpos[c]; o[3 + c] = cp.normal[c]; o[6 + c] = cp.la[c]; o[9 + c] = cp.lb[c];
+            }
+            o[12] = cp.dist;
+        }
+    }
+}
+}  // namespace
+
+#define PEO_API(P, R)                                                                                     \
+    extern "C" void* P##_world_create(const pe_scene_desc* sc, const pe_world_params* prm) {             \
+        return create<R>(sc, prm);                                                                       \
+    }                                                                                                    \
+    extern "C" void P##_world_destroy(void* h) { delete (Handle<R>*)h; }                                \
+    extern "C" void P##_set_color_order(void* h, int on) { ((Handle<R>*)h)->w.use_color_order = on != 0; } \
+    extern "C" void P##_step(void* h, int nsteps) {                                                      \
+        auto* hh = (Handle<R>*)h;                                                                        \
+        for (int i = 0; i < nsteps; i++) world_step(hh->w, hh->s);                                       \
+    }                                                                                                    \
+    extern "C" void P##_integrate(void* h) { stage_integrate(((Handle<R>*)h)->w); }                     \
+    extern "C" int P##_broadphase(void* h) {                                                             \
+        stage_broadphase(((Handle<R>*)h)->w);                                                            \
+        return (int)((Handle<R>*)h)->w.pairs.size();                                                     \
+    }                                                                                                    \
+    extern "C" void P##_narrowphase(void* h) { stage_narrowphase(((Handle<R>*)h)->w); }                 \
+    extern "C" void P##_solve(void* h) {                                                                 \
+        auto* hh = (Handle<R>*)h;                                                                        \
+        stage_solver_setup(hh->w, hh->s);                                                                \
+        if (hh->w.use_color_order) compute_color_order(hh->w);                                           \
+        stage_solve(hh->w, hh->s);                                                                       \
+    }                                                                                                    \
+    extern "C" int P##_num_bodies(void* h) { return (int)((Handle<R>*)h)->w.bodies.size(); }            \
+    extern "C" int P##_num_pairs(void* h) { return (int)((Handle<R>*)h)->w.pairs.size(); }              \
+    extern "C" int P##_get_axis(void* h) { return ((Handle<R>*)h)->w.axis; }                            \
+    extern "C" void P##_set_axis(void* h, int a) { ((Handle<R>*)h)->w.axis = a; }                       \
+    extern "C" void P##_get_pairs(void* h, int32_t* ab) {                                                \
+        auto& p = ((Handle<R>*)h)->w.pairs;                                                              \
+        for (size_t i = 0; i < p.size(); i++) { ab[2 * i] = p[i].first; ab[2 * i + 1] = p[i].second; }   \
+    }                                                                                                    \
+    extern "C" void P##_set_pairs(void* h, int n, const int32_t* ab) {                                   \
+        auto& p = ((Handle<R>*)h)->w.pairs;                                                              \
+        p.resize(n);                                                                                     \
+        for (int i = 0; i < n; i++) p[i] = {ab[2 * i], ab[2 * i + 1]};                                   \
+    }                                                                                                    \
+    extern "C" void P##_get_state(void* h, double* pos, double* rot, double* v, double* w) {             \
+        get_state((Handle<R>*)h, pos, rot, v, w);                                                        \
+    }                                                                                                    \
+    extern "C" void P##_set_state(void* h, const double* pos, const double* rot, const double* v,        \
+                                  const double* w) {                                                     \
+        set_state((Handle<R>*)h, pos, rot, v, w);                                                        \
+    }                                                                                                    \
+    extern "C" void P##_get_aabbs(void* h, double* mn, double* mx) {                                     \
+        auto& bs = ((Handle<R>*)h)->w.bodies;                                                            \
+        for (size_t i = 0; i < bs.size(); i++)                                                           \
+            for (int k = 0; k < 3; k++) { mn[3 * i + k] = bs[i].aabb_min[k]; mx[3 * i + k] = bs[i].aabb_max[k]; } \
+    }                                                                                                    \
+    extern "C" void P##_get_world_inv_inertia(void* h, double* out) {                                    \
+        auto& bs = ((Handle<R>*)h)->w.bodies;                                                            \
+        for (size_t i = 0; i < bs.size(); i++)                                                           \
+            for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) out[9 * i + 3 * r + c] = bs[i].world_inv_inertia(r, c); \
+    }                                                                                                    \
+    extern "C" void P##_get_mass_props(void* h, double* inv_mass, double* local_inv_inertia) {           \
+        auto& bs = ((Handle<R>*)h)->w.bodies;                                                            \
+        for (size_t i = 0; i < bs.size(); i++) {                                                         \
+            inv_mass[i] = bs[i].inv_mass;                                                                \
+            for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++)                                      \
+                local_inv_inertia[9 * i + 3 * r + c] = bs[i].local_inv_inertia(r, c);                    \
+        }                                                                                                \
+    }                                                                                                    \
+    extern "C" int P##_num_results(void* h) { return (int)((Handle<R>*)h)->w.results.size(); }          \
+    extern "C" void P##_get_contacts(void* h, int32_t* npts, int32_t* swap, double* pts) {               \
+        get_contacts((Handle<R>*)h, npts, swap, pts);                                                    \
+    }                                                                                                    \
+    extern "C" int P##_num_colors(void* h) { return ((Handle<R>*)h)->w.ncolors; }                       \
+    extern "C" int P##_get_solve_order(void* h, int32_t* order) {                                        \
+        auto& o = ((Handle<R>*)h)->w.solve_order;                                                        \
+        for (size_t i = 0; i < o.size(); i++) order[i] = o[i];                                           \
+        return (int)o.size();                                                                            \
+    }                                                                                                    \
+    extern "C" void P##_get_color_of(void* h, int32_t* col) {                                            \
+        auto& c = ((Handle<R>*)h)->w.color_of;                                                           \
+        for (size_t i = 0; i < c.size(); i++) col[i] = c[i];                                             \
+    }                                                                                                    \
+    extern "C" void P##_get_impulses(void* h, double* out) { /* per result: 8 x (ln, lt0, lt1) */       \
+        auto* hh = (Handle<R>*)h;                                                                        \
+        for (size_t i = 0; i < hh->s.rows.size(); i++)                                                   \
+            for (size_t k = 0; k < 8; k++) {                                                             \
+                double* o = out + (i * 8 + k) * 3;                                                       \
+                if (k < hh->s.rows[i].size()) {                                                          \
+                    o[0] = hh->s.rows[i][k].ln; o[1] = hh->s.rows[i][k].lt0; o[2] = hh->s.rows[i][k].lt1; \
+                } else { o[0] = o[1] = o[2] = 0; }                                                       \
+            }                                                                                            \
+    }
+
+PEO_API(peo32, float)
+PEO_API(peo64, double)

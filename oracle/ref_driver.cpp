@@ -1,0 +1,246 @@
+// CPU ORACLE -- test infrastructure only.
+// A C driver over the reference engine itself (tcy2002/Physics-Engine), compiled from its
+// own sources by oracle/Makefile into oracle/_ref/.  It builds a pe_intf::World from a
+// pe_scene_desc exactly as the reference's demos do (examples/bomb_demo/main.cpp:84-117:
+// new RigidBody, setMass, setCollisionShape, setTransform, ...), and exposes each stage of
+// World::step (src/intf/world.cpp:267-300) so the restatement can be pinned stage by stage.
+// This file is ours; no reference source is copied into the repository.
+#include "intf/world.h"
+#include "rigid/object/rigidbody.h"
+#include "rigid/shape/box_shape.h"
+#include "rigid/shape/sphere_shape.h"
+#include "rigid/shape/cylinder_shape.h"
+#include "rigid/collision/broad_phase/broad_phase_sweep_and_prune.h"
+#include "rigid/constraint/constraint_solver/sequential_impulse_solver.h"
+// The pool's members are private; this test driver needs them to retire the workers at exit.
+#define private public
+#include "utils/thread_pool.h"
+#undef private
+#include "../include/pe_amd.h"
+#include <pthread.h>
+#include <cstdlib>
+#include <unordered_map>
+#include <chrono>
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+struct SAP : public pe_phys_collision::BroadPhaseSweepAndPrune {
+    int axis() const { return _target_axis; }
+    void set_axis(int a) { _target_axis = a; }
+};
+
+struct RefWorld : public pe_intf::World {
+    SAP* sap = nullptr;
+    std::unordered_map<const pe_phys_object::RigidBody*, int> index;
+    std::vector<pe_phys_object::RigidBody*> owned;
+    void integrate() {
+        updateObjectStatus();
+        calcDamageEffects();
+        applyExternalForce();
+    }
+    int broadphase() {
+        _broad_phase->calcCollisionPairs(_rigid_objects, _collision_pairs);
+        return (int)_collision_pairs.size();
+    }
+    void narrowphase() {
+        _narrow_phase->calcContactResults(_collision_pairs, _contact_results);
+        execCollisionCallbacks();
+    }
+    void solve() {
+        _constraint_solver->setupSolver(getDt(), getGravity(), _rigid_objects, _contact_results, _constraints);
+        _constraint_solver->solve();
+    }
+    pe::Array<pe_phys_collision::CollisionPair>& pairs() { return _collision_pairs; }
+    pe::Array<pe_phys_collision::ContactResult*>& results() { return _contact_results; }
+};
+
+pe::Matrix3 basis_of(const double* r) {
+    pe::Matrix3 m;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m(i, j) = pe::Real(r[3 * i + j]);
+    return m;
+}
+
+}  // namespace
+
+REF_API int pref_real_size() { return (int)sizeof(pe::Real); }
+
+// ThreadPool::init(n) before the first World fixes the pool size (thread_pool.cpp:10-14);
+// n = 0 keeps the reference default, std::thread::hardware_concurrency().
+//
+// The reference's pool workers wait on the pool's condition variable forever, and the pool is
+// a function-local static: at process exit its destructor destroys that condition variable
+// under the waiting workers, which blocks in glibc.  An exit hook registered after the pool
+// exists runs first and retires every worker with one task each.
+namespace {
+void retire_pool_workers() {
+    auto& inst = utils::ThreadPool::getInstance();
+    const uint32_t n = inst._size;
+    if (n == 0) return;
+    {
+        std::unique_lock<std::mutex> lock(inst._mtx);
+        for (uint32_t i = 0; i < n; i++) {
+            inst._tasks.emplace([] {
+                auto& in = utils::ThreadPool::getInstance();
+                {
+                    std::unique_lock<std::mutex> l(in._mtx);
+                    in._task_count--;
+                    in._cv.notify_all();
+                }
+                pthread_exit(nullptr);
+            });
+            inst._task_count++;
+        }
+    }
+    inst._cv.notify_all();
+    utils::ThreadPool::join();
+    inst._size = 0;
+}
+bool g_hook = false;
+}  // namespace
+REF_API void pref_init_threads(int n) {
+    utils::ThreadPool::init((uint32_t)(n > 0 ? n : 0));
+    if (!g_hook) { g_hook = true; std::atexit(retire_pool_workers); }
+}
+
+REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* prm) {
+    auto* w = new RefWorld();
+    w->setDt(pe::Real(prm->dt));
+    w->setGravity(pe::Vector3(pe::Real(prm->gravity[0]), pe::Real(prm->gravity[1]), pe::Real(prm->gravity[2])));
+    auto* solver = new pe_phys_constraint::SequentialImpulseSolver();
+    solver->setIteration(prm->iterations);
+    w->setConstraintSolver(solver);
+    w->sap = new SAP();
+    w->setBroadPhase(w->sap);
+    for (int i = 0; i < sc->n; i++) {
+        auto* rb = new pe_phys_object::RigidBody();
+        rb->setMass(pe::Real(sc->mass[i]));
+        const double* sp = sc->shape_param + 4 * i;
+        pe_phys_shape::Shape* shape = nullptr;
+        switch (sc->shape_type[i]) {
+            case PE_SHAPE_BOX:
+                shape = new pe_phys_shape::BoxShape(pe::Vector3(pe::Real(sp[0]), pe::Real(sp[1]), pe::Real(sp[2])));
+                break;
+            case PE_SHAPE_SPHERE: shape = new pe_phys_shape::SphereShape(pe::Real(sp[0])); break;
+            case PE_SHAPE_CYLINDER: shape = new pe_phys_shape::CylinderShape(pe::Real(sp[0]), pe::Real(sp[1])); break;
+            default: delete rb; delete w; return nullptr;
+        }
+        rb->setCollisionShape(shape);
+        pe::Vector3 o(pe::Real(sc->position[3 * i]), pe::Real(sc->position[3 * i + 1]), pe::Real(sc->position[3 * i + 2]));
+        rb->setTransform(pe::Transform(basis_of(sc->rotation + 9 * i), o));
+        rb->setLinearVelocity(pe::Vector3(pe::Real(sc->linear_velocity[3 * i]), pe::Real(sc->linear_velocity[3 * i + 1]),
+                                          pe::Real(sc->linear_velocity[3 * i + 2])));
+        rb->setAngularVelocity(pe::Vector3(pe::Real(sc->angular_velocity[3 * i]), pe::Real(sc->angular_velocity[3 * i + 1]),
+                                           pe::Real(sc->angular_velocity[3 * i + 2])));
+        rb->setFrictionCoeff(pe::Real(sc->friction[i]));
+        rb->setRestitutionCoeff(pe::Real(sc->restitution[i]));
+        rb->setLinearDamping(sc->linear_damping ? pe::Real(sc->linear_damping[i]) : pe::Real(0));
+        rb->setAngularDamping(sc->angular_damping ? pe::Real(sc->angular_damping[i]) : pe::Real(0));
+        rb->setKinematic(sc->kinematic[i] != 0);
+        w->addRigidBody(rb);
+        w->index[rb] = i;
+        w->owned.push_back(rb);
+    }
+    return w;
+}
+
+REF_API void pref_world_destroy(void* h) {
+    // Bodies, shapes and stage objects are leaked on purpose: World does not own bodies
+    // (world.cpp:28-33) and replaced stage objects are never freed by World either.
+    (void)h;
+}
+
+REF_API void pref_step(void* h, int nsteps) {
+    auto* w = (RefWorld*)h;
+    for (int i = 0; i < nsteps; i++) w->step();
+}
+// Wall-clock seconds of nsteps World::step calls.
+REF_API double pref_step_timed(void* h, int nsteps) {
+    auto* w = (RefWorld*)h;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < nsteps; i++) w->step();
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+REF_API void pref_integrate(void* h) { ((RefWorld*)h)->integrate(); }
+REF_API int pref_broadphase(void* h) { return ((RefWorld*)h)->broadphase(); }
+REF_API void pref_narrowphase(void* h) { ((RefWorld*)h)->narrowphase(); }
+REF_API void pref_solve(void* h) { ((RefWorld*)h)->solve(); }
+REF_API int pref_get_axis(void* h) { return ((RefWorld*)h)->sap->axis(); }
+REF_API void pref_set_axis(void* h, int a) { ((RefWorld*)h)->sap->set_axis(a); }
+REF_API int pref_num_pairs(void* h) { return (int)((RefWorld*)h)->pairs().size(); }
+REF_API void pref_get_pairs(void* h, int32_t* ab) {
+    auto* w = (RefWorld*)h;
+    auto& p = w->pairs();
+    for (size_t i = 0; i < p.size(); i++) { ab[2 * i] = w->index[p[i].first]; ab[2 * i + 1] = w->index[p[i].second]; }
+}
+REF_API void pref_set_pairs(void* h, int n, const int32_t* ab) {
+    auto* w = (RefWorld*)h;
+    auto& p = w->pairs();
+    p.clear();
+    for (int i = 0; i < n; i++) p.emplace_back(w->owned[ab[2 * i]], w->owned[ab[2 * i + 1]]);
+}
+REF_API void pref_get_state(void* h, double* pos, double* rot, double* v, double* wv) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++) {
+        const auto* rb = w->owned[i];
+        const auto& t = rb->getTransform();
+        for (int k = 0; k < 3; k++) {
+            if (pos) pos[3 * i + k] = t.getOrigin()[k];
+            if (v) v[3 * i + k] = rb->getLinearVelocity()[k];
+            if (wv) wv[3 * i + k] = rb->getAngularVelocity()[k];
+        }
+        if (rot) for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) rot[9 * i + 3 * r + c] = t.getBasis()(r, c);
+    }
+}
+REF_API void pref_set_state(void* h, const double* pos, const double* rot, const double* v, const double* wv) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++) {
+        auto* rb = w->owned[i];
+        pe::Transform t = rb->getTransform();
+        if (pos) t.setOrigin(pe::Vector3(pe::Real(pos[3 * i]), pe::Real(pos[3 * i + 1]), pe::Real(pos[3 * i + 2])));
+        if (rot) t.setBasis(basis_of(rot + 9 * i));
+        rb->setTransform(t);
+        if (v) rb->setLinearVelocity(pe::Vector3(pe::Real(v[3 * i]), pe::Real(v[3 * i + 1]), pe::Real(v[3 * i + 2])));
+        if (wv) rb->setAngularVelocity(pe::Vector3(pe::Real(wv[3 * i]), pe::Real(wv[3 * i + 1]), pe::Real(wv[3 * i + 2])));
+    }
+}
+REF_API void pref_get_aabbs(void* h, double* mn, double* mx) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++)
+        for (int k = 0; k < 3; k++) { mn[3 * i + k] = w->owned[i]->getAABBMin()[k]; mx[3 * i + k] = w->owned[i]->getAABBMax()[k]; }
+}
+REF_API void pref_get_world_inv_inertia(void* h, double* out) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++)
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) out[9 * i + 3 * r + c] = w->owned[i]->getWorldInvInertia()(r, c);
+}
+REF_API void pref_get_mass_props(void* h, double* inv_mass, double* local_inv_inertia) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++) {
+        inv_mass[i] = w->owned[i]->getInvMass();
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++)
+            local_inv_inertia[9 * i + 3 * r + c] = w->owned[i]->getLocalInvInertia()(r, c);
+    }
+}
+REF_API int pref_num_results(void* h) { return (int)((RefWorld*)h)->results().size(); }
+// Per result: point count, swap flag, and all 8 cache slots (13 values each, invalid slots included).
+REF_API void pref_get_contacts(void* h, int32_t* npts, int32_t* swap, double* pts) {
+    auto* w = (RefWorld*)h;
+    auto& res = w->results();
+    for (size_t i = 0; i < res.size(); i++) {
+        if (npts) npts[i] = res[i]->getPointSize();
+        if (swap) swap[i] = res[i]->getSwapFlag() ? 1 : 0;
+        if (!pts) continue;
+        for (int k = 0; k < PE_CONTACT_CACHE_SIZE; k++) {
+            const auto& cp = res[i]->getContactPoint(k);
+            double* o = pts + (i * 8 + k) * 13;
+            for (int c = 0; c < 3; c++) {
+                o[c] = cp.getWorldPos()[c]; o[3 + c] = cp.getWorldNormal()[c];
+                o[6 + c] = cp.getLocalPosA()[c]; o[9 + c] = cp.getLocalPosB()[c];
+            }
+            o[12] = cp.getDistance();
+        }
+    }
+}

@@ -1,0 +1,960 @@
+// Single-world pipeline (the plain drop-in for one World::step) and its C ABI.
+//
+// Per step, on one HIP stream, no host synchronisation:
+//   k_integrate        World::updateObjectStatus + applyExternalForce    (thread / body)
+//   k_bp_keys + radix  BroadPhaseSweepAndPrune sort on aabb_min[axis]    (stable LSD, 4 passes)
+//   k_bp_ranges/scan   per sorted body, the candidate run j > i with min_j <= max_i
+//   k_bp_chunk_*       load-balanced sweep over all candidates, two passes (count, emit),
+//                      so pairs come out in the reference's (i, j) sweep order
+//   k_bp_axis          next axis = argmax variance of AABB centres
+//   k_narrow           thread / pair: shape dispatch + 8-slot contact cache + depth sort
+//   k_setup            thread / pair: solver rows (FrictionContactConstraint::initSequentialImpulse)
+//   k_solve            one 1024-thread workgroup: constraint-graph colouring, then the
+//                      Gauss-Seidel sweeps colour by colour, then velocity sync
+#include "pe_common.hpp"
+#include "pe_device.hpp"
+#include "pe_prims.hpp"
+#include "../../include/pe_amd.h"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+static thread_local std::string g_err;
+void pe_set_error(const std::string& msg) { g_err = msg; }
+
+using namespace pe;
+
+namespace {
+
+constexpr int MAX_COLORS = 1024;
+constexpr int CHUNK = 32;            // candidates per thread in the sweep
+constexpr int SOLVE_THREADS = 1024;
+constexpr int CLAIM_LDS = 12288;     // bodies whose colouring claims fit in LDS (48 KiB)
+
+struct DevCounts {          // device-side counters of one step
+    int32_t npairs;         // pairs emitted by the broadphase (clamped to capacity)
+    int32_t prev_npairs;    // pairs of the previous narrowphase (contact slots that survive)
+    int32_t axis;           // SAP axis for the next sort
+    int32_t overflow;       // set when pair capacity was exceeded
+    uint32_t ncand;         // total sweep candidates
+    uint32_t nchunk_pairs;  // pairs counted by the sweep (unclamped)
+    int32_t ncolors;
+    int32_t nordered;       // pairs in the solve order
+    int32_t npoints;        // contact points of the last narrowphase
+    uint32_t nchunks;       // live sweep chunks
+    int32_t pad[6];
+};
+
+struct Dev {
+    BodyView b;
+    int32_t* pa;            // [pcap]
+    int32_t* pb;
+    float* cp;              // [CF][pcap*8]
+    int32_t* npts;          // [pcap]
+    int32_t* swap;          // [pcap]
+    float* rows;            // [RF][pcap*8]
+    int32_t* order;         // [pcap]
+    int32_t* act;           // [2*pcap] colouring work lists
+    int32_t* color_start;   // [MAX_COLORS+1]
+    int32_t* claim;         // [n] (global claims when n > CLAIM_LDS)
+    uint32_t* keys;         // [n] x2 radix buffers
+    uint32_t* vals;
+    uint32_t* keys2;
+    uint32_t* vals2;
+    float* skey;            // [n] sorted keys as floats
+    uint32_t* cand;         // [n+1] candidate counts -> offsets
+    uint32_t* chunk;        // [chunk capacity] pairs per chunk -> offsets
+    uint32_t* hist;         // radix histograms
+    double* red;            // [6 * nred] axis reduction partials
+    DevCounts* cnt;
+    int pcap;
+    int chunk_cap;
+    float dt;
+    float g[3];
+    int iterations;
+};
+
+__device__ __forceinline__ f3 ld3(const BodyView& b, int f, int i) { return f3{b.at(f, i), b.at(f + 1, i), b.at(f + 2, i)}; }
+__device__ __forceinline__ void st3(const BodyView& b, int f, int i, f3 v) { b.at(f, i) = v.x; b.at(f + 1, i) = v.y; b.at(f + 2, i) = v.z; }
+__device__ __forceinline__ m33 ld9(const BodyView& b, int f, int i) {
+    m33 r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.m[k] = b.at(f + k, i);
+    return r;
+}
+__device__ __forceinline__ void st9(const BodyView& b, int f, int i, const m33& r) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) b.at(f + k, i) = r.m[k];
+}
+
+// ------------------------------------------------------------------ stage 0: integrate
+// World::updateObjectStatus (world.cpp:35-119) with zero sleep thresholds, then
+// World::applyExternalForce (world.cpp:121-127), then SequentialImpulseSolver::setupSolver's
+// clearTempVelocity (sequential_impulse_solver.cpp:17-20).
+__global__ void k_integrate(Dev d) {
+    const BodyView& b = d.b;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+        const int type = b.iat(I_TYPE, i);
+        const bool kin = b.iat(I_KIN, i) != 0;
+        f3 pos = ld3(b, F_POS, i);
+        m33 R = ld9(b, F_ROT, i);
+        float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
+        if (!kin) {
+            f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
+            integrate_pose(pos, R, v, w, d.dt);
+            const m33 L = ld9(b, F_LINV, i);
+            st9(b, F_IW, i, mmt(mm(R, L), R));
+            st3(b, F_POS, i, pos);
+            st9(b, F_ROT, i, R);
+            // RigidBody::applyDamping (rigidbody.cpp:201-205)
+            const float pl = powf(1.0f - b.at(F_LD, i), d.dt), pw = powf(1.0f - b.at(F_AD, i), d.dt);
+            v = mul(v, pl);
+            w = mul(w, pw);
+            // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
+            const float m = b.at(F_MASS, i), im = b.at(F_IM, i);
+            const f3 f{d.g[0] * m, d.g[1] * m, d.g[2] * m};
+            v = add(v, mul(mul(f, im), d.dt));
+            st3(b, F_V, i, v);
+            st3(b, F_W, i, w);
+            st3(b, F_TV, i, v);
+            st3(b, F_TW, i, w);
+        } else {
+            st3(b, F_TV, i, f3{0, 0, 0});
+            st3(b, F_TW, i, f3{0, 0, 0});
+        }
+        f3 mn, mx;
+        shape_aabb(type, sp, R, pos, mn, mx);
+        st3(b, F_AMIN, i, mn);
+        st3(b, F_AMAX, i, mx);
+        b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
+    }
+}
+
+// Recomputes AABBs and scales only (after pe_world_set_state).
+__global__ void k_aabb_only(Dev d) {
+    const BodyView& b = d.b;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+        const f3 pos = ld3(b, F_POS, i);
+        const m33 R = ld9(b, F_ROT, i);
+        float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
+        f3 mn, mx;
+        shape_aabb(b.iat(I_TYPE, i), sp, R, pos, mn, mx);
+        st3(b, F_AMIN, i, mn);
+        st3(b, F_AMAX, i, mx);
+        b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
+        st9(b, F_IW, i, mmt(mm(R, ld9(b, F_LINV, i)), R));
+    }
+}
+
+// ------------------------------------------------------------------ stage 1: broadphase
+__global__ void k_bp_keys(Dev d) {
+    const int ax = d.cnt->axis;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) {
+        d.keys[i] = float_key(d.b.at(F_AMIN + ax, i) + 0.0f);
+        d.vals[i] = (uint32_t)i;
+    }
+}
+
+// After the sort: sorted float keys and, per sorted position s, the number of candidates
+// j > s with key_j <= max_i (the reference's sweep breaks at the first key_j > max_i).
+__global__ void k_bp_ranges(Dev d) {
+    const int n = d.b.n;
+    const int ax = d.cnt->axis;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int i = (int)d.vals[s];
+        d.skey[s] = d.b.at(F_AMIN + ax, i) + 0.0f;
+    }
+}
+__global__ void k_bp_cand(Dev d) {
+    const int n = d.b.n;
+    const int ax = d.cnt->axis;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int i = (int)d.vals[s];
+        const float mx = d.b.at(F_AMAX + ax, i);
+        int lo = s + 1, hi = n;  // first position in [s+1, n) with skey > mx
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (d.skey[mid] > mx) hi = mid; else lo = mid + 1;
+        }
+        d.cand[s] = (uint32_t)(lo - s - 1);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.cand[n] = 0;
+}
+
+// BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
+__device__ __forceinline__ bool bp_valid(const BodyView& b, int i, int j) {
+    if (b.iat(I_KIN, i) && b.iat(I_KIN, j)) return false;
+    if (b.at(F_AMIN, i) > b.at(F_AMAX, j) || b.at(F_AMIN, j) > b.at(F_AMAX, i)) return false;
+    if (b.at(F_AMIN + 1, i) > b.at(F_AMAX + 1, j) || b.at(F_AMIN + 1, j) > b.at(F_AMAX + 1, i)) return false;
+    if (b.at(F_AMIN + 2, i) > b.at(F_AMAX + 2, j) || b.at(F_AMIN + 2, j) > b.at(F_AMAX + 2, i)) return false;
+    return true;
+}
+
+// Locates candidate t: sorted position s with cand[s] <= t < cand[s+1] (cand holds offsets).
+__device__ __forceinline__ int bp_find(const uint32_t* off, int n, uint32_t t) {
+    int lo = 0, hi = n;  // last s with off[s] <= t
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <bool EMIT>
+__global__ void k_bp_chunks(Dev d) {
+    const int n = d.b.n;
+    const uint32_t S = d.cnt->ncand;
+    const uint32_t nchunks = min((S + CHUNK - 1) / CHUNK, (uint32_t)d.chunk_cap);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        uint32_t t = c * CHUNK;
+        const uint32_t tend = min(S, t + CHUNK);
+        int s = bp_find(d.cand, n, t);
+        while (d.cand[s + 1] <= t) s++;
+        uint32_t out = EMIT ? d.chunk[c] : 0;
+        uint32_t cntv = 0;
+        const int i = (int)d.vals[s];
+        int cur_s = s, ci = i;
+        for (; t < tend; t++) {
+            while (d.cand[cur_s + 1] <= t) { cur_s++; ci = (int)d.vals[cur_s]; }
+            const int j = (int)d.vals[cur_s + 1 + (int)(t - d.cand[cur_s])];
+            if (bp_valid(d.b, ci, j)) {
+                if (EMIT) {
+                    if ((int)out < d.pcap) { d.pa[out] = ci; d.pb[out] = j; }
+                    out++;
+                } else {
+                    cntv++;
+                }
+            }
+        }
+        if (!EMIT) d.chunk[c] = cntv;
+    }
+}
+
+// Number of live sweep chunks, for the chunk scan.
+__global__ void k_bp_nchunks(Dev d, uint32_t* nchunks) {
+    const uint32_t want = (d.cnt->ncand + CHUNK - 1) / CHUNK;
+    *nchunks = min(want, (uint32_t)d.chunk_cap);
+    d.cnt->overflow = want > (uint32_t)d.chunk_cap ? 4 : 0;
+}
+
+__global__ void k_bp_finish(Dev d) {
+    const uint32_t total = d.cnt->nchunk_pairs;
+    d.cnt->overflow |= total > (uint32_t)d.pcap ? 1 : 0;
+    d.cnt->npairs = total > (uint32_t)d.pcap ? d.pcap : (int32_t)total;
+}
+
+// Next SAP axis (broad_phase_sweep_and_prune.cpp:37-40): variance of AABB centres, reduced in
+// FP64 over fixed blocks (deterministic), argmax with the reference's tie rule.
+constexpr int RED_THREADS = 256;
+__global__ void __launch_bounds__(RED_THREADS) k_bp_axis_partial(Dev d) {
+    __shared__ double sh[6][RED_THREADS];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float c = (d.b.at(F_AMIN + k, i) + d.b.at(F_AMAX + k, i)) * 0.5f;
+            acc[k] += (double)c;
+            acc[3 + k] += (double)(c * c);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+#pragma unroll
+            for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 6; k++) d.red[k * gridDim.x + blockIdx.x] = sh[k][0];
+}
+__global__ void k_bp_axis_final(Dev d, int nblocks) {
+    if (threadIdx.x != 0) return;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nblocks; b++)
+        for (int k = 0; k < 6; k++) s[k] += d.red[k * nblocks + b];
+    double v[3];
+    for (int k = 0; k < 3; k++) v[k] = s[3 + k] - s[k] * s[k] / (double)d.b.n;
+    int axis = 0;
+    if (v[1] > v[0]) axis = 1;
+    if (v[2] > v[axis]) axis = 2;
+    d.cnt->axis = axis;
+}
+
+// ------------------------------------------------------------------ stage 2: narrowphase
+__device__ __forceinline__ void load_pose(const BodyView& b, int i, Pose& P) {
+    P.R = ld9(b, F_ROT, i);
+    P.p = ld3(b, F_POS, i);
+    P.scale = b.at(F_SCALE, i);
+}
+
+// SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) for one pair slot.
+// Slots below the previous pair count keep their distances from the previous step (the
+// reference reuses its ContactResult objects); slots beyond it start fresh.
+__device__ void narrow_pair(const Dev& d, int p, int prev_np) {
+    const BodyView& b = d.b;
+    const int ia = d.pa[p], ib = d.pb[p];
+    Cache c;
+    const size_t slot0 = (size_t)p * 8;
+    const size_t CS = (size_t)d.pcap * 8;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        c.wx[k] = REAL_MAX; c.wy[k] = REAL_MAX; c.wz[k] = REAL_MAX;
+        c.nx[k] = 0; c.ny[k] = 0; c.nz[k] = 0;
+        c.ax[k] = 0; c.ay[k] = 0; c.az[k] = 0;
+        c.bx[k] = 0; c.by[k] = 0; c.bz[k] = 0;
+        c.d[k] = p < prev_np ? d.cp[C_D * CS + slot0 + k] : REAL_MAX;
+    }
+    PairCtx x;
+    load_pose(b, ia, x.A);
+    load_pose(b, ib, x.B);
+    x.swap = false;
+    {
+        const float sa = x.A.scale, sb = x.B.scale;
+        const float th = (sa < sb ? sa : sb) * 0.02f;
+        x.same_th2 = th * th;
+    }
+    const int ta = b.iat(I_TYPE, ia), tb = b.iat(I_TYPE, ib);
+    const float spa[3] = {b.at(F_SP, ia), b.at(F_SP + 1, ia), b.at(F_SP + 2, ia)};
+    const float spb[3] = {b.at(F_SP, ib), b.at(F_SP + 1, ib), b.at(F_SP + 2, ib)};
+    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, spa, spb);
+    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, spa, spb[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, spb, spa[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, spa[0], spb[0]);
+    const int np = sort_points(c);
+    d.npts[p] = np;
+    d.swap[p] = x.swap ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const size_t s = slot0 + k;
+        d.cp[(C_WPOS + 0) * CS + s] = c.wx[k]; d.cp[(C_WPOS + 1) * CS + s] = c.wy[k]; d.cp[(C_WPOS + 2) * CS + s] = c.wz[k];
+        d.cp[(C_N + 0) * CS + s] = c.nx[k]; d.cp[(C_N + 1) * CS + s] = c.ny[k]; d.cp[(C_N + 2) * CS + s] = c.nz[k];
+        d.cp[(C_LA + 0) * CS + s] = c.ax[k]; d.cp[(C_LA + 1) * CS + s] = c.ay[k]; d.cp[(C_LA + 2) * CS + s] = c.az[k];
+        d.cp[(C_LB + 0) * CS + s] = c.bx[k]; d.cp[(C_LB + 1) * CS + s] = c.by[k]; d.cp[(C_LB + 2) * CS + s] = c.bz[k];
+        d.cp[C_D * CS + s] = c.d[k];
+    }
+}
+
+__global__ void __launch_bounds__(128) k_narrow(Dev d) {
+    const int np = d.cnt->npairs;
+    const int prev = d.cnt->prev_npairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) narrow_pair(d, p, prev);
+}
+__global__ void k_narrow_finish(Dev d) {
+    d.cnt->prev_npairs = d.cnt->npairs;
+}
+
+// ------------------------------------------------------------------ stage 3: solver
+// SequentialImpulseSolver::setupSolver (sequential_impulse_solver.cpp:11-58), thread / pair.
+__global__ void k_setup(Dev d) {
+    const int np = d.cnt->npairs;
+    const BodyView& b = d.b;
+    const size_t CS = (size_t)d.pcap * 8;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const int n = d.npts[p];
+        if (n == 0) continue;
+        const int ia = d.pa[p], ib = d.pb[p];
+        const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
+        const m33 RA = ld9(b, F_ROT, ia), RB = ld9(b, F_ROT, ib);
+        m33 Ia, Ib;
+        if (ka) { for (int k = 0; k < 9; k++) Ia.m[k] = 0.0f; } else Ia = ld9(b, F_IW, ia);
+        if (kb) { for (int k = 0; k < 9; k++) Ib.m[k] = 0.0f; } else Ib = ld9(b, F_IW, ib);
+        const float ims = (ka ? 0.0f : b.at(F_IM, ia)) + (kb ? 0.0f : b.at(F_IM, ib));
+        const f3 va = ld3(b, F_V, ia), wa = ld3(b, F_W, ia), vb = ld3(b, F_V, ib), wb = ld3(b, F_W, ib);
+        const float mu = sqrtf(b.at(F_MU, ia) * b.at(F_MU, ib));
+        const float e = sqrtf(b.at(F_E, ia) * b.at(F_E, ib));
+        for (int k = 0; k < n; k++) {
+            const size_t s = (size_t)p * 8 + k;
+            const f3 la{d.cp[(C_LA) * CS + s], d.cp[(C_LA + 1) * CS + s], d.cp[(C_LA + 2) * CS + s]};
+            const f3 lb{d.cp[(C_LB) * CS + s], d.cp[(C_LB + 1) * CS + s], d.cp[(C_LB + 2) * CS + s]};
+            const f3 nn{d.cp[(C_N) * CS + s], d.cp[(C_N + 1) * CS + s], d.cp[(C_N + 2) * CS + s]};
+            const Row r = make_row(RA, RB, la, lb, nn, d.cp[C_D * CS + s], ims, Ia, Ib, va, wa, vb, wb, mu, e, d.dt);
+            float* R = d.rows;
+            R[(R_RA) * CS + s] = r.ra.x; R[(R_RA + 1) * CS + s] = r.ra.y; R[(R_RA + 2) * CS + s] = r.ra.z;
+            R[(R_RB) * CS + s] = r.rb.x; R[(R_RB + 1) * CS + s] = r.rb.y; R[(R_RB + 2) * CS + s] = r.rb.z;
+            R[(R_N) * CS + s] = r.n.x; R[(R_N + 1) * CS + s] = r.n.y; R[(R_N + 2) * CS + s] = r.n.z;
+            R[(R_T0) * CS + s] = r.t0.x; R[(R_T0 + 1) * CS + s] = r.t0.y; R[(R_T0 + 2) * CS + s] = r.t0.z;
+            R[(R_T1) * CS + s] = r.t1.x; R[(R_T1 + 1) * CS + s] = r.t1.y; R[(R_T1 + 2) * CS + s] = r.t1.z;
+            R[R_RHS * CS + s] = r.rhs; R[R_MN * CS + s] = r.mn; R[R_MT0 * CS + s] = r.mt0; R[R_MT1 * CS + s] = r.mt1;
+            R[R_MU * CS + s] = r.mu; R[R_LN * CS + s] = 0.0f; R[R_LT0 * CS + s] = 0.0f; R[R_LT1 * CS + s] = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ Row load_row(const float* R, size_t CS, size_t s) {
+    Row r;
+    r.ra = f3{R[(R_RA) * CS + s], R[(R_RA + 1) * CS + s], R[(R_RA + 2) * CS + s]};
+    r.rb = f3{R[(R_RB) * CS + s], R[(R_RB + 1) * CS + s], R[(R_RB + 2) * CS + s]};
+    r.n = f3{R[(R_N) * CS + s], R[(R_N + 1) * CS + s], R[(R_N + 2) * CS + s]};
+    r.t0 = f3{R[(R_T0) * CS + s], R[(R_T0 + 1) * CS + s], R[(R_T0 + 2) * CS + s]};
+    r.t1 = f3{R[(R_T1) * CS + s], R[(R_T1 + 1) * CS + s], R[(R_T1 + 2) * CS + s]};
+    r.rhs = R[R_RHS * CS + s]; r.mn = R[R_MN * CS + s]; r.mt0 = R[R_MT0 * CS + s]; r.mt1 = R[R_MT1 * CS + s];
+    r.mu = R[R_MU * CS + s];
+    return r;
+}
+
+// RigidBody::applyTempImpulse (rigidbody.cpp:166-170)
+__device__ __forceinline__ void apply_temp(const BodyView& b, int i, f3& tv, f3& tw, f3 r, f3 J) {
+    tv = add(tv, mul(J, b.at(F_IM, i)));
+    tw = add(tw, mv(ld9(b, F_IW, i), cross(r, J)));
+}
+
+// FrictionContactConstraint::iterateSequentialImpulse (friction_contact_constraint.cpp:75-108)
+// over all rows of one pair.
+__device__ void iterate_pair(const Dev& d, int p) {
+    const BodyView& b = d.b;
+    const size_t CS = (size_t)d.pcap * 8;
+    const int ia = d.pa[p], ib = d.pb[p];
+    const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
+    f3 tva = ld3(b, F_TV, ia), twa = ld3(b, F_TW, ia), tvb = ld3(b, F_TV, ib), twb = ld3(b, F_TW, ib);
+    const int n = d.npts[p];
+    for (int k = 0; k < n; k++) {
+        const size_t s = (size_t)p * 8 + k;
+        const Row r = load_row(d.rows, CS, s);
+        float ln = d.rows[R_LN * CS + s], lt0 = d.rows[R_LT0 * CS + s], lt1 = d.rows[R_LT1 * CS + s];
+        const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
+        d.rows[R_LN * CS + s] = ln; d.rows[R_LT0 * CS + s] = lt0; d.rows[R_LT1 * CS + s] = lt1;
+        if (!ka) apply_temp(b, ia, tva, twa, r.ra, J);
+        if (!kb) apply_temp(b, ib, tvb, twb, r.rb, neg(J));
+    }
+    if (!ka) { st3(b, F_TV, ia, tva); st3(b, F_TW, ia, twa); }
+    if (!kb) { st3(b, F_TV, ib, tvb); st3(b, F_TW, ib, twb); }
+}
+
+// Claims in global memory are updated by L2 atomics; read them with an L1-bypassing load.
+template <bool LDS_CLAIM>
+__device__ __forceinline__ int load_claim(int* claim, int i) {
+    if (LDS_CLAIM) return claim[i];
+    return __hip_atomic_load(&claim[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Colouring + colour-ordered Gauss-Seidel + velocity sync, one workgroup.
+// Round c gives colour c to every uncoloured pair that holds, for each of its dynamic bodies,
+// the lowest pair index among the uncoloured pairs on that body.  Pairs of one colour share no
+// dynamic body (kinematic bodies never change, rigidbody.cpp:167), so sweeping a colour in
+// parallel equals the sequential sweep in (colour, index) order.
+template <bool LDS_CLAIM>
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
+    extern __shared__ int s_claim[];
+    __shared__ int wsum[SOLVE_THREADS / WAVE + 1];
+    __shared__ int s_nact, s_out;
+    const BodyView& b = d.b;
+    const int np = d.cnt->npairs;
+    int* claim = LDS_CLAIM ? s_claim : d.claim;
+    const int INF = 0x7fffffff;
+    for (int i = threadIdx.x; i < b.n; i += blockDim.x) claim[i] = INF;
+    // active list (index order)
+    int* act = d.act;
+    int* act2 = d.act + d.pcap;
+    int nact = 0;
+    for (int base = 0; base < np; base += blockDim.x) {
+        const int p = base + threadIdx.x;
+        int flag = 0;
+        if (p < np && d.npts[p] > 0) flag = !(b.iat(I_KIN, d.pa[p]) && b.iat(I_KIN, d.pb[p]));
+        int tot;
+        const int ex = block_exclusive_scan(flag, wsum, &tot);
+        if (flag) act[nact + ex] = p;
+        nact += tot;
+    }
+    __syncthreads();
+    int ncol = 0, out = 0;
+    while (nact > 0 && ncol < MAX_COLORS) {
+        for (int q = threadIdx.x; q < nact; q += blockDim.x) {
+            const int p = act[q];
+            const int ia = d.pa[p], ib = d.pb[p];
+            if (!b.iat(I_KIN, ia)) atomicMin(&claim[ia], p);
+            if (!b.iat(I_KIN, ib)) atomicMin(&claim[ib], p);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) d.color_start[ncol] = out;
+        int nrest = 0;
+        for (int base = 0; base < nact; base += blockDim.x) {
+            const int q = base + threadIdx.x;
+            int win = 0, p = -1;
+            if (q < nact) {
+                p = act[q];
+                const int ia = d.pa[p], ib = d.pb[p];
+                win = (b.iat(I_KIN, ia) || load_claim<LDS_CLAIM>(claim, ia) == p) &&
+                      (b.iat(I_KIN, ib) || load_claim<LDS_CLAIM>(claim, ib) == p);
+            }
+            int totw, totl;
+            const int exw = block_exclusive_scan(win, wsum, &totw);
+            const int exl = block_exclusive_scan(q < nact && !win, wsum, &totl);
+            if (q < nact) {
+                if (win) d.order[out + exw] = p;
+                else act2[nrest + exl] = p;
+            }
+            out += totw;
+            nrest += totl;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nact; q += blockDim.x) {
+            const int p = act[q];
+            claim[d.pa[p]] = INF;
+            claim[d.pb[p]] = INF;
+        }
+        int* t = act; act = act2; act2 = t;
+        nact = nrest;
+        ncol++;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        d.color_start[ncol] = out;
+        d.cnt->ncolors = ncol;
+        d.cnt->nordered = out;
+        d.cnt->overflow |= (nact > 0) ? 2 : 0;
+    }
+    __syncthreads();
+    // Gauss-Seidel sweeps
+    for (int it = 0; it < d.iterations; it++) {
+        for (int c = 0; c < ncol; c++) {
+            const int c0 = d.color_start[c], c1 = d.color_start[c + 1];
+            for (int q = c0 + threadIdx.x; q < c1; q += blockDim.x) iterate_pair(d, d.order[q]);
+            __syncthreads();
+        }
+    }
+    // SequentialImpulseSolver::solve's syncTempVelocity (sequential_impulse_solver.cpp:71-80)
+    for (int i = threadIdx.x; i < b.n; i += blockDim.x) {
+        if (b.iat(I_KIN, i)) continue;
+        st3(b, F_V, i, ld3(b, F_TV, i));
+        st3(b, F_W, i, ld3(b, F_TW, i));
+    }
+    (void)s_nact; (void)s_out;
+}
+
+__global__ void k_count_points(Dev d) {
+    __shared__ int wsum[17];
+    int acc = 0;
+    const int np = d.cnt->npairs;
+    for (int base = 0; base < np; base += blockDim.x) {
+        const int p = base + threadIdx.x;
+        int tot;
+        block_exclusive_scan(p < np ? d.npts[p] : 0, wsum, &tot);
+        acc += tot;
+    }
+    if (threadIdx.x == 0) d.cnt->npoints = acc;
+}
+
+}  // namespace
+
+// ====================================================================== host side
+struct pe_world {
+    Dev d{};
+    int n = 0, stride = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    int rs_blocks = 0;
+    int red_blocks = 0;
+    bool timing = false;
+    hipEvent_t ev[5] = {};
+    double stage_ms[4] = {0, 0, 0, 0};
+    int n_dynamic = 0;
+
+    ~pe_world() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        for (auto& e : ev) if (e) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+    template <class T> int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        PE_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+        PE_HIP(hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), stream));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return PE_OK;
+    }
+    int grid(int work, int block) const { return std::max(1, std::min((work + block - 1) / block, 4096)); }
+
+    int launch_integrate() {
+        hipLaunchKernelGGL(k_integrate, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        return PE_OK;
+    }
+    int launch_broadphase() {
+        hipLaunchKernelGGL(k_bp_keys, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        uint32_t *ki = d.keys, *vi = d.vals, *ko = d.keys2, *vo = d.vals2;
+        for (int pass = 0; pass < 4; pass++) {
+            hipLaunchKernelGGL(k_radix_hist, dim3(rs_blocks), dim3(RS_THREADS), 0, stream, ki, n, pass * 8, d.hist, rs_blocks);
+            hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.hist, 256 * rs_blocks, (uint32_t*)nullptr);
+            hipLaunchKernelGGL(k_radix_scatter, dim3(rs_blocks), dim3(RS_THREADS), 0, stream, ki, vi, ko, vo, n, pass * 8,
+                               d.hist, rs_blocks);
+            std::swap(ki, ko);
+            std::swap(vi, vo);
+        }
+        // after 4 passes the sorted data is back in keys/vals
+        hipLaunchKernelGGL(k_bp_ranges, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        hipLaunchKernelGGL(k_bp_cand, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.cand, n + 1, &d.cnt->ncand);
+        hipLaunchKernelGGL(k_bp_nchunks, dim3(1), dim3(1), 0, stream, d, &d.cnt->nchunks);
+        hipLaunchKernelGGL(k_bp_chunks<false>, dim3(2048), dim3(256), 0, stream, d);
+        hipLaunchKernelGGL(k_scan_u32_dyn, dim3(1), dim3(1024), 0, stream, d.chunk, &d.cnt->nchunks, &d.cnt->nchunk_pairs);
+        hipLaunchKernelGGL(k_bp_chunks<true>, dim3(2048), dim3(256), 0, stream, d);
+        hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, stream, d);
+        hipLaunchKernelGGL(k_bp_axis_partial, dim3(red_blocks), dim3(RED_THREADS), 0, stream, d);
+        hipLaunchKernelGGL(k_bp_axis_final, dim3(1), dim3(64), 0, stream, d, red_blocks);
+        return PE_OK;
+    }
+    int launch_narrowphase() {
+        hipLaunchKernelGGL(k_narrow, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
+        hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
+        hipLaunchKernelGGL(k_count_points, dim3(1), dim3(1024), 0, stream, d);
+        return PE_OK;
+    }
+    int launch_solve() {
+        hipLaunchKernelGGL(k_setup, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
+        if (n <= CLAIM_LDS)
+            hipLaunchKernelGGL(k_solve<true>, dim3(1), dim3(SOLVE_THREADS), n * sizeof(int), stream, d);
+        else
+            hipLaunchKernelGGL(k_solve<false>, dim3(1), dim3(SOLVE_THREADS), 0, stream, d);
+        return PE_OK;
+    }
+    int step() {
+        if (timing) hipEventRecord(ev[0], stream);
+        launch_integrate();
+        if (timing) hipEventRecord(ev[1], stream);
+        launch_broadphase();
+        if (timing) hipEventRecord(ev[2], stream);
+        launch_narrowphase();
+        if (timing) hipEventRecord(ev[3], stream);
+        launch_solve();
+        if (timing) hipEventRecord(ev[4], stream);
+        PE_HIP(hipGetLastError());
+        if (timing) {
+            PE_HIP(hipEventSynchronize(ev[4]));
+            for (int k = 0; k < 4; k++) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                stage_ms[k] += ms;
+            }
+        }
+        return PE_OK;
+    }
+};
+
+namespace {
+int check_world(pe_world* w) {
+    if (!w) { pe_set_error("null world"); return PE_ERR_INVALID; }
+    return PE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* pe_last_error(void) { return g_err.c_str(); }
+
+int pe_device_count(int32_t* count) {
+    int c = 0;
+    PE_HIP(hipGetDeviceCount(&c));
+    if (count) *count = c;
+    return PE_OK;
+}
+
+int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t max_pairs, pe_world** out) {
+    if (!sc || !prm || !out || sc->n <= 0) { pe_set_error("pe_world_create: invalid argument"); return PE_ERR_INVALID; }
+    for (int i = 0; i < sc->n; i++) {
+        const int t = sc->shape_type[i];
+        if (t != PE_SHAPE_BOX && t != PE_SHAPE_SPHERE) {
+            pe_set_error("pe_world_create: shape type " + std::to_string(t) + " is not supported by this build");
+            return PE_ERR_UNSUPPORTED;
+        }
+    }
+    auto* w = new pe_world();
+    const int n = sc->n;
+    w->n = n;
+    w->stride = (n + 63) & ~63;
+    const int pcap = max_pairs > 0 ? max_pairs : std::max(1024, 16 * n);
+    auto fail = [&](int rc) { delete w; return rc; };
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) { pe_set_error("stream create failed"); return fail(PE_ERR_DEVICE); }
+    Dev& d = w->d;
+    d.pcap = pcap;
+    d.dt = (float)prm->dt;
+    for (int k = 0; k < 3; k++) d.g[k] = (float)prm->gravity[k];
+    d.iterations = prm->iterations;
+    d.b.n = n;
+    d.b.stride = w->stride;
+    const size_t S = w->stride;
+    int rc = PE_OK;
+    rc |= w->alloc(&d.b.f, NF * S);
+    rc |= w->alloc(&d.b.i, NI * S);
+    rc |= w->alloc(&d.pa, pcap);
+    rc |= w->alloc(&d.pb, pcap);
+    rc |= w->alloc(&d.cp, (size_t)CF * pcap * 8);
+    rc |= w->alloc(&d.npts, pcap);
+    rc |= w->alloc(&d.swap, pcap);
+    rc |= w->alloc(&d.rows, (size_t)RF * pcap * 8);
+    rc |= w->alloc(&d.order, pcap);
+    rc |= w->alloc(&d.act, 2 * (size_t)pcap);
+    rc |= w->alloc(&d.color_start, MAX_COLORS + 1);
+    rc |= w->alloc(&d.claim, n);
+    rc |= w->alloc(&d.keys, n);
+    rc |= w->alloc(&d.vals, n);
+    rc |= w->alloc(&d.keys2, n);
+    rc |= w->alloc(&d.vals2, n);
+    rc |= w->alloc(&d.skey, n);
+    rc |= w->alloc(&d.cand, n + 1);
+    w->rs_blocks = (n + RS_TILE - 1) / RS_TILE;
+    rc |= w->alloc(&d.hist, 256 * (size_t)w->rs_blocks);
+    w->red_blocks = std::max(1, std::min(256, (n + RED_THREADS - 1) / RED_THREADS));
+    rc |= w->alloc(&d.red, 6 * (size_t)w->red_blocks);
+    rc |= w->alloc(&d.cnt, 1);
+    // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
+    d.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / CHUNK + 1, (int64_t)1 << 26);
+    rc |= w->alloc(&d.chunk, d.chunk_cap);
+    if (rc != PE_OK) return fail(PE_ERR_DEVICE);
+
+    // host staging of the body store (FP32), mass properties as RigidBody::setMass +
+    // setCollisionShape compute them (rigidbody.cpp:11-41) for the diagonal shape inertias.
+    std::vector<float> hf(NF * S, 0.0f);
+    std::vector<int32_t> hi(NI * S, 0);
+    int ndyn = 0;
+    for (int i = 0; i < n; i++) {
+        auto F = [&](int f) -> float& { return hf[f * S + i]; };
+        const int t = sc->shape_type[i];
+        hi[I_TYPE * S + i] = t;
+        hi[I_KIN * S + i] = sc->kinematic[i] ? 1 : 0;
+        ndyn += sc->kinematic[i] ? 0 : 1;
+        for (int k = 0; k < 4; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
+        for (int k = 0; k < 3; k++) {
+            F(F_POS + k) = (float)sc->position[3 * i + k];
+            F(F_V + k) = (float)sc->linear_velocity[3 * i + k];
+            F(F_W + k) = (float)sc->angular_velocity[3 * i + k];
+        }
+        for (int k = 0; k < 9; k++) F(F_ROT + k) = (float)sc->rotation[9 * i + k];
+        const float mass = (float)sc->mass[i];
+        F(F_MASS) = mass;
+        F(F_IM) = mass > EPS ? 1.0f / mass : 0.0f;
+        // shape inertia at unit mass (box_shape.cpp:16-22, sphere_shape.cpp:5-12, cylinder_shape.cpp:23-31)
+        float I[3] = {0, 0, 0};
+        if (t == PE_SHAPE_BOX) {
+            const float x2 = F(F_SP) * F(F_SP), y2 = F(F_SP + 1) * F(F_SP + 1), z2 = F(F_SP + 2) * F(F_SP + 2);
+            I[0] = (y2 + z2) / 12.0f; I[1] = (x2 + z2) / 12.0f; I[2] = (x2 + y2) / 12.0f;
+        } else if (t == PE_SHAPE_SPHERE) {
+            const float r = F(F_SP);
+            I[0] = I[1] = I[2] = (float)(2.0 / 5.0) * r * r;
+        } else {
+            const float r = F(F_SP), h = F(F_SP + 1);
+            const float r_2 = r * r, h_2 = h * h;
+            I[1] = r_2 * 0.5f;
+            I[0] = I[2] = r_2 / 4.0f + h_2 / 12.0f;
+        }
+        // Eigen's 3x3 inverse (cofactors times 1/det) of diag(a, b, c) = mass * I
+        const float a = I[0] * mass, bb = I[1] * mass, c = I[2] * mass;
+        for (int k = 0; k < 9; k++) F(F_LINV + k) = 0.0f;
+        if (mass > EPS) {
+            const float c0 = bb * c - 0.0f * 0.0f;
+            const float det = c0 * a + (0.0f + 0.0f);
+            const float inv = 1.0f / det;
+            F(F_LINV + 0) = c0 * inv;
+            F(F_LINV + 4) = (c * a - 0.0f * 0.0f) * inv;
+            F(F_LINV + 8) = (a * bb - 0.0f * 0.0f) * inv;
+        }
+        F(F_MU) = (float)sc->friction[i];
+        F(F_E) = (float)sc->restitution[i];
+        F(F_LD) = sc->linear_damping ? (float)sc->linear_damping[i] : 0.0f;
+        F(F_AD) = sc->angular_damping ? (float)sc->angular_damping[i] : 0.0f;
+    }
+    w->n_dynamic = ndyn;
+    if (hipMemcpyAsync(d.b.f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, w->stream) != hipSuccess ||
+        hipMemcpyAsync(d.b.i, hi.data(), hi.size() * sizeof(int32_t), hipMemcpyHostToDevice, w->stream) != hipSuccess) {
+        pe_set_error("upload failed");
+        return fail(PE_ERR_DEVICE);
+    }
+    hipLaunchKernelGGL(k_aabb_only, dim3(w->grid(n, 256)), dim3(256), 0, w->stream, d);
+    for (auto& e : w->ev) hipEventCreate(&e);
+    if (hipStreamSynchronize(w->stream) != hipSuccess) { pe_set_error("init sync failed"); return fail(PE_ERR_DEVICE); }
+    *out = w;
+    return PE_OK;
+}
+
+int pe_world_destroy(pe_world* w) {
+    delete w;
+    return PE_OK;
+}
+
+int pe_world_step(pe_world* w, int32_t nsteps) {
+    if (int rc = check_world(w)) return rc;
+    for (int s = 0; s < nsteps; s++)
+        if (int rc = w->step()) return rc;
+    return PE_OK;
+}
+int pe_world_integrate(pe_world* w) {
+    if (int rc = check_world(w)) return rc;
+    w->launch_integrate();
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+int pe_world_broadphase(pe_world* w, int32_t* npairs) {
+    if (int rc = check_world(w)) return rc;
+    w->launch_broadphase();
+    PE_HIP(hipGetLastError());
+    if (npairs) {
+        DevCounts c;
+        PE_HIP(hipMemcpyAsync(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+        PE_HIP(hipStreamSynchronize(w->stream));
+        *npairs = c.npairs;
+        if (c.overflow & 1) { pe_set_error("pair capacity exceeded"); return PE_ERR_CAPACITY; }
+    }
+    return PE_OK;
+}
+int pe_world_narrowphase(pe_world* w) {
+    if (int rc = check_world(w)) return rc;
+    w->launch_narrowphase();
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+int pe_world_solve(pe_world* w) {
+    if (int rc = check_world(w)) return rc;
+    w->launch_solve();
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+int pe_world_sync(pe_world* w) {
+    if (int rc = check_world(w)) return rc;
+    PE_HIP(hipStreamSynchronize(w->stream));
+    DevCounts c;
+    PE_HIP(hipMemcpy(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost));
+    if (c.overflow & 1) { pe_set_error("pair capacity exceeded"); return PE_ERR_CAPACITY; }
+    if (c.overflow & 2) { pe_set_error("colour capacity exceeded"); return PE_ERR_CAPACITY; }
+    return PE_OK;
+}
+
+static int get_field(pe_world* w, int f0, int nf, float* out) {
+    const size_t S = w->stride;
+    std::vector<float> tmp((size_t)nf * S);
+    PE_HIP(hipMemcpyAsync(tmp.data(), w->d.b.f + (size_t)f0 * S, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    for (int i = 0; i < w->n; i++)
+        for (int k = 0; k < nf; k++) out[(size_t)i * nf + k] = tmp[(size_t)k * S + i];
+    return PE_OK;
+}
+static int set_field(pe_world* w, int f0, int nf, const float* in) {
+    const size_t S = w->stride;
+    std::vector<float> tmp((size_t)nf * S);
+    PE_HIP(hipMemcpyAsync(tmp.data(), w->d.b.f + (size_t)f0 * S, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    for (int i = 0; i < w->n; i++)
+        for (int k = 0; k < nf; k++) tmp[(size_t)k * S + i] = in[(size_t)i * nf + k];
+    PE_HIP(hipMemcpyAsync(w->d.b.f + (size_t)f0 * S, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    return PE_OK;
+}
+
+int pe_world_get_state(pe_world* w, float* pos, float* rot, float* v, float* av) {
+    if (int rc = check_world(w)) return rc;
+    int rc = PE_OK;
+    if (pos && (rc = get_field(w, F_POS, 3, pos))) return rc;
+    if (rot && (rc = get_field(w, F_ROT, 9, rot))) return rc;
+    if (v && (rc = get_field(w, F_V, 3, v))) return rc;
+    if (av && (rc = get_field(w, F_W, 3, av))) return rc;
+    return PE_OK;
+}
+int pe_world_set_state(pe_world* w, const float* pos, const float* rot, const float* v, const float* av) {
+    if (int rc = check_world(w)) return rc;
+    int rc = PE_OK;
+    if (pos && (rc = set_field(w, F_POS, 3, pos))) return rc;
+    if (rot && (rc = set_field(w, F_ROT, 9, rot))) return rc;
+    if (v && (rc = set_field(w, F_V, 3, v))) return rc;
+    if (av && (rc = set_field(w, F_W, 3, av))) return rc;
+    hipLaunchKernelGGL(k_aabb_only, dim3(w->grid(w->n, 256)), dim3(256), 0, w->stream, w->d);
+    PE_HIP(hipStreamSynchronize(w->stream));
+    return PE_OK;
+}
+int pe_world_get_aabbs(pe_world* w, float* mn, float* mx) {
+    if (int rc = check_world(w)) return rc;
+    int rc = PE_OK;
+    if (mn && (rc = get_field(w, F_AMIN, 3, mn))) return rc;
+    if (mx && (rc = get_field(w, F_AMAX, 3, mx))) return rc;
+    return PE_OK;
+}
+static int read_counts(pe_world* w, DevCounts& c) {
+    PE_HIP(hipMemcpyAsync(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    return PE_OK;
+}
+int pe_world_get_pairs(pe_world* w, int32_t* pairs, int32_t cap, int32_t* npairs) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    if (npairs) *npairs = c.npairs;
+    const int m = std::min(cap, c.npairs);
+    if (pairs && m > 0) {
+        std::vector<int32_t> a(m), b(m);
+        PE_HIP(hipMemcpy(a.data(), w->d.pa, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PE_HIP(hipMemcpy(b.data(), w->d.pb, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int i = 0; i < m; i++) { pairs[2 * i] = a[i]; pairs[2 * i + 1] = b[i]; }
+    }
+    return PE_OK;
+}
+int pe_world_get_contacts(pe_world* w, int32_t* npoints, float* points, int32_t cap) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    const int m = std::min(cap, c.npairs);
+    if (m <= 0) return PE_OK;
+    if (npoints) PE_HIP(hipMemcpy(npoints, w->d.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (points) {
+        const size_t CS = (size_t)w->d.pcap * 8;
+        std::vector<float> tmp((size_t)m * 8);
+        for (int f = 0; f < CF; f++) {
+            PE_HIP(hipMemcpy(tmp.data(), w->d.cp + f * CS, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t s = 0; s < tmp.size(); s++) points[s * CF + f] = tmp[s];
+        }
+    }
+    return PE_OK;
+}
+int pe_world_get_solve_order(pe_world* w, int32_t* order, int32_t cap, int32_t* nordered, int32_t* color_start,
+                             int32_t ccap, int32_t* ncolors) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    if (nordered) *nordered = c.nordered;
+    if (ncolors) *ncolors = c.ncolors;
+    if (order && cap > 0 && c.nordered > 0)
+        PE_HIP(hipMemcpy(order, w->d.order, std::min(cap, c.nordered) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (color_start && ccap > 0)
+        PE_HIP(hipMemcpy(color_start, w->d.color_start, std::min(ccap, c.ncolors + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return PE_OK;
+}
+int pe_world_get_axis(pe_world* w, int32_t* axis) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    if (axis) *axis = c.axis;
+    return PE_OK;
+}
+int pe_world_set_axis(pe_world* w, int32_t axis) {
+    if (int rc = check_world(w)) return rc;
+    if (axis < 0 || axis > 2) { pe_set_error("axis out of range"); return PE_ERR_INVALID; }
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    c.axis = axis;
+    PE_HIP(hipMemcpy(w->d.cnt, &c, sizeof(c), hipMemcpyHostToDevice));
+    return PE_OK;
+}
+int pe_world_set_timing(pe_world* w, int32_t enable) {
+    if (int rc = check_world(w)) return rc;
+    w->timing = enable != 0;
+    return PE_OK;
+}
+int pe_world_get_stage_times(pe_world* w, double* ms) {
+    if (int rc = check_world(w)) return rc;
+    for (int k = 0; k < 4; k++) ms[k] = w->stage_ms[k];
+    return PE_OK;
+}
+int pe_world_get_counts(pe_world* w, int64_t* counts) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    counts[0] = w->n;
+    counts[1] = w->n_dynamic;
+    counts[2] = c.npairs;
+    counts[3] = c.npoints;
+    counts[4] = c.ncolors;
+    return PE_OK;
+}
+
+}  // extern "C"

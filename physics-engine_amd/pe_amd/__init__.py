@@ -1,0 +1,9 @@
+"""pe_amd -- MI355X (gfx950) rigid-body step behind the reference engine's World interface.
+
+    from pe_amd import scenes, World
+    w = World(scenes.tower_scene())
+    w.step(60)
+"""
+from . import scenes  # noqa: F401
+from ._lib import PEError, check, lib, LIB_PATH  # noqa: F401
+from .world import World  # noqa: F401

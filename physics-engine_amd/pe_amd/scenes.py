@@ -1,0 +1,306 @@
+"""Seeded synthetic scenes for the rigid-body step.
+
+A Scene is a host-side structure of arrays in FP64 that maps 1:1 onto pe_scene_desc
+(include/pe_amd.h).  Every consumer (the HIP library, the CPU oracle, the compiled
+reference) rounds these values to its own precision, so all of them start from the same
+FP32 bits.
+
+Generators (numpy PCG64, seed = 1000 + workload index unless given):
+  tower_scene      -- ring tower of 20 x 10 bricks and a steel ball fired into it
+                      (the shape of examples/bomb_demo/main.cpp:58-82)
+  box_rain_scene   -- N boxes with U(0.2, 1.0) m edges and uniform random orientation
+                      dropped onto a kinematic ground box
+  batch_worlds     -- many independent worlds of log-uniform body counts, boxes and spheres
+  mixed_scene      -- boxes / spheres / cylinders mix over a ground box
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_CONVEX, SHAPE_CONCAVE = 0, 1, 2, 3, 4
+
+
+class SceneDesc(ctypes.Structure):
+    """ctypes mirror of pe_scene_desc."""
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("shape_type", ctypes.POINTER(ctypes.c_int32)),
+        ("shape_param", ctypes.POINTER(ctypes.c_double)),
+        ("mass", ctypes.POINTER(ctypes.c_double)),
+        ("position", ctypes.POINTER(ctypes.c_double)),
+        ("rotation", ctypes.POINTER(ctypes.c_double)),
+        ("linear_velocity", ctypes.POINTER(ctypes.c_double)),
+        ("angular_velocity", ctypes.POINTER(ctypes.c_double)),
+        ("friction", ctypes.POINTER(ctypes.c_double)),
+        ("restitution", ctypes.POINTER(ctypes.c_double)),
+        ("linear_damping", ctypes.POINTER(ctypes.c_double)),
+        ("angular_damping", ctypes.POINTER(ctypes.c_double)),
+        ("kinematic", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
+class WorldParams(ctypes.Structure):
+    """ctypes mirror of pe_world_params."""
+    _fields_ = [("dt", ctypes.c_double), ("gravity", ctypes.c_double * 3), ("iterations", ctypes.c_int32)]
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+@dataclass
+class Scene:
+    shape_type: np.ndarray           # int32 [n]
+    shape_param: np.ndarray          # f64 [n,4]
+    mass: np.ndarray                 # f64 [n]
+    position: np.ndarray             # f64 [n,3]
+    rotation: np.ndarray             # f64 [n,3,3] (row-major basis)
+    linear_velocity: np.ndarray      # f64 [n,3]
+    angular_velocity: np.ndarray     # f64 [n,3]
+    friction: np.ndarray             # f64 [n]
+    restitution: np.ndarray          # f64 [n]
+    kinematic: np.ndarray            # int32 [n]
+    linear_damping: np.ndarray = None
+    angular_damping: np.ndarray = None
+    dt: float = 1.0 / 60.0
+    gravity: tuple = (0.0, -9.8, 0.0)
+    iterations: int = 10
+    name: str = ""
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n(self) -> int:
+        return int(self.shape_type.shape[0])
+
+    def n_dynamic(self) -> int:
+        return int((self.kinematic == 0).sum())
+
+    def contiguous(self) -> "Scene":
+        for f in ("shape_param", "mass", "position", "rotation", "linear_velocity", "angular_velocity",
+                  "friction", "restitution", "linear_damping", "angular_damping"):
+            v = getattr(self, f)
+            if v is not None:
+                setattr(self, f, np.ascontiguousarray(v, dtype=np.float64))
+        self.shape_type = np.ascontiguousarray(self.shape_type, dtype=np.int32)
+        self.kinematic = np.ascontiguousarray(self.kinematic, dtype=np.int32)
+        if self.linear_damping is None:
+            self.linear_damping = np.zeros(self.n)
+        if self.angular_damping is None:
+            self.angular_damping = np.zeros(self.n)
+        return self
+
+    def desc(self) -> SceneDesc:
+        """Builds a pe_scene_desc pointing at this scene's arrays (kept alive by the scene)."""
+        self.contiguous()
+        d = SceneDesc(self.n, _iptr(self.shape_type), _dptr(self.shape_param), _dptr(self.mass),
+                      _dptr(self.position), _dptr(self.rotation), _dptr(self.linear_velocity),
+                      _dptr(self.angular_velocity), _dptr(self.friction), _dptr(self.restitution),
+                      _dptr(self.linear_damping), _dptr(self.angular_damping), _iptr(self.kinematic))
+        return d
+
+    def params(self) -> WorldParams:
+        return WorldParams(self.dt, (ctypes.c_double * 3)(*self.gravity), self.iterations)
+
+    def subset(self, idx) -> "Scene":
+        idx = np.asarray(idx)
+        return Scene(self.shape_type[idx], self.shape_param[idx], self.mass[idx], self.position[idx],
+                     self.rotation[idx], self.linear_velocity[idx], self.angular_velocity[idx],
+                     self.friction[idx], self.restitution[idx], self.kinematic[idx],
+                     None if self.linear_damping is None else self.linear_damping[idx],
+                     None if self.angular_damping is None else self.angular_damping[idx],
+                     self.dt, self.gravity, self.iterations, self.name).contiguous()
+
+    def footprint_bodies(self) -> int:
+        return self.n
+
+
+def concat(scenes, name="") -> Scene:
+    """Concatenates bodies of several scenes into one (same dt / gravity / iterations)."""
+    s0 = scenes[0]
+    cat = lambda f: np.concatenate([getattr(s, f) for s in scenes])
+    return Scene(cat("shape_type"), cat("shape_param"), cat("mass"), cat("position"), cat("rotation"),
+                 cat("linear_velocity"), cat("angular_velocity"), cat("friction"), cat("restitution"),
+                 cat("kinematic"), cat("linear_damping"), cat("angular_damping"),
+                 s0.dt, s0.gravity, s0.iterations, name).contiguous()
+
+
+# ------------------------------------------------------------------ rotations
+def rot_y(angle: float) -> np.ndarray:
+    """Basis of Eigen::AngleAxis(angle, UnitY)."""
+    c, s = math.cos(angle), math.sin(angle)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def quat_to_matrix(q: np.ndarray) -> np.ndarray:
+    """Rotation matrices from unit quaternions (x, y, z, w), shape [..., 4] -> [..., 3, 3]."""
+    x, y, z, w = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    m = np.empty(q.shape[:-1] + (3, 3))
+    m[..., 0, 0] = 1 - 2 * (y * y + z * z)
+    m[..., 0, 1] = 2 * (x * y - z * w)
+    m[..., 0, 2] = 2 * (x * z + y * w)
+    m[..., 1, 0] = 2 * (x * y + z * w)
+    m[..., 1, 1] = 1 - 2 * (x * x + z * z)
+    m[..., 1, 2] = 2 * (y * z - x * w)
+    m[..., 2, 0] = 2 * (x * z - y * w)
+    m[..., 2, 1] = 2 * (y * z + x * w)
+    m[..., 2, 2] = 1 - 2 * (x * x + y * y)
+    return m
+
+
+def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random unit quaternions (Shoemake's subgroup algorithm)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    return np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                     b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=-1)
+
+
+# ------------------------------------------------------------------ builders
+class _Builder:
+    def __init__(self):
+        self.rows = []
+
+    def add(self, shape, param, mass, pos, rot=None, v=(0, 0, 0), w=(0, 0, 0), mu=0.5, e=0.0, kin=0):
+        p = np.zeros(4)
+        p[:len(param)] = param
+        self.rows.append((shape, p, mass, np.asarray(pos, float),
+                          np.eye(3) if rot is None else np.asarray(rot, float),
+                          np.asarray(v, float), np.asarray(w, float), mu, e, kin))
+
+    def scene(self, **kw) -> Scene:
+        r = self.rows
+        return Scene(np.array([x[0] for x in r], np.int32), np.stack([x[1] for x in r]),
+                     np.array([x[2] for x in r], float), np.stack([x[3] for x in r]),
+                     np.stack([x[4] for x in r]), np.stack([x[5] for x in r]),
+                     np.stack([x[6] for x in r]), np.array([x[7] for x in r], float),
+                     np.array([x[8] for x in r], float), np.array([x[9] for x in r], np.int32),
+                     **kw).contiguous()
+
+
+def tower_scene(layers: int = 20, per_layer: int = 10, seed: int = 1000) -> Scene:
+    """Ring tower of layers x per_layer bricks (1.0 x 0.5 x 0.5 m, 1000 kg/m^3) on a kinematic
+    50 x 1 x 50 m ground box, and a r = 0.5 m steel sphere (7800 kg/m^3) at 30 m/s."""
+    b = _Builder()
+    b.add(SHAPE_BOX, (50.0, 1.0, 50.0), 1.0, (0.0, -0.5, 0.0), kin=1)
+    L, H, W = 1.0, 0.5, 0.5
+    angle = 2.0 * math.pi / per_layer
+    radius = 1.25 * L / angle
+    mass = 1000.0 * L * H * W
+    for i in range(layers):
+        off = (i % 2) * angle / 2.0
+        for k in range(per_layer):
+            th = off + k * angle
+            pos = (radius * math.cos(th), H * (0.5 + i), radius * math.sin(th))
+            b.add(SHAPE_BOX, (W, H, L), mass, pos, rot_y(-th))
+    r = 0.5
+    b.add(SHAPE_SPHERE, (r,), 7800.0 * 4.0 / 3.0 * math.pi * r ** 3, (0.0, 2.5, 12.0), v=(0.0, 0.0, -30.0))
+    return b.scene(name=f"tower{layers}x{per_layer}")
+
+
+def _rejection_centres(rng, radii, lo, hi, gap):
+    """Places spheres of the given radii uniformly in the box [lo, hi] so that no two come
+    within `gap` of each other (bounding-sphere rejection, grid accelerated)."""
+    n = len(radii)
+    cell = 2.0 * radii.max() + gap
+    grid = {}
+    out = np.empty((n, 3))
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    for i in range(n):
+        for _ in range(10000):
+            c = lo + rng.random(3) * (hi - lo)
+            key = tuple((c // cell).astype(int))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        for j in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                            d = out[j] - c
+                            if d @ d < (radii[i] + radii[j] + gap) ** 2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                out[i] = c
+                grid.setdefault(key, []).append(i)
+                break
+        else:
+            raise RuntimeError("rejection sampling failed: volume too small")
+    return out
+
+
+def box_rain_scene(n: int = 10000, seed: int = 1001, lo=(-20, 1, -20), hi=(20, 81, 20),
+                   ground=(200.0, 1.0, 200.0), density=1000.0, mu=0.5) -> Scene:
+    """n boxes, edges iid U(0.2, 1.0) m, uniform orientation, centres uniform in [lo, hi]
+    with 0.05 m bounding-sphere rejection, over a kinematic ground box (top face at y = 0)."""
+    rng = np.random.default_rng(seed)
+    size = rng.uniform(0.2, 1.0, (n, 3))
+    rot = quat_to_matrix(random_quaternions(rng, n))
+    radii = 0.5 * np.linalg.norm(size, axis=1)
+    pos = _rejection_centres(rng, radii, lo, hi, 0.05)
+    b = _Builder()
+    b.add(SHAPE_BOX, ground, 1.0, (0.0, -ground[1] / 2.0, 0.0), mu=mu, kin=1)
+    for i in range(n):
+        b.add(SHAPE_BOX, size[i], density * size[i].prod(), pos[i], rot[i], mu=mu)
+    return b.scene(name=f"box_rain{n}")
+
+
+def mixed_world(n: int, seed: int, p_sphere: float = 0.5, p_cyl: float = 0.0, density=1000.0) -> Scene:
+    """One small world: n bodies, each a box (edges U(0.2, 1.0)) or a sphere (r U(0.1, 0.6))
+    (or a cylinder r U(0.1, 0.4), h U(0.2, 1.0) with probability p_cyl), dropped onto its own
+    kinematic ground box.  The drop volume grows with n."""
+    rng = np.random.default_rng(seed)
+    half = 2.0 + 0.5 * math.sqrt(n)
+    height = 2.0 + n / 16.0
+    u = rng.random(n)
+    kind = np.where(u < p_cyl, SHAPE_CYLINDER, np.where(u < p_cyl + p_sphere, SHAPE_SPHERE, SHAPE_BOX))
+    size = rng.uniform(0.2, 1.0, (n, 3))
+    rad = rng.uniform(0.1, 0.6, n)
+    cyl_r = rng.uniform(0.1, 0.4, n)
+    cyl_h = rng.uniform(0.2, 1.0, n)
+    rot = quat_to_matrix(random_quaternions(rng, n))
+    bound = np.where(kind == SHAPE_BOX, 0.5 * np.linalg.norm(size, axis=1),
+                     np.where(kind == SHAPE_SPHERE, rad, np.sqrt(cyl_r ** 2 + 0.25 * cyl_h ** 2)))
+    pos = _rejection_centres(rng, bound, (-half, 1.0, -half), (half, 1.0 + height, half), 0.05)
+    b = _Builder()
+    g = 2.0 * half + 4.0
+    b.add(SHAPE_BOX, (g, 1.0, g), 1.0, (0.0, -0.5, 0.0), kin=1)
+    for i in range(n):
+        if kind[i] == SHAPE_BOX:
+            b.add(SHAPE_BOX, size[i], density * size[i].prod(), pos[i], rot[i])
+        elif kind[i] == SHAPE_SPHERE:
+            b.add(SHAPE_SPHERE, (rad[i],), density * 4.0 / 3.0 * math.pi * rad[i] ** 3, pos[i])
+        else:
+            b.add(SHAPE_CYLINDER, (cyl_r[i], cyl_h[i]), density * math.pi * cyl_r[i] ** 2 * cyl_h[i], pos[i], rot[i])
+    return b.scene(name=f"world{n}")
+
+
+def world_seed(seed: int, w: int) -> int:
+    """Per-world seed: SplitMix64 of (seed, w)."""
+    z = (seed * 0x9E3779B97F4A7C15 + (w + 1) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return (z ^ (z >> 31)) & 0x7FFFFFFF
+
+
+def log_uniform_counts(rng, n_worlds: int, lo: int, hi: int) -> np.ndarray:
+    return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), n_worlds))).astype(int)
+
+
+def batch_worlds(n_worlds: int = 4096, lo: int = 8, hi: int = 512, seed: int = 1003):
+    """Independent worlds with body counts log-uniform in [lo, hi]; boxes and spheres (p = 1/2)."""
+    rng = np.random.default_rng(seed)
+    counts = log_uniform_counts(rng, n_worlds, lo, hi)
+    return [mixed_world(int(c), world_seed(seed, w)) for w, c in enumerate(counts)]

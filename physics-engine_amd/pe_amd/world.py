@@ -1,0 +1,144 @@
+"""Python mirror of the reference's World for the HIP path.
+
+`World` follows pe_intf::World (src/intf/world.h): it is built from a Scene (the bodies
+World::addRigidBody would receive), `step()` runs World::step (world.cpp:267-300) on the GPU,
+and the four stage methods run the pipeline one stage at a time:
+
+    integrate()    -> World::updateObjectStatus + applyExternalForce
+    broadphase()   -> BroadPhaseSweepAndPrune::calcCollisionPairs
+    narrowphase()  -> SimpleNarrowPhase::calcContactResults
+    solve()        -> SequentialImpulseSolver::setupSolver + solve
+
+All state stays on the device; the getters copy it back.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F32P = ctypes.POINTER(ctypes.c_float)
+_F64P = ctypes.POINTER(ctypes.c_double)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+def _f(a):
+    return a.ctypes.data_as(_F32P)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I32P)
+
+
+class World:
+    def __init__(self, scene, max_pairs: int = 0):
+        self.scene = scene
+        self.n = scene.n
+        self._desc = scene.desc()
+        self._params = scene.params()
+        h = ctypes.c_void_p()
+        check(lib().pe_world_create(ctypes.byref(self._desc), ctypes.byref(self._params), ctypes.c_int32(max_pairs),
+                                    ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pe_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- World::step and its stages
+    def step(self, n: int = 1):
+        check(lib().pe_world_step(self.h, ctypes.c_int32(n)))
+
+    def integrate(self):
+        check(lib().pe_world_integrate(self.h))
+
+    def broadphase(self) -> int:
+        n = ctypes.c_int32()
+        check(lib().pe_world_broadphase(self.h, ctypes.byref(n)))
+        return n.value
+
+    def narrowphase(self):
+        check(lib().pe_world_narrowphase(self.h))
+
+    def solve(self):
+        check(lib().pe_world_solve(self.h))
+
+    def sync(self):
+        check(lib().pe_world_sync(self.h))
+
+    # ---- state
+    def state(self):
+        n = self.n
+        pos, rot = np.zeros((n, 3), np.float32), np.zeros((n, 3, 3), np.float32)
+        v, w = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        check(lib().pe_world_get_state(self.h, _f(pos), _f(rot), _f(v), _f(w)))
+        return pos, rot, v, w
+
+    def set_state(self, pos=None, rot=None, v=None, w=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        pos, rot, v, w = c(pos), c(rot), c(v), c(w)
+        p = lambda a: None if a is None else _f(a)
+        check(lib().pe_world_set_state(self.h, p(pos), p(rot), p(v), p(w)))
+
+    def aabbs(self):
+        mn, mx = np.zeros((self.n, 3), np.float32), np.zeros((self.n, 3), np.float32)
+        check(lib().pe_world_get_aabbs(self.h, _f(mn), _f(mx)))
+        return mn, mx
+
+    def pairs(self):
+        cnt = ctypes.c_int32()
+        check(lib().pe_world_get_pairs(self.h, None, 0, ctypes.byref(cnt)))
+        ab = np.zeros((max(cnt.value, 1), 2), np.int32)
+        check(lib().pe_world_get_pairs(self.h, _i(ab), ctypes.c_int32(cnt.value), ctypes.byref(cnt)))
+        return ab[:cnt.value].copy()
+
+    def contacts(self):
+        """(npts [P], points [P, 8, 13]) for every pair of the last narrowphase."""
+        cnt = ctypes.c_int32()
+        check(lib().pe_world_get_pairs(self.h, None, 0, ctypes.byref(cnt)))
+        P = cnt.value
+        npts = np.zeros(max(P, 1), np.int32)
+        pts = np.zeros((max(P, 1), 8, 13), np.float32)
+        check(lib().pe_world_get_contacts(self.h, _i(npts), _f(pts), ctypes.c_int32(P)))
+        return npts[:P].copy(), pts[:P].copy()
+
+    def solve_order(self):
+        """(order, color_start): pair indices grouped by colour."""
+        no, nc = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().pe_world_get_solve_order(self.h, None, 0, ctypes.byref(no), None, 0, ctypes.byref(nc)))
+        order = np.zeros(max(no.value, 1), np.int32)
+        cs = np.zeros(nc.value + 1, np.int32)
+        check(lib().pe_world_get_solve_order(self.h, _i(order), ctypes.c_int32(no.value), ctypes.byref(no), _i(cs),
+                                             ctypes.c_int32(nc.value + 1), ctypes.byref(nc)))
+        return order[:no.value].copy(), cs
+
+    def axis(self) -> int:
+        a = ctypes.c_int32()
+        check(lib().pe_world_get_axis(self.h, ctypes.byref(a)))
+        return a.value
+
+    def set_axis(self, a: int):
+        check(lib().pe_world_set_axis(self.h, ctypes.c_int32(a)))
+
+    def set_timing(self, on: bool):
+        check(lib().pe_world_set_timing(self.h, ctypes.c_int32(1 if on else 0)))
+
+    def stage_times(self):
+        ms = (ctypes.c_double * 4)()
+        check(lib().pe_world_get_stage_times(self.h, ms))
+        return list(ms)
+
+    def counts(self):
+        c = (ctypes.c_int64 * 5)()
+        check(lib().pe_world_get_counts(self.h, c))
+        return dict(zip(("bodies", "dynamic", "pairs", "points", "colors"), list(c)))

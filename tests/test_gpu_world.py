@@ -1,0 +1,117 @@
+"""Parity of the plain single-world HIP path against the CPU oracle (FP32 restatement of the
+reference, itself pinned bit-exact to the compiled reference in test_oracle_vs_ref.py).
+
+Tolerances (written here, applied below):
+  integrate / AABB / pair list / solve order .............. bit-exact
+  contact points (normals, depths, positions) ............. 1e-5 relative (abs floor 1e-6)
+  velocities after one solve .............................. 1e-5 relative (abs floor 1e-6)
+  60-step trajectories .................................... position RMS <= 1e-3 m, max <= 1e-2 m
+"""
+import numpy as np
+import pytest
+
+from helpers import brute_force_pairs, flat_state, mixed_small, pair_set, small_rain, valid_mask
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+ABS = 1e-6
+
+
+def _scenes():
+    from pe_amd import scenes
+    return {
+        "tower": scenes.tower_scene(),
+        "rain300": small_rain(300),
+        "mixed200": mixed_small(200),
+    }
+
+
+@pytest.fixture(scope="module")
+def all_scenes():
+    return _scenes()
+
+
+def _pair(name, all_scenes):
+    import pe_amd
+    from pyoracle import OracleWorld
+    sc = all_scenes[name]
+    return sc, pe_amd.World(sc), OracleWorld(sc, 32)
+
+
+def _close(a, b, rel=REL, abs_=ABS):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.all(np.abs(a - b) <= abs_ + rel * np.abs(b))
+
+
+@pytest.mark.parametrize("name", ["tower", "rain300", "mixed200"])
+def test_stages_match_oracle(name, all_scenes):
+    sc, g, o = _pair(name, all_scenes)
+    print(f"[stages] scene={name} n={sc.n}", flush=True)
+    for step in range(8):
+        g.integrate(); o.integrate()
+        g.sync()
+        # stage 0: bit-exact state and AABBs
+        assert np.array_equal(flat_state(g.state()), flat_state(o.state()).astype(np.float32)), f"state step {step}"
+        gmn, gmx = g.aabbs()
+        omn, omx = o.aabbs()
+        assert np.array_equal(gmn, omn.astype(np.float32)) and np.array_equal(gmx, omx.astype(np.float32))
+        # stage 1: pair list identical in order; equal to brute force as a set
+        npg = g.broadphase()
+        npo = o.broadphase()
+        gp, op = g.pairs(), o.pairs()
+        assert npg == npo
+        assert np.array_equal(gp, op), f"pair order step {step}"
+        assert pair_set(gp) == brute_force_pairs(gmn, gmx, sc.kinematic)
+        assert g.axis() == o.axis()
+        # stage 2: contact manifolds
+        g.narrowphase(); o.narrowphase()
+        gn, gpts = g.contacts()
+        on, _, opts = o.contacts()
+        assert np.array_equal(gn, on), f"point counts step {step}"
+        vm = valid_mask(opts)
+        assert np.array_equal(valid_mask(gpts), vm)
+        assert _close(gpts[vm], opts[vm]), f"contact points step {step}"
+        assert _close(gpts[..., 12], opts[..., 12])  # distances of every slot, stale ones included
+        # stage 3: solve in colour order
+        g.solve(); o.solve()
+        g.sync()
+        order, cs = g.solve_order()
+        assert np.array_equal(order, o.solve_order())
+        assert len(cs) - 1 == o.num_colors()
+        _, _, gv, gw = g.state()
+        _, _, ov, ow = o.state()
+        assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        # keep the two in lock-step on the exact same state
+        o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+
+
+@pytest.mark.parametrize("name", ["tower", "rain300"])
+def test_trajectory_60_steps(name, all_scenes):
+    sc, g, o = _pair(name, all_scenes)
+    print(f"[traj60] scene={name} n={sc.n}", flush=True)
+    g.step(60)
+    o.step(60)
+    g.sync()
+    pg, rg, _, _ = g.state()
+    po, ro, _, _ = o.state()
+    d = np.linalg.norm(pg - po, axis=1)
+    rms = float(np.sqrt(np.mean(d ** 2)))
+    print(f"[traj60] {name}: position rms {rms:.3e} max {d.max():.3e}", flush=True)
+    assert rms <= 1e-3 and d.max() <= 1e-2
+
+
+def test_solve_order_is_a_valid_coloring(all_scenes):
+    import pe_amd
+    sc = all_scenes["rain300"]
+    g = pe_amd.World(sc)
+    g.step(30)
+    g.sync()
+    order, cs = g.solve_order()
+    ab = g.pairs()
+    kin = sc.kinematic.astype(bool)
+    for c in range(len(cs) - 1):
+        bodies = ab[order[cs[c]:cs[c + 1]]].ravel()
+        dyn = bodies[~kin[bodies]]
+        assert len(dyn) == len(np.unique(dyn)), f"colour {c} shares a dynamic body"
