@@ -1,0 +1,641 @@
+// Batched and served stepping: many independent worlds per launch.
+//
+// k_world_step<BLOCK> runs one whole World::step for one world per workgroup:
+//   integrate (thread / body) -> SAP keys -> bitonic sort of (key, index) in LDS ->
+//   candidate runs + load-balanced sweep (two passes, (i, j) order) -> next axis ->
+//   narrowphase (thread / pair) -> solver rows -> colouring (claims in LDS) ->
+//   colour-ordered Gauss-Seidel sweeps -> velocity sync.
+// Every routine is the one the single-world pipeline uses (pe_pipeline.hpp), and the sort,
+// sweep order and colouring rule are the same, so a world stepped here is bit-identical to
+// the same world stepped alone -- batching never changes a result.
+//
+// Worlds are grouped into size classes; each class launches with its own block size (chosen by
+// an autotuner that times the candidates) and its LDS sized for the largest world in the class.
+// One step = one launch per class, captured once into a hipGraph and replayed.
+#include "pe_common.hpp"
+#include "pe_device.hpp"
+#include "pe_prims.hpp"
+#include "pe_pipeline.hpp"
+#include "../../include/pe_amd.h"
+#include "../../include/pe_amd_batch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <numeric>
+#include <vector>
+
+using namespace pe;
+
+namespace {
+
+struct WorldDesc {      // static per world
+    int32_t body_off;
+    int32_t n;
+    int32_t pair_off;
+    int32_t pair_cap;
+};
+struct WorldState {     // per world, updated by every step
+    int32_t npairs;
+    int32_t prev_npairs;
+    int32_t axis;
+    int32_t ncolors;
+    int32_t npoints;
+    int32_t overflow;
+    int32_t pad[2];
+};
+
+struct BatchDev {
+    BodyView b;
+    PairStore ps;
+    const WorldDesc* desc;
+    WorldState* st;
+    int32_t* act;       // [2 * total pair cap]
+    int32_t* order;     // [total pair cap]
+    int32_t* cstart;    // [total pair cap + nworlds]
+    float dt;
+    float g[3];
+    int iterations;
+};
+
+constexpr uint64_t KEY_PAD = ~0ull;
+
+// LDS layout for a class whose largest world has nmax bodies.
+__host__ __device__ inline int np2(int n) { int p = 1; while (p < n) p <<= 1; return p; }
+__host__ inline size_t class_lds_bytes(int nmax) {
+    const int P = np2(std::max(nmax, 2));
+    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/ + (size_t)nmax * 4 /*claim*/;
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int wsum[BLOCK / WAVE + 1];
+    __shared__ double red[6][BLOCK / WAVE];
+    const int w = wlist[blockIdx.x];
+    const WorldDesc wd = d.desc[w];
+    const int n = wd.n, off = wd.body_off, poff = wd.pair_off;
+    const BodyView& b = d.b;
+    const int tid = threadIdx.x;
+    const int P2 = np2(max(nmax, 2));
+    uint64_t* key = (uint64_t*)lds;
+    float* skey = (float*)(key + P2);
+    uint32_t* cand = (uint32_t*)(skey + nmax);
+    int* claim = (int*)(cand + nmax + 1);
+    const int axis = d.st[w].axis;
+    const int prev_np = d.st[w].prev_npairs;
+
+    // ---- stage 0: integrate (world.cpp:35-127)
+    for (int i = tid; i < n; i += BLOCK) integrate_body(b, off + i, d.dt, d.g);
+    __syncthreads();
+
+    // ---- stage 1: sweep and prune (broad_phase_sweep_and_prune.cpp:6-41)
+    for (int i = tid; i < P2; i += BLOCK)
+        key[i] = i < n ? (((uint64_t)float_key(b.at(F_AMIN + axis, off + i) + 0.0f) << 32) | (uint32_t)i) : KEY_PAD;
+    __syncthreads();
+    for (int k = 2; k <= P2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < P2; i += BLOCK) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = key[i], c = key[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > c) == up) { key[i] = c; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int s = tid; s < n; s += BLOCK) skey[s] = b.at(F_AMIN + axis, off + (int)(uint32_t)key[s]) + 0.0f;
+    __syncthreads();
+    // candidate run of sorted position s: j in (s, first skey > max_i)
+    int carry = 0;
+    for (int base = 0; base < n; base += BLOCK) {
+        const int s = base + tid;
+        int c = 0;
+        if (s < n) {
+            const int i = (int)(uint32_t)key[s];
+            const float mx = b.at(F_AMAX + axis, off + i);
+            int lo = s + 1, hi = n;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if (skey[mid] > mx) hi = mid; else lo = mid + 1; }
+            c = lo - s - 1;
+        }
+        int tot;
+        const int ex = block_exclusive_scan(c, wsum, &tot);
+        if (s < n) cand[s] = (uint32_t)(carry + ex);
+        carry += tot;
+    }
+    if (tid == 0) cand[n] = (uint32_t)carry;
+    __syncthreads();
+    const uint32_t S = cand[n];
+    const uint32_t C = (S + BLOCK - 1) / BLOCK;
+    const uint32_t t0 = min(S, (uint32_t)tid * C), t1 = min(S, t0 + C);
+    int npairs;
+    {
+        uint32_t cnt = 0;
+        int s0 = 0;
+        if (t0 < t1) {
+            int lo = 0, hi = n;
+            while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cand[mid] <= t0) lo = mid; else hi = mid; }
+            s0 = lo;
+        }
+        int s = s0;
+        for (uint32_t t = t0; t < t1; t++) {
+            while (cand[s + 1] <= t) s++;
+            const int i = (int)(uint32_t)key[s];
+            const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
+            cnt += bp_valid(b, off + i, off + j) ? 1u : 0u;
+        }
+        int tot;
+        int out = block_exclusive_scan((int)cnt, wsum, &tot);
+        npairs = min(tot, wd.pair_cap);
+        if (tid == 0 && tot > wd.pair_cap) d.st[w].overflow |= 1;
+        s = s0;
+        for (uint32_t t = t0; t < t1; t++) {
+            while (cand[s + 1] <= t) s++;
+            const int i = (int)(uint32_t)key[s];
+            const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
+            if (bp_valid(b, off + i, off + j)) {
+                if (out < wd.pair_cap) { d.ps.pa[poff + out] = off + i; d.ps.pb[poff + out] = off + j; }
+                out++;
+            }
+        }
+    }
+    // next axis: variance of AABB centres (broad_phase_sweep_and_prune.cpp:37-40)
+    {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = tid; i < n; i += BLOCK) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float c = (b.at(F_AMIN + k, off + i) + b.at(F_AMAX + k, off + i)) * 0.5f;
+                acc[k] += (double)c;
+                acc[3 + k] += (double)(c * c);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            double v = acc[k];
+            for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
+            if ((tid & (WAVE - 1)) == 0) red[k][tid / WAVE] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s6[6];
+            for (int k = 0; k < 6; k++) { s6[k] = 0; for (int q = 0; q < BLOCK / WAVE; q++) s6[k] += red[k][q]; }
+            double v[3];
+            for (int k = 0; k < 3; k++) v[k] = s6[3 + k] - s6[k] * s6[k] / (double)n;
+            int ax = 0;
+            if (v[1] > v[0]) ax = 1;
+            if (v[2] > v[ax]) ax = 2;
+            d.st[w].axis = ax;
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 2: narrowphase (simple_narrow_phase.cpp:7-41)
+    for (int p = tid; p < npairs; p += BLOCK) narrow_pair(b, d.ps, poff + p, poff + prev_np);
+    __syncthreads();
+
+    // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
+    for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
+    for (int i = tid; i < n; i += BLOCK) claim[i] = 0x7fffffff;
+    __syncthreads();
+    int* act = d.act + 2 * (size_t)poff;
+    int* act2 = act + wd.pair_cap;
+    int* order = d.order + poff;
+    int* cs = d.cstart + poff + w;
+    int nact = 0, npts = 0;
+    for (int base = 0; base < npairs; base += BLOCK) {
+        const int p = base + tid;
+        int flag = 0, np_ = 0;
+        if (p < npairs) {
+            np_ = d.ps.npts[poff + p];
+            if (np_ > 0) flag = !(b.iat(I_KIN, d.ps.pa[poff + p]) && b.iat(I_KIN, d.ps.pb[poff + p]));
+        }
+        int tot;
+        const int ex = block_exclusive_scan(flag, wsum, &tot);
+        if (flag) act[nact + ex] = poff + p;
+        nact += tot;
+        int totp;
+        block_exclusive_scan(np_, wsum, &totp);
+        npts += totp;
+    }
+    __syncthreads();
+    int ncol = 0, out = 0;
+    while (nact > 0) {
+        for (int q = tid; q < nact; q += BLOCK) {
+            const int p = act[q];
+            const int ia = d.ps.pa[p], ib = d.ps.pb[p];
+            if (!b.iat(I_KIN, ia)) atomicMin(&claim[ia - off], p);
+            if (!b.iat(I_KIN, ib)) atomicMin(&claim[ib - off], p);
+        }
+        __syncthreads();
+        if (tid == 0) cs[ncol] = out;
+        int nrest = 0;
+        for (int base = 0; base < nact; base += BLOCK) {
+            const int q = base + tid;
+            int win = 0, p = -1;
+            if (q < nact) {
+                p = act[q];
+                const int ia = d.ps.pa[p], ib = d.ps.pb[p];
+                win = (b.iat(I_KIN, ia) || claim[ia - off] == p) && (b.iat(I_KIN, ib) || claim[ib - off] == p);
+            }
+            int totw, totl;
+            const int exw = block_exclusive_scan(win, wsum, &totw);
+            const int exl = block_exclusive_scan(q < nact && !win, wsum, &totl);
+            if (q < nact) {
+                if (win) order[out + exw] = p;
+                else act2[nrest + exl] = p;
+            }
+            out += totw;
+            nrest += totl;
+        }
+        __syncthreads();
+        for (int q = tid; q < nact; q += BLOCK) {
+            const int p = act[q];
+            claim[d.ps.pa[p] - off] = 0x7fffffff;
+            claim[d.ps.pb[p] - off] = 0x7fffffff;
+        }
+        int* t = act; act = act2; act2 = t;
+        nact = nrest;
+        ncol++;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        cs[ncol] = out;
+        d.st[w].ncolors = ncol;
+        d.st[w].npairs = npairs;
+        d.st[w].prev_npairs = npairs;
+        d.st[w].npoints = npts;
+    }
+    __syncthreads();
+    for (int it = 0; it < d.iterations; it++) {
+        for (int c = 0; c < ncol; c++) {
+            const int c0 = cs[c], c1 = cs[c + 1];
+            for (int q = c0 + tid; q < c1; q += BLOCK) iterate_pair(b, d.ps, order[q]);
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
+}
+
+__global__ void k_refresh_range(BodyView b, int first, int count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) refresh_body(b, first + i);
+}
+
+// ------------------------------------------------------------------ host-side helpers
+int body_pair_cap(int n) { return 8 * n + 64; }
+
+// Host staging of one world's bodies into the SoA store (mass properties as
+// RigidBody::setMass + setCollisionShape compute them, rigidbody.cpp:11-41).
+void stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<int32_t>& hi, size_t stride, int off) {
+    for (int i = 0; i < sc->n; i++) {
+        const size_t col = (size_t)off + i;
+        auto F = [&](int f) -> float& { return hf[(size_t)f * stride + col]; };
+        const int t = sc->shape_type[i];
+        hi[(size_t)I_TYPE * stride + col] = t;
+        hi[(size_t)I_KIN * stride + col] = sc->kinematic[i] ? 1 : 0;
+        for (int k = 0; k < 4; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
+        for (int k = 0; k < 3; k++) {
+            F(F_POS + k) = (float)sc->position[3 * i + k];
+            F(F_V + k) = (float)sc->linear_velocity[3 * i + k];
+            F(F_W + k) = (float)sc->angular_velocity[3 * i + k];
+        }
+        for (int k = 0; k < 9; k++) F(F_ROT + k) = (float)sc->rotation[9 * i + k];
+        const float mass = (float)sc->mass[i];
+        F(F_MASS) = mass;
+        F(F_IM) = mass > EPS ? 1.0f / mass : 0.0f;
+        float I[3] = {0, 0, 0};
+        if (t == PE_SHAPE_BOX) {
+            const float x2 = F(F_SP) * F(F_SP), y2 = F(F_SP + 1) * F(F_SP + 1), z2 = F(F_SP + 2) * F(F_SP + 2);
+            I[0] = (y2 + z2) / 12.0f; I[1] = (x2 + z2) / 12.0f; I[2] = (x2 + y2) / 12.0f;
+        } else if (t == PE_SHAPE_SPHERE) {
+            const float r = F(F_SP);
+            I[0] = I[1] = I[2] = (float)(2.0 / 5.0) * r * r;
+        } else {
+            const float r = F(F_SP), h = F(F_SP + 1);
+            const float r_2 = r * r, h_2 = h * h;
+            I[1] = r_2 * 0.5f;
+            I[0] = I[2] = r_2 / 4.0f + h_2 / 12.0f;
+        }
+        const float a = I[0] * mass, bb = I[1] * mass, c = I[2] * mass;
+        for (int k = 0; k < 9; k++) F(F_LINV + k) = 0.0f;
+        if (mass > EPS) {
+            const float c0 = bb * c - 0.0f * 0.0f;
+            const float det = c0 * a + (0.0f + 0.0f);
+            const float inv = 1.0f / det;
+            F(F_LINV + 0) = c0 * inv;
+            F(F_LINV + 4) = (c * a - 0.0f * 0.0f) * inv;
+            F(F_LINV + 8) = (a * bb - 0.0f * 0.0f) * inv;
+        }
+        F(F_MU) = (float)sc->friction[i];
+        F(F_E) = (float)sc->restitution[i];
+        F(F_LD) = sc->linear_damping ? (float)sc->linear_damping[i] : 0.0f;
+        F(F_AD) = sc->angular_damping ? (float)sc->angular_damping[i] : 0.0f;
+    }
+}
+
+int check_scene(const pe_scene_desc* sc) {
+    if (!sc || sc->n <= 0) { pe_set_error("empty scene"); return PE_ERR_INVALID; }
+    for (int i = 0; i < sc->n; i++) {
+        const int t = sc->shape_type[i];
+        if (t != PE_SHAPE_BOX && t != PE_SHAPE_SPHERE) {
+            pe_set_error("shape type " + std::to_string(t) + " is not supported by this build");
+            return PE_ERR_UNSUPPORTED;
+        }
+    }
+    return PE_OK;
+}
+
+// Allows every instantiation the full LDS of a CU (worlds up to 4096 bodies need ~80 KiB).
+void set_lds_limits() {
+    static bool done = false;
+    if (done) return;
+    done = true;
+    const int bytes = 150 * 1024;
+    hipFuncSetAttribute((const void*)k_world_step<64>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_world_step<128>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_world_step<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_world_step<512>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_world_step<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// Launch one class: template dispatch over the supported block sizes.
+hipError_t launch_class(int block, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
+    const size_t lds = class_lds_bytes(nmax);
+    switch (block) {
+        case 64: hipLaunchKernelGGL(k_world_step<64>, dim3(count), dim3(64), lds, s, d, wl, nmax); break;
+        case 128: hipLaunchKernelGGL(k_world_step<128>, dim3(count), dim3(128), lds, s, d, wl, nmax); break;
+        case 256: hipLaunchKernelGGL(k_world_step<256>, dim3(count), dim3(256), lds, s, d, wl, nmax); break;
+        case 512: hipLaunchKernelGGL(k_world_step<512>, dim3(count), dim3(512), lds, s, d, wl, nmax); break;
+        default: hipLaunchKernelGGL(k_world_step<1024>, dim3(count), dim3(1024), lds, s, d, wl, nmax); break;
+    }
+    return hipGetLastError();
+}
+
+struct Class {
+    int lo, hi;         // body-count range [lo, hi]
+    int block;
+    int count = 0;      // worlds in the class
+    int nmax = 0;
+    int list_off = 0;   // offset into the device world list
+};
+
+// Default class table: body-count ranges and their block sizes.
+std::vector<Class> default_classes() {
+    return {{1, 24, 64}, {25, 64, 64}, {65, 160, 128}, {161, 384, 256}, {385, 1024, 256}, {1025, 1 << 30, 512}};
+}
+
+}  // namespace
+
+// ====================================================================== batch object
+struct pe_batch {
+    int nworlds = 0;
+    int total_bodies = 0, stride = 0;
+    int64_t total_pair_cap = 0;
+    std::vector<WorldDesc> hdesc;
+    BatchDev d{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    std::vector<Class> classes;
+    int32_t* dlist = nullptr;      // world ids grouped by class
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int64_t dyn_bodies = 0;
+
+    ~pe_batch() {
+        if (stream) hipStreamSynchronize(stream);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        for (void* p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+    template <class T> int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        PE_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+        PE_HIP(hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), stream));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return PE_OK;
+    }
+    // Groups worlds into classes (largest worlds first inside a class) and uploads the list.
+    int assign_classes() {
+        std::vector<std::vector<int>> members(classes.size());
+        for (int w = 0; w < nworlds; w++) {
+            const int n = hdesc[w].n;
+            for (size_t c = 0; c < classes.size(); c++)
+                if (n >= classes[c].lo && n <= classes[c].hi) { members[c].push_back(w); break; }
+        }
+        std::vector<int32_t> list;
+        for (size_t c = 0; c < classes.size(); c++) {
+            auto& m = members[c];
+            std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
+            classes[c].count = (int)m.size();
+            classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
+            classes[c].list_off = (int)list.size();
+            list.insert(list.end(), m.begin(), m.end());
+        }
+        if (!dlist && alloc(&dlist, std::max(1, nworlds)) != PE_OK) return PE_ERR_DEVICE;
+        PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        return PE_OK;
+    }
+    int launch_step() {
+        for (auto& c : classes) {
+            if (c.count == 0) continue;
+            hipError_t e = launch_class(c.block, d, dlist + c.list_off, c.count, c.nmax, stream);
+            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+        }
+        return PE_OK;
+    }
+    void drop_graph() {
+        if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { hipGraphDestroy(graph); graph = nullptr; }
+    }
+};
+
+extern "C" {
+
+int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes) {
+    if (!scene || !bytes) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    const int64_t n = scene->n, pc = body_pair_cap(scene->n);
+    *bytes = n * (int64_t)(NF + NI) * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 6 * 4 + 4 * 4) + (int64_t)sizeof(WorldDesc) +
+             (int64_t)sizeof(WorldState);
+    return PE_OK;
+}
+
+int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world_params* prm, pe_batch** out) {
+    if (nworlds <= 0 || !scenes || !prm || !out) { pe_set_error("pe_batch_create: invalid argument"); return PE_ERR_INVALID; }
+    for (int w = 0; w < nworlds; w++) {
+        if (int rc = check_scene(&scenes[w])) return rc;
+        if (scenes[w].n > PE_BATCH_MAX_WORLD_BODIES) { pe_set_error("world too large for the batched kernel"); return PE_ERR_INVALID; }
+    }
+    set_lds_limits();
+    auto* B = new pe_batch();
+    auto fail = [&](int rc) { delete B; return rc; };
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) { pe_set_error("stream"); return fail(PE_ERR_DEVICE); }
+    B->nworlds = nworlds;
+    B->hdesc.resize(nworlds);
+    int boff = 0;
+    int64_t poff = 0;
+    for (int w = 0; w < nworlds; w++) {
+        B->hdesc[w].body_off = boff;
+        B->hdesc[w].n = scenes[w].n;
+        B->hdesc[w].pair_off = (int32_t)poff;
+        B->hdesc[w].pair_cap = body_pair_cap(scenes[w].n);
+        boff += scenes[w].n;
+        poff += B->hdesc[w].pair_cap;
+    }
+    if (poff > (int64_t)1 << 30) { pe_set_error("batch too large"); return fail(PE_ERR_INVALID); }
+    B->total_bodies = boff;
+    B->stride = (boff + 63) & ~63;
+    B->total_pair_cap = poff;
+    BatchDev& d = B->d;
+    d.dt = (float)prm->dt;
+    for (int k = 0; k < 3; k++) d.g[k] = (float)prm->gravity[k];
+    d.iterations = prm->iterations;
+    d.b.n = boff;
+    d.b.stride = B->stride;
+    d.ps.cs = (size_t)poff * 8;
+    const size_t S = B->stride;
+    int rc = PE_OK;
+    rc |= B->alloc(&d.b.f, NF * S);
+    rc |= B->alloc(&d.b.i, NI * S);
+    rc |= B->alloc(&d.ps.pa, poff);
+    rc |= B->alloc(&d.ps.pb, poff);
+    rc |= B->alloc(&d.ps.cp, (size_t)CF * poff * 8);
+    rc |= B->alloc(&d.ps.npts, poff);
+    rc |= B->alloc(&d.ps.swap, poff);
+    rc |= B->alloc(&d.ps.rows, (size_t)RF * poff * 8);
+    rc |= B->alloc(&d.act, 2 * (size_t)poff);
+    rc |= B->alloc(&d.order, poff);
+    rc |= B->alloc(&d.cstart, poff + nworlds);
+    WorldDesc* ddesc = nullptr;
+    WorldState* dst = nullptr;
+    rc |= B->alloc(&ddesc, nworlds);
+    rc |= B->alloc(&dst, nworlds);
+    if (rc != PE_OK) return fail(PE_ERR_DEVICE);
+    d.desc = ddesc;
+    d.st = dst;
+    std::vector<float> hf(NF * S, 0.0f);
+    std::vector<int32_t> hi(NI * S, 0);
+    int64_t ndyn = 0;
+    for (int w = 0; w < nworlds; w++) {
+        stage_bodies(&scenes[w], hf, hi, S, B->hdesc[w].body_off);
+        for (int i = 0; i < scenes[w].n; i++) ndyn += scenes[w].kinematic[i] ? 0 : 1;
+    }
+    B->dyn_bodies = ndyn;
+    if (hipMemcpyAsync(d.b.f, hf.data(), hf.size() * 4, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipMemcpyAsync(d.b.i, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
+        hipMemcpyAsync(ddesc, B->hdesc.data(), nworlds * sizeof(WorldDesc), hipMemcpyHostToDevice, B->stream) != hipSuccess) {
+        pe_set_error("upload failed");
+        return fail(PE_ERR_DEVICE);
+    }
+    hipLaunchKernelGGL(k_refresh_range, dim3(std::min(4096, (boff + 255) / 256)), dim3(256), 0, B->stream, d.b, 0, boff);
+    B->classes = default_classes();
+    if (B->assign_classes() != PE_OK) return fail(PE_ERR_DEVICE);
+    if (hipStreamSynchronize(B->stream) != hipSuccess) { pe_set_error("init sync"); return fail(PE_ERR_DEVICE); }
+    *out = B;
+    return PE_OK;
+}
+
+int pe_batch_destroy(pe_batch* B) {
+    delete B;
+    return PE_OK;
+}
+
+int pe_batch_set_config(pe_batch* B, int32_t nclasses, const int32_t* upper, const int32_t* block) {
+    if (!B || nclasses <= 0 || !upper || !block) { pe_set_error("invalid config"); return PE_ERR_INVALID; }
+    std::vector<Class> cl;
+    int lo = 1;
+    for (int c = 0; c < nclasses; c++) {
+        const int bs = block[c];
+        if (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024) { pe_set_error("block size must be 64..1024, power of 2"); return PE_ERR_INVALID; }
+        const int hi = c == nclasses - 1 ? (1 << 30) : upper[c];
+        if (hi < lo) { pe_set_error("class bounds must increase"); return PE_ERR_INVALID; }
+        cl.push_back(Class{lo, hi, bs});
+        lo = hi + 1;
+    }
+    B->classes = cl;
+    B->drop_graph();
+    return B->assign_classes();
+}
+
+int pe_batch_get_config(pe_batch* B, int32_t* nclasses, int32_t* upper, int32_t* block, int32_t* count, int32_t cap) {
+    if (!B || !nclasses) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    *nclasses = (int32_t)B->classes.size();
+    for (int c = 0; c < (int)B->classes.size() && c < cap; c++) {
+        if (upper) upper[c] = std::min(B->classes[c].hi, 1 << 30);
+        if (block) block[c] = B->classes[c].block;
+        if (count) count[c] = B->classes[c].count;
+    }
+    return PE_OK;
+}
+
+int pe_batch_capture_graph(pe_batch* B) {
+    if (!B) { pe_set_error("null batch"); return PE_ERR_INVALID; }
+    B->drop_graph();
+    PE_HIP(hipStreamSynchronize(B->stream));
+    PE_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+    int rc = B->launch_step();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(B->stream, &g);
+    if (rc != PE_OK) return rc;
+    if (e != hipSuccess) { pe_set_error(std::string("capture: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+    B->graph = g;
+    PE_HIP(hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0));
+    return PE_OK;
+}
+
+int pe_batch_step(pe_batch* B, int32_t nsteps) {
+    if (!B) { pe_set_error("null batch"); return PE_ERR_INVALID; }
+    for (int s = 0; s < nsteps; s++) {
+        if (B->gexec) PE_HIP(hipGraphLaunch(B->gexec, B->stream));
+        else if (int rc = B->launch_step()) return rc;
+    }
+    return PE_OK;
+}
+
+int pe_batch_sync(pe_batch* B) {
+    if (!B) { pe_set_error("null batch"); return PE_ERR_INVALID; }
+    PE_HIP(hipStreamSynchronize(B->stream));
+    return PE_OK;
+}
+
+int pe_batch_get_world_state(pe_batch* B, int32_t w, float* pos, float* rot, float* v, float* av) {
+    if (!B || w < 0 || w >= B->nworlds) { pe_set_error("world index out of range"); return PE_ERR_INVALID; }
+    PE_HIP(hipStreamSynchronize(B->stream));
+    const WorldDesc& wd = B->hdesc[w];
+    std::vector<float> tmp(wd.n);
+    auto get = [&](int f0, int nf, float* out) -> int {
+        if (!out) return PE_OK;
+        for (int k = 0; k < nf; k++) {
+            PE_HIP(hipMemcpy(tmp.data(), B->d.b.f + (size_t)(f0 + k) * B->stride + wd.body_off, wd.n * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < wd.n; i++) out[(size_t)i * nf + k] = tmp[i];
+        }
+        return PE_OK;
+    };
+    int rc;
+    if ((rc = get(F_POS, 3, pos)) || (rc = get(F_ROT, 9, rot)) || (rc = get(F_V, 3, v)) || (rc = get(F_W, 3, av))) return rc;
+    return PE_OK;
+}
+
+int pe_batch_get_counts(pe_batch* B, int64_t* counts) {
+    if (!B || !counts) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    std::vector<WorldState> st(B->nworlds);
+    PE_HIP(hipMemcpyAsync(st.data(), B->d.st, st.size() * sizeof(WorldState), hipMemcpyDeviceToHost, B->stream));
+    PE_HIP(hipStreamSynchronize(B->stream));
+    int64_t pairs = 0, points = 0, maxcol = 0, ovf = 0;
+    for (auto& s : st) { pairs += s.npairs; points += s.npoints; maxcol = std::max<int64_t>(maxcol, s.ncolors); ovf += s.overflow != 0; }
+    counts[0] = B->total_bodies;
+    counts[1] = B->dyn_bodies;
+    counts[2] = pairs;
+    counts[3] = points;
+    counts[4] = maxcol;
+    counts[5] = ovf;
+    counts[6] = B->nworlds;
+    return PE_OK;
+}
+
+}  // extern "C"

@@ -52,7 +52,7 @@ constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 
 // Per-tile histogram of one 8-bit digit, written digit-major: hist[d * nblocks + b].
-__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+static __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
                                                            uint32_t* __restrict__ hist, int nblocks) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __res
 }
 
 // Exclusive scan of `n` uint32 in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_scan_u32(uint32_t* __restrict__ data, int n, uint32_t* __restrict__ total) {
+static __global__ void __launch_bounds__(1024) k_scan_u32(uint32_t* __restrict__ data, int n, uint32_t* __restrict__ total) {
     __shared__ int wsum[17];
     int carry = 0;
     for (int base = 0; base < n; base += 1024) {
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(1024) k_scan_u32(uint32_t* __restrict__ data, 
 }
 
 // Exclusive scan of *count uint32 in place (count read on the device), one block of 1024 threads.
-__global__ void __launch_bounds__(1024) k_scan_u32_dyn(uint32_t* __restrict__ data, const uint32_t* __restrict__ count,
+static __global__ void __launch_bounds__(1024) k_scan_u32_dyn(uint32_t* __restrict__ data, const uint32_t* __restrict__ count,
                                                        uint32_t* __restrict__ total) {
     __shared__ int wsum[17];
     const int n = (int)*count;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(1024) k_scan_u32_dyn(uint32_t* __restrict__ da
 
 // Stable scatter of one digit pass.  Sub-tiles of 256 keys are ranked in order: lanes with
 // equal digits are found with 8 ballots, waves are combined through LDS.
-__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+static __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                               uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
                                                               int shift, const uint32_t* __restrict__ offs, int nblocks) {
     __shared__ uint32_t running[256];

@@ -14,6 +14,7 @@
 #include "pe_common.hpp"
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
+#include "pe_pipeline.hpp"
 #include "../../include/pe_amd.h"
 
 #include <algorithm>
@@ -49,12 +50,7 @@ struct DevCounts {          // device-side counters of one step
 
 struct Dev {
     BodyView b;
-    int32_t* pa;            // [pcap]
-    int32_t* pb;
-    float* cp;              // [CF][pcap*8]
-    int32_t* npts;          // [pcap]
-    int32_t* swap;          // [pcap]
-    float* rows;            // [RF][pcap*8]
+    PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
     int32_t* order;         // [pcap]
     int32_t* act;           // [2*pcap] colouring work lists
     int32_t* color_start;   // [MAX_COLORS+1]
@@ -76,76 +72,17 @@ struct Dev {
     int iterations;
 };
 
-__device__ __forceinline__ f3 ld3(const BodyView& b, int f, int i) { return f3{b.at(f, i), b.at(f + 1, i), b.at(f + 2, i)}; }
-__device__ __forceinline__ void st3(const BodyView& b, int f, int i, f3 v) { b.at(f, i) = v.x; b.at(f + 1, i) = v.y; b.at(f + 2, i) = v.z; }
-__device__ __forceinline__ m33 ld9(const BodyView& b, int f, int i) {
-    m33 r;
-#pragma unroll
-    for (int k = 0; k < 9; k++) r.m[k] = b.at(f + k, i);
-    return r;
-}
-__device__ __forceinline__ void st9(const BodyView& b, int f, int i, const m33& r) {
-#pragma unroll
-    for (int k = 0; k < 9; k++) b.at(f + k, i) = r.m[k];
-}
-
 // ------------------------------------------------------------------ stage 0: integrate
 // World::updateObjectStatus (world.cpp:35-119) with zero sleep thresholds, then
 // World::applyExternalForce (world.cpp:121-127), then SequentialImpulseSolver::setupSolver's
 // clearTempVelocity (sequential_impulse_solver.cpp:17-20).
 __global__ void k_integrate(Dev d) {
-    const BodyView& b = d.b;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
-        const int type = b.iat(I_TYPE, i);
-        const bool kin = b.iat(I_KIN, i) != 0;
-        f3 pos = ld3(b, F_POS, i);
-        m33 R = ld9(b, F_ROT, i);
-        float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
-        if (!kin) {
-            f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
-            integrate_pose(pos, R, v, w, d.dt);
-            const m33 L = ld9(b, F_LINV, i);
-            st9(b, F_IW, i, mmt(mm(R, L), R));
-            st3(b, F_POS, i, pos);
-            st9(b, F_ROT, i, R);
-            // RigidBody::applyDamping (rigidbody.cpp:201-205)
-            const float pl = powf(1.0f - b.at(F_LD, i), d.dt), pw = powf(1.0f - b.at(F_AD, i), d.dt);
-            v = mul(v, pl);
-            w = mul(w, pw);
-            // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
-            const float m = b.at(F_MASS, i), im = b.at(F_IM, i);
-            const f3 f{d.g[0] * m, d.g[1] * m, d.g[2] * m};
-            v = add(v, mul(mul(f, im), d.dt));
-            st3(b, F_V, i, v);
-            st3(b, F_W, i, w);
-            st3(b, F_TV, i, v);
-            st3(b, F_TW, i, w);
-        } else {
-            st3(b, F_TV, i, f3{0, 0, 0});
-            st3(b, F_TW, i, f3{0, 0, 0});
-        }
-        f3 mn, mx;
-        shape_aabb(type, sp, R, pos, mn, mx);
-        st3(b, F_AMIN, i, mn);
-        st3(b, F_AMAX, i, mx);
-        b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) integrate_body(d.b, i, d.dt, d.g);
 }
 
 // Recomputes AABBs and scales only (after pe_world_set_state).
 __global__ void k_aabb_only(Dev d) {
-    const BodyView& b = d.b;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
-        const f3 pos = ld3(b, F_POS, i);
-        const m33 R = ld9(b, F_ROT, i);
-        float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
-        f3 mn, mx;
-        shape_aabb(b.iat(I_TYPE, i), sp, R, pos, mn, mx);
-        st3(b, F_AMIN, i, mn);
-        st3(b, F_AMAX, i, mx);
-        b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
-        st9(b, F_IW, i, mmt(mm(R, ld9(b, F_LINV, i)), R));
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) refresh_body(d.b, i);
 }
 
 // ------------------------------------------------------------------ stage 1: broadphase
@@ -183,15 +120,6 @@ __global__ void k_bp_cand(Dev d) {
     if (blockIdx.x == 0 && threadIdx.x == 0) d.cand[n] = 0;
 }
 
-// BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
-__device__ __forceinline__ bool bp_valid(const BodyView& b, int i, int j) {
-    if (b.iat(I_KIN, i) && b.iat(I_KIN, j)) return false;
-    if (b.at(F_AMIN, i) > b.at(F_AMAX, j) || b.at(F_AMIN, j) > b.at(F_AMAX, i)) return false;
-    if (b.at(F_AMIN + 1, i) > b.at(F_AMAX + 1, j) || b.at(F_AMIN + 1, j) > b.at(F_AMAX + 1, i)) return false;
-    if (b.at(F_AMIN + 2, i) > b.at(F_AMAX + 2, j) || b.at(F_AMIN + 2, j) > b.at(F_AMAX + 2, i)) return false;
-    return true;
-}
-
 // Locates candidate t: sorted position s with cand[s] <= t < cand[s+1] (cand holds offsets).
 __device__ __forceinline__ int bp_find(const uint32_t* off, int n, uint32_t t) {
     int lo = 0, hi = n;  // last s with off[s] <= t
@@ -221,7 +149,7 @@ __global__ void k_bp_chunks(Dev d) {
             const int j = (int)d.vals[cur_s + 1 + (int)(t - d.cand[cur_s])];
             if (bp_valid(d.b, ci, j)) {
                 if (EMIT) {
-                    if ((int)out < d.pcap) { d.pa[out] = ci; d.pb[out] = j; }
+                    if ((int)out < d.pcap) { d.ps.pa[out] = ci; d.ps.pb[out] = j; }
                     out++;
                 } else {
                     cntv++;
@@ -286,63 +214,10 @@ __global__ void k_bp_axis_final(Dev d, int nblocks) {
 }
 
 // ------------------------------------------------------------------ stage 2: narrowphase
-__device__ __forceinline__ void load_pose(const BodyView& b, int i, Pose& P) {
-    P.R = ld9(b, F_ROT, i);
-    P.p = ld3(b, F_POS, i);
-    P.scale = b.at(F_SCALE, i);
-}
-
-// SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) for one pair slot.
-// Slots below the previous pair count keep their distances from the previous step (the
-// reference reuses its ContactResult objects); slots beyond it start fresh.
-__device__ void narrow_pair(const Dev& d, int p, int prev_np) {
-    const BodyView& b = d.b;
-    const int ia = d.pa[p], ib = d.pb[p];
-    Cache c;
-    const size_t slot0 = (size_t)p * 8;
-    const size_t CS = (size_t)d.pcap * 8;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        c.wx[k] = REAL_MAX; c.wy[k] = REAL_MAX; c.wz[k] = REAL_MAX;
-        c.nx[k] = 0; c.ny[k] = 0; c.nz[k] = 0;
-        c.ax[k] = 0; c.ay[k] = 0; c.az[k] = 0;
-        c.bx[k] = 0; c.by[k] = 0; c.bz[k] = 0;
-        c.d[k] = p < prev_np ? d.cp[C_D * CS + slot0 + k] : REAL_MAX;
-    }
-    PairCtx x;
-    load_pose(b, ia, x.A);
-    load_pose(b, ib, x.B);
-    x.swap = false;
-    {
-        const float sa = x.A.scale, sb = x.B.scale;
-        const float th = (sa < sb ? sa : sb) * 0.02f;
-        x.same_th2 = th * th;
-    }
-    const int ta = b.iat(I_TYPE, ia), tb = b.iat(I_TYPE, ib);
-    const float spa[3] = {b.at(F_SP, ia), b.at(F_SP + 1, ia), b.at(F_SP + 2, ia)};
-    const float spb[3] = {b.at(F_SP, ib), b.at(F_SP + 1, ib), b.at(F_SP + 2, ib)};
-    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, spa, spb);
-    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, spa, spb[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, spb, spa[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, spa[0], spb[0]);
-    const int np = sort_points(c);
-    d.npts[p] = np;
-    d.swap[p] = x.swap ? 1 : 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const size_t s = slot0 + k;
-        d.cp[(C_WPOS + 0) * CS + s] = c.wx[k]; d.cp[(C_WPOS + 1) * CS + s] = c.wy[k]; d.cp[(C_WPOS + 2) * CS + s] = c.wz[k];
-        d.cp[(C_N + 0) * CS + s] = c.nx[k]; d.cp[(C_N + 1) * CS + s] = c.ny[k]; d.cp[(C_N + 2) * CS + s] = c.nz[k];
-        d.cp[(C_LA + 0) * CS + s] = c.ax[k]; d.cp[(C_LA + 1) * CS + s] = c.ay[k]; d.cp[(C_LA + 2) * CS + s] = c.az[k];
-        d.cp[(C_LB + 0) * CS + s] = c.bx[k]; d.cp[(C_LB + 1) * CS + s] = c.by[k]; d.cp[(C_LB + 2) * CS + s] = c.bz[k];
-        d.cp[C_D * CS + s] = c.d[k];
-    }
-}
-
 __global__ void __launch_bounds__(128) k_narrow(Dev d) {
     const int np = d.cnt->npairs;
     const int prev = d.cnt->prev_npairs;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) narrow_pair(d, p, prev);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) narrow_pair(d.b, d.ps, p, prev);
 }
 __global__ void k_narrow_finish(Dev d) {
     d.cnt->prev_npairs = d.cnt->npairs;
@@ -352,77 +227,7 @@ __global__ void k_narrow_finish(Dev d) {
 // SequentialImpulseSolver::setupSolver (sequential_impulse_solver.cpp:11-58), thread / pair.
 __global__ void k_setup(Dev d) {
     const int np = d.cnt->npairs;
-    const BodyView& b = d.b;
-    const size_t CS = (size_t)d.pcap * 8;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-        const int n = d.npts[p];
-        if (n == 0) continue;
-        const int ia = d.pa[p], ib = d.pb[p];
-        const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
-        const m33 RA = ld9(b, F_ROT, ia), RB = ld9(b, F_ROT, ib);
-        m33 Ia, Ib;
-        if (ka) { for (int k = 0; k < 9; k++) Ia.m[k] = 0.0f; } else Ia = ld9(b, F_IW, ia);
-        if (kb) { for (int k = 0; k < 9; k++) Ib.m[k] = 0.0f; } else Ib = ld9(b, F_IW, ib);
-        const float ims = (ka ? 0.0f : b.at(F_IM, ia)) + (kb ? 0.0f : b.at(F_IM, ib));
-        const f3 va = ld3(b, F_V, ia), wa = ld3(b, F_W, ia), vb = ld3(b, F_V, ib), wb = ld3(b, F_W, ib);
-        const float mu = sqrtf(b.at(F_MU, ia) * b.at(F_MU, ib));
-        const float e = sqrtf(b.at(F_E, ia) * b.at(F_E, ib));
-        for (int k = 0; k < n; k++) {
-            const size_t s = (size_t)p * 8 + k;
-            const f3 la{d.cp[(C_LA) * CS + s], d.cp[(C_LA + 1) * CS + s], d.cp[(C_LA + 2) * CS + s]};
-            const f3 lb{d.cp[(C_LB) * CS + s], d.cp[(C_LB + 1) * CS + s], d.cp[(C_LB + 2) * CS + s]};
-            const f3 nn{d.cp[(C_N) * CS + s], d.cp[(C_N + 1) * CS + s], d.cp[(C_N + 2) * CS + s]};
-            const Row r = make_row(RA, RB, la, lb, nn, d.cp[C_D * CS + s], ims, Ia, Ib, va, wa, vb, wb, mu, e, d.dt);
-            float* R = d.rows;
-            R[(R_RA) * CS + s] = r.ra.x; R[(R_RA + 1) * CS + s] = r.ra.y; R[(R_RA + 2) * CS + s] = r.ra.z;
-            R[(R_RB) * CS + s] = r.rb.x; R[(R_RB + 1) * CS + s] = r.rb.y; R[(R_RB + 2) * CS + s] = r.rb.z;
-            R[(R_N) * CS + s] = r.n.x; R[(R_N + 1) * CS + s] = r.n.y; R[(R_N + 2) * CS + s] = r.n.z;
-            R[(R_T0) * CS + s] = r.t0.x; R[(R_T0 + 1) * CS + s] = r.t0.y; R[(R_T0 + 2) * CS + s] = r.t0.z;
-            R[(R_T1) * CS + s] = r.t1.x; R[(R_T1 + 1) * CS + s] = r.t1.y; R[(R_T1 + 2) * CS + s] = r.t1.z;
-            R[R_RHS * CS + s] = r.rhs; R[R_MN * CS + s] = r.mn; R[R_MT0 * CS + s] = r.mt0; R[R_MT1 * CS + s] = r.mt1;
-            R[R_MU * CS + s] = r.mu; R[R_LN * CS + s] = 0.0f; R[R_LT0 * CS + s] = 0.0f; R[R_LT1 * CS + s] = 0.0f;
-        }
-    }
-}
-
-__device__ __forceinline__ Row load_row(const float* R, size_t CS, size_t s) {
-    Row r;
-    r.ra = f3{R[(R_RA) * CS + s], R[(R_RA + 1) * CS + s], R[(R_RA + 2) * CS + s]};
-    r.rb = f3{R[(R_RB) * CS + s], R[(R_RB + 1) * CS + s], R[(R_RB + 2) * CS + s]};
-    r.n = f3{R[(R_N) * CS + s], R[(R_N + 1) * CS + s], R[(R_N + 2) * CS + s]};
-    r.t0 = f3{R[(R_T0) * CS + s], R[(R_T0 + 1) * CS + s], R[(R_T0 + 2) * CS + s]};
-    r.t1 = f3{R[(R_T1) * CS + s], R[(R_T1 + 1) * CS + s], R[(R_T1 + 2) * CS + s]};
-    r.rhs = R[R_RHS * CS + s]; r.mn = R[R_MN * CS + s]; r.mt0 = R[R_MT0 * CS + s]; r.mt1 = R[R_MT1 * CS + s];
-    r.mu = R[R_MU * CS + s];
-    return r;
-}
-
-// RigidBody::applyTempImpulse (rigidbody.cpp:166-170)
-__device__ __forceinline__ void apply_temp(const BodyView& b, int i, f3& tv, f3& tw, f3 r, f3 J) {
-    tv = add(tv, mul(J, b.at(F_IM, i)));
-    tw = add(tw, mv(ld9(b, F_IW, i), cross(r, J)));
-}
-
-// FrictionContactConstraint::iterateSequentialImpulse (friction_contact_constraint.cpp:75-108)
-// over all rows of one pair.
-__device__ void iterate_pair(const Dev& d, int p) {
-    const BodyView& b = d.b;
-    const size_t CS = (size_t)d.pcap * 8;
-    const int ia = d.pa[p], ib = d.pb[p];
-    const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
-    f3 tva = ld3(b, F_TV, ia), twa = ld3(b, F_TW, ia), tvb = ld3(b, F_TV, ib), twb = ld3(b, F_TW, ib);
-    const int n = d.npts[p];
-    for (int k = 0; k < n; k++) {
-        const size_t s = (size_t)p * 8 + k;
-        const Row r = load_row(d.rows, CS, s);
-        float ln = d.rows[R_LN * CS + s], lt0 = d.rows[R_LT0 * CS + s], lt1 = d.rows[R_LT1 * CS + s];
-        const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
-        d.rows[R_LN * CS + s] = ln; d.rows[R_LT0 * CS + s] = lt0; d.rows[R_LT1 * CS + s] = lt1;
-        if (!ka) apply_temp(b, ia, tva, twa, r.ra, J);
-        if (!kb) apply_temp(b, ib, tvb, twb, r.rb, neg(J));
-    }
-    if (!ka) { st3(b, F_TV, ia, tva); st3(b, F_TW, ia, twa); }
-    if (!kb) { st3(b, F_TV, ib, tvb); st3(b, F_TW, ib, twb); }
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) setup_pair(d.b, d.ps, p, d.dt);
 }
 
 // Claims in global memory are updated by L2 atomics; read them with an L1-bypassing load.
@@ -454,7 +259,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
     for (int base = 0; base < np; base += blockDim.x) {
         const int p = base + threadIdx.x;
         int flag = 0;
-        if (p < np && d.npts[p] > 0) flag = !(b.iat(I_KIN, d.pa[p]) && b.iat(I_KIN, d.pb[p]));
+        if (p < np && d.ps.npts[p] > 0) flag = !(b.iat(I_KIN, d.ps.pa[p]) && b.iat(I_KIN, d.ps.pb[p]));
         int tot;
         const int ex = block_exclusive_scan(flag, wsum, &tot);
         if (flag) act[nact + ex] = p;
@@ -465,7 +270,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
     while (nact > 0 && ncol < MAX_COLORS) {
         for (int q = threadIdx.x; q < nact; q += blockDim.x) {
             const int p = act[q];
-            const int ia = d.pa[p], ib = d.pb[p];
+            const int ia = d.ps.pa[p], ib = d.ps.pb[p];
             if (!b.iat(I_KIN, ia)) atomicMin(&claim[ia], p);
             if (!b.iat(I_KIN, ib)) atomicMin(&claim[ib], p);
         }
@@ -477,7 +282,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
             int win = 0, p = -1;
             if (q < nact) {
                 p = act[q];
-                const int ia = d.pa[p], ib = d.pb[p];
+                const int ia = d.ps.pa[p], ib = d.ps.pb[p];
                 win = (b.iat(I_KIN, ia) || load_claim<LDS_CLAIM>(claim, ia) == p) &&
                       (b.iat(I_KIN, ib) || load_claim<LDS_CLAIM>(claim, ib) == p);
             }
@@ -494,8 +299,8 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
         __syncthreads();
         for (int q = threadIdx.x; q < nact; q += blockDim.x) {
             const int p = act[q];
-            claim[d.pa[p]] = INF;
-            claim[d.pb[p]] = INF;
+            claim[d.ps.pa[p]] = INF;
+            claim[d.ps.pb[p]] = INF;
         }
         int* t = act; act = act2; act2 = t;
         nact = nrest;
@@ -513,16 +318,12 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
     for (int it = 0; it < d.iterations; it++) {
         for (int c = 0; c < ncol; c++) {
             const int c0 = d.color_start[c], c1 = d.color_start[c + 1];
-            for (int q = c0 + threadIdx.x; q < c1; q += blockDim.x) iterate_pair(d, d.order[q]);
+            for (int q = c0 + threadIdx.x; q < c1; q += blockDim.x) iterate_pair(d.b, d.ps, d.order[q]);
             __syncthreads();
         }
     }
     // SequentialImpulseSolver::solve's syncTempVelocity (sequential_impulse_solver.cpp:71-80)
-    for (int i = threadIdx.x; i < b.n; i += blockDim.x) {
-        if (b.iat(I_KIN, i)) continue;
-        st3(b, F_V, i, ld3(b, F_TV, i));
-        st3(b, F_W, i, ld3(b, F_TW, i));
-    }
+    for (int i = threadIdx.x; i < b.n; i += blockDim.x) sync_body(b, i);
     (void)s_nact; (void)s_out;
 }
 
@@ -533,7 +334,7 @@ __global__ void k_count_points(Dev d) {
     for (int base = 0; base < np; base += blockDim.x) {
         const int p = base + threadIdx.x;
         int tot;
-        block_exclusive_scan(p < np ? d.npts[p] : 0, wsum, &tot);
+        block_exclusive_scan(p < np ? d.ps.npts[p] : 0, wsum, &tot);
         acc += tot;
     }
     if (threadIdx.x == 0) d.cnt->npoints = acc;
@@ -671,6 +472,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) { pe_set_error("stream create failed"); return fail(PE_ERR_DEVICE); }
     Dev& d = w->d;
     d.pcap = pcap;
+    d.ps.cs = (size_t)pcap * 8;
     d.dt = (float)prm->dt;
     for (int k = 0; k < 3; k++) d.g[k] = (float)prm->gravity[k];
     d.iterations = prm->iterations;
@@ -680,12 +482,12 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     int rc = PE_OK;
     rc |= w->alloc(&d.b.f, NF * S);
     rc |= w->alloc(&d.b.i, NI * S);
-    rc |= w->alloc(&d.pa, pcap);
-    rc |= w->alloc(&d.pb, pcap);
-    rc |= w->alloc(&d.cp, (size_t)CF * pcap * 8);
-    rc |= w->alloc(&d.npts, pcap);
-    rc |= w->alloc(&d.swap, pcap);
-    rc |= w->alloc(&d.rows, (size_t)RF * pcap * 8);
+    rc |= w->alloc(&d.ps.pa, pcap);
+    rc |= w->alloc(&d.ps.pb, pcap);
+    rc |= w->alloc(&d.ps.cp, (size_t)CF * pcap * 8);
+    rc |= w->alloc(&d.ps.npts, pcap);
+    rc |= w->alloc(&d.ps.swap, pcap);
+    rc |= w->alloc(&d.ps.rows, (size_t)RF * pcap * 8);
     rc |= w->alloc(&d.order, pcap);
     rc |= w->alloc(&d.act, 2 * (size_t)pcap);
     rc |= w->alloc(&d.color_start, MAX_COLORS + 1);
@@ -883,8 +685,8 @@ int pe_world_get_pairs(pe_world* w, int32_t* pairs, int32_t cap, int32_t* npairs
     const int m = std::min(cap, c.npairs);
     if (pairs && m > 0) {
         std::vector<int32_t> a(m), b(m);
-        PE_HIP(hipMemcpy(a.data(), w->d.pa, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-        PE_HIP(hipMemcpy(b.data(), w->d.pb, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PE_HIP(hipMemcpy(a.data(), w->d.ps.pa, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        PE_HIP(hipMemcpy(b.data(), w->d.ps.pb, m * sizeof(int32_t), hipMemcpyDeviceToHost));
         for (int i = 0; i < m; i++) { pairs[2 * i] = a[i]; pairs[2 * i + 1] = b[i]; }
     }
     return PE_OK;
@@ -895,12 +697,12 @@ int pe_world_get_contacts(pe_world* w, int32_t* npoints, float* points, int32_t 
     if (int rc = read_counts(w, c)) return rc;
     const int m = std::min(cap, c.npairs);
     if (m <= 0) return PE_OK;
-    if (npoints) PE_HIP(hipMemcpy(npoints, w->d.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (npoints) PE_HIP(hipMemcpy(npoints, w->d.ps.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (points) {
         const size_t CS = (size_t)w->d.pcap * 8;
         std::vector<float> tmp((size_t)m * 8);
         for (int f = 0; f < CF; f++) {
-            PE_HIP(hipMemcpy(tmp.data(), w->d.cp + f * CS, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            PE_HIP(hipMemcpy(tmp.data(), w->d.ps.cp + f * CS, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
             for (size_t s = 0; s < tmp.size(); s++) points[s * CF + f] = tmp[s];
         }
     }
