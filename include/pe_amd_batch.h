@@ -39,6 +39,43 @@ int pe_batch_get_counts(pe_batch* b, int64_t* counts /*7*/);
 /* Device bytes one world of this scene occupies (bodies, pair slots, solver rows). */
 int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes);
 
+/* Runs nsteps of the batch and returns the device time of those steps (hipEvents recorded on
+ * the batch's stream around them), in milliseconds. */
+int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
+
+/* ------------------------------------------------------------------ server
+ * Continuous batching of world-simulation requests.  A request (a scene and a horizon in
+ * steps) is queued by pe_server_submit.  Each pe_server_tick admits queued requests in
+ * arrival order while their device footprint fits the HBM budget and arena space, runs one
+ * step of every live world in one batched launch, and retires worlds that reached their
+ * horizon.  Uploads of admitted requests are issued on a copy stream while the current step
+ * runs.  Results are the final body states, bit-identical to stepping the same scene alone. */
+typedef struct pe_server pe_server;
+typedef struct pe_server_config {
+    int64_t hbm_budget_bytes;    /* device bytes for world state (bodies, pairs, rows, staging) */
+    int32_t max_worlds;          /* concurrent live worlds */
+    int32_t max_world_bodies;    /* largest admissible request (<= PE_BATCH_MAX_WORLD_BODIES) */
+    pe_world_params params;      /* dt, gravity, iterations shared by all requests */
+} pe_server_config;
+
+int pe_server_create(const pe_server_config* cfg, pe_server** out);
+int pe_server_destroy(pe_server* s);
+/* Device bytes a request of this scene holds while live. */
+int pe_server_footprint(const pe_scene_desc* scene, int64_t* bytes);
+/* Queues a request; returns PE_ERR_BUSY if it can never fit (larger than the budget). */
+int pe_server_submit(pe_server* s, const pe_scene_desc* scene, int32_t horizon, int64_t* request_id);
+/* One continuous-batching step; reports live worlds and bodies after admission. */
+int pe_server_tick(pe_server* s, int32_t* live_worlds, int64_t* live_bodies);
+/* Ids of requests that finished since the last poll. */
+int pe_server_poll(pe_server* s, int64_t* ids, int32_t cap, int32_t* count);
+/* Final state of a finished request (n = its body count); frees the stored result. */
+int pe_server_result(pe_server* s, int64_t id, float* position, float* rotation, float* linear_velocity,
+                     float* angular_velocity);
+/* Drops a queued or live request; its slot is reclaimed at the next tick. */
+int pe_server_cancel(pe_server* s, int64_t id);
+/* ticks, p50 tick ms, p99 tick ms, mean tick ms, body-steps done, live worlds, queued, admitted */
+int pe_server_stats(pe_server* s, double* out /*8*/);
+
 #ifdef __cplusplus
 }
 #endif

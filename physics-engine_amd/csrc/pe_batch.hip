@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <numeric>
@@ -635,6 +636,454 @@ int pe_batch_get_counts(pe_batch* B, int64_t* counts) {
     counts[4] = maxcol;
     counts[5] = ovf;
     counts[6] = B->nworlds;
+    return PE_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== timed batch steps
+extern "C" int pe_batch_step_timed(pe_batch* B, int32_t nsteps, double* ms) {
+    if (!B || !ms) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    hipEvent_t e0, e1;
+    PE_HIP(hipEventCreate(&e0));
+    PE_HIP(hipEventCreate(&e1));
+    PE_HIP(hipEventRecord(e0, B->stream));
+    int rc = pe_batch_step(B, nsteps);
+    PE_HIP(hipEventRecord(e1, B->stream));
+    PE_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PE_HIP(hipEventElapsedTime(&t, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t;
+    return rc;
+}
+
+// ====================================================================== server
+namespace {
+
+constexpr int NSTAGE = NF + NI;          // staged columns per body
+constexpr int NRESULT = 18;              // pos 3, rot 9, v 3, w 3
+
+// Writes a staged [NSTAGE][n] block into the arena columns of bodies [off, off+n), then the
+// derived quantities (AABB, scale, world inverse inertia).
+__global__ void k_admit(BodyView b, const float* __restrict__ stage, int off, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int f = 0; f < NF; f++) b.at(f, off + i) = stage[(size_t)f * n + i];
+        for (int f = 0; f < NI; f++) b.iat(f, off + i) = __float_as_int(stage[(size_t)(NF + f) * n + i]);
+        refresh_body(b, off + i);
+    }
+}
+// Packs pos/rot/v/w of bodies [off, off+n) into out[NRESULT][n].
+__global__ void k_gather(BodyView b, float* __restrict__ out, int off, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; k++) out[(size_t)k * n + i] = b.at(F_POS + k, off + i);
+        for (int k = 0; k < 9; k++) out[(size_t)(3 + k) * n + i] = b.at(F_ROT + k, off + i);
+        for (int k = 0; k < 3; k++) out[(size_t)(12 + k) * n + i] = b.at(F_V + k, off + i);
+        for (int k = 0; k < 3; k++) out[(size_t)(15 + k) * n + i] = b.at(F_W + k, off + i);
+    }
+}
+__global__ void k_reset_state(WorldState* st, int w) {
+    WorldState z{};
+    st[w] = z;
+}
+
+// First-fit allocator over [0, cap) with coalescing frees.
+struct RangeAlloc {
+    std::map<int64_t, int64_t> free_;    // offset -> length
+    void init(int64_t cap) { free_.clear(); if (cap > 0) free_[0] = cap; }
+    int64_t alloc(int64_t len) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= len) {
+                const int64_t off = it->first, rest = it->second - len;
+                free_.erase(it);
+                if (rest > 0) free_[off + len] = rest;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t len) {
+        auto it = free_.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+};
+
+struct Req {
+    int64_t id;
+    int n;
+    int horizon;
+    int done = 0;
+    int slot = -1;           // world slot while live
+    int64_t boff = -1, poff = -1;
+    int pcap = 0;
+    int64_t stage_off = 0;       // offset of its upload in the staging buffer
+    std::vector<float> staged;   // [NSTAGE][n]
+    bool cancelled = false;
+};
+
+int64_t req_footprint(int n) {
+    const int64_t pc = body_pair_cap(n);
+    return (int64_t)n * NSTAGE * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 7 * 4) + 4 * (int64_t)(pc + 1) + 64;
+}
+
+}  // namespace
+
+struct pe_server {
+    pe_server_config cfg{};
+    hipStream_t cs = nullptr;   // compute
+    hipStream_t xs = nullptr;   // copies
+    std::vector<void*> allocs;
+    BatchDev d{};
+    int64_t max_bodies = 0, max_pairs = 0;
+    RangeAlloc bodies, pairs;
+    std::vector<int> free_slots;
+    std::vector<WorldDesc> hdesc;
+    std::deque<Req*> queue;
+    std::vector<Req*> live;           // by admission order
+    std::vector<Req*> prepared;       // uploads in flight, admitted at the next tick
+    std::map<int64_t, Req*> all;
+    std::map<int64_t, std::vector<float>> results;   // id -> [NRESULT][n]
+    std::vector<int64_t> finished_ids;
+    int64_t next_id = 1;
+    float* dstage[2] = {nullptr, nullptr};
+    int64_t stage_cap = 0;
+    int stage_buf = 0;
+    hipEvent_t ev_upload[2] = {nullptr, nullptr};
+    float* dresult = nullptr;
+    int64_t result_cap = 0;
+    int32_t* dlist = nullptr;
+    std::vector<Class> classes;
+    bool dirty = true;
+    int64_t live_footprint = 0;
+    std::vector<double> tick_ms;
+    double body_steps = 0;
+    int64_t admitted = 0;
+    hipEvent_t ev_done = nullptr;
+
+    ~pe_server() {
+        if (cs) hipStreamSynchronize(cs);
+        if (xs) hipStreamSynchronize(xs);
+        for (void* p : allocs) hipFree(p);
+        for (auto& e : ev_upload) if (e) hipEventDestroy(e);
+        if (ev_done) hipEventDestroy(ev_done);
+        if (cs) hipStreamDestroy(cs);
+        if (xs) hipStreamDestroy(xs);
+        for (auto& kv : all) delete kv.second;
+    }
+    template <class T> int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        PE_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+        PE_HIP(hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), cs));
+        allocs.push_back(q);
+        *p = (T*)q;
+        return PE_OK;
+    }
+
+    int rebuild_lists() {
+        std::vector<std::vector<int>> members(classes.size());
+        for (Req* r : live) {
+            for (size_t c = 0; c < classes.size(); c++)
+                if (r->n >= classes[c].lo && r->n <= classes[c].hi) { members[c].push_back(r->slot); break; }
+        }
+        std::vector<int32_t> list;
+        for (size_t c = 0; c < classes.size(); c++) {
+            auto& m = members[c];
+            std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
+            classes[c].count = (int)m.size();
+            classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
+            classes[c].list_off = (int)list.size();
+            list.insert(list.end(), m.begin(), m.end());
+        }
+        if (!list.empty()) PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * 4, hipMemcpyHostToDevice, cs));
+        PE_HIP(hipStreamSynchronize(cs));  // the host list buffer is reused
+        dirty = false;
+        return PE_OK;
+    }
+
+    // Picks queued requests that fit and starts their uploads on the copy stream.
+    int prepare_admissions() {
+        const int buf = stage_buf;
+        int64_t used = 0;
+        while (!queue.empty()) {
+            Req* r = queue.front();
+            if (r->cancelled) { queue.pop_front(); continue; }
+            const int64_t fp = req_footprint(r->n);
+            const int64_t need = (int64_t)r->n * NSTAGE;
+            if (free_slots.empty() || live_footprint + fp > cfg.hbm_budget_bytes || used + need > stage_cap) break;
+            const int64_t bo = bodies.alloc(r->n);
+            if (bo < 0) break;
+            const int pc = body_pair_cap(r->n);
+            const int64_t po = pairs.alloc(pc + 1);
+            if (po < 0) { bodies.release(bo, r->n); break; }
+            queue.pop_front();
+            r->boff = bo;
+            r->poff = po;
+            r->pcap = pc;
+            r->slot = free_slots.back();
+            free_slots.pop_back();
+            live_footprint += fp;
+            PE_HIP(hipMemcpyAsync(dstage[buf] + used, r->staged.data(), need * 4, hipMemcpyHostToDevice, xs));
+            r->staged.clear();
+            r->staged.shrink_to_fit();
+            r->stage_off = used;
+            prepared.push_back(r);
+            used += need;
+        }
+        PE_HIP(hipEventRecord(ev_upload[buf], xs));
+        return PE_OK;
+    }
+
+    int admit_prepared() {
+        if (prepared.empty()) return PE_OK;
+        const int buf = stage_buf;
+        PE_HIP(hipStreamWaitEvent(cs, ev_upload[buf], 0));
+        for (Req* r : prepared) {
+            const int64_t soff = r->stage_off;
+            WorldDesc wd{(int32_t)r->boff, r->n, (int32_t)r->poff, r->pcap};
+            hdesc[r->slot] = wd;
+            PE_HIP(hipMemcpyAsync((void*)(d.desc + r->slot), &hdesc[r->slot], sizeof(WorldDesc), hipMemcpyHostToDevice, cs));
+            hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, cs, d.st, r->slot);
+            hipLaunchKernelGGL(k_admit, dim3((r->n + 255) / 256), dim3(256), 0, cs, d.b, dstage[buf] + soff, (int)r->boff, r->n);
+            live.push_back(r);
+            admitted++;
+        }
+        PE_HIP(hipStreamSynchronize(cs));  // descriptor staging on the host is reused
+        prepared.clear();
+        dirty = true;
+        return PE_OK;
+    }
+
+    void release(Req* r) {
+        bodies.release(r->boff, r->n);
+        pairs.release(r->poff, r->pcap + 1);
+        free_slots.push_back(r->slot);
+        live_footprint -= req_footprint(r->n);
+        r->slot = -1;
+    }
+
+    int tick(int32_t* lw, int64_t* lb) {
+        const auto t0 = std::chrono::steady_clock::now();
+        // drop cancelled live worlds
+        for (size_t i = 0; i < live.size();) {
+            if (live[i]->cancelled) { release(live[i]); live.erase(live.begin() + i); dirty = true; }
+            else i++;
+        }
+        if (int rc = admit_prepared()) return rc;
+        if (dirty) if (int rc = rebuild_lists()) return rc;
+        int64_t nb = 0;
+        for (Req* r : live) nb += r->n;
+        for (auto& c : classes) {
+            if (c.count == 0) continue;
+            hipError_t e = launch_class(c.block, d, dlist + c.list_off, c.count, c.nmax, cs);
+            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+        }
+        PE_HIP(hipEventRecord(ev_done, cs));
+        // overlap: next admissions upload while this step runs
+        stage_buf ^= 1;
+        if (int rc = prepare_admissions()) return rc;
+        PE_HIP(hipEventSynchronize(ev_done));
+        body_steps += (double)nb;
+        // retire finished worlds: one gather per world into the result buffer, one copy back
+        std::vector<Req*> fin;
+        for (Req* r : live) if (++r->done >= r->horizon) fin.push_back(r);
+        if (!fin.empty()) {
+            int64_t off = 0;
+            std::vector<int64_t> offs;
+            for (Req* r : fin) {
+                const int64_t need = (int64_t)r->n * NRESULT;
+                if (off + need > result_cap) break;
+                hipLaunchKernelGGL(k_gather, dim3((r->n + 255) / 256), dim3(256), 0, cs, d.b, dresult + off, (int)r->boff, r->n);
+                offs.push_back(off);
+                off += need;
+            }
+            std::vector<float> host(off);
+            PE_HIP(hipMemcpyAsync(host.data(), dresult, off * 4, hipMemcpyDeviceToHost, cs));
+            PE_HIP(hipStreamSynchronize(cs));
+            for (size_t k = 0; k < offs.size(); k++) {
+                Req* r = fin[k];
+                results[r->id] = std::vector<float>(host.begin() + offs[k], host.begin() + offs[k] + (int64_t)r->n * NRESULT);
+                finished_ids.push_back(r->id);
+                release(r);
+                live.erase(std::find(live.begin(), live.end(), r));
+                dirty = true;
+            }
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        if (nb > 0) tick_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+        if (lw) *lw = (int32_t)live.size();
+        if (lb) {
+            int64_t s = 0;
+            for (Req* r : live) s += r->n;
+            *lb = s;
+        }
+        return PE_OK;
+    }
+};
+
+extern "C" {
+
+int pe_server_footprint(const pe_scene_desc* scene, int64_t* bytes) {
+    if (!scene || !bytes) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    *bytes = req_footprint(scene->n);
+    return PE_OK;
+}
+
+int pe_server_create(const pe_server_config* cfg, pe_server** out) {
+    if (!cfg || !out || cfg->max_worlds <= 0 || cfg->max_world_bodies <= 0 ||
+        cfg->max_world_bodies > PE_BATCH_MAX_WORLD_BODIES || cfg->hbm_budget_bytes <= 0) {
+        pe_set_error("pe_server_create: invalid config");
+        return PE_ERR_INVALID;
+    }
+    set_lds_limits();
+    auto* S = new pe_server();
+    auto fail = [&](int rc) { delete S; return rc; };
+    S->cfg = *cfg;
+    if (hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->xs, hipStreamNonBlocking) != hipSuccess) { pe_set_error("streams"); return fail(PE_ERR_DEVICE); }
+    // arena sizes from the budget: per body its store and 8 pair slots of contacts and rows
+    const int64_t per_body = req_footprint(1024) / 1024 + 1;
+    S->stage_cap = (int64_t)cfg->max_world_bodies * NSTAGE * 8;            // floats per staging buffer
+    S->result_cap = (int64_t)cfg->max_world_bodies * NRESULT * 64;        // floats
+    const int64_t fixed = 2 * S->stage_cap * 4 + S->result_cap * 4 + (int64_t)cfg->max_worlds * 64;
+    S->max_bodies = std::max<int64_t>((cfg->hbm_budget_bytes - fixed) / per_body, cfg->max_world_bodies);
+    S->max_bodies = std::min<int64_t>(S->max_bodies, (int64_t)1 << 28);
+    S->max_pairs = 8 * S->max_bodies + 65 * (int64_t)cfg->max_worlds;
+    if (S->max_pairs > ((int64_t)1 << 31) / 8 - 1) { pe_set_error("budget too large for 32-bit pair indices"); return fail(PE_ERR_INVALID); }
+    BatchDev& d = S->d;
+    d.dt = (float)cfg->params.dt;
+    for (int k = 0; k < 3; k++) d.g[k] = (float)cfg->params.gravity[k];
+    d.iterations = cfg->params.iterations;
+    d.b.n = (int)S->max_bodies;
+    d.b.stride = (int)((S->max_bodies + 63) & ~63);
+    d.ps.cs = (size_t)S->max_pairs * 8;
+    const size_t SB = d.b.stride;
+    const int64_t P = S->max_pairs;
+    int rc = PE_OK;
+    rc |= S->alloc(&d.b.f, NF * SB);
+    rc |= S->alloc(&d.b.i, NI * SB);
+    rc |= S->alloc(&d.ps.pa, P);
+    rc |= S->alloc(&d.ps.pb, P);
+    rc |= S->alloc(&d.ps.cp, (size_t)CF * P * 8);
+    rc |= S->alloc(&d.ps.npts, P);
+    rc |= S->alloc(&d.ps.swap, P);
+    rc |= S->alloc(&d.ps.rows, (size_t)RF * P * 8);
+    rc |= S->alloc(&d.act, 2 * (size_t)P);
+    rc |= S->alloc(&d.order, P);
+    rc |= S->alloc(&d.cstart, P + cfg->max_worlds);
+    WorldDesc* ddesc = nullptr;
+    WorldState* dst = nullptr;
+    rc |= S->alloc(&ddesc, cfg->max_worlds);
+    rc |= S->alloc(&dst, cfg->max_worlds);
+    rc |= S->alloc(&S->dstage[0], S->stage_cap);
+    rc |= S->alloc(&S->dstage[1], S->stage_cap);
+    rc |= S->alloc(&S->dresult, S->result_cap);
+    rc |= S->alloc(&S->dlist, cfg->max_worlds);
+    if (rc != PE_OK) return fail(PE_ERR_DEVICE);
+    d.desc = ddesc;
+    d.st = dst;
+    S->bodies.init(S->max_bodies);
+    S->pairs.init(S->max_pairs);
+    S->hdesc.assign(cfg->max_worlds, WorldDesc{0, 0, 0, 0});
+    for (int w = cfg->max_worlds - 1; w >= 0; w--) S->free_slots.push_back(w);
+    S->classes = default_classes();
+    for (auto& e : S->ev_upload) if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(PE_ERR_DEVICE);
+    if (hipEventCreateWithFlags(&S->ev_done, hipEventDisableTiming) != hipSuccess) return fail(PE_ERR_DEVICE);
+    if (hipStreamSynchronize(S->cs) != hipSuccess) { pe_set_error("init sync"); return fail(PE_ERR_DEVICE); }
+    *out = S;
+    return PE_OK;
+}
+
+int pe_server_destroy(pe_server* S) {
+    delete S;
+    return PE_OK;
+}
+
+int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, int64_t* id) {
+    if (!S || !scene || horizon <= 0) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
+    if (int rc = check_scene(scene)) return rc;
+    if (scene->n > S->cfg.max_world_bodies || req_footprint(scene->n) > S->cfg.hbm_budget_bytes) {
+        pe_set_error("request larger than the server admits");
+        return PE_ERR_BUSY;
+    }
+    Req* r = new Req();
+    r->id = S->next_id++;
+    r->n = scene->n;
+    r->horizon = horizon;
+    r->staged.assign((size_t)NSTAGE * scene->n, 0.0f);
+    std::vector<float> hf((size_t)NF * scene->n, 0.0f);
+    std::vector<int32_t> hi((size_t)NI * scene->n, 0);
+    stage_bodies(scene, hf, hi, (size_t)scene->n, 0);
+    std::copy(hf.begin(), hf.end(), r->staged.begin());
+    std::memcpy(r->staged.data() + hf.size(), hi.data(), hi.size() * 4);
+    S->all[r->id] = r;
+    S->queue.push_back(r);
+    if (id) *id = r->id;
+    return PE_OK;
+}
+
+int pe_server_tick(pe_server* S, int32_t* live_worlds, int64_t* live_bodies) {
+    if (!S) { pe_set_error("null server"); return PE_ERR_INVALID; }
+    if (S->live.empty() && S->prepared.empty()) {
+        // nothing on the device yet: start uploads now so they admit at this tick
+        if (int rc = S->prepare_admissions()) return rc;
+    }
+    return S->tick(live_worlds, live_bodies);
+}
+
+int pe_server_poll(pe_server* S, int64_t* ids, int32_t cap, int32_t* count) {
+    if (!S || !count) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    const int m = std::min<int>(cap, (int)S->finished_ids.size());
+    for (int i = 0; i < m; i++) ids[i] = S->finished_ids[i];
+    S->finished_ids.erase(S->finished_ids.begin(), S->finished_ids.begin() + m);
+    *count = m;
+    return PE_OK;
+}
+
+int pe_server_result(pe_server* S, int64_t id, float* pos, float* rot, float* v, float* w) {
+    if (!S) { pe_set_error("null server"); return PE_ERR_INVALID; }
+    auto it = S->results.find(id);
+    if (it == S->results.end()) { pe_set_error("no result for this id"); return PE_ERR_NOT_FOUND; }
+    const auto& r = it->second;
+    const int n = (int)(r.size() / NRESULT);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) if (pos) pos[3 * i + k] = r[(size_t)k * n + i];
+        for (int k = 0; k < 9; k++) if (rot) rot[9 * i + k] = r[(size_t)(3 + k) * n + i];
+        for (int k = 0; k < 3; k++) if (v) v[3 * i + k] = r[(size_t)(12 + k) * n + i];
+        for (int k = 0; k < 3; k++) if (w) w[3 * i + k] = r[(size_t)(15 + k) * n + i];
+    }
+    S->results.erase(it);
+    auto a = S->all.find(id);
+    if (a != S->all.end()) { delete a->second; S->all.erase(a); }
+    return PE_OK;
+}
+
+int pe_server_cancel(pe_server* S, int64_t id) {
+    if (!S) { pe_set_error("null server"); return PE_ERR_INVALID; }
+    auto it = S->all.find(id);
+    if (it == S->all.end()) { pe_set_error("unknown request"); return PE_ERR_NOT_FOUND; }
+    it->second->cancelled = true;
+    return PE_OK;
+}
+
+int pe_server_stats(pe_server* S, double* out) {
+    if (!S || !out) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    std::vector<double> t = S->tick_ms;
+    std::sort(t.begin(), t.end());
+    auto pct = [&](double q) { return t.empty() ? 0.0 : t[std::min(t.size() - 1, (size_t)std::floor(q * (t.size() - 1) + 0.5))]; };
+    out[0] = (double)S->tick_ms.size();
+    out[1] = pct(0.50);
+    out[2] = pct(0.99);
+    out[3] = t.empty() ? 0.0 : std::accumulate(t.begin(), t.end(), 0.0) / t.size();
+    out[4] = S->body_steps;
+    out[5] = (double)S->live.size();
+    out[6] = (double)S->queue.size();
+    out[7] = (double)S->admitted;
     return PE_OK;
 }
 

@@ -46,6 +46,12 @@ class Batch:
     def sync(self):
         check(lib().pe_batch_sync(self.h))
 
+    def step_timed(self, n: int) -> float:
+        """Runs n steps; returns their device time in ms (hipEvents on the batch stream)."""
+        ms = ctypes.c_double()
+        check(lib().pe_batch_step_timed(self.h, ctypes.c_int32(n), ctypes.byref(ms)))
+        return ms.value
+
     def capture_graph(self):
         check(lib().pe_batch_capture_graph(self.h))
 

@@ -287,6 +287,48 @@ def mixed_world(n: int, seed: int, p_sphere: float = 0.5, p_cyl: float = 0.0, de
     return b.scene(name=f"world{n}")
 
 
+def grid_world(n: int, seed: int, p_sphere: float = 0.5, density: float = 1000.0) -> Scene:
+    """One world of n bodies (boxes with U(0.2, 1.0) m edges and uniform orientation, or spheres
+    with r U(0.1, 0.6) m, each with probability 1/2) dropped onto its own kinematic ground box.
+    Centres are jittered inside the cells of a random subset of a 1.8 m lattice, which keeps
+    bounding spheres at least 0.05 m apart without a rejection loop (vectorised, so hundreds
+    of thousands of bodies generate in seconds)."""
+    rng = np.random.default_rng(seed)
+    cell = 1.8                                   # >= 2 * max bounding radius (0.866) + 0.05
+    side = max(2, int(math.ceil(math.sqrt(n / 2.0))))
+    layers = int(math.ceil(n / (side * side))) + 1
+    cells = rng.permutation(side * side * layers)[:n]
+    cx, rest = cells % side, cells // side
+    cz, cy = rest % side, rest // side
+    is_sph = rng.random(n) < p_sphere
+    size = rng.uniform(0.2, 1.0, (n, 3))
+    rad = rng.uniform(0.1, 0.6, n)
+    rot = quat_to_matrix(random_quaternions(rng, n))
+    bound = np.where(is_sph, rad, 0.5 * np.linalg.norm(size, axis=1))
+    slack = np.maximum(cell / 2.0 - bound - 0.025, 0.0)
+    jit = rng.uniform(-1.0, 1.0, (n, 3)) * slack[:, None]
+    half = side * cell / 2.0
+    pos = np.stack([(cx + 0.5) * cell - half, 1.0 + (cy + 0.5) * cell, (cz + 0.5) * cell - half], axis=1) + jit
+    m = n + 1
+    st = np.zeros(m, np.int32)
+    sp = np.zeros((m, 4))
+    mass = np.ones(m)
+    P = np.zeros((m, 3))
+    Rm = np.tile(np.eye(3), (m, 1, 1))
+    kin = np.zeros(m, np.int32)
+    g = 2.0 * half + 4.0
+    sp[0, :3] = (g, 1.0, g)
+    P[0] = (0.0, -0.5, 0.0)
+    kin[0] = 1
+    st[1:] = np.where(is_sph, SHAPE_SPHERE, SHAPE_BOX)
+    sp[1:, :3] = np.where(is_sph[:, None], np.stack([rad, np.zeros(n), np.zeros(n)], 1), size)
+    mass[1:] = np.where(is_sph, density * 4.0 / 3.0 * math.pi * rad ** 3, density * size.prod(axis=1))
+    P[1:] = pos
+    Rm[1:] = np.where(is_sph[:, None, None], np.eye(3)[None], rot)
+    return Scene(st, sp, mass, P, Rm, np.zeros((m, 3)), np.zeros((m, 3)), np.full(m, 0.5), np.zeros(m), kin,
+                 name=f"grid_world{n}").contiguous()
+
+
 def world_seed(seed: int, w: int) -> int:
     """Per-world seed: SplitMix64 of (seed, w)."""
     z = (seed * 0x9E3779B97F4A7C15 + (w + 1) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
@@ -303,4 +345,4 @@ def batch_worlds(n_worlds: int = 4096, lo: int = 8, hi: int = 512, seed: int = 1
     """Independent worlds with body counts log-uniform in [lo, hi]; boxes and spheres (p = 1/2)."""
     rng = np.random.default_rng(seed)
     counts = log_uniform_counts(rng, n_worlds, lo, hi)
-    return [mixed_world(int(c), world_seed(seed, w)) for w, c in enumerate(counts)]
+    return [grid_world(int(c), world_seed(seed, w)) for w, c in enumerate(counts)]
