@@ -43,6 +43,11 @@ int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes);
  * the batch's stream around them), in milliseconds. */
 int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
 
+/* Runs one step recording per-world phase timestamps; us[k] is the sum over worlds of phase k's
+ * duration in microseconds: integrate, sort, sweep, narrowphase, solver setup, colouring,
+ * Gauss-Seidel sweeps.  Drops a captured graph. */
+int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
+
 /* ------------------------------------------------------------------ server
  * Continuous batching of world-simulation requests.  A request (a scene and a horizon in
  * steps) is queued by pe_server_submit.  Each pe_server_tick admits queued requests in

@@ -235,9 +235,16 @@ template <class R> inline void body_set_mass_props(Body<R>& b, R mass) {
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) b.local_inertia(i, j) = I(i, j) * mass;
     b.local_inv_inertia = mass > eps ? inverse3(b.local_inertia) : M3<R>();
 }
-// RigidBody::updateWorldInertia (rigidbody.cpp:43-46): (R * L) * R^T
+// RigidBody::updateWorldInertia (rigidbody.cpp:43-46): (R * L) * R^T.  In this expression the
+// reference (FP64 as well as FP32) sums every entry of the second product as p0 + (p1 + p2);
+// pinned against the compiled reference's own I_w values.
 template <class R> inline void body_update_world_inertia(Body<R>& b) {
-    b.world_inv_inertia = mmt(mm(b.xf.basis, b.local_inv_inertia), b.xf.basis);
+    const M3<R> T = mm(b.xf.basis, b.local_inv_inertia);
+    const M3<R>& Rb = b.xf.basis;
+    M3<R> r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r(i, j) = T(i, 0) * Rb(j, 0) + (T(i, 1) * Rb(j, 1) + T(i, 2) * Rb(j, 2));
+    b.world_inv_inertia = r;
 }
 
 // Shape::getAABB
@@ -849,40 +856,68 @@ template <class R> inline void stage_narrowphase(World<R>& w) {
     }
 }
 
-// Constraint-graph colouring that fixes the solve order.  Round c gives colour c to every
-// uncoloured pair (with at least one point) that has the lowest index among the uncoloured
-// pairs touching each of its dynamic bodies.  Within a colour, pairs go in index order.
-// Pairs in one colour share no dynamic body, so a parallel sweep of a colour equals a
-// sequential sweep in this order.
+// Constraint-graph colouring that fixes the solve order (the device library's rule; the
+// reference itself sweeps in plain pair order).  Colour c is a maximal independent set built in
+// sub-rounds: every remaining candidate claims each of its untaken dynamic bodies with its pair
+// index (lowest index wins); a candidate holding all its claims joins colour c and takes its
+// bodies; a candidate touching a body taken earlier in this colour waits for the next colour.
+// Kinematic bodies never change (rigidbody.cpp:167), so they impose no order.  Pairs of one
+// colour share no dynamic body, so any order inside a colour gives the same result; this
+// restatement lists each colour in pair-index order.
 template <class R> inline void compute_color_order(World<R>& w) {
     const int np = (int)w.results.size();
     const int nb = (int)w.bodies.size();
+    const int FREE = 0x7fffffff;
     w.color_of.assign(np, -1);
     w.solve_order.clear();
-    std::vector<int> claim(nb);
+    std::vector<int> claim(nb, FREE);
+    std::vector<char> taken(nb, 0);
     std::vector<int> active;
     for (int i = 0; i < np; i++) {
         const auto& cr = w.results[i];
         if (cr.npts > 0 && !(w.bodies[cr.a].kinematic && w.bodies[cr.b].kinematic)) active.push_back(i);
     }
+    auto dyn = [&](int b) { return !w.bodies[b].kinematic; };
     int color = 0;
     while (!active.empty()) {
-        const int BIG = 0x7fffffff;
-        std::fill(claim.begin(), claim.end(), BIG);
-        for (int i : active) {
-            const auto& cr = w.results[i];
-            if (!w.bodies[cr.a].kinematic) claim[cr.a] = std::min(claim[cr.a], i);
-            if (!w.bodies[cr.b].kinematic) claim[cr.b] = std::min(claim[cr.b], i);
+        std::vector<int> cand = active, deferred, members;
+        while (!cand.empty()) {
+            for (int p : cand) {
+                const int a = w.results[p].a, b = w.results[p].b;
+                if (dyn(a) && !taken[a]) claim[a] = std::min(claim[a], p);
+                if (dyn(b) && !taken[b]) claim[b] = std::min(claim[b], p);
+            }
+            std::vector<int> next, winners;
+            for (int p : cand) {
+                const int a = w.results[p].a, b = w.results[p].b;
+                const bool blk = (dyn(a) && taken[a]) || (dyn(b) && taken[b]);
+                const bool win = !blk && (!dyn(a) || claim[a] == p) && (!dyn(b) || claim[b] == p);
+                if (blk) deferred.push_back(p);
+                else if (win) winners.push_back(p);
+                else next.push_back(p);
+            }
+            for (int p : cand) {
+                const int a = w.results[p].a, b = w.results[p].b;
+                if (dyn(a) && !taken[a]) claim[a] = FREE;
+                if (dyn(b) && !taken[b]) claim[b] = FREE;
+            }
+            for (int p : winners) {
+                const int a = w.results[p].a, b = w.results[p].b;
+                if (dyn(a)) taken[a] = 1;
+                if (dyn(b)) taken[b] = 1;
+                members.push_back(p);
+            }
+            cand.swap(next);
         }
-        std::vector<int> rest;
-        for (int i : active) {
-            const auto& cr = w.results[i];
-            const bool wa = w.bodies[cr.a].kinematic || claim[cr.a] == i;
-            const bool wb = w.bodies[cr.b].kinematic || claim[cr.b] == i;
-            if (wa && wb) { w.color_of[i] = color; w.solve_order.push_back(i); }
-            else rest.push_back(i);
+        std::sort(members.begin(), members.end());
+        for (int p : members) {
+            w.color_of[p] = color;
+            w.solve_order.push_back(p);
+            taken[w.results[p].a] = 0;
+            taken[w.results[p].b] = 0;
         }
-        active.swap(rest);
+        std::sort(deferred.begin(), deferred.end());
+        active.swap(deferred);
         color++;
     }
     w.ncolors = color;
@@ -1000,6 +1035,7 @@ template <class R> inline void world_step(World<R>& w, Solver<R>& s) {
 //   stage_broadphase .......... pinned: pair set vs compiled reference and brute force
 //   box_box / box_sphere / sphere_sphere + cache ... pinned: vs compiled reference, per pair
 //   stage_solver_setup / solve  pinned: vs compiled reference in pair order (use_color_order=0)
-//   compute_color_order ....... pinned: invariant test (no dynamic body shared within a colour)
+//   compute_color_order ....... pinned: invariants (no dynamic body shared within a colour; every
+//                               colour maximal) and brute-force check on small graphs (tests/)
 
 }  // namespace pe_oracle

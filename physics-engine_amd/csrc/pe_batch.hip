@@ -53,15 +53,25 @@ struct BatchDev {
     PairStore ps;
     const WorldDesc* desc;
     WorldState* st;
-    int32_t* act;       // [2 * total pair cap]
+    int32_t* act;       // [3 * total pair cap]
     int32_t* order;     // [total pair cap]
     int32_t* cstart;    // [total pair cap + nworlds]
     float dt;
     float g[3];
     int iterations;
+    uint64_t* phase_ts;  // optional [nworlds][PHASES] timestamps (null: off)
 };
 
 constexpr uint64_t KEY_PAD = ~0ull;
+constexpr int PHASES = 8;   // start, integrate, sort, sweep, narrow, setup, colour, solve
+
+#define PE_PHASE(k)                                                                   \
+    do {                                                                              \
+        if (d.phase_ts) {                                                             \
+            __syncthreads();                                                          \
+            if (tid == 0) d.phase_ts[(size_t)w * PHASES + (k)] = wall_clock64();      \
+        }                                                                             \
+    } while (0)
 
 // LDS layout for a class whose largest world has nmax bodies.
 __host__ __device__ inline int np2(int n) { int p = 1; while (p < n) p <<= 1; return p; }
@@ -88,9 +98,11 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
     const int axis = d.st[w].axis;
     const int prev_np = d.st[w].prev_npairs;
 
+    PE_PHASE(0);
     // ---- stage 0: integrate (world.cpp:35-127)
     for (int i = tid; i < n; i += BLOCK) integrate_body(b, off + i, d.dt, d.g);
     __syncthreads();
+    PE_PHASE(1);
 
     // ---- stage 1: sweep and prune (broad_phase_sweep_and_prune.cpp:6-41)
     for (int i = tid; i < P2; i += BLOCK)
@@ -111,6 +123,7 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
     }
     for (int s = tid; s < n; s += BLOCK) skey[s] = b.at(F_AMIN + axis, off + (int)(uint32_t)key[s]) + 0.0f;
     __syncthreads();
+    PE_PHASE(2);
     // candidate run of sorted position s: j in (s, first skey > max_i)
     int carry = 0;
     for (int base = 0; base < n; base += BLOCK) {
@@ -195,16 +208,17 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
     }
     __syncthreads();
 
+    PE_PHASE(3);
     // ---- stage 2: narrowphase (simple_narrow_phase.cpp:7-41)
     for (int p = tid; p < npairs; p += BLOCK) narrow_pair(b, d.ps, poff + p, poff + prev_np);
     __syncthreads();
+    PE_PHASE(4);
 
     // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
     for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
     for (int i = tid; i < n; i += BLOCK) claim[i] = 0x7fffffff;
     __syncthreads();
-    int* act = d.act + 2 * (size_t)poff;
-    int* act2 = act + wd.pair_cap;
+    int* act = d.act + 3 * (size_t)poff;
     int* order = d.order + poff;
     int* cs = d.cstart + poff + w;
     int nact = 0, npts = 0;
@@ -224,48 +238,11 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
         npts += totp;
     }
     __syncthreads();
-    int ncol = 0, out = 0;
-    while (nact > 0) {
-        for (int q = tid; q < nact; q += BLOCK) {
-            const int p = act[q];
-            const int ia = d.ps.pa[p], ib = d.ps.pb[p];
-            if (!b.iat(I_KIN, ia)) atomicMin(&claim[ia - off], p);
-            if (!b.iat(I_KIN, ib)) atomicMin(&claim[ib - off], p);
-        }
-        __syncthreads();
-        if (tid == 0) cs[ncol] = out;
-        int nrest = 0;
-        for (int base = 0; base < nact; base += BLOCK) {
-            const int q = base + tid;
-            int win = 0, p = -1;
-            if (q < nact) {
-                p = act[q];
-                const int ia = d.ps.pa[p], ib = d.ps.pb[p];
-                win = (b.iat(I_KIN, ia) || claim[ia - off] == p) && (b.iat(I_KIN, ib) || claim[ib - off] == p);
-            }
-            int totw, totl;
-            const int exw = block_exclusive_scan(win, wsum, &totw);
-            const int exl = block_exclusive_scan(q < nact && !win, wsum, &totl);
-            if (q < nact) {
-                if (win) order[out + exw] = p;
-                else act2[nrest + exl] = p;
-            }
-            out += totw;
-            nrest += totl;
-        }
-        __syncthreads();
-        for (int q = tid; q < nact; q += BLOCK) {
-            const int p = act[q];
-            claim[d.ps.pa[p] - off] = 0x7fffffff;
-            claim[d.ps.pb[p] - off] = 0x7fffffff;
-        }
-        int* t = act; act = act2; act2 = t;
-        nact = nrest;
-        ncol++;
-        __syncthreads();
-    }
+    PE_PHASE(5);
+    int* bufs[3] = {act, act + wd.pair_cap, act + 2 * (size_t)wd.pair_cap};
+    const int ncol = colour_pairs<true>(b, d.ps, bufs, nact, order, cs, claim, off, wsum);
+    PE_PHASE(6);
     if (tid == 0) {
-        cs[ncol] = out;
         d.st[w].ncolors = ncol;
         d.st[w].npairs = npairs;
         d.st[w].prev_npairs = npairs;
@@ -279,6 +256,7 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
             __syncthreads();
         }
     }
+    PE_PHASE(7);
     for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
 }
 
@@ -461,7 +439,7 @@ extern "C" {
 int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes) {
     if (!scene || !bytes) { pe_set_error("null argument"); return PE_ERR_INVALID; }
     const int64_t n = scene->n, pc = body_pair_cap(scene->n);
-    *bytes = n * (int64_t)(NF + NI) * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 6 * 4 + 4 * 4) + (int64_t)sizeof(WorldDesc) +
+    *bytes = n * (int64_t)(NF + NI) * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 7 * 4 + 4 * 4) + (int64_t)sizeof(WorldDesc) +
              (int64_t)sizeof(WorldState);
     return PE_OK;
 }
@@ -509,7 +487,7 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     rc |= B->alloc(&d.ps.npts, poff);
     rc |= B->alloc(&d.ps.swap, poff);
     rc |= B->alloc(&d.ps.rows, (size_t)RF * poff * 8);
-    rc |= B->alloc(&d.act, 2 * (size_t)poff);
+    rc |= B->alloc(&d.act, 3 * (size_t)poff);
     rc |= B->alloc(&d.order, poff);
     rc |= B->alloc(&d.cstart, poff + nworlds);
     WorldDesc* ddesc = nullptr;
@@ -641,6 +619,29 @@ int pe_batch_get_counts(pe_batch* B, int64_t* counts) {
 
 }  // extern "C"
 
+// ====================================================================== phase profile
+// Runs one step with per-world phase timestamps and returns, per phase, the sum over worlds of
+// the phase's duration in microseconds (wall_clock64 runs at 100 MHz on gfx9).
+extern "C" int pe_batch_phase_profile(pe_batch* B, double* us /*8*/) {
+    if (!B || !us) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    uint64_t* ts = nullptr;
+    PE_HIP(hipMalloc(&ts, (size_t)B->nworlds * PHASES * 8));
+    B->d.phase_ts = ts;
+    B->drop_graph();
+    int rc = B->launch_step();
+    B->d.phase_ts = nullptr;
+    std::vector<uint64_t> h((size_t)B->nworlds * PHASES);
+    if (rc == PE_OK) {
+        PE_HIP(hipMemcpyAsync(h.data(), ts, h.size() * 8, hipMemcpyDeviceToHost, B->stream));
+        PE_HIP(hipStreamSynchronize(B->stream));
+    }
+    hipFree(ts);
+    for (int k = 0; k < PHASES; k++) us[k] = 0;
+    for (int w = 0; w < B->nworlds; w++)
+        for (int k = 1; k < PHASES; k++) us[k - 1] += (double)(h[(size_t)w * PHASES + k] - h[(size_t)w * PHASES + k - 1]) / 100.0;
+    return rc;
+}
+
 // ====================================================================== timed batch steps
 extern "C" int pe_batch_step_timed(pe_batch* B, int32_t nsteps, double* ms) {
     if (!B || !ms) { pe_set_error("null argument"); return PE_ERR_INVALID; }
@@ -729,7 +730,7 @@ struct Req {
 
 int64_t req_footprint(int n) {
     const int64_t pc = body_pair_cap(n);
-    return (int64_t)n * NSTAGE * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 7 * 4) + 4 * (int64_t)(pc + 1) + 64;
+    return (int64_t)n * NSTAGE * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 8 * 4) + 4 * (int64_t)(pc + 1) + 64;
 }
 
 }  // namespace
@@ -973,7 +974,7 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&d.ps.npts, P);
     rc |= S->alloc(&d.ps.swap, P);
     rc |= S->alloc(&d.ps.rows, (size_t)RF * P * 8);
-    rc |= S->alloc(&d.act, 2 * (size_t)P);
+    rc |= S->alloc(&d.act, 3 * (size_t)P);
     rc |= S->alloc(&d.order, P);
     rc |= S->alloc(&d.cstart, P + cfg->max_worlds);
     WorldDesc* ddesc = nullptr;

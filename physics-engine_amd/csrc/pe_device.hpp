@@ -243,47 +243,31 @@ PE_HD void add_point(Cache& c, const PairCtx& x, f3 world_normal, f3 world_pos, 
 }
 
 // ContactResult::sortContactPoints (contact_result.cpp:144-156): libstdc++ std::sort of 8
-// elements is an insertion sort, i.e. stable by distance, invalid slots included.
-// Returns point_size (index of the first invalid slot after sorting).
-PE_HD int sort_points(Cache& c) {
+// elements is an insertion sort, i.e. stable by distance, invalid slots included.  Returns
+// point_size (the first invalid position after sorting) and, per slot k, its sorted position.
+PE_HD int sort_rank(const Cache& c, int rank[8]) {
     int perm[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) perm[i] = i;
     float key[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) key[i] = c.d[i];
-    // stable insertion sort of (key, perm) -- same comparisons as std::__insertion_sort
+    for (int i = 0; i < 8; i++) { perm[i] = i; key[i] = c.d[i]; }
 #pragma unroll
     for (int i = 1; i < 8; i++) {
 #pragma unroll
         for (int k = i; k > 0; k--) {
             if (key[k] < key[k - 1]) {
-                float tk = key[k]; key[k] = key[k - 1]; key[k - 1] = tk;
-                int tp = perm[k]; perm[k] = perm[k - 1]; perm[k - 1] = tp;
+                const float tk = key[k]; key[k] = key[k - 1]; key[k - 1] = tk;
+                const int tp = perm[k]; perm[k] = perm[k - 1]; perm[k - 1] = tp;
             }
         }
     }
-    Cache o;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        const int s = perm[i];
-        float v[13];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (k == s) {
-                v[0] = c.wx[k]; v[1] = c.wy[k]; v[2] = c.wz[k]; v[3] = c.nx[k]; v[4] = c.ny[k]; v[5] = c.nz[k];
-                v[6] = c.ax[k]; v[7] = c.ay[k]; v[8] = c.az[k]; v[9] = c.bx[k]; v[10] = c.by[k]; v[11] = c.bz[k];
-                v[12] = c.d[k];
-            }
-        }
-        o.wx[i] = v[0]; o.wy[i] = v[1]; o.wz[i] = v[2]; o.nx[i] = v[3]; o.ny[i] = v[4]; o.nz[i] = v[5];
-        o.ax[i] = v[6]; o.ay[i] = v[7]; o.az[i] = v[8]; o.bx[i] = v[9]; o.by[i] = v[10]; o.bz[i] = v[11];
-        o.d[i] = v[12];
+        for (int k = 0; k < 8; k++) if (perm[i] == k) rank[k] = i;
     }
-    c = o;
     int np = 8;
 #pragma unroll
-    for (int i = 7; i >= 0; i--) if (!(c.wx[i] < REAL_MAX + 0.1f)) np = i;
+    for (int k = 0; k < 8; k++) if (!(c.wx[k] < REAL_MAX + 0.1f) && rank[k] < np) np = rank[k];
     return np;
 }
 

@@ -4,6 +4,7 @@
 #pragma once
 #include "pe_common.hpp"
 #include "pe_device.hpp"
+#include "pe_prims.hpp"
 
 namespace pe {
 
@@ -134,12 +135,13 @@ static __device__ __noinline__ void narrow_pair(const BodyView& b, const PairSto
     else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, spa, spb[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, spb, spa[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, spa[0], spb[0]);
-    const int np = sort_points(c);
+    int rank[8];
+    const int np = sort_rank(c, rank);
     ps.npts[p] = np;
     ps.swap[p] = x.swap ? 1 : 0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const size_t s = slot0 + k;
+        const size_t s = slot0 + rank[k];
         float* cp = ps.cp;
         cp[(C_WPOS + 0) * CS + s] = c.wx[k]; cp[(C_WPOS + 1) * CS + s] = c.wy[k]; cp[(C_WPOS + 2) * CS + s] = c.wz[k];
         cp[(C_N + 0) * CS + s] = c.nx[k]; cp[(C_N + 1) * CS + s] = c.ny[k]; cp[(C_N + 2) * CS + s] = c.nz[k];
@@ -197,29 +199,51 @@ __device__ __forceinline__ Row load_row(const float* R, size_t CS, size_t s) {
     return r;
 }
 
-// RigidBody::applyTempImpulse (rigidbody.cpp:166-170)
-__device__ __forceinline__ void apply_temp(const BodyView& b, int i, f3& tv, f3& tw, f3 r, f3 J) {
-    tv = add(tv, mul(J, b.at(F_IM, i)));
-    tw = add(tw, mv(ld9(b, F_IW, i), cross(r, J)));
+// RigidBody::applyTempImpulse (rigidbody.cpp:166-170), with the body's inverse mass and world
+// inverse inertia already in registers.
+__device__ __forceinline__ void apply_temp(f3& tv, f3& tw, float im, const m33& Iw, f3 r, f3 J) {
+    tv = add(tv, mul(J, im));
+    tw = add(tw, mv(Iw, cross(r, J)));
 }
 
 // FrictionContactConstraint::iterateSequentialImpulse (friction_contact_constraint.cpp:75-108)
-// over all rows of pair p.
+// over all rows of pair p.  All loads of a group of up to 4 rows and of both bodies are issued
+// before the dependent chain, so a pair costs a few memory round trips, not one per row.
 __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore& ps, int p) {
     const size_t CS = ps.cs;
     const int ia = ps.pa[p], ib = ps.pb[p];
+    const int n = ps.npts[p];
     const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
     f3 tva = ld3(b, F_TV, ia), twa = ld3(b, F_TW, ia), tvb = ld3(b, F_TV, ib), twb = ld3(b, F_TW, ib);
-    const int n = ps.npts[p];
+    const m33 IwA = ld9(b, F_IW, ia), IwB = ld9(b, F_IW, ib);
+    const float imA = b.at(F_IM, ia), imB = b.at(F_IM, ib);
     float* R = ps.rows;
-    for (int k = 0; k < n; k++) {
-        const size_t s = (size_t)p * 8 + k;
-        const Row r = load_row(R, CS, s);
-        float ln = R[R_LN * CS + s], lt0 = R[R_LT0 * CS + s], lt1 = R[R_LT1 * CS + s];
-        const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
-        R[R_LN * CS + s] = ln; R[R_LT0 * CS + s] = lt0; R[R_LT1 * CS + s] = lt1;
-        if (!ka) apply_temp(b, ia, tva, twa, r.ra, J);
-        if (!kb) apply_temp(b, ib, tvb, twb, r.rb, neg(J));
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        Row rw[4];
+        float ln[4], lt0[4], lt1[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k0 + u < n) {
+                const size_t s = (size_t)p * 8 + k0 + u;
+                rw[u] = load_row(R, CS, s);
+                ln[u] = R[R_LN * CS + s]; lt0[u] = R[R_LT0 * CS + s]; lt1[u] = R[R_LT1 * CS + s];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k0 + u < n) {
+                const f3 J = iterate_row(rw[u], ln[u], lt0[u], lt1[u], tva, twa, tvb, twb);
+                if (!ka) apply_temp(tva, twa, imA, IwA, rw[u].ra, J);
+                if (!kb) apply_temp(tvb, twb, imB, IwB, rw[u].rb, neg(J));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k0 + u < n) {
+                const size_t s = (size_t)p * 8 + k0 + u;
+                R[R_LN * CS + s] = ln[u]; R[R_LT0 * CS + s] = lt0[u]; R[R_LT1 * CS + s] = lt1[u];
+            }
+        }
     }
     if (!ka) { st3(b, F_TV, ia, tva); st3(b, F_TW, ia, twa); }
     if (!kb) { st3(b, F_TV, ib, tvb); st3(b, F_TW, ib, twb); }
@@ -230,6 +254,128 @@ __device__ __forceinline__ void sync_body(const BodyView& b, int i) {
     if (b.iat(I_KIN, i)) return;
     st3(b, F_V, i, ld3(b, F_TV, i));
     st3(b, F_W, i, ld3(b, F_TW, i));
+}
+
+// ------------------------------------------------------------------ constraint-graph colouring
+// Colours pairs so that no two pairs of one colour share a dynamic body (kinematic bodies never
+// change, rigidbody.cpp:167, so they impose no order).  Colour c is a maximal independent set
+// built in sub-rounds: every candidate claims its dynamic bodies with atomicMin of its pair
+// index; a candidate that holds all its claims joins the colour and takes its bodies; a
+// candidate touching a taken body waits for a later colour.  The colour classes depend only on
+// the pair set, not on list order, so sweeping a colour in parallel equals the sequential
+// Gauss-Seidel sweep over (colour, any order within the colour).
+// claim[] holds, per body, INT_MAX (free), -1 (taken by this colour) or the lowest claim.
+namespace colour {
+constexpr int FREE = 0x7fffffff;
+constexpr int TAKEN = -1;
+}
+
+template <bool LDS>
+__device__ __forceinline__ int claim_get(int* claim, int i) {
+    if (LDS) return claim[i];
+    return __hip_atomic_load(&claim[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Appends the lanes of the calling wave with pred set to a list whose length is *counter:
+// one atomic per wave.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_append(bool pred, int* counter) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if ((int)(threadIdx.x & (WAVE - 1)) == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, WAVE);
+    return pred ? base + (int)__popcll(m & lanemask_lt()) : -1;
+}
+
+// bufs: three lists of pair capacity; bufs[0] holds the nact active pairs on entry.
+// claim is indexed by (body - off).  Writes the solve order and colour starts; returns colours.
+// The lists are compacted with atomics, so the order inside a colour varies from run to run;
+// the colour classes do not, and neither does any result (pairs of a colour are independent).
+template <bool LDS>
+__device__ int colour_pairs(const BodyView& b, const PairStore& ps, int* bufs[3], int nact, int* order, int* cs,
+                            int* claim, int off, int* wsum) {
+    using namespace colour;
+    __shared__ int cnt[3];   // winners (order), deferred, staying candidates
+    const int tid = threadIdx.x, B = blockDim.x;
+    (void)wsum;
+    int ia = 0, ncol = 0;
+    if (tid == 0) cnt[0] = 0;
+    __syncthreads();
+    while (nact > 0) {
+        int* cand = bufs[ia];
+        int* nxt = bufs[(ia + 1) % 3];
+        int* def = bufs[(ia + 2) % 3];
+        int ncand = nact;
+        const int col_begin = cnt[0];
+        if (tid == 0) { cs[ncol] = col_begin; cnt[1] = 0; }
+        __syncthreads();
+        while (ncand > 0) {
+            for (int q = tid; q < ncand; q += B) {
+                const int p = cand[q];
+                const int a = ps.pa[p], c = ps.pb[p];
+                if (!b.iat(I_KIN, a)) atomicMin(&claim[a - off], p);
+                if (!b.iat(I_KIN, c)) atomicMin(&claim[c - off], p);
+            }
+            if (tid == 0) cnt[2] = 0;
+            __syncthreads();
+            const int sub_begin = cnt[0];
+            for (int base = 0; base < ncand; base += B) {
+                const int q = base + tid;
+                bool win = false, blk = false, stay = false;
+                int p = -1;
+                if (q < ncand) {
+                    p = cand[q];
+                    const int a = ps.pa[p], c = ps.pb[p];
+                    const bool ka = b.iat(I_KIN, a) != 0, kc = b.iat(I_KIN, c) != 0;
+                    const int ca = ka ? 0 : claim_get<LDS>(claim, a - off);
+                    const int cc = kc ? 0 : claim_get<LDS>(claim, c - off);
+                    blk = (!ka && ca == TAKEN) || (!kc && cc == TAKEN);
+                    win = !blk && (ka || ca == p) && (kc || cc == p);
+                    stay = !blk && !win;
+                }
+                const int iw = wave_append(win, &cnt[0]);
+                const int ib = wave_append(blk, &cnt[1]);
+                const int is = wave_append(stay, &cnt[2]);
+                if (win) order[iw] = p;
+                if (blk) def[ib] = p;
+                if (stay) nxt[is] = p;
+            }
+            __syncthreads();
+            const int sub_end = cnt[0], nstay = cnt[2];
+            // clear this sub-round's claims (taken bodies stay taken)
+            for (int q = tid; q < ncand; q += B) {
+                const int p = cand[q];
+                const int a = ps.pa[p], c = ps.pb[p];
+                if (!b.iat(I_KIN, a) && claim_get<LDS>(claim, a - off) != TAKEN) claim[a - off] = FREE;
+                if (!b.iat(I_KIN, c) && claim_get<LDS>(claim, c - off) != TAKEN) claim[c - off] = FREE;
+            }
+            __syncthreads();
+            for (int q = sub_begin + tid; q < sub_end; q += B) {
+                const int p = order[q];
+                const int a = ps.pa[p], c = ps.pb[p];
+                if (!b.iat(I_KIN, a)) claim[a - off] = TAKEN;
+                if (!b.iat(I_KIN, c)) claim[c - off] = TAKEN;
+            }
+            __syncthreads();
+            int* t = cand; cand = nxt; nxt = t;
+            ncand = nstay;
+        }
+        const int col_end = cnt[0];
+        for (int q = col_begin + tid; q < col_end; q += B) {
+            const int p = order[q];
+            claim[ps.pa[p] - off] = FREE;
+            claim[ps.pb[p] - off] = FREE;
+        }
+        const int ndef = cnt[1];
+        __syncthreads();
+        ia = (ia + 2) % 3;
+        nact = ndef;
+        ncol++;
+    }
+    if (tid == 0) cs[ncol] = cnt[0];
+    __syncthreads();
+    return ncol;
 }
 
 }  // namespace pe

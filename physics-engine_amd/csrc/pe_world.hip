@@ -52,8 +52,8 @@ struct Dev {
     BodyView b;
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
     int32_t* order;         // [pcap]
-    int32_t* act;           // [2*pcap] colouring work lists
-    int32_t* color_start;   // [MAX_COLORS+1]
+    int32_t* act;           // [3*pcap] colouring work lists
+    int32_t* color_start;   // [pcap+1]
     int32_t* claim;         // [n] (global claims when n > CLAIM_LDS)
     uint32_t* keys;         // [n] x2 radix buffers
     uint32_t* vals;
@@ -230,13 +230,6 @@ __global__ void k_setup(Dev d) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) setup_pair(d.b, d.ps, p, d.dt);
 }
 
-// Claims in global memory are updated by L2 atomics; read them with an L1-bypassing load.
-template <bool LDS_CLAIM>
-__device__ __forceinline__ int load_claim(int* claim, int i) {
-    if (LDS_CLAIM) return claim[i];
-    return __hip_atomic_load(&claim[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Colouring + colour-ordered Gauss-Seidel + velocity sync, one workgroup.
 // Round c gives colour c to every uncoloured pair that holds, for each of its dynamic bodies,
 // the lowest pair index among the uncoloured pairs on that body.  Pairs of one colour share no
@@ -254,7 +247,6 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
     for (int i = threadIdx.x; i < b.n; i += blockDim.x) claim[i] = INF;
     // active list (index order)
     int* act = d.act;
-    int* act2 = d.act + d.pcap;
     int nact = 0;
     for (int base = 0; base < np; base += blockDim.x) {
         const int p = base + threadIdx.x;
@@ -266,52 +258,11 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
         nact += tot;
     }
     __syncthreads();
-    int ncol = 0, out = 0;
-    while (nact > 0 && ncol < MAX_COLORS) {
-        for (int q = threadIdx.x; q < nact; q += blockDim.x) {
-            const int p = act[q];
-            const int ia = d.ps.pa[p], ib = d.ps.pb[p];
-            if (!b.iat(I_KIN, ia)) atomicMin(&claim[ia], p);
-            if (!b.iat(I_KIN, ib)) atomicMin(&claim[ib], p);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) d.color_start[ncol] = out;
-        int nrest = 0;
-        for (int base = 0; base < nact; base += blockDim.x) {
-            const int q = base + threadIdx.x;
-            int win = 0, p = -1;
-            if (q < nact) {
-                p = act[q];
-                const int ia = d.ps.pa[p], ib = d.ps.pb[p];
-                win = (b.iat(I_KIN, ia) || load_claim<LDS_CLAIM>(claim, ia) == p) &&
-                      (b.iat(I_KIN, ib) || load_claim<LDS_CLAIM>(claim, ib) == p);
-            }
-            int totw, totl;
-            const int exw = block_exclusive_scan(win, wsum, &totw);
-            const int exl = block_exclusive_scan(q < nact && !win, wsum, &totl);
-            if (q < nact) {
-                if (win) d.order[out + exw] = p;
-                else act2[nrest + exl] = p;
-            }
-            out += totw;
-            nrest += totl;
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < nact; q += blockDim.x) {
-            const int p = act[q];
-            claim[d.ps.pa[p]] = INF;
-            claim[d.ps.pb[p]] = INF;
-        }
-        int* t = act; act = act2; act2 = t;
-        nact = nrest;
-        ncol++;
-        __syncthreads();
-    }
+    int* bufs[3] = {act, act + d.pcap, act + 2 * (size_t)d.pcap};
+    const int ncol = colour_pairs<LDS_CLAIM>(b, d.ps, bufs, nact, d.order, d.color_start, claim, 0, wsum);
     if (threadIdx.x == 0) {
-        d.color_start[ncol] = out;
         d.cnt->ncolors = ncol;
-        d.cnt->nordered = out;
-        d.cnt->overflow |= (nact > 0) ? 2 : 0;
+        d.cnt->nordered = d.color_start[ncol];
     }
     __syncthreads();
     // Gauss-Seidel sweeps
@@ -489,8 +440,8 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.ps.swap, pcap);
     rc |= w->alloc(&d.ps.rows, (size_t)RF * pcap * 8);
     rc |= w->alloc(&d.order, pcap);
-    rc |= w->alloc(&d.act, 2 * (size_t)pcap);
-    rc |= w->alloc(&d.color_start, MAX_COLORS + 1);
+    rc |= w->alloc(&d.act, 3 * (size_t)pcap);
+    rc |= w->alloc(&d.color_start, pcap + 1);
     rc |= w->alloc(&d.claim, n);
     rc |= w->alloc(&d.keys, n);
     rc |= w->alloc(&d.vals, n);
@@ -620,7 +571,6 @@ int pe_world_sync(pe_world* w) {
     DevCounts c;
     PE_HIP(hipMemcpy(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost));
     if (c.overflow & 1) { pe_set_error("pair capacity exceeded"); return PE_ERR_CAPACITY; }
-    if (c.overflow & 2) { pe_set_error("colour capacity exceeded"); return PE_ERR_CAPACITY; }
     return PE_OK;
 }
 

@@ -52,6 +52,13 @@ class Batch:
         check(lib().pe_batch_step_timed(self.h, ctypes.c_int32(n), ctypes.byref(ms)))
         return ms.value
 
+    def phase_profile(self):
+        """One step with in-kernel phase timestamps: summed microseconds per phase."""
+        us = (ctypes.c_double * 8)()
+        check(lib().pe_batch_phase_profile(self.h, us))
+        names = ("integrate", "sort", "sweep", "narrowphase", "setup", "colouring", "gauss_seidel")
+        return dict(zip(names, list(us)[:7]))
+
     def capture_graph(self):
         check(lib().pe_batch_capture_graph(self.h))
 

@@ -164,7 +164,9 @@ def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
 
 
 # ------------------------------------------------------------------ builders
-class _Builder:
+class SceneBuilder:
+    """Collects bodies one by one (the order of World::addRigidBody) into a Scene."""
+
     def __init__(self):
         self.rows = []
 
@@ -188,7 +190,7 @@ class _Builder:
 def tower_scene(layers: int = 20, per_layer: int = 10, seed: int = 1000) -> Scene:
     """Ring tower of layers x per_layer bricks (1.0 x 0.5 x 0.5 m, 1000 kg/m^3) on a kinematic
     50 x 1 x 50 m ground box, and a r = 0.5 m steel sphere (7800 kg/m^3) at 30 m/s."""
-    b = _Builder()
+    b = SceneBuilder()
     b.add(SHAPE_BOX, (50.0, 1.0, 50.0), 1.0, (0.0, -0.5, 0.0), kin=1)
     L, H, W = 1.0, 0.5, 0.5
     angle = 2.0 * math.pi / per_layer
@@ -250,7 +252,7 @@ def box_rain_scene(n: int = 10000, seed: int = 1001, lo=(-20, 1, -20), hi=(20, 8
     rot = quat_to_matrix(random_quaternions(rng, n))
     radii = 0.5 * np.linalg.norm(size, axis=1)
     pos = _rejection_centres(rng, radii, lo, hi, 0.05)
-    b = _Builder()
+    b = SceneBuilder()
     b.add(SHAPE_BOX, ground, 1.0, (0.0, -ground[1] / 2.0, 0.0), mu=mu, kin=1)
     for i in range(n):
         b.add(SHAPE_BOX, size[i], density * size[i].prod(), pos[i], rot[i], mu=mu)
@@ -274,7 +276,7 @@ def mixed_world(n: int, seed: int, p_sphere: float = 0.5, p_cyl: float = 0.0, de
     bound = np.where(kind == SHAPE_BOX, 0.5 * np.linalg.norm(size, axis=1),
                      np.where(kind == SHAPE_SPHERE, rad, np.sqrt(cyl_r ** 2 + 0.25 * cyl_h ** 2)))
     pos = _rejection_centres(rng, bound, (-half, 1.0, -half), (half, 1.0 + height, half), 0.05)
-    b = _Builder()
+    b = SceneBuilder()
     g = 2.0 * half + 4.0
     b.add(SHAPE_BOX, (g, 1.0, g), 1.0, (0.0, -0.5, 0.0), kin=1)
     for i in range(n):

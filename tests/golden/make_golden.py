@@ -1,0 +1,53 @@
+"""Writes the golden fixtures in this directory from the reference engine compiled from its own
+sources (oracle/_ref/libperef{32,64}.so; run `make -C oracle ref` in a checkout of the reference
+first).  Each fixture holds the scene it was made from, the reference's pair list per step (so a
+restatement can follow the reference's std::sort tie order) and the body states after chosen steps.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "physics-engine_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+
+from pe_amd import scenes  # noqa: E402
+from pyoracle import RefWorld  # noqa: E402
+
+
+def record(name, sc, bits, steps, keep, contacts_at):
+    r = RefWorld(sc, bits, threads=4)
+    out = {k: getattr(sc, k) for k in ("shape_type", "shape_param", "mass", "position", "rotation",
+                                       "linear_velocity", "angular_velocity", "friction", "restitution", "kinematic")}
+    out["params"] = np.array([sc.dt, *sc.gravity, sc.iterations], float)
+    pair_lists = []
+    for s in range(1, steps + 1):
+        r.integrate()
+        r.broadphase()
+        pair_lists.append(r.pairs().astype(np.int32))
+        r.narrowphase()
+        if s == contacts_at:
+            npts, swap, pts = r.contacts()
+            out["contacts_step"] = np.array([s])
+            out["contacts_npts"] = npts
+            out["contacts_swap"] = swap
+            out["contacts_pts"] = pts.astype(np.float32 if bits == 32 else np.float64)
+        r.solve()
+        if s in keep:
+            pos, rot, v, w = r.state()
+            out[f"state_{s}"] = np.concatenate([pos, rot.reshape(-1, 9), v, w], axis=1)
+    out["pair_counts"] = np.array([len(p) for p in pair_lists], np.int32)
+    out["pairs"] = np.concatenate(pair_lists) if pair_lists else np.zeros((0, 2), np.int32)
+    path = os.path.join(HERE, f"{name}_fp{bits}.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    from helpers import small_rain
+    record("tower", scenes.tower_scene(), 32, 30, (1, 10, 30), 10)
+    record("tower", scenes.tower_scene(), 64, 30, (1, 10, 30), 10)
+    record("rain120", small_rain(120, seed=31), 32, 90, (1, 30, 90), 60)

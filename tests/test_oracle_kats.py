@@ -1,0 +1,222 @@
+"""The oracle against the reference's own known-answer tests (test/rigid/**/test.cpp) and
+against closed forms.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from pe_amd import scenes
+from pe_amd.scenes import SHAPE_BOX, SHAPE_CYLINDER, SHAPE_SPHERE, SceneBuilder
+from pyoracle import OracleWorld
+
+T_EPS = 1e-4   # test/test_general.h
+
+
+def _world(rows, bits=32, dt=0.0, g=(0.0, 0.0, 0.0), iterations=10):
+    b = SceneBuilder()
+    for r in rows:
+        b.add(**r)
+    sc = b.scene(dt=dt, gravity=g, iterations=iterations)
+    return sc, OracleWorld(sc, bits, color_order=False)
+
+
+def test_kat_broadphase_two_pairs():
+    # test/rigid/collision/broad_phase/test.cpp:15-33
+    rows = [dict(shape=SHAPE_BOX, param=(1, 1, 1), mass=1.0, pos=(0, y, 0), kin=k)
+            for y, k in ((0, 0), (0.5, 0), (1.1, 1), (1.6, 1))]
+    for bits in (32, 64):
+        _, o = _world(rows, bits)
+        o.integrate()
+        assert o.broadphase() == 2
+
+
+def test_kat_narrowphase_four_points():
+    # test/rigid/collision/narrow_phase/test.cpp:16-25
+    rows = [dict(shape=SHAPE_BOX, param=(1, 1, 1), mass=1.0, pos=(0, 0, 0)),
+            dict(shape=SHAPE_BOX, param=(1, 1, 1), mass=1.0, pos=(0, 0.99, 0))]
+    for bits in (32, 64):
+        _, o = _world(rows, bits)
+        o.integrate()
+        o.set_pairs(np.array([[0, 1]]))
+        o.narrowphase()
+        npts, _, pts = o.contacts()
+        assert len(npts) == 1 and npts[0] == 4
+        # closed form: face contact, depth = -(1 - 0.99), normal along y
+        assert np.allclose(pts[0, :4, 12], -0.01, atol=1e-6)
+        assert np.allclose(np.abs(pts[0, :4, 4]), 1.0)
+
+
+def test_kat_friction_contact_four_points_and_solve():
+    # test/rigid/constraint/friction_contact_constraint/test.cpp:20-47 (pair order {rb2, rb1})
+    rows = [dict(shape=SHAPE_BOX, param=(40, 2, 40), mass=1.0, pos=(0, -1, 0), mu=0.5, e=1.0),
+            dict(shape=SHAPE_BOX, param=(2, 2, 2), mass=1.0, pos=(0, 0.999, 0), v=(0, -4, 0), mu=0.5, e=1.0)]
+    _, o = _world(rows, 64, dt=0.01, iterations=8)
+    o.integrate()
+    o.set_pairs(np.array([[1, 0]]))
+    o.narrowphase()
+    npts, _, _ = o.contacts()
+    assert npts[0] == 4
+    o.solve()
+    _, _, v, _ = o.state()
+    # both bodies dynamic, equal mass: momentum is conserved and the normal impulse only
+    # separates them (restitution 1 reverses the approach speed of 4 m/s up to Baumgarte)
+    assert abs(v[0, 1] + v[1, 1] - (-4.0)) < 1e-9
+    assert v[1, 1] > v[0, 1]
+
+
+def test_kat_rigidbody_step():
+    # test/rigid/object/rigidbody/test.cpp:15-24: step(0.01), v = (1,0,0), w = (0,1,0)
+    rows = [dict(shape=SHAPE_BOX, param=(1, 1, 1), mass=1.0, pos=(0, 0, 0), v=(1, 0, 0), w=(0, 1, 0))]
+    for bits in (32, 64):
+        _, o = _world(rows, bits, dt=0.01)
+        o.integrate()
+        pos, rot, _, _ = o.state()
+        assert np.allclose(pos[0], (0.01, 0, 0), atol=T_EPS)
+        c, s = math.cos(0.01), math.sin(0.01)
+        assert np.allclose(rot[0], [[c, 0, s], [0, 1, 0], [-s, 0, c]], atol=T_EPS)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_kat_box_aabb(bits):
+    # test/rigid/shape/box_shape/test.cpp:14-45
+    rows = [dict(shape=SHAPE_BOX, param=(1, 2, 3), mass=1.0, pos=(0, 0, 0), kin=1),
+            dict(shape=SHAPE_BOX, param=(1, 2, 3), mass=1.0, pos=(1, 2, 3), kin=1),
+            dict(shape=SHAPE_BOX, param=(1, 2, 3), mass=1.0, pos=(0, 0, 0), rot=scenes.rot_y(math.pi / 4), kin=1)]
+    _, o = _world(rows, bits)
+    o.integrate()
+    mn, mx = o.aabbs()
+    assert np.allclose(mn[0], (-0.5, -1, -1.5)) and np.allclose(mx[0], (0.5, 1, 1.5))
+    assert np.allclose(mn[1], (0.5, 1, 1.5)) and np.allclose(mx[1], (1.5, 3, 4.5))
+    r2 = math.sqrt(2.0)
+    assert np.allclose(mn[2], (-r2, -1, -r2), atol=T_EPS) and np.allclose(mx[2], (r2, 1, r2), atol=T_EPS)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_kat_cylinder_aabb(bits):
+    # test/rigid/shape/cylinder_shape/test.cpp:18-48
+    c, s = math.cos(math.pi / 6), math.sin(math.pi / 6)
+    rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    rows = [dict(shape=SHAPE_CYLINDER, param=(0.5, 1.0), mass=1.0, pos=(0, 0, 0), kin=1),
+            dict(shape=SHAPE_CYLINDER, param=(0.5, 1.0), mass=1.0, pos=(1, 2, 3), kin=1),
+            dict(shape=SHAPE_CYLINDER, param=(0.5, 1.0), mass=1.0, pos=(1, 2, 3), rot=rz, kin=1)]
+    _, o = _world(rows, bits)
+    o.integrate()
+    mn, mx = o.aabbs()
+    assert np.allclose(mn[0], -0.5) and np.allclose(mx[0], 0.5)
+    assert np.allclose(mn[1], (0.5, 1.5, 2.5)) and np.allclose(mx[1], (1.5, 2.5, 3.5))
+    assert np.allclose(mn[2], (0.317, 1.317, 2.5), atol=1e-3) and np.allclose(mx[2], (1.683, 2.683, 3.5), atol=1e-3)
+
+
+def test_box_aabb_closed_form():
+    """AABB of a rotated box = origin -/+ |R| h (any association of the sum)."""
+    sc = scenes.box_rain_scene(200, seed=3, lo=(-5, 1, -5), hi=(5, 11, 5))
+    sc.kinematic[:] = 1
+    o = OracleWorld(sc, 64)
+    o.integrate()
+    mn, mx = o.aabbs()
+    h = sc.shape_param[:, :3] / 2
+    ext = np.einsum("nij,nj->ni", np.abs(sc.rotation), h)
+    assert np.allclose(mn, sc.position - ext, atol=1e-12) and np.allclose(mx, sc.position + ext, atol=1e-12)
+
+
+def test_sphere_sphere_closed_form():
+    rows = [dict(shape=SHAPE_SPHERE, param=(0.5,), mass=1.0, pos=(0, 0, 0)),
+            dict(shape=SHAPE_SPHERE, param=(0.3,), mass=1.0, pos=(0.6, 0.2, 0.1))]
+    _, o = _world(rows, 64)
+    o.integrate()
+    o.set_pairs(np.array([[0, 1]]))
+    o.narrowphase()
+    npts, _, pts = o.contacts()
+    d = math.sqrt(0.36 + 0.04 + 0.01)
+    n = -np.array([0.6, 0.2, 0.1]) / d            # B -> A
+    assert npts[0] == 1
+    assert np.allclose(pts[0, 0, 3:6], n) and abs(pts[0, 0, 12] - (d - 0.8)) < 1e-12
+    assert np.allclose(pts[0, 0, :3], np.array([0.6, 0.2, 0.1]) + n * 0.3)   # point on B
+
+
+def test_box_sphere_closed_form():
+    rows = [dict(shape=SHAPE_BOX, param=(2, 2, 2), mass=1.0, pos=(0, 0, 0)),
+            dict(shape=SHAPE_SPHERE, param=(0.5,), mass=1.0, pos=(0.3, 1.4, -0.2))]
+    _, o = _world(rows, 64)
+    o.integrate()
+    o.set_pairs(np.array([[0, 1]]))
+    o.narrowphase()
+    npts, _, pts = o.contacts()
+    assert npts[0] == 1
+    assert abs(pts[0, 0, 12] - (0.4 - 0.5)) < 1e-12                  # depth = gap - r
+    assert np.allclose(pts[0, 0, 3:6], (0, -1, 0))                   # B (sphere) -> A (box)
+
+
+def test_single_contact_impulse_closed_form():
+    """Sphere falling onto a kinematic box: one row, lambda_n = max(0, rhs - n.v m_n)."""
+    r, m, v0, dt = 0.5, 2.0, -3.0, 1.0 / 60
+    rows = [dict(shape=SHAPE_BOX, param=(10, 1, 10), mass=1.0, pos=(0, -0.5, 0), kin=1),
+            dict(shape=SHAPE_SPHERE, param=(r,), mass=m, pos=(0, 0.49, 0), v=(0, v0, 0), mu=0.5)]
+    _, o = _world(rows, 64, dt=dt, iterations=1)
+    o.set_state(pos=np.array([[0, -0.5, 0], [0, 0.49 - v0 * dt, 0]], float))
+    o.integrate()
+    o.broadphase()
+    o.narrowphase()
+    npts, _, pts = o.contacts()
+    assert npts[0] == 1
+    depth = pts[0, 0, 12]
+    mn = m                                       # r x n = 0, so m_n = 1 / (1/m)
+    rhs = (0.0 - 0.2 * max(depth, -0.3) / dt) * mn
+    lam = max(0.0, rhs - (v0 * 1.0) * mn)        # n = +y (B -> A with A = box? see sign below)
+    o.solve()
+    _, _, v, _ = o.state()
+    assert abs(v[1, 1] - (v0 + lam / m)) < 1e-9
+
+
+def test_coloured_sweep_equals_sequential_sweep():
+    """Gauss-Seidel over the colour order equals the plain sequential sweep over the pair list
+    permuted into that order (pairs of one colour share no dynamic body).  Each check starts
+    two fresh worlds from one state, so no contact-slot history differs between them."""
+    sc = scenes.box_rain_scene(150, seed=5, lo=(-4, 1, -4), hi=(4, 10, 4), ground=(20.0, 1.0, 20.0))
+    run = OracleWorld(sc, 32)
+    checked = 0
+    for step in range(90):
+        run.step(1)
+        if step % 15 != 14:
+            continue
+        st = run.state()
+        a = OracleWorld(sc, 32, color_order=True)
+        b = OracleWorld(sc, 32, color_order=False)
+        for w in (a, b):
+            w.set_state(*st)
+            w.integrate()
+            w.broadphase()
+        a.narrowphase()
+        a.solve()
+        order = a.solve_order()
+        pa = a.pairs()
+        rest = np.setdiff1d(np.arange(len(pa)), order)
+        b.set_pairs(pa[np.concatenate([order, rest]).astype(int)])
+        b.narrowphase()
+        b.solve()
+        for x, y in zip(a.state(), b.state()):
+            assert np.array_equal(x, y), f"step {step}"
+        checked += 1
+    assert checked == 6
+
+
+def test_colouring_invariants():
+    sc = scenes.box_rain_scene(300, seed=9, lo=(-4, 1, -4), hi=(4, 12, 4), ground=(30.0, 1.0, 30.0))
+    o = OracleWorld(sc, 32)
+    kin = sc.kinematic.astype(bool)
+    for _ in range(90):
+        o.step(1)
+    ab = o.pairs()
+    npts, _, _ = o.contacts()
+    col = o.color_of()
+    for c in range(o.num_colors()):
+        members = np.where(col == c)[0]
+        bodies = ab[members].ravel()
+        dyn = bodies[~kin[bodies]]
+        assert len(dyn) == len(np.unique(dyn))
+        # maximal: every later pair conflicts with a member of this colour on a dynamic body
+        taken = set(dyn.tolist())
+        for p in np.where(col > c)[0]:
+            assert any((not kin[x]) and x in taken for x in ab[p])
+    active = (npts > 0) & ~(kin[ab[:, 0]] & kin[ab[:, 1]])
+    assert np.array_equal(col >= 0, active)

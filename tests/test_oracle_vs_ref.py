@@ -1,0 +1,71 @@
+"""Pins the oracle (C++ restatement) to the reference engine compiled from its own sources
+(oracle/_ref).  FP32 and FP64: every stage bit-exact.
+The reference sorts with std::sort, whose order among equal keys is unspecified; where its
+pair order differs from the restatement's (ties), the reference's order is fed to the oracle
+so both continue from the same pair list.  Skipped where oracle/_ref is not built."""
+import numpy as np
+import pytest
+
+from helpers import flat_state, pair_set, small_rain
+from pyoracle import OracleWorld, RefWorld
+
+pytestmark = pytest.mark.ref
+
+
+def _run(sc, bits, steps, exact):
+    o = OracleWorld(sc, bits, color_order=False)
+    r = RefWorld(sc, bits, threads=4)
+    im_o, li_o = o.mass_props()
+    im_r, li_r = r.mass_props()
+    assert np.array_equal(im_o, im_r) and np.array_equal(li_o, li_r)
+    eq = (lambda a, b: np.array_equal(a, b)) if exact else (lambda a, b: np.allclose(a, b, rtol=1e-12, atol=1e-12))
+    reorders = 0
+    for step in range(steps):
+        o.integrate(); r.integrate()
+        assert eq(flat_state(o.state()), flat_state(r.state())), f"integrate step {step}"
+        assert eq(np.concatenate(o.aabbs(), 1), np.concatenate(r.aabbs(), 1)), f"aabb step {step}"
+        o.broadphase(); r.broadphase()
+        po, pr = o.pairs(), r.pairs()
+        assert pair_set(po) == pair_set(pr), f"pair set step {step}"
+        assert o.axis() == r.axis()
+        if not np.array_equal(po, pr):
+            reorders += 1
+            o.set_pairs(pr)
+        o.narrowphase(); r.narrowphase()
+        (no, so, pto), (nr, sr, ptr) = o.contacts(), r.contacts()
+        assert np.array_equal(no, nr) and np.array_equal(so, sr), f"point counts step {step}"
+        vm = ptr[..., 0] < 1e29
+        assert np.array_equal(pto[..., 0] < 1e29, vm)
+        assert eq(pto[vm], ptr[vm]) and eq(pto[..., 12], ptr[..., 12]), f"contacts step {step}"
+        o.solve(); r.solve()
+        if exact:
+            assert eq(flat_state(o.state()), flat_state(r.state())), f"solve step {step}"
+        else:
+            o.set_state(*r.state())
+    return reorders
+
+
+@pytest.mark.parametrize("name", ["tower", "rain"])
+def test_fp32_bit_exact(name):
+    from pe_amd import scenes
+    sc = scenes.tower_scene() if name == "tower" else small_rain(250, seed=21)
+    _run(sc, 32, 60 if name == "tower" else 120, exact=True)
+
+
+@pytest.mark.parametrize("name", ["tower", "rain"])
+def test_fp64_bit_exact(name):
+    from pe_amd import scenes
+    sc = scenes.tower_scene() if name == "tower" else small_rain(250, seed=21)
+    _run(sc, 64, 60 if name == "tower" else 120, exact=True)
+
+
+def test_reference_pair_set_matches_brute_force():
+    from helpers import brute_force_pairs
+    sc = small_rain(200, seed=4)
+    r = RefWorld(sc, 32, threads=2)
+    for _ in range(50):
+        r.step(1)
+    r.integrate()
+    r.broadphase()
+    mn, mx = r.aabbs()
+    assert pair_set(r.pairs()) == brute_force_pairs(mn, mx, sc.kinematic)
