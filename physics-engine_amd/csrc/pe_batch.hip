@@ -1,17 +1,20 @@
 // Batched and served stepping: many independent worlds per launch.
 //
-// k_world_step<BLOCK> runs one whole World::step for one world per workgroup:
-//   integrate (thread / body) -> SAP keys -> bitonic sort of (key, index) in LDS ->
-//   candidate runs + load-balanced sweep (two passes, (i, j) order) -> next axis ->
-//   narrowphase (thread / pair) -> solver rows -> colouring (claims in LDS) ->
-//   colour-ordered Gauss-Seidel sweeps -> velocity sync.
+// One step of every world is three launches:
+//   K1 k_world_broad<BLOCK>  one workgroup per world: integrate (thread / body), SAP keys,
+//                            bitonic sort of (key, index) in LDS, candidate runs, load-balanced
+//                            sweep in (i, j) order, next axis; the world's pairs are appended to
+//                            a narrowphase work list
+//   K2 k_narrow_list         one thread per pair of every world: narrowphase + contact cache
+//   K3 k_world_solve<BLOCK>  one workgroup per world: solver rows, colouring (claims in LDS),
+//                            colour-ordered Gauss-Seidel sweeps, velocity sync
 // Every routine is the one the single-world pipeline uses (pe_pipeline.hpp), and the sort,
 // sweep order and colouring rule are the same, so a world stepped here is bit-identical to
 // the same world stepped alone -- batching never changes a result.
 //
-// Worlds are grouped into size classes; each class launches with its own block size (chosen by
-// an autotuner that times the candidates) and its LDS sized for the largest world in the class.
-// One step = one launch per class, captured once into a hipGraph and replayed.
+// Worlds are grouped into size classes; K1 and K3 of each class launch with the block size an
+// autotuner picked for it and LDS sized for the largest world in the class.  The launches of a
+// step are captured once into a hipGraph and replayed.
 #include "pe_common.hpp"
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
@@ -60,6 +63,9 @@ struct BatchDev {
     float g[3];
     int iterations;
     uint64_t* phase_ts;  // optional [nworlds][PHASES] timestamps (null: off)
+    int32_t* nl_pair;    // [total pair cap] narrowphase work list: global pair slots of all worlds
+    int32_t* nl_limit;   // [total pair cap] per listed pair: first slot of its world that starts fresh
+    int32_t* nl_count;   // [1] entries in the list this step
 };
 
 constexpr uint64_t KEY_PAD = ~0ull;
@@ -75,16 +81,20 @@ constexpr int PHASES = 8;   // start, integrate, sort, sweep, narrow, setup, col
 
 // LDS layout for a class whose largest world has nmax bodies.
 __host__ __device__ inline int np2(int n) { int p = 1; while (p < n) p <<= 1; return p; }
-__host__ inline size_t class_lds_bytes(int nmax) {
+__host__ inline size_t broad_lds_bytes(int nmax) {
     const int P = np2(std::max(nmax, 2));
-    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/ + (size_t)nmax * 4 /*claim*/;
+    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
+__host__ inline size_t solve_lds_bytes(int nmax) { return (size_t)std::max(nmax, 1) * 4; /*claims*/ }
 
+// K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
+// the narrowphase work list.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
+__global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int wsum[BLOCK / WAVE + 1];
     __shared__ double red[6][BLOCK / WAVE];
+    __shared__ int s_base;
     const int w = wlist[blockIdx.x];
     const WorldDesc wd = d.desc[w];
     const int n = wd.n, off = wd.body_off, poff = wd.pair_off;
@@ -94,7 +104,6 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
     uint64_t* key = (uint64_t*)lds;
     float* skey = (float*)(key + P2);
     uint32_t* cand = (uint32_t*)(skey + nmax);
-    int* claim = (int*)(cand + nmax + 1);
     const int axis = d.st[w].axis;
     const int prev_np = d.st[w].prev_npairs;
 
@@ -209,11 +218,43 @@ __global__ void __launch_bounds__(BLOCK) k_world_step(BatchDev d, const int32_t*
     __syncthreads();
 
     PE_PHASE(3);
-    // ---- stage 2: narrowphase (simple_narrow_phase.cpp:7-41)
-    for (int p = tid; p < npairs; p += BLOCK) narrow_pair(b, d.ps, poff + p, poff + prev_np);
+    // hand the pairs to the narrowphase list (the order of worlds in the list does not matter)
+    if (tid == 0) {
+        s_base = atomicAdd(d.nl_count, npairs);
+        d.st[w].npairs = npairs;
+    }
     __syncthreads();
-    PE_PHASE(4);
+    for (int p = tid; p < npairs; p += BLOCK) {
+        d.nl_pair[s_base + p] = poff + p;
+        d.nl_limit[s_base + p] = poff + prev_np;
+    }
+}
 
+// K2: SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) over the pairs of
+// all worlds at once, one thread per pair.
+__global__ void __launch_bounds__(128) k_narrow_list(BatchDev d) {
+    const int cnt = *d.nl_count;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x)
+        narrow_pair(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
+}
+
+__global__ void k_list_reset(BatchDev d) { *d.nl_count = 0; }
+
+// K3: solver setup, colouring, colour-ordered Gauss-Seidel and velocity sync for one world per
+// workgroup (sequential_impulse_solver.cpp:11-81).
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int wsum[BLOCK / WAVE + 1];
+    const int w = wlist[blockIdx.x];
+    const WorldDesc wd = d.desc[w];
+    const int n = wd.n, off = wd.body_off, poff = wd.pair_off;
+    const BodyView& b = d.b;
+    const int tid = threadIdx.x;
+    int* claim = (int*)lds;
+    const int npairs = d.st[w].npairs;
+    (void)nmax;
+    PE_PHASE(4);
     // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
     for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
     for (int i = tid; i < n; i += BLOCK) claim[i] = 0x7fffffff;
@@ -328,28 +369,32 @@ int check_scene(const pe_scene_desc* sc) {
     return PE_OK;
 }
 
-// Allows every instantiation the full LDS of a CU (worlds up to 4096 bodies need ~80 KiB).
+// Allows every instantiation the full LDS of a CU (worlds up to 4096 bodies need ~64 KiB).
+template <int B> void set_lds_one() {
+    const int bytes = 150 * 1024;
+    hipFuncSetAttribute((const void*)k_world_broad<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_world_solve<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
 void set_lds_limits() {
     static bool done = false;
     if (done) return;
     done = true;
-    const int bytes = 150 * 1024;
-    hipFuncSetAttribute((const void*)k_world_step<64>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_world_step<128>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_world_step<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_world_step<512>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_world_step<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    set_lds_one<64>(); set_lds_one<128>(); set_lds_one<256>(); set_lds_one<512>(); set_lds_one<1024>();
 }
 
-// Launch one class: template dispatch over the supported block sizes.
-hipError_t launch_class(int block, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
-    const size_t lds = class_lds_bytes(nmax);
+// Launches K1 (phase 0) or K3 (phase 1) of one class: dispatch over the block sizes.
+template <int B>
+void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
+    if (phase == 0) hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
+    else hipLaunchKernelGGL(k_world_solve<B>, dim3(count), dim3(B), solve_lds_bytes(nmax), s, d, wl, nmax);
+}
+hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
     switch (block) {
-        case 64: hipLaunchKernelGGL(k_world_step<64>, dim3(count), dim3(64), lds, s, d, wl, nmax); break;
-        case 128: hipLaunchKernelGGL(k_world_step<128>, dim3(count), dim3(128), lds, s, d, wl, nmax); break;
-        case 256: hipLaunchKernelGGL(k_world_step<256>, dim3(count), dim3(256), lds, s, d, wl, nmax); break;
-        case 512: hipLaunchKernelGGL(k_world_step<512>, dim3(count), dim3(512), lds, s, d, wl, nmax); break;
-        default: hipLaunchKernelGGL(k_world_step<1024>, dim3(count), dim3(1024), lds, s, d, wl, nmax); break;
+        case 64: launch_one<64>(phase, d, wl, count, nmax, s); break;
+        case 128: launch_one<128>(phase, d, wl, count, nmax, s); break;
+        case 256: launch_one<256>(phase, d, wl, count, nmax, s); break;
+        case 512: launch_one<512>(phase, d, wl, count, nmax, s); break;
+        default: launch_one<1024>(phase, d, wl, count, nmax, s); break;
     }
     return hipGetLastError();
 }
@@ -361,6 +406,22 @@ struct Class {
     int nmax = 0;
     int list_off = 0;   // offset into the device world list
 };
+
+// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class.
+int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s) {
+    hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
+    for (int phase = 0; phase < 2; phase++) {
+        for (auto& c : classes) {
+            if (c.count == 0) continue;
+            hipError_t e = launch_class(phase, c.block, d, dlist + c.list_off, c.count, c.nmax, s);
+            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+        }
+        if (phase == 0) hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+    return PE_OK;
+}
 
 // Default class table: body-count ranges and their block sizes.
 std::vector<Class> default_classes() {
@@ -420,14 +481,7 @@ struct pe_batch {
         PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         return PE_OK;
     }
-    int launch_step() {
-        for (auto& c : classes) {
-            if (c.count == 0) continue;
-            hipError_t e = launch_class(c.block, d, dlist + c.list_off, c.count, c.nmax, stream);
-            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
-        }
-        return PE_OK;
-    }
+    int launch_step() { return launch_all(d, classes, dlist, stream); }
     void drop_graph() {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
@@ -439,7 +493,7 @@ extern "C" {
 int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes) {
     if (!scene || !bytes) { pe_set_error("null argument"); return PE_ERR_INVALID; }
     const int64_t n = scene->n, pc = body_pair_cap(scene->n);
-    *bytes = n * (int64_t)(NF + NI) * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 7 * 4 + 4 * 4) + (int64_t)sizeof(WorldDesc) +
+    *bytes = n * (int64_t)(NF + NI + CBW + SBW + 8) * 4 + pc * (8 * (int64_t)(CW + 1 + RW) * 4 + 9 * 4 + 4 * 4) + (int64_t)sizeof(WorldDesc) +
              (int64_t)sizeof(WorldState);
     return PE_OK;
 }
@@ -481,15 +535,22 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     int rc = PE_OK;
     rc |= B->alloc(&d.b.f, NF * S);
     rc |= B->alloc(&d.b.i, NI * S);
+    rc |= B->alloc(&d.b.cb, (size_t)CBW * S);
+    rc |= B->alloc(&d.b.sb, (size_t)SBW * S);
+    rc |= B->alloc(&d.b.ab, (size_t)8 * S);
     rc |= B->alloc(&d.ps.pa, poff);
     rc |= B->alloc(&d.ps.pb, poff);
-    rc |= B->alloc(&d.ps.cp, (size_t)CF * poff * 8);
+    rc |= B->alloc(&d.ps.cp, (size_t)CW * poff * 8);
+    rc |= B->alloc(&d.ps.cd, (size_t)poff * 8);
     rc |= B->alloc(&d.ps.npts, poff);
     rc |= B->alloc(&d.ps.swap, poff);
-    rc |= B->alloc(&d.ps.rows, (size_t)RF * poff * 8);
+    rc |= B->alloc(&d.ps.rows, (size_t)RW * poff * 8);
     rc |= B->alloc(&d.act, 3 * (size_t)poff);
     rc |= B->alloc(&d.order, poff);
     rc |= B->alloc(&d.cstart, poff + nworlds);
+    rc |= B->alloc(&d.nl_pair, poff);
+    rc |= B->alloc(&d.nl_limit, poff);
+    rc |= B->alloc(&d.nl_count, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= B->alloc(&ddesc, nworlds);
@@ -730,7 +791,7 @@ struct Req {
 
 int64_t req_footprint(int n) {
     const int64_t pc = body_pair_cap(n);
-    return (int64_t)n * NSTAGE * 4 + pc * (8 * (int64_t)(CF + RF) * 4 + 8 * 4) + 4 * (int64_t)(pc + 1) + 64;
+    return (int64_t)n * (NSTAGE + CBW + SBW + 8) * 4 + pc * (8 * (int64_t)(CW + 1 + RW) * 4 + 10 * 4) + 4 * (int64_t)(pc + 1) + 64;
 }
 
 }  // namespace
@@ -879,11 +940,8 @@ struct pe_server {
         if (dirty) if (int rc = rebuild_lists()) return rc;
         int64_t nb = 0;
         for (Req* r : live) nb += r->n;
-        for (auto& c : classes) {
-            if (c.count == 0) continue;
-            hipError_t e = launch_class(c.block, d, dlist + c.list_off, c.count, c.nmax, cs);
-            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
-        }
+        if (!live.empty())
+            if (int rc = launch_all(d, classes, dlist, cs)) return rc;
         PE_HIP(hipEventRecord(ev_done, cs));
         // overlap: next admissions upload while this step runs
         stage_buf ^= 1;
@@ -968,15 +1026,22 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     int rc = PE_OK;
     rc |= S->alloc(&d.b.f, NF * SB);
     rc |= S->alloc(&d.b.i, NI * SB);
+    rc |= S->alloc(&d.b.cb, (size_t)CBW * SB);
+    rc |= S->alloc(&d.b.sb, (size_t)SBW * SB);
+    rc |= S->alloc(&d.b.ab, (size_t)8 * SB);
     rc |= S->alloc(&d.ps.pa, P);
     rc |= S->alloc(&d.ps.pb, P);
-    rc |= S->alloc(&d.ps.cp, (size_t)CF * P * 8);
+    rc |= S->alloc(&d.ps.cp, (size_t)CW * P * 8);
+    rc |= S->alloc(&d.ps.cd, (size_t)P * 8);
     rc |= S->alloc(&d.ps.npts, P);
     rc |= S->alloc(&d.ps.swap, P);
-    rc |= S->alloc(&d.ps.rows, (size_t)RF * P * 8);
+    rc |= S->alloc(&d.ps.rows, (size_t)RW * P * 8);
     rc |= S->alloc(&d.act, 3 * (size_t)P);
     rc |= S->alloc(&d.order, P);
     rc |= S->alloc(&d.cstart, P + cfg->max_worlds);
+    rc |= S->alloc(&d.nl_pair, P);
+    rc |= S->alloc(&d.nl_limit, P);
+    rc |= S->alloc(&d.nl_count, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= S->alloc(&ddesc, cfg->max_worlds);

@@ -31,21 +31,38 @@ enum BodyField : int {
 };
 enum BodyIntField : int { I_TYPE = 0, I_KIN = 1, NI = 2 };
 
-// Contact slot fields: one float array of CF fields x (pair capacity * 8)
-enum ContactField : int { C_WPOS = 0, C_N = 3, C_LA = 6, C_LB = 9, C_D = 12, CF = 13 };
-// Solver row fields: one float array of RF fields x (pair capacity * 8)
+// The narrowphase and the solver visit bodies, contact slots and rows in pair order, i.e. at
+// random; for them each record is contiguous (array of structures, float4-aligned) so one
+// body or one row costs one or two cache lines instead of one line per field.
+//
+// Collision-body record, written by the integrate stage: pose, AABB scale, shape, material.
+enum CBField : int { CB_R = 0, CB_P = 9, CB_SCALE = 12, CB_SP = 13, CB_TYPE = 16, CB_KIN = 17, CB_MU = 18, CB_E = 19, CBW = 20 };
+// Solver-body record: temp velocities (updated by the sweeps), inverse mass, kinematic flag,
+// world inverse inertia, and the velocities the solver setup reads.
+enum SBField : int { SB_TV = 0, SB_TW = 3, SB_IM = 6, SB_KIN = 7, SB_IW = 8, SB_V = 17, SB_W = 20, SBW = 24 };
+// Contact slot record (8 per pair): world_pos (on B), normal (B -> A), local A, local B.
+// Distances live in their own array (read for all 8 slots of a pair at the narrowphase start).
+enum ContactField : int { C_WPOS = 0, C_N = 3, C_LA = 6, C_LB = 9, CW = 12, CF = 13 };
+// Solver row record (one per contact slot), FrictionContactConstraint::ConstraintInfo
+// (friction_contact_constraint.h:15-29).
 enum RowField : int {
     R_RA = 0, R_RB = 3, R_N = 6, R_T0 = 9, R_T1 = 12, R_RHS = 15, R_MN = 16, R_MT0 = 17, R_MT1 = 18,
-    R_MU = 19, R_LN = 20, R_LT0 = 21, R_LT1 = 22, RF = 23
+    R_MU = 19, R_LN = 20, R_LT0 = 21, R_LT1 = 22, RF = 23, RW = 24
 };
 
 struct BodyView {
-    float* f;
-    int32_t* i;
+    float* f;          // [NF][stride] field-major state (coalesced per-body kernels)
+    int32_t* i;        // [NI][stride]
+    float* cb;         // [n][CBW] collision-body records
+    float* sb;         // [n][SBW] solver-body records
+    float* ab;         // [n][8] AABB records: min xyz, kinematic flag, max xyz, 0
     int n;
     int stride;
     __host__ __device__ float& at(int field, int b) const { return f[(size_t)field * stride + b]; }
     __host__ __device__ int32_t& iat(int field, int b) const { return i[(size_t)field * stride + b]; }
+    __host__ __device__ float* cbr(int b) const { return cb + (size_t)b * CBW; }
+    __host__ __device__ float* sbr(int b) const { return sb + (size_t)b * SBW; }
+    __host__ __device__ float* abr(int b) const { return ab + (size_t)b * 8; }
 };
 
 }  // namespace pe

@@ -12,11 +12,12 @@ namespace pe {
 struct PairStore {
     int32_t* pa;      // body index of A
     int32_t* pb;      // body index of B
-    float* cp;        // [CF][cs]
+    float* cp;        // [cap*8][CW] contact slot records
+    float* cd;        // [cap*8] contact distances
     int32_t* npts;
     int32_t* swap;
-    float* rows;      // [RF][cs]
-    size_t cs;        // slot stride = pair capacity * 8
+    float* rows;      // [cap*8][RW] solver rows
+    size_t cs;        // slot capacity = pair capacity * 8
 };
 
 __device__ __forceinline__ f3 ld3(const BodyView& b, int f, int i) { return f3{b.at(f, i), b.at(f + 1, i), b.at(f + 2, i)}; }
@@ -31,6 +32,46 @@ __device__ __forceinline__ void st9(const BodyView& b, int f, int i, const m33& 
 #pragma unroll
     for (int k = 0; k < 9; k++) b.at(f + k, i) = r.m[k];
 }
+// Record access in float4 units (records are 16-byte aligned).
+template <int N4> __device__ __forceinline__ void ld_rec(const float* src, float* dst) {
+    const float4* s = reinterpret_cast<const float4*>(src);
+#pragma unroll
+    for (int k = 0; k < N4; k++) { const float4 v = s[k]; dst[4 * k] = v.x; dst[4 * k + 1] = v.y; dst[4 * k + 2] = v.z; dst[4 * k + 3] = v.w; }
+}
+template <int N4> __device__ __forceinline__ void st_rec(float* dst, const float* src) {
+    float4* d = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int k = 0; k < N4; k++) d[k] = make_float4(src[4 * k], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]);
+}
+
+// Writes the collision-body, solver-body and AABB records of body i.
+__device__ __forceinline__ void write_records(const BodyView& b, int i, int type, bool kin, const float sp[4], const m33& R, f3 pos,
+                                              float scale, f3 mn, f3 mx, const m33& Iw, float im, f3 v, f3 w, f3 tv, f3 tw) {
+    float c[CBW];
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[CB_R + k] = R.m[k];
+    c[CB_P] = pos.x; c[CB_P + 1] = pos.y; c[CB_P + 2] = pos.z;
+    c[CB_SCALE] = scale;
+    c[CB_SP] = sp[0]; c[CB_SP + 1] = sp[1]; c[CB_SP + 2] = sp[2];
+    c[CB_TYPE] = __int_as_float(type);
+    c[CB_KIN] = __int_as_float(kin ? 1 : 0);
+    c[CB_MU] = b.at(F_MU, i);
+    c[CB_E] = b.at(F_E, i);
+    st_rec<CBW / 4>(b.cbr(i), c);
+    float s[SBW];
+    s[SB_TV] = tv.x; s[SB_TV + 1] = tv.y; s[SB_TV + 2] = tv.z;
+    s[SB_TW] = tw.x; s[SB_TW + 1] = tw.y; s[SB_TW + 2] = tw.z;
+    s[SB_IM] = im;
+    s[SB_KIN] = kin ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s[SB_IW + k] = Iw.m[k];
+    s[SB_V] = v.x; s[SB_V + 1] = v.y; s[SB_V + 2] = v.z;
+    s[SB_W] = w.x; s[SB_W + 1] = w.y; s[SB_W + 2] = w.z;
+    s[23] = 0.0f;
+    st_rec<SBW / 4>(b.sbr(i), s);
+    const float a[8] = {mn.x, mn.y, mn.z, kin ? 1.0f : 0.0f, mx.x, mx.y, mx.z, 0.0f};
+    st_rec<2>(b.abr(i), a);
+}
 
 // ------------------------------------------------------------------ stage 0
 // World::updateObjectStatus (world.cpp:35-119) with zero sleep thresholds (a body never
@@ -42,11 +83,14 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
     f3 pos = ld3(b, F_POS, i);
     m33 R = ld9(b, F_ROT, i);
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
+    f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
+    m33 Iw;
+    float im = 0.0f;
+    f3 tv{0, 0, 0}, tw{0, 0, 0};   // kinematic: temp velocities stay zero (rigidbody.cpp:72-80)
     if (!kin) {
-        f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
         integrate_pose(pos, R, v, w, dt);
         const m33 L = ld9(b, F_LINV, i);
-        st9(b, F_IW, i, mmt(mm(R, L), R));
+        Iw = mmt(mm(R, L), R);
         st3(b, F_POS, i, pos);
         st9(b, F_ROT, i, R);
         // RigidBody::applyDamping (rigidbody.cpp:201-205)
@@ -54,52 +98,79 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
         v = mul(v, pl);
         w = mul(w, pw);
         // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
-        const float m = b.at(F_MASS, i), im = b.at(F_IM, i);
+        const float m = b.at(F_MASS, i);
+        im = b.at(F_IM, i);
         const f3 f{g[0] * m, g[1] * m, g[2] * m};
         v = add(v, mul(mul(f, im), dt));
         st3(b, F_V, i, v);
         st3(b, F_W, i, w);
-        st3(b, F_TV, i, v);
-        st3(b, F_TW, i, w);
+        tv = v;
+        tw = w;
     } else {
-        st3(b, F_TV, i, f3{0, 0, 0});
-        st3(b, F_TW, i, f3{0, 0, 0});
+#pragma unroll
+        for (int k = 0; k < 9; k++) Iw.m[k] = 0.0f;
     }
     f3 mn, mx;
     shape_aabb(type, sp, R, pos, mn, mx);
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
-    b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
+    write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw);
 }
 
-// AABB, scale and world inverse inertia of the current pose (RigidBody::setTransform).
+// AABB, scale, world inverse inertia and records of the current pose (RigidBody::setTransform).
 __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
+    const int type = b.iat(I_TYPE, i);
+    const bool kin = b.iat(I_KIN, i) != 0;
     const f3 pos = ld3(b, F_POS, i);
     const m33 R = ld9(b, F_ROT, i);
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
     f3 mn, mx;
-    shape_aabb(b.iat(I_TYPE, i), sp, R, pos, mn, mx);
+    shape_aabb(type, sp, R, pos, mn, mx);
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
-    b.at(F_SCALE, i) = aabb_scale(mn, mx, pos);
-    st9(b, F_IW, i, mmt(mm(R, ld9(b, F_LINV, i)), R));
+    m33 Iw;
+    if (kin) { for (int k = 0; k < 9; k++) Iw.m[k] = 0.0f; }
+    else Iw = mmt(mm(R, ld9(b, F_LINV, i)), R);
+    const f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
+    write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, kin ? 0.0f : b.at(F_IM, i), v, w,
+                  kin ? f3{0, 0, 0} : v, kin ? f3{0, 0, 0} : w);
 }
 
 // ------------------------------------------------------------------ stage 1
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
 __device__ __forceinline__ bool bp_valid(const BodyView& b, int i, int j) {
-    if (b.iat(I_KIN, i) && b.iat(I_KIN, j)) return false;
-    if (b.at(F_AMIN, i) > b.at(F_AMAX, j) || b.at(F_AMIN, j) > b.at(F_AMAX, i)) return false;
-    if (b.at(F_AMIN + 1, i) > b.at(F_AMAX + 1, j) || b.at(F_AMIN + 1, j) > b.at(F_AMAX + 1, i)) return false;
-    if (b.at(F_AMIN + 2, i) > b.at(F_AMAX + 2, j) || b.at(F_AMIN + 2, j) > b.at(F_AMAX + 2, i)) return false;
+    float A[8], B[8];
+    ld_rec<2>(b.abr(i), A);
+    ld_rec<2>(b.abr(j), B);
+    if (A[3] != 0.0f && B[3] != 0.0f) return false;
+    if (A[0] > B[4] || B[0] > A[4]) return false;
+    if (A[1] > B[5] || B[1] > A[5]) return false;
+    if (A[2] > B[6] || B[2] > A[6]) return false;
     return true;
 }
 
 // ------------------------------------------------------------------ stage 2
-__device__ __forceinline__ void load_pose(const BodyView& b, int i, Pose& P) {
-    P.R = ld9(b, F_ROT, i);
-    P.p = ld3(b, F_POS, i);
-    P.scale = b.at(F_SCALE, i);
+struct CollBody {
+    Pose P;
+    int type;
+    bool kin;
+    float sp[3];
+    float mu, e;
+};
+__device__ __forceinline__ CollBody load_coll(const BodyView& b, int i) {
+    float c[CBW];
+    ld_rec<CBW / 4>(b.cbr(i), c);
+    CollBody r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.P.R.m[k] = c[CB_R + k];
+    r.P.p = f3{c[CB_P], c[CB_P + 1], c[CB_P + 2]};
+    r.P.scale = c[CB_SCALE];
+    r.sp[0] = c[CB_SP]; r.sp[1] = c[CB_SP + 1]; r.sp[2] = c[CB_SP + 2];
+    r.type = __float_as_int(c[CB_TYPE]);
+    r.kin = __float_as_int(c[CB_KIN]) != 0;
+    r.mu = c[CB_MU];
+    r.e = c[CB_E];
+    return r;
 }
 
 // SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) for pair slot p.
@@ -110,93 +181,105 @@ static __device__ __noinline__ void narrow_pair(const BodyView& b, const PairSto
     const int ia = ps.pa[p], ib = ps.pb[p];
     Cache c;
     const size_t slot0 = (size_t)p * 8;
-    const size_t CS = ps.cs;
+    float dprev[8];
+    if (p < prev_np) ld_rec<2>(ps.cd + slot0, dprev);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         c.wx[k] = REAL_MAX; c.wy[k] = REAL_MAX; c.wz[k] = REAL_MAX;
         c.nx[k] = 0; c.ny[k] = 0; c.nz[k] = 0;
         c.ax[k] = 0; c.ay[k] = 0; c.az[k] = 0;
         c.bx[k] = 0; c.by[k] = 0; c.bz[k] = 0;
-        c.d[k] = p < prev_np ? ps.cp[C_D * CS + slot0 + k] : REAL_MAX;
+        c.d[k] = p < prev_np ? dprev[k] : REAL_MAX;
     }
+    const CollBody A = load_coll(b, ia), B = load_coll(b, ib);
     PairCtx x;
-    load_pose(b, ia, x.A);
-    load_pose(b, ib, x.B);
+    x.A = A.P;
+    x.B = B.P;
     x.swap = false;
     {
         const float sa = x.A.scale, sb = x.B.scale;
         const float th = (sa < sb ? sa : sb) * 0.02f;
         x.same_th2 = th * th;
     }
-    const int ta = b.iat(I_TYPE, ia), tb = b.iat(I_TYPE, ib);
-    const float spa[3] = {b.at(F_SP, ia), b.at(F_SP + 1, ia), b.at(F_SP + 2, ia)};
-    const float spb[3] = {b.at(F_SP, ib), b.at(F_SP + 1, ib), b.at(F_SP + 2, ib)};
-    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, spa, spb);
-    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, spa, spb[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, spb, spa[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, spa[0], spb[0]);
+    const int ta = A.type, tb = B.type;
+    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.sp, B.sp);
+    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.sp, B.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, B.sp, A.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.sp[0], B.sp[0]);
     int rank[8];
     const int np = sort_rank(c, rank);
     ps.npts[p] = np;
     ps.swap[p] = x.swap ? 1 : 0;
+    float dsorted[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const size_t s = slot0 + rank[k];
-        float* cp = ps.cp;
-        cp[(C_WPOS + 0) * CS + s] = c.wx[k]; cp[(C_WPOS + 1) * CS + s] = c.wy[k]; cp[(C_WPOS + 2) * CS + s] = c.wz[k];
-        cp[(C_N + 0) * CS + s] = c.nx[k]; cp[(C_N + 1) * CS + s] = c.ny[k]; cp[(C_N + 2) * CS + s] = c.nz[k];
-        cp[(C_LA + 0) * CS + s] = c.ax[k]; cp[(C_LA + 1) * CS + s] = c.ay[k]; cp[(C_LA + 2) * CS + s] = c.az[k];
-        cp[(C_LB + 0) * CS + s] = c.bx[k]; cp[(C_LB + 1) * CS + s] = c.by[k]; cp[(C_LB + 2) * CS + s] = c.bz[k];
-        cp[C_D * CS + s] = c.d[k];
+#pragma unroll
+        for (int q = 0; q < 8; q++) if (rank[q] == k) dsorted[k] = c.d[q];
+    }
+    st_rec<2>(ps.cd + slot0, dsorted);
+    // only the valid points are read later; write them at their sorted positions
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (rank[k] >= np) continue;
+        const float rec[CW] = {c.wx[k], c.wy[k], c.wz[k], c.nx[k], c.ny[k], c.nz[k],
+                               c.ax[k], c.ay[k], c.az[k], c.bx[k], c.by[k], c.bz[k]};
+        st_rec<CW / 4>(ps.cp + (slot0 + rank[k]) * CW, rec);
     }
 }
 
 // ------------------------------------------------------------------ stage 3
+struct SolvBody {
+    m33 Iw;
+    float im;
+    bool kin;
+    f3 tv, tw, v, w;
+};
+__device__ __forceinline__ SolvBody load_solv(const BodyView& b, int i) {
+    float s[SBW];
+    ld_rec<SBW / 4>(b.sbr(i), s);
+    SolvBody r;
+    r.tv = f3{s[SB_TV], s[SB_TV + 1], s[SB_TV + 2]};
+    r.tw = f3{s[SB_TW], s[SB_TW + 1], s[SB_TW + 2]};
+    r.im = s[SB_IM];
+    r.kin = s[SB_KIN] != 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.Iw.m[k] = s[SB_IW + k];
+    r.v = f3{s[SB_V], s[SB_V + 1], s[SB_V + 2]};
+    r.w = f3{s[SB_W], s[SB_W + 1], s[SB_W + 2]};
+    return r;
+}
+
 // FrictionContactConstraint::initSequentialImpulse (friction_contact_constraint.cpp:9-73) for
 // every point of pair p.
 __device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt) {
     const int n = ps.npts[p];
     if (n == 0) return;
-    const size_t CS = ps.cs;
     const int ia = ps.pa[p], ib = ps.pb[p];
-    const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
-    const m33 RA = ld9(b, F_ROT, ia), RB = ld9(b, F_ROT, ib);
-    m33 Ia, Ib;
-    if (ka) { for (int k = 0; k < 9; k++) Ia.m[k] = 0.0f; } else Ia = ld9(b, F_IW, ia);
-    if (kb) { for (int k = 0; k < 9; k++) Ib.m[k] = 0.0f; } else Ib = ld9(b, F_IW, ib);
-    const float ims = (ka ? 0.0f : b.at(F_IM, ia)) + (kb ? 0.0f : b.at(F_IM, ib));
-    const f3 va = ld3(b, F_V, ia), wa = ld3(b, F_W, ia), vb = ld3(b, F_V, ib), wb = ld3(b, F_W, ib);
+    const CollBody CA = load_coll(b, ia), CB = load_coll(b, ib);
+    const SolvBody A = load_solv(b, ia), B = load_solv(b, ib);
+    // getKinematicInvMass / getKinematicWorldInvInertia (rigidbody.cpp:48-62): zero when kinematic
+    const float ims = A.im + B.im;
     // ContactResult::setObjects (contact_result.cpp:80-86)
-    const float mu = sqrtf(b.at(F_MU, ia) * b.at(F_MU, ib));
-    const float e = sqrtf(b.at(F_E, ia) * b.at(F_E, ib));
-    float* R = ps.rows;
-    const float* cp = ps.cp;
+    const float mu = sqrtf(CA.mu * CB.mu);
+    const float e = sqrtf(CA.e * CB.e);
     for (int k = 0; k < n; k++) {
         const size_t s = (size_t)p * 8 + k;
-        const f3 la{cp[(C_LA) * CS + s], cp[(C_LA + 1) * CS + s], cp[(C_LA + 2) * CS + s]};
-        const f3 lb{cp[(C_LB) * CS + s], cp[(C_LB + 1) * CS + s], cp[(C_LB + 2) * CS + s]};
-        const f3 nn{cp[(C_N) * CS + s], cp[(C_N + 1) * CS + s], cp[(C_N + 2) * CS + s]};
-        const Row r = make_row(RA, RB, la, lb, nn, cp[C_D * CS + s], ims, Ia, Ib, va, wa, vb, wb, mu, e, dt);
-        R[(R_RA) * CS + s] = r.ra.x; R[(R_RA + 1) * CS + s] = r.ra.y; R[(R_RA + 2) * CS + s] = r.ra.z;
-        R[(R_RB) * CS + s] = r.rb.x; R[(R_RB + 1) * CS + s] = r.rb.y; R[(R_RB + 2) * CS + s] = r.rb.z;
-        R[(R_N) * CS + s] = r.n.x; R[(R_N + 1) * CS + s] = r.n.y; R[(R_N + 2) * CS + s] = r.n.z;
-        R[(R_T0) * CS + s] = r.t0.x; R[(R_T0 + 1) * CS + s] = r.t0.y; R[(R_T0 + 2) * CS + s] = r.t0.z;
-        R[(R_T1) * CS + s] = r.t1.x; R[(R_T1 + 1) * CS + s] = r.t1.y; R[(R_T1 + 2) * CS + s] = r.t1.z;
-        R[R_RHS * CS + s] = r.rhs; R[R_MN * CS + s] = r.mn; R[R_MT0 * CS + s] = r.mt0; R[R_MT1 * CS + s] = r.mt1;
-        R[R_MU * CS + s] = r.mu; R[R_LN * CS + s] = 0.0f; R[R_LT0 * CS + s] = 0.0f; R[R_LT1 * CS + s] = 0.0f;
+        float c[CW];
+        ld_rec<CW / 4>(ps.cp + s * CW, c);
+        const f3 nn{c[C_N], c[C_N + 1], c[C_N + 2]};
+        const f3 la{c[C_LA], c[C_LA + 1], c[C_LA + 2]};
+        const f3 lb{c[C_LB], c[C_LB + 1], c[C_LB + 2]};
+        const Row r = make_row(CA.P.R, CB.P.R, la, lb, nn, ps.cd[s], ims, A.Iw, B.Iw, A.v, A.w, B.v, B.w, mu, e, dt);
+        float o[RW];
+        o[R_RA] = r.ra.x; o[R_RA + 1] = r.ra.y; o[R_RA + 2] = r.ra.z;
+        o[R_RB] = r.rb.x; o[R_RB + 1] = r.rb.y; o[R_RB + 2] = r.rb.z;
+        o[R_N] = r.n.x; o[R_N + 1] = r.n.y; o[R_N + 2] = r.n.z;
+        o[R_T0] = r.t0.x; o[R_T0 + 1] = r.t0.y; o[R_T0 + 2] = r.t0.z;
+        o[R_T1] = r.t1.x; o[R_T1 + 1] = r.t1.y; o[R_T1 + 2] = r.t1.z;
+        o[R_RHS] = r.rhs; o[R_MN] = r.mn; o[R_MT0] = r.mt0; o[R_MT1] = r.mt1; o[R_MU] = r.mu;
+        o[R_LN] = 0.0f; o[R_LT0] = 0.0f; o[R_LT1] = 0.0f; o[23] = 0.0f;
+        st_rec<RW / 4>(ps.rows + s * RW, o);
     }
-}
-
-__device__ __forceinline__ Row load_row(const float* R, size_t CS, size_t s) {
-    Row r;
-    r.ra = f3{R[(R_RA) * CS + s], R[(R_RA + 1) * CS + s], R[(R_RA + 2) * CS + s]};
-    r.rb = f3{R[(R_RB) * CS + s], R[(R_RB + 1) * CS + s], R[(R_RB + 2) * CS + s]};
-    r.n = f3{R[(R_N) * CS + s], R[(R_N + 1) * CS + s], R[(R_N + 2) * CS + s]};
-    r.t0 = f3{R[(R_T0) * CS + s], R[(R_T0 + 1) * CS + s], R[(R_T0 + 2) * CS + s]};
-    r.t1 = f3{R[(R_T1) * CS + s], R[(R_T1 + 1) * CS + s], R[(R_T1 + 2) * CS + s]};
-    r.rhs = R[R_RHS * CS + s]; r.mn = R[R_MN * CS + s]; r.mt0 = R[R_MT0 * CS + s]; r.mt1 = R[R_MT1 * CS + s];
-    r.mu = R[R_MU * CS + s];
-    return r;
 }
 
 // RigidBody::applyTempImpulse (rigidbody.cpp:166-170), with the body's inverse mass and world
@@ -207,53 +290,61 @@ __device__ __forceinline__ void apply_temp(f3& tv, f3& tw, float im, const m33& 
 }
 
 // FrictionContactConstraint::iterateSequentialImpulse (friction_contact_constraint.cpp:75-108)
-// over all rows of pair p.  All loads of a group of up to 4 rows and of both bodies are issued
-// before the dependent chain, so a pair costs a few memory round trips, not one per row.
+// over all rows of pair p: both bodies and all rows are loaded before the dependent chain.
 __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore& ps, int p) {
-    const size_t CS = ps.cs;
     const int ia = ps.pa[p], ib = ps.pb[p];
     const int n = ps.npts[p];
-    const bool ka = b.iat(I_KIN, ia) != 0, kb = b.iat(I_KIN, ib) != 0;
-    f3 tva = ld3(b, F_TV, ia), twa = ld3(b, F_TW, ia), tvb = ld3(b, F_TV, ib), twb = ld3(b, F_TW, ib);
-    const m33 IwA = ld9(b, F_IW, ia), IwB = ld9(b, F_IW, ib);
-    const float imA = b.at(F_IM, ia), imB = b.at(F_IM, ib);
+    SolvBody A = load_solv(b, ia), B = load_solv(b, ib);
     float* R = ps.rows;
     for (int k0 = 0; k0 < n; k0 += 4) {
-        Row rw[4];
-        float ln[4], lt0[4], lt1[4];
+        float rr[4][RW];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (k0 + u < n) ld_rec<RW / 4>(R + ((size_t)p * 8 + k0 + u) * RW, rr[u]);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             if (k0 + u < n) {
-                const size_t s = (size_t)p * 8 + k0 + u;
-                rw[u] = load_row(R, CS, s);
-                ln[u] = R[R_LN * CS + s]; lt0[u] = R[R_LT0 * CS + s]; lt1[u] = R[R_LT1 * CS + s];
+                const float* o = rr[u];
+                Row r;
+                r.ra = f3{o[R_RA], o[R_RA + 1], o[R_RA + 2]};
+                r.rb = f3{o[R_RB], o[R_RB + 1], o[R_RB + 2]};
+                r.n = f3{o[R_N], o[R_N + 1], o[R_N + 2]};
+                r.t0 = f3{o[R_T0], o[R_T0 + 1], o[R_T0 + 2]};
+                r.t1 = f3{o[R_T1], o[R_T1 + 1], o[R_T1 + 2]};
+                r.rhs = o[R_RHS]; r.mn = o[R_MN]; r.mt0 = o[R_MT0]; r.mt1 = o[R_MT1]; r.mu = o[R_MU];
+                float ln = o[R_LN], lt0 = o[R_LT0], lt1 = o[R_LT1];
+                const f3 J = iterate_row(r, ln, lt0, lt1, A.tv, A.tw, B.tv, B.tw);
+                if (!A.kin) apply_temp(A.tv, A.tw, A.im, A.Iw, r.ra, J);
+                if (!B.kin) apply_temp(B.tv, B.tw, B.im, B.Iw, r.rb, neg(J));
+                rr[u][R_LN] = ln; rr[u][R_LT0] = lt0; rr[u][R_LT1] = lt1;
             }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             if (k0 + u < n) {
-                const f3 J = iterate_row(rw[u], ln[u], lt0[u], lt1[u], tva, twa, tvb, twb);
-                if (!ka) apply_temp(tva, twa, imA, IwA, rw[u].ra, J);
-                if (!kb) apply_temp(tvb, twb, imB, IwB, rw[u].rb, neg(J));
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            if (k0 + u < n) {
-                const size_t s = (size_t)p * 8 + k0 + u;
-                R[R_LN * CS + s] = ln[u]; R[R_LT0 * CS + s] = lt0[u]; R[R_LT1 * CS + s] = lt1[u];
+                float4* dst = reinterpret_cast<float4*>(R + ((size_t)p * 8 + k0 + u) * RW) + 5;   // floats 20..23
+                *dst = make_float4(rr[u][R_LN], rr[u][R_LT0], rr[u][R_LT1], 0.0f);
             }
         }
     }
-    if (!ka) { st3(b, F_TV, ia, tva); st3(b, F_TW, ia, twa); }
-    if (!kb) { st3(b, F_TV, ib, tvb); st3(b, F_TW, ib, twb); }
+    if (!A.kin) {
+        float4* d = reinterpret_cast<float4*>(b.sbr(ia));
+        d[0] = make_float4(A.tv.x, A.tv.y, A.tv.z, A.tw.x);
+        d[1].x = A.tw.y; d[1].y = A.tw.z;
+    }
+    if (!B.kin) {
+        float4* d = reinterpret_cast<float4*>(b.sbr(ib));
+        d[0] = make_float4(B.tv.x, B.tv.y, B.tv.z, B.tw.x);
+        d[1].x = B.tw.y; d[1].y = B.tw.z;
+    }
 }
 
 // SequentialImpulseSolver::solve's syncTempVelocity (sequential_impulse_solver.cpp:71-80)
 __device__ __forceinline__ void sync_body(const BodyView& b, int i) {
     if (b.iat(I_KIN, i)) return;
-    st3(b, F_V, i, ld3(b, F_TV, i));
-    st3(b, F_W, i, ld3(b, F_TW, i));
+    const float* s = b.sbr(i);
+    st3(b, F_V, i, f3{s[SB_TV], s[SB_TV + 1], s[SB_TV + 2]});
+    st3(b, F_W, i, f3{s[SB_TW], s[SB_TW + 1], s[SB_TW + 2]});
 }
 
 // ------------------------------------------------------------------ constraint-graph colouring

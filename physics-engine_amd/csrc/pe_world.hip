@@ -9,8 +9,9 @@
 //   k_bp_axis          next axis = argmax variance of AABB centres
 //   k_narrow           thread / pair: shape dispatch + 8-slot contact cache + depth sort
 //   k_setup            thread / pair: solver rows (FrictionContactConstraint::initSequentialImpulse)
-//   k_solve            one 1024-thread workgroup: constraint-graph colouring, then the
-//                      Gauss-Seidel sweeps colour by colour, then velocity sync
+//   k_colour           one 1024-thread workgroup: constraint-graph colouring (claims in LDS)
+//   k_gs_grid          cooperative launch over the whole GPU: Gauss-Seidel sweeps colour by
+//                      colour with a grid barrier between colours, then velocity sync
 #include "pe_common.hpp"
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
@@ -45,7 +46,9 @@ struct DevCounts {          // device-side counters of one step
     int32_t nordered;       // pairs in the solve order
     int32_t npoints;        // contact points of the last narrowphase
     uint32_t nchunks;       // live sweep chunks
-    int32_t pad[6];
+    uint32_t bar_count;     // grid barrier: arrivals (0 between barriers)
+    uint32_t bar_gen;       // grid barrier: generation
+    int32_t pad[4];
 };
 
 struct Dev {
@@ -230,22 +233,16 @@ __global__ void k_setup(Dev d) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) setup_pair(d.b, d.ps, p, d.dt);
 }
 
-// Colouring + colour-ordered Gauss-Seidel + velocity sync, one workgroup.
-// Round c gives colour c to every uncoloured pair that holds, for each of its dynamic bodies,
-// the lowest pair index among the uncoloured pairs on that body.  Pairs of one colour share no
-// dynamic body (kinematic bodies never change, rigidbody.cpp:167), so sweeping a colour in
-// parallel equals the sequential sweep in (colour, index) order.
+// Constraint-graph colouring of the pairs with contact points, one workgroup (colour_pairs in
+// pe_pipeline.hpp): writes the solve order, colour starts and colour count.
 template <bool LDS_CLAIM>
-__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
+__global__ void __launch_bounds__(SOLVE_THREADS) k_colour(Dev d) {
     extern __shared__ int s_claim[];
     __shared__ int wsum[SOLVE_THREADS / WAVE + 1];
-    __shared__ int s_nact, s_out;
     const BodyView& b = d.b;
     const int np = d.cnt->npairs;
     int* claim = LDS_CLAIM ? s_claim : d.claim;
-    const int INF = 0x7fffffff;
-    for (int i = threadIdx.x; i < b.n; i += blockDim.x) claim[i] = INF;
-    // active list (index order)
+    for (int i = threadIdx.x; i < b.n; i += blockDim.x) claim[i] = colour::FREE;
     int* act = d.act;
     int nact = 0;
     for (int base = 0; base < np; base += blockDim.x) {
@@ -264,18 +261,43 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(Dev d) {
         d.cnt->ncolors = ncol;
         d.cnt->nordered = d.color_start[ncol];
     }
+}
+
+// Gauss-Seidel sweeps (SequentialImpulseSolver::solve, sequential_impulse_solver.cpp:60-81) in
+// colour order across the whole GPU: the pairs of one colour share no dynamic body, so they run
+// in parallel; a grid barrier separates colours.  Then syncTempVelocity.  Launched cooperatively
+// (all workgroups co-resident).
+constexpr int GS_THREADS = 256;
+
+// Sense-reversing barrier over all workgroups of a cooperative launch (every workgroup is
+// resident, so every wait ends).  The last arrival resets the count and bumps the generation.
+__device__ __forceinline__ void grid_barrier(uint32_t* count, uint32_t* gen, uint32_t nblocks) {
     __syncthreads();
-    // Gauss-Seidel sweeps
+    if (threadIdx.x == 0) {
+        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
+    const int ncol = d.cnt->ncolors;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
     for (int it = 0; it < d.iterations; it++) {
         for (int c = 0; c < ncol; c++) {
             const int c0 = d.color_start[c], c1 = d.color_start[c + 1];
-            for (int q = c0 + threadIdx.x; q < c1; q += blockDim.x) iterate_pair(d.b, d.ps, d.order[q]);
-            __syncthreads();
+            for (int q = c0 + gtid; q < c1; q += gsz) iterate_pair(d.b, d.ps, d.order[q]);
+            __threadfence();   // release this colour's velocity updates
+            grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
+            __threadfence();   // acquire: later colours read velocities other CUs wrote
         }
     }
-    // SequentialImpulseSolver::solve's syncTempVelocity (sequential_impulse_solver.cpp:71-80)
-    for (int i = threadIdx.x; i < b.n; i += blockDim.x) sync_body(b, i);
-    (void)s_nact; (void)s_out;
+    for (int i = gtid; i < d.b.n; i += gsz) sync_body(d.b, i);
 }
 
 __global__ void k_count_points(Dev d) {
@@ -301,6 +323,8 @@ struct pe_world {
     std::vector<void*> allocs;
     int rs_blocks = 0;
     int red_blocks = 0;
+    int gs_blocks = 0;      // workgroups of the cooperative Gauss-Seidel kernel
+    int gs_max_blocks = 0;  // co-resident limit
     bool timing = false;
     hipEvent_t ev[5] = {};
     double stage_ms[4] = {0, 0, 0, 0};
@@ -359,9 +383,11 @@ struct pe_world {
     int launch_solve() {
         hipLaunchKernelGGL(k_setup, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
         if (n <= CLAIM_LDS)
-            hipLaunchKernelGGL(k_solve<true>, dim3(1), dim3(SOLVE_THREADS), n * sizeof(int), stream, d);
+            hipLaunchKernelGGL(k_colour<true>, dim3(1), dim3(SOLVE_THREADS), n * sizeof(int), stream, d);
         else
-            hipLaunchKernelGGL(k_solve<false>, dim3(1), dim3(SOLVE_THREADS), 0, stream, d);
+            hipLaunchKernelGGL(k_colour<false>, dim3(1), dim3(SOLVE_THREADS), 0, stream, d);
+        void* args[] = {(void*)&d};
+        PE_HIP(hipLaunchCooperativeKernel((const void*)k_gs_grid, dim3(gs_blocks), dim3(GS_THREADS), args, 0, stream));
         return PE_OK;
     }
     int step() {
@@ -435,10 +461,14 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.b.i, NI * S);
     rc |= w->alloc(&d.ps.pa, pcap);
     rc |= w->alloc(&d.ps.pb, pcap);
-    rc |= w->alloc(&d.ps.cp, (size_t)CF * pcap * 8);
+    rc |= w->alloc(&d.ps.cp, (size_t)CW * pcap * 8);
+    rc |= w->alloc(&d.ps.cd, (size_t)pcap * 8);
+    rc |= w->alloc(&d.b.cb, (size_t)CBW * S);
+    rc |= w->alloc(&d.b.sb, (size_t)SBW * S);
+    rc |= w->alloc(&d.b.ab, (size_t)8 * S);
     rc |= w->alloc(&d.ps.npts, pcap);
     rc |= w->alloc(&d.ps.swap, pcap);
-    rc |= w->alloc(&d.ps.rows, (size_t)RF * pcap * 8);
+    rc |= w->alloc(&d.ps.rows, (size_t)RW * pcap * 8);
     rc |= w->alloc(&d.order, pcap);
     rc |= w->alloc(&d.act, 3 * (size_t)pcap);
     rc |= w->alloc(&d.color_start, pcap + 1);
@@ -518,6 +548,16 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     }
     hipLaunchKernelGGL(k_aabb_only, dim3(w->grid(n, 256)), dim3(256), 0, w->stream, d);
     for (auto& e : w->ev) hipEventCreate(&e);
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs_grid, GS_THREADS, 0) != hipSuccess || per_cu < 1) {
+            pe_set_error("cannot size the cooperative solver grid");
+            return fail(PE_ERR_DEVICE);
+        }
+        w->gs_max_blocks = cus * per_cu;
+        w->gs_blocks = std::min(w->gs_max_blocks, std::max(4, std::min(64, n / 2500)));
+    }
     if (hipStreamSynchronize(w->stream) != hipSuccess) { pe_set_error("init sync failed"); return fail(PE_ERR_DEVICE); }
     *out = w;
     return PE_OK;
@@ -647,13 +687,19 @@ int pe_world_get_contacts(pe_world* w, int32_t* npoints, float* points, int32_t 
     if (int rc = read_counts(w, c)) return rc;
     const int m = std::min(cap, c.npairs);
     if (m <= 0) return PE_OK;
-    if (npoints) PE_HIP(hipMemcpy(npoints, w->d.ps.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int32_t> np(m);
+    PE_HIP(hipMemcpy(np.data(), w->d.ps.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (npoints) std::copy(np.begin(), np.end(), npoints);
     if (points) {
-        const size_t CS = (size_t)w->d.pcap * 8;
-        std::vector<float> tmp((size_t)m * 8);
-        for (int f = 0; f < CF; f++) {
-            PE_HIP(hipMemcpy(tmp.data(), w->d.ps.cp + f * CS, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
-            for (size_t s = 0; s < tmp.size(); s++) points[s * CF + f] = tmp[s];
+        // slots at or beyond the point count are reported invalid; every slot's distance is reported
+        std::vector<float> rec((size_t)m * 8 * CW), dist((size_t)m * 8);
+        PE_HIP(hipMemcpy(rec.data(), w->d.ps.cp, rec.size() * sizeof(float), hipMemcpyDeviceToHost));
+        PE_HIP(hipMemcpy(dist.data(), w->d.ps.cd, dist.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t s = 0; s < (size_t)m * 8; s++) {
+            float* o = points + s * CF;
+            const bool valid = (int)(s % 8) < np[s / 8];
+            for (int f = 0; f < CW; f++) o[f] = valid ? rec[s * CW + f] : (f < 3 ? REAL_MAX : 0.0f);
+            o[12] = dist[s];
         }
     }
     return PE_OK;
@@ -697,6 +743,13 @@ int pe_world_get_stage_times(pe_world* w, double* ms) {
     for (int k = 0; k < 4; k++) ms[k] = w->stage_ms[k];
     return PE_OK;
 }
+int pe_world_set_solver_blocks(pe_world* w, int32_t blocks) {
+    if (int rc = check_world(w)) return rc;
+    if (blocks < 0) { pe_set_error("blocks must be >= 0"); return PE_ERR_INVALID; }
+    w->gs_blocks = blocks == 0 ? std::min(w->gs_max_blocks, std::max(4, std::min(64, w->n / 2500))) : std::min(blocks, w->gs_max_blocks);
+    return PE_OK;
+}
+
 int pe_world_get_counts(pe_world* w, int64_t* counts) {
     if (int rc = check_world(w)) return rc;
     DevCounts c;

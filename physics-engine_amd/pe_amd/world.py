@@ -138,6 +138,9 @@ class World:
         check(lib().pe_world_get_stage_times(self.h, ms))
         return list(ms)
 
+    def set_solver_blocks(self, blocks: int):
+        check(lib().pe_world_set_solver_blocks(self.h, ctypes.c_int32(blocks)))
+
     def counts(self):
         c = (ctypes.c_int64 * 5)()
         check(lib().pe_world_get_counts(self.h, c))
