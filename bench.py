@@ -32,12 +32,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def stage_bytes(n, n_dyn, pairs, points, hull_bytes=0):
+    """Algorithmic (compulsory) HBM bytes of one step per launch group: every stage reads its
+    inputs once and writes its outputs once, one Gauss-Seidel sweep counted.
+      K1 integrate + broadphase: 148 B / dynamic body + 64 B / body + 8 B / emitted pair
+      K2 narrowphase:            96 B / candidate pair + hull bytes + 36 B / contact point
+      K3 setup + solve:          216 B + 200 B / contact point"""
+    return [148 * n_dyn + 64 * n + 8 * pairs, 96 * pairs + hull_bytes + 36 * points, 416 * points]
+
+
 def step_bytes(n, n_dyn, pairs, points, hull_bytes=0):
-    """Algorithmic (compulsory) HBM bytes of one step: every stage reads its inputs once and
-    writes its outputs once, one Gauss-Seidel sweep counted:
-      integrate 148 B / dynamic body, broadphase 64 B / body + 8 B / pair,
-      narrowphase 96 B / candidate pair + hull bytes, setup + solve 452 B / contact point."""
-    return 148 * n_dyn + 64 * n + 8 * pairs + 96 * pairs + hull_bytes + 452 * points
+    return sum(stage_bytes(n, n_dyn, pairs, points, hull_bytes))
 
 
 def run_batched(args, worlds):
@@ -59,8 +64,11 @@ def run_batched(args, worlds):
     c1 = b.counts()
     pairs = 0.5 * (c0["pairs"] + c1["pairs"])
     points = 0.5 * (c0["points"] + c1["points"])
+    nst = 10
+    st_ms = [x / nst for x in b.stage_times(nst)]
+    c2 = b.counts()
     return dict(wall=wall, dev_ms=dev_ms, counts0=c0, counts1=c1, pairs=pairs, points=points,
-                n=b.n_bodies, n_dyn=b.n_dynamic, tuned=tuned, batch=b)
+                n=b.n_bodies, n_dyn=b.n_dynamic, tuned=tuned, batch=b, stage_ms=st_ms, stage_counts=c2)
 
 
 def run_plain(scene, steps, warmup=0):
@@ -169,17 +177,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-steps", type=int, default=600)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device-to-device with torch (a kernel of known traffic, "
+                         "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build steps one GPU; --gpus {args.gpus} is not supported "
               f"(worlds shard across GPUs with no step-path traffic; see the design notes)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 1 GiB read + 1 GiB written")
+        del x, y
 
     from pe_amd import scenes
     t0 = time.perf_counter()
@@ -192,16 +213,25 @@ def main():
     ms_step = 1000.0 * bt["wall"] / K
     dev_step_ms = bt["dev_ms"] / K
     B = step_bytes(bt["n"], bt["n_dyn"], bt["pairs"], bt["points"])
-    achieved = B / (dev_step_ms * 1e-3) / 1e9
+    step_gbs = B / (dev_step_ms * 1e-3) / 1e9
+    sc = bt["stage_counts"]
+    sb = stage_bytes(sc["bodies"], sc["dynamic"], sc["pairs"], sc["points"])
+    names = ["k_world_broad (integrate + sweep-and-prune, all size classes)", "k_narrow_list (narrowphase)",
+             "k_world_solve (solver setup + colouring + Gauss-Seidel, all size classes)"]
+    k = max(range(3), key=lambda i: bt["stage_ms"][i])
+    achieved = sb[k] / (bt["stage_ms"][k] * 1e-3) / 1e9
     log(f"batched: {value:.3e} body-steps/s, {ms_step:.3f} ms/step wall, {dev_step_ms:.3f} ms/step device, "
-        f"pairs {bt['pairs']:.0f} points {bt['points']:.0f} bytes/step {B / 1e6:.1f} MB -> {achieved:.1f} GB/s")
+        f"pairs {bt['pairs']:.0f} points {bt['points']:.0f} bytes/step {B / 1e6:.1f} MB -> {step_gbs:.1f} GB/s; "
+        f"stage ms {['%.3f' % x for x in bt['stage_ms']]}, dominant {names[k]} {achieved:.1f} GB/s")
     bt["batch"].close()
 
+    # HBM bytes per launch group measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this same
+    # command, scaled by the counters' reading for a 1 GiB copy (profiles/batched_traffic.json).
     traffic = None
     tp = os.path.join(ROOT, "profiles", "batched_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("stage_bytes_per_launch", [None, None, None])[k]
 
     out = {
         "metric": "body-steps/s batched+served vs plain GPU at matched accuracy; p50/p99 step latency",
@@ -214,8 +244,9 @@ def main():
                    "autotuned_blocks": {str(k): v for k, v in bt["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_world_step (all size classes of one step)",
-                     "algorithmic_bytes_per_launch": B, "launch_ms": dev_step_ms},
+                     "kernel": names[k], "algorithmic_bytes_per_launch": sb[k], "launch_ms": bt["stage_ms"][k],
+                     "stage_ms": bt["stage_ms"], "stage_algorithmic_bytes": sb,
+                     "whole_step": {"algorithmic_bytes": B, "ms": dev_step_ms, "achieved_gbs": step_gbs}},
     }
     if not args.no_plain:
         c1 = scenes.box_rain_scene(10000, seed=1001)

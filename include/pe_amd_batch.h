@@ -48,6 +48,11 @@ int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
  * Gauss-Seidel sweeps.  Drops a captured graph. */
 int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
 
+/* Runs nsteps eagerly with a hipEvent pair around every kernel launch; ms[k] is the summed
+ * kernel time of integrate + broadphase (k = 0, all size classes), narrowphase (1), solver
+ * (2, all size classes), without the gaps between launches. */
+int pe_batch_stage_times(pe_batch* b, int32_t nsteps, double* ms /*3*/);
+
 /* ------------------------------------------------------------------ server
  * Continuous batching of world-simulation requests.  A request (a scene and a horizon in
  * steps) is queued by pe_server_submit.  Each pe_server_tick admits queued requests in

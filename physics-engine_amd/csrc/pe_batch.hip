@@ -408,15 +408,47 @@ struct Class {
 };
 
 // One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class.
-int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s) {
+// ev (optional, 4 events) is recorded before K1, after K1, after K2 and after K3.
+// Per-launch timing: an event pair around every kernel of the step, tagged with its stage
+// (0 = K1 integrate + broadphase, 1 = K2 narrowphase, 2 = K3 solve), so a stage's device time is
+// the sum of its kernels' durations without the gaps between launches.
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> stage;
+    int used = 0;
+    ~LaunchTimer() { for (auto e : ev) hipEventDestroy(e); }
+    hipError_t begin(int st, hipStream_t s) {
+        if (used + 2 > (int)ev.size()) {
+            for (int k = 0; k < 2; k++) {
+                hipEvent_t e;
+                hipError_t r = hipEventCreate(&e);
+                if (r != hipSuccess) return r;
+                ev.push_back(e);
+            }
+            stage.resize(ev.size() / 2);
+        }
+        stage[used / 2] = st;
+        return hipEventRecord(ev[used], s);
+    }
+    hipError_t end(hipStream_t s) { hipError_t r = hipEventRecord(ev[used + 1], s); used += 2; return r; }
+};
+
+int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s,
+               LaunchTimer* tm = nullptr) {
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
     for (int phase = 0; phase < 2; phase++) {
         for (auto& c : classes) {
             if (c.count == 0) continue;
+            if (tm) tm->begin(phase == 0 ? 0 : 2, s);
             hipError_t e = launch_class(phase, c.block, d, dlist + c.list_off, c.count, c.nmax, s);
+            if (tm) tm->end(s);
             if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
         }
-        if (phase == 0) hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
+        if (phase == 0) {
+            if (tm) tm->begin(1, s);
+            hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
+            if (tm) tm->end(s);
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
@@ -701,6 +733,27 @@ extern "C" int pe_batch_phase_profile(pe_batch* B, double* us /*8*/) {
     for (int w = 0; w < B->nworlds; w++)
         for (int k = 1; k < PHASES; k++) us[k - 1] += (double)(h[(size_t)w * PHASES + k] - h[(size_t)w * PHASES + k - 1]) / 100.0;
     return rc;
+}
+
+// ====================================================================== per-stage times
+// Runs nsteps eagerly with hipEvents between the three launch groups of each step and returns
+// the summed device milliseconds of K1 (integrate + broadphase), K2 (narrowphase) and K3
+// (solver setup + colouring + Gauss-Seidel).
+extern "C" int pe_batch_stage_times(pe_batch* B, int32_t nsteps, double* ms /*3*/) {
+    if (!B || !ms || nsteps <= 0) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
+    ms[0] = ms[1] = ms[2] = 0;
+    LaunchTimer tm;
+    for (int s = 0; s < nsteps; s++) {
+        tm.used = 0;
+        if (int rc = launch_all(B->d, B->classes, B->dlist, B->stream, &tm)) return rc;
+        PE_HIP(hipEventSynchronize(tm.ev[tm.used - 1]));
+        for (int k = 0; k < tm.used / 2; k++) {
+            float t = 0;
+            PE_HIP(hipEventElapsedTime(&t, tm.ev[2 * k], tm.ev[2 * k + 1]));
+            ms[tm.stage[k]] += t;
+        }
+    }
+    return PE_OK;
 }
 
 // ====================================================================== timed batch steps

@@ -52,6 +52,12 @@ class Batch:
         check(lib().pe_batch_step_timed(self.h, ctypes.c_int32(n), ctypes.byref(ms)))
         return ms.value
 
+    def stage_times(self, n: int):
+        """n eager steps with events between stages: summed ms of (broadphase, narrowphase, solver)."""
+        ms = (ctypes.c_double * 3)()
+        check(lib().pe_batch_stage_times(self.h, ctypes.c_int32(n), ms))
+        return list(ms)
+
     def phase_profile(self):
         """One step with in-kernel phase timestamps: summed microseconds per phase."""
         us = (ctypes.c_double * 8)()
