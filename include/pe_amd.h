@@ -37,16 +37,31 @@ extern "C" {
 #define PE_SHAPE_CONVEX    3
 #define PE_SHAPE_CONCAVE   4
 
+/* A polygon mesh as a reference user hands it to ConvexMeshShape::setMesh or
+ * ConcaveMeshShape::setMesh (pe::Mesh, common/mesh.h: vertex positions, faces as vertex index
+ * lists with unit normals).  The library runs setMesh itself: a convex body's mesh is moved to
+ * its centre of mass and its faces are reordered by the BVH build
+ * (convex_mesh_shape.cpp:12-132); a concave body's mesh keeps its frame (concave_mesh_shape.cpp:6-17). */
+typedef struct pe_mesh_desc {
+    int32_t nv;                       /* vertices */
+    const double* vertices;           /* [3 nv] */
+    int32_t nf;                       /* faces */
+    const int32_t* face_start;        /* [nf + 1] offsets into face_index */
+    const int32_t* face_index;        /* [face_start[nf]] vertex indices, counter-clockwise seen from outside */
+    const double* face_normal;        /* [3 nf] outward unit normals */
+} pe_mesh_desc;
+
 /* Scene description (host memory, structure of arrays, FP64 so every consumer rounds the
  * same value to its own precision).  Mirrors what the reference sets on a RigidBody before
  * World::addRigidBody (src/rigid/object/rigidbody.h): setMass, setCollisionShape,
  * setTransform, setLinearVelocity, setAngularVelocity, setFrictionCoeff,
- * setRestitutionCoeff, setLinearDamping, setAngularDamping, setKinematic. */
+ * setRestitutionCoeff, setLinearDamping, setAngularDamping, setKinematic.  Bodies that name
+ * the same mesh share one shape, as RigidBody::setCollisionShape allows. */
 typedef struct pe_scene_desc {
     int32_t n;                        /* number of bodies; body i is the i-th addRigidBody */
     const int32_t* shape_type;        /* [n] PE_SHAPE_* */
     const double* shape_param;        /* [4n] box: size x,y,z (BoxShape(size)); sphere: radius;
-                                         cylinder: radius, height */
+                                         cylinder: radius, height; convex / concave: unused */
     const double* mass;               /* [n] */
     const double* position;           /* [3n] transform origin */
     const double* rotation;           /* [9n] transform basis, row-major */
@@ -57,6 +72,9 @@ typedef struct pe_scene_desc {
     const double* linear_damping;     /* [n] or NULL (0) */
     const double* angular_damping;    /* [n] or NULL (0) */
     const int32_t* kinematic;         /* [n] 0/1 */
+    int32_t n_meshes;                 /* meshes of convex / concave bodies (0 if none) */
+    const pe_mesh_desc* meshes;       /* [n_meshes] */
+    const int32_t* mesh_index;        /* [n] mesh of body i if it is convex / concave; NULL if n_meshes = 0 */
 } pe_scene_desc;
 
 /* World settings: World::setDt / setGravity (src/intf/world.h:24-25) and

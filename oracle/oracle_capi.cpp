@@ -19,16 +19,42 @@ template <class R> Handle<R>* create(const pe_scene_desc* sc, const pe_world_par
     w.gravity = V3<R>(R(prm->gravity[0]), R(prm->gravity[1]), R(prm->gravity[2]));
     w.iterations = prm->iterations;
     w.bodies.resize(sc->n);
+    // one shape per (mesh, kind), shared by the bodies that name it
+    std::vector<std::shared_ptr<Poly<R>>> convex(sc->n_meshes), concave(sc->n_meshes);
+    auto mesh_poly = [&](int m, bool is_convex) {
+        auto& slot = is_convex ? convex[m] : concave[m];
+        if (!slot) {
+            slot = std::make_shared<Poly<R>>();
+            const pe_mesh_desc& md = sc->meshes[m];
+            for (int v = 0; v < md.nv; v++)
+                slot->v.push_back(V3<R>(R(md.vertices[3 * v]), R(md.vertices[3 * v + 1]), R(md.vertices[3 * v + 2])));
+            for (int f = 0; f < md.nf; f++) {
+                std::vector<int> idx(md.face_index + md.face_start[f], md.face_index + md.face_start[f + 1]);
+                slot->f.push_back(idx);
+                slot->fn.push_back(V3<R>(R(md.face_normal[3 * f]), R(md.face_normal[3 * f + 1]), R(md.face_normal[3 * f + 2])));
+            }
+            if (is_convex) convex_set_mesh(*slot);
+            else concave_set_mesh(*slot);
+            w.polys.push_back(slot);
+        }
+        return slot.get();
+    };
     for (int i = 0; i < sc->n; i++) {
         Body<R>& b = w.bodies[i];
         b.type = sc->shape_type[i];
         const double* sp = sc->shape_param + 4 * i;
         if (b.type == ST_BOX) {
             for (int k = 0; k < 3; k++) { b.size[k] = R(sp[k]); b.half[k] = b.size[k] / R(2); }
+            w.polys.push_back(std::make_shared<Poly<R>>(box_poly(b)));
+            b.poly = w.polys.back().get();
         } else if (b.type == ST_SPHERE) {
             b.radius = R(sp[0]);
         } else if (b.type == ST_CYLINDER) {
             b.radius = R(sp[0]); b.height = R(sp[1]);
+            w.polys.push_back(std::make_shared<Poly<R>>(cylinder_poly(b)));
+            b.poly = w.polys.back().get();
+        } else {
+            b.poly = mesh_poly(sc->mesh_index[i], b.type == ST_CONVEX);
         }
         for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) b.xf.basis(r, c) = R(sc->rotation[9 * i + 3 * r + c]);
         for (int k = 0; k < 3; k++) {
@@ -85,6 +111,28 @@ template <class R> void get_contacts(Handle<R>* h, int32_t* npts, int32_t* swap,
             o[12] = cp.dist;
         }
     }
+}
+// Polyhedral data of a body after shape set-up: counts[4] = vertices, faces, face indices,
+// unique edges; then vertex positions, face starts / indices / normals, edges (6 per edge) and
+// volume + unit-mass inertia (10).  Any output pointer may be null.
+template <class R> int get_poly(Handle<R>* h, int body, int32_t* counts, double* v, int32_t* fstart, int32_t* fidx,
+                                double* fn, double* e, double* vi) {
+    const Poly<R>* p = h->w.bodies[body].poly;
+    if (!p) return -1;
+    int nidx = 0;
+    for (auto& f : p->f) nidx += (int)f.size();
+    if (counts) { counts[0] = (int)p->v.size(); counts[1] = (int)p->f.size(); counts[2] = nidx; counts[3] = (int)p->e.size(); }
+    if (v) for (size_t i = 0; i < p->v.size(); i++) for (int k = 0; k < 3; k++) v[3 * i + k] = p->v[i][k];
+    int o = 0;
+    for (size_t f = 0; f < p->f.size(); f++) {
+        if (fstart) fstart[f] = o;
+        for (int x : p->f[f]) { if (fidx) fidx[o] = x; o++; }
+        if (fn) for (int k = 0; k < 3; k++) fn[3 * f + k] = p->fn[f][k];
+    }
+    if (fstart) fstart[p->f.size()] = o;
+    if (e) for (size_t i = 0; i < p->e.size(); i++) for (int k = 0; k < 3; k++) { e[6 * i + k] = p->e[i].first[k]; e[6 * i + 3 + k] = p->e[i].second[k]; }
+    if (vi) { vi[0] = p->volume; for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) vi[1 + 3 * r + c] = p->inertia(r, c); }
+    return 0;
 }
 }  // namespace
 
@@ -151,6 +199,10 @@ template <class R> void get_contacts(Handle<R>* h, int32_t* npts, int32_t* swap,
     extern "C" int P##_num_results(void* h) { return (int)((Handle<R>*)h)->w.results.size(); }          \
     extern "C" void P##_get_contacts(void* h, int32_t* npts, int32_t* swap, double* pts) {               \
         get_contacts((Handle<R>*)h, npts, swap, pts);                                                    \
+    }                                                                                                    \
+    extern "C" int P##_get_poly(void* h, int body, int32_t* counts, double* v, int32_t* fstart, int32_t* fidx, \
+                                double* fn, double* e, double* vi) {                                     \
+        return get_poly((Handle<R>*)h, body, counts, v, fstart, fidx, fn, e, vi);                        \
     }                                                                                                    \
     extern "C" int P##_num_colors(void* h) { return ((Handle<R>*)h)->w.ncolors; }                       \
     extern "C" int P##_get_solve_order(void* h, int32_t* order) {                                        \

@@ -26,6 +26,7 @@
 #include <vector>
 #include <algorithm>
 #include <utility>
+#include <memory>
 
 namespace pe_oracle {
 
@@ -194,6 +195,7 @@ template <class R> struct Xf {
 };
 
 // ---------------------------------------------------------------- bodies
+template <class R> struct Poly;
 template <class R> struct Body {
     int type = ST_BOX;
     R size[3] = {0, 0, 0};        // box full size
@@ -207,8 +209,11 @@ template <class R> struct Body {
     V3<R> aabb_min, aabb_max;
     R mu = R(0.5), e = R(0.5), lin_damp = 0, ang_damp = 0;
     bool kinematic = false, ignore = false;
+    const Poly<R>* poly = nullptr;   // mesh of a box / cylinder / convex / concave body
 };
 
+template <class R> M3<R> poly_inertia(const Poly<R>& p);
+template <class R> void poly_aabb(const Poly<R>& p, const M3<R>& rot, const V3<R>& pos, V3<R>& mn, V3<R>& mx);
 // shape local inertia at unit mass: box_shape.cpp:16-22, sphere_shape.cpp:5-12, cylinder_shape.cpp:23-31
 template <class R> inline M3<R> shape_inertia(const Body<R>& b) {
     M3<R> I;
@@ -222,6 +227,8 @@ template <class R> inline M3<R> shape_inertia(const Body<R>& b) {
         const R r_2 = b.radius * b.radius, h_2 = b.height * b.height;
         const R axis = r_2 * R(0.5), diag = r_2 / R(4.0) + h_2 / R(12.0);
         I(0, 0) = diag; I(1, 1) = axis; I(2, 2) = diag;
+    } else {
+        I = poly_inertia(*b.poly);   // convex_mesh_shape.cpp:93-100 / concave_mesh_shape.cpp:9-10
     }
     return I;
 }
@@ -273,6 +280,8 @@ template <class R> inline void body_aabb(Body<R>& b) {
         ext[0] += b.radius * std::sqrt(axis[1] * axis[1] + axis[2] * axis[2]);
         ext[2] += b.radius * std::sqrt(axis[0] * axis[0] + axis[1] * axis[1]);
         b.aabb_min = pos - ext; b.aabb_max = pos + ext;
+    } else {  // convex_mesh_shape.cpp:138-150 (the concave mesh inherits it)
+        poly_aabb(*b.poly, rot, pos, b.aabb_min, b.aabb_max);
     }
 }
 
@@ -358,6 +367,7 @@ template <class R> struct World {
     bool use_color_order = true;
     int ncolors = 0;
     std::vector<int> color_of;                // per result
+    std::vector<std::shared_ptr<Poly<R>>> polys;   // meshes the bodies point at
 };
 
 // ContactResult::addContactPoint (contact_result.cpp:88-142)
@@ -781,6 +791,10 @@ template <class R> inline bool sphere_sphere(const World<R>& w, ContactResult<R>
     return true;
 }
 
+}  // namespace pe_oracle
+#include "pe_oracle_convex.hpp"
+namespace pe_oracle {
+
 // ---------------------------------------------------------------- stages
 // World::updateObjectStatus + applyExternalForce (world.cpp:35-127).  Sleep thresholds are
 // zero in every supported scene, so a body never sleeps; the sleep bookkeeping is omitted.
@@ -848,10 +862,47 @@ template <class R> inline void stage_narrowphase(World<R>& w) {
         cr.mu = std::sqrt(A.mu * B.mu);
         cr.e = std::sqrt(A.e * B.e);
         const int ta = A.type, tb = B.type;
+        const R ref_scale = (body_aabb_scale(A) + body_aabb_scale(B)) * R(0.05);
+        // NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) and each algorithm's
+        // processCollision: which body plays which role, and the swap flag it sets.
+        auto is = [&](int x, int y) { return (ta == x && tb == y) || (ta == y && tb == x); };
+        auto pick = [&](int t) -> const Body<R>& { return ta == t ? A : B; };
+        auto other = [&](int t) -> const Body<R>& { return ta == t ? B : A; };
+        auto part = [](const Body<R>& b) { return Part<R>{b.type == ST_CONCAVE ? ST_CONVEX : b.type, &b, b.poly}; };
         if (ta == ST_BOX && tb == ST_BOX) box_box(w, cr, A, B);
-        else if ((ta == ST_BOX && tb == ST_SPHERE) || (ta == ST_SPHERE && tb == ST_BOX)) box_sphere(w, cr, A, B);
+        else if (is(ST_BOX, ST_SPHERE)) box_sphere(w, cr, A, B);
         else if (ta == ST_SPHERE && tb == ST_SPHERE) sphere_sphere(w, cr, A, B);
-        else continue;  // other shape pairs are not restated yet
+        else if (is(ST_BOX, ST_CYLINDER)) {            // box_cylinder_collision_algorithm.cpp:13-47
+            cr.swap = ta == ST_BOX;
+            box_cylinder(w, cr, pick(ST_BOX), pick(ST_CYLINDER));
+            cr.swap = false;
+        } else if (is(ST_SPHERE, ST_CYLINDER)) {       // sphere_cylinder_collision_algorithm.cpp:8-30
+            cr.swap = ta == ST_SPHERE;
+            sphere_cylinder(w, cr, pick(ST_SPHERE), pick(ST_CYLINDER));
+            cr.swap = false;
+        } else if ((ta == ST_CYLINDER && tb == ST_CYLINDER) || is(ST_BOX, ST_CONVEX) || (ta == ST_CONVEX && tb == ST_CONVEX)) {
+            // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
+            convex_convex(w, cr, part(A), part(B), A.xf, B.xf, ref_scale);
+        } else if (is(ST_SPHERE, ST_CONVEX)) {         // sphere_convex_collision_algorithm.cpp:11-34
+            cr.swap = ta == ST_CONVEX;
+            sphere_mesh(w, cr, pick(ST_SPHERE), pick(ST_CONVEX), *pick(ST_CONVEX).poly);
+            cr.swap = false;
+        } else if (is(ST_CYLINDER, ST_CONVEX)) {       // cylinder_convex_collision_algorithm.cpp:11-51
+            cr.swap = ta == ST_CONVEX;
+            cylinder_mesh(w, cr, pick(ST_CONVEX), *pick(ST_CONVEX).poly, pick(ST_CYLINDER));
+            cr.swap = false;
+        } else if ((ta == ST_CONCAVE) != (tb == ST_CONCAVE)) {
+            // concave_{box,sphere,cylinder,convex}_collision_algorithm.cpp: the other body is A
+            const Body<R>& mb = pick(ST_CONCAVE);
+            const Body<R>& ob = other(ST_CONCAVE);
+            cr.swap = ta == ST_CONCAVE;
+            if (ob.type == ST_SPHERE) sphere_mesh(w, cr, ob, mb, *mb.poly);
+            else if (ob.type == ST_CYLINDER) cylinder_mesh(w, cr, mb, *mb.poly, ob);
+            else concave_poly(w, cr, mb, *mb.poly, part(ob), ob, ref_scale);
+            cr.swap = false;
+        } else {
+            continue;  // concave-concave: no algorithm, the result stays cleared
+        }
         sort_contact_points(cr);
     }
 }
@@ -1037,5 +1088,11 @@ template <class R> inline void world_step(World<R>& w, Solver<R>& s) {
 //   stage_solver_setup / solve  pinned: vs compiled reference in pair order (use_color_order=0)
 //   compute_color_order ....... pinned: invariants (no dynamic body shared within a colour; every
 //                               colour maximal) and brute-force check on small graphs (tests/)
+//   pe_oracle_convex.hpp ...... pinned: box / cylinder meshes, convex and concave setMesh (volume,
+//                               inertia, centring, BVH face order, unique edges) equal to the compiled
+//                               reference's shapes; every shape pair type (box, sphere, cylinder,
+//                               convex hull, heightfield) bit-exact per stage vs the compiled reference
+//                               in FP32 and FP64 (test_oracle_vs_ref), and vs golden fixtures
+//                               (test_golden)
 
 }  // namespace pe_oracle

@@ -132,6 +132,22 @@ class _Base:
         return npts, swap, pts
 
 
+    def poly(self, body):
+        """Polyhedral data of a body's shape after set-up, or None for a sphere:
+        dict(v [nv,3], faces [list], fn [nf,3], edges [ne,2,3], volume, inertia [3,3])."""
+        f = self._f("get_poly")
+        f.restype = ctypes.c_int
+        cnt = np.zeros(4, np.int32)
+        if f(ctypes.c_void_p(self.h), ctypes.c_int(body), _i(cnt), None, None, None, None, None, None) != 0:
+            return None
+        nv, nf, ni, ne = (int(x) for x in cnt)
+        v, st, ix = np.zeros((nv, 3)), np.zeros(nf + 1, np.int32), np.zeros(max(ni, 1), np.int32)
+        fn, e, vi = np.zeros((nf, 3)), np.zeros((max(ne, 1), 2, 3)), np.zeros(10)
+        f(ctypes.c_void_p(self.h), ctypes.c_int(body), _i(cnt), _d(v), _i(st), _i(ix), _d(fn), _d(e), _d(vi))
+        faces = [ix[st[k]:st[k + 1]].tolist() for k in range(nf)]
+        return dict(v=v, faces=faces, fn=fn, edges=e[:ne], volume=vi[0], inertia=vi[1:].reshape(3, 3))
+
+
 class OracleWorld(_Base):
     """The C++ restatement (oracle/pe_oracle.hpp) in FP32 (bits=32) or FP64 (bits=64)."""
 

@@ -10,6 +10,8 @@
 #include "rigid/shape/box_shape.h"
 #include "rigid/shape/sphere_shape.h"
 #include "rigid/shape/cylinder_shape.h"
+#include "rigid/shape/convex_mesh_shape.h"
+#include "rigid/shape/concave_mesh_shape.h"
 #include "rigid/collision/broad_phase/broad_phase_sweep_and_prune.h"
 #include "rigid/constraint/constraint_solver/sequential_impulse_solver.h"
 // The pool's members are private; this test driver needs them to retire the workers at exit.
@@ -55,6 +57,24 @@ struct RefWorld : public pe_intf::World {
     pe::Array<pe_phys_collision::CollisionPair>& pairs() { return _collision_pairs; }
     pe::Array<pe_phys_collision::ContactResult*>& results() { return _contact_results; }
 };
+
+// pe::Mesh of a pe_mesh_desc (vertex normals are not used by the rigid pipeline: zero)
+pe::Mesh mesh_of(const pe_mesh_desc& md) {
+    pe::Mesh m;
+    for (int v = 0; v < md.nv; v++) {
+        pe::Mesh::Vertex x;
+        x.position = pe::Vector3(pe::Real(md.vertices[3 * v]), pe::Real(md.vertices[3 * v + 1]), pe::Real(md.vertices[3 * v + 2]));
+        x.normal = pe::Vector3::Zero();
+        m.vertices.push_back(x);
+    }
+    for (int f = 0; f < md.nf; f++) {
+        pe::Mesh::Face face;
+        for (int k = md.face_start[f]; k < md.face_start[f + 1]; k++) face.indices.push_back((uint32_t)md.face_index[k]);
+        face.normal = pe::Vector3(pe::Real(md.face_normal[3 * f]), pe::Real(md.face_normal[3 * f + 1]), pe::Real(md.face_normal[3 * f + 2]));
+        m.faces.push_back(face);
+    }
+    return m;
+}
 
 pe::Matrix3 basis_of(const double* r) {
     pe::Matrix3 m;
@@ -113,6 +133,8 @@ REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* 
     w->setConstraintSolver(solver);
     w->sap = new SAP();
     w->setBroadPhase(w->sap);
+    // one shape per (mesh, kind), shared by the bodies that name it
+    std::vector<pe_phys_shape::Shape*> convex(sc->n_meshes, nullptr), concave(sc->n_meshes, nullptr);
     for (int i = 0; i < sc->n; i++) {
         auto* rb = new pe_phys_object::RigidBody();
         rb->setMass(pe::Real(sc->mass[i]));
@@ -124,6 +146,26 @@ REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* 
                 break;
             case PE_SHAPE_SPHERE: shape = new pe_phys_shape::SphereShape(pe::Real(sp[0])); break;
             case PE_SHAPE_CYLINDER: shape = new pe_phys_shape::CylinderShape(pe::Real(sp[0]), pe::Real(sp[1])); break;
+            case PE_SHAPE_CONVEX: {
+                const int m = sc->mesh_index[i];
+                if (!convex[m]) {
+                    auto* cs = new pe_phys_shape::ConvexMeshShape();
+                    cs->setMesh(mesh_of(sc->meshes[m]));
+                    convex[m] = cs;
+                }
+                shape = convex[m];
+                break;
+            }
+            case PE_SHAPE_CONCAVE: {
+                const int m = sc->mesh_index[i];
+                if (!concave[m]) {
+                    auto* cs = new pe_phys_shape::ConcaveMeshShape();
+                    cs->setMesh(mesh_of(sc->meshes[m]));
+                    concave[m] = cs;
+                }
+                shape = concave[m];
+                break;
+            }
             default: delete rb; delete w; return nullptr;
         }
         rb->setCollisionShape(shape);
@@ -243,4 +285,50 @@ REF_API void pref_get_contacts(void* h, int32_t* npts, int32_t* swap, double* pt
             o[12] = cp.getDistance();
         }
     }
+}
+
+// Polyhedral data of body i's shape after its constructor / setMesh (same layout as the
+// oracle's *_get_poly): counts[4] = vertices, faces, face indices, unique edges.
+REF_API int pref_get_poly(void* h, int body, int32_t* counts, double* v, int32_t* fstart, int32_t* fidx, double* fn,
+                          double* e, double* vi) {
+    auto* w = (RefWorld*)h;
+    auto* shape = w->owned[body]->getCollisionShape();
+    const pe::Mesh* m = nullptr;
+    const pe::Array<pe::KV<pe::Vector3, pe::Vector3>>* ue = nullptr;
+    static const pe::Array<pe::KV<pe::Vector3, pe::Vector3>> none;
+    switch (shape->getType()) {
+        case pe_phys_shape::ShapeType::ST_Box: {
+            auto* s = dynamic_cast<pe_phys_shape::BoxShape*>(shape);
+            m = &s->getMesh(); ue = &s->getUniqueEdges(); break;
+        }
+        case pe_phys_shape::ShapeType::ST_Cylinder: {
+            auto* s = dynamic_cast<pe_phys_shape::CylinderShape*>(shape);
+            m = &s->getMesh(); ue = &s->getUniqueEdges(); break;
+        }
+        case pe_phys_shape::ShapeType::ST_ConvexMesh:
+        case pe_phys_shape::ShapeType::ST_ConcaveMesh: {
+            auto* s = dynamic_cast<pe_phys_shape::ConvexMeshShape*>(shape);
+            m = &s->getMesh();
+            ue = shape->getType() == pe_phys_shape::ShapeType::ST_ConvexMesh ? &s->getUniqueEdges() : &none;
+            break;
+        }
+        default: return -1;
+    }
+    int nidx = 0;
+    for (auto& f : m->faces) nidx += (int)f.indices.size();
+    if (counts) { counts[0] = (int)m->vertices.size(); counts[1] = (int)m->faces.size(); counts[2] = nidx; counts[3] = (int)ue->size(); }
+    if (v) for (size_t i = 0; i < m->vertices.size(); i++) for (int k = 0; k < 3; k++) v[3 * i + k] = m->vertices[i].position[k];
+    int o = 0;
+    for (size_t f = 0; f < m->faces.size(); f++) {
+        if (fstart) fstart[f] = o;
+        for (auto x : m->faces[f].indices) { if (fidx) fidx[o] = (int32_t)x; o++; }
+        if (fn) for (int k = 0; k < 3; k++) fn[3 * f + k] = m->faces[f].normal[k];
+    }
+    if (fstart) fstart[m->faces.size()] = o;
+    if (e) for (size_t i = 0; i < ue->size(); i++) for (int k = 0; k < 3; k++) { e[6 * i + k] = (*ue)[i].first[k]; e[6 * i + 3 + k] = (*ue)[i].second[k]; }
+    if (vi) {
+        vi[0] = shape->getVolume();
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) vi[1 + 3 * r + c] = shape->getLocalInertia()(r, c);
+    }
+    return 0;
 }

@@ -11,7 +11,9 @@ Generators (numpy PCG64, seed = 1000 + workload index unless given):
   box_rain_scene   -- N boxes with U(0.2, 1.0) m edges and uniform random orientation
                       dropped onto a kinematic ground box
   batch_worlds     -- many independent worlds of log-uniform body counts, boxes and spheres
-  mixed_scene      -- boxes / spheres / cylinders mix over a ground box
+  mixed_world      -- boxes / spheres / cylinders mix over a ground box
+  shape_mix_scene  -- boxes, spheres, cylinders and convex hulls (hulls of 32 Gaussian points)
+                      over a triangle-mesh heightfield: every narrowphase pair type
 """
 from __future__ import annotations
 
@@ -22,6 +24,33 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_CONVEX, SHAPE_CONCAVE = 0, 1, 2, 3, 4
+
+
+class MeshDesc(ctypes.Structure):
+    """ctypes mirror of pe_mesh_desc."""
+    _fields_ = [
+        ("nv", ctypes.c_int32),
+        ("vertices", ctypes.POINTER(ctypes.c_double)),
+        ("nf", ctypes.c_int32),
+        ("face_start", ctypes.POINTER(ctypes.c_int32)),
+        ("face_index", ctypes.POINTER(ctypes.c_int32)),
+        ("face_normal", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+@dataclass
+class Mesh:
+    """A polygon mesh as handed to ConvexMeshShape::setMesh / ConcaveMeshShape::setMesh."""
+    vertices: np.ndarray     # f64 [nv, 3]
+    faces: list              # list of int vertex-index lists (counter-clockwise seen from outside)
+    normals: np.ndarray      # f64 [nf, 3] outward unit normals
+
+    def flat(self):
+        start = np.zeros(len(self.faces) + 1, np.int32)
+        start[1:] = np.cumsum([len(f) for f in self.faces])
+        index = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in self.faces]), np.int32)
+        return (np.ascontiguousarray(self.vertices, np.float64), start, index,
+                np.ascontiguousarray(self.normals, np.float64))
 
 
 class SceneDesc(ctypes.Structure):
@@ -40,6 +69,9 @@ class SceneDesc(ctypes.Structure):
         ("linear_damping", ctypes.POINTER(ctypes.c_double)),
         ("angular_damping", ctypes.POINTER(ctypes.c_double)),
         ("kinematic", ctypes.POINTER(ctypes.c_int32)),
+        ("n_meshes", ctypes.c_int32),
+        ("meshes", ctypes.POINTER(MeshDesc)),
+        ("mesh_index", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -74,6 +106,8 @@ class Scene:
     gravity: tuple = (0.0, -9.8, 0.0)
     iterations: int = 10
     name: str = ""
+    meshes: list = None              # [Mesh] of the convex / concave bodies
+    mesh_index: np.ndarray = None    # int32 [n] mesh of each convex / concave body (-1 otherwise)
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -95,6 +129,11 @@ class Scene:
             self.linear_damping = np.zeros(self.n)
         if self.angular_damping is None:
             self.angular_damping = np.zeros(self.n)
+        if self.meshes is None:
+            self.meshes = []
+        if self.mesh_index is None:
+            self.mesh_index = np.full(self.n, -1, np.int32)
+        self.mesh_index = np.ascontiguousarray(self.mesh_index, dtype=np.int32)
         return self
 
     def desc(self) -> SceneDesc:
@@ -104,19 +143,30 @@ class Scene:
                       _dptr(self.position), _dptr(self.rotation), _dptr(self.linear_velocity),
                       _dptr(self.angular_velocity), _dptr(self.friction), _dptr(self.restitution),
                       _dptr(self.linear_damping), _dptr(self.angular_damping), _iptr(self.kinematic))
+        if self.meshes:
+            flats = [m.flat() for m in self.meshes]
+            arr = (MeshDesc * len(flats))()
+            for k, (v, st, ix, fn) in enumerate(flats):
+                arr[k] = MeshDesc(v.shape[0], _dptr(v), len(st) - 1, _iptr(st), _iptr(ix), _dptr(fn))
+            self._keep[:] = [flats, arr]
+            d.n_meshes = len(flats)
+            d.meshes = arr
+            d.mesh_index = _iptr(self.mesh_index)
         return d
 
     def params(self) -> WorldParams:
         return WorldParams(self.dt, (ctypes.c_double * 3)(*self.gravity), self.iterations)
 
     def subset(self, idx) -> "Scene":
+        self.contiguous()
         idx = np.asarray(idx)
         return Scene(self.shape_type[idx], self.shape_param[idx], self.mass[idx], self.position[idx],
                      self.rotation[idx], self.linear_velocity[idx], self.angular_velocity[idx],
                      self.friction[idx], self.restitution[idx], self.kinematic[idx],
                      None if self.linear_damping is None else self.linear_damping[idx],
                      None if self.angular_damping is None else self.angular_damping[idx],
-                     self.dt, self.gravity, self.iterations, self.name).contiguous()
+                     self.dt, self.gravity, self.iterations, self.name,
+                     list(self.meshes), self.mesh_index[idx]).contiguous()
 
     def footprint_bodies(self) -> int:
         return self.n
@@ -124,12 +174,18 @@ class Scene:
 
 def concat(scenes, name="") -> Scene:
     """Concatenates bodies of several scenes into one (same dt / gravity / iterations)."""
+    scenes = [s.contiguous() for s in scenes]
     s0 = scenes[0]
     cat = lambda f: np.concatenate([getattr(s, f) for s in scenes])
+    meshes, mi, off = [], [], 0
+    for s in scenes:
+        meshes += s.meshes
+        mi.append(np.where(s.mesh_index >= 0, s.mesh_index + off, -1))
+        off += len(s.meshes)
     return Scene(cat("shape_type"), cat("shape_param"), cat("mass"), cat("position"), cat("rotation"),
                  cat("linear_velocity"), cat("angular_velocity"), cat("friction"), cat("restitution"),
                  cat("kinematic"), cat("linear_damping"), cat("angular_damping"),
-                 s0.dt, s0.gravity, s0.iterations, name).contiguous()
+                 s0.dt, s0.gravity, s0.iterations, name, meshes, np.concatenate(mi)).contiguous()
 
 
 # ------------------------------------------------------------------ rotations
@@ -169,13 +225,18 @@ class SceneBuilder:
 
     def __init__(self):
         self.rows = []
+        self.meshes = []
 
-    def add(self, shape, param, mass, pos, rot=None, v=(0, 0, 0), w=(0, 0, 0), mu=0.5, e=0.0, kin=0):
+    def add_mesh(self, mesh: "Mesh") -> int:
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add(self, shape, param, mass, pos, rot=None, v=(0, 0, 0), w=(0, 0, 0), mu=0.5, e=0.0, kin=0, mesh=-1):
         p = np.zeros(4)
         p[:len(param)] = param
         self.rows.append((shape, p, mass, np.asarray(pos, float),
                           np.eye(3) if rot is None else np.asarray(rot, float),
-                          np.asarray(v, float), np.asarray(w, float), mu, e, kin))
+                          np.asarray(v, float), np.asarray(w, float), mu, e, kin, mesh))
 
     def scene(self, **kw) -> Scene:
         r = self.rows
@@ -184,6 +245,7 @@ class SceneBuilder:
                      np.stack([x[4] for x in r]), np.stack([x[5] for x in r]),
                      np.stack([x[6] for x in r]), np.array([x[7] for x in r], float),
                      np.array([x[8] for x in r], float), np.array([x[9] for x in r], np.int32),
+                     meshes=list(self.meshes), mesh_index=np.array([x[10] for x in r], np.int32),
                      **kw).contiguous()
 
 
@@ -348,3 +410,106 @@ def batch_worlds(n_worlds: int = 4096, lo: int = 8, hi: int = 512, seed: int = 1
     rng = np.random.default_rng(seed)
     counts = log_uniform_counts(rng, n_worlds, lo, hi)
     return [grid_world(int(c), world_seed(seed, w)) for w, c in enumerate(counts)]
+
+
+# ------------------------------------------------------------------ meshes
+def hull_mesh(points: np.ndarray) -> "Mesh":
+    """Convex hull of a point cloud as a triangle mesh (qhull), faces counter-clockwise seen
+    from outside, normals the normalised face cross products."""
+    from scipy.spatial import ConvexHull
+    h = ConvexHull(points)
+    used = np.unique(h.simplices)
+    remap = -np.ones(points.shape[0], np.int64)
+    remap[used] = np.arange(used.size)
+    verts = points[used]
+    centre = verts.mean(axis=0)
+    faces, normals = [], []
+    for tri in h.simplices:
+        a, b, c = (int(remap[t]) for t in tri)
+        n = np.cross(verts[b] - verts[a], verts[c] - verts[a])
+        if n @ (verts[a] - centre) < 0:
+            b, c = c, b
+            n = -n
+        faces.append([a, b, c])
+        normals.append(n / np.linalg.norm(n))
+    return Mesh(verts, faces, np.array(normals))
+
+
+def gaussian_hull(rng: np.random.Generator, npoints: int = 32, radius: float = 0.5) -> "Mesh":
+    """Hull of npoints standard-normal points scaled so the farthest lies at `radius`."""
+    p = rng.standard_normal((npoints, 3))
+    p *= radius / np.linalg.norm(p, axis=1).max()
+    return hull_mesh(p)
+
+
+def heightfield_mesh(rng: np.random.Generator, nx: int = 64, nz: int = 64, size=(100.0, 100.0),
+                     amplitude: float = 1.0) -> "Mesh":
+    """nx x nz vertex grid over size[0] x size[1] m centred on the origin, heights U(-a, a);
+    two triangles per cell, normals pointing up (+y)."""
+    xs = np.linspace(-size[0] / 2.0, size[0] / 2.0, nx)
+    zs = np.linspace(-size[1] / 2.0, size[1] / 2.0, nz)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    Y = rng.uniform(-amplitude, amplitude, X.shape)
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    vid = lambda i, k: i * nz + k
+    faces, normals = [], []
+    for i in range(nx - 1):
+        for k in range(nz - 1):
+            for tri in ((vid(i, k), vid(i, k + 1), vid(i + 1, k + 1)), (vid(i, k), vid(i + 1, k + 1), vid(i + 1, k))):
+                a, b, c = tri
+                n = np.cross(verts[b] - verts[a], verts[c] - verts[a])
+                if n[1] < 0:
+                    b, c = c, b
+                    n = -n
+                faces.append([a, b, c])
+                normals.append(n / np.linalg.norm(n))
+    return Mesh(verts, faces, np.array(normals))
+
+
+def shape_mix_scene(n: int = 50000, seed: int = 1002, volume=(100.0, 60.0, 100.0), grid: int = 64,
+                    ground_size=None, fractions=(0.40, 0.25, 0.15, 0.20), density: float = 1000.0,
+                    hull_points: int = 32, mu: float = 0.5) -> Scene:
+    """Mixed-shape world: boxes (edges U(0.2, 1.0) m), spheres (r U(0.1, 0.6) m), cylinders
+    (r U(0.1, 0.4) m, h U(0.2, 1.0) m) and convex hulls (hull of `hull_points` Gaussian points
+    scaled to 0.5 m, one mesh each) in the given fractions, all with uniform random orientation
+    (spheres unrotated), over a kinematic grid x grid heightfield with +-1 m uniform noise.
+    Centres are jittered in the cells of a 1.8 m lattice filling the volume from 2 m above the
+    ground, which keeps bounding spheres 0.05 m apart without a rejection loop."""
+    rng = np.random.default_rng(seed)
+    gx, gz = ground_size if ground_size is not None else (volume[0], volume[2])
+    b = SceneBuilder()
+    hf = b.add_mesh(heightfield_mesh(rng, grid, grid, (gx, gz), 1.0))
+    b.add(SHAPE_CONCAVE, (), 1.0, (0.0, 0.0, 0.0), kin=1, mesh=hf, mu=mu)
+    cell = 1.8
+    nx, nz = max(1, int(volume[0] // cell)), max(1, int(volume[2] // cell))
+    ny = int(math.ceil(n / (nx * nz))) + 1
+    cells = rng.permutation(nx * ny * nz)[:n]
+    cx, rest = cells % nx, cells // nx
+    cz, cy = rest % nz, rest // nz
+    u = rng.random(n)
+    f0, f1, f2, _ = np.cumsum(fractions)
+    kind = np.where(u < f0, SHAPE_BOX, np.where(u < f1, SHAPE_SPHERE, np.where(u < f2, SHAPE_CYLINDER, SHAPE_CONVEX)))
+    size = rng.uniform(0.2, 1.0, (n, 3))
+    rad = rng.uniform(0.1, 0.6, n)
+    cr = rng.uniform(0.1, 0.4, n)
+    ch = rng.uniform(0.2, 1.0, n)
+    rot = quat_to_matrix(random_quaternions(rng, n))
+    bound = np.select([kind == SHAPE_BOX, kind == SHAPE_SPHERE, kind == SHAPE_CYLINDER],
+                      [0.5 * np.linalg.norm(size, axis=1), rad, np.sqrt(cr ** 2 + 0.25 * ch ** 2)], 0.5)
+    slack = np.maximum(cell / 2.0 - bound - 0.025, 0.0)
+    jit = rng.uniform(-1.0, 1.0, (n, 3)) * slack[:, None]
+    pos = np.stack([(cx + 0.5) * cell - nx * cell / 2.0, 2.0 + (cy + 0.5) * cell,
+                    (cz + 0.5) * cell - nz * cell / 2.0], axis=1) + jit
+    for i in range(n):
+        k = kind[i]
+        if k == SHAPE_BOX:
+            b.add(SHAPE_BOX, size[i], density * size[i].prod(), pos[i], rot[i], mu=mu)
+        elif k == SHAPE_SPHERE:
+            b.add(SHAPE_SPHERE, (rad[i],), density * 4.0 / 3.0 * math.pi * rad[i] ** 3, pos[i], mu=mu)
+        elif k == SHAPE_CYLINDER:
+            b.add(SHAPE_CYLINDER, (cr[i], ch[i]), density * math.pi * cr[i] ** 2 * ch[i], pos[i], rot[i], mu=mu)
+        else:
+            m = gaussian_hull(rng, hull_points, 0.5)
+            vol = abs(sum(np.dot(m.vertices[f[0]], np.cross(m.vertices[f[1]], m.vertices[f[2]])) for f in m.faces)) / 6.0
+            b.add(SHAPE_CONVEX, (), density * vol, pos[i], rot[i], mu=mu, mesh=b.add_mesh(m))
+    return b.scene(name=f"shape_mix{n}")

@@ -23,6 +23,14 @@ def record(name, sc, bits, steps, keep, contacts_at):
     out = {k: getattr(sc, k) for k in ("shape_type", "shape_param", "mass", "position", "rotation",
                                        "linear_velocity", "angular_velocity", "friction", "restitution", "kinematic")}
     out["params"] = np.array([sc.dt, *sc.gravity, sc.iterations], float)
+    if sc.meshes:
+        out["mesh_index"] = sc.mesh_index
+        out["mesh_nv"] = np.array([m.vertices.shape[0] for m in sc.meshes], np.int32)
+        out["mesh_nf"] = np.array([len(m.faces) for m in sc.meshes], np.int32)
+        out["mesh_vertices"] = np.concatenate([m.vertices for m in sc.meshes])
+        out["mesh_face_len"] = np.array([len(f) for m in sc.meshes for f in m.faces], np.int32)
+        out["mesh_face_index"] = np.array([i for m in sc.meshes for f in m.faces for i in f], np.int32)
+        out["mesh_normals"] = np.concatenate([m.normals for m in sc.meshes])
     pair_lists = []
     for s in range(1, steps + 1):
         r.integrate()
@@ -51,3 +59,6 @@ if __name__ == "__main__":
     record("tower", scenes.tower_scene(), 32, 30, (1, 10, 30), 10)
     record("tower", scenes.tower_scene(), 64, 30, (1, 10, 30), 10)
     record("rain120", small_rain(120, seed=31), 32, 90, (1, 30, 90), 60)
+    mix = scenes.shape_mix_scene(200, seed=5, volume=(12, 8, 12), grid=16, ground_size=(16, 16))
+    record("shape_mix200", mix, 32, 120, (1, 60, 120), 100)
+    record("shape_mix200", mix, 64, 120, (1, 60, 120), 100)

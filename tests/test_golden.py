@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from pe_amd.scenes import Scene
+from pe_amd.scenes import Mesh, Scene
 from pyoracle import OracleWorld
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -16,13 +16,27 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def load(name):
     z = np.load(os.path.join(HERE, name))
     p = z["params"]
+    meshes, mesh_index = None, None
+    if "mesh_nv" in z:
+        meshes, vo, fo, io = [], 0, 0, 0
+        flen = z["mesh_face_len"]
+        for nv, nf in zip(z["mesh_nv"], z["mesh_nf"]):
+            faces = []
+            for k in range(fo, fo + nf):
+                faces.append(z["mesh_face_index"][io:io + flen[k]].tolist())
+                io += flen[k]
+            meshes.append(Mesh(z["mesh_vertices"][vo:vo + nv], faces, z["mesh_normals"][fo:fo + nf]))
+            vo += nv
+            fo += nf
+        mesh_index = z["mesh_index"]
     sc = Scene(z["shape_type"], z["shape_param"], z["mass"], z["position"], z["rotation"], z["linear_velocity"],
                z["angular_velocity"], z["friction"], z["restitution"], z["kinematic"], dt=float(p[0]),
-               gravity=tuple(p[1:4]), iterations=int(p[4])).contiguous()
+               gravity=tuple(p[1:4]), iterations=int(p[4]), meshes=meshes, mesh_index=mesh_index).contiguous()
     return z, sc
 
 
-@pytest.mark.parametrize("name,bits", [("tower_fp32.npz", 32), ("tower_fp64.npz", 64), ("rain120_fp32.npz", 32)])
+@pytest.mark.parametrize("name,bits", [("tower_fp32.npz", 32), ("tower_fp64.npz", 64), ("rain120_fp32.npz", 32),
+                                       ("shape_mix200_fp32.npz", 32), ("shape_mix200_fp64.npz", 64)])
 def test_oracle_reproduces_reference_fixture(name, bits):
     z, sc = load(name)
     o = OracleWorld(sc, bits, color_order=False)
