@@ -94,6 +94,18 @@ typedef struct pe_world_params {
 const char* pe_last_error(void);
 int pe_device_count(int32_t* count);
 
+/* ConvexMeshShape::setMesh (concave = 0, convex_mesh_shape.cpp:12-132) or
+ * ConcaveMeshShape::setMesh (concave = 1, concave_mesh_shape.cpp:6-17) as the library applies
+ * it to a body's mesh, on the host (no device needed).  counts[4] receives the prepared
+ * shape's vertex, face, face-index and unique-edge counts; each other output may be NULL or
+ * hold: vertices [3 nv] (moved to the centre of mass for a convex mesh), face_start [nf + 1],
+ * face_index, face_normal [3 nf] (faces in the BVH build's order), edges [6 ne] (the unique
+ * edges' end points), volume_inertia [10] (volume, then the unit-mass inertia row-major) and
+ * relocation [3] (the centre of mass setMesh returns).  FP32, as the device uses them. */
+int pe_mesh_prepare(const pe_mesh_desc* mesh, int32_t concave, int32_t* counts, float* vertices,
+                    int32_t* face_start, int32_t* face_index, float* face_normal, float* edges,
+                    float* volume_inertia, float* relocation);
+
 /* ------------------------------------------------------------------ plain single world */
 typedef struct pe_world pe_world;
 

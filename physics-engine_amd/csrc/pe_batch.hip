@@ -19,6 +19,7 @@
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
 #include "pe_pipeline.hpp"
+#include "pe_host.hpp"
 #include "../../include/pe_amd.h"
 #include "../../include/pe_amd_batch.h"
 
@@ -66,6 +67,8 @@ struct BatchDev {
     int32_t* nl_pair;    // [total pair cap] narrowphase work list: global pair slots of all worlds
     int32_t* nl_limit;   // [total pair cap] per listed pair: first slot of its world that starts fresh
     int32_t* nl_count;   // [1] entries in the list this step
+    int32_t* nl_shape;   // [total pair cap] list entries whose pair involves a cylinder or a mesh
+    int32_t* nl_shape_count;  // [1]
 };
 
 constexpr uint64_t KEY_PAD = ~0ull;
@@ -232,13 +235,30 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
 
 // K2: SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) over the pairs of
 // all worlds at once, one thread per pair.
+// Box and sphere pairs here; pairs with a cylinder or a mesh are listed for K2b.
 __global__ void __launch_bounds__(128) k_narrow_list(BatchDev d) {
     const int cnt = *d.nl_count;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x)
-        narrow_pair(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+        const int p = d.nl_pair[q];
+        const bool simple = pair_is_simple(d.b, d.ps, p);
+        const int slot = wave_append(!simple, d.nl_shape_count);
+        if (simple) narrow_pair<false>(d.b, d.ps, p, d.nl_limit[q]);
+        else d.nl_shape[slot] = q;
+    }
+}
+// K2b: the listed pairs that involve a cylinder, a convex mesh or a concave mesh.
+__global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d) {
+    const int cnt = *d.nl_shape_count;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int q = d.nl_shape[k];
+        narrow_pair<true>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
+    }
 }
 
-__global__ void k_list_reset(BatchDev d) { *d.nl_count = 0; }
+__global__ void k_list_reset(BatchDev d) {
+    *d.nl_count = 0;
+    *d.nl_shape_count = 0;
+}
 
 // K3: solver setup, colouring, colour-ordered Gauss-Seidel and velocity sync for one world per
 // workgroup (sequential_impulse_solver.cpp:11-81).
@@ -308,67 +328,6 @@ __global__ void k_refresh_range(BodyView b, int first, int count) {
 // ------------------------------------------------------------------ host-side helpers
 int body_pair_cap(int n) { return 8 * n + 64; }
 
-// Host staging of one world's bodies into the SoA store (mass properties as
-// RigidBody::setMass + setCollisionShape compute them, rigidbody.cpp:11-41).
-void stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<int32_t>& hi, size_t stride, int off) {
-    for (int i = 0; i < sc->n; i++) {
-        const size_t col = (size_t)off + i;
-        auto F = [&](int f) -> float& { return hf[(size_t)f * stride + col]; };
-        const int t = sc->shape_type[i];
-        hi[(size_t)I_TYPE * stride + col] = t;
-        hi[(size_t)I_KIN * stride + col] = sc->kinematic[i] ? 1 : 0;
-        for (int k = 0; k < 4; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
-        for (int k = 0; k < 3; k++) {
-            F(F_POS + k) = (float)sc->position[3 * i + k];
-            F(F_V + k) = (float)sc->linear_velocity[3 * i + k];
-            F(F_W + k) = (float)sc->angular_velocity[3 * i + k];
-        }
-        for (int k = 0; k < 9; k++) F(F_ROT + k) = (float)sc->rotation[9 * i + k];
-        const float mass = (float)sc->mass[i];
-        F(F_MASS) = mass;
-        F(F_IM) = mass > EPS ? 1.0f / mass : 0.0f;
-        float I[3] = {0, 0, 0};
-        if (t == PE_SHAPE_BOX) {
-            const float x2 = F(F_SP) * F(F_SP), y2 = F(F_SP + 1) * F(F_SP + 1), z2 = F(F_SP + 2) * F(F_SP + 2);
-            I[0] = (y2 + z2) / 12.0f; I[1] = (x2 + z2) / 12.0f; I[2] = (x2 + y2) / 12.0f;
-        } else if (t == PE_SHAPE_SPHERE) {
-            const float r = F(F_SP);
-            I[0] = I[1] = I[2] = (float)(2.0 / 5.0) * r * r;
-        } else {
-            const float r = F(F_SP), h = F(F_SP + 1);
-            const float r_2 = r * r, h_2 = h * h;
-            I[1] = r_2 * 0.5f;
-            I[0] = I[2] = r_2 / 4.0f + h_2 / 12.0f;
-        }
-        const float a = I[0] * mass, bb = I[1] * mass, c = I[2] * mass;
-        for (int k = 0; k < 9; k++) F(F_LINV + k) = 0.0f;
-        if (mass > EPS) {
-            const float c0 = bb * c - 0.0f * 0.0f;
-            const float det = c0 * a + (0.0f + 0.0f);
-            const float inv = 1.0f / det;
-            F(F_LINV + 0) = c0 * inv;
-            F(F_LINV + 4) = (c * a - 0.0f * 0.0f) * inv;
-            F(F_LINV + 8) = (a * bb - 0.0f * 0.0f) * inv;
-        }
-        F(F_MU) = (float)sc->friction[i];
-        F(F_E) = (float)sc->restitution[i];
-        F(F_LD) = sc->linear_damping ? (float)sc->linear_damping[i] : 0.0f;
-        F(F_AD) = sc->angular_damping ? (float)sc->angular_damping[i] : 0.0f;
-    }
-}
-
-int check_scene(const pe_scene_desc* sc) {
-    if (!sc || sc->n <= 0) { pe_set_error("empty scene"); return PE_ERR_INVALID; }
-    for (int i = 0; i < sc->n; i++) {
-        const int t = sc->shape_type[i];
-        if (t != PE_SHAPE_BOX && t != PE_SHAPE_SPHERE) {
-            pe_set_error("shape type " + std::to_string(t) + " is not supported by this build");
-            return PE_ERR_UNSUPPORTED;
-        }
-    }
-    return PE_OK;
-}
-
 // Allows every instantiation the full LDS of a CU (worlds up to 4096 bodies need ~64 KiB).
 template <int B> void set_lds_one() {
     const int bytes = 150 * 1024;
@@ -433,7 +392,7 @@ struct LaunchTimer {
     hipError_t end(hipStream_t s) { hipError_t r = hipEventRecord(ev[used + 1], s); used += 2; return r; }
 };
 
-int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s,
+int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s, bool shapes,
                LaunchTimer* tm = nullptr) {
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
     for (int phase = 0; phase < 2; phase++) {
@@ -448,6 +407,11 @@ int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32
             if (tm) tm->begin(1, s);
             hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
             if (tm) tm->end(s);
+            if (shapes) {
+                if (tm) tm->begin(1, s);
+                hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
+                if (tm) tm->end(s);
+            }
         }
     }
     hipError_t e = hipGetLastError();
@@ -477,11 +441,14 @@ struct pe_batch {
     hipGraphExec_t gexec = nullptr;
     int64_t dyn_bodies = 0;
 
+    DevMeshes meshes;
+    bool has_shapes = false;   // any cylinder / convex / concave body
     ~pe_batch() {
         if (stream) hipStreamSynchronize(stream);
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (void* p : allocs) hipFree(p);
+        meshes.release();
         if (stream) hipStreamDestroy(stream);
     }
     template <class T> int alloc(T** p, size_t count) {
@@ -513,7 +480,7 @@ struct pe_batch {
         PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         return PE_OK;
     }
-    int launch_step() { return launch_all(d, classes, dlist, stream); }
+    int launch_step() { return launch_all(d, classes, dlist, stream, has_shapes); }
     void drop_graph() {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
@@ -583,6 +550,8 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     rc |= B->alloc(&d.nl_pair, poff);
     rc |= B->alloc(&d.nl_limit, poff);
     rc |= B->alloc(&d.nl_count, 1);
+    rc |= B->alloc(&d.nl_shape, poff);
+    rc |= B->alloc(&d.nl_shape_count, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= B->alloc(&ddesc, nworlds);
@@ -593,10 +562,16 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     std::vector<float> hf(NF * S, 0.0f);
     std::vector<int32_t> hi(NI * S, 0);
     int64_t ndyn = 0;
+    HostMeshes hm;
     for (int w = 0; w < nworlds; w++) {
-        stage_bodies(&scenes[w], hf, hi, S, B->hdesc[w].body_off);
-        for (int i = 0; i < scenes[w].n; i++) ndyn += scenes[w].kinematic[i] ? 0 : 1;
+        if (int rc2 = stage_bodies(&scenes[w], hf, hi, S, B->hdesc[w].body_off, hm)) return fail(rc2);
+        for (int i = 0; i < scenes[w].n; i++) {
+            ndyn += scenes[w].kinematic[i] ? 0 : 1;
+            B->has_shapes |= scenes[w].shape_type[i] > PE_SHAPE_SPHERE;
+        }
     }
+    if (int rc2 = B->meshes.upload(hm, B->stream)) return fail(rc2);
+    d.b.ms = B->meshes.view();
     B->dyn_bodies = ndyn;
     if (hipMemcpyAsync(d.b.f, hf.data(), hf.size() * 4, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
         hipMemcpyAsync(d.b.i, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, B->stream) != hipSuccess ||
@@ -745,7 +720,7 @@ extern "C" int pe_batch_stage_times(pe_batch* B, int32_t nsteps, double* ms /*3*
     LaunchTimer tm;
     for (int s = 0; s < nsteps; s++) {
         tm.used = 0;
-        if (int rc = launch_all(B->d, B->classes, B->dlist, B->stream, &tm)) return rc;
+        if (int rc = launch_all(B->d, B->classes, B->dlist, B->stream, B->has_shapes, &tm)) return rc;
         PE_HIP(hipEventSynchronize(tm.ev[tm.used - 1]));
         for (int k = 0; k < tm.used / 2; k++) {
             float t = 0;
@@ -881,10 +856,13 @@ struct pe_server {
     int64_t admitted = 0;
     hipEvent_t ev_done = nullptr;
 
+    DevMeshes meshes;   // the unit cylinder mesh (requests carry no convex / concave meshes)
+    bool has_shapes = false;   // a cylinder was ever submitted
     ~pe_server() {
         if (cs) hipStreamSynchronize(cs);
         if (xs) hipStreamSynchronize(xs);
         for (void* p : allocs) hipFree(p);
+        meshes.release();
         for (auto& e : ev_upload) if (e) hipEventDestroy(e);
         if (ev_done) hipEventDestroy(ev_done);
         if (cs) hipStreamDestroy(cs);
@@ -994,7 +972,7 @@ struct pe_server {
         int64_t nb = 0;
         for (Req* r : live) nb += r->n;
         if (!live.empty())
-            if (int rc = launch_all(d, classes, dlist, cs)) return rc;
+            if (int rc = launch_all(d, classes, dlist, cs, has_shapes)) return rc;
         PE_HIP(hipEventRecord(ev_done, cs));
         // overlap: next admissions upload while this step runs
         stage_buf ^= 1;
@@ -1095,6 +1073,8 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&d.nl_pair, P);
     rc |= S->alloc(&d.nl_limit, P);
     rc |= S->alloc(&d.nl_count, 1);
+    rc |= S->alloc(&d.nl_shape, P);
+    rc |= S->alloc(&d.nl_shape_count, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= S->alloc(&ddesc, cfg->max_worlds);
@@ -1104,6 +1084,11 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&S->dresult, S->result_cap);
     rc |= S->alloc(&S->dlist, cfg->max_worlds);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
+    {
+        HostMeshes none;
+        if (int rc2 = S->meshes.upload(none, S->cs)) return fail(rc2);
+        d.b.ms = S->meshes.view();
+    }
     d.desc = ddesc;
     d.st = dst;
     S->bodies.init(S->max_bodies);
@@ -1126,6 +1111,11 @@ int pe_server_destroy(pe_server* S) {
 int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, int64_t* id) {
     if (!S || !scene || horizon <= 0) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
     if (int rc = check_scene(scene)) return rc;
+    for (int i = 0; i < scene->n; i++)
+        if (scene->shape_type[i] == PE_SHAPE_CONVEX || scene->shape_type[i] == PE_SHAPE_CONCAVE) {
+            pe_set_error("the server does not admit convex or concave mesh bodies");
+            return PE_ERR_UNSUPPORTED;
+        }
     if (scene->n > S->cfg.max_world_bodies || req_footprint(scene->n) > S->cfg.hbm_budget_bytes) {
         pe_set_error("request larger than the server admits");
         return PE_ERR_BUSY;
@@ -1137,7 +1127,9 @@ int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, 
     r->staged.assign((size_t)NSTAGE * scene->n, 0.0f);
     std::vector<float> hf((size_t)NF * scene->n, 0.0f);
     std::vector<int32_t> hi((size_t)NI * scene->n, 0);
-    stage_bodies(scene, hf, hi, (size_t)scene->n, 0);
+    HostMeshes none;
+    if (int rc = stage_bodies(scene, hf, hi, (size_t)scene->n, 0, none)) { delete r; return rc; }
+    for (int i = 0; i < scene->n; i++) S->has_shapes |= scene->shape_type[i] > PE_SHAPE_SPHERE;
     std::copy(hf.begin(), hf.end(), r->staged.begin());
     std::memcpy(r->staged.data() + hf.size(), hi.data(), hi.size() * 4);
     S->all[r->id] = r;

@@ -50,12 +50,29 @@ enum RowField : int {
     R_MU = 19, R_LN = 20, R_LT0 = 21, R_LT1 = 22, RF = 23, RW = 24
 };
 
+// Polyhedral shape data of the convex / concave meshes of a world or batch (prepared on the
+// host as ConvexMeshShape::setMesh / ConcaveMeshShape::setMesh do) in one float4 arena, one
+// record of arena offsets per mesh, plus the unit cylinder mesh shared by all cylinders.
+struct MeshSet {
+    const int32_t* rec;   // [MRW per mesh]
+    const float4* arena;
+    const float* cyl;     // unit cylinder: top ring x[20] z[20], bottom ring x[20] z[20], side normals [40][3]
+};
+// Mesh record: arena offsets (in float4 units) and counts.
+//   vertices: float4 xyz;  face normals: float4 xyz;  faces: int2 (first index, count) per face;
+//   face indices: int32, relative to the mesh's first vertex;  unique edges: 2 float4 (ends);
+//   BVH nodes: 2 float4 ((aa.xyz, left), (bb.xyz, right), children as int bits);  leaves: int2
+//   (face count, 0 = inner node; first face) per node.
+enum MeshRec : int { MR_V = 0, MR_NV, MR_FN, MR_NF, MR_FACE, MR_FI, MR_E, MR_NE, MR_NODE, MR_NB, MR_LEAF, MR_PAD, MRW };
+constexpr int CYL_UNIT_FLOATS = 80 + 120;
+
 struct BodyView {
     float* f;          // [NF][stride] field-major state (coalesced per-body kernels)
     int32_t* i;        // [NI][stride]
     float* cb;         // [n][CBW] collision-body records
     float* sb;         // [n][SBW] solver-body records
     float* ab;         // [n][8] AABB records: min xyz, kinematic flag, max xyz, 0
+    MeshSet ms;
     int n;
     int stride;
     __host__ __device__ float& at(int field, int b) const { return f[(size_t)field * stride + b]; }

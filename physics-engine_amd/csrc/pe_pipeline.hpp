@@ -5,6 +5,7 @@
 #include "pe_common.hpp"
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
+#include "pe_shapes.hpp"
 
 namespace pe {
 
@@ -111,7 +112,15 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
         for (int k = 0; k < 9; k++) Iw.m[k] = 0.0f;
     }
     f3 mn, mx;
-    shape_aabb(type, sp, R, pos, mn, mx);
+    if (type <= SHAPE_CYLINDER) {
+        shape_aabb(type, sp, R, pos, mn, mx);
+    } else if (kin) {
+        // a kinematic body's pose does not change in a step, so neither does its mesh AABB
+        mn = ld3(b, F_AMIN, i);
+        mx = ld3(b, F_AMAX, i);
+    } else {
+        mesh_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
+    }
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
     write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw);
@@ -125,7 +134,8 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
     const m33 R = ld9(b, F_ROT, i);
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
     f3 mn, mx;
-    shape_aabb(type, sp, R, pos, mn, mx);
+    if (type <= SHAPE_CYLINDER) shape_aabb(type, sp, R, pos, mn, mx);
+    else mesh_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
     m33 Iw;
@@ -173,10 +183,84 @@ __device__ __forceinline__ CollBody load_coll(const BodyView& b, int i) {
     return r;
 }
 
+// Wave-aggregated append: the active lanes with pred take consecutive slots of *counter (one
+// atomic per wave); returns the lane's slot or -1.
+__device__ __forceinline__ int wave_append(bool pred, int* counter) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if ((int)(threadIdx.x & (WAVE - 1)) == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, WAVE);
+    return pred ? base + (int)__popcll(m & lanemask_lt()) : -1;
+}
+
+// True when both bodies of pair p are boxes or spheres.
+__device__ __forceinline__ bool pair_is_simple(const BodyView& b, const PairStore& ps, int p) {
+    return b.iat(I_TYPE, ps.pa[p]) <= SHAPE_SPHERE && b.iat(I_TYPE, ps.pb[p]) <= SHAPE_SPHERE;
+}
+
+// The polyhedral view of a box, cylinder or convex-mesh body.
+__device__ __forceinline__ Poly body_poly(const MeshSet& ms, const CollBody& B) {
+    if (B.type == SHAPE_BOX) return poly_box(B.sp);
+    if (B.type == SHAPE_CYLINDER) return poly_cyl(&ms, B.sp[0], B.sp[1]);
+    return poly_mesh(&ms, __float_as_int(B.sp[0]));
+}
+// NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) for the pairs that involve a
+// cylinder, a convex mesh or a concave mesh, with the role and swap flag each algorithm's
+// processCollision gives the two bodies; the swap flag is cleared again afterwards, as
+// processCollision does.  SimpleNarrowPhase passes refScale = (scaleA + scaleB) * 0.05
+// (simple_narrow_phase.cpp:38).
+static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, PairCtx& x, const CollBody& A,
+                                                  const CollBody& B) {
+    const int ta = A.type, tb = B.type;
+    const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
+    auto is = [&](int p, int q) { return (ta == p && tb == q) || (ta == q && tb == p); };
+    if (is(SHAPE_BOX, SHAPE_CYLINDER)) {                     // box_cylinder_collision_algorithm.cpp:13-47
+        const CollBody& bx = ta == SHAPE_BOX ? A : B;
+        const CollBody& cy = ta == SHAPE_BOX ? B : A;
+        x.swap = ta == SHAPE_BOX;
+        box_cylinder(c, x, bx.P, bx.sp, cy.P, cy.sp[0], cy.sp[1]);
+    } else if (is(SHAPE_SPHERE, SHAPE_CYLINDER)) {           // sphere_cylinder_collision_algorithm.cpp:8-30
+        const CollBody& sp = ta == SHAPE_SPHERE ? A : B;
+        const CollBody& cy = ta == SHAPE_SPHERE ? B : A;
+        x.swap = ta == SHAPE_SPHERE;
+        sphere_cylinder(c, x, sp.P, sp.sp[0], cy.P, cy.sp[0], cy.sp[1]);
+    } else if ((ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) || is(SHAPE_BOX, SHAPE_CONVEX) ||
+               (ta == SHAPE_CONVEX && tb == SHAPE_CONVEX)) {
+        // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
+        const Poly PA = body_poly(ms, A), PB = body_poly(ms, B);
+        convex_convex(c, x, PA, PB, A.P, B.P, ref_scale);
+    } else if (is(SHAPE_SPHERE, SHAPE_CONVEX)) {             // sphere_convex_collision_algorithm.cpp:11-34
+        const CollBody& sp = ta == SHAPE_SPHERE ? A : B;
+        const CollBody& mb = ta == SHAPE_SPHERE ? B : A;
+        x.swap = ta == SHAPE_CONVEX;
+        sphere_mesh(c, x, ms, __float_as_int(mb.sp[0]), mb.P, sp.P, sp.sp[0]);
+    } else if (is(SHAPE_CYLINDER, SHAPE_CONVEX)) {           // cylinder_convex_collision_algorithm.cpp:11-51
+        const CollBody& cy = ta == SHAPE_CYLINDER ? A : B;
+        const CollBody& mb = ta == SHAPE_CYLINDER ? B : A;
+        x.swap = ta == SHAPE_CONVEX;
+        cylinder_mesh(c, x, ms, __float_as_int(mb.sp[0]), mb.P, cy.P, cy.sp[0], cy.sp[1]);
+    } else if ((ta == SHAPE_CONCAVE) != (tb == SHAPE_CONCAVE)) {
+        // concave_{box,sphere,cylinder,convex}_collision_algorithm.cpp: the other body is A
+        const CollBody& mb = ta == SHAPE_CONCAVE ? A : B;
+        const CollBody& ob = ta == SHAPE_CONCAVE ? B : A;
+        const int mesh = __float_as_int(mb.sp[0]);
+        x.swap = ta == SHAPE_CONCAVE;
+        if (ob.type == SHAPE_SPHERE) sphere_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0]);
+        else if (ob.type == SHAPE_CYLINDER) cylinder_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0], ob.sp[1]);
+        else concave_poly(c, x, ms, mesh, mb.P, body_poly(ms, ob), ob.P, ref_scale);
+    }
+    x.swap = false;
+}
+
 // SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) for pair slot p.
 // Slots that existed in the previous step (p < prev_np) keep their distances, because the
 // reference reuses its ContactResult objects and clearContactPoints only invalidates
 // positions (contact_result.cpp:158-164); newer slots start from a fresh ContactResult.
+// SHAPES = false handles box and sphere pairs only (the kernel then carries no polyhedral
+// code and keeps its register budget); SHAPES = true handles every pair type.
+template <bool SHAPES>
 static __device__ __noinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
     Cache c;
@@ -206,6 +290,7 @@ static __device__ __noinline__ void narrow_pair(const BodyView& b, const PairSto
     else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.sp, B.sp[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, B.sp, A.sp[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.sp[0], B.sp[0]);
+    else if (SHAPES) narrow_shapes(b.ms, c, x, A, B);
     int rank[8];
     const int np = sort_rank(c, rank);
     ps.npts[p] = np;
@@ -365,18 +450,6 @@ template <bool LDS>
 __device__ __forceinline__ int claim_get(int* claim, int i) {
     if (LDS) return claim[i];
     return __hip_atomic_load(&claim[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Appends the lanes of the calling wave with pred set to a list whose length is *counter:
-// one atomic per wave.  Every lane of the wave must call it.
-__device__ __forceinline__ int wave_append(bool pred, int* counter) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return -1;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if ((int)(threadIdx.x & (WAVE - 1)) == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader, WAVE);
-    return pred ? base + (int)__popcll(m & lanemask_lt()) : -1;
 }
 
 // bufs: three lists of pair capacity; bufs[0] holds the nact active pairs on entry.

@@ -16,6 +16,7 @@
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
 #include "pe_pipeline.hpp"
+#include "pe_host.hpp"
 #include "../../include/pe_amd.h"
 
 #include <algorithm>
@@ -48,12 +49,14 @@ struct DevCounts {          // device-side counters of one step
     uint32_t nchunks;       // live sweep chunks
     uint32_t bar_count;     // grid barrier: arrivals (0 between barriers)
     uint32_t bar_gen;       // grid barrier: generation
-    int32_t pad[4];
+    int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh
+    int32_t pad[3];
 };
 
 struct Dev {
     BodyView b;
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
+    int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes
     int32_t* order;         // [pcap]
     int32_t* act;           // [3*pcap] colouring work lists
     int32_t* color_start;   // [pcap+1]
@@ -217,13 +220,26 @@ __global__ void k_bp_axis_final(Dev d, int nblocks) {
 }
 
 // ------------------------------------------------------------------ stage 2: narrowphase
+// Box and sphere pairs here; pairs with a cylinder or a mesh are listed for k_narrow_shapes.
 __global__ void __launch_bounds__(128) k_narrow(Dev d) {
     const int np = d.cnt->npairs;
     const int prev = d.cnt->prev_npairs;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) narrow_pair(d.b, d.ps, p, prev);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const bool simple = pair_is_simple(d.b, d.ps, p);
+        const int slot = wave_append(!simple, &d.cnt->nshape);
+        if (simple) narrow_pair<false>(d.b, d.ps, p, prev);
+        else d.shape_list[slot] = p;
+    }
+}
+__global__ void __launch_bounds__(64) k_narrow_shapes(Dev d) {
+    const int ns = d.cnt->nshape;
+    const int prev = d.cnt->prev_npairs;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ns; q += gridDim.x * blockDim.x)
+        narrow_pair<true>(d.b, d.ps, d.shape_list[q], prev);
 }
 __global__ void k_narrow_finish(Dev d) {
     d.cnt->prev_npairs = d.cnt->npairs;
+    d.cnt->nshape = 0;
 }
 
 // ------------------------------------------------------------------ stage 3: solver
@@ -329,10 +345,13 @@ struct pe_world {
     hipEvent_t ev[5] = {};
     double stage_ms[4] = {0, 0, 0, 0};
     int n_dynamic = 0;
+    bool has_shapes = false;   // any cylinder / convex / concave body
+    DevMeshes meshes;
 
     ~pe_world() {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
+        meshes.release();
         for (auto& e : ev) if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
@@ -376,6 +395,7 @@ struct pe_world {
     }
     int launch_narrowphase() {
         hipLaunchKernelGGL(k_narrow, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
+        if (has_shapes) hipLaunchKernelGGL(k_narrow_shapes, dim3(grid(d.pcap, 64)), dim3(64), 0, stream, d);
         hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
         hipLaunchKernelGGL(k_count_points, dim3(1), dim3(1024), 0, stream, d);
         return PE_OK;
@@ -433,13 +453,7 @@ int pe_device_count(int32_t* count) {
 
 int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t max_pairs, pe_world** out) {
     if (!sc || !prm || !out || sc->n <= 0) { pe_set_error("pe_world_create: invalid argument"); return PE_ERR_INVALID; }
-    for (int i = 0; i < sc->n; i++) {
-        const int t = sc->shape_type[i];
-        if (t != PE_SHAPE_BOX && t != PE_SHAPE_SPHERE) {
-            pe_set_error("pe_world_create: shape type " + std::to_string(t) + " is not supported by this build");
-            return PE_ERR_UNSUPPORTED;
-        }
-    }
+    if (int rc = check_scene(sc)) return rc;
     auto* w = new pe_world();
     const int n = sc->n;
     w->n = n;
@@ -470,6 +484,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.ps.swap, pcap);
     rc |= w->alloc(&d.ps.rows, (size_t)RW * pcap * 8);
     rc |= w->alloc(&d.order, pcap);
+    rc |= w->alloc(&d.shape_list, pcap);
     rc |= w->alloc(&d.act, 3 * (size_t)pcap);
     rc |= w->alloc(&d.color_start, pcap + 1);
     rc |= w->alloc(&d.claim, n);
@@ -489,56 +504,17 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.chunk, d.chunk_cap);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
 
-    // host staging of the body store (FP32), mass properties as RigidBody::setMass +
-    // setCollisionShape compute them (rigidbody.cpp:11-41) for the diagonal shape inertias.
+    // host staging of the body store (FP32) and of the scene's meshes
     std::vector<float> hf(NF * S, 0.0f);
     std::vector<int32_t> hi(NI * S, 0);
+    HostMeshes hm;
+    if (int rc2 = stage_bodies(sc, hf, hi, S, 0, hm)) return fail(rc2);
+    if (int rc2 = w->meshes.upload(hm, w->stream)) return fail(rc2);
+    d.b.ms = w->meshes.view();
     int ndyn = 0;
     for (int i = 0; i < n; i++) {
-        auto F = [&](int f) -> float& { return hf[f * S + i]; };
-        const int t = sc->shape_type[i];
-        hi[I_TYPE * S + i] = t;
-        hi[I_KIN * S + i] = sc->kinematic[i] ? 1 : 0;
         ndyn += sc->kinematic[i] ? 0 : 1;
-        for (int k = 0; k < 4; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
-        for (int k = 0; k < 3; k++) {
-            F(F_POS + k) = (float)sc->position[3 * i + k];
-            F(F_V + k) = (float)sc->linear_velocity[3 * i + k];
-            F(F_W + k) = (float)sc->angular_velocity[3 * i + k];
-        }
-        for (int k = 0; k < 9; k++) F(F_ROT + k) = (float)sc->rotation[9 * i + k];
-        const float mass = (float)sc->mass[i];
-        F(F_MASS) = mass;
-        F(F_IM) = mass > EPS ? 1.0f / mass : 0.0f;
-        // shape inertia at unit mass (box_shape.cpp:16-22, sphere_shape.cpp:5-12, cylinder_shape.cpp:23-31)
-        float I[3] = {0, 0, 0};
-        if (t == PE_SHAPE_BOX) {
-            const float x2 = F(F_SP) * F(F_SP), y2 = F(F_SP + 1) * F(F_SP + 1), z2 = F(F_SP + 2) * F(F_SP + 2);
-            I[0] = (y2 + z2) / 12.0f; I[1] = (x2 + z2) / 12.0f; I[2] = (x2 + y2) / 12.0f;
-        } else if (t == PE_SHAPE_SPHERE) {
-            const float r = F(F_SP);
-            I[0] = I[1] = I[2] = (float)(2.0 / 5.0) * r * r;
-        } else {
-            const float r = F(F_SP), h = F(F_SP + 1);
-            const float r_2 = r * r, h_2 = h * h;
-            I[1] = r_2 * 0.5f;
-            I[0] = I[2] = r_2 / 4.0f + h_2 / 12.0f;
-        }
-        // Eigen's 3x3 inverse (cofactors times 1/det) of diag(a, b, c) = mass * I
-        const float a = I[0] * mass, bb = I[1] * mass, c = I[2] * mass;
-        for (int k = 0; k < 9; k++) F(F_LINV + k) = 0.0f;
-        if (mass > EPS) {
-            const float c0 = bb * c - 0.0f * 0.0f;
-            const float det = c0 * a + (0.0f + 0.0f);
-            const float inv = 1.0f / det;
-            F(F_LINV + 0) = c0 * inv;
-            F(F_LINV + 4) = (c * a - 0.0f * 0.0f) * inv;
-            F(F_LINV + 8) = (a * bb - 0.0f * 0.0f) * inv;
-        }
-        F(F_MU) = (float)sc->friction[i];
-        F(F_E) = (float)sc->restitution[i];
-        F(F_LD) = sc->linear_damping ? (float)sc->linear_damping[i] : 0.0f;
-        F(F_AD) = sc->angular_damping ? (float)sc->angular_damping[i] : 0.0f;
+        w->has_shapes |= sc->shape_type[i] > PE_SHAPE_SPHERE;
     }
     w->n_dynamic = ndyn;
     if (hipMemcpyAsync(d.b.f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, w->stream) != hipSuccess ||

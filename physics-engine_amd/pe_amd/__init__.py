@@ -7,3 +7,6 @@
 from . import scenes  # noqa: F401
 from ._lib import PEError, check, lib, LIB_PATH  # noqa: F401
 from .world import World  # noqa: F401
+from .batch import Batch  # noqa: F401
+from .server import Server, ServerConfig  # noqa: F401
+from .mesh import prepare_mesh  # noqa: F401

@@ -17,7 +17,7 @@ ERRORS = {-1: "PE_ERR_INVALID", -2: "PE_ERR_DEVICE", -3: "PE_ERR_CAPACITY", -4: 
 
 # Every symbol include/pe_amd.h declares (checked by tests/test_abi.py).
 EXPORTS = [
-    "pe_last_error", "pe_device_count",
+    "pe_last_error", "pe_device_count", "pe_mesh_prepare",
     "pe_world_create", "pe_world_destroy", "pe_world_step", "pe_world_integrate", "pe_world_broadphase",
     "pe_world_narrowphase", "pe_world_solve", "pe_world_sync", "pe_world_get_state", "pe_world_set_state",
     "pe_world_get_aabbs", "pe_world_get_pairs", "pe_world_get_contacts", "pe_world_get_solve_order",
