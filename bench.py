@@ -51,7 +51,7 @@ def run_batched(args, worlds):
     b = Batch(worlds)
     log(f"batch created: {len(worlds)} worlds, {b.n_bodies} bodies in {time.perf_counter() - t0:.1f}s")
     tuned = b.autotune(steps=2) if not args.no_tune else {}
-    log(f"autotuned block sizes per class: {tuned}; config {b.config()}")
+    log(f"autotuned (block, solver LDS rows/body x10) per class: {tuned}; config {b.config()}")
     b.capture_graph()
     b.step(args.warmup)
     b.sync()
@@ -141,6 +141,44 @@ def run_served(args):
                 p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"], budget_gb=args.serve_budget_gb)
 
 
+def run_shape_mix(args):
+    """Mixed-shape world (50k boxes / spheres / cylinders / convex hulls over a 64 x 64
+    heightfield) on the plain single-world path: args.mix_warm steps to let the drop land and
+    pile up, then args.mix_steps timed steps.  With the reference built and the CPU leg on, the
+    reference engine is timed for args.mix_cpu_steps steps from the same state (its contact
+    caches start empty there), on args.cpu_threads threads."""
+    from pe_amd import World, scenes
+    t0 = time.perf_counter()
+    sc = scenes.shape_mix_scene(args.mix_bodies, seed=1002)
+    log(f"shape mix: {sc.n} bodies, {len(sc.meshes)} meshes generated in {time.perf_counter() - t0:.1f}s")
+    w = World(sc)
+    w.step(args.mix_warm)
+    w.sync()
+    t0 = time.perf_counter()
+    w.step(args.mix_steps)
+    w.sync()
+    dt = time.perf_counter() - t0
+    w.set_timing(True)       # stage breakdown over 10 further steps (events between stages)
+    w.step(10)
+    w.sync()
+    st = [x / 10 for x in w.stage_times()]
+    out = {"workload": f"{sc.n - 1} mixed bodies (40% boxes, 25% spheres, 15% cylinders, 20% hulls of 32 Gaussian "
+                       f"points) over a 64x64 heightfield, steps {args.mix_warm}..{args.mix_warm + args.mix_steps}",
+           "body_steps_per_s": sc.n_dynamic() * args.mix_steps / dt, "ms_per_step": 1000 * dt / args.mix_steps,
+           "stage_ms": st, "counts": w.counts()}
+    if not args.no_cpu:
+        from pyoracle import RefWorld, ref_available
+        if ref_available(32):
+            r = RefWorld(sc, 32, threads=args.cpu_threads)
+            r.set_state(*[np.asarray(x, np.float64) for x in w.state()])
+            cdt = r.step_timed(args.mix_cpu_steps)
+            out["cpu_reference"] = {"body_steps_per_s": sc.n_dynamic() * args.mix_cpu_steps / cdt,
+                                    "ms_per_step": 1000 * cdt / args.mix_cpu_steps, "cores": args.cpu_threads,
+                                    "sample": f"{args.mix_cpu_steps} steps from the GPU state at step {args.mix_warm + args.mix_steps + 10}"}
+    w.close()
+    return out
+
+
 def cpu_baseline(worlds, threads, budget_s):
     """The reference engine itself (oracle/_ref/libperef32.so, compiled from its sources) when
     present, else the oracle restatement; worlds stepped one after another, each with the
@@ -183,6 +221,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-steps", type=int, default=600)
+    ap.add_argument("--no-mix", action="store_true")
+    ap.add_argument("--mix-bodies", type=int, default=50000)
+    ap.add_argument("--mix-warm", type=int, default=240)
+    ap.add_argument("--mix-steps", type=int, default=60)
+    ap.add_argument("--mix-cpu-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device with torch (a kernel of known traffic, "
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -241,7 +284,7 @@ def main():
         "config": {"workload": "batched: 4096 worlds, log-uniform 8-512 bodies (boxes + spheres), "
                                "~496k bodies, dt 1/60, 10 iterations, hipGraph step",
                    "worlds": len(worlds), "bodies": bt["n"], "dynamic_bodies": bt["n_dyn"],
-                   "autotuned_blocks": {str(k): v for k, v in bt["tuned"].items()}},
+                   "autotuned_block_and_solver_lds": {str(k): list(v) for k, v in bt["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": names[k], "algorithmic_bytes_per_launch": sb[k], "launch_ms": bt["stage_ms"][k],
@@ -260,6 +303,10 @@ def main():
                         "batched_over_plain_box_rain_10k": value / plain_c1}
         log(f"plain: c1 {plain_c1:.3e} body-steps/s ({1000 * dt1 / args.plain_steps:.2f} ms/step), "
             f"c3 worlds one at a time {plain_c3:.3e}")
+    if not args.no_mix:
+        mx = run_shape_mix(args)
+        out["shape_mix"] = mx
+        log(f"shape mix: {mx}")
     if not args.no_serve:
         sv = run_served(args)
         out["served"] = sv

@@ -31,6 +31,13 @@ int pe_batch_capture_graph(pe_batch* b);
 /* Size classes: class c holds worlds with upper[c-1] < n <= upper[c] and launches with
  * block[c] threads (64, 128, 256, 512 or 1024).  Drops a captured graph. */
 int pe_batch_set_config(pe_batch* b, int32_t nclasses, const int32_t* upper, const int32_t* block);
+/* Solver LDS budget per size class: the solve kernel keeps a world's bodies, pairs and solve
+ * order in LDS, and its contact rows too when they fit rows_x10[c] / 10 rows per body of the
+ * class's largest world (otherwise the rows stay in the global row store); 0 keeps every row in
+ * global memory.  One value per class (0..640).  Drops a captured graph.  Results do not depend
+ * on it. */
+int pe_batch_set_solver_lds(pe_batch* b, int32_t nclasses, const int32_t* rows_x10);
+int pe_batch_get_solver_lds(pe_batch* b, int32_t* nclasses, int32_t* rows_x10, int32_t cap);
 int pe_batch_get_config(pe_batch* b, int32_t* nclasses, int32_t* upper, int32_t* block, int32_t* count, int32_t cap);
 int pe_batch_get_world_state(pe_batch* b, int32_t world, float* position, float* rotation,
                              float* linear_velocity, float* angular_velocity);

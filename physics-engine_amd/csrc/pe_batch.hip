@@ -88,7 +88,14 @@ __host__ inline size_t broad_lds_bytes(int nmax) {
     const int P = np2(std::max(nmax, 2));
     return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
-__host__ inline size_t solve_lds_bytes(int nmax) { return (size_t)std::max(nmax, 1) * 4; /*claims*/ }
+// K3: claims, bodies (64 B), pair bodies / row starts / order for 3 pairs per body, rows for
+// 3 points per body; capped so a workgroup fits a CU.  Worlds whose rows do not fit fall back
+// to the global row store inside the kernel.
+constexpr size_t SOLVE_LDS_CAP = 144 * 1024;
+__host__ inline size_t solve_lds_bytes(int nmax, int rows_x10) {
+    const size_t n = (size_t)std::max(nmax, 1);
+    return std::min(SOLVE_LDS_CAP, 4 * n + 64 * n + 3 * 4 * (2 * n + 1) + 64 + (size_t)RW * 4 * (n * rows_x10 / 10));
+}
 
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
@@ -261,27 +268,43 @@ __global__ void k_list_reset(BatchDev d) {
 }
 
 // K3: solver setup, colouring, colour-ordered Gauss-Seidel and velocity sync for one world per
-// workgroup (sequential_impulse_solver.cpp:11-81).
+// workgroup (sequential_impulse_solver.cpp:11-81).  When the world's solver rows, bodies and
+// solve order fit the workgroup's LDS (lds_bytes), the 10 sweeps run entirely on the LDS copy
+// and only the velocities are written back; otherwise they run on the global row store.  Both
+// paths perform the same arithmetic in the same order.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
+__global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t* __restrict__ wlist, int nmax,
+                                                       int lds_bytes) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int wsum[BLOCK / WAVE + 1];
+    __shared__ int s_cs[65];
     const int w = wlist[blockIdx.x];
     const WorldDesc wd = d.desc[w];
     const int n = wd.n, off = wd.body_off, poff = wd.pair_off;
     const BodyView& b = d.b;
     const int tid = threadIdx.x;
-    int* claim = (int*)lds;
     const int npairs = d.st[w].npairs;
     (void)nmax;
+    // LDS layout: claims, bodies, pair bodies, row starts, solve order, rows
+    auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    int* claim = (int*)lds;
+    const size_t o_bs = al16((size_t)n * 4);
+    const size_t o_pab = o_bs + (size_t)n * 64;
+    const size_t o_rs = al16(o_pab + (size_t)npairs * 4);
+    const size_t o_ord = al16(o_rs + (size_t)(npairs + 1) * 4);
+    const size_t o_rows = al16(o_ord + (size_t)npairs * 4);
+    const bool fixed_fits = o_rows <= (size_t)lds_bytes;
+    LdsSolve L;
+    L.bs = (float*)(lds + o_bs);
+    L.pab = (int*)(lds + o_pab);
+    L.rs = (int*)(lds + o_rs);
+    L.rows = (float*)(lds + o_rows);
+    L.n = n;
+    int* ord_l = (int*)(lds + o_ord);
     PE_PHASE(4);
-    // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
-    for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
     for (int i = tid; i < n; i += BLOCK) claim[i] = 0x7fffffff;
-    __syncthreads();
+    // active pairs (points, not both kinematic) and the row start of every pair
     int* act = d.act + 3 * (size_t)poff;
-    int* order = d.order + poff;
-    int* cs = d.cstart + poff + w;
     int nact = 0, npts = 0;
     for (int base = 0; base < npairs; base += BLOCK) {
         const int p = base + tid;
@@ -295,13 +318,34 @@ __global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t
         if (flag) act[nact + ex] = poff + p;
         nact += tot;
         int totp;
-        block_exclusive_scan(np_, wsum, &totp);
+        const int exp_ = block_exclusive_scan(np_, wsum, &totp);
+        if (fixed_fits && p < npairs) {
+            L.rs[p] = npts + exp_;
+            L.pab[p] = (d.ps.pa[poff + p] - off) | ((d.ps.pb[poff + p] - off) << 16);
+        }
         npts += totp;
     }
+    const bool use_lds = fixed_fits;
+    const bool rows_lds = use_lds && o_rows + (size_t)npts * RW * 4 <= (size_t)lds_bytes;
+    if (use_lds && tid == 0) L.rs[npairs] = npts;
+    __syncthreads();
+    // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
+    if (rows_lds) {
+        for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt, L.rows + (size_t)L.rs[p] * RW);
+    } else {
+        for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
+    }
+    if (use_lds)
+        for (int i = tid; i < n; i += BLOCK) lds_load_body(L, b, off + i, i);
     __syncthreads();
     PE_PHASE(5);
+    int* order = use_lds ? ord_l : d.order + poff;
+    int* cs = d.cstart + poff + w;
     int* bufs[3] = {act, act + wd.pair_cap, act + 2 * (size_t)wd.pair_cap};
     const int ncol = colour_pairs<true>(b, d.ps, bufs, nact, order, cs, claim, off, wsum);
+    const bool cs_l = ncol < 64;
+    if (cs_l)
+        for (int c = tid; c <= ncol; c += BLOCK) s_cs[c] = cs[c];
     PE_PHASE(6);
     if (tid == 0) {
         d.st[w].ncolors = ncol;
@@ -310,15 +354,36 @@ __global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t
         d.st[w].npoints = npts;
     }
     __syncthreads();
-    for (int it = 0; it < d.iterations; it++) {
-        for (int c = 0; c < ncol; c++) {
-            const int c0 = cs[c], c1 = cs[c + 1];
-            for (int q = c0 + tid; q < c1; q += BLOCK) iterate_pair(b, d.ps, order[q]);
-            __syncthreads();
+    const int* csr = cs_l ? s_cs : cs;
+    if (use_lds) {
+        for (int it = 0; it < d.iterations; it++) {
+            for (int c = 0; c < ncol; c++) {
+                const int c0 = csr[c], c1 = csr[c + 1];
+                for (int q = c0 + tid; q < c1; q += BLOCK) {
+                    const int pl = order[q] - poff;
+                    iterate_pair_lds(L, pl, rows_lds ? L.rows + (size_t)L.rs[pl] * RW : d.ps.rows + (size_t)(poff + pl) * 8 * RW);
+                }
+                __syncthreads();
+            }
         }
+        PE_PHASE(7);
+        // syncTempVelocity from the LDS copy
+        for (int i = tid; i < n; i += BLOCK) {
+            if (L.bs[6 * n + i] < 0.0f) continue;
+            st3(b, F_V, off + i, f3{L.bs[i], L.bs[n + i], L.bs[2 * n + i]});
+            st3(b, F_W, off + i, f3{L.bs[3 * n + i], L.bs[4 * n + i], L.bs[5 * n + i]});
+        }
+    } else {
+        for (int it = 0; it < d.iterations; it++) {
+            for (int c = 0; c < ncol; c++) {
+                const int c0 = csr[c], c1 = csr[c + 1];
+                for (int q = c0 + tid; q < c1; q += BLOCK) iterate_pair(b, d.ps, order[q]);
+                __syncthreads();
+            }
+        }
+        PE_PHASE(7);
+        for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
     }
-    PE_PHASE(7);
-    for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
 }
 
 __global__ void k_refresh_range(BodyView b, int first, int count) {
@@ -343,24 +408,32 @@ void set_lds_limits() {
 
 // Launches K1 (phase 0) or K3 (phase 1) of one class: dispatch over the block sizes.
 template <int B>
-void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
-    if (phase == 0) hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
-    else hipLaunchKernelGGL(k_world_solve<B>, dim3(count), dim3(B), solve_lds_bytes(nmax), s, d, wl, nmax);
+void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, int rows_x10, hipStream_t s) {
+    if (phase == 0) {
+        hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
+    } else {
+        const size_t lb = solve_lds_bytes(nmax, rows_x10);
+        hipLaunchKernelGGL(k_world_solve<B>, dim3(count), dim3(B), lb, s, d, wl, nmax, (int)lb);
+    }
 }
-hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, hipStream_t s) {
+hipError_t launch_class(int phase, int block, int rows_x10, const BatchDev& d, const int32_t* wl, int count, int nmax,
+                        hipStream_t s) {
     switch (block) {
-        case 64: launch_one<64>(phase, d, wl, count, nmax, s); break;
-        case 128: launch_one<128>(phase, d, wl, count, nmax, s); break;
-        case 256: launch_one<256>(phase, d, wl, count, nmax, s); break;
-        case 512: launch_one<512>(phase, d, wl, count, nmax, s); break;
-        default: launch_one<1024>(phase, d, wl, count, nmax, s); break;
+        case 64: launch_one<64>(phase, d, wl, count, nmax, rows_x10, s); break;
+        case 128: launch_one<128>(phase, d, wl, count, nmax, rows_x10, s); break;
+        case 256: launch_one<256>(phase, d, wl, count, nmax, rows_x10, s); break;
+        case 512: launch_one<512>(phase, d, wl, count, nmax, rows_x10, s); break;
+        default: launch_one<1024>(phase, d, wl, count, nmax, rows_x10, s); break;
     }
     return hipGetLastError();
 }
 
+constexpr int DEFAULT_ROWS_X10 = 25;   // solver rows kept in LDS per body of the class's largest world, x10
+
 struct Class {
     int lo, hi;         // body-count range [lo, hi]
     int block;
+    int rows_x10 = DEFAULT_ROWS_X10;
     int count = 0;      // worlds in the class
     int nmax = 0;
     int list_off = 0;   // offset into the device world list
@@ -399,7 +472,7 @@ int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32
         for (auto& c : classes) {
             if (c.count == 0) continue;
             if (tm) tm->begin(phase == 0 ? 0 : 2, s);
-            hipError_t e = launch_class(phase, c.block, d, dlist + c.list_off, c.count, c.nmax, s);
+            hipError_t e = launch_class(phase, c.block, c.rows_x10, d, dlist + c.list_off, c.count, c.nmax, s);
             if (tm) tm->end(s);
             if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
         }
@@ -601,12 +674,30 @@ int pe_batch_set_config(pe_batch* B, int32_t nclasses, const int32_t* upper, con
         if (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024) { pe_set_error("block size must be 64..1024, power of 2"); return PE_ERR_INVALID; }
         const int hi = c == nclasses - 1 ? (1 << 30) : upper[c];
         if (hi < lo) { pe_set_error("class bounds must increase"); return PE_ERR_INVALID; }
-        cl.push_back(Class{lo, hi, bs});
+        cl.push_back(Class{lo, hi, bs, c < (int)B->classes.size() ? B->classes[c].rows_x10 : DEFAULT_ROWS_X10});
         lo = hi + 1;
     }
     B->classes = cl;
     B->drop_graph();
     return B->assign_classes();
+}
+
+int pe_batch_set_solver_lds(pe_batch* B, int32_t nclasses, const int32_t* rows_x10) {
+    if (!B || !rows_x10 || nclasses != (int32_t)B->classes.size()) { pe_set_error("one value per class expected"); return PE_ERR_INVALID; }
+    for (int c = 0; c < nclasses; c++) {
+        if (rows_x10[c] < 0 || rows_x10[c] > 640) { pe_set_error("rows per body x10 must be 0..640"); return PE_ERR_INVALID; }
+        B->classes[c].rows_x10 = rows_x10[c];
+    }
+    B->drop_graph();
+    return PE_OK;
+}
+
+int pe_batch_get_solver_lds(pe_batch* B, int32_t* nclasses, int32_t* rows_x10, int32_t cap) {
+    if (!B || !nclasses) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    *nclasses = (int32_t)B->classes.size();
+    for (int c = 0; c < (int)B->classes.size() && c < cap; c++)
+        if (rows_x10) rows_x10[c] = B->classes[c].rows_x10;
+    return PE_OK;
 }
 
 int pe_batch_get_config(pe_batch* B, int32_t* nclasses, int32_t* upper, int32_t* block, int32_t* count, int32_t cap) {

@@ -335,8 +335,9 @@ __device__ __forceinline__ SolvBody load_solv(const BodyView& b, int i) {
 }
 
 // FrictionContactConstraint::initSequentialImpulse (friction_contact_constraint.cpp:9-73) for
-// every point of pair p.
-__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt) {
+// every point of pair p; row k goes to out + k * RW (the pair's rows in the row store, or a
+// workgroup's LDS copy).
+__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt, float* out) {
     const int n = ps.npts[p];
     if (n == 0) return;
     const int ia = ps.pa[p], ib = ps.pb[p];
@@ -363,8 +364,11 @@ __device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& p
         o[R_T1] = r.t1.x; o[R_T1 + 1] = r.t1.y; o[R_T1 + 2] = r.t1.z;
         o[R_RHS] = r.rhs; o[R_MN] = r.mn; o[R_MT0] = r.mt0; o[R_MT1] = r.mt1; o[R_MU] = r.mu;
         o[R_LN] = 0.0f; o[R_LT0] = 0.0f; o[R_LT1] = 0.0f; o[23] = 0.0f;
-        st_rec<RW / 4>(ps.rows + s * RW, o);
+        st_rec<RW / 4>(out + (size_t)k * RW, o);
     }
+}
+__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt) {
+    setup_pair(b, ps, p, dt, ps.rows + (size_t)p * 8 * RW);
 }
 
 // RigidBody::applyTempImpulse (rigidbody.cpp:166-170), with the body's inverse mass and world
@@ -421,6 +425,68 @@ __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore&
         float4* d = reinterpret_cast<float4*>(b.sbr(ib));
         d[0] = make_float4(B.tv.x, B.tv.y, B.tv.z, B.tw.x);
         d[1].x = B.tw.y; d[1].y = B.tw.z;
+    }
+}
+
+// Solver state of a world held in LDS for the Gauss-Seidel sweeps: per body (local index) the
+// temp velocities, the inverse mass (negative for a kinematic body) and the world inverse
+// inertia, structure of arrays; per pair its local bodies and first row; rows in RW layout.
+struct LdsSolve {
+    float* bs;          // [16][n]: tv xyz, tw xyz, im (or -1: kinematic), Iw 9
+    int* pab;           // [npairs] local A | local B << 16
+    int* rs;            // [npairs + 1] first row of each pair
+    float* rows;        // [npts][RW]
+    int n;
+};
+__device__ __forceinline__ void lds_load_body(const LdsSolve& L, const BodyView& b, int gi, int li) {
+    const float* s = b.sbr(gi);
+    float r[SBW];
+    ld_rec<SBW / 4>(s, r);
+    const int n = L.n;
+#pragma unroll
+    for (int k = 0; k < 6; k++) L.bs[k * n + li] = r[SB_TV + k];
+    L.bs[6 * n + li] = r[SB_KIN] != 0.0f ? -1.0f : r[SB_IM];
+#pragma unroll
+    for (int k = 0; k < 9; k++) L.bs[(7 + k) * n + li] = r[SB_IW + k];
+}
+// iterate_pair on the LDS copy of the bodies (same arithmetic, same order); the pair's rows are
+// at rows (LDS or the global row store), RW floats apart.
+__device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, float* rows) {
+    const int ab = L.pab[pl];
+    const int a = ab & 0xffff, c = ab >> 16;
+    const int nr = L.rs[pl + 1] - L.rs[pl];
+    const int n = L.n;
+    const float* bs = L.bs;
+    f3 tva{bs[a], bs[n + a], bs[2 * n + a]}, twa{bs[3 * n + a], bs[4 * n + a], bs[5 * n + a]};
+    f3 tvb{bs[c], bs[n + c], bs[2 * n + c]}, twb{bs[3 * n + c], bs[4 * n + c], bs[5 * n + c]};
+    const float ima = bs[6 * n + a], imb = bs[6 * n + c];
+    m33 Ia, Ib;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { Ia.m[k] = bs[(7 + k) * n + a]; Ib.m[k] = bs[(7 + k) * n + c]; }
+    for (int k = 0; k < nr; k++) {
+        float* o = rows + (size_t)k * RW;
+        float rr[RW];
+        ld_rec<RW / 4>(o, rr);
+        Row r;
+        r.ra = f3{rr[R_RA], rr[R_RA + 1], rr[R_RA + 2]};
+        r.rb = f3{rr[R_RB], rr[R_RB + 1], rr[R_RB + 2]};
+        r.n = f3{rr[R_N], rr[R_N + 1], rr[R_N + 2]};
+        r.t0 = f3{rr[R_T0], rr[R_T0 + 1], rr[R_T0 + 2]};
+        r.t1 = f3{rr[R_T1], rr[R_T1 + 1], rr[R_T1 + 2]};
+        r.rhs = rr[R_RHS]; r.mn = rr[R_MN]; r.mt0 = rr[R_MT0]; r.mt1 = rr[R_MT1]; r.mu = rr[R_MU];
+        float ln = rr[R_LN], lt0 = rr[R_LT0], lt1 = rr[R_LT1];
+        const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
+        if (ima >= 0.0f) apply_temp(tva, twa, ima, Ia, r.ra, J);
+        if (imb >= 0.0f) apply_temp(tvb, twb, imb, Ib, r.rb, neg(J));
+        reinterpret_cast<float4*>(o)[5] = make_float4(ln, lt0, lt1, 0.0f);
+    }
+    if (ima >= 0.0f) {
+        float* q = L.bs + a;
+        q[0] = tva.x; q[n] = tva.y; q[2 * n] = tva.z; q[3 * n] = twa.x; q[4 * n] = twa.y; q[5 * n] = twa.z;
+    }
+    if (imb >= 0.0f) {
+        float* q = L.bs + c;
+        q[0] = tvb.x; q[n] = tvb.y; q[2 * n] = tvb.z; q[3 * n] = twb.x; q[4 * n] = twb.y; q[5 * n] = twb.z;
     }
 }
 

@@ -79,6 +79,16 @@ class Batch:
         check(lib().pe_batch_get_config(self.h, ctypes.byref(nc), u, bl, cnt, 32))
         return [(u[i], bl[i], cnt[i]) for i in range(nc.value)]
 
+    def set_solver_lds(self, rows_x10):
+        r = (ctypes.c_int32 * len(rows_x10))(*rows_x10)
+        check(lib().pe_batch_set_solver_lds(self.h, ctypes.c_int32(len(rows_x10)), r))
+
+    def solver_lds(self):
+        nc = ctypes.c_int32()
+        r = (ctypes.c_int32 * 32)()
+        check(lib().pe_batch_get_solver_lds(self.h, ctypes.byref(nc), r, 32))
+        return [r[i] for i in range(nc.value)]
+
     def world_state(self, w: int):
         n = self.scenes[w].n
         pos, rot = np.zeros((n, 3), np.float32), np.zeros((n, 3, 3), np.float32)
@@ -92,14 +102,24 @@ class Batch:
         check(lib().pe_batch_get_counts(self.h, c))
         return dict(zip(("bodies", "dynamic", "pairs", "points", "max_colors", "overflow_worlds", "worlds"), list(c)))
 
-    def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None):
-        """Times each candidate block size per size class (other classes at their current
-        setting) and keeps the fastest; returns {class upper bound: block}.  Stepping during
-        tuning advances the worlds (it is real simulation work)."""
+    def _time_steps(self, steps):
+        self.step(1)
+        self.sync()
+        t0 = time.perf_counter()
+        self.step(steps)
+        self.sync()
+        return time.perf_counter() - t0
+
+    def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None, lds_candidates=(0, 15, 25, 40)):
+        """Per size class (other classes at their current setting), times every candidate block
+        size, then every candidate solver LDS row budget (rows per body x10), and keeps the
+        fastest of each; returns {class upper bound: (block, rows_x10)}.  Stepping during tuning
+        advances the worlds (it is real simulation work); results never depend on the choice."""
         cur = self.config()
         uppers = [c[0] for c in cur] if classes is None else list(classes)
         blocks = [c[1] for c in cur] if classes is None else [256] * len(uppers)
         self.set_config(uppers, blocks)
+        rows = self.solver_lds()
         counts = [c[2] for c in self.config()]
         for ci in range(len(uppers)):
             if counts[ci] == 0:
@@ -109,14 +129,21 @@ class Batch:
                 trial = list(blocks)
                 trial[ci] = cand
                 self.set_config(uppers, trial)
-                self.step(1)
-                self.sync()
-                t0 = time.perf_counter()
-                self.step(steps)
-                self.sync()
-                t = time.perf_counter() - t0
+                self.set_solver_lds(rows)
+                t = self._time_steps(steps)
                 if best_t is None or t < best_t:
                     best, best_t = cand, t
             blocks[ci] = best
+            self.set_config(uppers, blocks)
+            best, best_t = rows[ci], None
+            for cand in lds_candidates:
+                trial = list(rows)
+                trial[ci] = cand
+                self.set_solver_lds(trial)
+                t = self._time_steps(steps)
+                if best_t is None or t < best_t:
+                    best, best_t = cand, t
+            rows[ci] = best
         self.set_config(uppers, blocks)
-        return dict(zip(uppers, blocks))
+        self.set_solver_lds(rows)
+        return {u: (b, r) for u, b, r in zip(uppers, blocks, rows)}
