@@ -41,7 +41,9 @@ PE_HD void setc(f3& a, int i, float v) { if (i == 0) a.x = v; else if (i == 1) a
 
 // 3x3 matrix, row-major m[r*3+c]
 struct m33 { float m[9]; };
-PE_HD f3 col(const m33& A, int j) { return f3{A.m[j], A.m[3 + j], A.m[6 + j]}; }
+PE_HD f3 col(const m33& A, int j) {   // by selects, so a run-time j keeps A in registers
+    return j == 0 ? f3{A.m[0], A.m[3], A.m[6]} : (j == 1 ? f3{A.m[1], A.m[4], A.m[7]} : f3{A.m[2], A.m[5], A.m[8]});
+}
 PE_HD f3 mv(const m33& A, f3 v) {     // A * v
     return f3{sum3(A.m[0] * v.x, A.m[1] * v.y, A.m[2] * v.z),
               sum3(A.m[3] * v.x, A.m[4] * v.y, A.m[5] * v.z),
@@ -197,6 +199,9 @@ struct PairCtx {
     float same_th2;    // (min(scaleA, scaleB) * PE_DIST_TH)^2
 };
 
+// All-ones when i == k, else zero: masks for selecting among registers by a run-time index.
+PE_HD int sel_mask(int i, int k) { return -(int)(i == k); }
+
 // ContactResult::addContactPoint (contact_result.cpp:88-142)
 PE_HD void add_point(Cache& c, const PairCtx& x, f3 world_normal, f3 world_pos, float depth) {
     f3 pa = world_pos, pb = world_pos, n = world_normal, point = world_pos;
@@ -220,7 +225,10 @@ PE_HD void add_point(Cache& c, const PairCtx& x, f3 world_normal, f3 world_pos, 
     }
     int slot = -1;
     if (near >= 0) {
-        if (depth < c.d[near]) slot = near;
+        int dn = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) dn |= __float_as_int(c.d[i]) & sel_mask(near, i);
+        if (depth < __int_as_float(dn)) slot = near;
     } else {
 #pragma unroll
         for (int i = 7; i >= 0; i--) if (!(c.wx[i] < REAL_MAX + 0.1f)) slot = i;
@@ -273,84 +281,143 @@ PE_HD int sort_rank(const Cache& c, int rank[8]) {
 
 // ------------------------------------------------------------------ box-box
 // BoxBoxCollisionAlgorithm (box_box_collision_algorithm.cpp:137-580), ODE dBoxBox.
-// intersectRectQuad2 (:20-71): clip the quad p against the rectangle |x| <= h[0], |y| <= h[1].
-PE_HD int intersect_rect_quad2(const float h[2], const float p[8], float ret[16]) {
-    float bufA[16], bufB[16];
-    int nq = 4;
+// Small arrays indexed at run time, kept in registers: every access is unrolled over the slots
+// and combines them with bit masks (a select chain would be folded back into an indexed
+// private-memory access), so no scratch memory is used.
+template <int N> struct RArr {
+    float v[N];
+    PE_HD float get(int i) const {
+        int r = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) bufA[i] = p[i];
-    float* q = bufA;
-    float* r = bufB;
-    int nr = 0;
-    for (int dir = 0; dir <= 1; dir++) {
-        for (int sign = -1; sign <= 1; sign += 2) {
-            float* pq = q;
-            float* pr = r;
-            nr = 0;
-            const float hd = h[dir], sgn = (float)sign;
-            for (int i = nq; i > 0; i--) {
-                if (sgn * pq[dir] < hd) {
-                    pr[0] = pq[0]; pr[1] = pq[1]; pr += 2; nr++;
-                    if (nr & 8) { q = r; goto done; }
+        for (int k = 0; k < N; k++) r |= __float_as_int(v[k]) & sel_mask(i, k);
+        return __int_as_float(r);
+    }
+    PE_HD void set(int i, float x) {
+        const int xb = __float_as_int(x);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const int m = sel_mask(i, k);
+            v[k] = __int_as_float((xb & m) | (__float_as_int(v[k]) & ~m));
+        }
+    }
+};
+template <int N> struct RArrI {
+    int v[N];
+    PE_HD int get(int i) const {
+        int r = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) r |= v[k] & sel_mask(i, k);
+        return r;
+    }
+    PE_HD void set(int i, int x) {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const int m = sel_mask(i, k);
+            v[k] = (x & m) | (v[k] & ~m);
+        }
+    }
+};
+
+// intersectRectQuad2 (:20-71): clips the quad (4 points) against the rectangle |x| <= h0,
+// |y| <= h1, edge by edge (x then y, negative side first); stops as soon as a stage has
+// produced 8 points.  Returns the point count; the points are (ox[k], oy[k]).
+PE_HD int intersect_rect_quad2(float h0, float h1, const float quad[8], RArr<8>& ox, RArr<8>& oy) {
+    RArr<8> qx, qy;
+#pragma unroll
+    for (int k = 0; k < 8; k++) { qx.v[k] = k < 4 ? quad[2 * k] : 0.0f; qy.v[k] = k < 4 ? quad[2 * k + 1] : 0.0f; }
+    int nq = 4;
+    bool done = false;
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+        const int dir = st >> 1;
+        const float sgn = (st & 1) ? 1.0f : -1.0f;
+        const float hd = dir == 0 ? h0 : h1;
+        RArr<8> rx, ry;
+#pragma unroll
+        for (int k = 0; k < 8; k++) { rx.v[k] = 0.0f; ry.v[k] = 0.0f; }
+        int nr = 0;
+        if (!done) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                if (i < nq && !done) {
+                    const float cx = qx.v[i], cy = qy.v[i];
+                    const bool wrap = !(i < 7 && i + 1 < nq);
+                    const float nx = wrap ? qx.v[0] : qx.v[i < 7 ? i + 1 : 0];
+                    const float ny = wrap ? qy.v[0] : qy.v[i < 7 ? i + 1 : 0];
+                    const float cd = dir == 0 ? cx : cy, nd = dir == 0 ? nx : ny;
+                    const float co = dir == 0 ? cy : cx, no = dir == 0 ? ny : nx;
+                    const bool cin = sgn * cd < hd;
+                    if (cin) {
+                        rx.set(nr, cx); ry.set(nr, cy);
+                        nr++;
+                        if (nr & 8) done = true;
+                    }
+                    if (!done && (cin ^ (sgn * nd < hd))) {
+                        const float other = co + (no - co) / (nd - cd) * (sgn * hd - cd);
+                        rx.set(nr, dir == 0 ? sgn * hd : other);
+                        ry.set(nr, dir == 0 ? other : sgn * hd);
+                        nr++;
+                        if (nr & 8) done = true;
+                    }
                 }
-                float* nq_ = (i > 1) ? pq + 2 : q;
-                if ((sgn * pq[dir] < hd) ^ (sgn * nq_[dir] < hd)) {
-                    pr[1 - dir] = pq[1 - dir] + (nq_[1 - dir] - pq[1 - dir]) / (nq_[dir] - pq[dir]) * (sgn * hd - pq[dir]);
-                    pr[dir] = sgn * hd;
-                    pr += 2; nr++;
-                    if (nr & 8) { q = r; goto done; }
-                }
-                pq += 2;
             }
-            float* t = q; q = r; r = t;
+            qx = rx; qy = ry;
             nq = nr;
         }
     }
-done:
-    for (int i = 0; i < 2 * nr; i++) ret[i] = q[i];
-    return nr;
+    ox = qx; oy = qy;
+    return nq;
 }
 
-// cullPoints2 (:73-135)
-PE_HD void cull_points2(int n, const float p[], int m, int i0, int i_ret[]) {
-    float a, cx, cy, q;
-    if (n == 1) { cx = p[0]; cy = p[1]; }
-    else if (n == 2) { cx = 0.5f * (p[0] + p[2]); cy = 0.5f * (p[1] + p[3]); }
-    else {
-        a = 0; cx = 0; cy = 0;
-        for (int i = 0; i < n - 1; i++) {
-            const float p0 = p[i * 2], p1 = p[i * 2 + 1], p2 = p[i * 2 + 2], p3 = p[i * 2 + 3];
+// cullPoints2 (:73-135) for m = 4 of n (5..8) points: keeps point i0 and the three points
+// closest in angle (around the centroid) to i0's angle + k 90 degrees.
+PE_HD void cull_points2(int n, const RArr<8>& px, const RArr<8>& py, int i0, int iret[4]) {
+    const int m = 4;
+    float a = 0, cx = 0, cy = 0, q;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        if (i < n - 1) {
+            const float p0 = px.v[i], p1 = py.v[i], p2 = px.v[i + 1], p3 = py.v[i + 1];
             q = p0 * p3 - p2 * p1;
             a += q;
             cx += q * (p0 + p2);
             cy += q * (p1 + p3);
         }
-        const float p0 = p[0], p1 = p[1], pm1 = p[n * 2 - 1], pm2 = p[n * 2 - 2];
+    }
+    {
+        const float p0 = px.v[0], p1 = py.v[0], pm1 = py.get(n - 1), pm2 = px.get(n - 1);
         q = pm2 * p1 - p0 * pm1;
         if (fabsf(a + q) > EPS) a = 1.0f / (3.0f * (a + q));
         else a = REAL_MAX;
         cx = a * (cx + q * (pm2 + p0));
         cy = a * (cy + q * (pm1 + p1));
     }
-    float A[8];
-    int avail[8];
-    for (int i = 0; i < n; i++) { A[i] = atan2f(p[i * 2 + 1] - cy, p[i * 2] - cx); avail[i] = 1; }
-    avail[i0] = 0;
-    i_ret[0] = i0;
+    RArr<8> A;
+    RArrI<8> avail;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        A.v[i] = i < n ? atan2f(py.v[i] - cy, px.v[i] - cx) : 0.0f;
+        avail.v[i] = i < n ? 1 : 0;
+    }
+    avail.set(i0, 0);
+    iret[0] = i0;
+    const float Ai0 = A.get(i0);
+#pragma unroll
     for (int j = 1; j < m; j++) {
-        a = (float)j * (2.0f * PI_R / (float)m) + A[i0];
+        a = (float)j * (2.0f * PI_R / (float)m) + Ai0;
         if (a > PI_R) a -= 2.0f * PI_R;
         float max_diff = REAL_MAX;
         int pick = i0;
-        for (int i = 0; i < n; i++) {
-            if (avail[i]) {
-                float diff = fabsf(A[i] - a);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (avail.v[i]) {
+                float diff = fabsf(A.v[i] - a);
                 if (diff > PI_R) diff = 2.0f * PI_R - diff;
                 if (diff < max_diff) { max_diff = diff; pick = i; }
             }
         }
-        i_ret[j] = pick;
-        avail[pick] = 0;
+        iret[j] = pick;
+        avail.set(pick, 0);
     }
 }
 
@@ -426,12 +493,14 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
 
     if (code > 6) {  // edge-edge (:281-348)
         f3 pa = p1;
+#pragma unroll
         for (int j = 0; j < 3; j++) {
             const f3 cj = col(R1, j);
             const float sign = dot(normal, cj) > 0 ? 1.0f : -1.0f;
             pa.x += sign * A[j] * cj.x; pa.y += sign * A[j] * cj.y; pa.z += sign * A[j] * cj.z;
         }
         f3 pb = p2;
+#pragma unroll
         for (int j = 0; j < 3; j++) {
             const f3 cj = col(R2, j);
             const float sign = dot(normal, cj) > 0 ? -1.0f : 1.0f;
@@ -453,12 +522,12 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
     }
 
     // face-something (:350-579)
-    const float* Sa;
-    const float* Sb;
+    const f3 SA3{A[0], A[1], A[2]}, SB3{B[0], B[1], B[2]};
+    f3 Sa, Sb;
     m33 Ra, Rb;
     f3 pa, pb;
-    if (code <= 3) { x.swap = false; Ra = R1; Rb = R2; pa = p1; pb = p2; Sa = A; Sb = B; }
-    else { x.swap = true; Ra = R2; Rb = R1; pa = p2; pb = p1; Sa = B; Sb = A; }
+    if (code <= 3) { x.swap = false; Ra = R1; Rb = R2; pa = p1; pb = p2; Sa = SA3; Sb = SB3; }
+    else { x.swap = true; Ra = R2; Rb = R1; pa = p2; pb = p1; Sa = SB3; Sb = SA3; }
     const f3 normal2 = code <= 3 ? normal : neg(normal);
     const f3 nr = mtv(Rb, normal2);
     const f3 anr{fabsf(nr.x), fabsf(nr.y), fabsf(nr.z)};
@@ -472,7 +541,7 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
     }
     f3 center;
     {
-        const float sbl = Sb[lan_r];
+        const float sbl = comp(Sb, lan_r);
         const f3 rl = col(Rb, lan_r);
         if (comp(nr, lan_r) < 0) center = f3{pb.x - pa.x + sbl * rl.x, pb.y - pa.y + sbl * rl.y, pb.z - pa.z + sbl * rl.z};
         else center = f3{pb.x - pa.x - sbl * rl.x, pb.y - pa.y - sbl * rl.y, pb.z - pa.z - sbl * rl.z};
@@ -487,32 +556,37 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
     float m11 = dot(rc1, rba1), m12 = dot(rc1, rba2), m21 = dot(rc2, rba1), m22 = dot(rc2, rba2);
     float quad[8];
     {
-        const float k1 = m11 * Sb[a1], k2 = m21 * Sb[a1], k3 = m12 * Sb[a2], k4 = m22 * Sb[a2];
+        const float sba1 = comp(Sb, a1), sba2 = comp(Sb, a2);
+        const float k1 = m11 * sba1, k2 = m21 * sba1, k3 = m12 * sba2, k4 = m22 * sba2;
         const float c1mk1 = c1 - k1, c2mk2 = c2 - k2;
         quad[0] = c1mk1 - k3; quad[1] = c2mk2 - k4; quad[2] = c1mk1 + k3; quad[3] = c2mk2 + k4;
         const float c1pk1 = c1 + k1, c2pk2 = c2 + k2;
         quad[4] = c1pk1 + k3; quad[5] = c2pk2 + k4; quad[6] = c1pk1 - k3; quad[7] = c2pk2 - k4;
     }
-    const float rect[2] = {Sa[code1], Sa[code2]};
-    float ret[16];
-    const int n = intersect_rect_quad2(rect, quad, ret);
+    RArr<8> rx, ry;
+    const int n = intersect_rect_quad2(comp(Sa, code1), comp(Sa, code2), quad, rx, ry);
     if (n < 1) return;
-    float point[24];
-    float dep[8];
+    RArr<8> ptx, pty, ptz, dep;
     const float det1 = 1.0f / (m11 * m22 - m12 * m21);
     m11 *= det1; m12 *= det1; m21 *= det1; m22 *= det1;
+    const float san = comp(Sa, codeN);
     int c_num = 0;
-    for (int j = 0; j < n; j++) {
-        const float k1 = m22 * (ret[j * 2] - c1) - m12 * (ret[j * 2 + 1] - c2);
-        const float k2 = -m21 * (ret[j * 2] - c1) + m11 * (ret[j * 2 + 1] - c2);
-        const f3 pt{center.x + k1 * rba1.x + k2 * rba2.x, center.y + k1 * rba1.y + k2 * rba2.y,
-                    center.z + k1 * rba1.z + k2 * rba2.z};
-        point[c_num * 3] = pt.x; point[c_num * 3 + 1] = pt.y; point[c_num * 3 + 2] = pt.z;
-        dep[c_num] = Sa[codeN] - dot(normal2, pt);
-        if (dep[c_num] >= 0) {
-            ret[c_num * 2] = ret[j * 2];
-            ret[c_num * 2 + 1] = ret[j * 2 + 1];
-            c_num++;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        if (j < n) {
+            const float rxj = rx.v[j], ryj = ry.v[j];
+            const float k1 = m22 * (rxj - c1) - m12 * (ryj - c2);
+            const float k2 = -m21 * (rxj - c1) + m11 * (ryj - c2);
+            const f3 pt{center.x + k1 * rba1.x + k2 * rba2.x, center.y + k1 * rba1.y + k2 * rba2.y,
+                        center.z + k1 * rba1.z + k2 * rba2.z};
+            ptx.set(c_num, pt.x); pty.set(c_num, pt.y); ptz.set(c_num, pt.z);
+            const float dj = san - dot(normal2, pt);
+            dep.set(c_num, dj);
+            if (dj >= 0) {
+                rx.set(c_num, rxj);
+                ry.set(c_num, ryj);
+                c_num++;
+            }
         }
     }
     if (c_num < 1) return;
@@ -521,22 +595,28 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
     const f3 nv = normal;
     const f3 marg = mul(nv, 0.0f);
     if (c_num <= max_c) {
-        for (int j = 0; j < c_num; j++) {
-            const f3 pw{point[j * 3] + pa.x, point[j * 3 + 1] + pa.y, point[j * 3 + 2] + pa.z};
-            if (code < 4) add_point(c, x, neg(nv), add(pw, marg), -dep[j] + 0.0f * 2);
-            else add_point(c, x, nv, sub(pw, marg), -dep[j] + 0.0f * 2);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j < c_num) {
+                const f3 pw{ptx.v[j] + pa.x, pty.v[j] + pa.y, ptz.v[j] + pa.z};
+                if (code < 4) add_point(c, x, neg(nv), add(pw, marg), -dep.v[j] + 0.0f * 2);
+                else add_point(c, x, nv, sub(pw, marg), -dep.v[j] + 0.0f * 2);
+            }
         }
     } else {
         int i1 = 0;
-        float maxd = dep[0];
-        for (int i = 1; i < c_num; i++) if (dep[i] > maxd) { maxd = dep[i]; i1 = i; }
-        int iret[8];
-        cull_points2(c_num, ret, max_c, i1, iret);
-        for (int j = 0; j < max_c; j++) {
+        float maxd = dep.v[0];
+#pragma unroll
+        for (int i = 1; i < 8; i++) if (i < c_num && dep.v[i] > maxd) { maxd = dep.v[i]; i1 = i; }
+        int iret[4];
+        cull_points2(c_num, rx, ry, i1, iret);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
             const int k = iret[j];
-            const f3 pw{point[k * 3] + pa.x, point[k * 3 + 1] + pa.y, point[k * 3 + 2] + pa.z};
-            if (code < 4) add_point(c, x, neg(nv), add(pw, marg), -dep[k] + 0.0f * 2);
-            else add_point(c, x, nv, sub(pw, marg), -dep[k] + 0.0f * 2);
+            const f3 pw{ptx.get(k) + pa.x, pty.get(k) + pa.y, ptz.get(k) + pa.z};
+            const float dk = dep.get(k);
+            if (code < 4) add_point(c, x, neg(nv), add(pw, marg), -dk + 0.0f * 2);
+            else add_point(c, x, nv, sub(pw, marg), -dk + 0.0f * 2);
         }
     }
 }

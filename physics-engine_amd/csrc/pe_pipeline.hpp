@@ -261,7 +261,7 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
 // SHAPES = false handles box and sphere pairs only (the kernel then carries no polyhedral
 // code and keeps its register budget); SHAPES = true handles every pair type.
 template <bool SHAPES>
-static __device__ __noinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
+static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
     Cache c;
     const size_t slot0 = (size_t)p * 8;
