@@ -51,7 +51,7 @@ def run_batched(args, worlds):
     b = Batch(worlds)
     log(f"batch created: {len(worlds)} worlds, {b.n_bodies} bodies in {time.perf_counter() - t0:.1f}s")
     tuned = b.autotune(steps=2) if not args.no_tune else {}
-    log(f"autotuned (block, solver LDS rows/body x10) per class: {tuned}; config {b.config()}")
+    log(f"autotuned (block, solver group bodies) per class: {tuned}; config {b.config()}")
     b.capture_graph()
     b.step(args.warmup)
     b.sync()
@@ -284,7 +284,7 @@ def main():
         "config": {"workload": "batched: 4096 worlds, log-uniform 8-512 bodies (boxes + spheres), "
                                "~496k bodies, dt 1/60, 10 iterations, hipGraph step",
                    "worlds": len(worlds), "bodies": bt["n"], "dynamic_bodies": bt["n_dyn"],
-                   "autotuned_block_and_solver_lds": {str(k): list(v) for k, v in bt["tuned"].items()}},
+                   "autotuned_block_and_solver_group": {str(k): list(v) for k, v in bt["tuned"].items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": names[k], "algorithmic_bytes_per_launch": sb[k], "launch_ms": bt["stage_ms"][k],

@@ -31,13 +31,13 @@ int pe_batch_capture_graph(pe_batch* b);
 /* Size classes: class c holds worlds with upper[c-1] < n <= upper[c] and launches with
  * block[c] threads (64, 128, 256, 512 or 1024).  Drops a captured graph. */
 int pe_batch_set_config(pe_batch* b, int32_t nclasses, const int32_t* upper, const int32_t* block);
-/* Solver LDS budget per size class: the solve kernel keeps a world's bodies, pairs and solve
- * order in LDS, and its contact rows too when they fit rows_x10[c] / 10 rows per body of the
- * class's largest world (otherwise the rows stay in the global row store); 0 keeps every row in
- * global memory.  One value per class (0..640).  Drops a captured graph.  Results do not depend
- * on it. */
-int pe_batch_set_solver_lds(pe_batch* b, int32_t nclasses, const int32_t* rows_x10);
-int pe_batch_get_solver_lds(pe_batch* b, int32_t* nclasses, int32_t* rows_x10, int32_t cap);
+/* Solver grouping per size class: the solve kernel takes consecutive worlds of a class (worlds
+ * are laid out largest first) up to group_bodies[c] bodies into one workgroup and solves them as
+ * one system, bodies, pairs and colouring in LDS; 0 solves one world per workgroup.  Worlds
+ * never share a body, so every world's result is the one it gets alone.  One value per class
+ * (0..2048).  Drops a captured graph. */
+int pe_batch_set_solver_groups(pe_batch* b, int32_t nclasses, const int32_t* group_bodies);
+int pe_batch_get_solver_groups(pe_batch* b, int32_t* nclasses, int32_t* group_bodies, int32_t cap);
 int pe_batch_get_config(pe_batch* b, int32_t* nclasses, int32_t* upper, int32_t* block, int32_t* count, int32_t cap);
 int pe_batch_get_world_state(pe_batch* b, int32_t world, float* position, float* rotation,
                              float* linear_velocity, float* angular_velocity);
@@ -55,9 +55,10 @@ int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
  * Gauss-Seidel sweeps.  Drops a captured graph. */
 int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
 
-/* Runs nsteps eagerly with a hipEvent pair around every kernel launch; ms[k] is the summed
- * kernel time of integrate + broadphase (k = 0, all size classes), narrowphase (1), solver
- * (2, all size classes), without the gaps between launches. */
+/* Runs nsteps eagerly with a hipEvent pair around every stage; ms[k] is the summed device time
+ * of integrate + broadphase (k = 0, all size classes), narrowphase (1) and solver (2, all size
+ * classes), each from before its first launch to after its last (the size classes of a stage
+ * run concurrently on side streams), without the gaps between stages. */
 int pe_batch_stage_times(pe_batch* b, int32_t nsteps, double* ms /*3*/);
 
 /* ------------------------------------------------------------------ server

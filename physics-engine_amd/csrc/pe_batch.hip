@@ -6,8 +6,10 @@
 //                            sweep in (i, j) order, next axis; the world's pairs are appended to
 //                            a narrowphase work list
 //   K2 k_narrow_list         one thread per pair of every world: narrowphase + contact cache
-//   K3 k_world_solve<BLOCK>  one workgroup per world: solver rows, colouring (claims in LDS),
-//                            colour-ordered Gauss-Seidel sweeps, velocity sync
+//   K3 k_group_prep<BLOCK>   one workgroup per group of worlds of a size class: solver rows and
+//                            colouring (LDS lists)
+//      k_group_sweep<BLOCK>  colour-ordered Gauss-Seidel sweeps on LDS copies of the group's
+//                            bodies and pairs, velocity sync
 // Every routine is the one the single-world pipeline uses (pe_pipeline.hpp), and the sort,
 // sweep order and colouring rule are the same, so a world stepped here is bit-identical to
 // the same world stepped alone -- batching never changes a result.
@@ -41,6 +43,8 @@ struct WorldDesc {      // static per world
     int32_t n;
     int32_t pair_off;
     int32_t pair_cap;
+    int32_t cs_off;     // colour starts of the global-memory solve path: pair_cap + 1 slots of d.cstart
+    int32_t pad[3];
 };
 struct WorldState {     // per world, updated by every step
     int32_t npairs;
@@ -88,15 +92,14 @@ __host__ inline size_t broad_lds_bytes(int nmax) {
     const int P = np2(std::max(nmax, 2));
     return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
-// K3: claims, bodies (64 B), pair bodies / row starts / order for 3 pairs per body, rows for
-// 3 points per body; capped so a workgroup fits a CU.  Worlds whose rows do not fit fall back
-// to the global row store inside the kernel.
+// K3: LDS of a group of G bodies (G = max(group bodies, largest world)): claims and body state
+// (68 B per body), and per pair (estimated at 2 per body) its global slot, local bodies, row
+// start, order slot and three colouring-list slots (28 B); capped so a workgroup fits a CU.
 constexpr size_t SOLVE_LDS_CAP = 144 * 1024;
-__host__ inline size_t solve_lds_bytes(int nmax, int rows_x10) {
-    const size_t n = (size_t)std::max(nmax, 1);
-    return std::min(SOLVE_LDS_CAP, 4 * n + 64 * n + 3 * 4 * (2 * n + 1) + 64 + (size_t)RW * 4 * (n * rows_x10 / 10));
+__host__ inline size_t solve_lds_bytes(int nmax, int group_bodies) {
+    const size_t g = (size_t)std::max(std::max(nmax, group_bodies), 1);
+    return std::min(SOLVE_LDS_CAP, 68 * g + 28 * 2 * g + 128);
 }
-
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
 template <int BLOCK>
@@ -267,43 +270,18 @@ __global__ void k_list_reset(BatchDev d) {
     *d.nl_shape_count = 0;
 }
 
-// K3: solver setup, colouring, colour-ordered Gauss-Seidel and velocity sync for one world per
-// workgroup (sequential_impulse_solver.cpp:11-81).  When the world's solver rows, bodies and
-// solve order fit the workgroup's LDS (lds_bytes), the 10 sweeps run entirely on the LDS copy
-// and only the velocities are written back; otherwise they run on the global row store.  Both
-// paths perform the same arithmetic in the same order.
+// K3 for one world on the global row store (claims in LDS): solver setup, colouring,
+// colour-ordered Gauss-Seidel sweeps, velocity sync (sequential_impulse_solver.cpp:11-81).
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t* __restrict__ wlist, int nmax,
-                                                       int lds_bytes) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int wsum[BLOCK / WAVE + 1];
-    __shared__ int s_cs[65];
-    const int w = wlist[blockIdx.x];
+__device__ void solve_world_global(const BatchDev& d, int w, int* claim, int* wsum) {
     const WorldDesc wd = d.desc[w];
     const int n = wd.n, off = wd.body_off, poff = wd.pair_off;
     const BodyView& b = d.b;
     const int tid = threadIdx.x;
     const int npairs = d.st[w].npairs;
-    (void)nmax;
-    // LDS layout: claims, bodies, pair bodies, row starts, solve order, rows
-    auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-    int* claim = (int*)lds;
-    const size_t o_bs = al16((size_t)n * 4);
-    const size_t o_pab = o_bs + (size_t)n * 64;
-    const size_t o_rs = al16(o_pab + (size_t)npairs * 4);
-    const size_t o_ord = al16(o_rs + (size_t)(npairs + 1) * 4);
-    const size_t o_rows = al16(o_ord + (size_t)npairs * 4);
-    const bool fixed_fits = o_rows <= (size_t)lds_bytes;
-    LdsSolve L;
-    L.bs = (float*)(lds + o_bs);
-    L.pab = (int*)(lds + o_pab);
-    L.rs = (int*)(lds + o_rs);
-    L.rows = (float*)(lds + o_rows);
-    L.n = n;
-    int* ord_l = (int*)(lds + o_ord);
-    PE_PHASE(4);
+    for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
     for (int i = tid; i < n; i += BLOCK) claim[i] = 0x7fffffff;
-    // active pairs (points, not both kinematic) and the row start of every pair
+    __syncthreads();
     int* act = d.act + 3 * (size_t)poff;
     int nact = 0, npts = 0;
     for (int base = 0; base < npairs; base += BLOCK) {
@@ -318,35 +296,14 @@ __global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t
         if (flag) act[nact + ex] = poff + p;
         nact += tot;
         int totp;
-        const int exp_ = block_exclusive_scan(np_, wsum, &totp);
-        if (fixed_fits && p < npairs) {
-            L.rs[p] = npts + exp_;
-            L.pab[p] = (d.ps.pa[poff + p] - off) | ((d.ps.pb[poff + p] - off) << 16);
-        }
+        block_exclusive_scan(np_, wsum, &totp);
         npts += totp;
     }
-    const bool use_lds = fixed_fits;
-    const bool rows_lds = use_lds && o_rows + (size_t)npts * RW * 4 <= (size_t)lds_bytes;
-    if (use_lds && tid == 0) L.rs[npairs] = npts;
     __syncthreads();
-    // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
-    if (rows_lds) {
-        for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt, L.rows + (size_t)L.rs[p] * RW);
-    } else {
-        for (int p = tid; p < npairs; p += BLOCK) setup_pair(b, d.ps, poff + p, d.dt);
-    }
-    if (use_lds)
-        for (int i = tid; i < n; i += BLOCK) lds_load_body(L, b, off + i, i);
-    __syncthreads();
-    PE_PHASE(5);
-    int* order = use_lds ? ord_l : d.order + poff;
-    int* cs = d.cstart + poff + w;
+    int* order = d.order + poff;
+    int* cs = d.cstart + wd.cs_off;
     int* bufs[3] = {act, act + wd.pair_cap, act + 2 * (size_t)wd.pair_cap};
     const int ncol = colour_pairs<true>(b, d.ps, bufs, nact, order, cs, claim, off, wsum);
-    const bool cs_l = ncol < 64;
-    if (cs_l)
-        for (int c = tid; c <= ncol; c += BLOCK) s_cs[c] = cs[c];
-    PE_PHASE(6);
     if (tid == 0) {
         d.st[w].ncolors = ncol;
         d.st[w].npairs = npairs;
@@ -354,35 +311,225 @@ __global__ void __launch_bounds__(BLOCK) k_world_solve(BatchDev d, const int32_t
         d.st[w].npoints = npts;
     }
     __syncthreads();
-    const int* csr = cs_l ? s_cs : cs;
-    if (use_lds) {
-        for (int it = 0; it < d.iterations; it++) {
-            for (int c = 0; c < ncol; c++) {
-                const int c0 = csr[c], c1 = csr[c + 1];
-                for (int q = c0 + tid; q < c1; q += BLOCK) {
-                    const int pl = order[q] - poff;
-                    iterate_pair_lds(L, pl, rows_lds ? L.rows + (size_t)L.rs[pl] * RW : d.ps.rows + (size_t)(poff + pl) * 8 * RW);
+    for (int it = 0; it < d.iterations; it++) {
+        for (int c = 0; c < ncol; c++) {
+            const int c0 = cs[c], c1 = cs[c + 1];
+            for (int q = c0 + tid; q < c1; q += BLOCK) iterate_pair(b, d.ps, order[q]);
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
+    __syncthreads();
+}
+
+constexpr int MAX_GROUP_WORLDS = 64;
+
+// A group of consecutive worlds of a size class (worlds are laid out in class order, so the
+// group's bodies and pair slots are contiguous) solved as one system: its pairs never share a
+// body across worlds, so the colouring yields each world's own colour classes and the sweeps
+// apply to every world the same updates in the same order as when it is solved alone.
+struct Group {
+    const int32_t* wl;
+    int nw, B0, GB, GP;
+};
+// LDS layout of the preparation kernel; the group is solved on the global path when it does
+// not fit lds_bytes (both kernels evaluate this the same way).
+struct PrepLayout {
+    size_t bs, pab, gi, rs, ord, buf, cs, end;
+    __device__ PrepLayout(int GB, int GP) {
+        auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+        bs = al16((size_t)GB * 4);
+        pab = bs + (size_t)GB * 64;
+        gi = al16(pab + (size_t)GP * 4);
+        rs = al16(gi + (size_t)GP * 4);
+        ord = al16(rs + (size_t)(GP + 1) * 4);
+        buf = al16(ord + (size_t)GP * 4);
+        cs = al16(buf + (size_t)GP * 12);
+        end = cs + (size_t)(GP + 2) * 4;
+    }
+};
+__device__ __forceinline__ bool group_on_lds(const PrepLayout& L, int GB, int lds_bytes) {
+    return L.end <= (size_t)lds_bytes && GB <= 65535;
+}
+// Reads the group's geometry; s_wb[k] = first local pair of world k (s_wb[nw] = GP).
+__device__ __forceinline__ Group load_group(const BatchDev& d, const int32_t* wlist, int2 g, int* s_wb) {
+    Group G;
+    G.wl = wlist + g.x;
+    G.nw = g.y;
+    G.B0 = d.desc[G.wl[0]].body_off;
+    G.GB = d.desc[G.wl[G.nw - 1]].body_off + d.desc[G.wl[G.nw - 1]].n - G.B0;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < G.nw; k++) { s_wb[k] = acc; acc += d.st[G.wl[k]].npairs; }
+        s_wb[G.nw] = acc;
+    }
+    __syncthreads();
+    G.GP = s_wb[G.nw];
+    return G;
+}
+// World k of the group holding local pair lp.
+__device__ __forceinline__ int world_of(const int* s_wb, int nw, int lp) {
+    int lo = 0, hi = nw - 1;
+    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (s_wb[mid] <= lp) lo = mid; else hi = mid - 1; }
+    return lo;
+}
+
+// K3a: solver rows (FrictionContactConstraint::initSequentialImpulse, to the global row store)
+// and the colouring of a group; the solve order (local pair indices) and colour starts go to
+// the first world's order and colour-start regions for K3b.  A group that does not fit LDS is
+// solved here completely, world by world, on the global path.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t* __restrict__ wlist,
+                                                      const int2* __restrict__ groups, int lds_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int wsum[BLOCK / WAVE + 1];
+    __shared__ int s_wb[MAX_GROUP_WORLDS + 1];
+    __shared__ int s_wpts[MAX_GROUP_WORLDS];
+    const int tid = threadIdx.x;
+    const BodyView& b = d.b;
+    const Group G = load_group(d, wlist, groups[blockIdx.x], s_wb);
+    const int w = G.wl[0];   // for the phase timestamps
+    PE_PHASE(4);
+    const PrepLayout Lo(G.GB, G.GP);
+    if (!group_on_lds(Lo, G.GB, lds_bytes)) {
+        for (int k = 0; k < G.nw; k++) solve_world_global<BLOCK>(d, G.wl[k], (int*)lds, wsum);
+        return;
+    }
+    const int GB = G.GB, GP = G.GP, nw = G.nw, B0 = G.B0;
+    if (tid < nw) s_wpts[tid] = 0;
+    int* claim = (int*)lds;
+    float* imk = (float*)(lds + Lo.bs);   // inverse mass, negative for kinematic bodies
+    int* pab = (int*)(lds + Lo.pab);
+    int* gidx = (int*)(lds + Lo.gi);
+    int* order = (int*)(lds + Lo.ord);
+    int* bufs[3] = {(int*)(lds + Lo.buf), (int*)(lds + Lo.buf) + GP, (int*)(lds + Lo.buf) + 2 * GP};
+    int* cs = (int*)(lds + Lo.cs);
+    for (int i = tid; i < GB; i += BLOCK) {
+        claim[i] = 0x7fffffff;
+        imk[i] = b.iat(I_KIN, B0 + i) ? -1.0f : 0.0f;
+    }
+    __syncthreads();
+    int nact = 0;
+    for (int base = 0; base < GP; base += BLOCK) {
+        const int lp = base + tid;
+        int flag = 0;
+        if (lp < GP) {
+            const int k = world_of(s_wb, nw, lp);
+            const int gs = d.desc[G.wl[k]].pair_off + (lp - s_wb[k]);
+            const int pa = d.ps.pa[gs], pb = d.ps.pb[gs];
+            const int np_ = d.ps.npts[gs];
+            gidx[lp] = gs;
+            pab[lp] = (pa - B0) | ((pb - B0) << 16);
+            if (np_ > 0) {
+                flag = !(imk[pa - B0] < 0.0f && imk[pb - B0] < 0.0f);
+                atomicAdd(&s_wpts[k], np_);
+            }
+        }
+        int tot;
+        const int ex = block_exclusive_scan(flag, wsum, &tot);
+        if (flag) bufs[0][nact + ex] = lp;
+        nact += tot;
+    }
+    __syncthreads();
+    // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
+    for (int lp = tid; lp < GP; lp += BLOCK) setup_pair(b, d.ps, gidx[lp], d.dt);
+    PE_PHASE(5);
+    const int ncol = colour_group(pab, imk, bufs, nact, order, cs, claim);
+    PE_PHASE(6);
+    const WorldDesc w0 = d.desc[G.wl[0]];
+    for (int q = tid; q < nact; q += BLOCK) d.order[w0.pair_off + q] = order[q];
+    for (int c = tid; c <= ncol; c += BLOCK) d.cstart[w0.cs_off + c] = cs[c];
+    for (int k = tid; k < nw; k += BLOCK) {
+        const int wk = G.wl[k];
+        const int np = s_wb[k + 1] - s_wb[k];
+        d.st[wk].ncolors = ncol;
+        d.st[wk].npairs = np;
+        d.st[wk].prev_npairs = np;
+        d.st[wk].npoints = s_wpts[k];
+    }
+}
+
+// K3b: the colour-ordered Gauss-Seidel sweeps of a prepared group on LDS copies of its bodies
+// and pairs (rows in the global row store), then syncTempVelocity.  Kept small so more waves
+// fit a SIMD: its passes are chains of dependent arithmetic separated by barriers.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_group_sweep(BatchDev d, const int32_t* __restrict__ wlist,
+                                                       const int2* __restrict__ groups, int prep_lds_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int wsum[BLOCK / WAVE + 1];
+    __shared__ int s_wb[MAX_GROUP_WORLDS + 1];
+    const int tid = threadIdx.x;
+    const BodyView& b = d.b;
+    const Group G = load_group(d, wlist, groups[blockIdx.x], s_wb);
+    if (!group_on_lds(PrepLayout(G.GB, G.GP), G.GB, prep_lds_bytes)) return;   // solved by K3a
+    const int GB = G.GB, GP = G.GP, nw = G.nw, B0 = G.B0;
+    const int w = G.wl[0];
+    auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    LdsSolve L;
+    L.bs = (float*)lds;
+    L.pab = (int*)(lds + (size_t)GB * 64);
+    int* gidx = L.pab + GP;
+    L.rs = gidx + GP;
+    int* order = (int*)(lds + al16((size_t)GB * 64 + (size_t)(3 * GP + 1) * 4));
+    L.n = GB;
+    const WorldDesc w0 = d.desc[w];
+    const int ncol = d.st[w].ncolors;
+    for (int i = tid; i < GB; i += BLOCK) lds_load_body(L, b, B0 + i, i);
+    int npts = 0;
+    for (int base = 0; base < GP; base += BLOCK) {
+        const int lp = base + tid;
+        int np_ = 0;
+        if (lp < GP) {
+            const int k = world_of(s_wb, nw, lp);
+            const int gs = d.desc[G.wl[k]].pair_off + (lp - s_wb[k]);
+            gidx[lp] = gs;
+            L.pab[lp] = (d.ps.pa[gs] - B0) | ((d.ps.pb[gs] - B0) << 16);
+            np_ = d.ps.npts[gs];
+        }
+        int tot;
+        const int ex = block_exclusive_scan(np_, wsum, &tot);
+        if (lp < GP) L.rs[lp] = npts + ex;
+        npts += tot;
+    }
+    if (tid == 0) L.rs[GP] = npts;
+    const int nord = d.cstart[w0.cs_off + ncol];
+    for (int q = tid; q < nord; q += BLOCK) order[q] = d.order[w0.pair_off + q];
+    int* cs = order + nord;   // colour starts (ncol + 1 <= GP + 1 entries fit the layout's bound)
+    for (int q = tid; q <= ncol; q += BLOCK) cs[q] = d.cstart[w0.cs_off + q];
+    __syncthreads();
+    // Thread tid solves the pairs at order slots cs[c] + tid + k * BLOCK of every colour c in every
+    // iteration, so it alone reads and writes their rows: the first row of its first pair of the
+    // next colour is fetched right after its pairs of this colour are done, ahead of the barrier.
+    const float* R = d.ps.rows;
+    float cur[RW];
+    int pend = -1;   // local pair whose first row cur holds
+    auto fetch = [&](int c) {
+        const int q = cs[c] + tid;
+        pend = q < cs[c + 1] ? order[q] : -1;
+        if (pend >= 0) prefetch_pair_row(L, pend, R + (size_t)gidx[pend] * 8 * RW, cur);
+    };
+    if (ncol > 0) fetch(0);
+    for (int it = 0; it < d.iterations; it++) {
+        for (int c = 0; c < ncol; c++) {
+            if (pend >= 0) {
+                iterate_pair_lds(L, pend, d.ps.rows + (size_t)gidx[pend] * 8 * RW, cur);
+                for (int q = cs[c] + tid + BLOCK; q < cs[c + 1]; q += BLOCK) {
+                    const int lp = order[q];
+                    float* rows = d.ps.rows + (size_t)gidx[lp] * 8 * RW;
+                    prefetch_pair_row(L, lp, rows, cur);
+                    iterate_pair_lds(L, lp, rows, cur);
                 }
-                __syncthreads();
             }
+            const bool last = it + 1 == d.iterations && c + 1 == ncol;
+            if (!last) fetch(c + 1 == ncol ? 0 : c + 1);
+            __syncthreads();
         }
-        PE_PHASE(7);
-        // syncTempVelocity from the LDS copy
-        for (int i = tid; i < n; i += BLOCK) {
-            if (L.bs[6 * n + i] < 0.0f) continue;
-            st3(b, F_V, off + i, f3{L.bs[i], L.bs[n + i], L.bs[2 * n + i]});
-            st3(b, F_W, off + i, f3{L.bs[3 * n + i], L.bs[4 * n + i], L.bs[5 * n + i]});
-        }
-    } else {
-        for (int it = 0; it < d.iterations; it++) {
-            for (int c = 0; c < ncol; c++) {
-                const int c0 = csr[c], c1 = csr[c + 1];
-                for (int q = c0 + tid; q < c1; q += BLOCK) iterate_pair(b, d.ps, order[q]);
-                __syncthreads();
-            }
-        }
-        PE_PHASE(7);
-        for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
+    }
+    PE_PHASE(7);
+    for (int i = tid; i < GB; i += BLOCK) {
+        if (L.bs[6 * GB + i] < 0.0f) continue;
+        st3(b, F_V, B0 + i, f3{L.bs[i], L.bs[GB + i], L.bs[2 * GB + i]});
+        st3(b, F_W, B0 + i, f3{L.bs[3 * GB + i], L.bs[4 * GB + i], L.bs[5 * GB + i]});
     }
 }
 
@@ -397,7 +544,8 @@ int body_pair_cap(int n) { return 8 * n + 64; }
 template <int B> void set_lds_one() {
     const int bytes = 150 * 1024;
     hipFuncSetAttribute((const void*)k_world_broad<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_world_solve<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_group_prep<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_group_sweep<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 void set_lds_limits() {
     static bool done = false;
@@ -407,43 +555,96 @@ void set_lds_limits() {
 }
 
 // Launches K1 (phase 0) or K3 (phase 1) of one class: dispatch over the block sizes.
+// K1 of a class: one workgroup per world of wl[0..count).  K3: one workgroup per group of
+// groups[0..ngroups) (first index into the world list, world count).
 template <int B>
-void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, int rows_x10, hipStream_t s) {
+void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups, int ngroups,
+                int group_bodies, hipStream_t s) {
     if (phase == 0) {
         hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
     } else {
-        const size_t lb = solve_lds_bytes(nmax, rows_x10);
-        hipLaunchKernelGGL(k_world_solve<B>, dim3(count), dim3(B), lb, s, d, wl, nmax, (int)lb);
+        const size_t lb = solve_lds_bytes(nmax, group_bodies);
+        hipLaunchKernelGGL(k_group_prep<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
+        hipLaunchKernelGGL(k_group_sweep<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
     }
 }
-hipError_t launch_class(int phase, int block, int rows_x10, const BatchDev& d, const int32_t* wl, int count, int nmax,
-                        hipStream_t s) {
+hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups,
+                        int ngroups, int group_bodies, hipStream_t s) {
     switch (block) {
-        case 64: launch_one<64>(phase, d, wl, count, nmax, rows_x10, s); break;
-        case 128: launch_one<128>(phase, d, wl, count, nmax, rows_x10, s); break;
-        case 256: launch_one<256>(phase, d, wl, count, nmax, rows_x10, s); break;
-        case 512: launch_one<512>(phase, d, wl, count, nmax, rows_x10, s); break;
-        default: launch_one<1024>(phase, d, wl, count, nmax, rows_x10, s); break;
+        case 64: launch_one<64>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
+        case 128: launch_one<128>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
+        case 256: launch_one<256>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
+        case 512: launch_one<512>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
+        default: launch_one<1024>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
     }
     return hipGetLastError();
 }
 
-constexpr int DEFAULT_ROWS_X10 = 25;   // solver rows kept in LDS per body of the class's largest world, x10
+constexpr int DEFAULT_GROUP_BODIES = 256;   // bodies solved together per K3 workgroup
+
 
 struct Class {
     int lo, hi;         // body-count range [lo, hi]
     int block;
-    int rows_x10 = DEFAULT_ROWS_X10;
+    int group_bodies = DEFAULT_GROUP_BODIES;   // K3 groups: consecutive worlds up to this many bodies
     int count = 0;      // worlds in the class
     int nmax = 0;
     int list_off = 0;   // offset into the device world list
+    int group_off = 0;  // offset into the device group list
+    int ngroups = 0;
 };
 
-// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class.
-// ev (optional, 4 events) is recorded before K1, after K1, after K2 and after K3.
-// Per-launch timing: an event pair around every kernel of the step, tagged with its stage
-// (0 = K1 integrate + broadphase, 1 = K2 narrowphase, 2 = K3 solve), so a stage's device time is
-// the sum of its kernels' durations without the gaps between launches.
+// Groups of consecutive class members (first index into the world list, count) holding at most
+// group_bodies bodies and MAX_GROUP_WORLDS worlds each (at least one world); group_bodies = 0
+// gives one world per group.
+void build_groups(Class& c, const std::vector<int32_t>& members, const std::vector<int>& nbodies, bool contiguous,
+                  std::vector<int2>& out) {
+    c.group_off = (int)out.size();
+    int k = 0;
+    const int m = (int)members.size();
+    while (k < m) {
+        int cnt = 1, tot = nbodies[members[k]];
+        if (contiguous && c.group_bodies > 0)
+            while (k + cnt < m && cnt < MAX_GROUP_WORLDS && tot + nbodies[members[k + cnt]] <= c.group_bodies) tot += nbodies[members[k + cnt++]];
+        out.push_back(make_int2(c.list_off + k, cnt));
+        k += cnt;
+    }
+    c.ngroups = (int)out.size() - c.group_off;
+}
+
+// Side streams the size classes of K1 and K3 run on concurrently (a class's launch alone rarely
+// fills the GPU: its runtime is set by its largest worlds' dependent passes, not by its work).
+// Classes are dealt round-robin onto the main stream and up to MAX_SIDE side streams, forked
+// from and joined back to the main stream by events, so the step stays one ordered unit on the
+// main stream (and a captured graph holds the same fork / join dependencies).
+constexpr int MAX_SIDE = 3;
+struct ClassStreams {
+    hipStream_t side[MAX_SIDE] = {};
+    hipEvent_t fork = nullptr, join[MAX_SIDE] = {};
+    int n = 0;
+    int create() {
+        if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return PE_ERR_DEVICE;
+        for (; n < MAX_SIDE; n++) {
+            if (hipStreamCreateWithFlags(&side[n], hipStreamNonBlocking) != hipSuccess) return PE_ERR_DEVICE;
+            if (hipEventCreateWithFlags(&join[n], hipEventDisableTiming) != hipSuccess) return PE_ERR_DEVICE;
+        }
+        return PE_OK;
+    }
+    void release() {
+        for (int k = 0; k < n; k++) {
+            if (side[k]) { hipStreamSynchronize(side[k]); hipStreamDestroy(side[k]); }
+            if (join[k]) hipEventDestroy(join[k]);
+        }
+        if (fork) hipEventDestroy(fork);
+        n = 0;
+        fork = nullptr;
+    }
+};
+
+// Per-stage timing: an event pair on the main stream around each stage (0 = K1 integrate +
+// broadphase of all classes, 1 = K2 narrowphase, 2 = K3 solve of all classes), so a stage's
+// device time is its span from before its first launch to after its last, without the gaps
+// between stages.
 struct LaunchTimer {
     std::vector<hipEvent_t> ev;
     std::vector<int> stage;
@@ -465,26 +666,38 @@ struct LaunchTimer {
     hipError_t end(hipStream_t s) { hipError_t r = hipEventRecord(ev[used + 1], s); used += 2; return r; }
 };
 
-int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, hipStream_t s, bool shapes,
-               LaunchTimer* tm = nullptr) {
+// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class;
+// the classes of K1 and of K3 run concurrently on the main stream s and the side streams.
+int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, const int2* dgroups, hipStream_t s,
+               const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr) {
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
+    // heaviest classes first: largest worlds, whose passes take longest
+    int ord[16], na = 0;
+    for (int k = (int)classes.size() - 1; k >= 0 && na < 16; k--)
+        if (classes[k].count > 0) ord[na++] = k;
+    const int nstreams = std::min(na, cs.n + 1);
     for (int phase = 0; phase < 2; phase++) {
-        for (auto& c : classes) {
-            if (c.count == 0) continue;
-            if (tm) tm->begin(phase == 0 ? 0 : 2, s);
-            hipError_t e = launch_class(phase, c.block, c.rows_x10, d, dlist + c.list_off, c.count, c.nmax, s);
-            if (tm) tm->end(s);
+        if (tm) tm->begin(phase == 0 ? 0 : 2, s);
+        if (nstreams > 1) hipEventRecord(cs.fork, s);
+        for (int k = 1; k < nstreams; k++) hipStreamWaitEvent(cs.side[k - 1], cs.fork, 0);
+        for (int k = 0; k < na; k++) {
+            const Class& c = classes[ord[k]];
+            hipStream_t t = k % nstreams == 0 ? s : cs.side[k % nstreams - 1];
+            hipError_t e = phase == 0 ? launch_class(0, c.block, d, dlist + c.list_off, c.count, c.nmax, nullptr, 0, 0, t)
+                                      : launch_class(1, c.block, d, dlist, c.count, c.nmax, dgroups + c.group_off, c.ngroups,
+                                                     c.group_bodies, t);
             if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
         }
+        for (int k = 1; k < nstreams; k++) {
+            hipEventRecord(cs.join[k - 1], cs.side[k - 1]);
+            hipStreamWaitEvent(s, cs.join[k - 1], 0);
+        }
+        if (tm) tm->end(s);
         if (phase == 0) {
             if (tm) tm->begin(1, s);
             hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
+            if (shapes) hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
             if (tm) tm->end(s);
-            if (shapes) {
-                if (tm) tm->begin(1, s);
-                hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
-                if (tm) tm->end(s);
-            }
         }
     }
     hipError_t e = hipGetLastError();
@@ -510,14 +723,17 @@ struct pe_batch {
     std::vector<void*> allocs;
     std::vector<Class> classes;
     int32_t* dlist = nullptr;      // world ids grouped by class
+    int2* dgroups = nullptr;       // K3 groups per class
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int64_t dyn_bodies = 0;
 
     DevMeshes meshes;
     bool has_shapes = false;   // any cylinder / convex / concave body
+    ClassStreams side;
     ~pe_batch() {
         if (stream) hipStreamSynchronize(stream);
+        side.release();
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (void* p : allocs) hipFree(p);
@@ -541,6 +757,9 @@ struct pe_batch {
                 if (n >= classes[c].lo && n <= classes[c].hi) { members[c].push_back(w); break; }
         }
         std::vector<int32_t> list;
+        std::vector<int2> grp;
+        std::vector<int> nb(nworlds);
+        for (int w = 0; w < nworlds; w++) nb[w] = hdesc[w].n;
         for (size_t c = 0; c < classes.size(); c++) {
             auto& m = members[c];
             std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
@@ -548,12 +767,18 @@ struct pe_batch {
             classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
             classes[c].list_off = (int)list.size();
             list.insert(list.end(), m.begin(), m.end());
+            // the layout follows this order (largest worlds first, ties by index), so consecutive
+            // members are contiguous in the body and pair stores
+            build_groups(classes[c], m, nb, true, grp);
         }
         if (!dlist && alloc(&dlist, std::max(1, nworlds)) != PE_OK) return PE_ERR_DEVICE;
+        if (!dgroups && alloc(&dgroups, std::max(1, nworlds)) != PE_OK) return PE_ERR_DEVICE;
         PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        PE_HIP(hipMemcpyAsync(dgroups, grp.data(), grp.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+        PE_HIP(hipStreamSynchronize(stream));
         return PE_OK;
     }
-    int launch_step() { return launch_all(d, classes, dlist, stream, has_shapes); }
+    int launch_step() { return launch_all(d, classes, dlist, dgroups, stream, side, has_shapes); }
     void drop_graph() {
         if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
         if (graph) { hipGraphDestroy(graph); graph = nullptr; }
@@ -579,16 +804,25 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     set_lds_limits();
     auto* B = new pe_batch();
     auto fail = [&](int rc) { delete B; return rc; };
-    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) { pe_set_error("stream"); return fail(PE_ERR_DEVICE); }
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess || B->side.create() != PE_OK) {
+        pe_set_error("stream");
+        return fail(PE_ERR_DEVICE);
+    }
     B->nworlds = nworlds;
     B->hdesc.resize(nworlds);
     int boff = 0;
     int64_t poff = 0;
-    for (int w = 0; w < nworlds; w++) {
+    // lay worlds out largest first (ties by index), the order of every size class's world list
+    std::vector<int> lay(nworlds);
+    std::iota(lay.begin(), lay.end(), 0);
+    std::stable_sort(lay.begin(), lay.end(), [&](int a, int b) { return scenes[a].n > scenes[b].n; });
+    int rank = 0;
+    for (int w : lay) {
         B->hdesc[w].body_off = boff;
         B->hdesc[w].n = scenes[w].n;
         B->hdesc[w].pair_off = (int32_t)poff;
         B->hdesc[w].pair_cap = body_pair_cap(scenes[w].n);
+        B->hdesc[w].cs_off = (int32_t)poff + rank++;
         boff += scenes[w].n;
         poff += B->hdesc[w].pair_cap;
     }
@@ -674,7 +908,7 @@ int pe_batch_set_config(pe_batch* B, int32_t nclasses, const int32_t* upper, con
         if (bs != 64 && bs != 128 && bs != 256 && bs != 512 && bs != 1024) { pe_set_error("block size must be 64..1024, power of 2"); return PE_ERR_INVALID; }
         const int hi = c == nclasses - 1 ? (1 << 30) : upper[c];
         if (hi < lo) { pe_set_error("class bounds must increase"); return PE_ERR_INVALID; }
-        cl.push_back(Class{lo, hi, bs, c < (int)B->classes.size() ? B->classes[c].rows_x10 : DEFAULT_ROWS_X10});
+        cl.push_back(Class{lo, hi, bs, c < (int)B->classes.size() ? B->classes[c].group_bodies : DEFAULT_GROUP_BODIES});
         lo = hi + 1;
     }
     B->classes = cl;
@@ -682,21 +916,21 @@ int pe_batch_set_config(pe_batch* B, int32_t nclasses, const int32_t* upper, con
     return B->assign_classes();
 }
 
-int pe_batch_set_solver_lds(pe_batch* B, int32_t nclasses, const int32_t* rows_x10) {
-    if (!B || !rows_x10 || nclasses != (int32_t)B->classes.size()) { pe_set_error("one value per class expected"); return PE_ERR_INVALID; }
+int pe_batch_set_solver_groups(pe_batch* B, int32_t nclasses, const int32_t* group_bodies) {
+    if (!B || !group_bodies || nclasses != (int32_t)B->classes.size()) { pe_set_error("one value per class expected"); return PE_ERR_INVALID; }
     for (int c = 0; c < nclasses; c++) {
-        if (rows_x10[c] < 0 || rows_x10[c] > 640) { pe_set_error("rows per body x10 must be 0..640"); return PE_ERR_INVALID; }
-        B->classes[c].rows_x10 = rows_x10[c];
+        if (group_bodies[c] < 0 || group_bodies[c] > 2048) { pe_set_error("group bodies must be 0..2048"); return PE_ERR_INVALID; }
+        B->classes[c].group_bodies = group_bodies[c];
     }
     B->drop_graph();
-    return PE_OK;
+    return B->assign_classes();
 }
 
-int pe_batch_get_solver_lds(pe_batch* B, int32_t* nclasses, int32_t* rows_x10, int32_t cap) {
+int pe_batch_get_solver_groups(pe_batch* B, int32_t* nclasses, int32_t* group_bodies, int32_t cap) {
     if (!B || !nclasses) { pe_set_error("null argument"); return PE_ERR_INVALID; }
     *nclasses = (int32_t)B->classes.size();
     for (int c = 0; c < (int)B->classes.size() && c < cap; c++)
-        if (rows_x10) rows_x10[c] = B->classes[c].rows_x10;
+        if (group_bodies) group_bodies[c] = B->classes[c].group_bodies;
     return PE_OK;
 }
 
@@ -811,7 +1045,7 @@ extern "C" int pe_batch_stage_times(pe_batch* B, int32_t nsteps, double* ms /*3*
     LaunchTimer tm;
     for (int s = 0; s < nsteps; s++) {
         tm.used = 0;
-        if (int rc = launch_all(B->d, B->classes, B->dlist, B->stream, B->has_shapes, &tm)) return rc;
+        if (int rc = launch_all(B->d, B->classes, B->dlist, B->dgroups, B->stream, B->side, B->has_shapes, &tm)) return rc;
         PE_HIP(hipEventSynchronize(tm.ev[tm.used - 1]));
         for (int k = 0; k < tm.used / 2; k++) {
             float t = 0;
@@ -939,6 +1173,7 @@ struct pe_server {
     float* dresult = nullptr;
     int64_t result_cap = 0;
     int32_t* dlist = nullptr;
+    int2* dgroups = nullptr;
     std::vector<Class> classes;
     bool dirty = true;
     int64_t live_footprint = 0;
@@ -949,9 +1184,11 @@ struct pe_server {
 
     DevMeshes meshes;   // the unit cylinder mesh (requests carry no convex / concave meshes)
     bool has_shapes = false;   // a cylinder was ever submitted
+    ClassStreams side;
     ~pe_server() {
         if (cs) hipStreamSynchronize(cs);
         if (xs) hipStreamSynchronize(xs);
+        side.release();
         for (void* p : allocs) hipFree(p);
         meshes.release();
         for (auto& e : ev_upload) if (e) hipEventDestroy(e);
@@ -976,6 +1213,9 @@ struct pe_server {
                 if (r->n >= classes[c].lo && r->n <= classes[c].hi) { members[c].push_back(r->slot); break; }
         }
         std::vector<int32_t> list;
+        std::vector<int2> grp;
+        std::vector<int> nb(hdesc.size());
+        for (size_t w = 0; w < hdesc.size(); w++) nb[w] = hdesc[w].n;
         for (size_t c = 0; c < classes.size(); c++) {
             auto& m = members[c];
             std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
@@ -983,8 +1223,10 @@ struct pe_server {
             classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
             classes[c].list_off = (int)list.size();
             list.insert(list.end(), m.begin(), m.end());
+            build_groups(classes[c], m, nb, false, grp);   // arena ranges are not contiguous: one world per group
         }
         if (!list.empty()) PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * 4, hipMemcpyHostToDevice, cs));
+        if (!grp.empty()) PE_HIP(hipMemcpyAsync(dgroups, grp.data(), grp.size() * sizeof(int2), hipMemcpyHostToDevice, cs));
         PE_HIP(hipStreamSynchronize(cs));  // the host list buffer is reused
         dirty = false;
         return PE_OK;
@@ -1029,7 +1271,8 @@ struct pe_server {
         PE_HIP(hipStreamWaitEvent(cs, ev_upload[buf], 0));
         for (Req* r : prepared) {
             const int64_t soff = r->stage_off;
-            WorldDesc wd{(int32_t)r->boff, r->n, (int32_t)r->poff, r->pcap};
+            // the pair range holds pcap + 1 slots: its start doubles as the colour-start region
+            WorldDesc wd{(int32_t)r->boff, r->n, (int32_t)r->poff, r->pcap, (int32_t)r->poff, {0, 0, 0}};
             hdesc[r->slot] = wd;
             PE_HIP(hipMemcpyAsync((void*)(d.desc + r->slot), &hdesc[r->slot], sizeof(WorldDesc), hipMemcpyHostToDevice, cs));
             hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, cs, d.st, r->slot);
@@ -1063,7 +1306,7 @@ struct pe_server {
         int64_t nb = 0;
         for (Req* r : live) nb += r->n;
         if (!live.empty())
-            if (int rc = launch_all(d, classes, dlist, cs, has_shapes)) return rc;
+            if (int rc = launch_all(d, classes, dlist, dgroups, cs, side, has_shapes)) return rc;
         PE_HIP(hipEventRecord(ev_done, cs));
         // overlap: next admissions upload while this step runs
         stage_buf ^= 1;
@@ -1126,7 +1369,10 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     auto fail = [&](int rc) { delete S; return rc; };
     S->cfg = *cfg;
     if (hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&S->xs, hipStreamNonBlocking) != hipSuccess) { pe_set_error("streams"); return fail(PE_ERR_DEVICE); }
+        hipStreamCreateWithFlags(&S->xs, hipStreamNonBlocking) != hipSuccess || S->side.create() != PE_OK) {
+        pe_set_error("streams");
+        return fail(PE_ERR_DEVICE);
+    }
     // arena sizes from the budget: per body its store and 8 pair slots of contacts and rows
     const int64_t per_body = req_footprint(1024) / 1024 + 1;
     S->stage_cap = (int64_t)cfg->max_world_bodies * NSTAGE * 8;            // floats per staging buffer
@@ -1174,6 +1420,7 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&S->dstage[1], S->stage_cap);
     rc |= S->alloc(&S->dresult, S->result_cap);
     rc |= S->alloc(&S->dlist, cfg->max_worlds);
+    rc |= S->alloc(&S->dgroups, cfg->max_worlds);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
     {
         HostMeshes none;
@@ -1184,7 +1431,7 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     d.st = dst;
     S->bodies.init(S->max_bodies);
     S->pairs.init(S->max_pairs);
-    S->hdesc.assign(cfg->max_worlds, WorldDesc{0, 0, 0, 0});
+    S->hdesc.assign(cfg->max_worlds, WorldDesc{0, 0, 0, 0, 0, {0, 0, 0}});
     for (int w = cfg->max_worlds - 1; w >= 0; w--) S->free_slots.push_back(w);
     S->classes = default_classes();
     for (auto& e : S->ev_upload) if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(PE_ERR_DEVICE);

@@ -428,14 +428,13 @@ __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore&
     }
 }
 
-// Solver state of a world held in LDS for the Gauss-Seidel sweeps: per body (local index) the
-// temp velocities, the inverse mass (negative for a kinematic body) and the world inverse
-// inertia, structure of arrays; per pair its local bodies and first row; rows in RW layout.
+// Solver state held in LDS for the Gauss-Seidel sweeps: per body (local index) the temp
+// velocities, the inverse mass (negative for a kinematic body) and the world inverse inertia,
+// structure of arrays; per pair its local bodies and first row (row counts are differences).
 struct LdsSolve {
     float* bs;          // [16][n]: tv xyz, tw xyz, im (or -1: kinematic), Iw 9
     int* pab;           // [npairs] local A | local B << 16
     int* rs;            // [npairs + 1] first row of each pair
-    float* rows;        // [npts][RW]
     int n;
 };
 __device__ __forceinline__ void lds_load_body(const LdsSolve& L, const BodyView& b, int gi, int li) {
@@ -450,8 +449,16 @@ __device__ __forceinline__ void lds_load_body(const LdsSolve& L, const BodyView&
     for (int k = 0; k < 9; k++) L.bs[(7 + k) * n + li] = r[SB_IW + k];
 }
 // iterate_pair on the LDS copy of the bodies (same arithmetic, same order); the pair's rows are
-// at rows (LDS or the global row store), RW floats apart.
-__device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, float* rows) {
+// at rows (the row store), RW floats apart.
+// Loads the first solver row of local pair pl (nothing for a pair without rows) into cur, ahead
+// of iterate_pair_lds: the sweep issues it one colour early so its latency overlaps the barrier.
+__device__ __forceinline__ void prefetch_pair_row(const LdsSolve& L, int pl, const float* rows, float (&cur)[RW]) {
+    if (L.rs[pl + 1] > L.rs[pl]) ld_rec<RW / 4>(rows, cur);
+}
+
+// One Gauss-Seidel pass over the rows of local pair pl on the LDS body state; cur holds its first
+// row (prefetch_pair_row); each later row is loaded while the one before it is solved.
+__device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, float* rows, float (&cur)[RW]) {
     const int ab = L.pab[pl];
     const int a = ab & 0xffff, c = ab >> 16;
     const int nr = L.rs[pl + 1] - L.rs[pl];
@@ -464,21 +471,25 @@ __device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, floa
 #pragma unroll
     for (int k = 0; k < 9; k++) { Ia.m[k] = bs[(7 + k) * n + a]; Ib.m[k] = bs[(7 + k) * n + c]; }
     for (int k = 0; k < nr; k++) {
-        float* o = rows + (size_t)k * RW;
-        float rr[RW];
-        ld_rec<RW / 4>(o, rr);
+        float nxt[RW];
+        const bool more = k + 1 < nr;
+        if (more) ld_rec<RW / 4>(rows + (size_t)(k + 1) * RW, nxt);
         Row r;
-        r.ra = f3{rr[R_RA], rr[R_RA + 1], rr[R_RA + 2]};
-        r.rb = f3{rr[R_RB], rr[R_RB + 1], rr[R_RB + 2]};
-        r.n = f3{rr[R_N], rr[R_N + 1], rr[R_N + 2]};
-        r.t0 = f3{rr[R_T0], rr[R_T0 + 1], rr[R_T0 + 2]};
-        r.t1 = f3{rr[R_T1], rr[R_T1 + 1], rr[R_T1 + 2]};
-        r.rhs = rr[R_RHS]; r.mn = rr[R_MN]; r.mt0 = rr[R_MT0]; r.mt1 = rr[R_MT1]; r.mu = rr[R_MU];
-        float ln = rr[R_LN], lt0 = rr[R_LT0], lt1 = rr[R_LT1];
+        r.ra = f3{cur[R_RA], cur[R_RA + 1], cur[R_RA + 2]};
+        r.rb = f3{cur[R_RB], cur[R_RB + 1], cur[R_RB + 2]};
+        r.n = f3{cur[R_N], cur[R_N + 1], cur[R_N + 2]};
+        r.t0 = f3{cur[R_T0], cur[R_T0 + 1], cur[R_T0 + 2]};
+        r.t1 = f3{cur[R_T1], cur[R_T1 + 1], cur[R_T1 + 2]};
+        r.rhs = cur[R_RHS]; r.mn = cur[R_MN]; r.mt0 = cur[R_MT0]; r.mt1 = cur[R_MT1]; r.mu = cur[R_MU];
+        float ln = cur[R_LN], lt0 = cur[R_LT0], lt1 = cur[R_LT1];
         const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
         if (ima >= 0.0f) apply_temp(tva, twa, ima, Ia, r.ra, J);
         if (imb >= 0.0f) apply_temp(tvb, twb, imb, Ib, r.rb, neg(J));
-        reinterpret_cast<float4*>(o)[5] = make_float4(ln, lt0, lt1, 0.0f);
+        reinterpret_cast<float4*>(rows + (size_t)k * RW)[5] = make_float4(ln, lt0, lt1, 0.0f);
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < RW; u++) cur[u] = nxt[u];
+        }
     }
     if (ima >= 0.0f) {
         float* q = L.bs + a;
@@ -596,6 +607,95 @@ __device__ int colour_pairs(const BodyView& b, const PairStore& ps, int* bufs[3]
             const int p = order[q];
             claim[ps.pa[p] - off] = FREE;
             claim[ps.pb[p] - off] = FREE;
+        }
+        const int ndef = cnt[1];
+        __syncthreads();
+        ia = (ia + 2) % 3;
+        nact = ndef;
+        ncol++;
+    }
+    if (tid == 0) cs[ncol] = cnt[0];
+    __syncthreads();
+    return ncol;
+}
+
+
+// colour_pairs on LDS data for a group of worlds solved together: pairs by local index, their
+// bodies by local index (pab: A | B << 16), kinematic bodies have a negative inverse mass in
+// bs[6 n + i].  Bodies of different worlds are disjoint, so the colours of the group are the
+// union of each world's own colours (the same sub-rounds, the same claims) and every world's
+// colour classes are those it gets alone.  All lists live in LDS; cs needs ncolours + 1 slots.
+__device__ int colour_group(const int* pab, const float* imk, int* bufs[3], int nact, int* order, int* cs,
+                            int* claim) {
+    using namespace colour;
+    __shared__ int cnt[3];
+    const int tid = threadIdx.x, B = blockDim.x;
+    int ia = 0, ncol = 0;
+    if (tid == 0) cnt[0] = 0;
+    __syncthreads();
+    while (nact > 0) {
+        int* cand = bufs[ia];
+        int* nxt = bufs[(ia + 1) % 3];
+        int* def = bufs[(ia + 2) % 3];
+        int ncand = nact;
+        const int col_begin = cnt[0];
+        if (tid == 0) { cs[ncol] = col_begin; cnt[1] = 0; }
+        __syncthreads();
+        while (ncand > 0) {
+            for (int q = tid; q < ncand; q += B) {
+                const int p = cand[q];
+                const int a = pab[p] & 0xffff, c = pab[p] >> 16;
+                if (imk[a] >= 0.0f) atomicMin(&claim[a], p);
+                if (imk[c] >= 0.0f) atomicMin(&claim[c], p);
+            }
+            if (tid == 0) cnt[2] = 0;
+            __syncthreads();
+            const int sub_begin = cnt[0];
+            for (int base = 0; base < ncand; base += B) {
+                const int q = base + tid;
+                bool win = false, blk = false, stay = false;
+                int p = -1;
+                if (q < ncand) {
+                    p = cand[q];
+                    const int a = pab[p] & 0xffff, c = pab[p] >> 16;
+                    const bool ka = imk[a] < 0.0f, kc = imk[c] < 0.0f;
+                    const int ca = ka ? 0 : claim[a];
+                    const int cc = kc ? 0 : claim[c];
+                    blk = (!ka && ca == TAKEN) || (!kc && cc == TAKEN);
+                    win = !blk && (ka || ca == p) && (kc || cc == p);
+                    stay = !blk && !win;
+                }
+                const int iw = wave_append(win, &cnt[0]);
+                const int ib = wave_append(blk, &cnt[1]);
+                const int is = wave_append(stay, &cnt[2]);
+                if (win) order[iw] = p;
+                if (blk) def[ib] = p;
+                if (stay) nxt[is] = p;
+            }
+            __syncthreads();
+            const int sub_end = cnt[0], nstay = cnt[2];
+            for (int q = tid; q < ncand; q += B) {
+                const int p = cand[q];
+                const int a = pab[p] & 0xffff, c = pab[p] >> 16;
+                if (imk[a] >= 0.0f && claim[a] != TAKEN) claim[a] = FREE;
+                if (imk[c] >= 0.0f && claim[c] != TAKEN) claim[c] = FREE;
+            }
+            __syncthreads();
+            for (int q = sub_begin + tid; q < sub_end; q += B) {
+                const int p = order[q];
+                const int a = pab[p] & 0xffff, c = pab[p] >> 16;
+                if (imk[a] >= 0.0f) claim[a] = TAKEN;
+                if (imk[c] >= 0.0f) claim[c] = TAKEN;
+            }
+            __syncthreads();
+            int* t = cand; cand = nxt; nxt = t;
+            ncand = nstay;
+        }
+        const int col_end = cnt[0];
+        for (int q = col_begin + tid; q < col_end; q += B) {
+            const int p = order[q];
+            claim[pab[p] & 0xffff] = FREE;
+            claim[pab[p] >> 16] = FREE;
         }
         const int ndef = cnt[1];
         __syncthreads();

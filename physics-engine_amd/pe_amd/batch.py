@@ -79,14 +79,14 @@ class Batch:
         check(lib().pe_batch_get_config(self.h, ctypes.byref(nc), u, bl, cnt, 32))
         return [(u[i], bl[i], cnt[i]) for i in range(nc.value)]
 
-    def set_solver_lds(self, rows_x10):
-        r = (ctypes.c_int32 * len(rows_x10))(*rows_x10)
-        check(lib().pe_batch_set_solver_lds(self.h, ctypes.c_int32(len(rows_x10)), r))
+    def set_solver_groups(self, group_bodies):
+        r = (ctypes.c_int32 * len(group_bodies))(*group_bodies)
+        check(lib().pe_batch_set_solver_groups(self.h, ctypes.c_int32(len(group_bodies)), r))
 
-    def solver_lds(self):
+    def solver_groups(self):
         nc = ctypes.c_int32()
         r = (ctypes.c_int32 * 32)()
-        check(lib().pe_batch_get_solver_lds(self.h, ctypes.byref(nc), r, 32))
+        check(lib().pe_batch_get_solver_groups(self.h, ctypes.byref(nc), r, 32))
         return [r[i] for i in range(nc.value)]
 
     def world_state(self, w: int):
@@ -110,16 +110,18 @@ class Batch:
         self.sync()
         return time.perf_counter() - t0
 
-    def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None, lds_candidates=(0, 15, 25, 40)):
+    def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None,
+                 group_candidates=(0, 128, 256, 512, 1024)):
         """Per size class (other classes at their current setting), times every candidate block
-        size, then every candidate solver LDS row budget (rows per body x10), and keeps the
-        fastest of each; returns {class upper bound: (block, rows_x10)}.  Stepping during tuning
-        advances the worlds (it is real simulation work); results never depend on the choice."""
+        size, then every candidate solver group size (bodies per solve workgroup), and keeps the
+        fastest of each; returns {class upper bound: (block, group bodies)}.  Stepping during
+        tuning advances the worlds (it is real simulation work); results never depend on the
+        choice."""
         cur = self.config()
         uppers = [c[0] for c in cur] if classes is None else list(classes)
         blocks = [c[1] for c in cur] if classes is None else [256] * len(uppers)
         self.set_config(uppers, blocks)
-        rows = self.solver_lds()
+        rows = self.solver_groups()
         counts = [c[2] for c in self.config()]
         for ci in range(len(uppers)):
             if counts[ci] == 0:
@@ -129,21 +131,21 @@ class Batch:
                 trial = list(blocks)
                 trial[ci] = cand
                 self.set_config(uppers, trial)
-                self.set_solver_lds(rows)
+                self.set_solver_groups(rows)
                 t = self._time_steps(steps)
                 if best_t is None or t < best_t:
                     best, best_t = cand, t
             blocks[ci] = best
             self.set_config(uppers, blocks)
             best, best_t = rows[ci], None
-            for cand in lds_candidates:
+            for cand in group_candidates:
                 trial = list(rows)
                 trial[ci] = cand
-                self.set_solver_lds(trial)
+                self.set_solver_groups(trial)
                 t = self._time_steps(steps)
                 if best_t is None or t < best_t:
                     best, best_t = cand, t
             rows[ci] = best
         self.set_config(uppers, blocks)
-        self.set_solver_lds(rows)
+        self.set_solver_groups(rows)
         return {u: (b, r) for u, b, r in zip(uppers, blocks, rows)}

@@ -51,12 +51,12 @@ def test_batch_graph_and_config_do_not_change_results(worlds):
     b.sync()
     for i in range(len(worlds)):
         assert np.array_equal(flat_state(a.world_state(i)), flat_state(b.world_state(i))), f"world {i}"
-    # the solver's LDS budget (rows in LDS, in global memory, or bodies too in global memory)
-    # does not change a result either
-    for rows in ([0, 0, 0, 0], [640, 640, 640, 640], [10, 25, 0, 40]):
+    # the solver grouping (one world per workgroup, or several solved as one system) does not
+    # change a result either
+    for rows in ([0, 0, 0, 0], [2048, 2048, 2048, 2048], [64, 300, 0, 1000]):
         c = Batch(worlds)
         c.set_config([16, 48, 100, 4096], [64, 128, 512, 256])
-        c.set_solver_lds(rows)
+        c.set_solver_groups(rows)
         c.step(20)
         c.sync()
         for i in range(len(worlds)):
