@@ -61,6 +61,12 @@ int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
  * run concurrently on side streams), without the gaps between stages. */
 int pe_batch_stage_times(pe_batch* b, int32_t nsteps, double* ms /*3*/);
 
+/* Runs nsteps eagerly with every size class launched alone, in turn; ms[2c] and ms[2c + 1] are
+ * class c's summed integrate + broadphase and solver device milliseconds (cap >= 2 * classes).
+ * Each class's per-step total then sets where it runs in concurrent steps: classes are dealt,
+ * longest first, onto the least-loaded of the batch's streams.  Drops a captured graph. */
+int pe_batch_class_times(pe_batch* b, int32_t nsteps, double* ms, int32_t cap);
+
 /* ------------------------------------------------------------------ server
  * Continuous batching of world-simulation requests.  A request (a scene and a horizon in
  * steps) is queued by pe_server_submit.  Each pe_server_tick admits queued requests in

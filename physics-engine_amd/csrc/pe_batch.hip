@@ -449,9 +449,43 @@ __global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t*
     }
 }
 
+// The colour-ordered passes of a group: thread tid solves the pairs at order slots
+// cs[c] + tid + k * BLOCK of every colour c in every iteration, so it alone reads and writes their
+// rows.  pair(lp, true, cur) loads the first row of local pair lp into cur, pair(lp, false, cur)
+// solves the pair from it; the first row of a thread's first pair of the next colour is loaded
+// right after its pairs of this colour are done, ahead of the barrier.
+template <int BLOCK, class PairFn>
+__device__ __forceinline__ void sweep_passes(int iterations, int ncol, const int* cs, const int* order, int tid,
+                                             PairFn pair) {
+    float cur[RW];
+    int pend = -1;   // local pair whose first row cur holds
+    auto fetch = [&](int c) {
+        const int q = cs[c] + tid;
+        pend = q < cs[c + 1] ? order[q] : -1;
+        if (pend >= 0) pair(pend, true, cur);
+    };
+    if (ncol > 0) fetch(0);
+    for (int it = 0; it < iterations; it++) {
+        for (int c = 0; c < ncol; c++) {
+            if (pend >= 0) {
+                pair(pend, false, cur);
+                for (int q = cs[c] + tid + BLOCK; q < cs[c + 1]; q += BLOCK) {
+                    const int lp = order[q];
+                    pair(lp, true, cur);
+                    pair(lp, false, cur);
+                }
+            }
+            const bool last = it + 1 == iterations && c + 1 == ncol;
+            if (!last) fetch(c + 1 == ncol ? 0 : c + 1);
+            __syncthreads();
+        }
+    }
+}
+
 // K3b: the colour-ordered Gauss-Seidel sweeps of a prepared group on LDS copies of its bodies
-// and pairs (rows in the global row store), then syncTempVelocity.  Kept small so more waves
-// fit a SIMD: its passes are chains of dependent arithmetic separated by barriers.
+// and pairs (rows in the global row store: staging them in LDS measured slower, as it lowers the
+// workgroups a CU holds), then syncTempVelocity.  Kept small so more waves fit a SIMD: its passes
+// are chains of dependent arithmetic separated by barriers.
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_group_sweep(BatchDev d, const int32_t* __restrict__ wlist,
                                                        const int2* __restrict__ groups, int prep_lds_bytes) {
@@ -497,34 +531,11 @@ __global__ void __launch_bounds__(BLOCK) k_group_sweep(BatchDev d, const int32_t
     int* cs = order + nord;   // colour starts (ncol + 1 <= GP + 1 entries fit the layout's bound)
     for (int q = tid; q <= ncol; q += BLOCK) cs[q] = d.cstart[w0.cs_off + q];
     __syncthreads();
-    // Thread tid solves the pairs at order slots cs[c] + tid + k * BLOCK of every colour c in every
-    // iteration, so it alone reads and writes their rows: the first row of its first pair of the
-    // next colour is fetched right after its pairs of this colour are done, ahead of the barrier.
-    const float* R = d.ps.rows;
-    float cur[RW];
-    int pend = -1;   // local pair whose first row cur holds
-    auto fetch = [&](int c) {
-        const int q = cs[c] + tid;
-        pend = q < cs[c + 1] ? order[q] : -1;
-        if (pend >= 0) prefetch_pair_row(L, pend, R + (size_t)gidx[pend] * 8 * RW, cur);
-    };
-    if (ncol > 0) fetch(0);
-    for (int it = 0; it < d.iterations; it++) {
-        for (int c = 0; c < ncol; c++) {
-            if (pend >= 0) {
-                iterate_pair_lds(L, pend, d.ps.rows + (size_t)gidx[pend] * 8 * RW, cur);
-                for (int q = cs[c] + tid + BLOCK; q < cs[c + 1]; q += BLOCK) {
-                    const int lp = order[q];
-                    float* rows = d.ps.rows + (size_t)gidx[lp] * 8 * RW;
-                    prefetch_pair_row(L, lp, rows, cur);
-                    iterate_pair_lds(L, lp, rows, cur);
-                }
-            }
-            const bool last = it + 1 == d.iterations && c + 1 == ncol;
-            if (!last) fetch(c + 1 == ncol ? 0 : c + 1);
-            __syncthreads();
-        }
-    }
+    sweep_passes<BLOCK>(d.iterations, ncol, cs, order, tid, [&](int lp, bool first, float (&cur)[RW]) {
+        float* rows = d.ps.rows + (size_t)gidx[lp] * 8 * RW;
+        if (first) prefetch_pair_row(L, lp, rows, cur);
+        else iterate_pair_lds(L, lp, rows, cur);
+    });
     PE_PHASE(7);
     for (int i = tid; i < GB; i += BLOCK) {
         if (L.bs[6 * GB + i] < 0.0f) continue;
@@ -592,6 +603,8 @@ struct Class {
     int list_off = 0;   // offset into the device world list
     int group_off = 0;  // offset into the device group list
     int ngroups = 0;
+    double cost = -1.0; // measured K1 + K3 device time per step in ms (< 0: not measured)
+    int stream = 0;     // 0: the main stream, k > 0: side stream k - 1
 };
 
 // Groups of consecutive class members (first index into the world list, count) holding at most
@@ -666,38 +679,66 @@ struct LaunchTimer {
     hipError_t end(hipStream_t s) { hipError_t r = hipEventRecord(ev[used + 1], s); used += 2; return r; }
 };
 
-// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class;
-// the classes of K1 and of K3 run concurrently on the main stream s and the side streams.
+// Deals the classes onto nstreams streams, longest first onto the least-loaded stream (the
+// heaviest class gets the main stream).  A class's load is its measured cost, or before any
+// measurement its largest world's body count (the length of its longest chain of passes).
+void schedule_classes(std::vector<Class>& classes, int nstreams) {
+    std::vector<int> ord;
+    for (int c = 0; c < (int)classes.size(); c++)
+        if (classes[c].count > 0) ord.push_back(c);
+    bool measured = true;
+    for (int c : ord) measured = measured && classes[c].cost >= 0.0;
+    auto load = [&](int c) { return measured ? classes[c].cost : (double)classes[c].nmax; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return load(a) > load(b); });
+    std::vector<double> acc(std::max(1, nstreams), 0.0);
+    for (int c : ord) {
+        const int k = (int)(std::min_element(acc.begin(), acc.end()) - acc.begin());
+        classes[c].stream = k;
+        acc[k] += load(c);
+    }
+}
+
+// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class.
+// The classes of K1 and of K3 run concurrently on the main stream s and the side streams as
+// schedule_classes dealt them, heaviest first on each stream.  With class_tm, every class runs
+// on s alone, in turn, timed by its own event pair (stage 3 + 2 * class + phase).
 int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, const int2* dgroups, hipStream_t s,
-               const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr) {
+               const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr, bool class_tm = false) {
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
-    // heaviest classes first: largest worlds, whose passes take longest
-    int ord[16], na = 0;
-    for (int k = (int)classes.size() - 1; k >= 0 && na < 16; k--)
-        if (classes[k].count > 0) ord[na++] = k;
-    const int nstreams = std::min(na, cs.n + 1);
+    std::vector<int> ord;
+    int nused = 0;
+    for (int c = 0; c < (int)classes.size(); c++)
+        if (classes[c].count > 0) { ord.push_back(c); nused = std::max(nused, classes[c].stream + 1); }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return classes[a].nmax > classes[b].nmax; });
+    const bool fork = !class_tm && nused > 1 && nused <= cs.n + 1;
     for (int phase = 0; phase < 2; phase++) {
-        if (tm) tm->begin(phase == 0 ? 0 : 2, s);
-        if (nstreams > 1) hipEventRecord(cs.fork, s);
-        for (int k = 1; k < nstreams; k++) hipStreamWaitEvent(cs.side[k - 1], cs.fork, 0);
-        for (int k = 0; k < na; k++) {
-            const Class& c = classes[ord[k]];
-            hipStream_t t = k % nstreams == 0 ? s : cs.side[k % nstreams - 1];
+        if (tm && !class_tm) tm->begin(phase == 0 ? 0 : 2, s);
+        if (fork) {
+            hipEventRecord(cs.fork, s);
+            for (int k = 1; k < nused; k++) hipStreamWaitEvent(cs.side[k - 1], cs.fork, 0);
+        }
+        for (int ci : ord) {
+            const Class& c = classes[ci];
+            hipStream_t t = fork && c.stream > 0 ? cs.side[c.stream - 1] : s;
+            if (class_tm) tm->begin(3 + 2 * ci + phase, s);
             hipError_t e = phase == 0 ? launch_class(0, c.block, d, dlist + c.list_off, c.count, c.nmax, nullptr, 0, 0, t)
                                       : launch_class(1, c.block, d, dlist, c.count, c.nmax, dgroups + c.group_off, c.ngroups,
                                                      c.group_bodies, t);
+            if (class_tm) tm->end(s);
             if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
         }
-        for (int k = 1; k < nstreams; k++) {
-            hipEventRecord(cs.join[k - 1], cs.side[k - 1]);
-            hipStreamWaitEvent(s, cs.join[k - 1], 0);
+        if (fork) {
+            for (int k = 1; k < nused; k++) {
+                hipEventRecord(cs.join[k - 1], cs.side[k - 1]);
+                hipStreamWaitEvent(s, cs.join[k - 1], 0);
+            }
         }
-        if (tm) tm->end(s);
+        if (tm && !class_tm) tm->end(s);
         if (phase == 0) {
-            if (tm) tm->begin(1, s);
+            if (tm && !class_tm) tm->begin(1, s);
             hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
             if (shapes) hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
-            if (tm) tm->end(s);
+            if (tm && !class_tm) tm->end(s);
         }
     }
     hipError_t e = hipGetLastError();
@@ -776,6 +817,7 @@ struct pe_batch {
         PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         PE_HIP(hipMemcpyAsync(dgroups, grp.data(), grp.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
         PE_HIP(hipStreamSynchronize(stream));
+        schedule_classes(classes, side.n + 1);
         return PE_OK;
     }
     int launch_step() { return launch_all(d, classes, dlist, dgroups, stream, side, has_shapes); }
@@ -921,6 +963,7 @@ int pe_batch_set_solver_groups(pe_batch* B, int32_t nclasses, const int32_t* gro
     for (int c = 0; c < nclasses; c++) {
         if (group_bodies[c] < 0 || group_bodies[c] > 2048) { pe_set_error("group bodies must be 0..2048"); return PE_ERR_INVALID; }
         B->classes[c].group_bodies = group_bodies[c];
+        B->classes[c].cost = -1.0;
     }
     B->drop_graph();
     return B->assign_classes();
@@ -1053,6 +1096,30 @@ extern "C" int pe_batch_stage_times(pe_batch* B, int32_t nsteps, double* ms /*3*
             ms[tm.stage[k]] += t;
         }
     }
+    return PE_OK;
+}
+
+// Runs nsteps eagerly with every size class launched alone, in turn, each launch timed by its own
+// event pair; ms[2c] / ms[2c + 1] are class c's summed K1 / K3 device milliseconds.  The
+// per-step sums become the classes' costs, which place them on streams for concurrent steps.
+extern "C" int pe_batch_class_times(pe_batch* B, int32_t nsteps, double* ms, int32_t cap) {
+    const int nc = B ? (int)B->classes.size() : 0;
+    if (!B || !ms || nsteps <= 0 || cap < 2 * nc) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
+    for (int k = 0; k < 2 * nc; k++) ms[k] = 0;
+    LaunchTimer tm;
+    for (int s = 0; s < nsteps; s++) {
+        tm.used = 0;
+        if (int rc = launch_all(B->d, B->classes, B->dlist, B->dgroups, B->stream, B->side, B->has_shapes, &tm, true)) return rc;
+        if (tm.used) PE_HIP(hipEventSynchronize(tm.ev[tm.used - 1]));
+        for (int k = 0; k < tm.used / 2; k++) {
+            float t = 0;
+            PE_HIP(hipEventElapsedTime(&t, tm.ev[2 * k], tm.ev[2 * k + 1]));
+            ms[tm.stage[k] - 3] += t;
+        }
+    }
+    for (int c = 0; c < nc; c++) B->classes[c].cost = B->classes[c].count ? (ms[2 * c] + ms[2 * c + 1]) / nsteps : -1.0;
+    schedule_classes(B->classes, B->side.n + 1);
+    B->drop_graph();
     return PE_OK;
 }
 

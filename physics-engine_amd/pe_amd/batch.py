@@ -28,6 +28,7 @@ class Batch:
         self.h = h
         self.n_bodies = sum(s.n for s in scenes)
         self.n_dynamic = sum(s.n_dynamic() for s in scenes)
+        self.steps_done = 0   # World::step calls made on every world so far
 
     def close(self):
         if getattr(self, "h", None):
@@ -42,6 +43,7 @@ class Batch:
 
     def step(self, n: int = 1):
         check(lib().pe_batch_step(self.h, ctypes.c_int32(n)))
+        self.steps_done += n
 
     def sync(self):
         check(lib().pe_batch_sync(self.h))
@@ -50,18 +52,21 @@ class Batch:
         """Runs n steps; returns their device time in ms (hipEvents on the batch stream)."""
         ms = ctypes.c_double()
         check(lib().pe_batch_step_timed(self.h, ctypes.c_int32(n), ctypes.byref(ms)))
+        self.steps_done += n
         return ms.value
 
     def stage_times(self, n: int):
         """n eager steps with events between stages: summed ms of (broadphase, narrowphase, solver)."""
         ms = (ctypes.c_double * 3)()
         check(lib().pe_batch_stage_times(self.h, ctypes.c_int32(n), ms))
+        self.steps_done += n
         return list(ms)
 
     def phase_profile(self):
         """One step with in-kernel phase timestamps: summed microseconds per phase."""
         us = (ctypes.c_double * 8)()
         check(lib().pe_batch_phase_profile(self.h, us))
+        self.steps_done += 1
         names = ("integrate", "sort", "sweep", "narrowphase", "setup", "colouring", "gauss_seidel")
         return dict(zip(names, list(us)[:7]))
 
@@ -110,42 +115,49 @@ class Batch:
         self.sync()
         return time.perf_counter() - t0
 
+    def class_times(self, n: int):
+        """n eager steps with every size class launched alone, in turn: per class the summed ms of
+        (integrate + broadphase, solver).  Also deals the classes onto streams by these costs."""
+        nc = len(self.config())
+        ms = (ctypes.c_double * (2 * nc))()
+        check(lib().pe_batch_class_times(self.h, ctypes.c_int32(n), ms, ctypes.c_int32(2 * nc)))
+        self.steps_done += n
+        return [(ms[2 * c], ms[2 * c + 1]) for c in range(nc)]
+
     def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None,
                  group_candidates=(0, 128, 256, 512, 1024)):
-        """Per size class (other classes at their current setting), times every candidate block
-        size, then every candidate solver group size (bodies per solve workgroup), and keeps the
-        fastest of each; returns {class upper bound: (block, group bodies)}.  Stepping during
-        tuning advances the worlds (it is real simulation work); results never depend on the
-        choice."""
+        """Times every candidate block size on all size classes at once, each class launched
+        alone and timed by its own events so the classes do not disturb each other's figures,
+        and keeps each class's fastest (integrate + broadphase + solver); then the same for the
+        solver group size (bodies per solve workgroup; solver time).  Finally times the chosen
+        configuration, which deals the classes onto streams by cost.  Returns {class upper
+        bound: (block, group bodies)}.  Stepping during tuning advances the worlds (it is real
+        simulation work); results never depend on the choice."""
         cur = self.config()
         uppers = [c[0] for c in cur] if classes is None else list(classes)
-        blocks = [c[1] for c in cur] if classes is None else [256] * len(uppers)
-        self.set_config(uppers, blocks)
+        nc = len(uppers)
+        self.set_config(uppers, [256] * nc)
         rows = self.solver_groups()
-        counts = [c[2] for c in self.config()]
-        for ci in range(len(uppers)):
-            if counts[ci] == 0:
-                continue
-            best, best_t = blocks[ci], None
-            for cand in candidates:
-                trial = list(blocks)
-                trial[ci] = cand
-                self.set_config(uppers, trial)
-                self.set_solver_groups(rows)
-                t = self._time_steps(steps)
-                if best_t is None or t < best_t:
-                    best, best_t = cand, t
-            blocks[ci] = best
-            self.set_config(uppers, blocks)
-            best, best_t = rows[ci], None
-            for cand in group_candidates:
-                trial = list(rows)
-                trial[ci] = cand
-                self.set_solver_groups(trial)
-                t = self._time_steps(steps)
-                if best_t is None or t < best_t:
-                    best, best_t = cand, t
-            rows[ci] = best
+        best = [None] * nc
+        for cand in candidates:
+            self.set_config(uppers, [cand] * nc)
+            self.set_solver_groups(rows)
+            self.class_times(1)
+            t = self.class_times(steps)
+            for c in range(nc):
+                tot = t[c][0] + t[c][1]
+                if best[c] is None or tot < best[c][1]:
+                    best[c] = (cand, tot)
+        blocks = [b[0] for b in best]
         self.set_config(uppers, blocks)
+        best = [None] * nc
+        for cand in group_candidates:
+            self.set_solver_groups([cand] * nc)
+            t = self.class_times(steps)
+            for c in range(nc):
+                if best[c] is None or t[c][1] < best[c][1]:
+                    best[c] = (cand, t[c][1])
+        rows = [b[0] for b in best]
         self.set_solver_groups(rows)
+        self.class_times(steps)
         return {u: (b, r) for u, b, r in zip(uppers, blocks, rows)}

@@ -57,7 +57,10 @@ def test_batch_graph_and_config_do_not_change_results(worlds):
         c = Batch(worlds)
         c.set_config([16, 48, 100, 4096], [64, 128, 512, 256])
         c.set_solver_groups(rows)
-        c.step(20)
+        # class-by-class timed steps, then steps on the streams their costs dealt them to
+        t = c.class_times(3)
+        assert len(t) == 4 and all(k1 >= 0 and k3 >= 0 for k1, k3 in t)
+        c.step(17)
         c.sync()
         for i in range(len(worlds)):
             assert np.array_equal(flat_state(a.world_state(i)), flat_state(c.world_state(i))), f"world {i} rows {rows}"
