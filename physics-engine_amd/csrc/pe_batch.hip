@@ -71,8 +71,11 @@ struct BatchDev {
     int32_t* nl_pair;    // [total pair cap] narrowphase work list: global pair slots of all worlds
     int32_t* nl_limit;   // [total pair cap] per listed pair: first slot of its world that starts fresh
     int32_t* nl_count;   // [1] entries in the list this step
-    int32_t* nl_shape;   // [total pair cap] list entries whose pair involves a cylinder or a mesh
+    int32_t* nl_shape;   // [nl_cap] list entries whose pair involves a cylinder or a mesh: from the
+                         // front those of k_narrow_shape_list, from the back those of k_narrow_sat_list
     int32_t* nl_shape_count;  // [1]
+    int32_t* nl_sat_count;    // [1]
+    int32_t nl_cap;
 };
 
 constexpr uint64_t KEY_PAD = ~0ull;
@@ -251,8 +254,11 @@ __global__ void __launch_bounds__(128) k_narrow_list(BatchDev d) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
         const int p = d.nl_pair[q];
         const bool simple = pair_is_simple(d.b, d.ps, p);
-        const int slot = wave_append(!simple, d.nl_shape_count);
+        const bool sat = !simple && pair_is_sat(d.b, d.ps, p);
+        const int slot = wave_append(!simple && !sat, d.nl_shape_count);
+        const int sslot = wave_append(sat, d.nl_sat_count);
         if (simple) narrow_pair<false>(d.b, d.ps, p, d.nl_limit[q]);
+        else if (sat) d.nl_shape[d.nl_cap - 1 - sslot] = q;
         else d.nl_shape[slot] = q;
     }
 }
@@ -265,9 +271,20 @@ __global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d) {
     }
 }
 
+// K2c: the listed separating-axis pairs (pair_is_sat), one wave each.
+__global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d) {
+    const int cnt = *d.nl_sat_count;
+    const int nw = gridDim.x * (blockDim.x / WAVE);
+    for (int k = blockIdx.x * (blockDim.x / WAVE) + (int)(threadIdx.x / WAVE); k < cnt; k += nw) {
+        const int q = d.nl_shape[d.nl_cap - 1 - k];
+        narrow_pair<true, WAVE>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
+    }
+}
+
 __global__ void k_list_reset(BatchDev d) {
     *d.nl_count = 0;
     *d.nl_shape_count = 0;
+    *d.nl_sat_count = 0;
 }
 
 // K3 for one world on the global row store (claims in LDS): solver setup, colouring,
@@ -737,7 +754,10 @@ int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32
         if (phase == 0) {
             if (tm && !class_tm) tm->begin(1, s);
             hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
-            if (shapes) hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
+            if (shapes) {
+                hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
+                hipLaunchKernelGGL(k_narrow_sat_list, dim3(2048), dim3(256), 0, s, d);
+            }
             if (tm && !class_tm) tm->end(s);
         }
     }
@@ -901,6 +921,8 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     rc |= B->alloc(&d.nl_count, 1);
     rc |= B->alloc(&d.nl_shape, poff);
     rc |= B->alloc(&d.nl_shape_count, 1);
+    rc |= B->alloc(&d.nl_sat_count, 1);
+    d.nl_cap = (int32_t)poff;
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= B->alloc(&ddesc, nworlds);
@@ -1479,6 +1501,8 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&d.nl_count, 1);
     rc |= S->alloc(&d.nl_shape, P);
     rc |= S->alloc(&d.nl_shape_count, 1);
+    rc |= S->alloc(&d.nl_sat_count, 1);
+    d.nl_cap = (int32_t)P;
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= S->alloc(&ddesc, cfg->max_worlds);

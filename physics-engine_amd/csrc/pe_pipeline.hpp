@@ -199,6 +199,14 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter) {
 __device__ __forceinline__ bool pair_is_simple(const BodyView& b, const PairStore& ps, int p) {
     return b.iat(I_TYPE, ps.pa[p]) <= SHAPE_SPHERE && b.iat(I_TYPE, ps.pb[p]) <= SHAPE_SPHERE;
 }
+// True when pair p runs the convex separating-axis routine on two polyhedra with edges (cylinder
+// and cylinder, box and convex mesh, convex mesh and convex mesh), whose edge-pair axes number in
+// the thousands: such pairs are solved by a whole wave each.
+__device__ __forceinline__ bool pair_is_sat(const BodyView& b, const PairStore& ps, int p) {
+    const int ta = b.iat(I_TYPE, ps.pa[p]), tb = b.iat(I_TYPE, ps.pb[p]);
+    return (ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) || (ta == SHAPE_BOX && tb == SHAPE_CONVEX) ||
+           (ta == SHAPE_CONVEX && tb == SHAPE_BOX) || (ta == SHAPE_CONVEX && tb == SHAPE_CONVEX);
+}
 
 // The polyhedral view of a box, cylinder or convex-mesh body.
 __device__ __forceinline__ Poly body_poly(const MeshSet& ms, const CollBody& B) {
@@ -230,7 +238,7 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
                (ta == SHAPE_CONVEX && tb == SHAPE_CONVEX)) {
         // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
         const Poly PA = body_poly(ms, A), PB = body_poly(ms, B);
-        convex_convex(c, x, PA, PB, A.P, B.P, ref_scale);
+        convex_convex<1>(c, x, PA, PB, A.P, B.P, ref_scale);
     } else if (is(SHAPE_SPHERE, SHAPE_CONVEX)) {             // sphere_convex_collision_algorithm.cpp:11-34
         const CollBody& sp = ta == SHAPE_SPHERE ? A : B;
         const CollBody& mb = ta == SHAPE_SPHERE ? B : A;
@@ -260,7 +268,9 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
 // positions (contact_result.cpp:158-164); newer slots start from a fresh ContactResult.
 // SHAPES = false handles box and sphere pairs only (the kernel then carries no polyhedral
 // code and keeps its register budget); SHAPES = true handles every pair type.
-template <bool SHAPES>
+// G = 64 (a whole wave on one pair, every lane holding the same values) is for the pairs
+// pair_is_sat selects: the separating-axis loops are split over the lanes and lane 0 writes.
+template <bool SHAPES, int G = 1>
 static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
     Cache c;
@@ -286,13 +296,17 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
         x.same_th2 = th * th;
     }
     const int ta = A.type, tb = B.type;
-    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.sp, B.sp);
+    if (G > 1) {
+        const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
+        convex_convex<G>(c, x, body_poly(b.ms, A), body_poly(b.ms, B), A.P, B.P, ref_scale);
+    } else if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.sp, B.sp);
     else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.sp, B.sp[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, B.sp, A.sp[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.sp[0], B.sp[0]);
     else if (SHAPES) narrow_shapes(b.ms, c, x, A, B);
     int rank[8];
     const int np = sort_rank(c, rank);
+    if (G > 1 && Grp<G>::rank() != 0) return;
     ps.npts[p] = np;
     ps.swap[p] = x.swap ? 1 : 0;
     float dsorted[8];

@@ -244,28 +244,62 @@ PE_HD int clip_face(const f3* in, int nin, f3* out, f3 pn, float pe) {
     }
     return no;
 }
+// Lanes that work on one pair together: G = 1 is a thread alone; G = 64 is a whole wave, all
+// of whose lanes hold the same pair, run the same control flow on the same values, and split
+// the long axis loops of the separating-axis test (lane r takes items r, r + G, ...).
+template <int G>
+struct Grp {
+    static_assert(G == 1 || G == WAVE, "a pair is handled by one lane or by a whole wave");
+    static __device__ __forceinline__ int rank() { return G == 1 ? 0 : (int)(threadIdx.x & (G - 1)); }
+    static __device__ __forceinline__ bool any(bool v) {
+        if (G == 1) return v;
+        return __ballot(v) != 0ull;
+    }
+    // The lexicographically least (d, i) of the group in every lane; i = INT_MAX marks a lane
+    // without a candidate.  Each lane offers the first index of its own least d, so the result
+    // is the first index (in item order) reaching the least d, as a sequential scan with a
+    // strict comparison finds it.
+    static __device__ __forceinline__ void argmin(float& d, int& i) {
+        if (G == 1) return;
+#pragma unroll
+        for (int m = 1; m < G; m <<= 1) {
+            const float od = __shfl_xor(d, m);
+            const int oi = __shfl_xor(i, m);
+            if (oi != 0x7fffffff && (i == 0x7fffffff || od < d || (od == d && oi < i))) { d = od; i = oi; }
+        }
+    }
+    static __device__ __forceinline__ float bcast(float v, int src) { return G == 1 ? v : __shfl(v, src); }
+    static __device__ __forceinline__ f3 bcast(f3 v, int src) { return G == 1 ? v : f3{__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src)}; }
+};
+
 // clipHullAgainstHull + clipFaceAgainstHull (:63-170)
+template <int G>
 __device__ __noinline__ void clip_hull_hull(Cache& c, const PairCtx& x, f3 sep, const Poly& A, const Poly& B,
                                             const Pose& ta, const Pose& tb, float min_dist, float max_dist) {
-    int cb = -1;
-    float dmax = REAL_MIN;
+    using L = Grp<G>;
+    // B's face most along sep (first of the largest), A's face most against it (first of the
+    // least): as argmin of (-d, f) and (d, f)
+    float bd = REAL_MAX;
+    int cb = 0x7fffffff;
     const int nfb = n_faces(B);
-    for (int f = 0; f < nfb; f++) {
+    for (int f = L::rank(); f < nfb; f += G) {
         const float d = dot(mv(tb.R, face_normal(B, f)), sep);
-        if (d > dmax) { dmax = d; cb = f; }
+        if (d > REAL_MIN && -d < bd) { bd = -d; cb = f; }
     }
-    if (cb < 0) return;
+    L::argmin(bd, cb);
+    if (cb == 0x7fffffff) return;
+    float ad = REAL_MAX;
+    int ca = 0x7fffffff;
+    const int nfa = n_faces(A);
+    for (int f = L::rank(); f < nfa; f += G) {
+        const float d = dot(mv(ta.R, face_normal(A, f)), sep);
+        if (d < ad) { ad = d; ca = f; }
+    }
+    L::argmin(ad, ca);
+    if (ca == 0x7fffffff) return;
     f3 buf0[CLIP_CAP], buf1[CLIP_CAP];
     int n = face_len(B, cb);
     for (int k = 0; k < n; k++) buf0[k] = xf_apply(tb, face_vertex(B, cb, k));
-    int ca = -1;
-    float dmin = REAL_MAX;
-    const int nfa = n_faces(A);
-    for (int f = 0; f < nfa; f++) {
-        const float d = dot(mv(ta.R, face_normal(A, f)), sep);
-        if (d < dmin) { dmin = d; ca = f; }
-    }
-    if (ca < 0) return;
     const int nva = face_len(A, ca);
     const f3 wn = mv(ta.R, face_normal(A, ca));
     f3* pin = buf0;
@@ -306,40 +340,65 @@ PE_HD bool on_line(f3 v0, f3 v1, f3 v2) {
 }
 // findSeparatingAxis (:251-402) + ConvexConvexCollisionAlgorithm::getClosestPoints (:408-432).
 // The deep-point contact is added before the edge axes are tested and stays even when an edge
-// axis then separates the shapes.
+// axis then separates the shapes.  With G lanes, the face axes and the edge-pair axes are split
+// over the lanes: the shapes are separated if any axis separates them (the sequential scan
+// returns at the first), and the axis kept is the first in scan order of the least depth among
+// those the scan would accept (an edge axis only below the face depth and with both witness
+// points on their edges), which is the one the sequential scan ends with.
+template <int G>
 __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Poly& A, const Poly& B, const Pose& ta,
                                            const Pose& tb, float ref_scale) {
+    using L = Grp<G>;
     const f3 dc = sub(ta.p, tb.p);
     float dmin = REAL_MAX;
     f3 dmin_pt{0, 0, 0}, sep{0, 0, 0};
     bool pt_on_a = false;
     const int nfa = n_faces(A), nfb = n_faces(B);
-    for (int f = 0; f < nfa; f++) {
-        f3 n = mv(ta.R, face_normal(A, f));
-        if (dot(dc, n) < 0) n = mul(n, -1.0f);
-        float d;
-        f3 wa, wb;
-        if (!test_axis(A, B, ta, tb, n, d, wa, wb)) return;
-        if (d < dmin) { dmin = d; dmin_pt = wb; pt_on_a = false; sep = n; }
-    }
-    for (int f = 0; f < nfb; f++) {
-        f3 n = mv(tb.R, face_normal(B, f));
-        if (dot(dc, n) < 0) n = mul(n, -1.0f);
-        float d;
-        f3 wa, wb;
-        if (!test_axis(A, B, ta, tb, n, d, wa, wb)) return;
-        if (d < dmin) { dmin = d; dmin_pt = add(wa, mul(n, d)); pt_on_a = true; sep = n; }
+    {
+        float bd = REAL_MAX;
+        int bi = 0x7fffffff;
+        f3 bn{0, 0, 0}, bpt{0, 0, 0};
+        bool separated = false;
+        for (int i = L::rank(); i < nfa + nfb; i += G) {
+            const bool on_b = i >= nfa;
+            f3 n = on_b ? mv(tb.R, face_normal(B, i - nfa)) : mv(ta.R, face_normal(A, i));
+            if (dot(dc, n) < 0) n = mul(n, -1.0f);
+            float d;
+            f3 wa, wb;
+            if (!test_axis(A, B, ta, tb, n, d, wa, wb)) { separated = true; break; }
+            if (d < bd) { bd = d; bi = i; bn = n; bpt = on_b ? add(wa, mul(n, d)) : wb; }
+        }
+        if (L::any(separated)) return;
+        L::argmin(bd, bi);
+        if (bi != 0x7fffffff) {
+            const int src = bi & (G - 1);
+            dmin = bd;
+            sep = L::bcast(bn, src);
+            dmin_pt = L::bcast(bpt, src);
+            pt_on_a = bi >= nfa;
+        }
     }
     if ((pt_on_a && inside(B, xf_inv(tb, dmin_pt))) || (!pt_on_a && inside(A, xf_inv(ta, dmin_pt))))
         add_point(c, x, sep, sub(dmin_pt, mul(sep, 0.0f)), -dmin + 0.0f * 2);
     int ea = -1, eb = -1;
     f3 wea{0, 0, 0}, web{0, 0, 0}, wpa{0, 0, 0}, wpb{0, 0, 0};
     const int nea = n_edges(A), neb = n_edges(B);
-    for (int e0 = 0; e0 < nea; e0++) {
-        f3 a0, a1;
-        edge_of(A, e0, a0, a1);
-        const f3 we0 = mv(ta.R, normalized(sub(a0, a1)));
-        for (int e1 = 0; e1 < neb; e1++) {
+    if (nea > 0 && neb > 0) {
+        float bd = dmin;
+        int bi = 0x7fffffff;
+        f3 bsep{0, 0, 0}, bwea{0, 0, 0}, bweb{0, 0, 0}, bwpa{0, 0, 0}, bwpb{0, 0, 0};
+        bool separated = false;
+        const int ne = nea * neb;
+        int t = L::rank();
+        int e0 = t / neb, e1 = t - e0 * neb;
+        int cur0 = -1;
+        f3 a0, a1, we0;
+        for (; t < ne; t += G) {
+            if (e0 != cur0) {
+                edge_of(A, e0, a0, a1);
+                we0 = mv(ta.R, normalized(sub(a0, a1)));
+                cur0 = e0;
+            }
             f3 b0, b1;
             edge_of(B, e1, b0, b1);
             const f3 we1 = mv(tb.R, normalized(sub(b0, b1)));
@@ -349,12 +408,26 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
                 if (dot(dc, cr) < 0) cr = mul(cr, -1.0f);
                 float dist;
                 f3 wa, wb;
-                if (!test_axis(A, B, ta, tb, cr, dist, wa, wb)) return;
-                if (dist < dmin) {
-                    if (!on_line(a0, a1, xf_inv(ta, wa)) || !on_line(b0, b1, xf_inv(tb, wb))) continue;
-                    dmin = dist; sep = cr; ea = e0; eb = e1; wea = we0; web = we1; wpa = wa; wpb = wb;
+                if (!test_axis(A, B, ta, tb, cr, dist, wa, wb)) { separated = true; break; }
+                if (dist < bd && on_line(a0, a1, xf_inv(ta, wa)) && on_line(b0, b1, xf_inv(tb, wb))) {
+                    bd = dist; bi = t; bsep = cr; bwea = we0; bweb = we1; bwpa = wa; bwpb = wb;
                 }
             }
+            e1 += G;
+            while (e1 >= neb) { e1 -= neb; e0++; }
+        }
+        if (L::any(separated)) return;
+        L::argmin(bd, bi);
+        if (bi != 0x7fffffff) {
+            const int src = bi & (G - 1);
+            dmin = bd;
+            ea = bi / neb;
+            eb = bi - ea * neb;
+            sep = L::bcast(bsep, src);
+            wea = L::bcast(bwea, src);
+            web = L::bcast(bweb, src);
+            wpa = L::bcast(bwpa, src);
+            wpb = L::bcast(bwpb, src);
         }
     }
     if (ea >= 0 && eb >= 0) {
@@ -394,7 +467,7 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
         }
     }
     if (dot(dc, sep) < 0) sep = neg(sep);
-    clip_hull_hull(c, x, sep, A, B, ta, tb, -ref_scale, 0.0f);
+    clip_hull_hull<G>(c, x, sep, A, B, ta, tb, -ref_scale, 0.0f);
 }
 
 // ------------------------------------------------------------------ sphere-cylinder
@@ -840,7 +913,7 @@ __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const Mesh
         T.t[1] = face_vertex(M, f, 1);
         T.t[2] = face_vertex(M, f, 2);
         T.tn = face_normal(M, f);
-        convex_convex(c, x, A, T, pa, pm, ref_scale);
+        convex_convex<1>(c, x, A, T, pa, pm, ref_scale);
     });
 }
 

@@ -49,14 +49,16 @@ struct DevCounts {          // device-side counters of one step
     uint32_t nchunks;       // live sweep chunks
     uint32_t bar_count;     // grid barrier: arrivals (0 between barriers)
     uint32_t bar_gen;       // grid barrier: generation
-    int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh
-    int32_t pad[3];
+    int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh ...
+    int32_t nsat;           // ... of which these run the polyhedral separating-axis routine
+    int32_t pad[2];
 };
 
 struct Dev {
     BodyView b;
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
-    int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes
+    int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes (from the front)
+                            // and to k_narrow_sat (from the back)
     int32_t* order;         // [pcap]
     int32_t* act;           // [3*pcap] colouring work lists
     int32_t* color_start;   // [pcap+1]
@@ -226,8 +228,11 @@ __global__ void __launch_bounds__(128) k_narrow(Dev d) {
     const int prev = d.cnt->prev_npairs;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         const bool simple = pair_is_simple(d.b, d.ps, p);
-        const int slot = wave_append(!simple, &d.cnt->nshape);
+        const bool sat = !simple && pair_is_sat(d.b, d.ps, p);
+        const int slot = wave_append(!simple && !sat, &d.cnt->nshape);
+        const int sslot = wave_append(sat, &d.cnt->nsat);
         if (simple) narrow_pair<false>(d.b, d.ps, p, prev);
+        else if (sat) d.shape_list[d.pcap - 1 - sslot] = p;
         else d.shape_list[slot] = p;
     }
 }
@@ -237,9 +242,18 @@ __global__ void __launch_bounds__(64) k_narrow_shapes(Dev d) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ns; q += gridDim.x * blockDim.x)
         narrow_pair<true>(d.b, d.ps, d.shape_list[q], prev);
 }
+// One wave per listed separating-axis pair.
+__global__ void __launch_bounds__(256) k_narrow_sat(Dev d) {
+    const int ns = d.cnt->nsat;
+    const int prev = d.cnt->prev_npairs;
+    const int nw = gridDim.x * (blockDim.x / WAVE);
+    for (int q = blockIdx.x * (blockDim.x / WAVE) + (int)(threadIdx.x / WAVE); q < ns; q += nw)
+        narrow_pair<true, WAVE>(d.b, d.ps, d.shape_list[d.pcap - 1 - q], prev);
+}
 __global__ void k_narrow_finish(Dev d) {
     d.cnt->prev_npairs = d.cnt->npairs;
     d.cnt->nshape = 0;
+    d.cnt->nsat = 0;
 }
 
 // ------------------------------------------------------------------ stage 3: solver
@@ -395,7 +409,10 @@ struct pe_world {
     }
     int launch_narrowphase() {
         hipLaunchKernelGGL(k_narrow, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
-        if (has_shapes) hipLaunchKernelGGL(k_narrow_shapes, dim3(grid(d.pcap, 64)), dim3(64), 0, stream, d);
+        if (has_shapes) {
+            hipLaunchKernelGGL(k_narrow_shapes, dim3(grid(d.pcap, 64)), dim3(64), 0, stream, d);
+            hipLaunchKernelGGL(k_narrow_sat, dim3(std::min(grid(d.pcap, 4), 4096)), dim3(256), 0, stream, d);
+        }
         hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
         hipLaunchKernelGGL(k_count_points, dim3(1), dim3(1024), 0, stream, d);
         return PE_OK;
