@@ -115,9 +115,14 @@ def run_served(args):
     sizes = scenes.log_uniform_counts(rng, n_req, 16, 4096)
     horizons = rng.integers(30, 601, n_req)
     t0 = time.perf_counter()
-    reqs = [scenes.grid_world(int(s), scenes.world_seed(1004, i)) for i, s in enumerate(sizes)]
-    log(f"served: {n_req} requests generated in {time.perf_counter() - t0:.1f}s "
-        f"(mean {sizes.mean():.0f} bodies, mean horizon {horizons.mean():.0f})")
+    # mixed shapes as configs[2] (boxes, spheres, cylinders, hulls over a heightfield); a pool of
+    # distinct request scenes, sized from the same distribution, is cycled through
+    pool = [scenes.served_shape_world(int(s), scenes.world_seed(1004, i))
+            for i, s in enumerate(sizes[:args.serve_pool])]
+    reqs = [pool[i % len(pool)] for i in range(n_req)]
+    sizes = np.array([r.n - 1 for r in reqs])
+    log(f"served: {len(pool)} distinct mixed-shape request worlds generated in {time.perf_counter() - t0:.1f}s for "
+        f"{n_req} arrivals (mean {sizes.mean():.0f} bodies, mean horizon {horizons.mean():.0f})")
     srv = Server(hbm_budget_bytes=int(args.serve_budget_gb * (1 << 30)), max_worlds=16384, max_world_bodies=4096)
     nxt = 0
     done = 0
@@ -140,7 +145,10 @@ def run_served(args):
     elapsed = time.perf_counter() - start
     st = srv.stats()
     srv.close()
-    return dict(rate=lam, seconds=elapsed, submitted=nxt, completed=done, ticks=ticks,
+    return dict(workload=f"Poisson arrivals at {lam:g}/s for {T:g} s; body counts log-uniform 16-4096, horizons "
+                         f"U(30, 600); mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a "
+                         f"16x16 heightfield), {len(pool)} distinct request worlds cycled",
+                rate=lam, seconds=elapsed, submitted=nxt, completed=done, ticks=ticks,
                 body_steps=st["body_steps"], body_steps_per_s=st["body_steps"] / elapsed,
                 p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"], budget_gb=args.serve_budget_gb)
 
@@ -224,6 +232,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-pool", type=int, default=192, help="distinct request worlds the arrivals cycle through")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-steps", type=int, default=600)

@@ -1228,12 +1228,19 @@ struct Req {
     int pcap = 0;
     int64_t stage_off = 0;       // offset of its upload in the staging buffer
     std::vector<float> staged;   // [NSTAGE][n]
+    HostMeshes meshes;           // its convex / concave meshes, prepared at submission
+    int64_t mrec_off = -1, marena_off = -1;   // their records / arena range while live
+    int64_t footprint = 0;
     bool cancelled = false;
 };
 
 int64_t req_footprint(int n) {
     const int64_t pc = body_pair_cap(n);
     return (int64_t)n * (NSTAGE + CBW + SBW + 8) * 4 + pc * (8 * (int64_t)(CW + 1 + RW) * 4 + 10 * 4) + 4 * (int64_t)(pc + 1) + 64;
+}
+// Device bytes of a request's prepared meshes (records and arena).
+int64_t mesh_footprint(const HostMeshes& m) {
+    return (int64_t)m.rec.size() * 4 + (int64_t)m.arena.size() * (int64_t)sizeof(float4);
 }
 
 }  // namespace
@@ -1271,8 +1278,12 @@ struct pe_server {
     int64_t admitted = 0;
     hipEvent_t ev_done = nullptr;
 
-    DevMeshes meshes;   // the unit cylinder mesh (requests carry no convex / concave meshes)
-    bool has_shapes = false;   // a cylinder was ever submitted
+    DevMeshes meshes;   // the unit cylinder mesh
+    int32_t* mrec = nullptr;   // [max_meshes * MRW] mesh records of the live requests
+    float4* marena = nullptr;  // [mesh_arena_cap] their vertices, faces, edges, BVH
+    int64_t max_meshes = 0, mesh_arena_cap = 0;
+    RangeAlloc mesh_recs, mesh_mem;
+    bool has_shapes = false;   // a cylinder or a mesh was ever submitted
     ClassStreams side;
     ~pe_server() {
         if (cs) hipStreamSynchronize(cs);
@@ -1317,6 +1328,7 @@ struct pe_server {
         if (!list.empty()) PE_HIP(hipMemcpyAsync(dlist, list.data(), list.size() * 4, hipMemcpyHostToDevice, cs));
         if (!grp.empty()) PE_HIP(hipMemcpyAsync(dgroups, grp.data(), grp.size() * sizeof(int2), hipMemcpyHostToDevice, cs));
         PE_HIP(hipStreamSynchronize(cs));  // the host list buffer is reused
+        schedule_classes(classes, side.n + 1);
         dirty = false;
         return PE_OK;
     }
@@ -1328,7 +1340,7 @@ struct pe_server {
         while (!queue.empty()) {
             Req* r = queue.front();
             if (r->cancelled) { queue.pop_front(); continue; }
-            const int64_t fp = req_footprint(r->n);
+            const int64_t fp = r->footprint;
             const int64_t need = (int64_t)r->n * NSTAGE;
             if (free_slots.empty() || live_footprint + fp > cfg.hbm_budget_bytes || used + need > stage_cap) break;
             const int64_t bo = bodies.alloc(r->n);
@@ -1336,6 +1348,35 @@ struct pe_server {
             const int pc = body_pair_cap(r->n);
             const int64_t po = pairs.alloc(pc + 1);
             if (po < 0) { bodies.release(bo, r->n); break; }
+            const int nm = r->meshes.count();
+            const int64_t nar = (int64_t)r->meshes.arena.size();
+            int64_t mo = -1, ao = -1;
+            if (nm > 0) {
+                mo = mesh_recs.alloc(nm);
+                ao = mo < 0 ? -1 : mesh_mem.alloc(std::max<int64_t>(nar, 1));
+                if (mo < 0 || ao < 0) {
+                    if (mo >= 0) mesh_recs.release(mo, nm);
+                    bodies.release(bo, r->n);
+                    pairs.release(po, pc + 1);
+                    break;
+                }
+                // the meshes move to their arena range, and the bodies name them by server-wide id
+                r->meshes.rebase((int32_t)ao);
+                float* sp = r->staged.data() + (size_t)F_SP * r->n;
+                const int32_t* ty = reinterpret_cast<const int32_t*>(r->staged.data()) + (size_t)(NF + I_TYPE) * r->n;
+                for (int i = 0; i < r->n; i++) {
+                    if (ty[i] != SHAPE_CONVEX && ty[i] != SHAPE_CONCAVE) continue;
+                    int32_t id;
+                    std::memcpy(&id, sp + i, 4);
+                    id += (int32_t)mo;
+                    std::memcpy(sp + i, &id, 4);
+                }
+                PE_HIP(hipMemcpyAsync(mrec + mo * MRW, r->meshes.rec.data(), r->meshes.rec.size() * 4, hipMemcpyHostToDevice, xs));
+                if (nar > 0)
+                    PE_HIP(hipMemcpyAsync(marena + ao, r->meshes.arena.data(), nar * sizeof(float4), hipMemcpyHostToDevice, xs));
+            }
+            r->mrec_off = mo;
+            r->marena_off = ao;
             queue.pop_front();
             r->boff = bo;
             r->poff = po;
@@ -1378,8 +1419,14 @@ struct pe_server {
     void release(Req* r) {
         bodies.release(r->boff, r->n);
         pairs.release(r->poff, r->pcap + 1);
+        if (r->mrec_off >= 0) {
+            mesh_recs.release(r->mrec_off, r->meshes.count());
+            mesh_mem.release(r->marena_off, std::max<int64_t>((int64_t)r->meshes.arena.size(), 1));
+            r->mrec_off = r->marena_off = -1;
+        }
+        r->meshes = HostMeshes();
         free_slots.push_back(r->slot);
-        live_footprint -= req_footprint(r->n);
+        live_footprint -= r->footprint;
         r->slot = -1;
     }
 
@@ -1443,7 +1490,12 @@ extern "C" {
 
 int pe_server_footprint(const pe_scene_desc* scene, int64_t* bytes) {
     if (!scene || !bytes) { pe_set_error("null argument"); return PE_ERR_INVALID; }
-    *bytes = req_footprint(scene->n);
+    if (int rc = check_scene(scene)) return rc;
+    std::vector<float> hf((size_t)NF * scene->n, 0.0f);
+    std::vector<int32_t> hi((size_t)NI * scene->n, 0);
+    HostMeshes hm;
+    if (int rc = stage_bodies(scene, hf, hi, (size_t)scene->n, 0, hm)) return rc;
+    *bytes = req_footprint(scene->n) + mesh_footprint(hm);
     return PE_OK;
 }
 
@@ -1466,7 +1518,12 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     const int64_t per_body = req_footprint(1024) / 1024 + 1;
     S->stage_cap = (int64_t)cfg->max_world_bodies * NSTAGE * 8;            // floats per staging buffer
     S->result_cap = (int64_t)cfg->max_world_bodies * NRESULT * 64;        // floats
-    const int64_t fixed = 2 * S->stage_cap * 4 + S->result_cap * 4 + (int64_t)cfg->max_worlds * 64;
+    // a sixteenth of the budget (at most 8 GiB) holds the requests' meshes: records for one mesh per KiB
+    S->mesh_arena_cap = std::min<int64_t>(cfg->hbm_budget_bytes / 16, (int64_t)8 << 30) / (int64_t)sizeof(float4);
+    S->mesh_arena_cap = std::min<int64_t>(std::max<int64_t>(S->mesh_arena_cap, 4096), ((int64_t)1 << 31) - 1);
+    S->max_meshes = std::min<int64_t>(std::max<int64_t>(S->mesh_arena_cap / 64, 64), (int64_t)1 << 24);
+    const int64_t fixed = 2 * S->stage_cap * 4 + S->result_cap * 4 + (int64_t)cfg->max_worlds * 64 +
+                          S->mesh_arena_cap * (int64_t)sizeof(float4) + S->max_meshes * MRW * 4;
     S->max_bodies = std::max<int64_t>((cfg->hbm_budget_bytes - fixed) / per_body, cfg->max_world_bodies);
     S->max_bodies = std::min<int64_t>(S->max_bodies, (int64_t)1 << 28);
     S->max_pairs = 8 * S->max_bodies + 65 * (int64_t)cfg->max_worlds;
@@ -1512,12 +1569,16 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&S->dresult, S->result_cap);
     rc |= S->alloc(&S->dlist, cfg->max_worlds);
     rc |= S->alloc(&S->dgroups, cfg->max_worlds);
+    rc |= S->alloc(&S->mrec, (size_t)S->max_meshes * MRW);
+    rc |= S->alloc(&S->marena, (size_t)S->mesh_arena_cap);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
     {
         HostMeshes none;
         if (int rc2 = S->meshes.upload(none, S->cs)) return fail(rc2);
-        d.b.ms = S->meshes.view();
+        d.b.ms = MeshSet{S->mrec, S->marena, S->meshes.cyl};
     }
+    S->mesh_recs.init(S->max_meshes);
+    S->mesh_mem.init(S->mesh_arena_cap);
     d.desc = ddesc;
     d.st = dst;
     S->bodies.init(S->max_bodies);
@@ -1540,24 +1601,25 @@ int pe_server_destroy(pe_server* S) {
 int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, int64_t* id) {
     if (!S || !scene || horizon <= 0) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
     if (int rc = check_scene(scene)) return rc;
-    for (int i = 0; i < scene->n; i++)
-        if (scene->shape_type[i] == PE_SHAPE_CONVEX || scene->shape_type[i] == PE_SHAPE_CONCAVE) {
-            pe_set_error("the server does not admit convex or concave mesh bodies");
-            return PE_ERR_UNSUPPORTED;
-        }
-    if (scene->n > S->cfg.max_world_bodies || req_footprint(scene->n) > S->cfg.hbm_budget_bytes) {
+    if (scene->n > S->cfg.max_world_bodies) {
         pe_set_error("request larger than the server admits");
         return PE_ERR_BUSY;
     }
     Req* r = new Req();
-    r->id = S->next_id++;
     r->n = scene->n;
     r->horizon = horizon;
     r->staged.assign((size_t)NSTAGE * scene->n, 0.0f);
     std::vector<float> hf((size_t)NF * scene->n, 0.0f);
     std::vector<int32_t> hi((size_t)NI * scene->n, 0);
-    HostMeshes none;
-    if (int rc = stage_bodies(scene, hf, hi, (size_t)scene->n, 0, none)) { delete r; return rc; }
+    if (int rc = stage_bodies(scene, hf, hi, (size_t)scene->n, 0, r->meshes)) { delete r; return rc; }
+    r->footprint = req_footprint(scene->n) + mesh_footprint(r->meshes);
+    if (r->footprint > S->cfg.hbm_budget_bytes || r->meshes.count() > S->max_meshes ||
+        (int64_t)r->meshes.arena.size() > S->mesh_arena_cap) {
+        delete r;
+        pe_set_error("request larger than the server admits");
+        return PE_ERR_BUSY;
+    }
+    r->id = S->next_id++;
     for (int i = 0; i < scene->n; i++) S->has_shapes |= scene->shape_type[i] > PE_SHAPE_SPHERE;
     std::copy(hf.begin(), hf.end(), r->staged.begin());
     std::memcpy(r->staged.data() + hf.size(), hi.data(), hi.size() * 4);

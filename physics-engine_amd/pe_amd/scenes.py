@@ -468,13 +468,15 @@ def heightfield_mesh(rng: np.random.Generator, nx: int = 64, nz: int = 64, size=
 
 def shape_mix_scene(n: int = 50000, seed: int = 1002, volume=(100.0, 60.0, 100.0), grid: int = 64,
                     ground_size=None, fractions=(0.40, 0.25, 0.15, 0.20), density: float = 1000.0,
-                    hull_points: int = 32, mu: float = 0.5) -> Scene:
+                    hull_points: int = 32, mu: float = 0.5, hull_pool: int = 0) -> Scene:
     """Mixed-shape world: boxes (edges U(0.2, 1.0) m), spheres (r U(0.1, 0.6) m), cylinders
     (r U(0.1, 0.4) m, h U(0.2, 1.0) m) and convex hulls (hull of `hull_points` Gaussian points
-    scaled to 0.5 m, one mesh each) in the given fractions, all with uniform random orientation
-    (spheres unrotated), over a kinematic grid x grid heightfield with +-1 m uniform noise.
-    Centres are jittered in the cells of a 1.8 m lattice filling the volume from 2 m above the
-    ground, which keeps bounding spheres 0.05 m apart without a rejection loop."""
+    scaled to 0.5 m) in the given fractions, all with uniform random orientation (spheres
+    unrotated), over a kinematic grid x grid heightfield with +-1 m uniform noise.  Each hull body
+    gets its own mesh, or with hull_pool > 0 one of hull_pool shared meshes (fewer meshes to
+    prepare, as for many small served requests).  Centres are jittered in the cells of a 1.8 m
+    lattice filling the volume from 2 m above the ground, which keeps bounding spheres 0.05 m
+    apart without a rejection loop."""
     rng = np.random.default_rng(seed)
     gx, gz = ground_size if ground_size is not None else (volume[0], volume[2])
     b = SceneBuilder()
@@ -500,6 +502,7 @@ def shape_mix_scene(n: int = 50000, seed: int = 1002, volume=(100.0, 60.0, 100.0
     jit = rng.uniform(-1.0, 1.0, (n, 3)) * slack[:, None]
     pos = np.stack([(cx + 0.5) * cell - nx * cell / 2.0, 2.0 + (cy + 0.5) * cell,
                     (cz + 0.5) * cell - nz * cell / 2.0], axis=1) + jit
+    pool = []
     for i in range(n):
         k = kind[i]
         if k == SHAPE_BOX:
@@ -509,7 +512,26 @@ def shape_mix_scene(n: int = 50000, seed: int = 1002, volume=(100.0, 60.0, 100.0
         elif k == SHAPE_CYLINDER:
             b.add(SHAPE_CYLINDER, (cr[i], ch[i]), density * math.pi * cr[i] ** 2 * ch[i], pos[i], rot[i], mu=mu)
         else:
-            m = gaussian_hull(rng, hull_points, 0.5)
-            vol = abs(sum(np.dot(m.vertices[f[0]], np.cross(m.vertices[f[1]], m.vertices[f[2]])) for f in m.faces)) / 6.0
-            b.add(SHAPE_CONVEX, (), density * vol, pos[i], rot[i], mu=mu, mesh=b.add_mesh(m))
+            if hull_pool > 0:
+                if not pool:
+                    for _ in range(hull_pool):
+                        m = gaussian_hull(rng, hull_points, 0.5)
+                        vol = abs(sum(np.dot(m.vertices[f[0]], np.cross(m.vertices[f[1]], m.vertices[f[2]]))
+                                      for f in m.faces)) / 6.0
+                        pool.append((b.add_mesh(m), vol))
+                mid, vol = pool[int(rng.integers(hull_pool))]
+            else:
+                m = gaussian_hull(rng, hull_points, 0.5)
+                vol = abs(sum(np.dot(m.vertices[f[0]], np.cross(m.vertices[f[1]], m.vertices[f[2]])) for f in m.faces)) / 6.0
+                mid = b.add_mesh(m)
+            b.add(SHAPE_CONVEX, (), density * vol, pos[i], rot[i], mu=mu, mesh=mid)
     return b.scene(name=f"shape_mix{n}")
+
+
+def served_shape_world(n: int, seed: int, hull_pool: int = 8, grid: int = 16) -> Scene:
+    """A served request's world in the configs[2] shape mix: n bodies (40 % boxes, 25 % spheres,
+    15 % cylinders, 20 % convex hulls from a pool of hull_pool meshes) over a grid x grid
+    heightfield sized to a footprint that stacks the lattice about four cells high."""
+    side = 1.8 * max(3, int(math.ceil(math.sqrt(n / 4.0))))
+    return shape_mix_scene(n, seed=seed, volume=(side, 4 * 1.8, side), grid=grid,
+                           ground_size=(side + 4.0, side + 4.0), hull_pool=hull_pool)

@@ -85,3 +85,34 @@ def test_server_cancel_leaves_others_unchanged():
             break
     for k in (0, 2, 3):
         assert np.array_equal(res[ids[k]], _alone(*reqs[k])), f"request {k}"
+
+
+def test_server_shape_mix_equals_alone():
+    """Requests holding cylinders, convex hulls and a heightfield (their meshes placed in the
+    server's mesh arena, the ranges of a cancelled and of finished requests reused by later
+    ones) end bit-identical to the same scene stepped alone."""
+    from pe_amd import scenes
+    from pe_amd.server import Server
+    sizes = [40, 90, 25, 120, 60, 35, 75]
+    reqs = [(scenes.shape_mix_scene(n, seed=300 + i, volume=(8, 6, 8), grid=8, ground_size=(10, 10), hull_pool=4),
+             int(h)) for i, (n, h) in enumerate(zip(sizes, [12, 30, 8, 20, 16, 10, 14]))]
+    print(f"[server shapes] sizes {[r[0].n for r in reqs]} meshes {[len(r[0].meshes) for r in reqs]}", flush=True)
+    srv = Server(hbm_budget_bytes=1 << 30, max_worlds=8)
+    ids, done = {}, {}
+    pending = list(enumerate(reqs))
+    victim = srv.submit(reqs[1][0], 200)   # cancelled while live: its ranges are reused
+    for t in range(300):
+        if pending and t % 3 == 0:
+            i, (sc, h) = pending.pop(0)
+            ids[srv.submit(sc, h)] = i
+        if t == 4:
+            srv.cancel(victim)
+        srv.tick()
+        for rid in srv.poll():
+            if rid in ids:
+                done[ids[rid]] = flat_state(srv.result(rid))
+        if len(done) == len(reqs):
+            break
+    assert len(done) == len(reqs)
+    for i, (sc, h) in enumerate(reqs):
+        assert np.array_equal(done[i], _alone(sc, h)), f"request {i}"
