@@ -199,13 +199,15 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter) {
 __device__ __forceinline__ bool pair_is_simple(const BodyView& b, const PairStore& ps, int p) {
     return b.iat(I_TYPE, ps.pa[p]) <= SHAPE_SPHERE && b.iat(I_TYPE, ps.pb[p]) <= SHAPE_SPHERE;
 }
-// True when pair p runs the convex separating-axis routine on two polyhedra with edges (cylinder
-// and cylinder, box and convex mesh, convex mesh and convex mesh), whose edge-pair axes number in
-// the thousands: such pairs are solved by a whole wave each.
+// True when pair p runs the convex separating-axis routine on a polyhedron with many faces or
+// edges: cylinder and cylinder, box and convex mesh, convex mesh and convex mesh (edge-pair axes
+// in the thousands), convex mesh and concave mesh (one SAT of the hull's faces per triangle).
+// Such pairs are solved by a whole wave each.
 __device__ __forceinline__ bool pair_is_sat(const BodyView& b, const PairStore& ps, int p) {
     const int ta = b.iat(I_TYPE, ps.pa[p]), tb = b.iat(I_TYPE, ps.pb[p]);
-    return (ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) || (ta == SHAPE_BOX && tb == SHAPE_CONVEX) ||
-           (ta == SHAPE_CONVEX && tb == SHAPE_BOX) || (ta == SHAPE_CONVEX && tb == SHAPE_CONVEX);
+    const int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+    return (lo == SHAPE_CYLINDER && hi == SHAPE_CYLINDER) || (lo == SHAPE_BOX && hi == SHAPE_CONVEX) ||
+           (lo == SHAPE_CONVEX && (hi == SHAPE_CONVEX || hi == SHAPE_CONCAVE));
 }
 
 // The polyhedral view of a box, cylinder or convex-mesh body.
@@ -257,7 +259,7 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
         x.swap = ta == SHAPE_CONCAVE;
         if (ob.type == SHAPE_SPHERE) sphere_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0]);
         else if (ob.type == SHAPE_CYLINDER) cylinder_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0], ob.sp[1]);
-        else concave_poly(c, x, ms, mesh, mb.P, body_poly(ms, ob), ob.P, ref_scale);
+        else concave_poly<1>(c, x, ms, mesh, mb.P, body_poly(ms, ob), ob.P, ref_scale);
     }
     x.swap = false;
 }
@@ -273,7 +275,14 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
 template <bool SHAPES, int G = 1>
 static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
-    Cache c;
+    // The thread-per-pair shape kernels (at most 64 threads per workgroup) keep each lane's
+    // contact cache in LDS: their algorithms take it by reference across calls that are not
+    // inlined, which would otherwise put it in scratch memory.
+    constexpr bool LDS_CACHE = SHAPES && G == 1;
+    struct CachePad { Cache c; float pad; };   // an odd row length: lanes hit distinct banks
+    __shared__ CachePad s_cache[LDS_CACHE ? 64 : 1];
+    Cache c_reg;
+    Cache& c = LDS_CACHE ? s_cache[threadIdx.x % 64].c : c_reg;
     const size_t slot0 = (size_t)p * 8;
     float dprev[8];
     if (p < prev_np) ld_rec<2>(ps.cd + slot0, dprev);
@@ -298,7 +307,16 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
     const int ta = A.type, tb = B.type;
     if (G > 1) {
         const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
-        convex_convex<G>(c, x, body_poly(b.ms, A), body_poly(b.ms, B), A.P, B.P, ref_scale);
+        if (ta == SHAPE_CONCAVE || tb == SHAPE_CONCAVE) {
+            // concave_convex_collision_algorithm.cpp: the convex body is A (narrow_shapes)
+            const CollBody& mb = ta == SHAPE_CONCAVE ? A : B;
+            const CollBody& ob = ta == SHAPE_CONCAVE ? B : A;
+            x.swap = ta == SHAPE_CONCAVE;
+            concave_poly<G>(c, x, b.ms, __float_as_int(mb.sp[0]), mb.P, body_poly(b.ms, ob), ob.P, ref_scale);
+            x.swap = false;
+        } else {
+            convex_convex<G>(c, x, body_poly(b.ms, A), body_poly(b.ms, B), A.P, B.P, ref_scale);
+        }
     } else if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.sp, B.sp);
     else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.sp, B.sp[0]);
     else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, B.sp, A.sp[0]);

@@ -659,13 +659,26 @@ __device__ __noinline__ void box_cylinder(Cache& c, const PairCtx& x, const Pose
 // ------------------------------------------------------------------ BVH search
 // BVH::search (bvh.cpp:105-125): depth first, right child popped before left; calls
 // visit(face) for every face of every overlapped leaf, in the reference's output order.
-template <class F>
+// The stack lives in LDS (a private array would live in scratch memory, whose every push and
+// pop is a round trip beyond L1): one row per lane when each lane walks its own pair (G = 1,
+// kernels of at most BVH_ROWS threads per workgroup), one row per wave when the wave's lanes walk
+// one pair together (G = WAVE: every lane writes the same value to the same word; kernels of at
+// most BVH_ROWS waves per workgroup).
+constexpr int BVH_STACK = 48;
+constexpr int BVH_ROWS = 64;
+template <int G>
+__device__ __forceinline__ int* bvh_stack_row() {
+    __shared__ int s_stack[(G == 1 ? BVH_ROWS : 4) * (BVH_STACK + 1)];   // rows padded to an odd length
+    const int row = G == 1 ? (int)(threadIdx.x % BVH_ROWS) : (int)(threadIdx.x / WAVE) % 4;
+    return s_stack + row * (BVH_STACK + 1);
+}
+template <int G = 1, class F>
 __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& visit) {
     const int32_t* r = ms.rec + MRW * mesh;
     if (r[MR_NB] == 0) return;
     const float4* node = ms.arena + r[MR_NODE];
     const int2* leaf = reinterpret_cast<const int2*>(ms.arena + r[MR_LEAF]);
-    int st[48];
+    int* st = bvh_stack_row<G>();
     int sp = 0;
     st[sp++] = 0;
     while (sp > 0) {
@@ -677,7 +690,7 @@ __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& 
         const int2 lf = leaf[id];
         if (lf.x > 0) {
             for (int i = 0; i < lf.x; i++) visit(lf.y + i);
-        } else if (sp + 2 <= 48) {
+        } else if (sp + 2 <= BVH_STACK) {
             st[sp++] = __float_as_int(lo.w);
             st[sp++] = __float_as_int(hi.w);
         }
@@ -886,6 +899,7 @@ __device__ __noinline__ void sphere_mesh(Cache& c, const PairCtx& x, const MeshS
 // ConcaveConvexCollisionAlgorithm::getClosestPoints (concave_convex_collision_algorithm.cpp:38-68):
 // each triangle of a BVH leaf overlapping the polyhedron's box (in the mesh frame) is a one-face
 // shape without unique edges, run through the convex routine with the polyhedron as A.
+template <int G>
 __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const MeshSet& ms, int mesh, const Pose& pm,
                                           const Poly& A, const Pose& pa, float ref_scale) {
     const Pose rel = pose_mul(pose_inv(pm), pa);
@@ -905,7 +919,7 @@ __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const Mesh
         bb = add(bb, rel.p);
     }
     const Poly M = poly_mesh(&ms, mesh);
-    bvh_visit(ms, mesh, aa, bb, [&](int f) {
+    bvh_visit<G>(ms, mesh, aa, bb, [&](int f) {
         Poly T;
         T.kind = PK_TRI;
         T.ms = &ms;
@@ -913,7 +927,7 @@ __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const Mesh
         T.t[1] = face_vertex(M, f, 1);
         T.t[2] = face_vertex(M, f, 2);
         T.tn = face_normal(M, f);
-        convex_convex<1>(c, x, A, T, pa, pm, ref_scale);
+        convex_convex<G>(c, x, A, T, pa, pm, ref_scale);
     });
 }
 
