@@ -199,15 +199,16 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter) {
 __device__ __forceinline__ bool pair_is_simple(const BodyView& b, const PairStore& ps, int p) {
     return b.iat(I_TYPE, ps.pa[p]) <= SHAPE_SPHERE && b.iat(I_TYPE, ps.pb[p]) <= SHAPE_SPHERE;
 }
-// True when pair p runs the convex separating-axis routine on a polyhedron with many faces or
-// edges: cylinder and cylinder, box and convex mesh, convex mesh and convex mesh (edge-pair axes
-// in the thousands), convex mesh and concave mesh (one SAT of the hull's faces per triangle).
-// Such pairs are solved by a whole wave each.
+// True when pair p runs the convex separating-axis routine (and its clipper): cylinder and
+// cylinder, box and convex mesh, convex mesh and convex mesh (edge-pair axes in the thousands),
+// box or convex mesh and concave mesh (one test per overlapped triangle).  Such pairs are solved
+// by a whole wave each.
 __device__ __forceinline__ bool pair_is_sat(const BodyView& b, const PairStore& ps, int p) {
     const int ta = b.iat(I_TYPE, ps.pa[p]), tb = b.iat(I_TYPE, ps.pb[p]);
     const int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
     return (lo == SHAPE_CYLINDER && hi == SHAPE_CYLINDER) || (lo == SHAPE_BOX && hi == SHAPE_CONVEX) ||
-           (lo == SHAPE_CONVEX && (hi == SHAPE_CONVEX || hi == SHAPE_CONCAVE));
+           (lo == SHAPE_CONVEX && hi == SHAPE_CONVEX) ||
+           ((lo == SHAPE_BOX || lo == SHAPE_CONVEX) && hi == SHAPE_CONCAVE);
 }
 
 // The polyhedral view of a box, cylinder or convex-mesh body.
