@@ -96,12 +96,13 @@ __host__ inline size_t broad_lds_bytes(int nmax) {
     return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
 // K3: LDS of a group of G bodies (G = max(group bodies, largest world)): claims and body state
-// (68 B per body), and per pair (estimated at 2 per body) its global slot, local bodies, row
-// start, order slot and three colouring-list slots (28 B); capped so a workgroup fits a CU.
+// (68 B per body), and per pair (estimated at 2 per body) its global slot, local bodies, claim
+// priority, row start, order slot and three colouring-list slots (32 B); capped so a workgroup
+// fits a CU.
 constexpr size_t SOLVE_LDS_CAP = 144 * 1024;
 __host__ inline size_t solve_lds_bytes(int nmax, int group_bodies) {
     const size_t g = (size_t)std::max(std::max(nmax, group_bodies), 1);
-    return std::min(SOLVE_LDS_CAP, 68 * g + 28 * 2 * g + 128);
+    return std::min(SOLVE_LDS_CAP, 68 * g + 32 * 2 * g + 144);
 }
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
@@ -320,7 +321,7 @@ __device__ void solve_world_global(const BatchDev& d, int w, int* claim, int* ws
     int* order = d.order + poff;
     int* cs = d.cstart + wd.cs_off;
     int* bufs[3] = {act, act + wd.pair_cap, act + 2 * (size_t)wd.pair_cap};
-    const int ncol = colour_pairs<true>(b, d.ps, bufs, nact, order, cs, claim, off, wsum);
+    const int ncol = colour_pairs<true>(b, d.ps, bufs, nact, order, cs, claim, off, poff, wsum);
     if (tid == 0) {
         d.st[w].ncolors = ncol;
         d.st[w].npairs = npairs;
@@ -352,13 +353,14 @@ struct Group {
 // LDS layout of the preparation kernel; the group is solved on the global path when it does
 // not fit lds_bytes (both kernels evaluate this the same way).
 struct PrepLayout {
-    size_t bs, pab, gi, rs, ord, buf, cs, end;
+    size_t bs, pab, gi, pr, rs, ord, buf, cs, end;
     __device__ PrepLayout(int GB, int GP) {
         auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
         bs = al16((size_t)GB * 4);
         pab = bs + (size_t)GB * 64;
         gi = al16(pab + (size_t)GP * 4);
-        rs = al16(gi + (size_t)GP * 4);
+        pr = al16(gi + (size_t)GP * 4);
+        rs = al16(pr + (size_t)GP * 4);
         ord = al16(rs + (size_t)(GP + 1) * 4);
         buf = al16(ord + (size_t)GP * 4);
         cs = al16(buf + (size_t)GP * 12);
@@ -418,6 +420,7 @@ __global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t*
     float* imk = (float*)(lds + Lo.bs);   // inverse mass, negative for kinematic bodies
     int* pab = (int*)(lds + Lo.pab);
     int* gidx = (int*)(lds + Lo.gi);
+    int* pri = (int*)(lds + Lo.pr);
     int* order = (int*)(lds + Lo.ord);
     int* bufs[3] = {(int*)(lds + Lo.buf), (int*)(lds + Lo.buf) + GP, (int*)(lds + Lo.buf) + 2 * GP};
     int* cs = (int*)(lds + Lo.cs);
@@ -436,6 +439,7 @@ __global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t*
             const int pa = d.ps.pa[gs], pb = d.ps.pb[gs];
             const int np_ = d.ps.npts[gs];
             gidx[lp] = gs;
+            pri[lp] = colour::prio(lp - s_wb[k]);
             pab[lp] = (pa - B0) | ((pb - B0) << 16);
             if (np_ > 0) {
                 flag = !(imk[pa - B0] < 0.0f && imk[pb - B0] < 0.0f);
@@ -451,7 +455,7 @@ __global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t*
     // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
     for (int lp = tid; lp < GP; lp += BLOCK) setup_pair(b, d.ps, gidx[lp], d.dt);
     PE_PHASE(5);
-    const int ncol = colour_group(pab, imk, bufs, nact, order, cs, claim);
+    const int ncol = colour_group(pab, imk, pri, bufs, nact, order, cs, claim);
     PE_PHASE(6);
     const WorldDesc w0 = d.desc[G.wl[0]];
     for (int q = tid; q < nact; q += BLOCK) d.order[w0.pair_off + q] = order[q];

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_colour(Dev d) {
     }
     __syncthreads();
     int* bufs[3] = {act, act + d.pcap, act + 2 * (size_t)d.pcap};
-    const int ncol = colour_pairs<LDS_CLAIM>(b, d.ps, bufs, nact, d.order, d.color_start, claim, 0, wsum);
+    const int ncol = colour_pairs<LDS_CLAIM>(b, d.ps, bufs, nact, d.order, d.color_start, claim, 0, 0, wsum);
     if (threadIdx.x == 0) {
         d.cnt->ncolors = ncol;
         d.cnt->nordered = d.color_start[ncol];
