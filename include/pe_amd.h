@@ -142,6 +142,10 @@ int pe_world_get_pairs(pe_world* w, int32_t* pairs /*2*cap*/, int32_t cap, int32
  * 8 slots in cache order (PE_CONTACT_FLOATS floats each). */
 int pe_world_get_contacts(pe_world* w, int32_t* npoints /*cap*/, float* points /*cap*8*13*/,
                           int32_t cap);
+/* Accumulated impulses of the last solve, per pair and contact slot (slots in cache order, zero
+ * beyond the point count): normal, tangent 0, tangent 1 -- ConstraintInfo::n_applied_impulse,
+ * t0_applied_impulse, t1_applied_impulse (friction_contact_constraint.h:25-27). */
+int pe_world_get_impulses(pe_world* w, float* impulses /*cap*8*3*/, int32_t cap);
 /* Solve order of the last solve: pair indices grouped by colour, colour start offsets. */
 int pe_world_get_solve_order(pe_world* w, int32_t* order /*cap*/, int32_t cap, int32_t* nordered,
                              int32_t* color_start /*ccap+1*/, int32_t ccap, int32_t* ncolors);

@@ -697,6 +697,23 @@ int pe_world_get_contacts(pe_world* w, int32_t* npoints, float* points, int32_t 
     }
     return PE_OK;
 }
+int pe_world_get_impulses(pe_world* w, float* impulses, int32_t cap) {
+    if (int rc = check_world(w)) return rc;
+    if (!impulses) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    const int m = std::min(cap, c.npairs);
+    if (m <= 0) return PE_OK;
+    std::vector<int32_t> np(m);
+    std::vector<float> rows((size_t)m * 8 * RW);
+    PE_HIP(hipMemcpy(np.data(), w->d.ps.npts, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PE_HIP(hipMemcpy(rows.data(), w->d.ps.rows, rows.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < (size_t)m * 8; s++) {
+        const bool valid = (int)(s % 8) < np[s / 8];
+        for (int k = 0; k < 3; k++) impulses[s * 3 + k] = valid ? rows[s * RW + R_LN + k] : 0.0f;
+    }
+    return PE_OK;
+}
 int pe_world_get_solve_order(pe_world* w, int32_t* order, int32_t cap, int32_t* nordered, int32_t* color_start,
                              int32_t ccap, int32_t* ncolors) {
     if (int rc = check_world(w)) return rc;

@@ -20,7 +20,7 @@ EXPORTS = [
     "pe_last_error", "pe_device_count", "pe_mesh_prepare",
     "pe_world_create", "pe_world_destroy", "pe_world_step", "pe_world_integrate", "pe_world_broadphase",
     "pe_world_narrowphase", "pe_world_solve", "pe_world_sync", "pe_world_get_state", "pe_world_set_state",
-    "pe_world_get_aabbs", "pe_world_get_pairs", "pe_world_get_contacts", "pe_world_get_solve_order",
+    "pe_world_get_aabbs", "pe_world_get_pairs", "pe_world_get_contacts", "pe_world_get_impulses", "pe_world_get_solve_order",
     "pe_world_get_axis", "pe_world_set_axis", "pe_world_set_timing", "pe_world_get_stage_times",
     "pe_world_get_counts", "pe_world_set_solver_blocks",
     "pe_batch_create", "pe_batch_destroy", "pe_batch_step", "pe_batch_sync", "pe_batch_get_world_state",

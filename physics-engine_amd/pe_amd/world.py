@@ -112,6 +112,15 @@ class World:
         check(lib().pe_world_get_contacts(self.h, _i(npts), _f(pts), ctypes.c_int32(P)))
         return npts[:P].copy(), pts[:P].copy()
 
+    def impulses(self):
+        """[P, 8, 3] accumulated (normal, tangent 0, tangent 1) impulses of the last solve."""
+        cnt = ctypes.c_int32()
+        check(lib().pe_world_get_pairs(self.h, None, 0, ctypes.byref(cnt)))
+        P = cnt.value
+        out = np.zeros((max(P, 1), 8, 3), np.float32)
+        check(lib().pe_world_get_impulses(self.h, _f(out), ctypes.c_int32(P)))
+        return out[:P].copy()
+
     def solve_order(self):
         """(order, color_start): pair indices grouped by colour."""
         no, nc = ctypes.c_int32(), ctypes.c_int32()

@@ -35,3 +35,36 @@ def brute_force_pairs(mn, mx, kin):
 
 def valid_mask(pts):
     return pts[..., 0] < 1e29
+
+
+def normal_residual(ab, npts, pts, imp, rot, v, w, inv_mass, local_inv_inertia, kin, dt):
+    """FP64 natural residual of the normal contact constraints after a solve.  For every contact
+    row, r = lambda - max(0, lambda + m_n (b - n . v_rel)): the change one more projected
+    Gauss-Seidel pass would make to the accumulated normal impulse (zero at the solution of the
+    row's complementarity problem), with v_rel = (v_A + w_A x r_A) - (v_B + w_B x r_B),
+    m_n = 1 / (1/m_A + 1/m_B + (I_A^-1 (r_A x n)).(r_A x n) + (I_B^-1 (r_B x n)).(r_B x n)) and the
+    target b = -0.2 max(d, -0.3) / dt for restitution 0
+    (FrictionContactConstraint::initSequentialImpulse / iterateSequentialImpulse,
+    friction_contact_constraint.cpp:10-108).  Returns (RMS residual, RMS normal impulse, rows)."""
+    res, lam = [], []
+    for p in range(len(ab)):
+        a, b = int(ab[p][0]), int(ab[p][1])
+        Ra, Rb = np.asarray(rot[a], np.float64), np.asarray(rot[b], np.float64)
+        ima = 0.0 if kin[a] else float(inv_mass[a])
+        imb = 0.0 if kin[b] else float(inv_mass[b])
+        Ia = np.zeros((3, 3)) if kin[a] else Ra @ local_inv_inertia[a] @ Ra.T
+        Ib = np.zeros((3, 3)) if kin[b] else Rb @ local_inv_inertia[b] @ Rb.T
+        for k in range(int(npts[p])):
+            c = np.asarray(pts[p, k], np.float64)
+            n, la, lb, d = c[3:6], c[6:9], c[9:12], c[12]
+            ra, rb = Ra @ la, Rb @ lb
+            xa, xb = np.cross(ra, n), np.cross(rb, n)
+            mn = 1.0 / (ima + imb + (Ia @ xa) @ xa + (Ib @ xb) @ xb)
+            vr = (np.asarray(v[a], np.float64) + np.cross(w[a], ra)) - (np.asarray(v[b], np.float64) + np.cross(w[b], rb))
+            target = -0.2 * max(d, -0.3) / dt
+            lmb = float(imp[p, k, 0])
+            res.append(lmb - max(0.0, lmb + mn * (target - n @ vr)))
+            lam.append(lmb)
+    res, lam = np.asarray(res), np.asarray(lam)
+    rms = lambda x: float(np.sqrt(np.mean(x ** 2))) if len(x) else 0.0
+    return rms(res), rms(lam), len(res)

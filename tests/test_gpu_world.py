@@ -10,7 +10,7 @@ Tolerances (written here, applied below):
 import numpy as np
 import pytest
 
-from helpers import brute_force_pairs, flat_state, mixed_small, pair_set, small_rain, valid_mask
+from helpers import brute_force_pairs, flat_state, mixed_small, normal_residual, pair_set, small_rain, valid_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -117,3 +117,36 @@ def test_solve_order_is_a_valid_coloring(all_scenes):
         bodies = ab[order[cs[c]:cs[c + 1]]].ravel()
         dyn = bodies[~kin[bodies]]
         assert len(dyn) == len(np.unique(dyn)), f"colour {c} shares a dynamic body"
+
+
+def test_fp32_residual_against_fp64_oracle():
+    """The FP32 device solve's normal-contact residual (measured in FP64) beside the residual of the
+    FP64 oracle's own solve from the same state: both carry the 10-iteration Gauss-Seidel's
+    unconverged part; FP32 rounding must not add materially to it (bound written below)."""
+    import pe_amd
+    from pyoracle import OracleWorld
+    sc = small_rain(300)
+    assert np.all(sc.restitution == 0)
+    kin = sc.kinematic.astype(bool)
+    g, o = pe_amd.World(sc), OracleWorld(sc, 64)
+    inv_mass, linv = o.mass_props()
+    g.step(60)
+    r32, r64, l32, l64, rows = [], [], [], [], 0
+    for step in range(30):
+        g.sync()
+        o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+        g.integrate(); g.broadphase(); g.narrowphase(); g.solve()
+        o.integrate(); o.broadphase(); o.narrowphase(); o.solve()
+        g.sync()
+        pos, rot, v, w = g.state()
+        ab, (npts, pts), imp = g.pairs(), g.contacts(), g.impulses()
+        a, b_, n_ = normal_residual(ab, npts, pts, imp, rot, v, w, inv_mass, linv, kin, sc.dt)
+        opos, orot, ov, ow = o.state()
+        onp, _, opts = o.contacts()
+        c, d, _ = normal_residual(o.pairs(), onp, opts, o.impulses(), orot, ov, ow, inv_mass, linv, kin, sc.dt)
+        r32.append(a); l32.append(b_); r64.append(c); l64.append(d); rows += n_
+    R32, R64 = float(np.sqrt(np.mean(np.square(r32)))), float(np.sqrt(np.mean(np.square(r64))))
+    L = float(np.sqrt(np.mean(np.square(l64))))
+    print(f"[residual] {rows} normal rows over 30 steps: FP32 device RMS {R32:.4e}, FP64 oracle RMS {R64:.4e} "
+          f"(N s; RMS normal impulse {L:.4e}, relative {R32 / L:.3e} vs {R64 / L:.3e})", flush=True)
+    assert R32 <= 1.5 * R64 + 1e-4 * L
