@@ -145,7 +145,11 @@ def run_served(args):
     elapsed = time.perf_counter() - start
     st = srv.stats()
     srv.close()
-    return dict(workload=f"Poisson arrivals at {lam:g}/s for {T:g} s; body counts log-uniform 16-4096, horizons "
+    # the plain single-world path on the same request worlds, one at a time from their first step
+    plain_same = run_plain_sample(pool[:32], 20)
+    return dict(plain_same_worlds_body_steps_per_s=plain_same,
+                served_over_plain_same_worlds=st["body_steps"] / elapsed / plain_same,
+                workload=f"Poisson arrivals at {lam:g}/s for {T:g} s; body counts log-uniform 16-4096, horizons "
                          f"U(30, 600); mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a "
                          f"16x16 heightfield), {len(pool)} distinct request worlds cycled",
                 rate=lam, seconds=elapsed, submitted=nxt, completed=done, ticks=ticks,

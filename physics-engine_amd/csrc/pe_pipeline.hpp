@@ -276,14 +276,15 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
 template <bool SHAPES, int G = 1>
 static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
-    // The thread-per-pair shape kernels (at most 64 threads per workgroup) keep each lane's
-    // contact cache in LDS: their algorithms take it by reference across calls that are not
-    // inlined, which would otherwise put it in scratch memory.
-    constexpr bool LDS_CACHE = SHAPES && G == 1;
+    // The shape kernels keep the contact cache in LDS: their algorithms take it by reference across
+    // calls that are not inlined, which would otherwise put it in scratch memory.  One row per lane
+    // in the thread-per-pair kernels (at most 64 threads per workgroup), one per wave when a wave
+    // works on one pair (at most 4 waves per workgroup; every lane writes the same values).
+    constexpr bool LDS_CACHE = SHAPES;
     struct CachePad { Cache c; float pad; };   // an odd row length: lanes hit distinct banks
-    __shared__ CachePad s_cache[LDS_CACHE ? 64 : 1];
+    __shared__ CachePad s_cache[LDS_CACHE ? (G == 1 ? 64 : 4) : 1];
     Cache c_reg;
-    Cache& c = LDS_CACHE ? s_cache[threadIdx.x % 64].c : c_reg;
+    Cache& c = LDS_CACHE ? s_cache[G == 1 ? threadIdx.x % 64 : (threadIdx.x / WAVE) % 4].c : c_reg;
     const size_t slot0 = (size_t)p * 8;
     float dprev[8];
     if (p < prev_np) ld_rec<2>(ps.cd + slot0, dprev);

@@ -297,7 +297,12 @@ __device__ __noinline__ void clip_hull_hull(Cache& c, const PairCtx& x, f3 sep, 
     }
     L::argmin(ad, ca);
     if (ca == 0x7fffffff) return;
-    f3 buf0[CLIP_CAP], buf1[CLIP_CAP];
+    // polygon buffers: a lane's own (scratch) when it works alone; one copy per wave in LDS when the
+    // wave's lanes work together (every lane writes the same value to the same word)
+    f3 pbuf0[G == 1 ? CLIP_CAP : 1], pbuf1[G == 1 ? CLIP_CAP : 1];
+    __shared__ f3 s_clip[G == 1 ? 1 : 4][2][CLIP_CAP];
+    f3* buf0 = G == 1 ? pbuf0 : s_clip[(threadIdx.x / WAVE) % 4][0];
+    f3* buf1 = G == 1 ? pbuf1 : s_clip[(threadIdx.x / WAVE) % 4][1];
     int n = face_len(B, cb);
     for (int k = 0; k < n; k++) buf0[k] = xf_apply(tb, face_vertex(B, cb, k));
     const int nva = face_len(A, ca);
