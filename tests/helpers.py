@@ -68,3 +68,35 @@ def normal_residual(ab, npts, pts, imp, rot, v, w, inv_mass, local_inv_inertia, 
     res, lam = np.asarray(res), np.asarray(lam)
     rms = lambda x: float(np.sqrt(np.mean(x ** 2))) if len(x) else 0.0
     return rms(res), rms(lam), len(res)
+
+
+def reference_sample_scene(path):
+    """A scene file of the reference (SampleScenes/*.json, as Simulator::loadScene reads it,
+    src/intf/simulator.cpp:330-400): boxes (scale = full size) and spheres (scale[0] = radius),
+    pose, mass, friction, restitution, kinematic flag; velocities and damping for dynamic bodies
+    only; gravity from the configuration."""
+    import json
+    from pe_amd.scenes import SHAPE_BOX, SHAPE_SPHERE, SceneBuilder
+    data = json.load(open(path))
+    b = SceneBuilder()
+    lin, ang = [], []
+    for rb in data["RigidBodies"]:
+        kin = 1 if rb.get("kinematic", False) else 0
+        rot = np.asarray(rb.get("rotation", np.eye(3).ravel()), float).reshape(3, 3)
+        v = rb.get("linear_velocity", (0, 0, 0)) if not kin else (0, 0, 0)
+        w = rb.get("angular_velocity", (0, 0, 0)) if not kin else (0, 0, 0)
+        if rb["type"] == "box":
+            shape, param = SHAPE_BOX, tuple(rb.get("scale", (1, 1, 1)))
+        elif rb["type"] == "sphere":
+            shape, param = SHAPE_SPHERE, (rb.get("scale", (1,))[0],)
+        else:
+            raise ValueError(f"shape {rb['type']} not in this loader")
+        b.add(shape, param, rb.get("mass", 1.0), rb.get("position", (0, 0, 0)), rot=rot, v=v, w=w,
+              mu=rb.get("friction", 0.5), e=rb.get("restitution", 0.0), kin=kin)
+        lin.append(0.0 if kin else rb.get("linear_damping", 0.0))
+        ang.append(0.0 if kin else rb.get("angular_damping", 0.0))
+    g = tuple(data.get("Configuration", {}).get("gravity", (0.0, -9.8, 0.0)))
+    sc = b.scene(gravity=g, name="bomb_crash_tower_smaller")
+    sc.linear_damping = np.asarray(lin, float)
+    sc.angular_damping = np.asarray(ang, float)
+    return sc.contiguous()

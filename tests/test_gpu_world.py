@@ -10,7 +10,8 @@ Tolerances (written here, applied below):
 import numpy as np
 import pytest
 
-from helpers import brute_force_pairs, flat_state, mixed_small, normal_residual, pair_set, small_rain, valid_mask
+from helpers import (brute_force_pairs, flat_state, mixed_small, normal_residual, pair_set, reference_sample_scene,
+                     small_rain, valid_mask)
 
 pytestmark = pytest.mark.gpu
 
@@ -150,3 +151,38 @@ def test_fp32_residual_against_fp64_oracle():
     print(f"[residual] {rows} normal rows over 30 steps: FP32 device RMS {R32:.4e}, FP64 oracle RMS {R64:.4e} "
           f"(N s; RMS normal impulse {L:.4e}, relative {R32 / L:.3e} vs {R64 / L:.3e})", flush=True)
     assert R32 <= 1.5 * R64 + 1e-4 * L
+
+
+def test_reference_sample_scene_stages():
+    """The reference's SampleScenes/bomb_crash_tower_smaller.json (restitution 0.5, angular damping
+    0.8, a 70 m/s sphere into a ring of boxes) in lock step with the oracle for 150 steps, every
+    stage compared on every step (tolerances above), the oracle restarted from the device state
+    after each step."""
+    import os
+    import pe_amd
+    from pyoracle import OracleWorld
+    sc = reference_sample_scene(os.path.join(os.path.dirname(__file__), "golden", "bomb_crash_tower_smaller.json"))
+    g, o = pe_amd.World(sc), OracleWorld(sc, 32)
+    print(f"[sample scene] n={sc.n}", flush=True)
+    contacts = 0
+    for step in range(150):
+        g.integrate(); o.integrate()
+        g.sync()
+        assert np.array_equal(flat_state(g.state()), flat_state(o.state()).astype(np.float32)), f"state step {step}"
+        g.broadphase(); o.broadphase()
+        gp, op = g.pairs(), o.pairs()
+        assert np.array_equal(gp, op), f"pair order step {step}"
+        g.narrowphase(); o.narrowphase()
+        gn, gpts = g.contacts()
+        on, _, opts = o.contacts()
+        assert np.array_equal(gn, on), f"point counts step {step}"
+        vm = np.arange(8)[None, :] < on[:, None]
+        assert _close(gpts[vm], opts[vm]), f"contact points step {step}"
+        contacts += int(on.sum())
+        g.solve(); o.solve()
+        g.sync()
+        _, _, gv, gw = g.state()
+        _, _, ov, ow = o.state()
+        assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+    assert contacts > 0

@@ -1,12 +1,13 @@
 """The oracle against the reference's own known-answer tests (test/rigid/**/test.cpp) and
 against closed forms.  CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
 
 from pe_amd import scenes
-from pe_amd.scenes import SHAPE_BOX, SHAPE_CYLINDER, SHAPE_SPHERE, SceneBuilder
+from pe_amd.scenes import SHAPE_BOX, SHAPE_CONVEX, SHAPE_CYLINDER, SHAPE_SPHERE, SceneBuilder
 from pyoracle import OracleWorld
 
 T_EPS = 1e-4   # test/test_general.h
@@ -220,3 +221,78 @@ def test_colouring_invariants():
             assert any((not kin[x]) and x in taken for x in ab[p])
     active = (npts > 0) & ~(kin[ab[:, 0]] & kin[ab[:, 1]])
     assert np.array_equal(col >= 0, active)
+
+
+def _load_obj(path):
+    """Vertices and faces (0-based vertex indices) of a Wavefront .obj file."""
+    v, f = [], []
+    for line in open(path):
+        p = line.split()
+        if not p:
+            continue
+        if p[0] == "v":
+            v.append([float(x) for x in p[1:4]])
+        elif p[0] == "f":
+            f.append([int(t.split("/")[0]) - 1 for t in p[1:]])
+    return np.array(v), f
+
+
+CYL_OBJ = os.path.join(os.path.dirname(__file__), "golden", "cylinder_default_mesh_setmesh.obj")
+
+
+def _face_key(v, face):
+    return tuple(sorted(tuple(np.round(v[i], 5)) for i in face))
+
+
+def _cylinder_default(bits):
+    """The oracle's unit cylinder (r 0.5, h 1): the default cylinder mesh a CylinderShape uses."""
+    rows = [dict(shape=SHAPE_CYLINDER, param=(0.5, 1.0), mass=1.0, pos=(0, 0, 0), kin=1)]
+    _, o = _world(rows, bits)
+    return o.poly(0)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_kat_cylinder_mesh_setmesh(bits):
+    """test/rigid/shape/convex_mesh_shape/test.cpp:70-81: the default cylinder mesh through
+    ConvexMeshShape::setMesh has 80 unique edges, 80 vertices and 42 faces.  The mesh.obj beside
+    that test (kept here as tests/golden/cylinder_default_mesh_setmesh.obj) is an output of that
+    test: the prepared mesh, re-centred.
+    * The oracle's cylinder mesh has the same counts and the same faces (as vertex sets, to the
+      file's six significant digits).
+    * The same mesh through the oracle's setMesh as a convex mesh keeps the counts and the face
+      set.  (The file's face order is not used: the reference built from its current sources, FP32
+      or FP64, orders these faces differently from the file, and the oracle follows the build --
+      test_oracle_vs_ref.py::test_cylinder_mesh_setmesh_order_matches_reference.)"""
+    P = _cylinder_default(bits)
+    assert len(P["v"]) == 80 and len(P["faces"]) == 42 and len(P["edges"]) == 80
+    v_ref, f_ref = _load_obj(CYL_OBJ)
+    assert len(v_ref) == 80 and len(f_ref) == 42
+    ref_set = sorted(_face_key(v_ref, f) for f in f_ref)
+    assert sorted(_face_key(P["v"], f) for f in P["faces"]) == ref_set
+    mesh = scenes.Mesh(vertices=P["v"], faces=P["faces"], normals=P["fn"])
+    b = SceneBuilder()
+    b.add(SHAPE_CONVEX, (), 1.0, (0, 0, 0), kin=1, mesh=b.add_mesh(mesh))
+    o = OracleWorld(b.scene(dt=0.0, gravity=(0, 0, 0)), bits, color_order=False)
+    Q = o.poly(0)
+    assert len(Q["v"]) == 80 and len(Q["faces"]) == 42 and len(Q["edges"]) == 80
+    assert sorted(_face_key(Q["v"], f) for f in Q["faces"]) == ref_set
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_kat_convex_mesh_aabb(bits):
+    """test/rigid/shape/convex_mesh_shape/test.cpp:83-105: a convex mesh body holding the default
+    cylinder mesh has the AABB [-0.5, 0.5]^3 at rest and [0.5, 1.5] x [1.5, 2.5] x [2.5, 3.5]
+    rotated by pi/4 about y and moved to (1, 2, 3) (T_EPS)."""
+    P = _cylinder_default(bits)
+    mesh = scenes.Mesh(vertices=P["v"], faces=P["faces"], normals=P["fn"])
+    c, s = math.cos(math.pi / 4), math.sin(math.pi / 4)
+    ry = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    b = SceneBuilder()
+    m = b.add_mesh(mesh)
+    b.add(SHAPE_CONVEX, (), 1.0, (0, 0, 0), kin=1, mesh=m)
+    b.add(SHAPE_CONVEX, (), 1.0, (1, 2, 3), rot=ry, kin=1, mesh=m)
+    o = OracleWorld(b.scene(dt=0.0, gravity=(0, 0, 0)), bits, color_order=False)
+    o.integrate()
+    mn, mx = o.aabbs()
+    assert np.all(np.abs(mn[0] + 0.5) < T_EPS) and np.all(np.abs(mx[0] - 0.5) < T_EPS)
+    assert np.all(np.abs(mn[1] - (0.5, 1.5, 2.5)) < T_EPS) and np.all(np.abs(mx[1] - (1.5, 2.5, 3.5)) < T_EPS)

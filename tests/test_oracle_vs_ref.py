@@ -6,7 +6,7 @@ so both continue from the same pair list.  Skipped where oracle/_ref is not buil
 import numpy as np
 import pytest
 
-from helpers import flat_state, pair_set, small_rain
+from helpers import flat_state, pair_set, reference_sample_scene, small_rain
 from pyoracle import OracleWorld, RefWorld
 
 pytestmark = pytest.mark.ref
@@ -101,7 +101,40 @@ def test_shape_meshes_match_reference(bits):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+def test_cylinder_mesh_setmesh_order_matches_reference(bits):
+    """The default cylinder mesh handed to setMesh as a convex mesh (the reference's own
+    convex_mesh_shape test case): vertices, faces in the post-BVH order, normals, unique edges,
+    volume and inertia equal the compiled reference's, bit for bit."""
+    from pe_amd import scenes
+    from pe_amd.scenes import SHAPE_CONVEX, SHAPE_CYLINDER, SceneBuilder
+    b = SceneBuilder()
+    b.add(SHAPE_CYLINDER, (0.5, 1.0), 1.0, (0, 0, 0), kin=1)
+    P = OracleWorld(b.scene(dt=0.0, gravity=(0, 0, 0)), bits, color_order=False).poly(0)
+    b = SceneBuilder()
+    b.add(SHAPE_CONVEX, (), 1.0, (0, 0, 0), kin=1,
+          mesh=b.add_mesh(scenes.Mesh(vertices=P["v"], faces=P["faces"], normals=P["fn"])))
+    sc = b.scene(dt=0.0, gravity=(0, 0, 0))
+    a, r = OracleWorld(sc, bits, color_order=False).poly(0), RefWorld(sc, bits, threads=2).poly(0)
+    for k in ("v", "fn", "edges"):
+        assert np.array_equal(a[k], r[k]), k
+    assert a["faces"] == r["faces"]
+    assert a["volume"] == r["volume"] and np.array_equal(a["inertia"], r["inertia"])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
 def test_shape_mix_bit_exact(bits):
     """Boxes, spheres, cylinders and convex hulls over a heightfield: all 14 shape pair
     types, every stage bit-exact for 150 steps."""
     _run(_shape_mix(), bits, 150, exact=True)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_reference_sample_scene_bit_exact(bits):
+    """SampleScenes/bomb_crash_tower_smaller.json (tests/golden/): a ring of 16 boxes hit by a 70 m/s
+    sphere, restitution 0.5, angular damping 0.8 -- the restitution and damping paths the synthetic
+    scenes leave at zero.  150 steps, through the impact (about step 60).  The file's sleep
+    thresholds are not applied: the World's defaults (0) keep every body awake in both engines."""
+    import os
+    sc = reference_sample_scene(os.path.join(os.path.dirname(__file__), "golden", "bomb_crash_tower_smaller.json"))
+    assert sc.n == 17 and np.any(sc.restitution > 0) and np.any(sc.angular_damping > 0)
+    _run(sc, bits, 150, exact=True)
