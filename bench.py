@@ -59,6 +59,8 @@ def run_batched(args, worlds):
     b.step(args.warmup)
     b.sync()
     bt_start = b.steps_done
+    # the worlds' states at the first timed step, for the CPU baseline
+    start_states = [b.world_state(i) for i in range(len(worlds))] if not args.no_cpu else []
     c0 = b.counts()
     b.sync()
     w0 = time.perf_counter()
@@ -72,6 +74,7 @@ def run_batched(args, worlds):
     st_ms = [x / nst for x in b.stage_times(nst)]
     c2 = b.counts()
     return dict(wall=wall, dev_ms=dev_ms, counts0=c0, counts1=c1, pairs=pairs, points=points, start_step=bt_start,
+                start_states=start_states,
                 n=b.n_bodies, n_dyn=b.n_dynamic, tuned=tuned, batch=b, stage_ms=st_ms, stage_counts=c2)
 
 
@@ -88,12 +91,17 @@ def run_plain(scene, steps, warmup=0):
     return dt, w.counts()
 
 
-def run_plain_sample(worlds, steps):
+def run_plain_sample(worlds, steps, settle=0):
+    """Body-steps/s of the plain single-world path on the given worlds one at a time: each world
+    steps `settle` untimed steps (so the timed ones see the contact load the batched or served
+    steps see), then `steps` timed ones."""
     import pe_amd
     tot = 0.0
     bodies = 0
     for sc in worlds:
         w = pe_amd.World(sc)
+        if settle:
+            w.step(settle)
         w.sync()
         t0 = time.perf_counter()
         w.step(steps)
@@ -146,8 +154,9 @@ def run_served(args):
     st = srv.stats()
     srv.close()
     # the plain single-world path on the same request worlds, one at a time from their first step
-    plain_same = run_plain_sample(pool[:32], 20)
+    plain_same = run_plain_sample(pool[:32], 20, settle=150)
     return dict(plain_same_worlds_body_steps_per_s=plain_same,
+                plain_same_worlds_sample="first 32 request worlds, 150 untimed then 20 timed steps each",
                 served_over_plain_same_worlds=st["body_steps"] / elapsed / plain_same,
                 workload=f"Poisson arrivals at {lam:g}/s for {T:g} s; body counts log-uniform 16-4096, horizons "
                          f"U(30, 600); mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a "
@@ -195,20 +204,25 @@ def run_shape_mix(args):
     return out
 
 
-def cpu_baseline(worlds, threads, budget_s):
+def cpu_baseline(worlds, states, start_step, threads, budget_s):
     """The reference engine itself (oracle/_ref/libperef32.so, compiled from its sources) when
     present, else the oracle restatement; worlds stepped one after another, each with the
-    reference's batched-loop thread pool, until about budget_s seconds of CPU work."""
+    reference's batched-loop thread pool, until about budget_s seconds of CPU work.  Each world
+    starts from its state at the batched leg's first timed step (its contact caches empty), so
+    the CPU sees the contact load the timed GPU steps see."""
     from pyoracle import OracleWorld, RefWorld, ref_available
     kind = "reference" if ref_available(32) else "port"
     steps = 10
     tot_t, body_steps, nw = 0.0, 0, 0
-    for sc in worlds:
+    for sc, st in zip(worlds, states):
+        st64 = [np.asarray(x, np.float64) for x in st]
         if kind == "reference":
             w = RefWorld(sc, 32, threads=threads)
+            w.set_state(*st64)
             dt = w.step_timed(steps)
         else:
             w = OracleWorld(sc, 32)
+            w.set_state(*st64)
             t0 = time.perf_counter()
             w.step(steps)
             dt = time.perf_counter() - t0
@@ -218,7 +232,8 @@ def cpu_baseline(worlds, threads, budget_s):
         if tot_t >= budget_s:
             break
     return dict(value=body_steps / tot_t, unit="body-steps/s", cores=threads if kind == "reference" else 1,
-                kind=kind, sample=f"first {nw} of the 4096 worlds, {steps} steps each, sequential, FP32, "
+                kind=kind, sample=f"first {nw} of the 4096 worlds, {steps} steps each from their state at step "
+                                  f"{start_step}, sequential, FP32, "
                                   f"{'ThreadPool of ' + str(threads) + ' threads' if kind == 'reference' else 'single thread'}")
 
 
@@ -315,10 +330,12 @@ def main():
         c1 = scenes.box_rain_scene(10000, seed=1001)
         dt1, cnt1 = run_plain(c1, args.plain_steps)
         plain_c1 = c1.n_dynamic() * args.plain_steps / dt1
-        plain_c3 = run_plain_sample(worlds[:64], 20)
+        plain_c3 = run_plain_sample(worlds[:64], 20, settle=args.settle)
         out["plain"] = {"box_rain_10k_body_steps_per_s": plain_c1, "box_rain_10k_steps": args.plain_steps,
                         "box_rain_10k_ms_per_step": 1000 * dt1 / args.plain_steps, "box_rain_10k_final_counts": cnt1,
                         "batch_worlds_one_at_a_time_body_steps_per_s": plain_c3,
+                        "batch_worlds_one_at_a_time_sample": f"first 64 worlds, {args.settle} untimed then 20 timed "
+                                                             f"steps each",
                         "batched_over_plain_same_worlds": value / plain_c3,
                         "batched_over_plain_box_rain_10k": value / plain_c1}
         log(f"plain: c1 {plain_c1:.3e} body-steps/s ({1000 * dt1 / args.plain_steps:.2f} ms/step), "
@@ -332,7 +349,7 @@ def main():
         out["served"] = sv
         log(f"served: {sv}")
     if not args.no_cpu:
-        cb = cpu_baseline(worlds, args.cpu_threads, args.cpu_seconds)
+        cb = cpu_baseline(worlds, bt["start_states"], bt["start_step"], args.cpu_threads, args.cpu_seconds)
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb}")
     print(json.dumps(out), flush=True)
