@@ -294,7 +294,7 @@ def main():
     sc = bt["stage_counts"]
     sb = stage_bytes(sc["bodies"], sc["dynamic"], sc["pairs"], sc["points"])
     names = ["k_world_broad (integrate + sweep-and-prune, all size classes)", "k_narrow_list (narrowphase)",
-             "k_group_prep + k_group_sweep (solver setup + colouring + Gauss-Seidel, all size classes)"]
+             "k_group_solve (solver setup + colouring + Gauss-Seidel, all size classes)"]
     k = max(range(3), key=lambda i: bt["stage_ms"][i])
     achieved = sb[k] / (bt["stage_ms"][k] * 1e-3) / 1e9
     log(f"batched: {value:.3e} body-steps/s, {ms_step:.3f} ms/step wall, {dev_step_ms:.3f} ms/step device, "

@@ -6,10 +6,9 @@
 //                            sweep in (i, j) order, next axis; the world's pairs are appended to
 //                            a narrowphase work list
 //   K2 k_narrow_list         one thread per pair of every world: narrowphase + contact cache
-//   K3 k_group_prep<BLOCK>   one workgroup per group of worlds of a size class: solver rows and
-//                            colouring (LDS lists)
-//      k_group_sweep<BLOCK>  colour-ordered Gauss-Seidel sweeps on LDS copies of the group's
-//                            bodies and pairs, velocity sync
+//   K3 k_group_solve<BLOCK>  one workgroup per group of worlds of a size class: solver rows and
+//                            colouring (LDS lists), then colour-ordered Gauss-Seidel sweeps on LDS
+//                            copies of the group's bodies and pairs, velocity sync
 // Every routine is the one the single-world pipeline uses (pe_pipeline.hpp), and the sort,
 // sweep order and colouring rule are the same, so a world stepped here is bit-identical to
 // the same world stepped alone -- batching never changes a result.
@@ -393,13 +392,13 @@ __device__ __forceinline__ int world_of(const int* s_wb, int nw, int lp) {
     return lo;
 }
 
-// K3a: solver rows (FrictionContactConstraint::initSequentialImpulse, to the global row store)
-// and the colouring of a group; the solve order (local pair indices) and colour starts go to
-// the first world's order and colour-start regions for K3b.  A group that does not fit LDS is
+// First half of K3: solver rows (FrictionContactConstraint::initSequentialImpulse, to the global
+// row store) and the colouring of a group; the solve order (local pair indices) and colour starts
+// go to the first world's order and colour-start regions for the sweeps.  A group that does not fit LDS is
 // solved here completely, world by world, on the global path.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_group_prep(BatchDev d, const int32_t* __restrict__ wlist,
-                                                      const int2* __restrict__ groups, int lds_bytes) {
+__device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __restrict__ wlist,
+                                           const int2* __restrict__ groups, int lds_bytes) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int wsum[BLOCK / WAVE + 1];
     __shared__ int s_wb[MAX_GROUP_WORLDS + 1];
@@ -503,20 +502,20 @@ __device__ __forceinline__ void sweep_passes(int iterations, int ncol, const int
     }
 }
 
-// K3b: the colour-ordered Gauss-Seidel sweeps of a prepared group on LDS copies of its bodies
+// Second half of K3: the colour-ordered Gauss-Seidel sweeps of a prepared group on LDS copies of its bodies
 // and pairs (rows in the global row store: staging them in LDS measured slower, as it lowers the
-// workgroups a CU holds), then syncTempVelocity.  Kept small so more waves fit a SIMD: its passes
-// are chains of dependent arithmetic separated by barriers.
+// workgroups a CU holds), then syncTempVelocity.  Its passes are chains of dependent arithmetic
+// separated by barriers.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_group_sweep(BatchDev d, const int32_t* __restrict__ wlist,
-                                                       const int2* __restrict__ groups, int prep_lds_bytes) {
+__device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __restrict__ wlist,
+                                            const int2* __restrict__ groups, int prep_lds_bytes) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int wsum[BLOCK / WAVE + 1];
     __shared__ int s_wb[MAX_GROUP_WORLDS + 1];
     const int tid = threadIdx.x;
     const BodyView& b = d.b;
     const Group G = load_group(d, wlist, groups[blockIdx.x], s_wb);
-    if (!group_on_lds(PrepLayout(G.GB, G.GP), G.GB, prep_lds_bytes)) return;   // solved by K3a
+    if (!group_on_lds(PrepLayout(G.GB, G.GP), G.GB, prep_lds_bytes)) return;   // solved by group_prep
     const int GB = G.GB, GP = G.GP, nw = G.nw, B0 = G.B0;
     const int w = G.wl[0];
     auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
@@ -565,6 +564,16 @@ __global__ void __launch_bounds__(BLOCK) k_group_sweep(BatchDev d, const int32_t
     }
 }
 
+// K3: a group's preparation (solver rows, colouring) and its Gauss-Seidel sweeps in one launch, so
+// the sweeps of a group start as soon as its own preparation is done rather than after every group's.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_group_solve(BatchDev d, const int32_t* __restrict__ wlist,
+                                                       const int2* __restrict__ groups, int lds_bytes) {
+    group_prep<BLOCK>(d, wlist, groups, lds_bytes);
+    __syncthreads();
+    group_sweep<BLOCK>(d, wlist, groups, lds_bytes);
+}
+
 __global__ void k_refresh_range(BodyView b, int first, int count) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) refresh_body(b, first + i);
 }
@@ -576,8 +585,7 @@ int body_pair_cap(int n) { return 8 * n + 64; }
 template <int B> void set_lds_one() {
     const int bytes = 150 * 1024;
     hipFuncSetAttribute((const void*)k_world_broad<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_group_prep<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_group_sweep<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_group_solve<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 void set_lds_limits() {
     static bool done = false;
@@ -596,8 +604,7 @@ void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int 
         hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
     } else {
         const size_t lb = solve_lds_bytes(nmax, group_bodies);
-        hipLaunchKernelGGL(k_group_prep<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
-        hipLaunchKernelGGL(k_group_sweep<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
+        hipLaunchKernelGGL(k_group_solve<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
     }
 }
 hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups,
