@@ -51,7 +51,8 @@ def run_batched(args, worlds):
     b = Batch(worlds)
     log(f"batch created: {len(worlds)} worlds, {b.n_bodies} bodies in {time.perf_counter() - t0:.1f}s")
     # the worlds settle for args.settle steps before tuning, so the launch configuration is
-    # chosen for the contact load the timed steps see; tuning steps are real steps
+    # chosen for the contact load the timed steps see; the tuner times every candidate from a
+    # snapshot of that state and restores it, so the timed steps follow the settled state
     b.step(args.settle)
     tuned = b.autotune(steps=2) if not args.no_tune else {}
     log(f"autotuned (block, solver group bodies) per class: {tuned}; config {b.config()}")

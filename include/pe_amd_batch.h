@@ -67,6 +67,15 @@ int pe_batch_stage_times(pe_batch* b, int32_t nsteps, double* ms /*3*/);
  * longest first, onto the least-loaded of the batch's streams.  Drops a captured graph. */
 int pe_batch_class_times(pe_batch* b, int32_t nsteps, double* ms, int32_t cap);
 
+/* No counterpart in the reference (it has no tuner): pe_batch_snapshot copies the whole device
+ * state of every world (bodies, contact caches, solver rows, per-world counters) aside, on the
+ * device; pe_batch_restore puts it back, so the next step is exactly the one that followed the
+ * snapshot.  A tuner times its candidates from one snapshot and restores it afterwards, so tuning
+ * neither advances the worlds nor times candidates on different states.  The first snapshot
+ * allocates a second copy of the batch's state; the launch configuration is not part of it. */
+int pe_batch_snapshot(pe_batch* b);
+int pe_batch_restore(pe_batch* b);
+
 /* ------------------------------------------------------------------ server
  * Continuous batching of world-simulation requests.  A request (a scene and a horizon in
  * steps) is queued by pe_server_submit.  Each pe_server_tick admits queued requests in

@@ -94,14 +94,36 @@ __host__ inline size_t broad_lds_bytes(int nmax) {
     const int P = np2(std::max(nmax, 2));
     return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
-// K3: LDS of a group of G bodies (G = max(group bodies, largest world)): claims and body state
-// (68 B per body), and per pair (estimated at 2 per body) its global slot, local bodies, claim
-// priority, row start, order slot and three colouring-list slots (32 B); capped so a workgroup
-// fits a CU.
+// LDS of K3 for a group of GB bodies and GP pairs.  The preparation uses per body its claim and
+// inverse mass (8 B) and per pair its local bodies, global slot, claim priority, order slot and
+// three colouring-list slots (32 B); the sweeps then reuse the same memory for the body state
+// (64 B per body) and per pair its local bodies, global slot and first row (12 B) plus the solve
+// order and colour starts.  end covers both; a group is solved on the global path when it does
+// not fit the launch's LDS (both halves evaluate this the same way).
+struct PrepLayout {
+    size_t bs, pab, gi, pr, ord, buf, cs, end;
+    __host__ __device__ PrepLayout(int GB, int GP) {
+        auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+        bs = al16((size_t)GB * 4);
+        pab = al16(bs + (size_t)GB * 4);
+        gi = al16(pab + (size_t)GP * 4);
+        pr = al16(gi + (size_t)GP * 4);
+        ord = al16(pr + (size_t)GP * 4);
+        buf = al16(ord + (size_t)GP * 4);
+        cs = al16(buf + (size_t)GP * 12);
+        const size_t prep_end = cs + (size_t)(GP + 2) * 4;
+        const size_t sweep_end =
+            al16((size_t)GB * 64 + (size_t)(3 * GP + 1) * 4) + (size_t)GP * 4 + (size_t)(GP + 1) * 4;
+        end = prep_end > sweep_end ? prep_end : sweep_end;
+    }
+};
+// K3's launch LDS for a size class: the layout of a group of G = max(group bodies, largest world)
+// bodies and 2 pairs per body (a group with more pairs takes the global path), capped so a
+// workgroup fits a CU.
 constexpr size_t SOLVE_LDS_CAP = 144 * 1024;
 __host__ inline size_t solve_lds_bytes(int nmax, int group_bodies) {
-    const size_t g = (size_t)std::max(std::max(nmax, group_bodies), 1);
-    return std::min(SOLVE_LDS_CAP, 68 * g + 32 * 2 * g + 144);
+    const int g = std::max(std::max(nmax, group_bodies), 1);
+    return std::min(SOLVE_LDS_CAP, PrepLayout(g, 2 * g).end);
 }
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
@@ -349,23 +371,6 @@ struct Group {
     const int32_t* wl;
     int nw, B0, GB, GP;
 };
-// LDS layout of the preparation kernel; the group is solved on the global path when it does
-// not fit lds_bytes (both kernels evaluate this the same way).
-struct PrepLayout {
-    size_t bs, pab, gi, pr, rs, ord, buf, cs, end;
-    __device__ PrepLayout(int GB, int GP) {
-        auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
-        bs = al16((size_t)GB * 4);
-        pab = bs + (size_t)GB * 64;
-        gi = al16(pab + (size_t)GP * 4);
-        pr = al16(gi + (size_t)GP * 4);
-        rs = al16(pr + (size_t)GP * 4);
-        ord = al16(rs + (size_t)(GP + 1) * 4);
-        buf = al16(ord + (size_t)GP * 4);
-        cs = al16(buf + (size_t)GP * 12);
-        end = cs + (size_t)(GP + 2) * 4;
-    }
-};
 __device__ __forceinline__ bool group_on_lds(const PrepLayout& L, int GB, int lds_bytes) {
     return L.end <= (size_t)lds_bytes && GB <= 65535;
 }
@@ -566,9 +571,14 @@ __device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __
 
 // K3: a group's preparation (solver rows, colouring) and its Gauss-Seidel sweeps in one launch, so
 // the sweeps of a group start as soon as its own preparation is done rather than after every group's.
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_group_solve(BatchDev d, const int32_t* __restrict__ wlist,
-                                                       const int2* __restrict__ groups, int lds_bytes) {
+// WAVES is the minimum number of waves per SIMD the register allocation must allow: 4 (at most
+// 128 VGPRs) for classes whose groups are solved in LDS -- the sweeps are chains of dependent
+// arithmetic, so the more workgroups a CU holds the better; it spills only on the global path,
+// which such classes take only for a group with more than 2 pairs per body.  Classes of worlds too
+// large for LDS run the global path and use the unconstrained build (WAVES = 1).
+template <int BLOCK, int WAVES>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_group_solve(BatchDev d, const int32_t* __restrict__ wlist, const int2* __restrict__ groups, int lds_bytes) {
     group_prep<BLOCK>(d, wlist, groups, lds_bytes);
     __syncthreads();
     group_sweep<BLOCK>(d, wlist, groups, lds_bytes);
@@ -585,7 +595,8 @@ int body_pair_cap(int n) { return 8 * n + 64; }
 template <int B> void set_lds_one() {
     const int bytes = 150 * 1024;
     hipFuncSetAttribute((const void*)k_world_broad<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    hipFuncSetAttribute((const void*)k_group_solve<B>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_group_solve<B, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    hipFuncSetAttribute((const void*)k_group_solve<B, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 void set_lds_limits() {
     static bool done = false;
@@ -604,7 +615,10 @@ void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int 
         hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
     } else {
         const size_t lb = solve_lds_bytes(nmax, group_bodies);
-        hipLaunchKernelGGL(k_group_solve<B>, dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
+        if (lb < SOLVE_LDS_CAP)
+            hipLaunchKernelGGL((k_group_solve<B, 4>), dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
+        else
+            hipLaunchKernelGGL((k_group_solve<B, 1>), dim3(ngroups), dim3(B), lb, s, d, wl, groups, (int)lb);
     }
 }
 hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups,
@@ -793,6 +807,8 @@ struct pe_batch {
     BatchDev d{};
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;
+    std::vector<void*> snap;       // copies of the state allocations (pe_batch_snapshot)
     std::vector<Class> classes;
     int32_t* dlist = nullptr;      // world ids grouped by class
     int2* dgroups = nullptr;       // K3 groups per class
@@ -809,6 +825,7 @@ struct pe_batch {
         if (gexec) hipGraphExecDestroy(gexec);
         if (graph) hipGraphDestroy(graph);
         for (void* p : allocs) hipFree(p);
+        for (void* p : snap) if (p) hipFree(p);
         meshes.release();
         if (stream) hipStreamDestroy(stream);
     }
@@ -817,7 +834,27 @@ struct pe_batch {
         PE_HIP(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
         PE_HIP(hipMemsetAsync(q, 0, std::max<size_t>(count, 1) * sizeof(T), stream));
         allocs.push_back(q);
+        alloc_bytes.push_back(std::max<size_t>(count, 1) * sizeof(T));
         *p = (T*)q;
+        return PE_OK;
+    }
+    // Copies every state allocation to (save) or from (restore) its snapshot copy; the class
+    // lists are configuration, not state, and are left alone.
+    int copy_state(bool save) {
+        if (save && snap.empty()) {
+            snap.assign(allocs.size(), nullptr);
+            for (size_t k = 0; k < allocs.size(); k++) {
+                if (allocs[k] == (void*)dlist || allocs[k] == (void*)dgroups) continue;
+                PE_HIP(hipMalloc(&snap[k], alloc_bytes[k]));
+            }
+        }
+        if (snap.size() != allocs.size()) { pe_set_error("no snapshot to restore"); return PE_ERR_INVALID; }
+        for (size_t k = 0; k < allocs.size(); k++) {
+            if (!snap[k]) continue;
+            PE_HIP(hipMemcpyAsync(save ? snap[k] : allocs[k], save ? allocs[k] : snap[k], alloc_bytes[k],
+                                  hipMemcpyDeviceToDevice, stream));
+        }
+        PE_HIP(hipStreamSynchronize(stream));
         return PE_OK;
     }
     // Groups worlds into classes (largest worlds first inside a class) and uploads the list.
@@ -1154,6 +1191,16 @@ extern "C" int pe_batch_class_times(pe_batch* B, int32_t nsteps, double* ms, int
     schedule_classes(B->classes, B->side.n + 1);
     B->drop_graph();
     return PE_OK;
+}
+
+// ====================================================================== snapshot
+extern "C" int pe_batch_snapshot(pe_batch* B) {
+    if (!B) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    return B->copy_state(true);
+}
+extern "C" int pe_batch_restore(pe_batch* B) {
+    if (!B) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    return B->copy_state(false);
 }
 
 // ====================================================================== timed batch steps

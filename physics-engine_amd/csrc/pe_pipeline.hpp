@@ -504,26 +504,29 @@ __device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, floa
     m33 Ia, Ib;
 #pragma unroll
     for (int k = 0; k < 9; k++) { Ia.m[k] = bs[(7 + k) * n + a]; Ib.m[k] = bs[(7 + k) * n + c]; }
-    for (int k = 0; k < nr; k++) {
-        float nxt[RW];
-        const bool more = k + 1 < nr;
-        if (more) ld_rec<RW / 4>(rows + (size_t)(k + 1) * RW, nxt);
+    // One row of the chain from buffer o; row k's accumulated impulses go back to the row store.
+    auto row = [&](const float (&o)[RW], int k) {
         Row r;
-        r.ra = f3{cur[R_RA], cur[R_RA + 1], cur[R_RA + 2]};
-        r.rb = f3{cur[R_RB], cur[R_RB + 1], cur[R_RB + 2]};
-        r.n = f3{cur[R_N], cur[R_N + 1], cur[R_N + 2]};
-        r.t0 = f3{cur[R_T0], cur[R_T0 + 1], cur[R_T0 + 2]};
-        r.t1 = f3{cur[R_T1], cur[R_T1 + 1], cur[R_T1 + 2]};
-        r.rhs = cur[R_RHS]; r.mn = cur[R_MN]; r.mt0 = cur[R_MT0]; r.mt1 = cur[R_MT1]; r.mu = cur[R_MU];
-        float ln = cur[R_LN], lt0 = cur[R_LT0], lt1 = cur[R_LT1];
+        r.ra = f3{o[R_RA], o[R_RA + 1], o[R_RA + 2]};
+        r.rb = f3{o[R_RB], o[R_RB + 1], o[R_RB + 2]};
+        r.n = f3{o[R_N], o[R_N + 1], o[R_N + 2]};
+        r.t0 = f3{o[R_T0], o[R_T0 + 1], o[R_T0 + 2]};
+        r.t1 = f3{o[R_T1], o[R_T1 + 1], o[R_T1 + 2]};
+        r.rhs = o[R_RHS]; r.mn = o[R_MN]; r.mt0 = o[R_MT0]; r.mt1 = o[R_MT1]; r.mu = o[R_MU];
+        float ln = o[R_LN], lt0 = o[R_LT0], lt1 = o[R_LT1];
         const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
         if (ima >= 0.0f) apply_temp(tva, twa, ima, Ia, r.ra, J);
         if (imb >= 0.0f) apply_temp(tvb, twb, imb, Ib, r.rb, neg(J));
         reinterpret_cast<float4*>(rows + (size_t)k * RW)[5] = make_float4(ln, lt0, lt1, 0.0f);
-        if (more) {
-#pragma unroll
-            for (int u = 0; u < RW; u++) cur[u] = nxt[u];
-        }
+    };
+    // Rows alternate between cur and nxt, each loaded while the other is solved (no copies).
+    float nxt[RW];
+    for (int k = 0; k < nr; k += 2) {
+        if (k + 1 < nr) ld_rec<RW / 4>(rows + (size_t)(k + 1) * RW, nxt);
+        row(cur, k);
+        if (k + 1 >= nr) break;
+        if (k + 2 < nr) ld_rec<RW / 4>(rows + (size_t)(k + 2) * RW, cur);
+        row(nxt, k + 1);
     }
     if (ima >= 0.0f) {
         float* q = L.bs + a;

@@ -115,6 +115,59 @@ class Batch:
         self.sync()
         return time.perf_counter() - t0
 
+    def _whole_step_ms(self, steps: int) -> float:
+        """Device ms per step of the current configuration as a captured graph replays it, from
+        the snapshot, best of three runs of steps steps, after dealing the classes onto streams
+        by their cost.  (Eager steps rank configurations differently: their launch gaps differ.)"""
+        self.restore()
+        self.class_times(2)
+        self.capture_graph()
+        best = float("inf")
+        for _ in range(3):
+            self.restore()
+            best = min(best, self.step_timed(steps) / steps)
+        return best
+
+    def snapshot(self):
+        """Copies the device state of every world aside (pe_batch_snapshot)."""
+        check(lib().pe_batch_snapshot(self.h))
+        self._snap_steps = self.steps_done
+
+    def restore(self):
+        """Puts the last snapshot back (pe_batch_restore): the next step is the one that followed it."""
+        check(lib().pe_batch_restore(self.h))
+        self.steps_done = self._snap_steps
+
+    def _refine(self, uppers, blocks, rows, candidates, group_candidates, steps, passes=2):
+        def apply(bl, rw):
+            self.set_config(uppers, bl)
+            self.set_solver_groups(rw)
+        apply(blocks, rows)
+        self.restore()
+        cost = self.class_times(steps)
+        counts = [c[2] for c in self.config()]
+        steps = max(steps, 5)
+        best_t = self._whole_step_ms(steps)
+        for _ in range(passes):
+            changed = False
+            for c in sorted(range(len(uppers)), key=lambda k: -(cost[k][0] + cost[k][1])):
+                if counts[c] == 0:
+                    continue
+                for cb in candidates:
+                    for cg in group_candidates:
+                        if (cb, cg) == (blocks[c], rows[c]):
+                            continue
+                        tb, tr = list(blocks), list(rows)
+                        tb[c], tr[c] = cb, cg
+                        apply(tb, tr)
+                        t = self._whole_step_ms(steps)
+                        if t < best_t * 0.995:
+                            best_t, blocks, rows, changed = t, tb, tr, True
+                apply(blocks, rows)
+            if not changed:
+                break
+        return blocks, rows
+
     def class_times(self, n: int):
         """n eager steps with every size class launched alone, in turn: per class the summed ms of
         (integrate + broadphase, solver).  Also deals the classes onto streams by these costs."""
@@ -125,25 +178,39 @@ class Batch:
         return [(ms[2 * c], ms[2 * c + 1]) for c in range(nc)]
 
     def autotune(self, candidates=(64, 128, 256, 512), steps: int = 3, classes=None,
-                 group_candidates=(0, 128, 256, 512, 1024)):
-        """Times every candidate block size on all size classes at once, each class launched
-        alone and timed by its own events so the classes do not disturb each other's figures,
-        and keeps each class's fastest (integrate + broadphase + solver); then the same for the
-        solver group size (bodies per solve workgroup; solver time).  Finally times the chosen
-        configuration, which deals the classes onto streams by cost.  Returns {class upper
-        bound: (block, group bodies)}.  Stepping during tuning advances the worlds (it is real
-        simulation work); results never depend on the choice."""
+                 group_candidates=(0, 128, 256, 512, 1024), refine: bool = True):
+        """Picks each size class's block size and solver group size (bodies per solve workgroup).
+        Every timing starts from one snapshot of the worlds (snapshot / restore), so all
+        candidates are timed on the same state, and the worlds are left as they were: tuning
+        does not advance them.  Results never depend on the choice.
+
+        First every candidate block size is timed on all size classes at once, each class
+        launched alone and timed by its own events so the classes do not disturb each other's
+        figures, and each class keeps its fastest (integrate + broadphase + solver); then the same
+        for the group size (solver time).  With refine, the choice is revised on the whole step,
+        where the classes run concurrently and compete for the CUs: class by class, costliest
+        first, every (block, group) pair is timed in whole graph-replayed steps and kept when it
+        shortens them (at most two passes over the classes).  Finally the chosen configuration is
+        timed once more, which deals the classes onto streams by cost.  Returns {class upper
+        bound: (block, group bodies)}."""
         cur = self.config()
         uppers = [c[0] for c in cur] if classes is None else list(classes)
         nc = len(uppers)
+        self.snapshot()
         self.set_config(uppers, [256] * nc)
         rows = self.solver_groups()
+
+        def class_times():
+            self.restore()
+            self.class_times(1)
+            self.restore()
+            return self.class_times(steps)
+
         best = [None] * nc
         for cand in candidates:
             self.set_config(uppers, [cand] * nc)
             self.set_solver_groups(rows)
-            self.class_times(1)
-            t = self.class_times(steps)
+            t = class_times()
             for c in range(nc):
                 tot = t[c][0] + t[c][1]
                 if best[c] is None or tot < best[c][1]:
@@ -153,11 +220,14 @@ class Batch:
         best = [None] * nc
         for cand in group_candidates:
             self.set_solver_groups([cand] * nc)
-            t = self.class_times(steps)
+            t = class_times()
             for c in range(nc):
                 if best[c] is None or t[c][1] < best[c][1]:
                     best[c] = (cand, t[c][1])
         rows = [b[0] for b in best]
         self.set_solver_groups(rows)
-        self.class_times(steps)
+        if refine:
+            blocks, rows = self._refine(uppers, blocks, rows, candidates, group_candidates, steps)
+        class_times()
+        self.restore()
         return {u: (b, r) for u, b, r in zip(uppers, blocks, rows)}

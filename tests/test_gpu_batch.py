@@ -66,6 +66,38 @@ def test_batch_graph_and_config_do_not_change_results(worlds):
             assert np.array_equal(flat_state(a.world_state(i)), flat_state(c.world_state(i))), f"world {i} rows {rows}"
 
 
+def test_batch_snapshot_restore_and_autotune(worlds):
+    """A restored snapshot steps exactly like the state it saved, and the tuner (which times
+    every candidate from a snapshot) leaves the worlds where it found them."""
+    from pe_amd.batch import Batch
+    a = Batch(worlds)
+    a.step(20)
+    a.sync()
+    b = Batch(worlds)
+    b.step(12)
+    b.snapshot()
+    b.step(8)
+    b.sync()
+    for i in range(len(worlds)):
+        assert np.array_equal(flat_state(a.world_state(i)), flat_state(b.world_state(i))), f"world {i}"
+    b.step(5)
+    b.restore()
+    assert b.steps_done == 12
+    b.step(8)
+    b.sync()
+    for i in range(len(worlds)):
+        assert np.array_equal(flat_state(a.world_state(i)), flat_state(b.world_state(i))), f"restored world {i}"
+    c = Batch(worlds)
+    c.step(12)
+    tuned = c.autotune(candidates=(64, 256), steps=1, group_candidates=(0, 256))
+    print(f"[snapshot] tuned {tuned}", flush=True)
+    assert c.steps_done == 12
+    c.step(8)
+    c.sync()
+    for i in range(len(worlds)):
+        assert np.array_equal(flat_state(a.world_state(i)), flat_state(c.world_state(i))), f"tuned world {i}"
+
+
 def test_batch_subset_alone_vs_in_mix(worlds):
     """A world's result does not depend on its neighbours in the batch."""
     from pe_amd.batch import Batch
