@@ -95,13 +95,13 @@ __host__ inline size_t broad_lds_bytes(int nmax) {
     return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
 }
 // LDS of K3 for a group of GB bodies and GP pairs.  The preparation uses per body its claim and
-// inverse mass (8 B) and per pair its local bodies, global slot, claim priority, order slot and
-// three colouring-list slots (32 B); the sweeps then reuse the same memory for the body state
+// inverse mass (8 B) and per pair its local bodies, global slot, claim priority, order slot,
+// three colouring-list slots and first row (36 B); the sweeps then reuse the same memory for the body state
 // (64 B per body) and per pair its local bodies, global slot and first row (12 B) plus the solve
 // order and colour starts.  end covers both; a group is solved on the global path when it does
 // not fit the launch's LDS (both halves evaluate this the same way).
 struct PrepLayout {
-    size_t bs, pab, gi, pr, ord, buf, cs, end;
+    size_t bs, pab, gi, pr, ord, buf, cs, rs, end;
     __host__ __device__ PrepLayout(int GB, int GP) {
         auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
         bs = al16((size_t)GB * 4);
@@ -111,7 +111,8 @@ struct PrepLayout {
         ord = al16(pr + (size_t)GP * 4);
         buf = al16(ord + (size_t)GP * 4);
         cs = al16(buf + (size_t)GP * 12);
-        const size_t prep_end = cs + (size_t)(GP + 2) * 4;
+        rs = al16(cs + (size_t)(GP + 2) * 4);
+        const size_t prep_end = rs + (size_t)(GP + 1) * 4;
         const size_t sweep_end =
             al16((size_t)GB * 64 + (size_t)(3 * GP + 1) * 4) + (size_t)GP * 4 + (size_t)(GP + 1) * 4;
         end = prep_end > sweep_end ? prep_end : sweep_end;
@@ -428,20 +429,21 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
     int* order = (int*)(lds + Lo.ord);
     int* bufs[3] = {(int*)(lds + Lo.buf), (int*)(lds + Lo.buf) + GP, (int*)(lds + Lo.buf) + 2 * GP};
     int* cs = (int*)(lds + Lo.cs);
+    int* rs = (int*)(lds + Lo.rs);
     for (int i = tid; i < GB; i += BLOCK) {
         claim[i] = 0x7fffffff;
         imk[i] = b.iat(I_KIN, B0 + i) ? -1.0f : 0.0f;
     }
     __syncthreads();
-    int nact = 0;
+    int nact = 0, nrows = 0;
     for (int base = 0; base < GP; base += BLOCK) {
         const int lp = base + tid;
-        int flag = 0;
+        int flag = 0, np_ = 0;
         if (lp < GP) {
             const int k = world_of(s_wb, nw, lp);
             const int gs = d.desc[G.wl[k]].pair_off + (lp - s_wb[k]);
             const int pa = d.ps.pa[gs], pb = d.ps.pb[gs];
-            const int np_ = d.ps.npts[gs];
+            np_ = d.ps.npts[gs];
             gidx[lp] = gs;
             pri[lp] = colour::prio(lp - s_wb[k]);
             pab[lp] = (pa - B0) | ((pb - B0) << 16);
@@ -454,10 +456,24 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
         const int ex = block_exclusive_scan(flag, wsum, &tot);
         if (flag) bufs[0][nact + ex] = lp;
         nact += tot;
+        const int exr = block_exclusive_scan(np_, wsum, &tot);
+        if (lp < GP) rs[lp] = nrows + exr;
+        nrows += tot;
     }
+    if (tid == 0) rs[GP] = nrows;
     __syncthreads();
     // ---- stage 3: solver (sequential_impulse_solver.cpp:11-81)
-    for (int lp = tid; lp < GP; lp += BLOCK) setup_pair(b, d.ps, gidx[lp], d.dt);
+    // Solver rows, one thread per contact point of the group (a pair's rows are independent):
+    // row r belongs to the last local pair whose first row is at most r.
+    for (int r = tid; r < nrows; r += BLOCK) {
+        int lo = 0, hi = GP - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rs[mid] <= r) lo = mid;
+            else hi = mid - 1;
+        }
+        setup_row(b, d.ps, gidx[lo], r - rs[lo], d.dt);
+    }
     PE_PHASE(5);
     const int ncol = colour_group(pab, imk, pri, bufs, nact, order, cs, claim);
     PE_PHASE(6);

@@ -369,40 +369,51 @@ __device__ __forceinline__ SolvBody load_solv(const BodyView& b, int i) {
 }
 
 // FrictionContactConstraint::initSequentialImpulse (friction_contact_constraint.cpp:9-73) for
-// every point of pair p; row k goes to out + k * RW (the pair's rows in the row store, or a
-// workgroup's LDS copy).
-__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt, float* out) {
+// point k of pair p, given the pair's bodies and pair coefficients; the row goes to out.
+// The pair-level quantities: getKinematicInvMass (rigidbody.cpp:48-62, zero when kinematic)
+// summed, and ContactResult::setObjects's combined coefficients (contact_result.cpp:80-86).
+struct PairCoef {
+    float ims, mu, e;
+};
+__device__ __forceinline__ PairCoef pair_coef(const CollBody& CA, const CollBody& CB, const SolvBody& A, const SolvBody& B) {
+    return PairCoef{A.im + B.im, sqrtf(CA.mu * CB.mu), sqrtf(CA.e * CB.e)};
+}
+__device__ __forceinline__ void setup_point(const PairStore& ps, int p, int k, const CollBody& CA, const CollBody& CB,
+                                            const SolvBody& A, const SolvBody& B, const PairCoef& q, float dt, float* out) {
+    const size_t s = (size_t)p * 8 + k;
+    float c[CW];
+    ld_rec<CW / 4>(ps.cp + s * CW, c);
+    const f3 nn{c[C_N], c[C_N + 1], c[C_N + 2]};
+    const f3 la{c[C_LA], c[C_LA + 1], c[C_LA + 2]};
+    const f3 lb{c[C_LB], c[C_LB + 1], c[C_LB + 2]};
+    const Row r = make_row(CA.P.R, CB.P.R, la, lb, nn, ps.cd[s], q.ims, A.Iw, B.Iw, A.v, A.w, B.v, B.w, q.mu, q.e, dt);
+    float o[RW];
+    o[R_RA] = r.ra.x; o[R_RA + 1] = r.ra.y; o[R_RA + 2] = r.ra.z;
+    o[R_RB] = r.rb.x; o[R_RB + 1] = r.rb.y; o[R_RB + 2] = r.rb.z;
+    o[R_N] = r.n.x; o[R_N + 1] = r.n.y; o[R_N + 2] = r.n.z;
+    o[R_T0] = r.t0.x; o[R_T0 + 1] = r.t0.y; o[R_T0 + 2] = r.t0.z;
+    o[R_T1] = r.t1.x; o[R_T1 + 1] = r.t1.y; o[R_T1 + 2] = r.t1.z;
+    o[R_RHS] = r.rhs; o[R_MN] = r.mn; o[R_MT0] = r.mt0; o[R_MT1] = r.mt1; o[R_MU] = r.mu;
+    o[R_LN] = 0.0f; o[R_LT0] = 0.0f; o[R_LT1] = 0.0f; o[23] = 0.0f;
+    st_rec<RW / 4>(out, o);
+}
+// Every point of pair p, to the row store (row k of the pair at slot p * 8 + k).
+__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt) {
     const int n = ps.npts[p];
     if (n == 0) return;
     const int ia = ps.pa[p], ib = ps.pb[p];
     const CollBody CA = load_coll(b, ia), CB = load_coll(b, ib);
     const SolvBody A = load_solv(b, ia), B = load_solv(b, ib);
-    // getKinematicInvMass / getKinematicWorldInvInertia (rigidbody.cpp:48-62): zero when kinematic
-    const float ims = A.im + B.im;
-    // ContactResult::setObjects (contact_result.cpp:80-86)
-    const float mu = sqrtf(CA.mu * CB.mu);
-    const float e = sqrtf(CA.e * CB.e);
-    for (int k = 0; k < n; k++) {
-        const size_t s = (size_t)p * 8 + k;
-        float c[CW];
-        ld_rec<CW / 4>(ps.cp + s * CW, c);
-        const f3 nn{c[C_N], c[C_N + 1], c[C_N + 2]};
-        const f3 la{c[C_LA], c[C_LA + 1], c[C_LA + 2]};
-        const f3 lb{c[C_LB], c[C_LB + 1], c[C_LB + 2]};
-        const Row r = make_row(CA.P.R, CB.P.R, la, lb, nn, ps.cd[s], ims, A.Iw, B.Iw, A.v, A.w, B.v, B.w, mu, e, dt);
-        float o[RW];
-        o[R_RA] = r.ra.x; o[R_RA + 1] = r.ra.y; o[R_RA + 2] = r.ra.z;
-        o[R_RB] = r.rb.x; o[R_RB + 1] = r.rb.y; o[R_RB + 2] = r.rb.z;
-        o[R_N] = r.n.x; o[R_N + 1] = r.n.y; o[R_N + 2] = r.n.z;
-        o[R_T0] = r.t0.x; o[R_T0 + 1] = r.t0.y; o[R_T0 + 2] = r.t0.z;
-        o[R_T1] = r.t1.x; o[R_T1 + 1] = r.t1.y; o[R_T1 + 2] = r.t1.z;
-        o[R_RHS] = r.rhs; o[R_MN] = r.mn; o[R_MT0] = r.mt0; o[R_MT1] = r.mt1; o[R_MU] = r.mu;
-        o[R_LN] = 0.0f; o[R_LT0] = 0.0f; o[R_LT1] = 0.0f; o[23] = 0.0f;
-        st_rec<RW / 4>(out + (size_t)k * RW, o);
-    }
+    const PairCoef q = pair_coef(CA, CB, A, B);
+    for (int k = 0; k < n; k++) setup_point(ps, p, k, CA, CB, A, B, q, dt, ps.rows + ((size_t)p * 8 + k) * RW);
 }
-__device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& ps, int p, float dt) {
-    setup_pair(b, ps, p, dt, ps.rows + (size_t)p * 8 * RW);
+// Point k of pair p alone, with the same arithmetic as setup_pair; the rows of a pair can so be
+// set up by different threads.
+__device__ __forceinline__ void setup_row(const BodyView& b, const PairStore& ps, int p, int k, float dt) {
+    const int ia = ps.pa[p], ib = ps.pb[p];
+    const CollBody CA = load_coll(b, ia), CB = load_coll(b, ib);
+    const SolvBody A = load_solv(b, ia), B = load_solv(b, ib);
+    setup_point(ps, p, k, CA, CB, A, B, pair_coef(CA, CB, A, B), dt, ps.rows + ((size_t)p * 8 + k) * RW);
 }
 
 // RigidBody::applyTempImpulse (rigidbody.cpp:166-170), with the body's inverse mass and world
