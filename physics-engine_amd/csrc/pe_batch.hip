@@ -541,7 +541,7 @@ __device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __
     const int w = G.wl[0];
     auto al16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
     LdsSolve L;
-    L.bs = (float*)lds;
+    L.bs = (float4*)lds;
     L.pab = (int*)(lds + (size_t)GB * 64);
     int* gidx = L.pab + GP;
     L.rs = gidx + GP;
@@ -579,9 +579,10 @@ __device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __
     });
     PE_PHASE(7);
     for (int i = tid; i < GB; i += BLOCK) {
-        if (L.bs[6 * GB + i] < 0.0f) continue;
-        st3(b, F_V, B0 + i, f3{L.bs[i], L.bs[GB + i], L.bs[2 * GB + i]});
-        st3(b, F_W, B0 + i, f3{L.bs[3 * GB + i], L.bs[4 * GB + i], L.bs[5 * GB + i]});
+        const float4 q0 = L.bs[4 * i], q1 = L.bs[4 * i + 1];
+        if (q1.z < 0.0f) continue;
+        st3(b, F_V, B0 + i, f3{q0.x, q0.y, q0.z});
+        st3(b, F_W, B0 + i, f3{q0.w, q1.x, q1.y});
     }
 }
 

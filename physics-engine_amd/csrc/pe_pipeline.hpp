@@ -473,11 +473,12 @@ __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore&
     }
 }
 
-// Solver state held in LDS for the Gauss-Seidel sweeps: per body (local index) the temp
-// velocities, the inverse mass (negative for a kinematic body) and the world inverse inertia,
-// structure of arrays; per pair its local bodies and first row (row counts are differences).
+// Solver state held in LDS for the Gauss-Seidel sweeps: per body (local index) one 64-byte record
+// of the temp velocities, the inverse mass (negative for a kinematic body) and the world inverse
+// inertia, read with four 128-bit loads; per pair its local bodies and first row (row counts are
+// differences).
 struct LdsSolve {
-    float* bs;          // [16][n]: tv xyz, tw xyz, im (or -1: kinematic), Iw 9
+    float4* bs;         // [n][4]: (tv xyz, tw x), (tw yz, im or -1: kinematic, Iw0), (Iw1..4), (Iw5..8)
     int* pab;           // [npairs] local A | local B << 16
     int* rs;            // [npairs + 1] first row of each pair
     int n;
@@ -486,12 +487,11 @@ __device__ __forceinline__ void lds_load_body(const LdsSolve& L, const BodyView&
     const float* s = b.sbr(gi);
     float r[SBW];
     ld_rec<SBW / 4>(s, r);
-    const int n = L.n;
-#pragma unroll
-    for (int k = 0; k < 6; k++) L.bs[k * n + li] = r[SB_TV + k];
-    L.bs[6 * n + li] = r[SB_KIN] != 0.0f ? -1.0f : r[SB_IM];
-#pragma unroll
-    for (int k = 0; k < 9; k++) L.bs[(7 + k) * n + li] = r[SB_IW + k];
+    float4* q = L.bs + 4 * li;
+    q[0] = make_float4(r[SB_TV], r[SB_TV + 1], r[SB_TV + 2], r[SB_TW]);
+    q[1] = make_float4(r[SB_TW + 1], r[SB_TW + 2], r[SB_KIN] != 0.0f ? -1.0f : r[SB_IM], r[SB_IW]);
+    q[2] = make_float4(r[SB_IW + 1], r[SB_IW + 2], r[SB_IW + 3], r[SB_IW + 4]);
+    q[3] = make_float4(r[SB_IW + 5], r[SB_IW + 6], r[SB_IW + 7], r[SB_IW + 8]);
 }
 // iterate_pair on the LDS copy of the bodies (same arithmetic, same order); the pair's rows are
 // at rows (the row store), RW floats apart.
@@ -507,14 +507,13 @@ __device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, floa
     const int ab = L.pab[pl];
     const int a = ab & 0xffff, c = ab >> 16;
     const int nr = L.rs[pl + 1] - L.rs[pl];
-    const int n = L.n;
-    const float* bs = L.bs;
-    f3 tva{bs[a], bs[n + a], bs[2 * n + a]}, twa{bs[3 * n + a], bs[4 * n + a], bs[5 * n + a]};
-    f3 tvb{bs[c], bs[n + c], bs[2 * n + c]}, twb{bs[3 * n + c], bs[4 * n + c], bs[5 * n + c]};
-    const float ima = bs[6 * n + a], imb = bs[6 * n + c];
-    m33 Ia, Ib;
-#pragma unroll
-    for (int k = 0; k < 9; k++) { Ia.m[k] = bs[(7 + k) * n + a]; Ib.m[k] = bs[(7 + k) * n + c]; }
+    const float4 a0 = L.bs[4 * a], a1 = L.bs[4 * a + 1], a2 = L.bs[4 * a + 2], a3 = L.bs[4 * a + 3];
+    const float4 c0 = L.bs[4 * c], c1 = L.bs[4 * c + 1], c2 = L.bs[4 * c + 2], c3 = L.bs[4 * c + 3];
+    f3 tva{a0.x, a0.y, a0.z}, twa{a0.w, a1.x, a1.y};
+    f3 tvb{c0.x, c0.y, c0.z}, twb{c0.w, c1.x, c1.y};
+    const float ima = a1.z, imb = c1.z;
+    const m33 Ia{{a1.w, a2.x, a2.y, a2.z, a2.w, a3.x, a3.y, a3.z, a3.w}};
+    const m33 Ib{{c1.w, c2.x, c2.y, c2.z, c2.w, c3.x, c3.y, c3.z, c3.w}};
     // One row of the chain from buffer o; row k's accumulated impulses go back to the row store.
     auto row = [&](const float (&o)[RW], int k) {
         Row r;
@@ -540,12 +539,12 @@ __device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, floa
         row(nxt, k + 1);
     }
     if (ima >= 0.0f) {
-        float* q = L.bs + a;
-        q[0] = tva.x; q[n] = tva.y; q[2 * n] = tva.z; q[3 * n] = twa.x; q[4 * n] = twa.y; q[5 * n] = twa.z;
+        L.bs[4 * a] = make_float4(tva.x, tva.y, tva.z, twa.x);
+        L.bs[4 * a + 1] = make_float4(twa.y, twa.z, ima, a1.w);
     }
     if (imb >= 0.0f) {
-        float* q = L.bs + c;
-        q[0] = tvb.x; q[n] = tvb.y; q[2 * n] = tvb.z; q[3 * n] = twb.x; q[4 * n] = twb.y; q[5 * n] = twb.z;
+        L.bs[4 * c] = make_float4(tvb.x, tvb.y, tvb.z, twb.x);
+        L.bs[4 * c + 1] = make_float4(twb.y, twb.z, imb, c1.w);
     }
 }
 
