@@ -29,7 +29,9 @@ int pe_batch_sync(pe_batch* b);
 /* Captures one step (one launch per size class) into a hipGraph used by pe_batch_step. */
 int pe_batch_capture_graph(pe_batch* b);
 /* Size classes: class c holds worlds with upper[c-1] < n <= upper[c] and launches with
- * block[c] threads (64, 128, 256, 512 or 1024).  Drops a captured graph. */
+ * block[c] threads (64, 128, 256, 512 or 1024); at most 32 classes.  Each class's integrate +
+ * broadphase, narrowphase and solver launch in sequence on the stream it was dealt, and the
+ * classes of different streams overlap.  Drops a captured graph. */
 int pe_batch_set_config(pe_batch* b, int32_t nclasses, const int32_t* upper, const int32_t* block);
 /* Solver grouping per size class: the solve kernel takes consecutive worlds of a class (worlds
  * are laid out largest first) up to group_bodies[c] bodies into one workgroup and solves them as
@@ -55,16 +57,19 @@ int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
  * Gauss-Seidel sweeps.  Drops a captured graph. */
 int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
 
-/* Runs nsteps eagerly with a hipEvent pair around every stage; ms[k] is the summed device time
- * of integrate + broadphase (k = 0, all size classes), narrowphase (1) and solver (2, all size
- * classes), each from before its first launch to after its last (the size classes of a stage
- * run concurrently on side streams), without the gaps between stages. */
+/* Runs nsteps eagerly, staged for measurement (every class's integrate + broadphase, then every
+ * class's narrowphase, then every class's solver), with a hipEvent pair around every stage;
+ * ms[k] is the summed device time of integrate + broadphase (k = 0, all size classes),
+ * narrowphase (1, all classes) and solver (2, all size classes), each from before its first
+ * launch to after its last (the size classes of stages 0 and 2 run concurrently on side streams),
+ * without the gaps between stages. */
 int pe_batch_stage_times(pe_batch* b, int32_t nsteps, double* ms /*3*/);
 
 /* Runs nsteps eagerly with every size class launched alone, in turn; ms[2c] and ms[2c + 1] are
- * class c's summed integrate + broadphase and solver device milliseconds (cap >= 2 * classes).
- * Each class's per-step total then sets where it runs in concurrent steps: classes are dealt,
- * longest first, onto the least-loaded of the batch's streams.  Drops a captured graph. */
+ * class c's summed integrate + broadphase + narrowphase and solver device milliseconds
+ * (cap >= 2 * classes).  Each class's per-step total then sets where it runs in concurrent
+ * steps: classes are dealt, longest first, onto the least-loaded of the batch's streams.  Drops
+ * a captured graph. */
 int pe_batch_class_times(pe_batch* b, int32_t nsteps, double* ms, int32_t cap);
 
 /* No counterpart in the reference (it has no tuner): pe_batch_snapshot copies the whole device

@@ -67,14 +67,20 @@ struct BatchDev {
     float g[3];
     int iterations;
     uint64_t* phase_ts;  // optional [nworlds][PHASES] timestamps (null: off)
-    int32_t* nl_pair;    // [total pair cap] narrowphase work list: global pair slots of all worlds
+    // Narrowphase work lists, one segment per size class (NarrowSeg): the global pair slots of the
+    // class's worlds, appended by its K1 and worked by its K2.
+    int32_t* nl_pair;    // [total pair cap] listed pair slots
     int32_t* nl_limit;   // [total pair cap] per listed pair: first slot of its world that starts fresh
-    int32_t* nl_count;   // [1] entries in the list this step
-    int32_t* nl_shape;   // [nl_cap] list entries whose pair involves a cylinder or a mesh: from the
-                         // front those of k_narrow_shape_list, from the back those of k_narrow_sat_list
-    int32_t* nl_shape_count;  // [1]
-    int32_t* nl_sat_count;    // [1]
-    int32_t nl_cap;
+    int32_t* nl_shape;   // [total pair cap] per segment, list entries whose pair involves a cylinder or a
+                         // mesh: from the front those of k_narrow_shape_list, from the back those of
+                         // k_narrow_sat_list
+    int32_t* nl_count;   // [3 * MAX_CLASSES] per class: listed pairs, shape entries, SAT entries
+};
+constexpr int MAX_CLASSES = 32;
+// A class's narrowphase list segment: entries [off, off + cap) of the list arrays, counters
+// nl_count[3 * cls .. 3 * cls + 2].
+struct NarrowSeg {
+    int off, cap, cls;
 };
 
 constexpr uint64_t KEY_PAD = ~0ull;
@@ -129,7 +135,7 @@ __host__ inline size_t solve_lds_bytes(int nmax, int group_bodies) {
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t* __restrict__ wlist, int nmax) {
+__global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t* __restrict__ wlist, int nmax, NarrowSeg g) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int wsum[BLOCK / WAVE + 1];
     __shared__ double red[6][BLOCK / WAVE];
@@ -257,9 +263,9 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
     __syncthreads();
 
     PE_PHASE(3);
-    // hand the pairs to the narrowphase list (the order of worlds in the list does not matter)
+    // hand the pairs to the class's narrowphase list (the order of worlds in it does not matter)
     if (tid == 0) {
-        s_base = atomicAdd(d.nl_count, npairs);
+        s_base = g.off + atomicAdd(&d.nl_count[3 * g.cls], npairs);
         d.st[w].npairs = npairs;
     }
     __syncthreads();
@@ -272,42 +278,41 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
 // K2: SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) over the pairs of
 // all worlds at once, one thread per pair.
 // Box and sphere pairs here; pairs with a cylinder or a mesh are listed for K2b.
-__global__ void __launch_bounds__(128) k_narrow_list(BatchDev d) {
-    const int cnt = *d.nl_count;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(128) k_narrow_list(BatchDev d, NarrowSeg g) {
+    const int cnt = d.nl_count[3 * g.cls];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
+        const int q = g.off + k;
         const int p = d.nl_pair[q];
         const bool simple = pair_is_simple(d.b, d.ps, p);
         const bool sat = !simple && pair_is_sat(d.b, d.ps, p);
-        const int slot = wave_append(!simple && !sat, d.nl_shape_count);
-        const int sslot = wave_append(sat, d.nl_sat_count);
+        const int slot = wave_append(!simple && !sat, &d.nl_count[3 * g.cls + 1]);
+        const int sslot = wave_append(sat, &d.nl_count[3 * g.cls + 2]);
         if (simple) narrow_pair<false>(d.b, d.ps, p, d.nl_limit[q]);
-        else if (sat) d.nl_shape[d.nl_cap - 1 - sslot] = q;
-        else d.nl_shape[slot] = q;
+        else if (sat) d.nl_shape[g.off + g.cap - 1 - sslot] = q;
+        else d.nl_shape[g.off + slot] = q;
     }
 }
 // K2b: the listed pairs that involve a cylinder, a convex mesh or a concave mesh.
-__global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d) {
-    const int cnt = *d.nl_shape_count;
+__global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d, NarrowSeg g) {
+    const int cnt = d.nl_count[3 * g.cls + 1];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-        const int q = d.nl_shape[k];
+        const int q = d.nl_shape[g.off + k];
         narrow_pair<true>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
     }
 }
 
 // K2c: the listed separating-axis pairs (pair_is_sat), one wave each.
-__global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d) {
-    const int cnt = *d.nl_sat_count;
+__global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d, NarrowSeg g) {
+    const int cnt = d.nl_count[3 * g.cls + 2];
     const int nw = gridDim.x * (blockDim.x / WAVE);
     for (int k = blockIdx.x * (blockDim.x / WAVE) + (int)(threadIdx.x / WAVE); k < cnt; k += nw) {
-        const int q = d.nl_shape[d.nl_cap - 1 - k];
+        const int q = d.nl_shape[g.off + g.cap - 1 - k];
         narrow_pair<true, WAVE>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
     }
 }
 
 __global__ void k_list_reset(BatchDev d) {
-    *d.nl_count = 0;
-    *d.nl_shape_count = 0;
-    *d.nl_sat_count = 0;
+    for (int k = threadIdx.x; k < 3 * MAX_CLASSES; k += blockDim.x) d.nl_count[k] = 0;
 }
 
 // K3 for one world on the global row store (claims in LDS): solver setup, colouring,
@@ -627,9 +632,9 @@ void set_lds_limits() {
 // groups[0..ngroups) (first index into the world list, world count).
 template <int B>
 void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups, int ngroups,
-                int group_bodies, hipStream_t s) {
+                int group_bodies, NarrowSeg g, hipStream_t s) {
     if (phase == 0) {
-        hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax);
+        hipLaunchKernelGGL(k_world_broad<B>, dim3(count), dim3(B), broad_lds_bytes(nmax), s, d, wl, nmax, g);
     } else {
         const size_t lb = solve_lds_bytes(nmax, group_bodies);
         if (lb < SOLVE_LDS_CAP)
@@ -639,13 +644,13 @@ void launch_one(int phase, const BatchDev& d, const int32_t* wl, int count, int 
     }
 }
 hipError_t launch_class(int phase, int block, const BatchDev& d, const int32_t* wl, int count, int nmax, const int2* groups,
-                        int ngroups, int group_bodies, hipStream_t s) {
+                        int ngroups, int group_bodies, NarrowSeg g, hipStream_t s) {
     switch (block) {
-        case 64: launch_one<64>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
-        case 128: launch_one<128>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
-        case 256: launch_one<256>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
-        case 512: launch_one<512>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
-        default: launch_one<1024>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, s); break;
+        case 64: launch_one<64>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, g, s); break;
+        case 128: launch_one<128>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, g, s); break;
+        case 256: launch_one<256>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, g, s); break;
+        case 512: launch_one<512>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, g, s); break;
+        default: launch_one<1024>(phase, d, wl, count, nmax, groups, ngroups, group_bodies, g, s); break;
     }
     return hipGetLastError();
 }
@@ -662,8 +667,10 @@ struct Class {
     int list_off = 0;   // offset into the device world list
     int group_off = 0;  // offset into the device group list
     int ngroups = 0;
-    double cost = -1.0; // measured K1 + K3 device time per step in ms (< 0: not measured)
+    double cost = -1.0; // measured K1 + K2 + K3 device time per step in ms (< 0: not measured)
     int stream = 0;     // 0: the main stream, k > 0: side stream k - 1
+    int nl_off = 0;     // narrowphase list segment: the summed pair capacity of the classes before it
+    int nl_cap = 0;     // and of its own worlds
 };
 
 // Groups of consecutive class members (first index into the world list, count) holding at most
@@ -757,53 +764,96 @@ void schedule_classes(std::vector<Class>& classes, int nstreams) {
     }
 }
 
-// One step of every listed world: list reset, K1 per class, K2 over all pairs, K3 per class.
-// The classes of K1 and of K3 run concurrently on the main stream s and the side streams as
-// schedule_classes dealt them, heaviest first on each stream.  With class_tm, every class runs
-// on s alone, in turn, timed by its own event pair (stage 3 + 2 * class + phase).
+// K2 of one class over its list segment; grids scale with the segment's pair capacity (about six
+// slots per listed pair) and the kernels stride over what was listed.
+void launch_narrow(const BatchDev& d, const Class& c, int ci, bool shapes, hipStream_t t) {
+    const NarrowSeg g{c.nl_off, c.nl_cap, ci};
+    auto grid = [&](int per, int hi) { return dim3((unsigned)std::max(1, std::min(hi, c.nl_cap / per))); };
+    hipLaunchKernelGGL(k_narrow_list, grid(1024, 4096), dim3(128), 0, t, d, g);
+    if (shapes) {
+        hipLaunchKernelGGL(k_narrow_shape_list, grid(2048, 4096), dim3(64), 0, t, d, g);
+        hipLaunchKernelGGL(k_narrow_sat_list, grid(4096, 2048), dim3(256), 0, t, d, g);
+    }
+}
+
+// One step of every listed world: list reset, then per class K1, K2 over its own pairs and K3.
+// Classes share nothing within a step, so each class's three launch groups run in sequence on the
+// stream schedule_classes dealt it (the main stream s or a side stream, heaviest class first on
+// each), and the classes of different streams overlap freely: no stage waits for every class.
+// With a timer the step is staged instead, for measurement: with class_tm every class runs on s
+// alone, in turn, its K1 + K2 and its K3 each timed by their own event pair (stage 3 + 2 * class
+// + 0 or 1); otherwise K1 of all classes (concurrently), K2 of all classes, K3 of all classes
+// (concurrently), each group timed as stage 0, 1 and 2.
 int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, const int2* dgroups, hipStream_t s,
                const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr, bool class_tm = false) {
-    hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, s, d);
+    hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, d);
     std::vector<int> ord;
     int nused = 0;
     for (int c = 0; c < (int)classes.size(); c++)
         if (classes[c].count > 0) { ord.push_back(c); nused = std::max(nused, classes[c].stream + 1); }
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return classes[a].nmax > classes[b].nmax; });
     const bool fork = !class_tm && nused > 1 && nused <= cs.n + 1;
-    for (int phase = 0; phase < 2; phase++) {
-        if (tm && !class_tm) tm->begin(phase == 0 ? 0 : 2, s);
-        if (fork) {
-            hipEventRecord(cs.fork, s);
-            for (int k = 1; k < nused; k++) hipStreamWaitEvent(cs.side[k - 1], cs.fork, 0);
+    auto stream_of = [&](const Class& c) { return fork && c.stream > 0 ? cs.side[c.stream - 1] : s; };
+    auto k1 = [&](int ci, hipStream_t t) {
+        const Class& c = classes[ci];
+        return launch_class(0, c.block, d, dlist + c.list_off, c.count, c.nmax, nullptr, 0, 0,
+                            NarrowSeg{c.nl_off, c.nl_cap, ci}, t);
+    };
+    auto k3 = [&](int ci, hipStream_t t) {
+        const Class& c = classes[ci];
+        return launch_class(1, c.block, d, dlist, c.count, c.nmax, dgroups + c.group_off, c.ngroups, c.group_bodies,
+                            NarrowSeg{c.nl_off, c.nl_cap, ci}, t);
+    };
+    auto fork_out = [&]() {
+        if (!fork) return;
+        hipEventRecord(cs.fork, s);
+        for (int k = 1; k < nused; k++) hipStreamWaitEvent(cs.side[k - 1], cs.fork, 0);
+    };
+    auto join_in = [&]() {
+        if (!fork) return;
+        for (int k = 1; k < nused; k++) {
+            hipEventRecord(cs.join[k - 1], cs.side[k - 1]);
+            hipStreamWaitEvent(s, cs.join[k - 1], 0);
+        }
+    };
+    hipError_t e = hipSuccess;
+    if (!tm) {
+        fork_out();
+        for (int ci : ord) {
+            const hipStream_t t = stream_of(classes[ci]);
+            if ((e = k1(ci, t)) != hipSuccess) break;
+            launch_narrow(d, classes[ci], ci, shapes, t);
+            if ((e = k3(ci, t)) != hipSuccess) break;
+        }
+        join_in();
+    } else if (class_tm) {
+        for (int ci : ord) {
+            tm->begin(3 + 2 * ci, s);
+            if ((e = k1(ci, s)) != hipSuccess) break;
+            launch_narrow(d, classes[ci], ci, shapes, s);
+            tm->end(s);
         }
         for (int ci : ord) {
-            const Class& c = classes[ci];
-            hipStream_t t = fork && c.stream > 0 ? cs.side[c.stream - 1] : s;
-            if (class_tm) tm->begin(3 + 2 * ci + phase, s);
-            hipError_t e = phase == 0 ? launch_class(0, c.block, d, dlist + c.list_off, c.count, c.nmax, nullptr, 0, 0, t)
-                                      : launch_class(1, c.block, d, dlist, c.count, c.nmax, dgroups + c.group_off, c.ngroups,
-                                                     c.group_bodies, t);
-            if (class_tm) tm->end(s);
-            if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
+            tm->begin(3 + 2 * ci + 1, s);
+            if ((e = k3(ci, s)) != hipSuccess) break;
+            tm->end(s);
         }
-        if (fork) {
-            for (int k = 1; k < nused; k++) {
-                hipEventRecord(cs.join[k - 1], cs.side[k - 1]);
-                hipStreamWaitEvent(s, cs.join[k - 1], 0);
-            }
-        }
-        if (tm && !class_tm) tm->end(s);
-        if (phase == 0) {
-            if (tm && !class_tm) tm->begin(1, s);
-            hipLaunchKernelGGL(k_narrow_list, dim3(4096), dim3(128), 0, s, d);
-            if (shapes) {
-                hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, s, d);
-                hipLaunchKernelGGL(k_narrow_sat_list, dim3(2048), dim3(256), 0, s, d);
-            }
-            if (tm && !class_tm) tm->end(s);
-        }
+    } else {
+        tm->begin(0, s);
+        fork_out();
+        for (int ci : ord) if ((e = k1(ci, stream_of(classes[ci]))) != hipSuccess) break;
+        join_in();
+        tm->end(s);
+        tm->begin(1, s);
+        for (int ci : ord) launch_narrow(d, classes[ci], ci, shapes, s);
+        tm->end(s);
+        tm->begin(2, s);
+        fork_out();
+        for (int ci : ord) if (e == hipSuccess && (e = k3(ci, stream_of(classes[ci]))) != hipSuccess) break;
+        join_in();
+        tm->end(s);
     }
-    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
     return PE_OK;
 }
@@ -886,11 +936,16 @@ struct pe_batch {
         std::vector<int2> grp;
         std::vector<int> nb(nworlds);
         for (int w = 0; w < nworlds; w++) nb[w] = hdesc[w].n;
+        int nl = 0;
         for (size_t c = 0; c < classes.size(); c++) {
             auto& m = members[c];
             std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
             classes[c].count = (int)m.size();
             classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
+            classes[c].nl_off = nl;
+            classes[c].nl_cap = 0;
+            for (int w : m) classes[c].nl_cap += hdesc[w].pair_cap;
+            nl += classes[c].nl_cap;
             classes[c].list_off = (int)list.size();
             list.insert(list.end(), m.begin(), m.end());
             // the layout follows this order (largest worlds first, ties by index), so consecutive
@@ -983,11 +1038,8 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     rc |= B->alloc(&d.cstart, poff + nworlds);
     rc |= B->alloc(&d.nl_pair, poff);
     rc |= B->alloc(&d.nl_limit, poff);
-    rc |= B->alloc(&d.nl_count, 1);
     rc |= B->alloc(&d.nl_shape, poff);
-    rc |= B->alloc(&d.nl_shape_count, 1);
-    rc |= B->alloc(&d.nl_sat_count, 1);
-    d.nl_cap = (int32_t)poff;
+    rc |= B->alloc(&d.nl_count, 3 * MAX_CLASSES);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= B->alloc(&ddesc, nworlds);
@@ -1030,6 +1082,7 @@ int pe_batch_destroy(pe_batch* B) {
 
 int pe_batch_set_config(pe_batch* B, int32_t nclasses, const int32_t* upper, const int32_t* block) {
     if (!B || nclasses <= 0 || !upper || !block) { pe_set_error("invalid config"); return PE_ERR_INVALID; }
+    if (nclasses > MAX_CLASSES) { pe_set_error("at most 32 size classes"); return PE_ERR_INVALID; }
     std::vector<Class> cl;
     int lo = 1;
     for (int c = 0; c < nclasses; c++) {
@@ -1391,11 +1444,16 @@ struct pe_server {
         std::vector<int2> grp;
         std::vector<int> nb(hdesc.size());
         for (size_t w = 0; w < hdesc.size(); w++) nb[w] = hdesc[w].n;
+        int nl = 0;   // the live worlds' pair capacities sum to at most the pair arena's size
         for (size_t c = 0; c < classes.size(); c++) {
             auto& m = members[c];
             std::stable_sort(m.begin(), m.end(), [&](int a, int b) { return hdesc[a].n > hdesc[b].n; });
             classes[c].count = (int)m.size();
             classes[c].nmax = m.empty() ? 0 : hdesc[m[0]].n;
+            classes[c].nl_off = nl;
+            classes[c].nl_cap = 0;
+            for (int w : m) classes[c].nl_cap += hdesc[w].pair_cap;
+            nl += classes[c].nl_cap;
             classes[c].list_off = (int)list.size();
             list.insert(list.end(), m.begin(), m.end());
             build_groups(classes[c], m, nb, false, grp);   // arena ranges are not contiguous: one world per group
@@ -1630,11 +1688,8 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&d.cstart, P + cfg->max_worlds);
     rc |= S->alloc(&d.nl_pair, P);
     rc |= S->alloc(&d.nl_limit, P);
-    rc |= S->alloc(&d.nl_count, 1);
     rc |= S->alloc(&d.nl_shape, P);
-    rc |= S->alloc(&d.nl_shape_count, 1);
-    rc |= S->alloc(&d.nl_sat_count, 1);
-    d.nl_cap = (int32_t)P;
+    rc |= S->alloc(&d.nl_count, 3 * MAX_CLASSES);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= S->alloc(&ddesc, cfg->max_worlds);

@@ -170,7 +170,8 @@ class Batch:
 
     def class_times(self, n: int):
         """n eager steps with every size class launched alone, in turn: per class the summed ms of
-        (integrate + broadphase, solver).  Also deals the classes onto streams by these costs."""
+        (integrate + broadphase + narrowphase, solver).  Also deals the classes onto streams by these
+        costs."""
         nc = len(self.config())
         ms = (ctypes.c_double * (2 * nc))()
         check(lib().pe_batch_class_times(self.h, ctypes.c_int32(n), ms, ctypes.c_int32(2 * nc)))
