@@ -32,6 +32,26 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+class StdoutToStderr:
+    """Sends file descriptor 1 to stderr while the compiled reference engine runs: it logs to
+    stdout, which must carry nothing but the result line.  The C library's buffers are flushed
+    before fd 1 is restored."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def stage_bytes(n, n_dyn, pairs, points, hull_bytes=0):
     """Algorithmic (compulsory) HBM bytes of one step per launch group: every stage reads its
     inputs once and writes its outputs once, one Gauss-Seidel sweep counted.
@@ -195,9 +215,11 @@ def run_shape_mix(args):
     if not args.no_cpu:
         from pyoracle import RefWorld, ref_available
         if ref_available(32):
-            r = RefWorld(sc, 32, threads=args.cpu_threads)
-            r.set_state(*[np.asarray(x, np.float64) for x in w.state()])
-            cdt = r.step_timed(args.mix_cpu_steps)
+            with StdoutToStderr():
+                r = RefWorld(sc, 32, threads=args.cpu_threads)
+                r.set_state(*[np.asarray(x, np.float64) for x in w.state()])
+                cdt = r.step_timed(args.mix_cpu_steps)
+                del r
             out["cpu_reference"] = {"body_steps_per_s": sc.n_dynamic() * args.mix_cpu_steps / cdt,
                                     "ms_per_step": 1000 * cdt / args.mix_cpu_steps, "cores": args.cpu_threads,
                                     "sample": f"{args.mix_cpu_steps} steps from the GPU state at step {args.mix_warm + args.mix_steps + 10}"}
@@ -218,9 +240,11 @@ def cpu_baseline(worlds, states, start_step, threads, budget_s):
     for sc, st in zip(worlds, states):
         st64 = [np.asarray(x, np.float64) for x in st]
         if kind == "reference":
-            w = RefWorld(sc, 32, threads=threads)
-            w.set_state(*st64)
-            dt = w.step_timed(steps)
+            with StdoutToStderr():
+                w = RefWorld(sc, 32, threads=threads)
+                w.set_state(*st64)
+                dt = w.step_timed(steps)
+                del w
         else:
             w = OracleWorld(sc, 32)
             w.set_state(*st64)

@@ -82,6 +82,11 @@ constexpr int MAX_CLASSES = 32;
 struct NarrowSeg {
     int off, cap, cls;
 };
+// Every class's segment, for the shape kernels, which run once over all classes.
+struct NarrowSegs {
+    int n;
+    int off[MAX_CLASSES], cap[MAX_CLASSES];
+};
 
 constexpr uint64_t KEY_PAD = ~0ull;
 constexpr int PHASES = 8;   // start, integrate, sort, sweep, narrow, setup, colour, solve
@@ -292,21 +297,44 @@ __global__ void __launch_bounds__(128) k_narrow_list(BatchDev d, NarrowSeg g) {
         else d.nl_shape[g.off + slot] = q;
     }
 }
-// K2b: the listed pairs that involve a cylinder, a convex mesh or a concave mesh.
-__global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d, NarrowSeg g) {
-    const int cnt = d.nl_count[3 * g.cls + 1];
+// Entry k of the concatenation, over all classes, of the shape entries (which = 1, from the front
+// of each segment) or of the separating-axis entries (which = 2, from the back): its list index.
+// pre[c] = entries of the classes before c (pre[n] = all), in LDS.
+__device__ __forceinline__ int seg_entry(const BatchDev& d, const NarrowSegs& g, const int* pre, int which, int k) {
+    int c = 0;
+    while (c + 1 < g.n && pre[c + 1] <= k) c++;
+    const int j = k - pre[c];
+    return which == 1 ? d.nl_shape[g.off[c] + j] : d.nl_shape[g.off[c] + g.cap[c] - 1 - j];
+}
+__device__ __forceinline__ void seg_prefix(const BatchDev& d, const NarrowSegs& g, int which, int* pre) {
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int c = 0; c < g.n; c++) { pre[c] = acc; acc += d.nl_count[3 * c + which]; }
+        pre[g.n] = acc;
+    }
+    __syncthreads();
+}
+// K2b: the listed pairs of every class that involve a cylinder, a convex mesh or a concave mesh.
+// One launch for all classes: these kernels carry large scratch frames, and launching them per
+// class multiplied their launch cost (measured: K2 of 96 shape worlds 2.6 -> 4.7 ms).
+__global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d, NarrowSegs g) {
+    __shared__ int pre[MAX_CLASSES + 1];
+    seg_prefix(d, g, 1, pre);
+    const int cnt = pre[g.n];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += gridDim.x * blockDim.x) {
-        const int q = d.nl_shape[g.off + k];
+        const int q = seg_entry(d, g, pre, 1, k);
         narrow_pair<true>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
     }
 }
 
-// K2c: the listed separating-axis pairs (pair_is_sat), one wave each.
-__global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d, NarrowSeg g) {
-    const int cnt = d.nl_count[3 * g.cls + 2];
+// K2c: the listed separating-axis pairs (pair_is_sat) of every class, one wave each.
+__global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d, NarrowSegs g) {
+    __shared__ int pre[MAX_CLASSES + 1];
+    seg_prefix(d, g, 2, pre);
+    const int cnt = pre[g.n];
     const int nw = gridDim.x * (blockDim.x / WAVE);
     for (int k = blockIdx.x * (blockDim.x / WAVE) + (int)(threadIdx.x / WAVE); k < cnt; k += nw) {
-        const int q = d.nl_shape[g.off + g.cap - 1 - k];
+        const int q = seg_entry(d, g, pre, 2, k);
         narrow_pair<true, WAVE>(d.b, d.ps, d.nl_pair[q], d.nl_limit[q]);
     }
 }
@@ -764,26 +792,32 @@ void schedule_classes(std::vector<Class>& classes, int nstreams) {
     }
 }
 
-// K2 of one class over its list segment; grids scale with the segment's pair capacity (about six
-// slots per listed pair) and the kernels stride over what was listed.
-void launch_narrow(const BatchDev& d, const Class& c, int ci, bool shapes, hipStream_t t) {
+// K2 of one class over its list segment (box and sphere pairs; the shape pairs it lists run in
+// launch_shapes); the grid scales with the segment's pair capacity (about six slots per listed
+// pair: about a pair and a half per thread) and the kernel strides over what was listed.
+void launch_narrow(const BatchDev& d, const Class& c, int ci, hipStream_t t) {
     const NarrowSeg g{c.nl_off, c.nl_cap, ci};
-    auto grid = [&](int per, int hi) { return dim3((unsigned)std::max(1, std::min(hi, c.nl_cap / per))); };
-    hipLaunchKernelGGL(k_narrow_list, grid(1024, 4096), dim3(128), 0, t, d, g);
-    if (shapes) {
-        hipLaunchKernelGGL(k_narrow_shape_list, grid(2048, 4096), dim3(64), 0, t, d, g);
-        hipLaunchKernelGGL(k_narrow_sat_list, grid(4096, 2048), dim3(256), 0, t, d, g);
-    }
+    hipLaunchKernelGGL(k_narrow_list, dim3((unsigned)std::max(1, std::min(4096, c.nl_cap / 1024))), dim3(128), 0, t, d, g);
+}
+// K2b and K2c over the shape pairs every class listed.
+void launch_shapes(const BatchDev& d, const std::vector<Class>& classes, hipStream_t t) {
+    NarrowSegs g{};
+    g.n = (int)classes.size();
+    for (int c = 0; c < g.n; c++) { g.off[c] = classes[c].nl_off; g.cap[c] = classes[c].nl_cap; }
+    hipLaunchKernelGGL(k_narrow_shape_list, dim3(4096), dim3(64), 0, t, d, g);
+    hipLaunchKernelGGL(k_narrow_sat_list, dim3(2048), dim3(256), 0, t, d, g);
 }
 
 // One step of every listed world: list reset, then per class K1, K2 over its own pairs and K3.
-// Classes share nothing within a step, so each class's three launch groups run in sequence on the
-// stream schedule_classes dealt it (the main stream s or a side stream, heaviest class first on
-// each), and the classes of different streams overlap freely: no stage waits for every class.
-// With a timer the step is staged instead, for measurement: with class_tm every class runs on s
-// alone, in turn, its K1 + K2 and its K3 each timed by their own event pair (stage 3 + 2 * class
-// + 0 or 1); otherwise K1 of all classes (concurrently), K2 of all classes, K3 of all classes
-// (concurrently), each group timed as stage 0, 1 and 2.
+// Pipelined (worlds of boxes and spheres): classes share nothing within a step, so each class's
+// three launch groups run in sequence on the stream schedule_classes dealt it (the main stream s
+// or a side stream, heaviest class first on each), and the classes of different streams overlap
+// freely: no stage waits for every class.  Staged (worlds with other shapes, and every timed
+// step): K1 of all classes, then K2 of all classes followed by one launch of the shape kernels
+// over every class's shape pairs (launch_shapes), then K3 of all classes, the classes of each
+// stage concurrently on their streams.  With a timer each stage is timed as stage 0, 1 and 2;
+// with class_tm every class runs on s alone, in turn, its K1 + K2 and its K3 each timed by their
+// own event pair (stage 3 + 2 * class + 0 or 1; the shape kernels run untimed between the two).
 int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, const int2* dgroups, hipStream_t s,
                const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr, bool class_tm = false) {
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, d);
@@ -817,12 +851,12 @@ int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32
         }
     };
     hipError_t e = hipSuccess;
-    if (!tm) {
+    if (!tm && !shapes) {
         fork_out();
         for (int ci : ord) {
             const hipStream_t t = stream_of(classes[ci]);
             if ((e = k1(ci, t)) != hipSuccess) break;
-            launch_narrow(d, classes[ci], ci, shapes, t);
+            launch_narrow(d, classes[ci], ci, t);
             if ((e = k3(ci, t)) != hipSuccess) break;
         }
         join_in();
@@ -830,28 +864,32 @@ int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32
         for (int ci : ord) {
             tm->begin(3 + 2 * ci, s);
             if ((e = k1(ci, s)) != hipSuccess) break;
-            launch_narrow(d, classes[ci], ci, shapes, s);
+            launch_narrow(d, classes[ci], ci, s);
             tm->end(s);
         }
+        if (shapes) launch_shapes(d, classes, s);
         for (int ci : ord) {
             tm->begin(3 + 2 * ci + 1, s);
             if ((e = k3(ci, s)) != hipSuccess) break;
             tm->end(s);
         }
     } else {
-        tm->begin(0, s);
+        if (tm) tm->begin(0, s);
         fork_out();
         for (int ci : ord) if ((e = k1(ci, stream_of(classes[ci]))) != hipSuccess) break;
         join_in();
-        tm->end(s);
-        tm->begin(1, s);
-        for (int ci : ord) launch_narrow(d, classes[ci], ci, shapes, s);
-        tm->end(s);
-        tm->begin(2, s);
+        if (tm) tm->end(s);
+        if (tm) tm->begin(1, s);
+        fork_out();
+        for (int ci : ord) launch_narrow(d, classes[ci], ci, stream_of(classes[ci]));
+        join_in();
+        if (shapes) launch_shapes(d, classes, s);
+        if (tm) tm->end(s);
+        if (tm) tm->begin(2, s);
         fork_out();
         for (int ci : ord) if (e == hipSuccess && (e = k3(ci, stream_of(classes[ci]))) != hipSuccess) break;
         join_in();
-        tm->end(s);
+        if (tm) tm->end(s);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) { pe_set_error(std::string("launch: ") + hipGetErrorString(e)); return PE_ERR_DEVICE; }
