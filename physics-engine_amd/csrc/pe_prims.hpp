@@ -81,22 +81,48 @@ static __global__ void __launch_bounds__(1024) k_scan_u32(uint32_t* __restrict__
     if (threadIdx.x == 0 && total) *total = (uint32_t)carry;
 }
 
-// Exclusive scan of *count uint32 in place (count read on the device), one block of 1024 threads.
-static __global__ void __launch_bounds__(1024) k_scan_u32_dyn(uint32_t* __restrict__ data, const uint32_t* __restrict__ count,
-                                                       uint32_t* __restrict__ total) {
+// Grid-wide exclusive scan of n uint32 in place (n = *count when count is given, read on the
+// device), in three launches: k_scan_tiles scans each tile of SCAN_TILE elements and writes the
+// tile's total to partial[tile]; k_scan_u32 scans the partials (and writes the grand total);
+// k_scan_add adds each tile's offset.  Blocks past n do nothing (their partial is 0).
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = 1024 * SCAN_ITEMS;
+static __global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* __restrict__ data, int n_static,
+                                                           const uint32_t* __restrict__ count, uint32_t* __restrict__ partial) {
     __shared__ int wsum[17];
-    const int n = (int)*count;
-    int carry = 0;
-    for (int base = 0; base < n; base += 1024) {
-        const int e = base + threadIdx.x;
-        const int v = e < n ? (int)data[e] : 0;
-        int tot;
-        const int ex = block_exclusive_scan(v, wsum, &tot);
-        if (e < n) data[e] = (uint32_t)(carry + ex);
-        carry += tot;
+    const int n = count ? (int)*count : n_static;
+    const int base = blockIdx.x * SCAN_TILE;
+    if (base >= n) {   // the same for every thread of the block
+        if (threadIdx.x == 0) partial[blockIdx.x] = 0;
+        return;
     }
-    if (threadIdx.x == 0 && total) *total = (uint32_t)carry;
+    const int e0 = base + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = e0 + k < n ? data[e0 + k] : 0u;
+        sum += (int)v[k];
+    }
+    int tot;
+    int run = block_exclusive_scan(sum, wsum, &tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (e0 + k < n) data[e0 + k] = (uint32_t)run;
+        run += (int)v[k];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = (uint32_t)tot;
 }
+static __global__ void __launch_bounds__(1024) k_scan_add(uint32_t* __restrict__ data, int n_static,
+                                                         const uint32_t* __restrict__ count, const uint32_t* __restrict__ partial) {
+    const int n = count ? (int)*count : n_static;
+    const int base = blockIdx.x * SCAN_TILE;
+    if (base >= n) return;
+    const uint32_t add = partial[blockIdx.x];
+    for (int k = threadIdx.x; k < SCAN_TILE; k += 1024)
+        if (base + k < n) data[base + k] += add;
+}
+
 
 // Stable scatter of one digit pass.  Sub-tiles of 256 keys are ranked in order: lanes with
 // equal digits are found with 8 ballots, waves are combined through LDS.

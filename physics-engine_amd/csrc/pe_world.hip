@@ -70,6 +70,7 @@ struct Dev {
     float* skey;            // [n] sorted keys as floats
     uint32_t* cand;         // [n+1] candidate counts -> offsets
     uint32_t* chunk;        // [chunk capacity] pairs per chunk -> offsets
+    uint32_t* scan_part;    // [scan_tiles] tile totals of the grid-wide scans
     uint32_t* hist;         // radix histograms
     double* red;            // [6 * nred] axis reduction partials
     DevCounts* cnt;
@@ -383,6 +384,13 @@ struct pe_world {
         hipLaunchKernelGGL(k_integrate, dim3(grid(n, 256)), dim3(256), 0, stream, d);
         return PE_OK;
     }
+    // Grid-wide exclusive scan of n elements (of *count when count is given; at most cap).
+    void scan(uint32_t* data, int cap, int n, const uint32_t* count, uint32_t* total) {
+        const int nb = std::max(1, (cap + SCAN_TILE - 1) / SCAN_TILE);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(1024), 0, stream, data, n, count, d.scan_part);
+        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.scan_part, nb, total);
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, stream, data, n, count, d.scan_part);
+    }
     int launch_broadphase() {
         hipLaunchKernelGGL(k_bp_keys, dim3(grid(n, 256)), dim3(256), 0, stream, d);
         uint32_t *ki = d.keys, *vi = d.vals, *ko = d.keys2, *vo = d.vals2;
@@ -397,10 +405,10 @@ struct pe_world {
         // after 4 passes the sorted data is back in keys/vals
         hipLaunchKernelGGL(k_bp_ranges, dim3(grid(n, 256)), dim3(256), 0, stream, d);
         hipLaunchKernelGGL(k_bp_cand, dim3(grid(n, 256)), dim3(256), 0, stream, d);
-        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.cand, n + 1, &d.cnt->ncand);
+        scan(d.cand, n + 1, n + 1, nullptr, &d.cnt->ncand);
         hipLaunchKernelGGL(k_bp_nchunks, dim3(1), dim3(1), 0, stream, d, &d.cnt->nchunks);
         hipLaunchKernelGGL(k_bp_chunks<false>, dim3(2048), dim3(256), 0, stream, d);
-        hipLaunchKernelGGL(k_scan_u32_dyn, dim3(1), dim3(1024), 0, stream, d.chunk, &d.cnt->nchunks, &d.cnt->nchunk_pairs);
+        scan(d.chunk, d.chunk_cap, 0, &d.cnt->nchunks, &d.cnt->nchunk_pairs);
         hipLaunchKernelGGL(k_bp_chunks<true>, dim3(2048), dim3(256), 0, stream, d);
         hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, stream, d);
         hipLaunchKernelGGL(k_bp_axis_partial, dim3(red_blocks), dim3(RED_THREADS), 0, stream, d);
@@ -519,6 +527,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
     d.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / CHUNK + 1, (int64_t)1 << 26);
     rc |= w->alloc(&d.chunk, d.chunk_cap);
+    rc |= w->alloc(&d.scan_part, (std::max(d.chunk_cap, n + 1) + SCAN_TILE - 1) / SCAN_TILE + 1);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
 
     // host staging of the body store (FP32) and of the scene's meshes
