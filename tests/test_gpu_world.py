@@ -186,3 +186,37 @@ def test_reference_sample_scene_stages():
         assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
         o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
     assert contacts > 0
+
+
+def test_large_world_colouring_matches_oracle():
+    """A world beyond the bodies whose colouring claims fit LDS (12 288) is coloured with its
+    claims in global memory: its colour classes equal the oracle's (bit-exact) and the solved
+    velocities stay within 1e-5, step after step in lock-step."""
+    import pe_amd
+    from pe_amd import scenes
+    from pyoracle import OracleWorld
+    sc = scenes.grid_world(13000, seed=77)
+    g, o = pe_amd.World(sc), OracleWorld(sc, 32)
+    print(f"[large colouring] n={sc.n}", flush=True)
+    checked = 0
+    for step in range(95):
+        g.integrate(); o.integrate()
+        g.broadphase(); o.broadphase()
+        g.narrowphase(); o.narrowphase()
+        g.solve(); o.solve()
+        g.sync()
+        if step >= 85:
+            gp = g.pairs()
+            assert np.array_equal(gp, o.pairs()), f"pairs step {step}"
+            order, cs = g.solve_order()
+            gcol = np.full(len(gp), -1)
+            for c in range(len(cs) - 1):
+                gcol[order[cs[c]:cs[c + 1]]] = c
+            assert np.array_equal(gcol, o.color_of()), f"colour classes step {step}"
+            _, _, gv, gw = g.state()
+            _, _, ov, ow = o.state()
+            assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+            checked += 1
+        o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+    print(f"[large colouring] pairs {len(g.pairs())}, colours {len(cs) - 1}, steps checked {checked}", flush=True)
+    assert len(cs) - 1 >= 3
