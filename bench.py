@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--mix-warm", type=int, default=240)
     ap.add_argument("--mix-steps", type=int, default=60)
     ap.add_argument("--mix-cpu-steps", type=int, default=2)
+    ap.add_argument("--profile-leg", choices=["box_rain_10k", "shape_mix"], default=None,
+                    help="run only this plain-path leg on the GPU (for a profiler) and print its own line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device-to-device with torch (a kernel of known traffic, "
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
@@ -305,6 +307,16 @@ def main():
         del x, y
 
     from pe_amd import scenes
+    if args.profile_leg == "box_rain_10k":
+        c1 = scenes.box_rain_scene(10000, seed=1001)
+        dt1, cnt1 = run_plain(c1, args.plain_steps)
+        print(json.dumps({"profile_leg": args.profile_leg, "steps": args.plain_steps,
+                          "ms_per_step": 1000 * dt1 / args.plain_steps, "final_counts": cnt1}), flush=True)
+        return
+    if args.profile_leg == "shape_mix":
+        args.no_cpu = True
+        print(json.dumps({"profile_leg": args.profile_leg, **run_shape_mix(args)}), flush=True)
+        return
     t0 = time.perf_counter()
     worlds = scenes.batch_worlds(4096, 8, 512, seed=1003)
     log(f"generated 4096 worlds in {time.perf_counter() - t0:.1f}s")
