@@ -214,23 +214,36 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
             while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cand[mid] <= t0) lo = mid; else hi = mid; }
             s0 = lo;
         }
-        int s = s0;
+        // a thread's candidates come in runs of one body i: its AABB record stays in registers
+        int s = s0, si = -1, i = 0;
+        float A[8], Bj[8];
         for (uint32_t t = t0; t < t1; t++) {
             while (cand[s + 1] <= t) s++;
-            const int i = (int)(uint32_t)key[s];
+            if (s != si) {
+                si = s;
+                i = (int)(uint32_t)key[s];
+                ld_rec<2>(b.abr(off + i), A);
+            }
             const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
-            cnt += bp_valid(b, off + i, off + j) ? 1u : 0u;
+            ld_rec<2>(b.abr(off + j), Bj);
+            cnt += bp_valid_rec(A, Bj) ? 1u : 0u;
         }
         int tot;
         int out = block_exclusive_scan((int)cnt, wsum, &tot);
         npairs = min(tot, wd.pair_cap);
         if (tid == 0 && tot > wd.pair_cap) d.st[w].overflow |= 1;
         s = s0;
+        si = -1;
         for (uint32_t t = t0; t < t1; t++) {
             while (cand[s + 1] <= t) s++;
-            const int i = (int)(uint32_t)key[s];
+            if (s != si) {
+                si = s;
+                i = (int)(uint32_t)key[s];
+                ld_rec<2>(b.abr(off + i), A);
+            }
             const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
-            if (bp_valid(b, off + i, off + j)) {
+            ld_rec<2>(b.abr(off + j), Bj);
+            if (bp_valid_rec(A, Bj)) {
                 if (out < wd.pair_cap) { d.ps.pa[poff + out] = off + i; d.ps.pb[poff + out] = off + j; }
                 out++;
             }

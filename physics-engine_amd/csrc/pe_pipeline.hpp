@@ -148,15 +148,19 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
 
 // ------------------------------------------------------------------ stage 1
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
-__device__ __forceinline__ bool bp_valid(const BodyView& b, int i, int j) {
-    float A[8], B[8];
-    ld_rec<2>(b.abr(i), A);
-    ld_rec<2>(b.abr(j), B);
+// The same test on AABB records already loaded (min xyz, kinematic flag, max xyz, pad).
+__device__ __forceinline__ bool bp_valid_rec(const float (&A)[8], const float (&B)[8]) {
     if (A[3] != 0.0f && B[3] != 0.0f) return false;
     if (A[0] > B[4] || B[0] > A[4]) return false;
     if (A[1] > B[5] || B[1] > A[5]) return false;
     if (A[2] > B[6] || B[2] > A[6]) return false;
     return true;
+}
+__device__ __forceinline__ bool bp_valid(const BodyView& b, int i, int j) {
+    float A[8], B[8];
+    ld_rec<2>(b.abr(i), A);
+    ld_rec<2>(b.abr(j), B);
+    return bp_valid_rec(A, B);
 }
 
 // ------------------------------------------------------------------ stage 2
