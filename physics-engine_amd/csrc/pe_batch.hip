@@ -214,7 +214,11 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
             while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (cand[mid] <= t0) lo = mid; else hi = mid; }
             s0 = lo;
         }
-        // a thread's candidates come in runs of one body i: its AABB record stays in registers
+        // a thread's candidates come in runs of one body i: its AABB record stays in registers.
+        // With at most 64 candidates a thread keeps the outcome of each test in a bit mask, and
+        // the emitting pass reads only the sorted order, not the records again.
+        const bool masked = t1 - t0 <= 64;
+        uint64_t hit = 0;
         int s = s0, si = -1, i = 0;
         float A[8], Bj[8];
         for (uint32_t t = t0; t < t1; t++) {
@@ -226,26 +230,42 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
             }
             const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
             ld_rec<2>(b.abr(off + j), Bj);
-            cnt += bp_valid_rec(A, Bj) ? 1u : 0u;
+            if (bp_valid_rec(A, Bj)) {
+                cnt++;
+                if (masked) hit |= 1ull << (t - t0);
+            }
         }
         int tot;
         int out = block_exclusive_scan((int)cnt, wsum, &tot);
         npairs = min(tot, wd.pair_cap);
         if (tid == 0 && tot > wd.pair_cap) d.st[w].overflow |= 1;
         s = s0;
-        si = -1;
-        for (uint32_t t = t0; t < t1; t++) {
-            while (cand[s + 1] <= t) s++;
-            if (s != si) {
-                si = s;
-                i = (int)(uint32_t)key[s];
-                ld_rec<2>(b.abr(off + i), A);
-            }
-            const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
-            ld_rec<2>(b.abr(off + j), Bj);
-            if (bp_valid_rec(A, Bj)) {
-                if (out < wd.pair_cap) { d.ps.pa[poff + out] = off + i; d.ps.pb[poff + out] = off + j; }
+        if (masked) {
+            while (hit) {
+                const uint32_t t = t0 + (uint32_t)(__ffsll((unsigned long long)hit) - 1);
+                hit &= hit - 1;
+                while (cand[s + 1] <= t) s++;
+                if (out < wd.pair_cap) {
+                    d.ps.pa[poff + out] = off + (int)(uint32_t)key[s];
+                    d.ps.pb[poff + out] = off + (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
+                }
                 out++;
+            }
+        } else {
+            si = -1;
+            for (uint32_t t = t0; t < t1; t++) {
+                while (cand[s + 1] <= t) s++;
+                if (s != si) {
+                    si = s;
+                    i = (int)(uint32_t)key[s];
+                    ld_rec<2>(b.abr(off + i), A);
+                }
+                const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
+                ld_rec<2>(b.abr(off + j), Bj);
+                if (bp_valid_rec(A, Bj)) {
+                    if (out < wd.pair_cap) { d.ps.pa[poff + out] = off + i; d.ps.pb[poff + out] = off + j; }
+                    out++;
+                }
             }
         }
     }
