@@ -156,7 +156,9 @@ int pe_world_set_axis(pe_world* w, int32_t axis);
  * the four timers of World (world.h:58-61).  enable = 1 records hipEvents around stages. */
 int pe_world_set_timing(pe_world* w, int32_t enable);
 int pe_world_get_stage_times(pe_world* w, double* ms /*4*/);
-/* Workgroups of the grid-wide Gauss-Seidel sweep (0 = default; clamped to the co-resident limit). */
+/* Workgroups of the grid-wide Gauss-Seidel sweep (0 = the default: 4 to 16 by body count, about one
+ * colour's pair per thread; clamped to the co-resident limit).  No counterpart in the reference; results
+ * do not depend on it. */
 int pe_world_set_solver_blocks(pe_world* w, int32_t blocks);
 /* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours. */
 int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
