@@ -33,6 +33,7 @@ namespace {
 
 constexpr int MAX_COLORS = 1024;
 constexpr int CHUNK = 32;            // candidates per thread in the sweep
+static_assert(CHUNK <= 32, "a chunk's test outcomes are one 32-bit mask");
 constexpr int SOLVE_THREADS = 1024;
 constexpr int CLAIM_LDS = 12288;     // bodies whose colouring claims fit in LDS (48 KiB)
 
@@ -70,6 +71,7 @@ struct Dev {
     float* skey;            // [n] sorted keys as floats
     uint32_t* cand;         // [n+1] candidate counts -> offsets
     uint32_t* chunk;        // [chunk capacity] pairs per chunk -> offsets
+    uint32_t* chunk_hit;    // [chunk capacity] per chunk, bit k: its candidate k is a pair
     uint32_t* scan_part;    // [scan_tiles] tile totals of the grid-wide scans
     uint32_t* hist;         // radix histograms
     double* red;            // [6 * nred] axis reduction partials
@@ -149,23 +151,44 @@ __global__ void k_bp_chunks(Dev d) {
         const uint32_t tend = min(S, t + CHUNK);
         int s = bp_find(d.cand, n, t);
         while (d.cand[s + 1] <= t) s++;
-        uint32_t out = EMIT ? d.chunk[c] : 0;
-        uint32_t cntv = 0;
-        const int i = (int)d.vals[s];
-        int cur_s = s, ci = i;
-        for (; t < tend; t++) {
-            while (d.cand[cur_s + 1] <= t) { cur_s++; ci = (int)d.vals[cur_s]; }
-            const int j = (int)d.vals[cur_s + 1 + (int)(t - d.cand[cur_s])];
-            if (bp_valid(d.b, ci, j)) {
-                if (EMIT) {
-                    if ((int)out < d.pcap) { d.ps.pa[out] = ci; d.ps.pb[out] = j; }
-                    out++;
-                } else {
+        if (EMIT) {
+            // the counting pass left the outcome of each of the chunk's tests in chunk_hit[c]
+            uint32_t out = d.chunk[c];
+            uint32_t hit = d.chunk_hit[c];
+            const uint32_t t0 = t;
+            int cur_s = s;
+            while (hit) {
+                const uint32_t tt = t0 + (uint32_t)(__ffs(hit) - 1);
+                hit &= hit - 1;
+                while (d.cand[cur_s + 1] <= tt) cur_s++;
+                if ((int)out < d.pcap) {
+                    d.ps.pa[out] = (int)d.vals[cur_s];
+                    d.ps.pb[out] = (int)d.vals[cur_s + 1 + (int)(tt - d.cand[cur_s])];
+                }
+                out++;
+            }
+        } else {
+            uint32_t cntv = 0, hit = 0;
+            const uint32_t t0 = t;
+            int cur_s = s, ci = (int)d.vals[s];
+            float A[8], B[8];
+            ld_rec<2>(d.b.abr(ci), A);   // the run body's record stays in registers
+            for (; t < tend; t++) {
+                while (d.cand[cur_s + 1] <= t) {
+                    cur_s++;
+                    ci = (int)d.vals[cur_s];
+                    ld_rec<2>(d.b.abr(ci), A);
+                }
+                const int j = (int)d.vals[cur_s + 1 + (int)(t - d.cand[cur_s])];
+                ld_rec<2>(d.b.abr(j), B);
+                if (bp_valid_rec(A, B)) {
                     cntv++;
+                    hit |= 1u << (t - t0);
                 }
             }
+            d.chunk[c] = cntv;
+            d.chunk_hit[c] = hit;
         }
-        if (!EMIT) d.chunk[c] = cntv;
     }
 }
 
@@ -255,6 +278,7 @@ __global__ void k_narrow_finish(Dev d) {
     d.cnt->prev_npairs = d.cnt->npairs;
     d.cnt->nshape = 0;
     d.cnt->nsat = 0;
+    d.cnt->npoints = 0;   // summed by k_count_points next
 }
 
 // ------------------------------------------------------------------ stage 3: solver
@@ -331,17 +355,16 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
     for (int i = gtid; i < d.b.n; i += gsz) sync_body(d.b, i);
 }
 
-__global__ void k_count_points(Dev d) {
-    __shared__ int wsum[17];
-    int acc = 0;
+// Contact points of this narrowphase: per-workgroup sums added to the counter k_narrow_finish
+// cleared.
+__global__ void __launch_bounds__(256) k_count_points(Dev d) {
+    __shared__ int wsum[256 / WAVE + 1];
     const int np = d.cnt->npairs;
-    for (int base = 0; base < np; base += blockDim.x) {
-        const int p = base + threadIdx.x;
-        int tot;
-        block_exclusive_scan(p < np ? d.ps.npts[p] : 0, wsum, &tot);
-        acc += tot;
-    }
-    if (threadIdx.x == 0) d.cnt->npoints = acc;
+    int acc = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) acc += d.ps.npts[p];
+    int tot;
+    block_exclusive_scan(acc, wsum, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(&d.cnt->npoints, tot);
 }
 
 }  // namespace
@@ -428,7 +451,7 @@ struct pe_world {
             hipLaunchKernelGGL(k_narrow_sat, dim3(std::min(grid(d.pcap, 4), 4096)), dim3(256), 0, stream, d);
         }
         hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
-        hipLaunchKernelGGL(k_count_points, dim3(1), dim3(1024), 0, stream, d);
+        hipLaunchKernelGGL(k_count_points, dim3(grid(d.pcap, 1024)), dim3(256), 0, stream, d);
         return PE_OK;
     }
     int launch_solve() {
@@ -533,6 +556,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
     d.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / CHUNK + 1, (int64_t)1 << 26);
     rc |= w->alloc(&d.chunk, d.chunk_cap);
+    rc |= w->alloc(&d.chunk_hit, d.chunk_cap);
     rc |= w->alloc(&d.scan_part, (std::max(d.chunk_cap, n + 1) + SCAN_TILE - 1) / SCAN_TILE + 1);
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
 
