@@ -222,14 +222,14 @@ __device__ __forceinline__ Poly body_poly(const MeshSet& ms, const CollBody& B) 
     return poly_mesh(&ms, __float_as_int(B.sp[0]));
 }
 // NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) for the pairs that involve a
-// cylinder, a convex mesh or a concave mesh, with the role and swap flag each algorithm's
-// processCollision gives the two bodies; the swap flag is cleared again afterwards, as
-// processCollision does.  SimpleNarrowPhase passes refScale = (scaleA + scaleB) * 0.05
-// (simple_narrow_phase.cpp:38).
+// cylinder, a convex mesh or a concave mesh and are not separating-axis pairs (pair_is_sat:
+// those run a wave each, in narrow_pair's G > 1 branch, and never reach this thread-per-pair
+// routine, which so carries neither the separating-axis routine's registers nor its clip
+// buffers), with the role and swap flag each algorithm's processCollision gives the two bodies;
+// the swap flag is cleared again afterwards, as processCollision does.
 static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, PairCtx& x, const CollBody& A,
                                                   const CollBody& B) {
     const int ta = A.type, tb = B.type;
-    const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
     auto is = [&](int p, int q) { return (ta == p && tb == q) || (ta == q && tb == p); };
     if (is(SHAPE_BOX, SHAPE_CYLINDER)) {                     // box_cylinder_collision_algorithm.cpp:13-47
         const CollBody& bx = ta == SHAPE_BOX ? A : B;
@@ -241,11 +241,6 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
         const CollBody& cy = ta == SHAPE_SPHERE ? B : A;
         x.swap = ta == SHAPE_SPHERE;
         sphere_cylinder(c, x, sp.P, sp.sp[0], cy.P, cy.sp[0], cy.sp[1]);
-    } else if ((ta == SHAPE_CYLINDER && tb == SHAPE_CYLINDER) || is(SHAPE_BOX, SHAPE_CONVEX) ||
-               (ta == SHAPE_CONVEX && tb == SHAPE_CONVEX)) {
-        // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
-        const Poly PA = body_poly(ms, A), PB = body_poly(ms, B);
-        convex_convex<1>(c, x, PA, PB, A.P, B.P, ref_scale);
     } else if (is(SHAPE_SPHERE, SHAPE_CONVEX)) {             // sphere_convex_collision_algorithm.cpp:11-34
         const CollBody& sp = ta == SHAPE_SPHERE ? A : B;
         const CollBody& mb = ta == SHAPE_SPHERE ? B : A;
@@ -264,7 +259,6 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
         x.swap = ta == SHAPE_CONCAVE;
         if (ob.type == SHAPE_SPHERE) sphere_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0]);
         else if (ob.type == SHAPE_CYLINDER) cylinder_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0], ob.sp[1]);
-        else concave_poly<1>(c, x, ms, mesh, mb.P, body_poly(ms, ob), ob.P, ref_scale);
     }
     x.swap = false;
 }
