@@ -144,7 +144,7 @@ def run_served(args):
     sizes = scenes.log_uniform_counts(rng, n_req, 16, 4096)
     horizons = rng.integers(30, 601, n_req)
     t0 = time.perf_counter()
-    # mixed shapes as configs[2] (boxes, spheres, cylinders, hulls over a heightfield); a pool of
+    # mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a heightfield); a pool of
     # distinct request scenes, sized from the same distribution, is cycled through
     pool = [scenes.served_shape_world(int(s), scenes.world_seed(1004, i))
             for i, s in enumerate(sizes[:args.serve_pool])]

@@ -529,7 +529,7 @@ def shape_mix_scene(n: int = 50000, seed: int = 1002, volume=(100.0, 60.0, 100.0
 
 
 def served_shape_world(n: int, seed: int, hull_pool: int = 8, grid: int = 16) -> Scene:
-    """A served request's world in the configs[2] shape mix: n bodies (40 % boxes, 25 % spheres,
+    """A served request's world in the 50k shape mix's proportions: n bodies (40 % boxes, 25 % spheres,
     15 % cylinders, 20 % convex hulls from a pool of hull_pool meshes) over a grid x grid
     heightfield sized to a footprint that stacks the lattice about four cells high."""
     side = 1.8 * max(3, int(math.ceil(math.sqrt(n / 4.0))))
