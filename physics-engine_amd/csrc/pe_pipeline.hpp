@@ -94,10 +94,12 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
         Iw = mmt(mm(R, L), R);
         st3(b, F_POS, i, pos);
         st9(b, F_ROT, i, R);
-        // RigidBody::applyDamping (rigidbody.cpp:201-205)
-        const float pl = powf(1.0f - b.at(F_LD, i), dt), pw = powf(1.0f - b.at(F_AD, i), dt);
-        v = mul(v, pl);
-        w = mul(w, pw);
+        // RigidBody::applyDamping (rigidbody.cpp:201-205).  With zero damping the factor is
+        // pow(1, dt) = 1 exactly and the product leaves every component as it is, so the power
+        // is only evaluated for a damped body.
+        const float ld = b.at(F_LD, i), ad = b.at(F_AD, i);
+        if (ld != 0.0f) v = mul(v, powf(1.0f - ld, dt));
+        if (ad != 0.0f) w = mul(w, powf(1.0f - ad, dt));
         // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
         const float m = b.at(F_MASS, i);
         im = b.at(F_IM, i);
