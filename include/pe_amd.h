@@ -41,7 +41,10 @@ extern "C" {
  * ConcaveMeshShape::setMesh (pe::Mesh, common/mesh.h: vertex positions, faces as vertex index
  * lists with unit normals).  The library runs setMesh itself: a convex body's mesh is moved to
  * its centre of mass and its faces are reordered by the BVH build
- * (convex_mesh_shape.cpp:12-132); a concave body's mesh keeps its frame (concave_mesh_shape.cpp:6-17). */
+ * (convex_mesh_shape.cpp:12-132); a concave body's mesh keeps its frame (concave_mesh_shape.cpp:6-17).
+ * Limits of the device narrowphase, checked here (PE_ERR_INVALID beyond them, where the reference
+ * has none): at most 32 vertices per face, and a BVH at most 46 levels deep (always true below
+ * 2^44 faces: the tree is a median split). */
 typedef struct pe_mesh_desc {
     int32_t nv;                       /* vertices */
     const double* vertices;           /* [3 nv] */
@@ -120,14 +123,18 @@ int pe_world_step(pe_world* w, int32_t nsteps);
 /* Stage entry points; one call of each, in this order, is one World::step. */
 /* World::updateObjectStatus + applyExternalForce (world.cpp:35-127) */
 int pe_world_integrate(pe_world* w);
-/* BroadPhaseBase::calcCollisionPairs, sweep-and-prune (broad_phase_sweep_and_prune.cpp:6-41) */
+/* BroadPhaseBase::calcCollisionPairs, sweep-and-prune (broad_phase_sweep_and_prune.cpp:6-41).
+ * With npairs non-null it waits for the stage and reports capacity errors as pe_world_sync does. */
 int pe_world_broadphase(pe_world* w, int32_t* npairs);
 /* NarrowPhaseBase::calcContactResults (simple_narrow_phase.cpp:7-63) */
 int pe_world_narrowphase(pe_world* w);
 /* ConstraintSolver::setupSolver + solve (sequential_impulse_solver.cpp:11-81) */
 int pe_world_solve(pe_world* w);
 
-/* Synchronises the world's stream. */
+/* Synchronises the world's stream, then reports capacity errors raised by the steps since the
+ * last report (PE_ERR_CAPACITY, the causes in pe_last_error): more broadphase pairs than max_pairs
+ * (the extra pairs are dropped), more sweep candidates than the sweep's chunk capacity, or a
+ * clipped contact polygon beyond its buffer.  The flags are cleared once reported. */
 int pe_world_sync(pe_world* w);
 
 /* State access (FP32, device precision).  Any pointer may be NULL. */

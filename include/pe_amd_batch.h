@@ -25,6 +25,10 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
 int pe_batch_destroy(pe_batch* b);
 /* nsteps World::step of every world (replays the captured hipGraph when one exists). */
 int pe_batch_step(pe_batch* b, int32_t nsteps);
+/* Waits for the batch's work, then reports capacity errors raised since the last report: a world
+ * whose broadphase found more pairs than its capacity (16 per body + 64; the extra pairs are
+ * dropped) or whose clipper lost contact points returns PE_ERR_CAPACITY naming the worlds.  Each
+ * world's flag stays set (pe_batch_get_counts counts them). */
 int pe_batch_sync(pe_batch* b);
 /* Captures one step (one launch per size class) into a hipGraph used by pe_batch_step. */
 int pe_batch_capture_graph(pe_batch* b);
@@ -43,7 +47,7 @@ int pe_batch_get_solver_groups(pe_batch* b, int32_t* nclasses, int32_t* group_bo
 int pe_batch_get_config(pe_batch* b, int32_t* nclasses, int32_t* upper, int32_t* block, int32_t* count, int32_t cap);
 int pe_batch_get_world_state(pe_batch* b, int32_t world, float* position, float* rotation,
                              float* linear_velocity, float* angular_velocity);
-/* bodies, dynamic bodies, pairs, contact points, max colours, worlds with overflow, worlds */
+/* bodies, dynamic bodies, pairs, contact points, max colours, worlds that ever exceeded a capacity, worlds */
 int pe_batch_get_counts(pe_batch* b, int64_t* counts /*7*/);
 /* Device bytes one world of this scene occupies (bodies, pair slots, solver rows). */
 int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes);
@@ -106,13 +110,22 @@ int pe_server_submit(pe_server* s, const pe_scene_desc* scene, int32_t horizon, 
 int pe_server_tick(pe_server* s, int32_t* live_worlds, int64_t* live_bodies);
 /* Ids of requests that finished since the last poll. */
 int pe_server_poll(pe_server* s, int64_t* ids, int32_t cap, int32_t* count);
-/* Final state of a finished request (n = its body count); frees the stored result. */
+/* Final state of a finished request (n = its body count); frees the stored result.  A request
+ * whose world exceeded a capacity while stepping (more pairs than 16 per body + 64, or a clipped
+ * contact polygon beyond its buffer) returns PE_ERR_CAPACITY and no state. */
 int pe_server_result(pe_server* s, int64_t id, float* position, float* rotation, float* linear_velocity,
                      float* angular_velocity);
-/* Drops a queued or live request; its slot is reclaimed at the next tick. */
+/* Drops a queued or live request: a queued one at once, a live one (or one whose upload is in
+ * flight) at the next tick, which reclaims its slot and memory.  A finished request is not
+ * cancelled (PE_ERR_INVALID); its result stays available. */
 int pe_server_cancel(pe_server* s, int64_t id);
-/* ticks, p50 tick ms, p99 tick ms, mean tick ms, body-steps done, live worlds, queued, admitted */
+/* ticks, p50 tick ms, p99 tick ms, mean tick ms, body-steps done, live worlds, queued, admitted.
+ * A tick steps every live world once, so the tick time is each live world's step latency. */
 int pe_server_stats(pe_server* s, double* out /*8*/);
+/* Per-request figures: p50 and p99 admission wait (submission to admission, ms), p50 and p99
+ * latency (submission to result available, ms), mean and maximum queue depth over ticks,
+ * requests finished, requests cancelled, results refused with PE_ERR_CAPACITY. */
+int pe_server_request_stats(pe_server* s, double* out /*9*/);
 
 #ifdef __cplusplus
 }

@@ -51,9 +51,14 @@ struct WorldState {     // per world, updated by every step
     int32_t axis;
     int32_t ncolors;
     int32_t npoints;
-    int32_t overflow;
+    int32_t overflow;   // sticky capacity-error bits (WOVF_*) since the world was created / admitted
     int32_t pad[2];
 };
+// Capacity errors of a world: its pairs beyond its pair capacity were dropped, or a clipped
+// contact polygon outgrew its buffer.  Each is also OR-ed into BatchDev::ovf_any, which the host
+// reads at every synchronisation (one word) and reports as PE_ERR_CAPACITY.
+constexpr int WOVF_PAIRS = 1;
+constexpr int WOVF_CLIP = 8;
 
 struct BatchDev {
     BodyView b;
@@ -75,6 +80,7 @@ struct BatchDev {
                          // mesh: from the front those of k_narrow_shape_list, from the back those of
                          // k_narrow_sat_list
     int32_t* nl_count;   // [3 * MAX_CLASSES] per class: listed pairs, shape entries, SAT entries
+    int32_t* ovf_any;    // [1] OR of the capacity-error bits raised since the host last read it
 };
 constexpr int MAX_CLASSES = 32;
 // A class's narrowphase list segment: entries [off, off + cap) of the list arrays, counters
@@ -238,7 +244,10 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
         int tot;
         int out = block_exclusive_scan((int)cnt, wsum, &tot);
         npairs = min(tot, wd.pair_cap);
-        if (tid == 0 && tot > wd.pair_cap) d.st[w].overflow |= 1;
+        if (tid == 0 && tot > wd.pair_cap) {
+            d.st[w].overflow |= WOVF_PAIRS;
+            atomicOr(d.ovf_any, WOVF_PAIRS);
+        }
         s = s0;
         if (masked) {
             while (hit) {
@@ -396,6 +405,10 @@ __device__ void solve_world_global(const BatchDev& d, int w, int* claim, int* ws
         if (p < npairs) {
             np_ = d.ps.npts[poff + p];
             if (np_ > 0) flag = !(b.iat(I_KIN, d.ps.pa[poff + p]) && b.iat(I_KIN, d.ps.pb[poff + p]));
+            if (d.ps.swap[poff + p] & 2) {   // the narrowphase's clipper lost points of this pair
+                atomicOr(&d.st[w].overflow, WOVF_CLIP);
+                atomicOr(d.ovf_any, WOVF_CLIP);
+            }
         }
         int tot;
         const int ex = block_exclusive_scan(flag, wsum, &tot);
@@ -510,6 +523,10 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
             const int gs = d.desc[G.wl[k]].pair_off + (lp - s_wb[k]);
             const int pa = d.ps.pa[gs], pb = d.ps.pb[gs];
             np_ = d.ps.npts[gs];
+            if (d.ps.swap[gs] & 2) {   // the narrowphase's clipper lost points of this pair
+                atomicOr(&d.st[G.wl[k]].overflow, WOVF_CLIP);
+                atomicOr(d.ovf_any, WOVF_CLIP);
+            }
             gidx[lp] = gs;
             pri[lp] = colour::prio(lp - s_wb[k]);
             pab[lp] = (pa - B0) | ((pb - B0) << 16);
@@ -672,7 +689,10 @@ __global__ void k_refresh_range(BodyView b, int first, int count) {
 }
 
 // ------------------------------------------------------------------ host-side helpers
-int body_pair_cap(int n) { return 8 * n + 64; }
+// Pair slots per world: 16 per body (a resting cubic packing of equal boxes has 13 AABB pairs per
+// body: 26 neighbours across faces, edges and corners).  A world that finds more has its extra
+// pairs dropped and raises WOVF_PAIRS, reported as PE_ERR_CAPACITY.
+int body_pair_cap(int n) { return 16 * n + 64; }
 
 // Allows every instantiation the full LDS of a CU (worlds up to 4096 bodies need ~64 KiB).
 template <int B> void set_lds_one() {
@@ -826,11 +846,12 @@ void schedule_classes(std::vector<Class>& classes, int nstreams) {
 }
 
 // K2 of one class over its list segment (box and sphere pairs; the shape pairs it lists run in
-// launch_shapes); the grid scales with the segment's pair capacity (about six slots per listed
-// pair: about a pair and a half per thread) and the kernel strides over what was listed.
+// launch_shapes); the grid scales with the segment's pair capacity (16 slots per body; about
+// twelve slots per listed pair: about a pair and a half per thread) and the kernel strides over
+// what was listed.
 void launch_narrow(const BatchDev& d, const Class& c, int ci, hipStream_t t) {
     const NarrowSeg g{c.nl_off, c.nl_cap, ci};
-    hipLaunchKernelGGL(k_narrow_list, dim3((unsigned)std::max(1, std::min(4096, c.nl_cap / 1024))), dim3(128), 0, t, d, g);
+    hipLaunchKernelGGL(k_narrow_list, dim3((unsigned)std::max(1, std::min(4096, c.nl_cap / 2048))), dim3(128), 0, t, d, g);
 }
 // K2b and K2c over the shape pairs every class listed.
 void launch_shapes(const BatchDev& d, const std::vector<Class>& classes, hipStream_t t) {
@@ -1111,6 +1132,7 @@ int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world
     rc |= B->alloc(&d.nl_limit, poff);
     rc |= B->alloc(&d.nl_shape, poff);
     rc |= B->alloc(&d.nl_count, 3 * MAX_CLASSES);
+    rc |= B->alloc(&d.ovf_any, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= B->alloc(&ddesc, nworlds);
@@ -1223,10 +1245,30 @@ int pe_batch_step(pe_batch* B, int32_t nsteps) {
     return PE_OK;
 }
 
+// Drains the batch stream, then reports the capacity errors raised since the last report
+// (PE_ERR_CAPACITY naming the worlds; their flags stay set for pe_batch_get_counts).
 int pe_batch_sync(pe_batch* B) {
     if (!B) { pe_set_error("null batch"); return PE_ERR_INVALID; }
     PE_HIP(hipStreamSynchronize(B->stream));
-    return PE_OK;
+    int32_t any = 0;
+    PE_HIP(hipMemcpy(&any, B->d.ovf_any, sizeof(any), hipMemcpyDeviceToHost));
+    if (any == 0) return PE_OK;
+    std::vector<WorldState> st(B->nworlds);
+    PE_HIP(hipMemcpy(st.data(), B->d.st, st.size() * sizeof(WorldState), hipMemcpyDeviceToHost));
+    PE_HIP(hipMemsetAsync(B->d.ovf_any, 0, sizeof(int32_t), B->stream));
+    PE_HIP(hipStreamSynchronize(B->stream));
+    std::string m = "capacity exceeded in worlds";
+    int listed = 0, total = 0;
+    for (int w = 0; w < B->nworlds; w++) {
+        if (!st[w].overflow) continue;
+        total++;
+        if (listed++ < 8)
+            m += " " + std::to_string(w) + ((st[w].overflow & WOVF_PAIRS) ? " (pairs beyond " + std::to_string(B->hdesc[w].pair_cap) + ")" : "") +
+                 ((st[w].overflow & WOVF_CLIP) ? " (clip buffer)" : "");
+    }
+    if (total > listed) m += " and " + std::to_string(total - listed) + " more";
+    pe_set_error(m);
+    return PE_ERR_CAPACITY;
 }
 
 int pe_batch_get_world_state(pe_batch* B, int32_t w, float* pos, float* rot, float* v, float* av) {
@@ -1377,8 +1419,10 @@ __global__ void k_admit(BodyView b, const float* __restrict__ stage, int off, in
         refresh_body(b, off + i);
     }
 }
-// Packs pos/rot/v/w of bodies [off, off+n) into out[NRESULT][n].
-__global__ void k_gather(BodyView b, float* __restrict__ out, int off, int n) {
+// Packs pos/rot/v/w of bodies [off, off+n) into out[NRESULT][n], then the world's capacity-error
+// bits (int) into out[NRESULT * n].
+__global__ void k_gather(BodyView b, const WorldState* __restrict__ st, int slot, float* __restrict__ out, int off, int n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[(size_t)NRESULT * n] = __int_as_float(st[slot].overflow);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         for (int k = 0; k < 3; k++) out[(size_t)k * n + i] = b.at(F_POS + k, off + i);
         for (int k = 0; k < 9; k++) out[(size_t)(3 + k) * n + i] = b.at(F_ROT + k, off + i);
@@ -1431,6 +1475,8 @@ struct Req {
     int64_t mrec_off = -1, marena_off = -1;   // their records / arena range while live
     int64_t footprint = 0;
     bool cancelled = false;
+    bool prepared = false;       // its upload is in flight (admitted at the next tick)
+    std::chrono::steady_clock::time_point t_submit, t_admit;
 };
 
 int64_t req_footprint(int n) {
@@ -1458,7 +1504,11 @@ struct pe_server {
     std::vector<Req*> live;           // by admission order
     std::vector<Req*> prepared;       // uploads in flight, admitted at the next tick
     std::map<int64_t, Req*> all;
-    std::map<int64_t, std::vector<float>> results;   // id -> [NRESULT][n]
+    std::map<int64_t, std::vector<float>> results;   // id -> [NRESULT][n], then the capacity-error bits
+    std::vector<double> wait_ms;      // per admitted request: submission -> admission
+    std::vector<double> latency_ms;   // per finished request: submission -> result available
+    double queue_sum = 0;             // queued requests summed over ticks (mean queue depth)
+    int64_t queue_ticks = 0, queue_max = 0, cancelled_n = 0, capacity_errors = 0;
     std::vector<int64_t> finished_ids;
     int64_t next_id = 1;
     float* dstage[2] = {nullptr, nullptr};
@@ -1543,7 +1593,6 @@ struct pe_server {
         int64_t used = 0;
         while (!queue.empty()) {
             Req* r = queue.front();
-            if (r->cancelled) { queue.pop_front(); continue; }
             const int64_t fp = r->footprint;
             const int64_t need = (int64_t)r->n * NSTAGE;
             if (free_slots.empty() || live_footprint + fp > cfg.hbm_budget_bytes || used + need > stage_cap) break;
@@ -1592,6 +1641,7 @@ struct pe_server {
             r->staged.clear();
             r->staged.shrink_to_fit();
             r->stage_off = used;
+            r->prepared = true;
             prepared.push_back(r);
             used += need;
         }
@@ -1603,7 +1653,16 @@ struct pe_server {
         if (prepared.empty()) return PE_OK;
         const int buf = stage_buf;
         PE_HIP(hipStreamWaitEvent(cs, ev_upload[buf], 0));
+        const auto now = std::chrono::steady_clock::now();
         for (Req* r : prepared) {
+            r->prepared = false;
+            if (r->cancelled) {   // cancelled while its upload was in flight: never stepped
+                release(r);
+                forget(r);
+                continue;
+            }
+            r->t_admit = now;
+            wait_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
             const int64_t soff = r->stage_off;
             // the pair range holds pcap + 1 slots: its start doubles as the colour-start region
             WorldDesc wd{(int32_t)r->boff, r->n, (int32_t)r->poff, r->pcap, (int32_t)r->poff, {0, 0, 0}};
@@ -1620,6 +1679,12 @@ struct pe_server {
         return PE_OK;
     }
 
+    // Drops a request that produced no result (cancelled) from the id table.
+    void forget(Req* r) {
+        all.erase(r->id);
+        delete r;
+        cancelled_n++;
+    }
     void release(Req* r) {
         bodies.release(r->boff, r->n);
         pairs.release(r->poff, r->pcap + 1);
@@ -1638,8 +1703,15 @@ struct pe_server {
         const auto t0 = std::chrono::steady_clock::now();
         // drop cancelled live worlds
         for (size_t i = 0; i < live.size();) {
-            if (live[i]->cancelled) { release(live[i]); live.erase(live.begin() + i); dirty = true; }
-            else i++;
+            if (live[i]->cancelled) {
+                Req* r = live[i];
+                release(r);
+                live.erase(live.begin() + i);
+                forget(r);
+                dirty = true;
+            } else {
+                i++;
+            }
         }
         if (int rc = admit_prepared()) return rc;
         if (dirty) if (int rc = rebuild_lists()) return rc;
@@ -1653,31 +1725,43 @@ struct pe_server {
         if (int rc = prepare_admissions()) return rc;
         PE_HIP(hipEventSynchronize(ev_done));
         body_steps += (double)nb;
-        // retire finished worlds: one gather per world into the result buffer, one copy back
+        // retire every world that reached its horizon: gathers into the result buffer and one copy
+        // back per round, as many rounds as the buffer needs (a world is never stepped past its
+        // horizon)
         std::vector<Req*> fin;
         for (Req* r : live) if (++r->done >= r->horizon) fin.push_back(r);
-        if (!fin.empty()) {
+        const auto t_fin = std::chrono::steady_clock::now();
+        for (size_t k0 = 0; k0 < fin.size();) {
             int64_t off = 0;
             std::vector<int64_t> offs;
-            for (Req* r : fin) {
-                const int64_t need = (int64_t)r->n * NRESULT;
-                if (off + need > result_cap) break;
-                hipLaunchKernelGGL(k_gather, dim3((r->n + 255) / 256), dim3(256), 0, cs, d.b, dresult + off, (int)r->boff, r->n);
+            size_t k1 = k0;
+            for (; k1 < fin.size(); k1++) {
+                Req* r = fin[k1];
+                const int64_t need = (int64_t)r->n * NRESULT + 1;
+                if (off + need > result_cap) break;   // result_cap holds at least one largest world
+                hipLaunchKernelGGL(k_gather, dim3((r->n + 255) / 256), dim3(256), 0, cs, d.b, d.st, r->slot, dresult + off,
+                                   (int)r->boff, r->n);
                 offs.push_back(off);
                 off += need;
             }
             std::vector<float> host(off);
             PE_HIP(hipMemcpyAsync(host.data(), dresult, off * 4, hipMemcpyDeviceToHost, cs));
             PE_HIP(hipStreamSynchronize(cs));
-            for (size_t k = 0; k < offs.size(); k++) {
+            for (size_t k = k0; k < k1; k++) {
                 Req* r = fin[k];
-                results[r->id] = std::vector<float>(host.begin() + offs[k], host.begin() + offs[k] + (int64_t)r->n * NRESULT);
+                const int64_t o = offs[k - k0], len = (int64_t)r->n * NRESULT + 1;
+                results[r->id] = std::vector<float>(host.begin() + o, host.begin() + o + len);
                 finished_ids.push_back(r->id);
+                latency_ms.push_back(std::chrono::duration<double, std::milli>(t_fin - r->t_submit).count());
                 release(r);
                 live.erase(std::find(live.begin(), live.end(), r));
                 dirty = true;
             }
+            k0 = k1;
         }
+        queue_sum += (double)queue.size();
+        queue_ticks++;
+        queue_max = std::max<int64_t>(queue_max, (int64_t)queue.size());
         const auto t1 = std::chrono::steady_clock::now();
         if (nb > 0) tick_ms.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
         if (lw) *lw = (int32_t)live.size();
@@ -1721,7 +1805,7 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     // arena sizes from the budget: per body its store and 8 pair slots of contacts and rows
     const int64_t per_body = req_footprint(1024) / 1024 + 1;
     S->stage_cap = (int64_t)cfg->max_world_bodies * NSTAGE * 8;            // floats per staging buffer
-    S->result_cap = (int64_t)cfg->max_world_bodies * NRESULT * 64;        // floats
+    S->result_cap = ((int64_t)cfg->max_world_bodies * NRESULT + 1) * 64;  // floats: 64 largest worlds
     // a sixteenth of the budget (at most 8 GiB) holds the requests' meshes: records for one mesh per KiB
     S->mesh_arena_cap = std::min<int64_t>(cfg->hbm_budget_bytes / 16, (int64_t)8 << 30) / (int64_t)sizeof(float4);
     S->mesh_arena_cap = std::min<int64_t>(std::max<int64_t>(S->mesh_arena_cap, 4096), ((int64_t)1 << 31) - 1);
@@ -1761,6 +1845,7 @@ int pe_server_create(const pe_server_config* cfg, pe_server** out) {
     rc |= S->alloc(&d.nl_limit, P);
     rc |= S->alloc(&d.nl_shape, P);
     rc |= S->alloc(&d.nl_count, 3 * MAX_CLASSES);
+    rc |= S->alloc(&d.ovf_any, 1);
     WorldDesc* ddesc = nullptr;
     WorldState* dst = nullptr;
     rc |= S->alloc(&ddesc, cfg->max_worlds);
@@ -1821,6 +1906,7 @@ int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, 
         return PE_ERR_BUSY;
     }
     r->id = S->next_id++;
+    r->t_submit = std::chrono::steady_clock::now();
     for (int i = 0; i < scene->n; i++) S->has_shapes |= scene->shape_type[i] > PE_SHAPE_SPHERE;
     std::copy(hf.begin(), hf.end(), r->staged.begin());
     std::memcpy(r->staged.data() + hf.size(), hi.data(), hi.size() * 4);
@@ -1853,16 +1939,27 @@ int pe_server_result(pe_server* S, int64_t id, float* pos, float* rot, float* v,
     auto it = S->results.find(id);
     if (it == S->results.end()) { pe_set_error("no result for this id"); return PE_ERR_NOT_FOUND; }
     const auto& r = it->second;
-    const int n = (int)(r.size() / NRESULT);
-    for (int i = 0; i < n; i++) {
-        for (int k = 0; k < 3; k++) if (pos) pos[3 * i + k] = r[(size_t)k * n + i];
-        for (int k = 0; k < 9; k++) if (rot) rot[9 * i + k] = r[(size_t)(3 + k) * n + i];
-        for (int k = 0; k < 3; k++) if (v) v[3 * i + k] = r[(size_t)(12 + k) * n + i];
-        for (int k = 0; k < 3; k++) if (w) w[3 * i + k] = r[(size_t)(15 + k) * n + i];
+    const int n = (int)((r.size() - 1) / NRESULT);
+    int32_t ovf;
+    std::memcpy(&ovf, &r[(size_t)NRESULT * n], 4);
+    if (ovf == 0) {
+        for (int i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++) if (pos) pos[3 * i + k] = r[(size_t)k * n + i];
+            for (int k = 0; k < 9; k++) if (rot) rot[9 * i + k] = r[(size_t)(3 + k) * n + i];
+            for (int k = 0; k < 3; k++) if (v) v[3 * i + k] = r[(size_t)(12 + k) * n + i];
+            for (int k = 0; k < 3; k++) if (w) w[3 * i + k] = r[(size_t)(15 + k) * n + i];
+        }
     }
     S->results.erase(it);
     auto a = S->all.find(id);
     if (a != S->all.end()) { delete a->second; S->all.erase(a); }
+    if (ovf != 0) {
+        S->capacity_errors++;
+        pe_set_error(std::string("request ") + std::to_string(id) + " exceeded a capacity while stepping:" +
+                     ((ovf & WOVF_PAIRS) ? " pairs beyond its pair capacity (16 per body + 64)" : "") +
+                     ((ovf & WOVF_CLIP) ? " a clipped contact polygon outgrew its buffer" : "") + "; no result");
+        return PE_ERR_CAPACITY;
+    }
     return PE_OK;
 }
 
@@ -1870,7 +1967,15 @@ int pe_server_cancel(pe_server* S, int64_t id) {
     if (!S) { pe_set_error("null server"); return PE_ERR_INVALID; }
     auto it = S->all.find(id);
     if (it == S->all.end()) { pe_set_error("unknown request"); return PE_ERR_NOT_FOUND; }
-    it->second->cancelled = true;
+    Req* r = it->second;
+    if (S->results.count(id)) { pe_set_error("request already finished"); return PE_ERR_INVALID; }
+    auto q = std::find(S->queue.begin(), S->queue.end(), r);
+    if (q != S->queue.end()) {   // still queued: it holds nothing on the device
+        S->queue.erase(q);
+        S->forget(r);
+        return PE_OK;
+    }
+    r->cancelled = true;   // live or uploading: released at the next tick
     return PE_OK;
 }
 
@@ -1887,6 +1992,25 @@ int pe_server_stats(pe_server* S, double* out) {
     out[5] = (double)S->live.size();
     out[6] = (double)S->queue.size();
     out[7] = (double)S->admitted;
+    return PE_OK;
+}
+
+int pe_server_request_stats(pe_server* S, double* out) {
+    if (!S || !out) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    auto pct = [](std::vector<double> t, double q) {
+        if (t.empty()) return 0.0;
+        std::sort(t.begin(), t.end());
+        return t[std::min(t.size() - 1, (size_t)std::floor(q * (t.size() - 1) + 0.5))];
+    };
+    out[0] = pct(S->wait_ms, 0.50);
+    out[1] = pct(S->wait_ms, 0.99);
+    out[2] = pct(S->latency_ms, 0.50);
+    out[3] = pct(S->latency_ms, 0.99);
+    out[4] = S->queue_ticks ? S->queue_sum / (double)S->queue_ticks : 0.0;
+    out[5] = (double)S->queue_max;
+    out[6] = (double)S->latency_ms.size();
+    out[7] = (double)S->cancelled_n;
+    out[8] = (double)S->capacity_errors;
     return PE_OK;
 }
 

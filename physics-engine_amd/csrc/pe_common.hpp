@@ -65,6 +65,12 @@ struct MeshSet {
 //   (face count, 0 = inner node; first face) per node.
 enum MeshRec : int { MR_V = 0, MR_NV, MR_FN, MR_NF, MR_FACE, MR_FI, MR_E, MR_NE, MR_NODE, MR_NB, MR_LEAF, MR_PAD, MRW };
 constexpr int CYL_UNIT_FLOATS = 80 + 120;
+// Limits of the polyhedral narrowphase, checked when a mesh is prepared: a face has at most
+// MAX_FACE_VERTS vertices (an incident face clipped by a reference face's side planes then has at
+// most 2 * MAX_FACE_VERTS, the clip buffer), and the BVH walk's stack holds BVH_STACK entries (the
+// median-split tree of F faces is about log2(F) deep; the walk needs at most depth + 2 entries).
+constexpr int MAX_FACE_VERTS = 32;
+constexpr int BVH_STACK = 48;
 
 struct BodyView {
     float* f;          // [NF][stride] field-major state (coalesced per-body kernels)

@@ -182,7 +182,9 @@ struct Cache {
     float ax[8], ay[8], az[8];      // local_pos_a
     float bx[8], by[8], bz[8];      // local_pos_b
     float d[8];                     // distance (kept on invalid slots, as in the reference)
+    int trunc;                      // set when a clipped polygon outgrew its buffer (points lost)
 };
+static_assert((sizeof(Cache) / 4) % 2 == 1, "an odd row length: lanes of an LDS cache array hit distinct banks");
 
 // Body pose needed by the narrowphase
 struct Pose {

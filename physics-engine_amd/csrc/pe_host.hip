@@ -142,6 +142,10 @@ int HostMeshes::add(const pe_mesh_desc& md, bool concave, float* relocation, flo
             pe_set_error(concave ? "concave mesh: every face must be a triangle" : "mesh: a face has fewer than 3 vertices");
             return -1;
         }
+        if (b - a > MAX_FACE_VERTS) {
+            pe_set_error("mesh: a face has more than " + std::to_string(MAX_FACE_VERTS) + " vertices (the clipper's limit)");
+            return -1;
+        }
         for (int k = a; k < b; k++) {
             const int q = md.face_index[k];
             if (q < 0 || q >= md.nv) { pe_set_error("mesh: face index out of range"); return -1; }
@@ -214,6 +218,18 @@ int HostMeshes::add(const pe_mesh_desc& md, bool concave, float* relocation, flo
     }
     std::vector<BvhNode> nodes;
     build_bvh(faces, v, 0, md.nf - 1, nmax, nodes);
+    {
+        // the device walk pushes both children of an inner node and holds at most one pending
+        // sibling per level above it, so a tree of depth D needs D + 2 stack entries
+        std::vector<int> depth(nodes.size(), 0);
+        int dmax = 0;
+        for (size_t k = 0; k < nodes.size(); k++) {
+            dmax = std::max(dmax, depth[k]);
+            if (nodes[k].left >= 0) depth[nodes[k].left] = depth[k] + 1;
+            if (nodes[k].right >= 0) depth[nodes[k].right] = depth[k] + 1;
+        }
+        if (dmax + 2 > BVH_STACK) { pe_set_error("mesh: BVH deeper than the device walk's stack"); return -1; }
+    }
     // unique edges (convex_mesh_shape.cpp:110-128), convex meshes only
     std::vector<std::pair<V3f, V3f>> edges;
     if (!concave) {

@@ -281,10 +281,9 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
     // in the thread-per-pair kernels (at most 64 threads per workgroup), one per wave when a wave
     // works on one pair (at most 4 waves per workgroup; every lane writes the same values).
     constexpr bool LDS_CACHE = SHAPES;
-    struct CachePad { Cache c; float pad; };   // an odd row length: lanes hit distinct banks
-    __shared__ CachePad s_cache[LDS_CACHE ? (G == 1 ? 64 : 4) : 1];
+    __shared__ Cache s_cache[LDS_CACHE ? (G == 1 ? 64 : 4) : 1];   // rows of odd length: distinct banks
     Cache c_reg;
-    Cache& c = LDS_CACHE ? s_cache[G == 1 ? threadIdx.x % 64 : (threadIdx.x / WAVE) % 4].c : c_reg;
+    Cache& c = LDS_CACHE ? s_cache[G == 1 ? threadIdx.x % 64 : (threadIdx.x / WAVE) % 4] : c_reg;
     const size_t slot0 = (size_t)p * 8;
     float dprev[8];
     if (p < prev_np) ld_rec<2>(ps.cd + slot0, dprev);
@@ -296,6 +295,7 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
         c.bx[k] = 0; c.by[k] = 0; c.bz[k] = 0;
         c.d[k] = p < prev_np ? dprev[k] : REAL_MAX;
     }
+    c.trunc = 0;
     const CollBody A = load_coll(b, ia), B = load_coll(b, ib);
     PairCtx x;
     x.A = A.P;
@@ -328,7 +328,8 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
     const int np = sort_rank(c, rank);
     if (G > 1 && Grp<G>::rank() != 0) return;
     ps.npts[p] = np;
-    ps.swap[p] = x.swap ? 1 : 0;
+    // bit 0: the swap flag; bit 1: contact points were lost to a full clip buffer (a capacity error)
+    ps.swap[p] = (x.swap ? 1 : 0) | (c.trunc ? 2 : 0);
     float dsorted[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {

@@ -221,23 +221,31 @@ PE_HD bool inside(const Poly& S, f3 q) {
 
 // ------------------------------------------------------------------ convex SAT + clipping
 // convex_convex_collision_algorithm.cpp
-constexpr int CLIP_CAP = 64;   // clipped polygon capacity (a 20-gon cap clipped by 20 planes needs 40)
+constexpr int CLIP_CAP = 2 * MAX_FACE_VERTS;   // clipped polygon capacity (a 20-gon cap clipped by 20 planes needs 40)
 
-// clipFace (:29-61)
-PE_HD int clip_face(const f3* in, int nin, f3* out, f3 pn, float pe) {
+// clipFace (:29-61).  Clipping a convex polygon by a plane adds at most one vertex, and every
+// mesh face has at most MAX_FACE_VERTS vertices (checked when a mesh is prepared), so an incident
+// face clipped by a reference face's side planes stays within CLIP_CAP; a polygon that rounding
+// made non-convex could still grow beyond it, and then trunc is set (reported as a capacity error)
+// instead of points being dropped silently.
+PE_HD int clip_face(const f3* in, int nin, f3* out, f3 pn, float pe, int& trunc) {
     if (nin < 2) return 0;
     int no = 0;
     f3 first = in[nin - 1];
     float ds = dot(pn, first) + pe;
+    auto put = [&](f3 q) {
+        if (no < CLIP_CAP) out[no++] = q;
+        else trunc = 1;
+    };
     for (int ve = 0; ve < nin; ve++) {
         const f3 end = in[ve];
         const float de = dot(pn, end) + pe;
         if (ds < 0) {
-            if (de < 0) { if (no < CLIP_CAP) out[no++] = end; }
-            else if (no < CLIP_CAP) out[no++] = add(first, mul(sub(end, first), ds * 1.0f / (ds - de)));
+            if (de < 0) put(end);
+            else put(add(first, mul(sub(end, first), ds * 1.0f / (ds - de))));
         } else if (de < 0) {
-            if (no < CLIP_CAP) out[no++] = add(first, mul(sub(end, first), ds * 1.0f / (ds - de)));
-            if (no < CLIP_CAP) out[no++] = end;
+            put(add(first, mul(sub(end, first), ds * 1.0f / (ds - de))));
+            put(end);
         }
         first = end;
         ds = de;
@@ -315,7 +323,7 @@ __device__ __noinline__ void clip_hull_hull(Cache& c, const PairCtx& x, f3 sep, 
         const f3 pn = neg(cross(ew, wn));
         const f3 wa = xf_apply(ta, a);
         const float pe = -dot(wa, pn);
-        n = clip_face(pin, n, pout, pn, pe);
+        n = clip_face(pin, n, pout, pn, pe, c.trunc);
         f3* t = pin; pin = pout; pout = t;
     }
     const f3 pv = face_vertex(A, ca, 0);
@@ -669,7 +677,6 @@ __device__ __noinline__ void box_cylinder(Cache& c, const PairCtx& x, const Pose
 // kernels of at most BVH_ROWS threads per workgroup), one row per wave when the wave's lanes walk
 // one pair together (G = WAVE: every lane writes the same value to the same word; kernels of at
 // most BVH_ROWS waves per workgroup).
-constexpr int BVH_STACK = 48;
 constexpr int BVH_ROWS = 64;
 template <int G>
 __device__ __forceinline__ int* bvh_stack_row() {

@@ -36,12 +36,18 @@ constexpr int CHUNK = 32;            // candidates per thread in the sweep
 static_assert(CHUNK <= 32, "a chunk's test outcomes are one 32-bit mask");
 constexpr int SOLVE_THREADS = 1024;
 constexpr int CLAIM_LDS = 12288;     // bodies whose colouring claims fit in LDS (48 KiB)
+// Capacity errors, OR-ed into DevCounts::overflow by the step that hits them and kept until the
+// host reads and reports them (pe_world_sync, pe_world_broadphase): a result that lost pairs or
+// contact points is never returned as a success.
+constexpr int OVF_PAIRS = 1;         // the broadphase found more pairs than the pair capacity
+constexpr int OVF_CHUNKS = 4;        // the sweep had more candidates than its chunk capacity
+constexpr int OVF_CLIP = 8;          // a clipped polygon outgrew its buffer (contact points lost)
 
 struct DevCounts {          // device-side counters of one step
     int32_t npairs;         // pairs emitted by the broadphase (clamped to capacity)
     int32_t prev_npairs;    // pairs of the previous narrowphase (contact slots that survive)
     int32_t axis;           // SAP axis for the next sort
-    int32_t overflow;       // set when pair capacity was exceeded
+    int32_t overflow;       // sticky capacity-error bits (OVF_*), cleared when the host reports them
     uint32_t ncand;         // total sweep candidates
     uint32_t nchunk_pairs;  // pairs counted by the sweep (unclamped)
     int32_t ncolors;
@@ -196,12 +202,12 @@ __global__ void k_bp_chunks(Dev d) {
 __global__ void k_bp_nchunks(Dev d, uint32_t* nchunks) {
     const uint32_t want = (d.cnt->ncand + CHUNK - 1) / CHUNK;
     *nchunks = min(want, (uint32_t)d.chunk_cap);
-    d.cnt->overflow = want > (uint32_t)d.chunk_cap ? 4 : 0;
+    if (want > (uint32_t)d.chunk_cap) d.cnt->overflow |= OVF_CHUNKS;
 }
 
 __global__ void k_bp_finish(Dev d) {
     const uint32_t total = d.cnt->nchunk_pairs;
-    d.cnt->overflow |= total > (uint32_t)d.pcap ? 1 : 0;
+    if (total > (uint32_t)d.pcap) d.cnt->overflow |= OVF_PAIRS;
     d.cnt->npairs = total > (uint32_t)d.pcap ? d.pcap : (int32_t)total;
 }
 
@@ -356,15 +362,19 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
 }
 
 // Contact points of this narrowphase: per-workgroup sums added to the counter k_narrow_finish
-// cleared.
+// cleared; a pair whose clipper lost points (bit 1 of its swap word) raises OVF_CLIP.
 __global__ void __launch_bounds__(256) k_count_points(Dev d) {
     __shared__ int wsum[256 / WAVE + 1];
     const int np = d.cnt->npairs;
-    int acc = 0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) acc += d.ps.npts[p];
+    int acc = 0, lost = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        acc += d.ps.npts[p];
+        lost |= d.ps.swap[p] & 2;
+    }
     int tot;
     block_exclusive_scan(acc, wsum, &tot);
     if (threadIdx.x == 0 && tot) atomicAdd(&d.cnt->npoints, tot);
+    if (lost) atomicOr(&d.cnt->overflow, OVF_CLIP);
 }
 
 }  // namespace
@@ -491,6 +501,19 @@ namespace {
 int check_world(pe_world* w) {
     if (!w) { pe_set_error("null world"); return PE_ERR_INVALID; }
     return PE_OK;
+}
+// Reports the capacity errors the steps since the last report raised (PE_ERR_CAPACITY, with the
+// causes in the message) and clears them; c holds counters read after the stream drained.
+int report_overflow(pe_world* w, const DevCounts& c) {
+    if (c.overflow == 0) return PE_OK;
+    std::string m = "capacity exceeded:";
+    if (c.overflow & OVF_PAIRS) m += " more broadphase pairs than max_pairs (" + std::to_string(w->d.pcap) + ")";
+    if (c.overflow & OVF_CHUNKS) m += " more sweep candidates than the sweep's chunk capacity";
+    if (c.overflow & OVF_CLIP) m += " a clipped contact polygon outgrew its buffer";
+    PE_HIP(hipMemsetAsync(&w->d.cnt->overflow, 0, sizeof(int32_t), w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    pe_set_error(m);
+    return PE_ERR_CAPACITY;
 }
 }  // namespace
 
@@ -621,7 +644,7 @@ int pe_world_broadphase(pe_world* w, int32_t* npairs) {
         PE_HIP(hipMemcpyAsync(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
         PE_HIP(hipStreamSynchronize(w->stream));
         *npairs = c.npairs;
-        if (c.overflow & 1) { pe_set_error("pair capacity exceeded"); return PE_ERR_CAPACITY; }
+        if (int rc = report_overflow(w, c)) return rc;
     }
     return PE_OK;
 }
@@ -642,8 +665,7 @@ int pe_world_sync(pe_world* w) {
     PE_HIP(hipStreamSynchronize(w->stream));
     DevCounts c;
     PE_HIP(hipMemcpy(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost));
-    if (c.overflow & 1) { pe_set_error("pair capacity exceeded"); return PE_ERR_CAPACITY; }
-    return PE_OK;
+    return report_overflow(w, c);
 }
 
 static int get_field(pe_world* w, int f0, int nf, float* out) {

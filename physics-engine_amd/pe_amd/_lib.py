@@ -28,7 +28,7 @@ EXPORTS = [
     "pe_batch_world_footprint", "pe_batch_step_timed", "pe_batch_set_solver_groups", "pe_batch_get_solver_groups", "pe_batch_phase_profile", "pe_batch_stage_times",
     "pe_batch_class_times", "pe_batch_snapshot", "pe_batch_restore",
     "pe_server_create", "pe_server_destroy", "pe_server_footprint", "pe_server_submit", "pe_server_tick",
-    "pe_server_poll", "pe_server_result", "pe_server_stats", "pe_server_cancel",
+    "pe_server_poll", "pe_server_result", "pe_server_stats", "pe_server_cancel", "pe_server_request_stats",
 ]
 
 

@@ -103,7 +103,10 @@ def load_scene(path: str, dt: float = 1.0 / 60.0, iterations: int = 10) -> Scene
             raise ValueError(f"body {k}: fracturable bodies are not part of the device step")
         if rb.get("ignore_collision", False):
             raise ValueError(f"body {k}: ignore_collision is not part of the device step")
-        if float(rb.get("lifetime", 1e100)) < 1e99:
+        life = float(rb.get("lifetime", 1e100))
+        # saveScene writes PE_REAL_MAX for an unlimited lifetime: 1e30 in the FP32 build, 1e100 in
+        # the FP64 one (phys_general.h:36-42); loadScene takes a negative value as unlimited too
+        if 0.0 <= life < 1e30:
             raise ValueError(f"body {k}: a finite lifetime is not part of the device step")
         typ = rb.get("type")
         if typ not in _TYPES:
@@ -126,7 +129,7 @@ def load_scene(path: str, dt: float = 1.0 / 60.0, iterations: int = 10) -> Scene
         if shape == SHAPE_BOX:
             param = tuple(scale[:3]) if scale and len(scale) >= 3 else (1.0, 1.0, 1.0)
         elif shape == SHAPE_SPHERE:
-            param = (scale[0],) if scale else (0.5,)
+            param = (scale[0],) if scale else (1.0,)   # loadScene's sphere default: SphereShape(1)
         elif shape == SHAPE_CYLINDER:
             param = tuple(scale[:2]) if scale and len(scale) >= 2 else (0.5, 1.0)
         else:

@@ -78,6 +78,15 @@ class Server:
     def cancel(self, rid: int):
         check(lib().pe_server_cancel(self.h, ctypes.c_int64(rid)))
 
+    def request_stats(self):
+        """Per-request figures: admission wait and submission-to-result latency (p50 / p99, ms), queue
+        depth over ticks (mean / max), requests finished, cancelled, refused for capacity."""
+        out = (ctypes.c_double * 9)()
+        check(lib().pe_server_request_stats(self.h, out))
+        keys = ("wait_p50_ms", "wait_p99_ms", "latency_p50_ms", "latency_p99_ms", "queue_mean", "queue_max",
+                "finished", "cancelled", "capacity_errors")
+        return dict(zip(keys, list(out)))
+
     def stats(self):
         out = (ctypes.c_double * 8)()
         check(lib().pe_server_stats(self.h, out))

@@ -100,3 +100,26 @@ def reference_sample_scene(path):
     sc.linear_damping = np.asarray(lin, float)
     sc.angular_damping = np.asarray(ang, float)
     return sc.contiguous()
+
+
+def rotation_angle(ra, rb):
+    """Angle in radians of the rotation between two stacks of rotation matrices [n, 3, 3]: for
+    rotations A, B, ||A - B||_F = 2 sqrt(2) sin(theta / 2) (theta the angle of A^T B), which stays
+    well conditioned at small angles where arccos of the trace would not."""
+    a = np.asarray(ra, np.float64).reshape(-1, 3, 3)
+    b = np.asarray(rb, np.float64).reshape(-1, 3, 3)
+    f = np.linalg.norm((a - b).reshape(len(a), 9), axis=1)
+    return 2.0 * np.arcsin(np.clip(f / (2.0 * np.sqrt(2.0)), 0.0, 1.0))
+
+
+def impulse_error(gi, oi, on):
+    """Largest deviation of the device's accumulated impulses (normal, tangent 0, tangent 1 of
+    every valid contact row) from the oracle's, relative to max(|oracle impulse|, 1e-6 x the
+    step's largest impulse): returns (max relative error, rows compared)."""
+    vm = np.arange(8)[None, :] < np.asarray(on)[:, None]
+    a = np.asarray(gi, np.float64)[vm]
+    b = np.asarray(oi, np.float64)[vm]
+    if a.size == 0:
+        return 0.0, 0
+    floor = 1e-6 * max(float(np.abs(b).max()), 1e-30)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))), int(vm.sum())

@@ -6,12 +6,14 @@ Tolerances (written here, applied below):
   integrate / AABB / pair list / colour classes ........... bit-exact
   contact points (normals, depths, positions) ............. 1e-5 relative (abs floor 1e-6)
   velocities after one solve .............................. 1e-5 relative (abs floor 1e-6)
-  60-step trajectories .................................... position RMS <= 1e-3 m, max <= 1e-2 m
+  accumulated impulses after one solve .................... 1e-5 relative (floor 1e-6 x the step's largest)
+  60-step trajectories .................................... position RMS <= 1e-3 m, max <= 1e-2 m,
+                                                            orientation max <= 1e-3 rad
 """
 import numpy as np
 import pytest
 
-from helpers import brute_force_pairs, flat_state, pair_set
+from helpers import brute_force_pairs, flat_state, impulse_error, pair_set, rotation_angle
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +40,7 @@ def test_shape_stages_match_oracle():
     g, o = pe_amd.World(sc), OracleWorld(sc, 32)
     T = sc.shape_type
     seen = set()
+    worst = 0.0
     print(f"[shapes] n={sc.n} meshes={len(sc.meshes)}", flush=True)
     for step in range(140):
         g.integrate(); o.integrate()
@@ -72,8 +75,11 @@ def test_shape_stages_match_oracle():
         _, _, gv, gw = g.state()
         _, _, ov, ow = o.state()
         assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        err, rows = impulse_error(g.impulses(), o.impulses(), on)
+        worst = max(worst, err)
+        assert err <= 1e-5, f"impulses step {step}: max relative error {err:.3e} over {rows} rows"
         o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
-    print(f"[shapes] contact pair types seen: {sorted(seen)}", flush=True)
+    print(f"[shapes] contact pair types seen: {sorted(seen)}; largest relative impulse error {worst:.3e}", flush=True)
     # the heightfield against every other shape, and the polyhedral pairs among bodies
     for t in (0, 1, 2, 3):
         assert (t, 4) in seen
@@ -92,12 +98,14 @@ def test_shape_trajectory_60_steps():
     g.step(110)
     o.step(110)
     g.sync()
-    pg = g.state()[0]
-    po = o.state()[0]
+    pg, rg = g.state()[:2]
+    po, ro = o.state()[:2]
     d = np.linalg.norm(pg - po, axis=1)
     rms = float(np.sqrt(np.mean(d ** 2)))
-    print(f"[shapes traj] position rms {rms:.3e} max {d.max():.3e}", flush=True)
+    ang = rotation_angle(rg, ro)
+    print(f"[shapes traj] position rms {rms:.3e} max {d.max():.3e}, orientation max {ang.max():.3e} rad", flush=True)
     assert rms <= 1e-3 and d.max() <= 1e-2
+    assert ang.max() <= 1e-3
 
 
 def test_shape_worlds_batched_equal_alone():

@@ -5,18 +5,22 @@ Tolerances (written here, applied below):
   integrate / AABB / pair list / solve order .............. bit-exact
   contact points (normals, depths, positions) ............. 1e-5 relative (abs floor 1e-6)
   velocities after one solve .............................. 1e-5 relative (abs floor 1e-6)
-  60-step trajectories .................................... position RMS <= 1e-3 m, max <= 1e-2 m
+  accumulated impulses after one solve .................... 1e-5 relative (floor 1e-6 x the step's largest)
+  60-step trajectories .................................... position RMS <= 1e-3 m, max <= 1e-2 m,
+                                                            orientation max <= 1e-3 rad
 """
 import numpy as np
 import pytest
 
-from helpers import (brute_force_pairs, flat_state, mixed_small, normal_residual, pair_set, reference_sample_scene,
-                     small_rain, valid_mask)
+from helpers import (brute_force_pairs, flat_state, impulse_error, mixed_small, normal_residual, pair_set,
+                     reference_sample_scene, rotation_angle, small_rain, valid_mask)
 
 pytestmark = pytest.mark.gpu
 
 REL = 1e-5
 ABS = 1e-6
+IMP_REL = 1e-5
+ROT_MAX = 1e-3
 
 
 def _scenes():
@@ -50,6 +54,7 @@ def _close(a, b, rel=REL, abs_=ABS):
 def test_stages_match_oracle(name, all_scenes):
     sc, g, o = _pair(name, all_scenes)
     print(f"[stages] scene={name} n={sc.n}", flush=True)
+    worst = 0.0
     for step in range(8):
         g.integrate(); o.integrate()
         g.sync()
@@ -86,8 +91,13 @@ def test_stages_match_oracle(name, all_scenes):
         _, _, gv, gw = g.state()
         _, _, ov, ow = o.state()
         assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        # solved impulses (FrictionContactConstraint's accumulated impulses), same colour order
+        err, rows = impulse_error(g.impulses(), o.impulses(), on)
+        worst = max(worst, err)
+        assert err <= IMP_REL, f"impulses step {step}: max relative error {err:.3e} over {rows} rows"
         # keep the two in lock-step on the exact same state
         o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+    print(f"[stages] {name}: largest relative impulse error {worst:.3e}", flush=True)
 
 
 @pytest.mark.parametrize("name", ["tower", "rain300"])
@@ -101,8 +111,11 @@ def test_trajectory_60_steps(name, all_scenes):
     po, ro, _, _ = o.state()
     d = np.linalg.norm(pg - po, axis=1)
     rms = float(np.sqrt(np.mean(d ** 2)))
-    print(f"[traj60] {name}: position rms {rms:.3e} max {d.max():.3e}", flush=True)
+    ang = rotation_angle(rg, ro)
+    print(f"[traj60] {name}: position rms {rms:.3e} max {d.max():.3e}, orientation max {ang.max():.3e} rad",
+          flush=True)
     assert rms <= 1e-3 and d.max() <= 1e-2
+    assert ang.max() <= ROT_MAX
 
 
 def test_solve_order_is_a_valid_coloring(all_scenes):
@@ -184,6 +197,8 @@ def test_reference_sample_scene_stages():
         _, _, gv, gw = g.state()
         _, _, ov, ow = o.state()
         assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        err, rows = impulse_error(g.impulses(), o.impulses(), on)
+        assert err <= IMP_REL, f"impulses step {step}: max relative error {err:.3e} over {rows} rows"
         o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
     assert contacts > 0
 
