@@ -167,8 +167,72 @@ int pe_world_get_stage_times(pe_world* w, double* ms /*4*/);
  * colour's pair per thread; clamped to the co-resident limit).  No counterpart in the reference; results
  * do not depend on it. */
 int pe_world_set_solver_blocks(pe_world* w, int32_t blocks);
+/* The per-body records the stage entry points take (layouts under "stage entry points" below),
+ * as the world's last integrate stage wrote them: AABB records [n][8], collision records [n][20],
+ * solver records [n][24] (velocities after gravity, before the solve).  Any pointer may be NULL. */
+int pe_world_get_records(pe_world* w, float* aabb, float* body, float* solver);
 /* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours. */
 int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
+
+/* ------------------------------------------------------------------ stage entry points
+ * The reference's World calls three replaceable stages (World::setBroadPhase / setNarrowPhase /
+ * setConstraintSolver, src/intf/world.h:97-100).  These entry points run one stage each on the
+ * caller's device buffers, enqueued on the caller's stream (a hipStream_t; NULL is the null
+ * stream) without synchronising, so a host that keeps the reference's World can replace a stage
+ * at a time (the C++ classes implementing the stage interfaces over them are in INTEGRATION.md).
+ * All buffers are device memory.  A pe_stage holds the working memory; one host thread uses a
+ * pe_stage at a time.  Record layouts (FP32, 16-byte aligned rows):
+ *   AABB record      [8]  min x y z, kinematic (0 / 1), max x y z, ignore-collision (0 / 1)
+ *   collision record [20] basis (9, row-major), origin (3), AABB scale (RigidBody::getAABBScale),
+ *                         shape parameters (3: box size / sphere radius / cylinder radius, height /
+ *                         mesh id of pe_stage_set_meshes), shape type (PE_SHAPE_*, int bits),
+ *                         kinematic (int bits), friction, restitution
+ *   solver record    [24] temp velocity (3), temp angular velocity (3) (both ignored on input),
+ *                         inverse mass, kinematic (0 / 1), world inverse inertia (9, row-major),
+ *                         linear velocity (3), angular velocity (3), unused
+ *   contact slot     [13] as pe_world_get_contacts */
+typedef void* pe_stream_t;
+#define PE_AABB_FLOATS 8
+#define PE_BODY_FLOATS 20
+#define PE_SOLVER_BODY_FLOATS 24
+
+typedef struct pe_stage pe_stage;
+/* Working memory for worlds of up to max_bodies bodies and max_pairs pairs. */
+int pe_stage_create(int32_t max_bodies, int32_t max_pairs, pe_stage** out);
+int pe_stage_destroy(pe_stage* s);
+/* The meshes that collision records of convex / concave bodies name (mesh id = index).  With
+ * prepared = 1 each mesh is taken as the reference's shape holds it after
+ * ConvexMeshShape::setMesh / ConcaveMeshShape::setMesh (convex_mesh_shape.cpp:12-132: vertices
+ * about the centre of mass, faces in the BVH build's order); with 0 setMesh is applied here, as
+ * pe_world_create does.  Synchronous. */
+int pe_stage_set_meshes(pe_stage* s, int32_t n_meshes, const pe_mesh_desc* meshes, const int32_t* concave,
+                        int32_t prepared);
+/* BroadPhaseSweepAndPrune::calcCollisionPairs (broad_phase_sweep_and_prune.cpp:6-41) over n AABB
+ * records sorted on `axis`: pairs (A, B) in the reference's sweep order to pair_a / pair_b (at most
+ * cap); result[3] (device) = pairs found (beyond cap when the output overflowed: then result[2] has
+ * bit 0 and only cap pairs are written), the next axis, capacity-error bits (bit 2: more sweep
+ * candidates than the workspace holds).  With n < 2 nothing is written but result, whose pair
+ * count is 0 and axis unchanged (the reference leaves its pair list untouched then). */
+int pe_stage_broadphase(pe_stage* s, int32_t n, const float* aabb, int32_t axis, int32_t* pair_a, int32_t* pair_b,
+                        int32_t cap, int32_t* result, pe_stream_t stream);
+/* SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41): one contact result per
+ * pair into points [npairs][8][13] (slots in cache order, invalid slots with the position at
+ * 1e30 and their distance kept), npoints [npairs], swap [npairs] (ContactResult::swap_flag).
+ * On input the distances of the slots of pairs below prev_npairs are the previous call's (the
+ * reference reuses its ContactResult objects by pair index, contact_result.cpp:158-164).
+ * result[1] (device) = capacity-error bits (bit 3: a clipped polygon outgrew its buffer). */
+int pe_stage_narrowphase(pe_stage* s, int32_t n, const float* body, int32_t npairs, const int32_t* pair_a,
+                         const int32_t* pair_b, int32_t prev_npairs, float* points, int32_t* npoints, int32_t* swap,
+                         int32_t* result, pe_stream_t stream);
+/* SequentialImpulseSolver::setupSolver + solve (sequential_impulse_solver.cpp:11-81) over the
+ * contact results of the narrowphase, in colour order (pairs of one colour share no dynamic body;
+ * the same as the reference's sequential sweep over the pairs grouped by colour): the solved
+ * velocities go to the velocity fields of solver_body (kinematic bodies untouched), the
+ * accumulated impulses to impulses [npairs][8][3] if not NULL; result[2] (device) = colours,
+ * capacity-error bits. */
+int pe_stage_solve(pe_stage* s, int32_t n, const float* body, float* solver_body, int32_t npairs, const int32_t* pair_a,
+                   const int32_t* pair_b, const int32_t* npoints, const float* points, float dt, int32_t iterations,
+                   float* impulses, int32_t* result, pe_stream_t stream);
 
 #ifdef __cplusplus
 }

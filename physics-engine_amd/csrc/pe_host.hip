@@ -36,9 +36,12 @@ struct Face { std::vector<int> idx; float n[3]; };
 // utils/bvh.cpp:45-83 over faces[l..r]: node bounds, leaf when at most nmax faces, otherwise
 // the faces are ordered with std::sort by their centroid along the longest axis of the node
 // and split at the middle.  Nodes are numbered in creation order (root 0, then the left
-// subtree, then the right subtree).
+// subtree, then the right subtree).  With sorted = true the faces are already in the order a
+// build left them (a shape after setMesh): every node's range then holds the faces the build put
+// there (later sorts only permute inside halves), so the same tree follows without sorting.
 struct BvhNode { float aa[3], bb[3]; int left = -1, right = -1, n = 0, index = 0; };
-int build_bvh(std::vector<Face>& faces, const std::vector<V3f>& v, int l, int r, int nmax, std::vector<BvhNode>& out) {
+int build_bvh(std::vector<Face>& faces, const std::vector<V3f>& v, int l, int r, int nmax, std::vector<BvhNode>& out,
+              bool sorted = false) {
     if (l > r) return -1;
     const int id = (int)out.size();
     out.emplace_back();
@@ -71,10 +74,11 @@ int build_bvh(std::vector<Face>& faces, const std::vector<V3f>& v, int l, int r,
         s /= (float)f.idx.size();
         return s;
     };
-    std::sort(faces.begin() + l, faces.begin() + r + 1, [&](const Face& a, const Face& b) { return centre(a) < centre(b); });
+    if (!sorted)
+        std::sort(faces.begin() + l, faces.begin() + r + 1, [&](const Face& a, const Face& b) { return centre(a) < centre(b); });
     const int mid = (l + r) / 2;
-    nd.left = build_bvh(faces, v, l, mid, nmax, out);
-    nd.right = build_bvh(faces, v, mid + 1, r, nmax, out);
+    nd.left = build_bvh(faces, v, l, mid, nmax, out, sorted);
+    nd.right = build_bvh(faces, v, mid + 1, r, nmax, out, sorted);
     out[id] = nd;
     return id;
 }
@@ -127,7 +131,7 @@ void cylinder_unit_table(float out[CYL_UNIT_FLOATS]) {
     }
 }
 
-int HostMeshes::add(const pe_mesh_desc& md, bool concave, float* relocation, float* volume_inertia) {
+int HostMeshes::add(const pe_mesh_desc& md, bool concave, float* relocation, float* volume_inertia, bool prepared) {
     if (md.nv <= 0 || md.nf <= 0 || !md.vertices || !md.face_start || !md.face_index || !md.face_normal) {
         pe_set_error("mesh: empty or null arrays");
         return -1;
@@ -210,14 +214,14 @@ int HostMeshes::add(const pe_mesh_desc& md, bool concave, float* relocation, flo
         I[2] = I[6] = -in[9] + V * cm[2] * cm[0];
         vi[0] = V;
         for (int k = 0; k < 9; k++) vi[1 + k] = I[k] / V;
-        for (auto& p : v) p = V3f{p.x - cm[0], p.y - cm[1], p.z - cm[2]};
+        if (!prepared) for (auto& p : v) p = V3f{p.x - cm[0], p.y - cm[1], p.z - cm[2]};
         nmax = std::max(md.nf / 8191, 1);
     } else {
         vi[0] = 0.0f;   // concave_mesh_shape.cpp:9-10: identity inertia, zero volume
         nmax = std::max(md.nf / 2047, 1);
     }
     std::vector<BvhNode> nodes;
-    build_bvh(faces, v, 0, md.nf - 1, nmax, nodes);
+    build_bvh(faces, v, 0, md.nf - 1, nmax, nodes, prepared);
     {
         // the device walk pushes both children of an inner node and holds at most one pending
         // sibling per level above it, so a tree of depth D needs D + 2 stack entries

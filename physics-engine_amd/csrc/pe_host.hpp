@@ -21,8 +21,11 @@ struct HostMeshes {
     // ConvexMeshShape::setMesh (concave = false) or ConcaveMeshShape::setMesh (concave = true)
     // of a mesh; returns its mesh id, or -1 (with pe_last_error set) if the mesh is malformed.
     // Optional outputs: relocation (the centre of mass the vertices were moved by) and the
-    // volume followed by the unit-mass inertia (10 floats).
-    int add(const pe_mesh_desc& md, bool concave, float* relocation = nullptr, float* volume_inertia = nullptr);
+    // volume followed by the unit-mass inertia (10 floats).  prepared = true takes a mesh as a
+    // shape holds it after setMesh (vertices already about the centre of mass, faces already in
+    // the BVH build's order): nothing is moved or reordered, the BVH and edges follow from it.
+    int add(const pe_mesh_desc& md, bool concave, float* relocation = nullptr, float* volume_inertia = nullptr,
+            bool prepared = false);
     void rebase(int32_t arena_offset);
     int nv(int m) const { return rec[MRW * m + MR_NV]; }
 };

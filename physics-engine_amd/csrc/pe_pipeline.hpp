@@ -150,8 +150,10 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
 
 // ------------------------------------------------------------------ stage 1
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
-// The same test on AABB records already loaded (min xyz, kinematic flag, max xyz, pad).
+// The same test on AABB records already loaded (min xyz, kinematic flag, max xyz, ignore-collision
+// flag); the same test as sap_pair_ok (pe_sap.hpp).
 __device__ __forceinline__ bool bp_valid_rec(const float (&A)[8], const float (&B)[8]) {
+    if (A[7] != 0.0f || B[7] != 0.0f) return false;
     if (A[3] != 0.0f && B[3] != 0.0f) return false;
     if (A[0] > B[4] || B[0] > A[4]) return false;
     if (A[1] > B[5] || B[1] > A[5]) return false;

@@ -2,11 +2,9 @@
 //
 // Per step, on one HIP stream, no host synchronisation:
 //   k_integrate        World::updateObjectStatus + applyExternalForce    (thread / body)
-//   k_bp_keys + radix  BroadPhaseSweepAndPrune sort on aabb_min[axis]    (stable LSD, 4 passes)
-//   k_bp_ranges/scan   per sorted body, the candidate run j > i with min_j <= max_i
-//   k_bp_chunk_*       load-balanced sweep over all candidates, two passes (count, emit),
-//                      so pairs come out in the reference's (i, j) sweep order
-//   k_bp_axis          next axis = argmax variance of AABB centres
+//   k_sap_*            sweep-and-prune broadphase over the body store (pe_sap.hpp): stable
+//                      radix sort on aabb_min[axis], candidate runs, load-balanced sweep in the
+//                      reference's (i, j) order, next axis
 //   k_narrow           thread / pair: shape dispatch + 8-slot contact cache + depth sort
 //   k_setup            thread / pair: solver rows (FrictionContactConstraint::initSequentialImpulse)
 //   k_colour           one 1024-thread workgroup: constraint-graph colouring (claims in LDS)
@@ -16,6 +14,7 @@
 #include "pe_device.hpp"
 #include "pe_prims.hpp"
 #include "pe_pipeline.hpp"
+#include "pe_sap.hpp"
 #include "pe_host.hpp"
 #include "../../include/pe_amd.h"
 
@@ -32,58 +31,21 @@ using namespace pe;
 namespace {
 
 constexpr int MAX_COLORS = 1024;
-constexpr int CHUNK = 32;            // candidates per thread in the sweep
-static_assert(CHUNK <= 32, "a chunk's test outcomes are one 32-bit mask");
 constexpr int SOLVE_THREADS = 1024;
 constexpr int CLAIM_LDS = 12288;     // bodies whose colouring claims fit in LDS (48 KiB)
-// Capacity errors, OR-ed into DevCounts::overflow by the step that hits them and kept until the
-// host reads and reports them (pe_world_sync, pe_world_broadphase): a result that lost pairs or
-// contact points is never returned as a success.
-constexpr int OVF_PAIRS = 1;         // the broadphase found more pairs than the pair capacity
-constexpr int OVF_CHUNKS = 4;        // the sweep had more candidates than its chunk capacity
-constexpr int OVF_CLIP = 8;          // a clipped polygon outgrew its buffer (contact points lost)
-
-struct DevCounts {          // device-side counters of one step
-    int32_t npairs;         // pairs emitted by the broadphase (clamped to capacity)
-    int32_t prev_npairs;    // pairs of the previous narrowphase (contact slots that survive)
-    int32_t axis;           // SAP axis for the next sort
-    int32_t overflow;       // sticky capacity-error bits (OVF_*), cleared when the host reports them
-    uint32_t ncand;         // total sweep candidates
-    uint32_t nchunk_pairs;  // pairs counted by the sweep (unclamped)
-    int32_t ncolors;
-    int32_t nordered;       // pairs in the solve order
-    int32_t npoints;        // contact points of the last narrowphase
-    uint32_t nchunks;       // live sweep chunks
-    uint32_t bar_count;     // grid barrier: arrivals (0 between barriers)
-    uint32_t bar_gen;       // grid barrier: generation
-    int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh ...
-    int32_t nsat;           // ... of which these run the polyhedral separating-axis routine
-    int32_t pad[2];
-};
 
 struct Dev {
     BodyView b;
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
+    SapView sap;            // the broadphase over the body store's AABBs (pairs to ps.pa / ps.pb)
     int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes (from the front)
                             // and to k_narrow_sat (from the back)
     int32_t* order;         // [pcap]
     int32_t* act;           // [3*pcap] colouring work lists
     int32_t* color_start;   // [pcap+1]
     int32_t* claim;         // [n] (global claims when n > CLAIM_LDS)
-    uint32_t* keys;         // [n] x2 radix buffers
-    uint32_t* vals;
-    uint32_t* keys2;
-    uint32_t* vals2;
-    float* skey;            // [n] sorted keys as floats
-    uint32_t* cand;         // [n+1] candidate counts -> offsets
-    uint32_t* chunk;        // [chunk capacity] pairs per chunk -> offsets
-    uint32_t* chunk_hit;    // [chunk capacity] per chunk, bit k: its candidate k is a pair
-    uint32_t* scan_part;    // [scan_tiles] tile totals of the grid-wide scans
-    uint32_t* hist;         // radix histograms
-    double* red;            // [6 * nred] axis reduction partials
     DevCounts* cnt;
     int pcap;
-    int chunk_cap;
     float dt;
     float g[3];
     int iterations;
@@ -102,154 +64,7 @@ __global__ void k_aabb_only(Dev d) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) refresh_body(d.b, i);
 }
 
-// ------------------------------------------------------------------ stage 1: broadphase
-__global__ void k_bp_keys(Dev d) {
-    const int ax = d.cnt->axis;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) {
-        d.keys[i] = float_key(d.b.at(F_AMIN + ax, i) + 0.0f);
-        d.vals[i] = (uint32_t)i;
-    }
-}
-
-// After the sort: sorted float keys and, per sorted position s, the number of candidates
-// j > s with key_j <= max_i (the reference's sweep breaks at the first key_j > max_i).
-__global__ void k_bp_ranges(Dev d) {
-    const int n = d.b.n;
-    const int ax = d.cnt->axis;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        const int i = (int)d.vals[s];
-        d.skey[s] = d.b.at(F_AMIN + ax, i) + 0.0f;
-    }
-}
-__global__ void k_bp_cand(Dev d) {
-    const int n = d.b.n;
-    const int ax = d.cnt->axis;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        const int i = (int)d.vals[s];
-        const float mx = d.b.at(F_AMAX + ax, i);
-        int lo = s + 1, hi = n;  // first position in [s+1, n) with skey > mx
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (d.skey[mid] > mx) hi = mid; else lo = mid + 1;
-        }
-        d.cand[s] = (uint32_t)(lo - s - 1);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.cand[n] = 0;
-}
-
-// Locates candidate t: sorted position s with cand[s] <= t < cand[s+1] (cand holds offsets).
-__device__ __forceinline__ int bp_find(const uint32_t* off, int n, uint32_t t) {
-    int lo = 0, hi = n;  // last s with off[s] <= t
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (off[mid] <= t) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-template <bool EMIT>
-__global__ void k_bp_chunks(Dev d) {
-    const int n = d.b.n;
-    const uint32_t S = d.cnt->ncand;
-    const uint32_t nchunks = min((S + CHUNK - 1) / CHUNK, (uint32_t)d.chunk_cap);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
-        uint32_t t = c * CHUNK;
-        const uint32_t tend = min(S, t + CHUNK);
-        int s = bp_find(d.cand, n, t);
-        while (d.cand[s + 1] <= t) s++;
-        if (EMIT) {
-            // the counting pass left the outcome of each of the chunk's tests in chunk_hit[c]
-            uint32_t out = d.chunk[c];
-            uint32_t hit = d.chunk_hit[c];
-            const uint32_t t0 = t;
-            int cur_s = s;
-            while (hit) {
-                const uint32_t tt = t0 + (uint32_t)(__ffs(hit) - 1);
-                hit &= hit - 1;
-                while (d.cand[cur_s + 1] <= tt) cur_s++;
-                if ((int)out < d.pcap) {
-                    d.ps.pa[out] = (int)d.vals[cur_s];
-                    d.ps.pb[out] = (int)d.vals[cur_s + 1 + (int)(tt - d.cand[cur_s])];
-                }
-                out++;
-            }
-        } else {
-            uint32_t cntv = 0, hit = 0;
-            const uint32_t t0 = t;
-            int cur_s = s, ci = (int)d.vals[s];
-            float A[8], B[8];
-            ld_rec<2>(d.b.abr(ci), A);   // the run body's record stays in registers
-            for (; t < tend; t++) {
-                while (d.cand[cur_s + 1] <= t) {
-                    cur_s++;
-                    ci = (int)d.vals[cur_s];
-                    ld_rec<2>(d.b.abr(ci), A);
-                }
-                const int j = (int)d.vals[cur_s + 1 + (int)(t - d.cand[cur_s])];
-                ld_rec<2>(d.b.abr(j), B);
-                if (bp_valid_rec(A, B)) {
-                    cntv++;
-                    hit |= 1u << (t - t0);
-                }
-            }
-            d.chunk[c] = cntv;
-            d.chunk_hit[c] = hit;
-        }
-    }
-}
-
-// Number of live sweep chunks, for the chunk scan.
-__global__ void k_bp_nchunks(Dev d, uint32_t* nchunks) {
-    const uint32_t want = (d.cnt->ncand + CHUNK - 1) / CHUNK;
-    *nchunks = min(want, (uint32_t)d.chunk_cap);
-    if (want > (uint32_t)d.chunk_cap) d.cnt->overflow |= OVF_CHUNKS;
-}
-
-__global__ void k_bp_finish(Dev d) {
-    const uint32_t total = d.cnt->nchunk_pairs;
-    if (total > (uint32_t)d.pcap) d.cnt->overflow |= OVF_PAIRS;
-    d.cnt->npairs = total > (uint32_t)d.pcap ? d.pcap : (int32_t)total;
-}
-
-// Next SAP axis (broad_phase_sweep_and_prune.cpp:37-40): variance of AABB centres, reduced in
-// FP64 over fixed blocks (deterministic), argmax with the reference's tie rule.
-constexpr int RED_THREADS = 256;
-__global__ void __launch_bounds__(RED_THREADS) k_bp_axis_partial(Dev d) {
-    __shared__ double sh[6][RED_THREADS];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const float c = (d.b.at(F_AMIN + k, i) + d.b.at(F_AMAX + k, i)) * 0.5f;
-            acc[k] += (double)c;
-            acc[3 + k] += (double)(c * c);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
-#pragma unroll
-            for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int k = 0; k < 6; k++) d.red[k * gridDim.x + blockIdx.x] = sh[k][0];
-}
-__global__ void k_bp_axis_final(Dev d, int nblocks) {
-    if (threadIdx.x != 0) return;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nblocks; b++)
-        for (int k = 0; k < 6; k++) s[k] += d.red[k * nblocks + b];
-    double v[3];
-    for (int k = 0; k < 3; k++) v[k] = s[3 + k] - s[k] * s[k] / (double)d.b.n;
-    int axis = 0;
-    if (v[1] > v[0]) axis = 1;
-    if (v[2] > v[axis]) axis = 2;
-    d.cnt->axis = axis;
-}
+// ------------------------------------------------------------------ stage 1: broadphase (pe_sap.hpp)
 
 // ------------------------------------------------------------------ stage 2: narrowphase
 // Box and sphere pairs here; pairs with a cylinder or a mesh are listed for k_narrow_shapes.
@@ -391,8 +206,6 @@ struct pe_world {
     int n = 0, stride = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
-    int rs_blocks = 0;
-    int red_blocks = 0;
     int gs_blocks = 0;      // workgroups of the cooperative Gauss-Seidel kernel
     int gs_max_blocks = 0;  // co-resident limit
     bool timing = false;
@@ -423,35 +236,8 @@ struct pe_world {
         hipLaunchKernelGGL(k_integrate, dim3(grid(n, 256)), dim3(256), 0, stream, d);
         return PE_OK;
     }
-    // Grid-wide exclusive scan of n elements (of *count when count is given; at most cap).
-    void scan(uint32_t* data, int cap, int n, const uint32_t* count, uint32_t* total) {
-        const int nb = std::max(1, (cap + SCAN_TILE - 1) / SCAN_TILE);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(1024), 0, stream, data, n, count, d.scan_part);
-        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.scan_part, nb, total);
-        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, stream, data, n, count, d.scan_part);
-    }
     int launch_broadphase() {
-        hipLaunchKernelGGL(k_bp_keys, dim3(grid(n, 256)), dim3(256), 0, stream, d);
-        uint32_t *ki = d.keys, *vi = d.vals, *ko = d.keys2, *vo = d.vals2;
-        for (int pass = 0; pass < 4; pass++) {
-            hipLaunchKernelGGL(k_radix_hist, dim3(rs_blocks), dim3(RS_THREADS), 0, stream, ki, n, pass * 8, d.hist, rs_blocks);
-            hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, stream, d.hist, 256 * rs_blocks, (uint32_t*)nullptr);
-            hipLaunchKernelGGL(k_radix_scatter, dim3(rs_blocks), dim3(RS_THREADS), 0, stream, ki, vi, ko, vo, n, pass * 8,
-                               d.hist, rs_blocks);
-            std::swap(ki, ko);
-            std::swap(vi, vo);
-        }
-        // after 4 passes the sorted data is back in keys/vals
-        hipLaunchKernelGGL(k_bp_ranges, dim3(grid(n, 256)), dim3(256), 0, stream, d);
-        hipLaunchKernelGGL(k_bp_cand, dim3(grid(n, 256)), dim3(256), 0, stream, d);
-        scan(d.cand, n + 1, n + 1, nullptr, &d.cnt->ncand);
-        hipLaunchKernelGGL(k_bp_nchunks, dim3(1), dim3(1), 0, stream, d, &d.cnt->nchunks);
-        hipLaunchKernelGGL(k_bp_chunks<false>, dim3(2048), dim3(256), 0, stream, d);
-        scan(d.chunk, d.chunk_cap, 0, &d.cnt->nchunks, &d.cnt->nchunk_pairs);
-        hipLaunchKernelGGL(k_bp_chunks<true>, dim3(2048), dim3(256), 0, stream, d);
-        hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, stream, d);
-        hipLaunchKernelGGL(k_bp_axis_partial, dim3(red_blocks), dim3(RED_THREADS), 0, stream, d);
-        hipLaunchKernelGGL(k_bp_axis_final, dim3(1), dim3(64), 0, stream, d, red_blocks);
+        sap_launch(d.sap, stream);
         return PE_OK;
     }
     int launch_narrowphase() {
@@ -565,22 +351,31 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.act, 3 * (size_t)pcap);
     rc |= w->alloc(&d.color_start, pcap + 1);
     rc |= w->alloc(&d.claim, n);
-    rc |= w->alloc(&d.keys, n);
-    rc |= w->alloc(&d.vals, n);
-    rc |= w->alloc(&d.keys2, n);
-    rc |= w->alloc(&d.vals2, n);
-    rc |= w->alloc(&d.skey, n);
-    rc |= w->alloc(&d.cand, n + 1);
-    w->rs_blocks = (n + RS_TILE - 1) / RS_TILE;
-    rc |= w->alloc(&d.hist, 256 * (size_t)w->rs_blocks);
-    w->red_blocks = std::max(1, std::min(256, (n + RED_THREADS - 1) / RED_THREADS));
-    rc |= w->alloc(&d.red, 6 * (size_t)w->red_blocks);
     rc |= w->alloc(&d.cnt, 1);
+    SapView& sp = d.sap;
+    sp.n = n;
+    sp.amin = d.b.f + (size_t)F_AMIN * w->stride;
+    sp.amax = d.b.f + (size_t)F_AMAX * w->stride;
+    sp.fs = w->stride;
+    sp.bs = 1;
+    sp.rec = d.b.ab;
+    sp.cnt = d.cnt;
+    sp.pa = d.ps.pa;
+    sp.pb = d.ps.pb;
+    sp.pcap = pcap;
+    rc |= w->alloc(&sp.keys, n);
+    rc |= w->alloc(&sp.vals, n);
+    rc |= w->alloc(&sp.keys2, n);
+    rc |= w->alloc(&sp.vals2, n);
+    rc |= w->alloc(&sp.skey, n);
+    rc |= w->alloc(&sp.cand, n + 1);
+    rc |= w->alloc(&sp.hist, 256 * (size_t)sap_radix_blocks(n));
+    rc |= w->alloc(&sp.red, 6 * (size_t)sap_red_blocks(n));
     // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
-    d.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / CHUNK + 1, (int64_t)1 << 26);
-    rc |= w->alloc(&d.chunk, d.chunk_cap);
-    rc |= w->alloc(&d.chunk_hit, d.chunk_cap);
-    rc |= w->alloc(&d.scan_part, (std::max(d.chunk_cap, n + 1) + SCAN_TILE - 1) / SCAN_TILE + 1);
+    sp.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / SAP_CHUNK + 1, (int64_t)1 << 26);
+    rc |= w->alloc(&sp.chunk, sp.chunk_cap);
+    rc |= w->alloc(&sp.chunk_hit, sp.chunk_cap);
+    rc |= w->alloc(&sp.scan_part, sap_scan_parts(n, sp.chunk_cap));
     if (rc != PE_OK) return fail(PE_ERR_DEVICE);
 
     // host staging of the body store (FP32) and of the scene's meshes
@@ -821,6 +616,16 @@ int pe_world_set_solver_blocks(pe_world* w, int32_t blocks) {
     return PE_OK;
 }
 
+int pe_world_get_records(pe_world* w, float* aabb, float* body, float* solver) {
+    if (int rc = check_world(w)) return rc;
+    const size_t n = (size_t)w->n;
+    PE_HIP(hipStreamSynchronize(w->stream));
+    if (aabb) PE_HIP(hipMemcpy(aabb, w->d.b.ab, n * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    if (body) PE_HIP(hipMemcpy(body, w->d.b.cb, n * CBW * sizeof(float), hipMemcpyDeviceToHost));
+    if (solver) PE_HIP(hipMemcpy(solver, w->d.b.sb, n * SBW * sizeof(float), hipMemcpyDeviceToHost));
+    return PE_OK;
+}
+
 int pe_world_get_counts(pe_world* w, int64_t* counts) {
     if (int rc = check_world(w)) return rc;
     DevCounts c;
@@ -830,6 +635,282 @@ int pe_world_get_counts(pe_world* w, int64_t* counts) {
     counts[2] = c.npairs;
     counts[3] = c.npoints;
     counts[4] = c.ncolors;
+    return PE_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== stage entry points
+// The three stage seams of the reference's World (broadphase, narrowphase, constraint solver) on
+// a caller's device buffers and stream, so a host that keeps the reference's World can replace one
+// stage at a time.  A pe_stage is a pe_world used as workspace: each call loads the caller's
+// records into its body and pair stores, runs the same kernels as pe_world_step, and writes the
+// results back to the caller's buffers; nothing is synchronised (the caller waits on its stream).
+namespace {
+
+// Collision records of the caller ([n][CBW]: basis, origin, AABB scale, shape parameters, shape
+// type and kinematic flag as int bits, friction, restitution) into the body store; the shape type
+// and kinematic flag also into the int fields the pair dispatch and the colouring read.
+__global__ void k_stage_bodies(Dev d, const float* __restrict__ body, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4* s = reinterpret_cast<const float4*>(body + (size_t)i * CBW);
+        float4* o = reinterpret_cast<float4*>(d.b.cbr(i));
+#pragma unroll
+        for (int k = 0; k < CBW / 4; k++) o[k] = s[k];
+        d.b.iat(I_TYPE, i) = __float_as_int(body[(size_t)i * CBW + CB_TYPE]);
+        d.b.iat(I_KIN, i) = __float_as_int(body[(size_t)i * CBW + CB_KIN]) != 0 ? 1 : 0;
+    }
+}
+// Solver records of the caller ([n][SBW]: inverse mass, kinematic flag, world inverse inertia,
+// velocities; the temp velocities are ignored) into the body store, with the temp velocities set
+// as SequentialImpulseSolver::setupSolver's clearTempVelocity sets them (the velocities, zero for a
+// kinematic body).
+__global__ void k_stage_solver_in(Dev d, const float* __restrict__ sb, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float r[SBW];
+        ld_rec<SBW / 4>(sb + (size_t)i * SBW, r);
+        const bool kin = r[SB_KIN] != 0.0f;
+        for (int k = 0; k < 3; k++) {
+            r[SB_TV + k] = kin ? 0.0f : r[SB_V + k];
+            r[SB_TW + k] = kin ? 0.0f : r[SB_W + k];
+        }
+        r[SB_KIN] = kin ? 1.0f : 0.0f;
+        if (kin) r[SB_IM] = 0.0f;
+        st_rec<SBW / 4>(d.b.sbr(i), r);
+        d.b.iat(I_KIN, i) = kin ? 1 : 0;
+    }
+}
+// The solved velocities (syncTempVelocity) into the caller's solver records.
+__global__ void k_stage_solver_out(Dev d, float* __restrict__ sb, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (d.b.iat(I_KIN, i)) continue;
+        const float* t = d.b.sbr(i);
+        float* o = sb + (size_t)i * SBW;
+        for (int k = 0; k < 3; k++) {
+            o[SB_V + k] = t[SB_TV + k];
+            o[SB_W + k] = t[SB_TW + k];
+        }
+    }
+}
+// Pairs (and, for the solver, point counts and contact slots; for the narrowphase, the distances
+// of the slots that survive from the previous call) into the pair store.
+__global__ void k_stage_pairs(Dev d, const int32_t* __restrict__ pa, const int32_t* __restrict__ pb, int np, int prev,
+                              const int32_t* __restrict__ npts, const float* __restrict__ points) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.cnt->npairs = np;
+        d.cnt->prev_npairs = prev;
+        d.cnt->nshape = 0;
+        d.cnt->nsat = 0;
+        d.cnt->npoints = 0;
+        d.cnt->overflow = 0;
+    }
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        d.ps.pa[p] = pa[p];
+        d.ps.pb[p] = pb[p];
+        if (npts) d.ps.npts[p] = npts[p];
+        if (!points) continue;
+        for (int k = 0; k < 8; k++) {
+            const float* c = points + ((size_t)p * 8 + k) * CF;
+            const size_t s = (size_t)p * 8 + k;
+            if (npts) {
+                if (k < npts[p])
+                    for (int f = 0; f < CW; f++) d.ps.cp[s * CW + f] = c[f];
+                d.ps.cd[s] = c[12];
+            } else if (p < prev) {
+                d.ps.cd[s] = c[12];
+            }
+        }
+    }
+}
+// Contact results of the narrowphase into the caller's slots ([np][8][CF]: valid slots in cache
+// order, invalid ones with the position at PE_REAL_MAX and their distance kept), point counts and
+// swap flags.
+__global__ void k_stage_contacts_out(Dev d, int np, float* __restrict__ points, int32_t* __restrict__ npts,
+                                     int32_t* __restrict__ swap) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const int m = d.ps.npts[p];
+        if (npts) npts[p] = m;
+        if (swap) swap[p] = d.ps.swap[p] & 1;
+        for (int k = 0; k < 8; k++) {
+            const size_t s = (size_t)p * 8 + k;
+            float* o = points + s * CF;
+            for (int f = 0; f < CW; f++) o[f] = k < m ? d.ps.cp[s * CW + f] : (f < 3 ? REAL_MAX : 0.0f);
+            o[12] = d.ps.cd[s];
+        }
+    }
+}
+__global__ void k_stage_impulses_out(Dev d, int np, float* __restrict__ imp) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const int m = d.ps.npts[p];
+        for (int k = 0; k < 8; k++)
+            for (int q = 0; q < 3; q++)
+                imp[((size_t)p * 8 + k) * 3 + q] = k < m ? d.ps.rows[((size_t)p * 8 + k) * RW + R_LN + q] : 0.0f;
+    }
+}
+// result[0] = pairs the sweep found (more than the capacity when it overflowed), result[1] = the
+// next axis, result[2] = capacity-error bits.
+__global__ void k_stage_sap_result(SapView s, int32_t* result) {
+    result[0] = (int32_t)s.cnt->nchunk_pairs;
+    result[1] = s.cnt->axis;
+    result[2] = s.cnt->overflow;
+}
+__global__ void k_stage_result(Dev d, int32_t* result, int which) {
+    result[0] = which == 0 ? d.cnt->overflow : d.cnt->ncolors;
+    if (which == 1) result[1] = d.cnt->overflow;
+}
+
+// A sweep over a caller's AABB records ([n][8]) with the given working memory and pair outputs.
+SapView sap_over_records(const SapView& work, int n, const float* aabb, int32_t* pa, int32_t* pb, int cap) {
+    SapView s = work;
+    s.n = n;
+    s.amin = aabb;
+    s.amax = aabb + 4;
+    s.fs = 1;
+    s.bs = 8;
+    s.rec = aabb;
+    s.pa = pa;
+    s.pb = pb;
+    s.pcap = cap;
+    return s;
+}
+
+int stage_grid(int work, int block) { return std::max(1, std::min((work + block - 1) / block, 4096)); }
+
+}  // namespace
+
+struct pe_stage {
+    pe_world* w = nullptr;
+    int max_bodies = 0, max_pairs = 0;
+    ~pe_stage() { delete w; }
+};
+
+extern "C" {
+
+int pe_stage_create(int32_t max_bodies, int32_t max_pairs, pe_stage** out) {
+    if (!out || max_bodies < 1 || max_pairs < 1) { pe_set_error("pe_stage_create: invalid argument"); return PE_ERR_INVALID; }
+    // the workspace is a world of max_bodies unit spheres (their state is overwritten by each call)
+    std::vector<int32_t> type(max_bodies, PE_SHAPE_SPHERE), kin(max_bodies, 1);
+    std::vector<double> param(4 * (size_t)max_bodies, 0.0), mass(max_bodies, 1.0), zero3(3 * (size_t)max_bodies, 0.0),
+        rot(9 * (size_t)max_bodies, 0.0), mu(max_bodies, 0.5), e(max_bodies, 0.0);
+    for (int i = 0; i < max_bodies; i++) {
+        param[4 * (size_t)i] = 1.0;
+        rot[9 * (size_t)i] = rot[9 * (size_t)i + 4] = rot[9 * (size_t)i + 8] = 1.0;
+    }
+    pe_scene_desc sc{};
+    sc.n = max_bodies;
+    sc.shape_type = type.data();
+    sc.shape_param = param.data();
+    sc.mass = mass.data();
+    sc.position = zero3.data();
+    sc.rotation = rot.data();
+    sc.linear_velocity = zero3.data();
+    sc.angular_velocity = zero3.data();
+    sc.friction = mu.data();
+    sc.restitution = e.data();
+    sc.kinematic = kin.data();
+    pe_world_params prm{1.0 / 60.0, {0.0, 0.0, 0.0}, 10};
+    auto* S = new pe_stage();
+    if (int rc = pe_world_create(&sc, &prm, max_pairs, &S->w)) { delete S; return rc; }
+    S->w->has_shapes = true;
+    S->max_bodies = max_bodies;
+    S->max_pairs = max_pairs;
+    *out = S;
+    return PE_OK;
+}
+
+int pe_stage_destroy(pe_stage* s) {
+    delete s;
+    return PE_OK;
+}
+
+int pe_stage_set_meshes(pe_stage* s, int32_t n_meshes, const pe_mesh_desc* meshes, const int32_t* concave,
+                        int32_t prepared) {
+    if (!s || n_meshes < 0 || (n_meshes > 0 && (!meshes || !concave))) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
+    HostMeshes hm;
+    for (int k = 0; k < n_meshes; k++)
+        if (hm.add(meshes[k], concave[k] != 0, nullptr, nullptr, prepared != 0) < 0) return PE_ERR_INVALID;
+    pe_world* w = s->w;
+    PE_HIP(hipStreamSynchronize(w->stream));
+    w->meshes.release();
+    if (int rc = w->meshes.upload(hm, w->stream)) return rc;
+    PE_HIP(hipStreamSynchronize(w->stream));
+    w->d.b.ms = w->meshes.view();
+    return PE_OK;
+}
+
+int pe_stage_broadphase(pe_stage* s, int32_t n, const float* aabb, int32_t axis, int32_t* pair_a, int32_t* pair_b,
+                        int32_t cap, int32_t* result, pe_stream_t stream) {
+    if (!s || n < 0 || n > s->max_bodies || axis < 0 || axis > 2 || cap < 0 || !result || (n > 0 && !aabb) ||
+        (cap > 0 && (!pair_a || !pair_b))) {
+        pe_set_error("pe_stage_broadphase: invalid argument");
+        return PE_ERR_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const SapView v = sap_over_records(s->w->d.sap, n, aabb, pair_a, pair_b, cap);
+    hipLaunchKernelGGL(k_sap_begin, dim3(1), dim3(1), 0, st, v, (int)axis);
+    if (n >= 2) sap_launch(v, st);   // fewer than 2 bodies: no pairs, the axis stays (broad_phase_sweep_and_prune.cpp:8)
+    hipLaunchKernelGGL(k_stage_sap_result, dim3(1), dim3(1), 0, st, v, result);
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+
+int pe_stage_narrowphase(pe_stage* s, int32_t n, const float* body, int32_t npairs, const int32_t* pair_a,
+                         const int32_t* pair_b, int32_t prev_npairs, float* points, int32_t* npoints, int32_t* swap,
+                         int32_t* result, pe_stream_t stream) {
+    if (!s || n < 1 || n > s->max_bodies || npairs < 0 || npairs > s->max_pairs || prev_npairs < 0 ||
+        prev_npairs > s->max_pairs || !body || !result || (npairs > 0 && (!pair_a || !pair_b || !points))) {
+        pe_set_error("pe_stage_narrowphase: invalid argument");
+        return PE_ERR_INVALID;
+    }
+    pe_world* w = s->w;
+    hipStream_t keep = w->stream;
+    w->stream = (hipStream_t)stream;
+    w->n = n;
+    w->d.b.n = n;
+    hipLaunchKernelGGL(k_stage_bodies, dim3(stage_grid(n, 256)), dim3(256), 0, w->stream, w->d, body, (int)n);
+    hipLaunchKernelGGL(k_stage_pairs, dim3(stage_grid(npairs, 256)), dim3(256), 0, w->stream, w->d, pair_a, pair_b,
+                       (int)npairs, (int)prev_npairs, (const int32_t*)nullptr, (const float*)points);
+    w->launch_narrowphase();
+    if (npairs > 0)
+        hipLaunchKernelGGL(k_stage_contacts_out, dim3(stage_grid(npairs, 128)), dim3(128), 0, w->stream, w->d,
+                           (int)npairs, points, npoints, swap);
+    hipLaunchKernelGGL(k_stage_result, dim3(1), dim3(1), 0, w->stream, w->d, result, 0);
+    w->stream = keep;
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+
+int pe_stage_solve(pe_stage* s, int32_t n, const float* body, float* solver_body, int32_t npairs, const int32_t* pair_a,
+                   const int32_t* pair_b, const int32_t* npoints, const float* points, float dt, int32_t iterations,
+                   float* impulses, int32_t* result, pe_stream_t stream) {
+    if (!s || n < 1 || n > s->max_bodies || npairs < 0 || npairs > s->max_pairs || !body || !solver_body || !result ||
+        iterations < 0 || (npairs > 0 && (!pair_a || !pair_b || !npoints || !points))) {
+        pe_set_error("pe_stage_solve: invalid argument");
+        return PE_ERR_INVALID;
+    }
+    pe_world* w = s->w;
+    hipStream_t keep = w->stream;
+    w->stream = (hipStream_t)stream;
+    w->n = n;
+    w->d.b.n = n;
+    w->d.dt = dt;
+    w->d.iterations = iterations;
+    w->gs_blocks = default_gs_blocks(n, w->gs_max_blocks);
+    hipLaunchKernelGGL(k_stage_bodies, dim3(stage_grid(n, 256)), dim3(256), 0, w->stream, w->d, body, (int)n);
+    hipLaunchKernelGGL(k_stage_solver_in, dim3(stage_grid(n, 256)), dim3(256), 0, w->stream, w->d, (const float*)solver_body, (int)n);
+    hipLaunchKernelGGL(k_stage_pairs, dim3(stage_grid(npairs, 256)), dim3(256), 0, w->stream, w->d, pair_a, pair_b,
+                       (int)npairs, 0, npoints, points);
+    int rc = w->launch_solve();
+    if (rc == PE_OK) {
+        hipLaunchKernelGGL(k_stage_solver_out, dim3(stage_grid(n, 256)), dim3(256), 0, w->stream, w->d, solver_body, (int)n);
+        if (impulses && npairs > 0)
+            hipLaunchKernelGGL(k_stage_impulses_out, dim3(stage_grid(npairs, 128)), dim3(128), 0, w->stream, w->d,
+                               (int)npairs, impulses);
+        hipLaunchKernelGGL(k_stage_result, dim3(1), dim3(1), 0, w->stream, w->d, result, 1);
+    }
+    w->stream = keep;
+    if (rc != PE_OK) return rc;
+    PE_HIP(hipGetLastError());
     return PE_OK;
 }
 

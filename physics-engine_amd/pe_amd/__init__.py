@@ -13,3 +13,4 @@ from .world import World  # noqa: F401
 from .batch import Batch  # noqa: F401
 from .server import Server, ServerConfig  # noqa: F401
 from .mesh import prepare_mesh  # noqa: F401
+from .stage import Stage  # noqa: F401

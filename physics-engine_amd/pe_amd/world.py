@@ -150,6 +150,15 @@ class World:
     def set_solver_blocks(self, blocks: int):
         check(lib().pe_world_set_solver_blocks(self.h, ctypes.c_int32(blocks)))
 
+    def records(self):
+        """(AABB records [n, 8], collision records [n, 20], solver records [n, 24]) as the last
+        integrate stage wrote them: the inputs of the stage entry points (pe_amd.stage)."""
+        a = np.zeros((self.n, 8), np.float32)
+        b = np.zeros((self.n, 20), np.float32)
+        c = np.zeros((self.n, 24), np.float32)
+        check(lib().pe_world_get_records(self.h, _f(a), _f(b), _f(c)))
+        return a, b, c
+
     def counts(self):
         c = (ctypes.c_int64 * 5)()
         check(lib().pe_world_get_counts(self.h, c))
