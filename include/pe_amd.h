@@ -120,6 +120,13 @@ int pe_world_destroy(pe_world* w);
 /* World::step (src/intf/world.cpp:267-300), nsteps times, on the world's stream. */
 int pe_world_step(pe_world* w, int32_t nsteps);
 
+/* pe_world_step enqueued on the caller's stream (a hipStream_t, NULL for the null stream; the
+ * type is declared below) instead of the world's own: the steps run after the caller's earlier
+ * work on that stream and after the world's pending work, and the world's later calls (state
+ * getters, pe_world_sync, further steps) wait for them.  Nothing is synchronised here. */
+typedef void* pe_stream_t;
+int pe_world_step_async(pe_world* w, int32_t nsteps, pe_stream_t stream);
+
 /* Stage entry points; one call of each, in this order, is one World::step. */
 /* World::updateObjectStatus + applyExternalForce (world.cpp:35-127) */
 int pe_world_integrate(pe_world* w);
@@ -156,6 +163,14 @@ int pe_world_get_impulses(pe_world* w, float* impulses /*cap*8*3*/, int32_t cap)
 /* Solve order of the last solve: pair indices grouped by colour, colour start offsets. */
 int pe_world_get_solve_order(pe_world* w, int32_t* order /*cap*/, int32_t cap, int32_t* nordered,
                              int32_t* color_start /*ccap+1*/, int32_t ccap, int32_t* ncolors);
+/* The contact cache the next narrowphase starts from: the pair count of the last narrowphase and
+ * the distances of its pairs' 8 slots ([prev_npairs][8], stale ones included).  The reference
+ * reuses its ContactResult objects by pair index and keeps invalid slots' distances
+ * (simple_narrow_phase.cpp:10-22, contact_result.cpp:158-164), so these decide which points the
+ * next step keeps; together with the body state (pe_world_get_state / set_state) and the SAP axis
+ * they are everything a step reads from the steps before it, i.e. a checkpoint. */
+int pe_world_get_cache(pe_world* w, float* distances /*8*cap*/, int32_t cap, int32_t* prev_npairs);
+int pe_world_set_cache(pe_world* w, const float* distances /*8*prev_npairs*/, int32_t prev_npairs);
 /* SAP axis that the next broadphase sorts on (BroadPhaseSweepAndPrune::_target_axis). */
 int pe_world_get_axis(pe_world* w, int32_t* axis);
 int pe_world_set_axis(pe_world* w, int32_t axis);
@@ -191,7 +206,6 @@ int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
  *                         inverse mass, kinematic (0 / 1), world inverse inertia (9, row-major),
  *                         linear velocity (3), angular velocity (3), unused
  *   contact slot     [13] as pe_world_get_contacts */
-typedef void* pe_stream_t;
 #define PE_AABB_FLOATS 8
 #define PE_BODY_FLOATS 20
 #define PE_SOLVER_BODY_FLOATS 24

@@ -210,6 +210,7 @@ struct pe_world {
     int gs_max_blocks = 0;  // co-resident limit
     bool timing = false;
     hipEvent_t ev[5] = {};
+    hipEvent_t order_ev[2] = {};   // orders work on a caller's stream with the world's own stream
     double stage_ms[4] = {0, 0, 0, 0};
     int n_dynamic = 0;
     bool has_shapes = false;   // any cylinder / convex / concave body
@@ -220,6 +221,7 @@ struct pe_world {
         for (void* p : allocs) hipFree(p);
         meshes.release();
         for (auto& e : ev) if (e) hipEventDestroy(e);
+        for (auto& e : order_ev) if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
     template <class T> int alloc(T** p, size_t count) {
@@ -398,6 +400,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     }
     hipLaunchKernelGGL(k_aabb_only, dim3(w->grid(n, 256)), dim3(256), 0, w->stream, d);
     for (auto& e : w->ev) hipEventCreate(&e);
+    for (auto& e : w->order_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
     {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -422,6 +425,51 @@ int pe_world_step(pe_world* w, int32_t nsteps) {
     if (int rc = check_world(w)) return rc;
     for (int s = 0; s < nsteps; s++)
         if (int rc = w->step()) return rc;
+    return PE_OK;
+}
+static int read_counts(pe_world* w, DevCounts& c) {
+    PE_HIP(hipMemcpyAsync(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    return PE_OK;
+}
+int pe_world_step_async(pe_world* w, int32_t nsteps, pe_stream_t stream) {
+    if (int rc = check_world(w)) return rc;
+    if (nsteps < 0) { pe_set_error("nsteps must be >= 0"); return PE_ERR_INVALID; }
+    hipStream_t cs = (hipStream_t)stream;
+    // the caller's stream first waits for the world's pending work ...
+    PE_HIP(hipEventRecord(w->order_ev[0], w->stream));
+    PE_HIP(hipStreamWaitEvent(cs, w->order_ev[0], 0));
+    hipStream_t keep = w->stream;
+    w->stream = cs;
+    int rc = PE_OK;
+    for (int s = 0; s < nsteps && rc == PE_OK; s++) rc = w->step();
+    w->stream = keep;
+    // ... and the world's later calls wait for the steps enqueued on it
+    PE_HIP(hipEventRecord(w->order_ev[1], cs));
+    PE_HIP(hipStreamWaitEvent(w->stream, w->order_ev[1], 0));
+    return rc;
+}
+int pe_world_get_cache(pe_world* w, float* distances, int32_t cap, int32_t* prev_npairs) {
+    if (int rc = check_world(w)) return rc;
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    if (prev_npairs) *prev_npairs = c.prev_npairs;
+    const int m = std::min(cap, c.prev_npairs);
+    if (distances && m > 0) PE_HIP(hipMemcpy(distances, w->d.ps.cd, (size_t)m * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    return PE_OK;
+}
+int pe_world_set_cache(pe_world* w, const float* distances, int32_t prev_npairs) {
+    if (int rc = check_world(w)) return rc;
+    if (prev_npairs < 0 || prev_npairs > w->d.pcap || (prev_npairs > 0 && !distances)) {
+        pe_set_error("pe_world_set_cache: pair count beyond the world's pair capacity");
+        return PE_ERR_INVALID;
+    }
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    c.prev_npairs = prev_npairs;
+    if (prev_npairs > 0)
+        PE_HIP(hipMemcpy(w->d.ps.cd, distances, (size_t)prev_npairs * 8 * sizeof(float), hipMemcpyHostToDevice));
+    PE_HIP(hipMemcpy(w->d.cnt, &c, sizeof(c), hipMemcpyHostToDevice));
     return PE_OK;
 }
 int pe_world_integrate(pe_world* w) {
@@ -509,11 +557,6 @@ int pe_world_get_aabbs(pe_world* w, float* mn, float* mx) {
     int rc = PE_OK;
     if (mn && (rc = get_field(w, F_AMIN, 3, mn))) return rc;
     if (mx && (rc = get_field(w, F_AMAX, 3, mx))) return rc;
-    return PE_OK;
-}
-static int read_counts(pe_world* w, DevCounts& c) {
-    PE_HIP(hipMemcpyAsync(&c, w->d.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
-    PE_HIP(hipStreamSynchronize(w->stream));
     return PE_OK;
 }
 int pe_world_get_pairs(pe_world* w, int32_t* pairs, int32_t cap, int32_t* npairs) {

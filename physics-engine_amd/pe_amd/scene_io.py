@@ -56,6 +56,16 @@ def quaternion_basis(w: float, x: float, y: float, z: float) -> np.ndarray:
                      [txz - twy, tyz + twx, 1.0 - (txx + tyy)]])
 
 
+def _face_normals(mesh: Mesh) -> np.ndarray:
+    """normalize((p1 - p0) x (p2 - p0)) per face, as Mesh::perFaceNormal computes it."""
+    v = np.asarray(mesh.vertices, np.float64)
+    out = []
+    for fc in mesh.faces:
+        n = np.cross(v[fc[1]] - v[fc[0]], v[fc[2]] - v[fc[0]])
+        out.append(n / math.sqrt(float(n @ n)))
+    return np.asarray(out, float).reshape(-1, 3)
+
+
 def load_obj(path: str, scale: float = 1.0) -> Mesh:
     """A Wavefront .obj read as Mesh::loadFromObj reads it (mesh.cpp:36-67), coordinates times scale."""
     verts, faces = [], []
@@ -69,13 +79,12 @@ def load_obj(path: str, scale: float = 1.0) -> Mesh:
             elif tok[0] == "f":
                 faces.append([int(t.split("/")[0]) - 1 for t in tok[1:]])
     v = np.asarray(verts, float).reshape(-1, 3)
-    normals = []
     for fc in faces:
         if len(fc) < 3:
             raise ValueError(f"{path}: a face with fewer than 3 vertices")
-        n = np.cross(v[fc[1]] - v[fc[0]], v[fc[2]] - v[fc[0]])
-        normals.append(n / math.sqrt(float(n @ n)))
-    return Mesh(v, faces, np.asarray(normals, float).reshape(-1, 3))
+    m = Mesh(v, faces, np.zeros((len(faces), 3)))
+    m.normals = _face_normals(m)
+    return m
 
 
 def save_obj(mesh: Mesh, path: str) -> None:
@@ -162,10 +171,11 @@ def load_scene(path: str, dt: float = 1.0 / 60.0, iterations: int = 10) -> Scene
     return sc
 
 
-def save_scene(sc: Scene, path: str, mesh_paths=None) -> None:
+def save_scene(sc: Scene, path: str, mesh_paths=None, extra_config=None, extra=None) -> None:
     """Writes sc in the reference's scene format (Simulator::saveScene, simulator.cpp:79-250).
     mesh_paths maps a mesh index to the .obj path written for it; meshes without one are saved
-    beside the scene file as <scene>_mesh<k>.obj (at scale 1)."""
+    beside the scene file as <scene>_mesh<k>.obj (at scale 1).  extra_config adds keys to
+    ``Configuration`` and extra adds top-level keys (the reference's loader ignores both)."""
     sc = sc.contiguous()
     mesh_paths = dict(mesh_paths or {})
     base = os.path.splitext(path)[0]
@@ -210,5 +220,61 @@ def save_scene(sc: Scene, path: str, mesh_paths=None) -> None:
                               "sleep_angular_velocity2_threshold": sleep[1],
                               "sleep_time_threshold": sleep[2]},
             "DamageSources": [], "RigidBodies": rbs}
+    data["Configuration"].update(extra_config or {})
+    data.update(extra or {})
     with open(path, "w") as f:
         json.dump(data, f, indent=1)
+
+
+def save_checkpoint(world, path: str, mesh_paths=None) -> None:
+    """A scene file from which load_checkpoint resumes `world` exactly where it is: the next step of
+    the loaded world is bit-identical to the next step of this one.
+
+    It is the reference's scene format (so the reference's Simulator::loadScene opens it too) with
+    the bodies' current poses and velocities -- FP32 values written as the exact decimal of their
+    FP64 widening -- and the meshes as the scene gave them (every load prepares them alike: setMesh
+    is deterministic, so the device sees the same faces in the same order).  What a step reads from
+    the steps before it beyond the body state goes under keys the reference ignores:
+    ``Configuration.dt``, ``Configuration.iterations``, ``Configuration.sap_axis`` (the axis the next
+    sweep sorts on, broad_phase_sweep_and_prune.cpp:37-40) and ``ContactCache`` (the pair count of
+    the last narrowphase and its slots' distances, FP32 little-endian in base64: the reference keeps
+    them between steps in its reused ContactResult objects, contact_result.cpp:158-164).  A
+    kinematic body's velocity is not written (loadScene never sets one, simulator.cpp:350-356)."""
+    import base64
+    import copy
+    sc = copy.copy(world.scene.contiguous())
+    for k, m in enumerate(sc.meshes):
+        if not np.array_equal(np.asarray(m.normals, np.float64), _face_normals(m)):
+            raise ValueError(f"mesh {k}: its face normals are not the ones a scene file's loader computes from its "
+                             f"vertices (Mesh::perFaceNormal), so a loaded checkpoint would not resume it exactly")
+    pos, rot, v, w = world.state()
+    sc.position = np.asarray(pos, np.float64)
+    sc.rotation = np.asarray(rot, np.float64)
+    kin = sc.kinematic.astype(bool)
+    sc.linear_velocity = np.where(kin[:, None], 0.0, np.asarray(v, np.float64))
+    sc.angular_velocity = np.where(kin[:, None], 0.0, np.asarray(w, np.float64))
+    k, dist = world.cache()
+    blob = base64.b64encode(np.ascontiguousarray(dist, dtype="<f4").tobytes()).decode("ascii")
+    save_scene(sc, path, mesh_paths,
+               extra_config={"dt": float(sc.dt), "iterations": int(sc.iterations), "sap_axis": int(world.axis())},
+               extra={"ContactCache": {"pairs": int(k), "distances_f32": blob}})
+
+
+def load_checkpoint(path: str):
+    """(World, Scene) resumed from a file save_checkpoint wrote (see there); a plain scene file
+    loads as a fresh world (axis 0, empty contact cache)."""
+    import base64
+    from .world import World
+    with open(path) as f:
+        data = json.load(f)
+    cfg = data.get("Configuration", {})
+    sc = load_scene(path, dt=float(cfg.get("dt", 1.0 / 60.0)), iterations=int(cfg.get("iterations", 10)))
+    w = World(sc)
+    w.set_axis(int(cfg.get("sap_axis", 0)))
+    cache = data.get("ContactCache")
+    if cache and int(cache.get("pairs", 0)) > 0:
+        d = np.frombuffer(base64.b64decode(cache["distances_f32"]), dtype="<f4").reshape(-1, 8)
+        if len(d) != int(cache["pairs"]):
+            raise ValueError("ContactCache: distances do not match the pair count")
+        w.set_cache(d)
+    return w, sc

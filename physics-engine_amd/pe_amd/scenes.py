@@ -415,7 +415,9 @@ def batch_worlds(n_worlds: int = 4096, lo: int = 8, hi: int = 512, seed: int = 1
 # ------------------------------------------------------------------ meshes
 def hull_mesh(points: np.ndarray) -> "Mesh":
     """Convex hull of a point cloud as a triangle mesh (qhull), faces counter-clockwise seen
-    from outside, normals the normalised face cross products."""
+    from outside, normals the normalised face cross products (n / sqrt(n . n), as
+    Mesh::perFaceNormal computes them when a scene file is loaded, so meshes survive a save and
+    load bit for bit)."""
     from scipy.spatial import ConvexHull
     h = ConvexHull(points)
     used = np.unique(h.simplices)
@@ -431,7 +433,7 @@ def hull_mesh(points: np.ndarray) -> "Mesh":
             b, c = c, b
             n = -n
         faces.append([a, b, c])
-        normals.append(n / np.linalg.norm(n))
+        normals.append(n / math.sqrt(float(n @ n)))   # as a scene file's loader computes them
     return Mesh(verts, faces, np.array(normals))
 
 
@@ -462,7 +464,7 @@ def heightfield_mesh(rng: np.random.Generator, nx: int = 64, nz: int = 64, size=
                     b, c = c, b
                     n = -n
                 faces.append([a, b, c])
-                normals.append(n / np.linalg.norm(n))
+                normals.append(n / math.sqrt(float(n @ n)))   # as a scene file's loader computes them
     return Mesh(verts, faces, np.array(normals))
 
 

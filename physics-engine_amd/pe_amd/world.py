@@ -59,6 +59,24 @@ class World:
     def step(self, n: int = 1):
         check(lib().pe_world_step(self.h, ctypes.c_int32(n)))
 
+    def step_async(self, n: int, stream):
+        """n steps enqueued on `stream` (a hipStream_t handle as an int, e.g.
+        pe_amd.hip.Stream().handle.value; 0 is the null stream); ordered after the world's pending
+        work, and its later calls wait for them."""
+        check(lib().pe_world_step_async(self.h, ctypes.c_int32(n), ctypes.c_void_p(stream)))
+
+    def cache(self):
+        """(prev_npairs, distances [prev_npairs, 8]): the contact cache the next narrowphase starts from."""
+        k = ctypes.c_int32()
+        check(lib().pe_world_get_cache(self.h, None, 0, ctypes.byref(k)))
+        d = np.zeros((max(k.value, 1), 8), np.float32)
+        check(lib().pe_world_get_cache(self.h, _f(d), ctypes.c_int32(k.value), ctypes.byref(k)))
+        return k.value, d[:k.value].copy()
+
+    def set_cache(self, distances):
+        d = np.ascontiguousarray(distances, dtype=np.float32).reshape(-1, 8)
+        check(lib().pe_world_set_cache(self.h, _f(d) if len(d) else None, ctypes.c_int32(len(d))))
+
     def integrate(self):
         check(lib().pe_world_integrate(self.h))
 

@@ -33,8 +33,8 @@ def _scenes():
 
 @pytest.mark.parametrize("name", ["rain300", "mixed200", "shapes"])
 def test_stages_equal_world_stages(name):
-    import torch
     import pe_amd
+    from pe_amd.hip import DeviceArray, synchronize
     from pe_amd.stage import Stage
     sc = _scenes()[name]
     print(f"[stage] scene={name} n={sc.n}", flush=True)
@@ -45,45 +45,44 @@ def test_stages_equal_world_stages(name):
     meshes, concave = _mesh_table(sc)
     if meshes:
         st.set_meshes(meshes, concave)
-    dev = torch.device("cuda:0")
-    points = torch.zeros((max_pairs, 8, 13), dtype=torch.float32, device=dev)
+    points = DeviceArray.from_numpy(np.zeros((max_pairs, 8, 13), np.float32))
     prev = 0
     checked = 0
     for step in range(70):
         axis = w.axis()
         w.integrate()
         w.sync()
-        aabb, body, solver = (torch.from_numpy(x).to(dev) for x in w.records())
+        aabb, body, solver = (DeviceArray.from_numpy(x) for x in w.records())
         # stage 1
         pa, pb, res = st.broadphase(aabb, axis)
         npg = w.broadphase()
-        torch.cuda.synchronize()
-        r = res.cpu().numpy()
+        synchronize()
+        r = res.numpy()
         gp = w.pairs()
         assert r[0] == npg and r[2] == 0, (step, r, npg)
         assert r[1] == w.axis(), step
-        sp = np.stack([pa[:npg].cpu().numpy(), pb[:npg].cpu().numpy()], 1)
+        sp = np.stack([pa.numpy()[:npg], pb.numpy()[:npg]], 1)
         assert np.array_equal(sp, gp), f"pairs step {step}"
         # stage 2 (the slots carry the previous step's distances in)
         npts, swap, res2 = st.narrowphase(body, pa, pb, npg, prev, points)
         w.narrowphase()
-        torch.cuda.synchronize()
+        synchronize()
         gn, gpts = w.contacts()
-        assert int(res2.cpu()[0]) == 0
-        assert np.array_equal(npts.cpu().numpy(), gn), f"point counts step {step}"
-        assert np.array_equal(points[:npg].cpu().numpy(), gpts), f"contact slots step {step}"
+        assert int(res2.numpy()[0]) == 0
+        assert np.array_equal(npts.numpy()[:npg], gn), f"point counts step {step}"
+        assert np.array_equal(points.numpy()[:npg], gpts), f"contact slots step {step}"
         prev = npg
         # stage 3
-        imp = torch.zeros((max(npg, 1), 8, 3), dtype=torch.float32, device=dev)
+        imp = DeviceArray.from_numpy(np.zeros((max(npg, 1), 8, 3), np.float32))
         res3 = st.solve(body, solver, pa, pb, npg, npts, points, sc.dt, sc.iterations, imp)
         w.solve()
         w.sync()
-        torch.cuda.synchronize()
+        synchronize()
         _, _, gv, gw = w.state()
-        sv = solver.cpu().numpy()
+        sv = solver.numpy()
         assert np.array_equal(sv[:, 17:20], gv) and np.array_equal(sv[:, 20:23], gw), f"velocities step {step}"
-        assert np.array_equal(imp[:npg].cpu().numpy(), w.impulses()), f"impulses step {step}"
-        assert int(res3.cpu()[0]) == w.counts()["colors"]
+        assert np.array_equal(imp.numpy()[:npg], w.impulses()), f"impulses step {step}"
+        assert int(res3.numpy()[0]) == w.counts()["colors"]
         checked += int(gn.sum() > 0)
     print(f"[stage] {name}: {checked} steps with contacts, last step pairs {prev}", flush=True)
     assert checked > 10
@@ -93,30 +92,32 @@ def test_stages_equal_world_stages(name):
 
 def test_stage_broadphase_small_and_flags():
     """n < 2 leaves the pairs alone and keeps the axis; ignore-collision and both-kinematic bodies
-    are in no pair; a full output reports the overflow with the true pair count."""
-    import torch
+    are in no pair; a full output reports the overflow with the true pair count.  On a stream of
+    the caller's."""
+    from pe_amd.hip import DeviceArray, Stream
     from pe_amd.stage import Stage
-    dev = torch.device("cuda:0")
     st = Stage(64, 64)
-    one = torch.tensor([[0, 0, 0, 0, 1, 1, 1, 0]], dtype=torch.float32, device=dev)
-    pa = torch.full((4,), -7, dtype=torch.int32, device=dev)
-    pb = torch.full((4,), -7, dtype=torch.int32, device=dev)
-    _, _, res = st.broadphase(one, 2, 4, pa, pb)
-    torch.cuda.synchronize()
-    assert res.cpu().tolist() == [0, 2, 0] and pa.cpu().tolist() == [-7] * 4
+    s = Stream()
+    one = DeviceArray.from_numpy(np.array([[0, 0, 0, 0, 1, 1, 1, 0]], np.float32))
+    pa = DeviceArray.full(4, -7, np.int32)
+    pb = DeviceArray.full(4, -7, np.int32)
+    _, _, res = st.broadphase(one, 2, 4, pa, pb, stream=s)
+    s.synchronize()
+    assert res.numpy().tolist() == [0, 2, 0] and pa.numpy().tolist() == [-7] * 4
     # five unit boxes overlapping in a row along x; body 1 ignores collisions, 3 and 4 kinematic
     rec = []
     for i in range(5):
         rec.append([i * 0.9, 0, 0, 1.0 if i >= 3 else 0.0, i * 0.9 + 1, 1, 1, 1.0 if i == 1 else 0.0])
-    aabb = torch.tensor(rec, dtype=torch.float32, device=dev)
-    pa, pb, res = st.broadphase(aabb, 0, 8)
-    torch.cuda.synchronize()
-    k = int(res.cpu()[0])
-    got = sorted(zip(pa[:k].cpu().tolist(), pb[:k].cpu().tolist()))
+    aabb = DeviceArray.from_numpy(np.array(rec, np.float32))
+    pa, pb, res = st.broadphase(aabb, 0, 8, stream=s)
+    s.synchronize()
+    k = int(res.numpy()[0])
+    got = sorted(zip(pa.numpy()[:k].tolist(), pb.numpy()[:k].tolist()))
     assert got == [(2, 3)], got        # (0,1), (1,2) ignored; (3,4) both kinematic
-    big = torch.tensor([[0, 0, 0, 0, 1, 1, 1, 0]] * 20, dtype=torch.float32, device=dev)
-    pa, pb, res = st.broadphase(big, 1, 50)
-    torch.cuda.synchronize()
-    r = res.cpu().tolist()
+    big = DeviceArray.from_numpy(np.array([[0, 0, 0, 0, 1, 1, 1, 0]] * 20, np.float32))
+    pa, pb, res = st.broadphase(big, 1, 50, stream=s)
+    s.synchronize()
+    r = res.numpy().tolist()
     assert r[0] == 190 and r[2] & 1, r   # 20 * 19 / 2 pairs found, 50 written
     st.close()
+    s.close()
