@@ -25,7 +25,7 @@ EXPORTS = [
     "pe_world_get_counts", "pe_world_set_solver_blocks", "pe_world_get_records", "pe_world_step_async",
     "pe_world_get_cache", "pe_world_set_cache",
     "pe_stage_create", "pe_stage_destroy", "pe_stage_set_meshes", "pe_stage_broadphase", "pe_stage_narrowphase",
-    "pe_stage_solve",
+    "pe_stage_solve", "pe_stage_broadphase_noalloc",
     "pe_batch_create", "pe_batch_destroy", "pe_batch_step", "pe_batch_sync", "pe_batch_get_world_state",
     "pe_batch_get_counts", "pe_batch_set_config", "pe_batch_get_config", "pe_batch_capture_graph",
     "pe_batch_world_footprint", "pe_batch_step_timed", "pe_batch_set_solver_groups", "pe_batch_get_solver_groups", "pe_batch_phase_profile", "pe_batch_stage_times",
