@@ -41,6 +41,11 @@ def ref_available(bits: int = 32) -> bool:
     return os.path.exists(os.path.join(HERE, "_ref", f"libperef{bits}.so"))
 
 
+def ref_amd_available() -> bool:
+    """The reference's World with the library's stages swapped in (oracle/_ref/libperef32_amd.so)."""
+    return os.path.exists(os.path.join(HERE, "_ref", "libperef32_amd.so"))
+
+
 class _Base:
     prefix = ""
 
@@ -194,15 +199,50 @@ class OracleWorld(_Base):
 
 
 class RefWorld(_Base):
-    """The reference engine itself (oracle/_ref/libperef{bits}.so), stage by stage."""
+    """The reference engine itself (oracle/_ref/libperef{bits}.so), stage by stage.  With
+    amd_stages (bit 0: broadphase, 1: narrowphase, 2: solver) the World's stages are replaced by
+    the library's through the product's reference-side adapter (FP32 build with the adapter,
+    oracle/_ref/libperef32_amd.so); it then loads libpe_amd.so."""
 
-    def __init__(self, scene, bits: int = 32, threads: int = 0):
+    def __init__(self, scene, bits: int = 32, threads: int = 0, amd_stages: int = 0):
         self.prefix = "pref"
-        lib = _load(os.path.join(HERE, "_ref", f"libperef{bits}.so"))
+        if amd_stages:
+            assert bits == 32
+            import pe_amd  # noqa: F401  (the library, loaded from its package path first)
+            pe_amd.lib()
+        lib = _load(os.path.join(HERE, "_ref", "libperef32_amd.so" if amd_stages else f"libperef{bits}.so"))
         lib.pref_init_threads(ctypes.c_int(threads))
         lib.pref_step_timed.restype = ctypes.c_double
         lib.pref_step_timed.argtypes = [_VP, ctypes.c_int]
         super().__init__(lib, scene)
+        if amd_stages:
+            lib.pref_use_amd_stages.restype = ctypes.c_int
+            lib.pref_step_checked.restype = ctypes.c_int
+            if lib.pref_use_amd_stages(ctypes.c_void_p(self.h), ctypes.c_int(amd_stages),
+                                       ctypes.c_int(scene.iterations)) != 0:
+                raise RuntimeError("the adapter could not replace the stages")
+
+    def step_checked(self, n=1):
+        """World::step through the installed stages; raises when an adapter stage throws."""
+        if self.lib.pref_step_checked(ctypes.c_void_p(self.h), ctypes.c_int(n)) != 0:
+            raise RuntimeError("World::step failed in a replaced stage")
+
+    def _checked(self, r):
+        if self.lib.pref_failed():
+            raise RuntimeError("a stage of the reference's World threw (message on stderr)")
+        return r
+
+    def integrate(self):
+        return self._checked(super().integrate())
+
+    def broadphase(self):
+        return self._checked(super().broadphase())
+
+    def narrowphase(self):
+        return self._checked(super().narrowphase())
+
+    def solve(self):
+        return self._checked(super().solve())
 
     def step_timed(self, n=1) -> float:
         return self.lib.pref_step_timed(ctypes.c_void_p(self.h), n)

@@ -19,8 +19,15 @@
 #include "utils/thread_pool.h"
 #undef private
 #include "../include/pe_amd.h"
+#ifdef PE_AMD_INTEGRATION
+// The library's stages behind the reference's stage interfaces (the product's reference-side
+// adapter, built into this driver's second variant only).
+#include "../physics-engine_amd/integration/pe_amd_stages.hpp"
+#endif
 #include <pthread.h>
+#include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
 #include <unordered_map>
 #include <chrono>
 
@@ -35,6 +42,10 @@ struct SAP : public pe_phys_collision::BroadPhaseSweepAndPrune {
 
 struct RefWorld : public pe_intf::World {
     SAP* sap = nullptr;
+#ifdef PE_AMD_INTEGRATION
+    pe_amd_integration::DeviceContext* amd = nullptr;
+    pe_amd_integration::AmdBroadPhase* amd_bp = nullptr;
+#endif
     std::unordered_map<const pe_phys_object::RigidBody*, int> index;
     std::vector<pe_phys_object::RigidBody*> owned;
     void integrate() {
@@ -193,9 +204,24 @@ REF_API void pref_world_destroy(void* h) {
     (void)h;
 }
 
+// A stage that throws (the library's stages behind the adapter report errors by exception) sets
+// the failure flag pref_failed reads, with the message on stderr, instead of ending the process.
+namespace {
+int g_failed = 0;
+template <class F> auto guarded(const char* what, F f) -> decltype(f()) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        fprintf(stderr, "%s: %s\n", what, e.what());
+        g_failed = 1;
+        return decltype(f())();
+    }
+}
+}  // namespace
+REF_API int pref_failed() { const int f = g_failed; g_failed = 0; return f; }
 REF_API void pref_step(void* h, int nsteps) {
     auto* w = (RefWorld*)h;
-    for (int i = 0; i < nsteps; i++) w->step();
+    guarded("step", [&] { for (int i = 0; i < nsteps; i++) w->step(); });
 }
 // Wall-clock seconds of nsteps World::step calls.
 REF_API double pref_step_timed(void* h, int nsteps) {
@@ -205,12 +231,59 @@ REF_API double pref_step_timed(void* h, int nsteps) {
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }
-REF_API void pref_integrate(void* h) { ((RefWorld*)h)->integrate(); }
-REF_API int pref_broadphase(void* h) { return ((RefWorld*)h)->broadphase(); }
-REF_API void pref_narrowphase(void* h) { ((RefWorld*)h)->narrowphase(); }
-REF_API void pref_solve(void* h) { ((RefWorld*)h)->solve(); }
-REF_API int pref_get_axis(void* h) { return ((RefWorld*)h)->sap->axis(); }
-REF_API void pref_set_axis(void* h, int a) { ((RefWorld*)h)->sap->set_axis(a); }
+REF_API void pref_integrate(void* h) { guarded("integrate", [&] { ((RefWorld*)h)->integrate(); }); }
+REF_API int pref_broadphase(void* h) { return guarded("broadphase", [&] { return ((RefWorld*)h)->broadphase(); }); }
+REF_API void pref_narrowphase(void* h) { guarded("narrowphase", [&] { ((RefWorld*)h)->narrowphase(); }); }
+REF_API void pref_solve(void* h) { guarded("solve", [&] { ((RefWorld*)h)->solve(); }); }
+REF_API int pref_get_axis(void* h) {
+#ifdef PE_AMD_INTEGRATION
+    if (((RefWorld*)h)->amd_bp) return ((RefWorld*)h)->amd_bp->targetAxis();
+#endif
+    return ((RefWorld*)h)->sap->axis();
+}
+REF_API void pref_set_axis(void* h, int a) {
+#ifdef PE_AMD_INTEGRATION
+    if (((RefWorld*)h)->amd_bp) ((RefWorld*)h)->amd_bp->setTargetAxis(a);
+#endif
+    ((RefWorld*)h)->sap->set_axis(a);
+}
+
+#ifdef PE_AMD_INTEGRATION
+// Replaces the World's stages by the library's (mask bit 0: broadphase, 1: narrowphase, 2:
+// solver) through World::setBroadPhase / setNarrowPhase / setConstraintSolver, as a host
+// application would.  Returns 0, or -1 with the message in *err when the adapter throws.
+REF_API int pref_use_amd_stages(void* h, int mask, int iterations) {
+    auto* w = (RefWorld*)h;
+    try {
+        if (!w->amd) w->amd = new pe_amd_integration::DeviceContext();
+        if (mask & 1) {
+            w->amd_bp = new pe_amd_integration::AmdBroadPhase(*w->amd);
+            w->amd_bp->setTargetAxis(w->sap->axis());
+            w->setBroadPhase(w->amd_bp);
+        }
+        if (mask & 2) w->setNarrowPhase(new pe_amd_integration::AmdNarrowPhase(*w->amd));
+        if (mask & 4) {
+            auto* s = new pe_amd_integration::AmdSequentialImpulseSolver(*w->amd);
+            s->setIteration(iterations);
+            w->setConstraintSolver(s);
+        }
+    } catch (const std::exception& e) {
+        fprintf(stderr, "pref_use_amd_stages: %s\n", e.what());
+        return -1;
+    }
+    return 0;
+}
+// World::step through whatever stages are installed; -1 when an adapter stage throws.
+REF_API int pref_step_checked(void* h, int nsteps) {
+    try {
+        for (int i = 0; i < nsteps; i++) ((RefWorld*)h)->step();
+    } catch (const std::exception& e) {
+        fprintf(stderr, "pref_step_checked: %s\n", e.what());
+        return -1;
+    }
+    return 0;
+}
+#endif
 REF_API int pref_num_pairs(void* h) { return (int)((RefWorld*)h)->pairs().size(); }
 REF_API void pref_get_pairs(void* h, int32_t* ab) {
     auto* w = (RefWorld*)h;

@@ -874,6 +874,7 @@ void launch_shapes(const BatchDev& d, const std::vector<Class>& classes, hipStre
 // own event pair (stage 3 + 2 * class + 0 or 1; the shape kernels run untimed between the two).
 int launch_all(const BatchDev& d, const std::vector<Class>& classes, const int32_t* dlist, const int2* dgroups, hipStream_t s,
                const ClassStreams& cs, bool shapes, LaunchTimer* tm = nullptr, bool class_tm = false) {
+    pe_clear_stale_error();
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, d);
     std::vector<int> ord;
     int nused = 0;

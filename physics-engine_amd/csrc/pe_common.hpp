@@ -92,6 +92,10 @@ struct BodyView {
 
 // ------------------------------------------------------------------ host error helpers
 void pe_set_error(const std::string& msg);
+// HIP keeps the last error of any runtime call per host thread until hipGetLastError reads it.  An
+// entry point that checks hipGetLastError after its own launches first drops what earlier calls
+// left there (their own return codes reported it), so it reports only its own launches.
+inline void pe_clear_stale_error() { (void)hipGetLastError(); }
 #define PE_HIP(expr)                                                                     \
     do {                                                                                 \
         hipError_t _e = (expr);                                                          \

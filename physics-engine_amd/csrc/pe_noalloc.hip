@@ -95,6 +95,7 @@ extern "C" int pe_stage_broadphase_noalloc(int32_t n, const float* aabb, int32_t
         pe_set_error("pe_stage_broadphase_noalloc: invalid argument");
         return PE_ERR_INVALID;
     }
+    pe_clear_stale_error();
     unsigned char* base = nullptr;
     if (int rc = scratch_address(&base)) return rc;
     SapView s{};

@@ -263,6 +263,7 @@ struct pe_world {
         return PE_OK;
     }
     int step() {
+        pe_clear_stale_error();
         if (timing) hipEventRecord(ev[0], stream);
         launch_integrate();
         if (timing) hipEventRecord(ev[1], stream);
@@ -474,12 +475,14 @@ int pe_world_set_cache(pe_world* w, const float* distances, int32_t prev_npairs)
 }
 int pe_world_integrate(pe_world* w) {
     if (int rc = check_world(w)) return rc;
+    pe_clear_stale_error();
     w->launch_integrate();
     PE_HIP(hipGetLastError());
     return PE_OK;
 }
 int pe_world_broadphase(pe_world* w, int32_t* npairs) {
     if (int rc = check_world(w)) return rc;
+    pe_clear_stale_error();
     w->launch_broadphase();
     PE_HIP(hipGetLastError());
     if (npairs) {
@@ -493,12 +496,14 @@ int pe_world_broadphase(pe_world* w, int32_t* npairs) {
 }
 int pe_world_narrowphase(pe_world* w) {
     if (int rc = check_world(w)) return rc;
+    pe_clear_stale_error();
     w->launch_narrowphase();
     PE_HIP(hipGetLastError());
     return PE_OK;
 }
 int pe_world_solve(pe_world* w) {
     if (int rc = check_world(w)) return rc;
+    pe_clear_stale_error();
     w->launch_solve();
     PE_HIP(hipGetLastError());
     return PE_OK;
@@ -888,6 +893,7 @@ int pe_stage_broadphase(pe_stage* s, int32_t n, const float* aabb, int32_t axis,
         pe_set_error("pe_stage_broadphase: invalid argument");
         return PE_ERR_INVALID;
     }
+    pe_clear_stale_error();
     hipStream_t st = (hipStream_t)stream;
     const SapView v = sap_over_records(s->w->d.sap, n, aabb, pair_a, pair_b, cap);
     hipLaunchKernelGGL(k_sap_begin, dim3(1), dim3(1), 0, st, v, (int)axis);
@@ -905,6 +911,7 @@ int pe_stage_narrowphase(pe_stage* s, int32_t n, const float* body, int32_t npai
         pe_set_error("pe_stage_narrowphase: invalid argument");
         return PE_ERR_INVALID;
     }
+    pe_clear_stale_error();
     pe_world* w = s->w;
     hipStream_t keep = w->stream;
     w->stream = (hipStream_t)stream;
@@ -931,6 +938,7 @@ int pe_stage_solve(pe_stage* s, int32_t n, const float* body, float* solver_body
         pe_set_error("pe_stage_solve: invalid argument");
         return PE_ERR_INVALID;
     }
+    pe_clear_stale_error();
     pe_world* w = s->w;
     hipStream_t keep = w->stream;
     w->stream = (hipStream_t)stream;

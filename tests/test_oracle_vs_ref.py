@@ -138,3 +138,18 @@ def test_reference_sample_scene_bit_exact(bits):
     sc = reference_sample_scene(os.path.join(os.path.dirname(__file__), "golden", "bomb_crash_tower_smaller.json"))
     assert sc.n == 17 and np.any(sc.restitution > 0) and np.any(sc.angular_damping > 0)
     _run(sc, bits, 150, exact=True)
+
+
+def test_adapter_host_library_exports_its_entry_points():
+    """The reference's World with the adapter (built beside the reference, linked to libpe_amd.so)
+    loads without a GPU and exports the swap and checked-step entry points."""
+    import ctypes
+    import os
+    from pyoracle import HERE, ref_amd_available
+    if not ref_amd_available():
+        pytest.skip("oracle/_ref/libperef32_amd.so not built")
+    import pe_amd
+    pe_amd.lib()
+    lib = ctypes.CDLL(os.path.join(HERE, "_ref", "libperef32_amd.so"))
+    for name in ("pref_use_amd_stages", "pref_step_checked", "pref_world_create", "pref_step"):
+        assert hasattr(lib, name), name
