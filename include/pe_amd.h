@@ -78,7 +78,26 @@ typedef struct pe_scene_desc {
     int32_t n_meshes;                 /* meshes of convex / concave bodies (0 if none) */
     const pe_mesh_desc* meshes;       /* [n_meshes] */
     const int32_t* mesh_index;        /* [n] mesh of body i if it is convex / concave; NULL if n_meshes = 0 */
+    const int32_t* ignore_collision;  /* [n] RigidBody::setIgnoreCollision (0/1: the body is in no pair), or NULL */
 } pe_scene_desc;
+
+/* Joint constraints (src/rigid/constraint/constraint/{ball,hinge,slider}_joint_constraint.h), as a
+ * reference user configures them before World::addConstraint (world.h:90): the two bodies
+ * (Constraint::setObjectA / setObjectB), the body-local anchors and axes, and for a hinge its
+ * angle limits and motor.  Values of pe_phys_constraint::ConstraintType. */
+#define PE_JOINT_BALL   0
+#define PE_JOINT_HINGE  1
+#define PE_JOINT_SLIDER 2
+typedef struct pe_joint_desc {
+    int32_t type;                     /* PE_JOINT_* */
+    int32_t body_a, body_b;           /* body indices */
+    int32_t use_limits, use_motor;    /* hinge: setUseLimits, setUseMotor */
+    int32_t reserved;
+    double anchor_a[3], anchor_b[3];  /* setAnchorA / setAnchorB (body-local) */
+    double axis_a[3], axis_b[3];      /* setAxisA / setAxisB (hinge, slider; body-local) */
+    double min_angle, max_angle;      /* hinge: setMinAngle / setMaxAngle (radians) */
+    double target_speed;              /* hinge: setTargetSpeed */
+} pe_joint_desc;
 
 /* World settings: World::setDt / setGravity (src/intf/world.h:24-25) and
  * SequentialImpulseSolver::setIteration (sequential_impulse_solver.cpp:8). */
@@ -137,6 +156,17 @@ int pe_world_broadphase(pe_world* w, int32_t* npairs);
 int pe_world_narrowphase(pe_world* w);
 /* ConstraintSolver::setupSolver + solve (sequential_impulse_solver.cpp:11-81) */
 int pe_world_solve(pe_world* w);
+
+/* World::addConstraint (world.h:90) for each of n joints (appended to the world's joints, in this
+ * order): they are set up with the contacts and swept after them in every Gauss-Seidel iteration
+ * (sequential_impulse_solver.cpp:49-51, 66-68), in an order grouped by a greedy colouring of the joints
+ * (joint k takes the lowest colour no earlier joint sharing a dynamic body of it holds; joints of one
+ * colour share no dynamic body, so the sweep equals the sequential one over that order).  At most
+ * 65536 joints per world. */
+int pe_world_add_joints(pe_world* w, int32_t n, const pe_joint_desc* joints);
+/* RigidBody::addIgnoreCollisionId for n body pairs [2n]: the two bodies of a pair never form a
+ * broadphase pair (broad_phase_base.cpp:12-13), as for the bodies a joint connects. */
+int pe_world_add_ignore_pairs(pe_world* w, int32_t n, const int32_t* pairs);
 
 /* Synchronises the world's stream, then reports capacity errors raised by the steps since the
  * last report (PE_ERR_CAPACITY, the causes in pe_last_error): more broadphase pairs than max_pairs
@@ -197,7 +227,9 @@ int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
  * at a time (the C++ classes implementing the stage interfaces over them are in INTEGRATION.md).
  * All buffers are device memory.  A pe_stage holds the working memory; one host thread uses a
  * pe_stage at a time.  Record layouts (FP32, 16-byte aligned rows):
- *   AABB record      [8]  min x y z, kinematic (0 / 1), max x y z, ignore-collision (0 / 1)
+ *   AABB record      [8]  min x y z, kinematic (0 / 1), max x y z, collision flag (0: normal,
+ *                         1: setIgnoreCollision, in no pair; a world's records hold 2 for a body
+ *                         with ignore partners, which the stage entry points treat as 0)
  *   collision record [20] basis (9, row-major), origin (3), AABB scale (RigidBody::getAABBScale),
  *                         shape parameters (3: box size / sphere radius / cylinder radius, height /
  *                         mesh id of pe_stage_set_meshes), shape type (PE_SHAPE_*, int bits),

@@ -69,6 +69,7 @@ template <class R> Handle<R>* create(const pe_scene_desc* sc, const pe_world_par
         b.lin_damp = sc->linear_damping ? R(sc->linear_damping[i]) : R(0);
         b.ang_damp = sc->angular_damping ? R(sc->angular_damping[i]) : R(0);
         b.kinematic = sc->kinematic[i] != 0;
+        b.ignore = sc->ignore_collision && sc->ignore_collision[i] != 0;
     }
     return h;
 }
@@ -141,6 +142,26 @@ template <class R> int get_poly(Handle<R>* h, int body, int32_t* counts, double*
         return create<R>(sc, prm);                                                                       \
     }                                                                                                    \
     extern "C" void P##_world_destroy(void* h) { delete (Handle<R>*)h; }                                \
+    extern "C" void P##_add_joint(void* h, const pe_joint_desc* d) {                                     \
+        auto& w = ((Handle<R>*)h)->w;                                                                    \
+        Joint<R> j;                                                                                      \
+        j.type = d->type; j.a = d->body_a; j.b = d->body_b;                                             \
+        for (int k = 0; k < 3; k++) {                                                                    \
+            j.anchor_a[k] = R(d->anchor_a[k]); j.anchor_b[k] = R(d->anchor_b[k]);                       \
+            j.axis_a[k] = R(d->axis_a[k]); j.axis_b[k] = R(d->axis_b[k]);                               \
+        }                                                                                                \
+        j.use_limits = d->use_limits != 0; j.use_motor = d->use_motor != 0;                             \
+        j.min_angle = R(d->min_angle); j.max_angle = R(d->max_angle); j.target_speed = R(d->target_speed); \
+        w.joints.push_back(j);                                                                           \
+    }                                                                                                    \
+    extern "C" void P##_add_ignore(void* h, int a, int b) {                                              \
+        ((Handle<R>*)h)->w.ignore.insert(std::make_pair(std::min(a, b), std::max(a, b)));               \
+    }                                                                                                    \
+    extern "C" int P##_get_joint_order(void* h, int32_t* out) {                                          \
+        auto& w = ((Handle<R>*)h)->w;                                                                    \
+        for (size_t k = 0; k < w.joint_order.size(); k++) out[k] = w.joint_order[k];                    \
+        return (int)w.joint_order.size();                                                                \
+    }                                                                                                    \
     extern "C" void P##_set_color_order(void* h, int on) { ((Handle<R>*)h)->w.use_color_order = on != 0; } \
     extern "C" void P##_step(void* h, int nsteps) {                                                      \
         auto* hh = (Handle<R>*)h;                                                                        \

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <utility>
 #include <memory>
+#include <set>
 
 namespace pe_oracle {
 
@@ -355,8 +356,12 @@ template <class R> struct ContactResult {
 };
 
 // ---------------------------------------------------------------- world
+template <class R> struct Joint;
 template <class R> struct World {
     std::vector<Body<R>> bodies;
+    std::vector<Joint<R>> joints;             // World::addConstraint order
+    std::vector<int> joint_order;             // joint sweep order (colour order when use_color_order)
+    std::set<std::pair<int, int>> ignore;     // body pairs (lo, hi) that never collide (addIgnoreCollisionId)
     V3<R> gravity{0, R(-9.8), 0};
     R dt = R(0.01);
     int iterations = 10;
@@ -838,7 +843,9 @@ template <class R> inline void stage_broadphase(World<R>& w) {
         for (int j = i + 1; j < n; j++) {
             const Body<R>& b2 = w.bodies[order[j]];
             if (b2.aabb_min[ax] > b1.aabb_max[ax]) break;
-            if (validate_pair(b1, b2)) w.pairs.emplace_back(order[i], order[j]);
+            if (validate_pair(b1, b2) &&
+                !w.ignore.count(std::make_pair(std::min(order[i], order[j]), std::max(order[i], order[j]))))
+                w.pairs.emplace_back(order[i], order[j]);
         }
     }
     const V3<R> v = s2 - V3<R>(s[0] * s[0], s[1] * s[1], s[2] * s[2]) / R(n);
@@ -1040,6 +1047,8 @@ template <class R> inline void stage_solver_setup(World<R>& w, Solver<R>& s) {
             ci.mu = cr.mu;
         }
     }
+    for (auto& j : w.joints) joint_setup(w, j);
+    compute_joint_order(w);
 }
 // RigidBody::applyTempImpulse (rigidbody.cpp:166-170)
 template <class R> inline void apply_temp_impulse(Body<R>& b, const V3<R>& r, const V3<R>& J) {
@@ -1076,11 +1085,203 @@ template <class R> inline void stage_solve(World<R>& w, Solver<R>& s) {
         } else {
             for (size_t i = 0; i < w.results.size(); i++) iterate_rows(w, w.results[i], s.rows[i]);
         }
+        for (int k : w.joint_order) joint_iterate(w, w.joints[k]);
     }
     for (auto& b : w.bodies) {
         if (b.kinematic) continue;
         b.v = b.tv; b.w = b.tw;
     }
+}
+
+// ---------------------------------------------------------------- joint constraints
+// BallJointConstraint, HingeJointConstraint, SliderJointConstraint
+// (constraint/{ball,hinge,slider}_joint_constraint.cpp): set up in setupSolver after the contacts,
+// iterated after the contacts in every Gauss-Seidel sweep (sequential_impulse_solver.cpp:49-51, 66-68).
+enum { JT_BALL = 0, JT_HINGE = 1, JT_SLIDER = 2 };
+template <class R> struct Joint {
+    int type = JT_BALL, a = 0, b = 0;
+    V3<R> anchor_a, anchor_b, axis_a{1, 0, 0}, axis_b{1, 0, 0};
+    bool use_limits = false, use_motor = false;
+    R min_angle = 0, max_angle = 0, target_speed = 0;
+    // initSequentialImpulse state
+    V3<R> ra, rb, wax_a, wax_b, wt_a[2], wt_b[2];
+    M3<R> jmj_ball, jmj_rot;
+    V3<R> rhs_ball, rhs_rot;
+    R rhs_hinge[2] = {0, 0}, jmj_hinge[2] = {0, 0}, rhs_pos[2] = {0, 0}, jmj_pos[2] = {0, 0};
+    R rhs_motor = 0, rhs_limit = 0, jmj_ml = 0;
+    bool limit_exceeded = false;
+};
+
+// Constraint::getSkewSymmetricMatrix (constraint.cpp:9-13)
+template <class R> inline M3<R> skew(const V3<R>& v) {
+    M3<R> m;
+    m(0, 0) = 0; m(0, 1) = -v[2]; m(0, 2) = v[1];
+    m(1, 0) = v[2]; m(1, 1) = 0; m(1, 2) = -v[0];
+    m(2, 0) = -v[1]; m(2, 1) = v[0]; m(2, 2) = 0;
+    return m;
+}
+// rx * I * rx^T with the second product summed as in (R * L) * R^T (body_update_world_inertia)
+template <class R> inline M3<R> sandwich(const M3<R>& rx, const M3<R>& I) {
+    const M3<R> T = mm(rx, I);
+    M3<R> r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r(i, j) = T(i, 0) * rx(j, 0) + (T(i, 1) * rx(j, 1) + T(i, 2) * rx(j, 2));
+    return r;
+}
+template <class R> inline M3<R> madd(const M3<R>& A, const M3<R>& B) {
+    M3<R> r;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r(i, j) = A(i, j) + B(i, j);
+    return r;
+}
+// (I * (ima + imb) + rxA Ia rxA^T + rxB Ib rxB^T)^-1, the ball part's effective mass
+template <class R> inline M3<R> ball_jmj(R ims, const V3<R>& ra, const M3<R>& Ia, const V3<R>& rb, const M3<R>& Ib) {
+    M3<R> S;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) S(i, j) = (i == j ? R(1) : R(0)) * ims;
+    return inverse3(madd(madd(S, sandwich(skew(ra), Ia)), sandwich(skew(rb), Ib)));
+}
+template <class R> inline R kin_inv_mass(const Body<R>& b) { return b.kinematic ? R(0) : b.inv_mass; }
+template <class R> inline M3<R> kin_inv_inertia(const Body<R>& b) { return b.kinematic ? M3<R>() : b.world_inv_inertia; }
+// RigidBody::applyTempAngularImpulse (rigidbody.cpp:178-181)
+template <class R> inline void apply_temp_angular(Body<R>& b, const V3<R>& J) {
+    if (b.kinematic) return;
+    b.tw = b.tw + mv(b.world_inv_inertia, J);
+}
+
+template <class R> inline void joint_setup(World<R>& w, Joint<R>& j) {
+    const Body<R>& A = w.bodies[j.a];
+    const Body<R>& B = w.bodies[j.b];
+    const R kerp = R(0.2), dt = w.dt;
+    j.ra = mv(A.xf.basis, j.anchor_a);
+    j.rb = mv(B.xf.basis, j.anchor_b);
+    const R ims = kin_inv_mass(A) + kin_inv_mass(B);
+    const M3<R> Ia = kin_inv_inertia(A), Ib = kin_inv_inertia(B);
+    const V3<R> rel = A.xf.apply(j.anchor_a) - B.xf.apply(j.anchor_b);
+    if (j.type == JT_BALL || j.type == JT_HINGE) {   // ball_joint_constraint.cpp:6-19, hinge_joint_constraint.cpp:25-34
+        j.jmj_ball = ball_jmj(ims, j.ra, Ia, j.rb, Ib);
+        j.rhs_ball = mv(j.jmj_ball, rel) * (-kerp / dt);
+    }
+    if (j.type == JT_HINGE || j.type == JT_SLIDER) {
+        j.wax_a = mv(A.xf.basis, j.axis_a);
+        j.wax_b = mv(B.xf.basis, j.axis_b);
+        ortho_units(j.axis_a, j.wt_a[0], j.wt_a[1]);
+        j.wt_a[0] = mv(A.xf.basis, j.wt_a[0]);
+        j.wt_a[1] = mv(A.xf.basis, j.wt_a[1]);
+        if (j.type == JT_SLIDER || j.use_limits) {
+            ortho_units(j.axis_b, j.wt_b[0], j.wt_b[1]);
+            j.wt_b[0] = mv(B.xf.basis, j.wt_b[0]);
+            j.wt_b[1] = mv(B.xf.basis, j.wt_b[1]);
+        }
+    }
+    if (j.type == JT_HINGE) {   // hinge_joint_constraint.cpp:36-70
+        const M3<R> Is = madd(Ia, Ib);
+        j.jmj_hinge[0] = R(1.0) / dot(j.wt_a[0], mv(Is, j.wt_a[0]));
+        j.jmj_hinge[1] = R(1.0) / dot(j.wt_a[1], mv(Is, j.wt_a[1]));
+        const V3<R> u = cross(j.wax_a, j.wax_b);
+        j.rhs_hinge[0] = j.jmj_hinge[0] * (dot(u, j.wt_a[0]) * kerp / dt);
+        j.rhs_hinge[1] = j.jmj_hinge[1] * (dot(u, j.wt_a[1]) * kerp / dt);
+        if (j.use_motor || j.use_limits) j.jmj_ml = R(1.0) / dot(j.wax_a, mv(Is, j.wax_a));
+        if (j.use_motor) j.rhs_motor = j.jmj_ml * -j.target_speed;
+        if (j.use_limits) {
+            const R ac = dot(j.wt_a[0], j.wt_b[0]);
+            const R as = dot(cross(j.wt_a[0], j.wt_b[0]), j.wax_a);
+            const R angle = std::atan2(as, ac);
+            j.limit_exceeded = false;
+            if (angle < j.min_angle) {
+                j.limit_exceeded = true;
+                j.rhs_limit = -j.jmj_ml * (j.min_angle - angle) * kerp / dt;
+            } else if (angle > j.max_angle) {
+                j.limit_exceeded = true;
+                j.rhs_limit = -j.jmj_ml * (j.max_angle - angle) * kerp / dt;
+            }
+        }
+    }
+    if (j.type == JT_SLIDER) {   // slider_joint_constraint.cpp:24-53
+        auto jmj_dir = [&](const V3<R>& d) {
+            const V3<R> xa = cross(j.ra, d), xb = cross(j.rb, d);
+            return R(1.0) / (ims + dot(mv(Ia, xa), xa) + dot(mv(Ib, xb), xb));
+        };
+        j.jmj_pos[0] = jmj_dir(j.wt_a[0]);
+        j.jmj_pos[1] = jmj_dir(j.wt_a[1]);
+        // the reference scales the second row by the first row's effective mass (:45)
+        j.rhs_pos[0] = j.jmj_pos[0] * (dot(rel, j.wt_a[0]) * (-kerp / dt));
+        j.rhs_pos[1] = j.jmj_pos[0] * (dot(rel, j.wt_a[1]) * (-kerp / dt));
+        j.jmj_rot = inverse3(madd(Ia, Ib));
+        j.rhs_rot = mv(j.jmj_rot, cross(j.wt_a[0], j.wt_b[0]) + cross(j.wt_a[1], j.wt_b[1])) * (kerp / dt);
+    }
+}
+
+template <class R> inline void joint_iterate(World<R>& w, Joint<R>& j) {
+    Body<R>& A = w.bodies[j.a];
+    Body<R>& B = w.bodies[j.b];
+    if (j.type == JT_BALL || j.type == JT_HINGE) {
+        const V3<R> va = A.tv + cross(A.tw, j.ra);
+        const V3<R> vb = B.tv + cross(B.tw, j.rb);
+        const V3<R> imp = j.rhs_ball - mv(j.jmj_ball, va - vb);
+        apply_temp_impulse(A, j.ra, imp);
+        apply_temp_impulse(B, j.rb, -imp);
+    }
+    if (j.type == JT_HINGE) {
+        // the angular velocities are read live: each impulse sees the ones applied before it
+        const V3<R> dw = A.tw - B.tw;
+        const R h0 = j.rhs_hinge[0] - j.jmj_hinge[0] * dot(j.wt_a[0], dw);
+        const R h1 = j.rhs_hinge[1] - j.jmj_hinge[1] * dot(j.wt_a[1], dw);
+        const V3<R> iv = j.wt_a[0] * h0 + j.wt_a[1] * h1;
+        apply_temp_angular(A, iv);
+        apply_temp_angular(B, -iv);
+        if (j.use_motor) {
+            const R m = j.rhs_motor - j.jmj_ml * dot(j.wax_a, A.tw - B.tw);
+            apply_temp_angular(A, j.wax_a * m);
+            apply_temp_angular(B, -j.wax_a * m);
+        }
+        if (j.use_limits && j.limit_exceeded) {
+            const R delta = dot(j.wax_a, A.tw - B.tw);
+            const R nd = delta * j.rhs_limit > 0 ? R(0) : delta;
+            const R l = j.rhs_limit - j.jmj_ml * nd;
+            apply_temp_angular(A, j.wax_a * l);
+            apply_temp_angular(B, -j.wax_a * l);
+        }
+    }
+    if (j.type == JT_SLIDER) {
+        const V3<R> va = A.tv + cross(A.tw, j.ra);
+        const V3<R> vb = B.tv + cross(B.tw, j.rb);
+        const R p0 = j.rhs_pos[0] - j.jmj_pos[0] * dot(j.wt_a[0], va - vb);
+        const R p1 = j.rhs_pos[1] - j.jmj_pos[1] * dot(j.wt_a[1], va - vb);
+        const V3<R> pv = j.wt_a[0] * p0 + j.wt_a[1] * p1;
+        apply_temp_impulse(A, j.ra, pv);
+        apply_temp_impulse(B, j.rb, -pv);
+        const V3<R> rot = -mv(j.jmj_rot, A.tw - B.tw);
+        apply_temp_angular(A, rot);
+        apply_temp_angular(B, -rot);
+    }
+}
+
+// Joint sweep order: with use_color_order, joints grouped by a greedy colouring in joint order
+// (joint k takes the lowest colour no earlier joint sharing one of its dynamic bodies holds); the
+// device library colours its joints by the same rule.  Joints of one colour share no dynamic body.
+template <class R> inline void compute_joint_order(World<R>& w) {
+    const int nj = (int)w.joints.size();
+    w.joint_order.resize(nj);
+    if (!w.use_color_order) {
+        for (int k = 0; k < nj; k++) w.joint_order[k] = k;
+        return;
+    }
+    std::vector<int> col(nj, 0);
+    int ncol = 0;
+    for (int k = 0; k < nj; k++) {
+        std::set<int> used;
+        for (int q = 0; q < k; q++) {
+            const auto dyn_shared = [&](int x) { return !w.bodies[x].kinematic && (x == w.joints[q].a || x == w.joints[q].b); };
+            if (dyn_shared(w.joints[k].a) || dyn_shared(w.joints[k].b)) used.insert(col[q]);
+        }
+        int c = 0;
+        while (used.count(c)) c++;
+        col[k] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    int o = 0;
+    for (int c = 0; c < ncol; c++)
+        for (int k = 0; k < nj; k++)
+            if (col[k] == c) w.joint_order[o++] = k;
 }
 
 // World::step (world.cpp:267-300)

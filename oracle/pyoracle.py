@@ -61,6 +61,10 @@ class _Base:
         self.h = f("world_create")(ctypes.byref(self._desc), ctypes.byref(self._params))
         if not self.h:
             raise ValueError("world_create failed (unsupported shape?)")
+        for a, b in getattr(scene, "ignore_pairs", None) or []:
+            f("add_ignore")(ctypes.c_void_p(self.h), ctypes.c_int(a), ctypes.c_int(b))
+        for j in getattr(scene, "joints", None) or []:
+            f("add_joint")(ctypes.c_void_p(self.h), ctypes.byref(j))
         for name in ("broadphase", "num_pairs", "get_axis", "num_results"):
             f(name).restype = ctypes.c_int
             f(name).argtypes = [_VP]
@@ -183,6 +187,11 @@ class OracleWorld(_Base):
 
     def num_colors(self):
         return self._f("num_colors")(ctypes.c_void_p(self.h))
+
+    def joint_order(self):
+        out = np.zeros(max(1, len(getattr(self.scene, "joints", None) or [])), np.int32)
+        k = self._f("get_joint_order")(ctypes.c_void_p(self.h), _i(out))
+        return out[:k].copy()
 
     def impulses(self):
         n = self._f("num_results")(ctypes.c_void_p(self.h))

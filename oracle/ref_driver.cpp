@@ -14,6 +14,9 @@
 #include "rigid/shape/concave_mesh_shape.h"
 #include "rigid/collision/broad_phase/broad_phase_sweep_and_prune.h"
 #include "rigid/constraint/constraint_solver/sequential_impulse_solver.h"
+#include "rigid/constraint/constraint/ball_joint_constraint.h"
+#include "rigid/constraint/constraint/hinge_joint_constraint.h"
+#include "rigid/constraint/constraint/slider_joint_constraint.h"
 // The pool's members are private; this test driver needs them to retire the workers at exit.
 #define private public
 #include "utils/thread_pool.h"
@@ -191,11 +194,52 @@ REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* 
         rb->setLinearDamping(sc->linear_damping ? pe::Real(sc->linear_damping[i]) : pe::Real(0));
         rb->setAngularDamping(sc->angular_damping ? pe::Real(sc->angular_damping[i]) : pe::Real(0));
         rb->setKinematic(sc->kinematic[i] != 0);
+        rb->setIgnoreCollision(sc->ignore_collision && sc->ignore_collision[i] != 0);
         w->addRigidBody(rb);
         w->index[rb] = i;
         w->owned.push_back(rb);
     }
     return w;
+}
+
+namespace {
+pe::Vector3 vec_of(const double* v) { return pe::Vector3(pe::Real(v[0]), pe::Real(v[1]), pe::Real(v[2])); }
+}  // namespace
+// World::addConstraint of a ball / hinge / slider joint configured from a pe_joint_desc, as the
+// reference's constraints demo configures them (examples/constraints_demo/main.cpp:46-163).
+REF_API int pref_add_joint(void* h, const pe_joint_desc* d) {
+    auto* w = (RefWorld*)h;
+    pe_phys_constraint::Constraint* c = nullptr;
+    if (d->type == PE_JOINT_BALL) {
+        auto* j = new pe_phys_constraint::BallJointConstraint();
+        j->setAnchorA(vec_of(d->anchor_a)); j->setAnchorB(vec_of(d->anchor_b));
+        c = j;
+    } else if (d->type == PE_JOINT_HINGE) {
+        auto* j = new pe_phys_constraint::HingeJointConstraint();
+        j->setAnchorA(vec_of(d->anchor_a)); j->setAnchorB(vec_of(d->anchor_b));
+        j->setAxisA(vec_of(d->axis_a)); j->setAxisB(vec_of(d->axis_b));
+        j->setUseLimits(d->use_limits != 0);
+        j->setMinAngle(pe::Real(d->min_angle)); j->setMaxAngle(pe::Real(d->max_angle));
+        j->setUseMotor(d->use_motor != 0);
+        j->setTargetSpeed(pe::Real(d->target_speed));
+        c = j;
+    } else if (d->type == PE_JOINT_SLIDER) {
+        auto* j = new pe_phys_constraint::SliderJointConstraint();
+        j->setAnchorA(vec_of(d->anchor_a)); j->setAnchorB(vec_of(d->anchor_b));
+        j->setAxisA(vec_of(d->axis_a)); j->setAxisB(vec_of(d->axis_b));
+        c = j;
+    } else {
+        return -1;
+    }
+    c->setObjectA(w->owned[d->body_a]);
+    c->setObjectB(w->owned[d->body_b]);
+    w->addConstraint(c);
+    return 0;
+}
+// RigidBody::addIgnoreCollisionId: bodies a and b never form a pair (broad_phase_base.cpp:12-13).
+REF_API void pref_add_ignore(void* h, int a, int b) {
+    auto* w = (RefWorld*)h;
+    w->owned[a]->addIgnoreCollisionId(w->owned[b]->getGlobalId());
 }
 
 REF_API void pref_world_destroy(void* h) {

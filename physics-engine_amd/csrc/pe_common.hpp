@@ -19,7 +19,8 @@ enum BodyField : int {
     F_IM = 33,     // inverse mass
     F_MASS = 34,   // mass
     F_LINV = 35,   // 9: local inverse inertia
-    F_SP = 44,     // 4: shape parameters (box size / sphere radius / cylinder radius, height)
+    F_SP = 44,     // 4: shape parameters (box size / sphere radius / cylinder radius, height / mesh id),
+                   //    then the collision flag (COLL_*)
     F_MU = 48,     // friction
     F_E = 49,      // restitution
     F_LD = 50,     // linear damping
@@ -30,6 +31,13 @@ enum BodyField : int {
     NF = 59
 };
 enum BodyIntField : int { I_TYPE = 0, I_KIN = 1, NI = 2 };
+// Collision flag of a body, in its fourth shape parameter and in the last float of its AABB record:
+// COLL_IGNORE (RigidBody::setIgnoreCollision) puts it in no pair; COLL_PARTNERS marks a body that some
+// other body ignores (addIgnoreCollisionId), whose candidate pairs are looked up in the world's
+// ignore-pair list.
+constexpr float COLL_NORMAL = 0.0f;
+constexpr float COLL_IGNORE = 1.0f;
+constexpr float COLL_PARTNERS = 2.0f;
 
 // The narrowphase and the solver visit bodies, contact slots and rows in pair order, i.e. at
 // random; for them each record is contiguous (array of structures, float4-aligned) so one

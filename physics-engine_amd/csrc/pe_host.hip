@@ -345,7 +345,9 @@ int stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<in
         const int t = sc->shape_type[i];
         hi[(size_t)I_TYPE * stride + col] = t;
         hi[(size_t)I_KIN * stride + col] = sc->kinematic[i] ? 1 : 0;
-        for (int k = 0; k < 4; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
+        for (int k = 0; k < 3; k++) F(F_SP + k) = (float)sc->shape_param[4 * i + k];
+        // the fourth shape parameter carries the body's collision flag (COLL_*)
+        F(F_SP + 3) = (sc->ignore_collision && sc->ignore_collision[i]) ? COLL_IGNORE : COLL_NORMAL;
         for (int k = 0; k < 3; k++) {
             F(F_POS + k) = (float)sc->position[3 * i + k];
             F(F_V + k) = (float)sc->linear_velocity[3 * i + k];
@@ -382,7 +384,7 @@ int stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<in
             for (int k = 0; k < 9; k++) I[k] = meshes.props[10 * id + 1 + k];
             int32_t bits = id;
             std::memcpy(&F(F_SP), &bits, 4);
-            F(F_SP + 1) = F(F_SP + 2) = F(F_SP + 3) = 0.0f;
+            F(F_SP + 1) = F(F_SP + 2) = 0.0f;
         }
         float L[9];
         for (int k = 0; k < 9; k++) L[k] = I[k] * mass;

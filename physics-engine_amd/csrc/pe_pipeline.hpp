@@ -70,7 +70,7 @@ __device__ __forceinline__ void write_records(const BodyView& b, int i, int type
     s[SB_W] = w.x; s[SB_W + 1] = w.y; s[SB_W + 2] = w.z;
     s[23] = 0.0f;
     st_rec<SBW / 4>(b.sbr(i), s);
-    const float a[8] = {mn.x, mn.y, mn.z, kin ? 1.0f : 0.0f, mx.x, mx.y, mx.z, 0.0f};
+    const float a[8] = {mn.x, mn.y, mn.z, kin ? 1.0f : 0.0f, mx.x, mx.y, mx.z, sp[3]};   // sp[3]: COLL_*
     st_rec<2>(b.abr(i), a);
 }
 
@@ -150,10 +150,10 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
 
 // ------------------------------------------------------------------ stage 1
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
-// The same test on AABB records already loaded (min xyz, kinematic flag, max xyz, ignore-collision
-// flag); the same test as sap_pair_ok (pe_sap.hpp).
+// The same test on AABB records already loaded (min xyz, kinematic flag, max xyz, collision flag);
+// the same test as sap_pair_ok (pe_sap.hpp; batched worlds have no ignore-pair lists).
 __device__ __forceinline__ bool bp_valid_rec(const float (&A)[8], const float (&B)[8]) {
-    if (A[7] != 0.0f || B[7] != 0.0f) return false;
+    if (A[7] == COLL_IGNORE || B[7] == COLL_IGNORE) return false;
     if (A[3] != 0.0f && B[3] != 0.0f) return false;
     if (A[0] > B[4] || B[0] > A[4]) return false;
     if (A[1] > B[5] || B[1] > A[5]) return false;

@@ -66,6 +66,8 @@ struct SapView {
     int32_t* pb;
     int pcap;
     int chunk_cap;
+    const uint64_t* ignore; // [nignore] sorted keys (lo << 32 | hi) of body pairs that never collide
+    int nignore;
     __device__ __forceinline__ float mn(int k, int i) const { return amin[(size_t)k * fs + (size_t)i * bs]; }
     __device__ __forceinline__ float mx(int k, int i) const { return amax[(size_t)k * fs + (size_t)i * bs]; }
 };
@@ -75,14 +77,28 @@ __host__ inline int sap_red_blocks(int n) { return n < 1 ? 1 : (n + SAP_RED_THRE
 __host__ inline int sap_scan_parts(int n, int chunk_cap) { return ((chunk_cap > n + 1 ? chunk_cap : n + 1) + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19) on two AABB records: the
-// ignore-collision flags, both kinematic, then closed-interval overlap on x, y, z.
+// ignore-collision flags, both kinematic, then closed-interval overlap on x, y, z.  Ignore-id pairs
+// (broad_phase_base.cpp:12-13) are tested by sap_pair_allowed.
 __device__ __forceinline__ bool sap_pair_ok(const float (&A)[8], const float (&B)[8]) {
-    if (A[7] != 0.0f || B[7] != 0.0f) return false;
+    if (A[7] == COLL_IGNORE || B[7] == COLL_IGNORE) return false;
     if (A[3] != 0.0f && B[3] != 0.0f) return false;
     if (A[0] > B[4] || B[0] > A[4]) return false;
     if (A[1] > B[5] || B[1] > A[5]) return false;
     if (A[2] > B[6] || B[2] > A[6]) return false;
     return true;
+}
+// sap_pair_ok, then for two bodies that both have ignore partners, the ignore-pair list (binary
+// search of the pair's key).
+__device__ __forceinline__ bool sap_pair_allowed(const SapView& s, const float (&A)[8], const float (&B)[8], int i, int j) {
+    if (!sap_pair_ok(A, B)) return false;
+    if (A[7] != COLL_PARTNERS || B[7] != COLL_PARTNERS) return true;
+    const uint64_t key = i < j ? ((uint64_t)i << 32 | (uint32_t)j) : ((uint64_t)j << 32 | (uint32_t)i);
+    int lo = 0, hi = s.nignore;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s.ignore[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return !(lo < s.nignore && s.ignore[lo] == key);
 }
 __device__ __forceinline__ void sap_ld_rec(const float* src, float (&dst)[8]) {
     const float4* s = reinterpret_cast<const float4*>(src);
@@ -183,7 +199,7 @@ static __global__ void k_sap_chunks(SapView s) {
                 }
                 const int j = (int)s.vals[cur + 1 + (int)(t - s.cand[cur])];
                 sap_ld_rec(s.rec + (size_t)j * 8, B);
-                if (sap_pair_ok(A, B)) {
+                if (sap_pair_allowed(s, A, B, ci, j)) {
                     cntv++;
                     hit |= 1u << (t - t0);
                 }

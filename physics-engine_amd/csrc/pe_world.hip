@@ -15,6 +15,7 @@
 #include "pe_prims.hpp"
 #include "pe_pipeline.hpp"
 #include "pe_sap.hpp"
+#include "pe_joints.hpp"
 #include "pe_host.hpp"
 #include "../../include/pe_amd.h"
 
@@ -49,6 +50,12 @@ struct Dev {
     float dt;
     float g[3];
     int iterations;
+    // joints (World::addConstraint): definitions, per-step rows, sweep order grouped by colour
+    const JointDef* joints;
+    JointRow* jrows;
+    const int32_t* jorder;   // [njoints] joint indices, colour by colour
+    const int32_t* jcs;      // [njcol + 1] colour starts in jorder
+    int njoints, njcol;
 };
 
 // ------------------------------------------------------------------ stage 0: integrate
@@ -107,6 +114,13 @@ __global__ void k_narrow_finish(Dev d) {
 __global__ void k_setup(Dev d) {
     const int np = d.cnt->npairs;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) setup_pair(d.b, d.ps, p, d.dt);
+}
+
+// BallJointConstraint / HingeJointConstraint / SliderJointConstraint::initSequentialImpulse, thread / joint
+// (after the contacts' rows, as setupSolver initialises them, sequential_impulse_solver.cpp:49-51).
+__global__ void k_joint_setup(Dev d) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < d.njoints; k += gridDim.x * blockDim.x)
+        joint_setup(d.b, d.joints[k], d.jrows[k], d.dt);
 }
 
 // Constraint-graph colouring of the pairs with contact points, one workgroup (colour_pairs in
@@ -172,6 +186,16 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
             grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
             __threadfence();   // acquire: later colours read velocities other CUs wrote
         }
+        // the joints after the contacts, colour by colour (sequential_impulse_solver.cpp:66-68)
+        for (int c = 0; c < d.njcol; c++) {
+            for (int q = d.jcs[c] + gtid; q < d.jcs[c + 1]; q += gsz) {
+                const int k = d.jorder[q];
+                joint_iterate(d.b, d.joints[k], d.jrows[k]);
+            }
+            __threadfence();
+            grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
+            __threadfence();
+        }
     }
     for (int i = gtid; i < d.b.n; i += gsz) sync_body(d.b, i);
 }
@@ -215,10 +239,17 @@ struct pe_world {
     int n_dynamic = 0;
     bool has_shapes = false;   // any cylinder / convex / concave body
     DevMeshes meshes;
+    std::vector<JointDef> joints;            // host copies of the joint definitions
+    std::vector<uint64_t> ignore_keys;       // sorted ignore-pair keys
+    std::vector<char> kinematic;             // per body (joint colouring)
+    void* joint_mem[4] = {};                 // device: definitions, rows, order, colour starts
+    void* ignore_mem = nullptr;
 
     ~pe_world() {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
+        for (void* p : joint_mem) if (p) hipFree(p);
+        if (ignore_mem) hipFree(ignore_mem);
         meshes.release();
         for (auto& e : ev) if (e) hipEventDestroy(e);
         for (auto& e : order_ev) if (e) hipEventDestroy(e);
@@ -254,6 +285,7 @@ struct pe_world {
     }
     int launch_solve() {
         hipLaunchKernelGGL(k_setup, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
+        if (d.njoints > 0) hipLaunchKernelGGL(k_joint_setup, dim3(grid(d.njoints, 64)), dim3(64), 0, stream, d);
         if (n <= CLAIM_LDS)
             hipLaunchKernelGGL(k_colour<true>, dim3(1), dim3(SOLVE_THREADS), n * sizeof(int), stream, d);
         else
@@ -389,8 +421,10 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     if (int rc2 = w->meshes.upload(hm, w->stream)) return fail(rc2);
     d.b.ms = w->meshes.view();
     int ndyn = 0;
+    w->kinematic.resize(n);
     for (int i = 0; i < n; i++) {
         ndyn += sc->kinematic[i] ? 0 : 1;
+        w->kinematic[i] = sc->kinematic[i] ? 1 : 0;
         w->has_shapes |= sc->shape_type[i] > PE_SHAPE_SPHERE;
     }
     w->n_dynamic = ndyn;
@@ -414,6 +448,104 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     }
     if (hipStreamSynchronize(w->stream) != hipSuccess) { pe_set_error("init sync failed"); return fail(PE_ERR_DEVICE); }
     *out = w;
+    return PE_OK;
+}
+
+int pe_world_add_joints(pe_world* w, int32_t n, const pe_joint_desc* jd) {
+    if (int rc = check_world(w)) return rc;
+    if (n < 0 || (n > 0 && !jd) || (int64_t)w->joints.size() + n > 65536) { pe_set_error("pe_world_add_joints: invalid argument"); return PE_ERR_INVALID; }
+    for (int k = 0; k < n; k++) {
+        const pe_joint_desc& s = jd[k];
+        if (s.type < PE_JOINT_BALL || s.type > PE_JOINT_SLIDER || s.body_a < 0 || s.body_a >= w->n || s.body_b < 0 ||
+            s.body_b >= w->n || s.body_a == s.body_b) {
+            pe_set_error("pe_world_add_joints: joint " + std::to_string(k) + " has an unknown type or invalid bodies");
+            return PE_ERR_INVALID;
+        }
+        JointDef j{};
+        j.type = s.type; j.a = s.body_a; j.b = s.body_b;
+        j.flags = (s.use_limits ? JF_LIMITS : 0) | (s.use_motor ? JF_MOTOR : 0);
+        for (int c = 0; c < 3; c++) {
+            j.anchor_a[c] = (float)s.anchor_a[c]; j.anchor_b[c] = (float)s.anchor_b[c];
+            j.axis_a[c] = (float)s.axis_a[c]; j.axis_b[c] = (float)s.axis_b[c];
+        }
+        j.min_angle = (float)s.min_angle; j.max_angle = (float)s.max_angle; j.target_speed = (float)s.target_speed;
+        w->joints.push_back(j);
+    }
+    // greedy colouring in joint order: joint k takes the lowest colour no earlier joint sharing a
+    // dynamic body with it holds
+    const int nj = (int)w->joints.size();
+    std::vector<int> col(nj, 0), order;
+    std::vector<std::vector<int>> body_cols(w->n);
+    int ncol = 0;
+    for (int k = 0; k < nj; k++) {
+        std::vector<char> used(ncol + 1, 0);
+        for (int x : {w->joints[k].a, w->joints[k].b})
+            if (!w->kinematic[x]) for (int c : body_cols[x]) used[c] = 1;
+        int c = 0;
+        while (used[c]) c++;
+        col[k] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int x : {w->joints[k].a, w->joints[k].b})
+            if (!w->kinematic[x]) body_cols[x].push_back(c);
+    }
+    std::vector<int32_t> jorder, jcs;
+    for (int c = 0; c < ncol; c++) {
+        jcs.push_back((int32_t)jorder.size());
+        for (int k = 0; k < nj; k++) if (col[k] == c) jorder.push_back(k);
+    }
+    jcs.push_back((int32_t)jorder.size());
+    PE_HIP(hipStreamSynchronize(w->stream));
+    for (void*& p : w->joint_mem) if (p) { hipFree(p); p = nullptr; }
+    const size_t bytes[4] = {nj * sizeof(JointDef), nj * sizeof(JointRow), nj * sizeof(int32_t), jcs.size() * sizeof(int32_t)};
+    const void* src[4] = {w->joints.data(), nullptr, jorder.data(), jcs.data()};
+    for (int k = 0; k < 4; k++) {
+        PE_HIP(hipMalloc(&w->joint_mem[k], std::max<size_t>(bytes[k], 16)));
+        if (src[k] && bytes[k]) PE_HIP(hipMemcpy(w->joint_mem[k], src[k], bytes[k], hipMemcpyHostToDevice));
+    }
+    w->d.joints = (const JointDef*)w->joint_mem[0];
+    w->d.jrows = (JointRow*)w->joint_mem[1];
+    w->d.jorder = (const int32_t*)w->joint_mem[2];
+    w->d.jcs = (const int32_t*)w->joint_mem[3];
+    w->d.njoints = nj;
+    w->d.njcol = ncol;
+    return PE_OK;
+}
+
+// Marks the bodies in ignore pairs (COLL_PARTNERS in their fourth shape parameter, unless they ignore
+// every collision) and refreshes their AABB records.
+__global__ void k_mark_partners(Dev d, const uint64_t* keys, int nkeys) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nkeys; k += gridDim.x * blockDim.x) {
+        const int i = (int)(keys[k] >> 32), j = (int)(uint32_t)keys[k];
+        for (int x : {i, j}) {
+            float& f = d.b.at(F_SP + 3, x);
+            if (f != COLL_IGNORE) f = COLL_PARTNERS;
+            d.b.abr(x)[7] = f;
+        }
+    }
+}
+
+int pe_world_add_ignore_pairs(pe_world* w, int32_t n, const int32_t* pairs) {
+    if (int rc = check_world(w)) return rc;
+    if (n < 0 || (n > 0 && !pairs)) { pe_set_error("pe_world_add_ignore_pairs: invalid argument"); return PE_ERR_INVALID; }
+    for (int k = 0; k < n; k++) {
+        const int a = pairs[2 * k], b = pairs[2 * k + 1];
+        if (a < 0 || a >= w->n || b < 0 || b >= w->n || a == b) { pe_set_error("pe_world_add_ignore_pairs: invalid body index"); return PE_ERR_INVALID; }
+        w->ignore_keys.push_back((uint64_t)std::min(a, b) << 32 | (uint32_t)std::max(a, b));
+    }
+    std::sort(w->ignore_keys.begin(), w->ignore_keys.end());
+    w->ignore_keys.erase(std::unique(w->ignore_keys.begin(), w->ignore_keys.end()), w->ignore_keys.end());
+    PE_HIP(hipStreamSynchronize(w->stream));
+    if (w->ignore_mem) { hipFree(w->ignore_mem); w->ignore_mem = nullptr; }
+    const size_t bytes = w->ignore_keys.size() * sizeof(uint64_t);
+    PE_HIP(hipMalloc(&w->ignore_mem, std::max<size_t>(bytes, 16)));
+    if (bytes) PE_HIP(hipMemcpy(w->ignore_mem, w->ignore_keys.data(), bytes, hipMemcpyHostToDevice));
+    w->d.sap.ignore = (const uint64_t*)w->ignore_mem;
+    w->d.sap.nignore = (int)w->ignore_keys.size();
+    if (!w->ignore_keys.empty())
+        hipLaunchKernelGGL(k_mark_partners, dim3(w->grid((int)w->ignore_keys.size(), 256)), dim3(256), 0, w->stream, w->d,
+                           (const uint64_t*)w->ignore_mem, (int)w->ignore_keys.size());
+    PE_HIP(hipGetLastError());
+    PE_HIP(hipStreamSynchronize(w->stream));
     return PE_OK;
 }
 

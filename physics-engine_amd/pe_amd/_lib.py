@@ -23,7 +23,7 @@ EXPORTS = [
     "pe_world_get_aabbs", "pe_world_get_pairs", "pe_world_get_contacts", "pe_world_get_impulses", "pe_world_get_solve_order",
     "pe_world_get_axis", "pe_world_set_axis", "pe_world_set_timing", "pe_world_get_stage_times",
     "pe_world_get_counts", "pe_world_set_solver_blocks", "pe_world_get_records", "pe_world_step_async",
-    "pe_world_get_cache", "pe_world_set_cache",
+    "pe_world_get_cache", "pe_world_set_cache", "pe_world_add_joints", "pe_world_add_ignore_pairs",
     "pe_stage_create", "pe_stage_destroy", "pe_stage_set_meshes", "pe_stage_broadphase", "pe_stage_narrowphase",
     "pe_stage_solve", "pe_stage_broadphase_noalloc",
     "pe_batch_create", "pe_batch_destroy", "pe_batch_step", "pe_batch_sync", "pe_batch_get_world_state",

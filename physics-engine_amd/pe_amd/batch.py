@@ -17,9 +17,18 @@ _F32P = ctypes.POINTER(ctypes.c_float)
 _I32P = ctypes.POINTER(ctypes.c_int32)
 
 
+def _plain_scene(scene):
+    """Batched worlds carry no joints and no ignore-pair lists (pe_world_add_joints and
+    pe_world_add_ignore_pairs are single-world entry points)."""
+    if getattr(scene, "joints", None) or getattr(scene, "ignore_pairs", None):
+        raise ValueError(f"scene {scene.name!r}: joints and ignore pairs need a World, not a batch")
+
+
 class Batch:
     def __init__(self, scenes):
         assert len(scenes) > 0
+        for s in scenes:
+            _plain_scene(s)
         self.scenes = scenes
         self._descs = (type(scenes[0].desc()) * len(scenes))(*[s.desc() for s in scenes])
         self._params = scenes[0].params()

@@ -72,7 +72,35 @@ class SceneDesc(ctypes.Structure):
         ("n_meshes", ctypes.c_int32),
         ("meshes", ctypes.POINTER(MeshDesc)),
         ("mesh_index", ctypes.POINTER(ctypes.c_int32)),
+        ("ignore_collision", ctypes.POINTER(ctypes.c_int32)),
     ]
+
+
+class JointDesc(ctypes.Structure):
+    """ctypes mirror of pe_joint_desc (a ball / hinge / slider joint between two bodies)."""
+    _fields_ = [("type", ctypes.c_int32), ("body_a", ctypes.c_int32), ("body_b", ctypes.c_int32),
+                ("use_limits", ctypes.c_int32), ("use_motor", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("anchor_a", ctypes.c_double * 3), ("anchor_b", ctypes.c_double * 3),
+                ("axis_a", ctypes.c_double * 3), ("axis_b", ctypes.c_double * 3),
+                ("min_angle", ctypes.c_double), ("max_angle", ctypes.c_double), ("target_speed", ctypes.c_double)]
+
+
+JOINT_BALL, JOINT_HINGE, JOINT_SLIDER = 0, 1, 2
+
+
+def joint(kind, a, b, anchor_a=(0, 0, 0), anchor_b=(0, 0, 0), axis_a=(1, 0, 0), axis_b=(1, 0, 0),
+          limits=None, motor_speed=None) -> JointDesc:
+    """A joint as World::addConstraint receives it: kind JOINT_*, bodies a and b, body-local anchors and
+    axes, hinge limits (min, max) in radians and a hinge motor's target speed."""
+    d = JointDesc()
+    d.type, d.body_a, d.body_b = kind, a, b
+    d.anchor_a[:], d.anchor_b[:] = [float(x) for x in anchor_a], [float(x) for x in anchor_b]
+    d.axis_a[:], d.axis_b[:] = [float(x) for x in axis_a], [float(x) for x in axis_b]
+    d.use_limits = 1 if limits is not None else 0
+    d.min_angle, d.max_angle = (float(limits[0]), float(limits[1])) if limits is not None else (0.0, 0.0)
+    d.use_motor = 1 if motor_speed is not None else 0
+    d.target_speed = float(motor_speed) if motor_speed is not None else 0.0
+    return d
 
 
 class WorldParams(ctypes.Structure):
@@ -108,6 +136,9 @@ class Scene:
     name: str = ""
     meshes: list = None              # [Mesh] of the convex / concave bodies
     mesh_index: np.ndarray = None    # int32 [n] mesh of each convex / concave body (-1 otherwise)
+    joints: list = field(default_factory=list)          # [JointDesc] in World::addConstraint order
+    ignore_pairs: list = field(default_factory=list)    # [(a, b)] bodies that never collide (addIgnoreCollisionId)
+    ignore_collision: np.ndarray = None                 # int32 [n] 0/1 (setIgnoreCollision), or None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -152,6 +183,9 @@ class Scene:
             d.n_meshes = len(flats)
             d.meshes = arr
             d.mesh_index = _iptr(self.mesh_index)
+        if self.ignore_collision is not None:
+            self.ignore_collision = np.ascontiguousarray(self.ignore_collision, dtype=np.int32)
+            d.ignore_collision = _iptr(self.ignore_collision)
         return d
 
     def params(self) -> WorldParams:
@@ -166,7 +200,8 @@ class Scene:
                      None if self.linear_damping is None else self.linear_damping[idx],
                      None if self.angular_damping is None else self.angular_damping[idx],
                      self.dt, self.gravity, self.iterations, self.name,
-                     list(self.meshes), self.mesh_index[idx]).contiguous()
+                     list(self.meshes), self.mesh_index[idx],
+                     ignore_collision=None if self.ignore_collision is None else self.ignore_collision[idx]).contiguous()
 
     def footprint_bodies(self) -> int:
         return self.n
@@ -237,6 +272,7 @@ class SceneBuilder:
         self.rows.append((shape, p, mass, np.asarray(pos, float),
                           np.eye(3) if rot is None else np.asarray(rot, float),
                           np.asarray(v, float), np.asarray(w, float), mu, e, kin, mesh))
+        return len(self.rows) - 1
 
     def scene(self, **kw) -> Scene:
         r = self.rows
@@ -537,3 +573,80 @@ def served_shape_world(n: int, seed: int, hull_pool: int = 8, grid: int = 16) ->
     side = 1.8 * max(3, int(math.ceil(math.sqrt(n / 4.0))))
     return shape_mix_scene(n, seed=seed, volume=(side, 4 * 1.8, side), grid=grid,
                            ground_size=(side + 4.0, side + 4.0), hull_pool=hull_pool)
+
+
+def _axis_angle(axis, angle):
+    """Basis of Eigen::AngleAxis(angle, axis) (Rodrigues)."""
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1.0 - math.cos(angle)) * (K @ K)
+
+
+def joint_demo_scene(gravity=(0.0, -9.8, 0.0), motor_speed=2.0) -> Scene:
+    """The joints of the reference's constraints demo (examples/constraints_demo/main.cpp:22-163): a
+    two-link pendulum of ball joints (three joints, sphere and cylinder links), a hinged arm with angle
+    limits and a second hinged arm driven by a motor, and a box sliding on a rail between two end
+    stops; jointed bodies never collide with each other (addIgnoreCollisionId).  Under gravity, on a
+    kinematic ground."""
+    b = SceneBuilder()
+    pi = math.pi
+    b.add(SHAPE_BOX, (30.0, 10.0, 30.0), 10000.0, (0, -5, 0), kin=1, e=0.5)                          # 0 ground
+    b.add(SHAPE_SPHERE, (0.5,), 1.0, (0, 0.5, 0), kin=1)                                             # 1 ball base
+    s6, c6 = math.sin(pi / 6), math.cos(pi / 6)
+    b.add(SHAPE_CYLINDER, (0.2, 6.0), 1.0, (-3.3 * s6, 3.3 * c6 + 0.5, 0), rot=_axis_angle((0, 0, 1), pi / 6))  # 2
+    b.add(SHAPE_SPHERE, (0.4,), 1.0, (-6.6 * s6, 6.6 * c6 + 0.5, 0))                                 # 3
+    b.add(SHAPE_CYLINDER, (0.2, 6.0), 1.0, (-3.3 - 6.6 * s6, 6.6 * c6 + 0.5, 0), rot=_axis_angle((0, 0, 1), pi / 2))  # 4
+    R5, o5 = _axis_angle((1, 0, 0), pi / 6), np.array([4.0, 8.0, 0.0])
+    b.add(SHAPE_CYLINDER, (0.2, 1.0), 1.0, o5, rot=R5, kin=1)                                         # 5 pole base
+    R2, o2 = _axis_angle((0, 0, -1), pi / 2), np.array([2.0, 0.0, 0.0])
+    b.add(SHAPE_CYLINDER, (0.2, 4.0), 1.0, R5 @ o2 + o5, rot=R5 @ R2)                                 # 6 arm
+    R3, o3 = _axis_angle((1, 0, 0), pi / 2), np.array([4.0, 0.0, 0.75])
+    b.add(SHAPE_CYLINDER, (0.2, 2.0), 1.0, R5 @ o3 + o5, rot=R5 @ R3)                                 # 7 second arm
+    R0, o0 = _axis_angle((0, 1, 0), -pi / 4), np.array([-1.0, 6.0, 0.0])
+    Rt = R0 @ _axis_angle((0, 0, 1), pi / 3)
+    b.add(SHAPE_BOX, (0.3, 10.0, 0.3), 1.0, o0, rot=Rt, kin=1)                                       # 8 rail
+    b.add(SHAPE_BOX, (1.0, 0.4, 1.0), 1.0, Rt @ np.array([0, 5.2, 0]) + o0, rot=Rt, kin=1)           # 9 stop
+    b.add(SHAPE_BOX, (1.0, 0.4, 1.0), 1.0, Rt @ np.array([0, -5.2, 0]) + o0, rot=Rt, kin=1)          # 10 stop
+    b.add(SHAPE_BOX, (1.0, 1.2, 1.0), 1.0, Rt @ np.array([0, 4.4, 0]) + o0, rot=Rt)                  # 11 slider
+    sc = b.scene(gravity=tuple(gravity), name="joint_demo")
+    sc.joints = [
+        joint(JOINT_BALL, 1, 2, anchor_b=(0, -3.3, 0)),
+        joint(JOINT_BALL, 3, 4, anchor_b=(0, -3.3, 0)),
+        joint(JOINT_BALL, 2, 3, anchor_a=(0, 3.3, 0)),
+        joint(JOINT_HINGE, 5, 6, anchor_b=(0, -2, 0), axis_a=(0, 1, 0), axis_b=(-1, 0, 0), limits=(-pi / 3, pi / 3)),
+        joint(JOINT_HINGE, 6, 7, anchor_a=(0, 2, 0), anchor_b=(0, -0.75, 0), axis_a=(0, 1, 0), axis_b=(0, 0, -1),
+              motor_speed=motor_speed),
+        joint(JOINT_SLIDER, 8, 11, axis_a=(0, 1, 0), axis_b=(0, 1, 0)),
+    ]
+    sc.ignore_pairs = [(1, 2), (3, 2), (3, 4), (5, 6), (6, 7), (8, 11)]
+    return sc
+
+
+def joint_chain_scene(chains=48, links=8, seed=5, spacing=2.5) -> Scene:
+    """Hanging chains of ball joints under gravity: each chain a kinematic anchor sphere and `links`
+    capsule-like cylinders (radius 0.15, length 1) joined end to end, started swinging sideways;
+    neighbouring links ignore each other (addIgnoreCollisionId), the chains' ends strike a ground
+    box.  Joint colouring alternates along a chain, so a colour holds one joint of many chains."""
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    side = int(math.ceil(math.sqrt(chains)))
+    b.add(SHAPE_BOX, (side * spacing + 10.0, 1.0, side * spacing + 10.0), 1.0, (0.0, -0.5, 0.0), kin=1)
+    top = links * 1.0 + 2.0
+    joints, ignore = [], []
+    for c in range(chains):
+        x = (c % side - side / 2) * spacing
+        z = (c // side - side / 2) * spacing
+        anchor = b.add(SHAPE_SPHERE, (0.2,), 1.0, (x, top, z), kin=1)
+        tilt = _axis_angle((0.0, 0.0, 1.0), float(rng.uniform(0.3, 0.9)))
+        prev, prev_end = anchor, np.zeros(3)
+        for k in range(links):
+            centre = np.array([x, top, z]) + tilt @ np.array([0.0, -(k + 0.5), 0.0])
+            body = b.add(SHAPE_CYLINDER, (0.15, 1.0), 1.0, tuple(centre), rot=tilt)
+            joints.append(joint(JOINT_BALL, prev, body, anchor_a=tuple(prev_end), anchor_b=(0.0, 0.5, 0.0)))
+            ignore.append((prev, body))
+            prev, prev_end = body, np.array([0.0, -0.5, 0.0])
+    sc = b.scene(name=f"chains{chains}x{links}")
+    sc.joints = joints
+    sc.ignore_pairs = ignore
+    return sc

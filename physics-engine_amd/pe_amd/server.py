@@ -6,6 +6,7 @@ import ctypes
 import numpy as np
 
 from ._lib import check, lib
+from .batch import _plain_scene
 from .scenes import WorldParams
 
 _F32P = ctypes.POINTER(ctypes.c_float)
@@ -46,6 +47,7 @@ class Server:
         return b.value
 
     def submit(self, scene, horizon: int) -> int:
+        _plain_scene(scene)
         d = scene.desc()
         rid = ctypes.c_int64()
         check(lib().pe_server_submit(self.h, ctypes.byref(d), ctypes.c_int32(horizon), ctypes.byref(rid)))

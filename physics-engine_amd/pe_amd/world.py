@@ -43,6 +43,21 @@ class World:
         check(lib().pe_world_create(ctypes.byref(self._desc), ctypes.byref(self._params), ctypes.c_int32(max_pairs),
                                     ctypes.byref(h)))
         self.h = h
+        if scene.ignore_pairs:
+            self.add_ignore_pairs(scene.ignore_pairs)
+        if scene.joints:
+            self.add_joints(scene.joints)
+
+    def add_joints(self, joints):
+        """World::addConstraint for each JointDesc, in order (pe_world_add_joints)."""
+        from .scenes import JointDesc
+        arr = (JointDesc * len(joints))(*joints)
+        check(lib().pe_world_add_joints(self.h, ctypes.c_int32(len(joints)), arr))
+
+    def add_ignore_pairs(self, pairs):
+        """RigidBody::addIgnoreCollisionId on both bodies of each (a, b) (pe_world_add_ignore_pairs)."""
+        p = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        check(lib().pe_world_add_ignore_pairs(self.h, ctypes.c_int32(p.shape[0]), _i(p)))
 
     def close(self):
         if getattr(self, "h", None):

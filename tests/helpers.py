@@ -123,3 +123,18 @@ def impulse_error(gi, oi, on):
         return 0.0, 0
     floor = 1e-6 * max(float(np.abs(b).max()), 1e-30)
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))), int(vm.sum())
+
+
+def ortho_units(n):
+    """ContactPoint::getOrthoUnits (contact_result.cpp:8-31) in FP64: the two tangents the hinge
+    measures its angle with."""
+    n = np.asarray(n, float) / np.linalg.norm(n)
+    if abs(n[2]) > 0.7071:
+        a = n[1] ** 2 + n[2] ** 2
+        k = 1 / np.sqrt(a)
+        t1 = np.array([0.0, -n[2] * k, n[1] * k])
+        return t1, np.array([a * k, -n[0] * t1[2], n[0] * t1[1]])
+    a = n[0] ** 2 + n[1] ** 2
+    k = 1 / np.sqrt(a)
+    t1 = np.array([-n[1] * k, n[0] * k, 0.0])
+    return t1, np.array([-n[2] * t1[1], n[2] * t1[0], a * k])

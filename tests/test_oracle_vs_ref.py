@@ -6,7 +6,7 @@ so both continue from the same pair list.  Skipped where oracle/_ref is not buil
 import numpy as np
 import pytest
 
-from helpers import flat_state, pair_set, reference_sample_scene, small_rain
+from helpers import flat_state, ortho_units, pair_set, reference_sample_scene, small_rain
 from pyoracle import OracleWorld, RefWorld
 
 pytestmark = pytest.mark.ref
@@ -153,3 +153,112 @@ def test_adapter_host_library_exports_its_entry_points():
     lib = ctypes.CDLL(os.path.join(HERE, "_ref", "libperef32_amd.so"))
     for name in ("pref_use_amd_stages", "pref_step_checked", "pref_world_create", "pref_step"):
         assert hasattr(lib, name), name
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("bits", [32, 64])
+def test_joints_bit_exact(bits):
+    """Ball, hinge (angle limits, motor) and slider joints of the reference's constraints demo, under
+    gravity with contacts: the oracle in the reference's sweep order (contacts, then joints in
+    World::addConstraint order) equals the compiled reference bit for bit after every step
+    (BallJointConstraint / HingeJointConstraint / SliderJointConstraint, sequential_impulse_solver.cpp:66-68)."""
+    from pe_amd import scenes
+    sc = scenes.joint_demo_scene()
+    r = RefWorld(sc, bits, threads=2)
+    o = OracleWorld(sc, bits, color_order=False)
+    moved = 0.0
+    p0 = r.state()[0].copy()
+    for step in range(300):
+        r.integrate(); o.integrate()
+        r.broadphase(); o.broadphase()
+        o.set_pairs(r.pairs())
+        r.narrowphase(); o.narrowphase()
+        r.solve(); o.solve()
+        rs, os_ = r.state(), o.state()
+        for a, b in zip(rs, os_):
+            assert np.array_equal(a, b), f"step {step}"
+        o.set_state(*rs)
+    moved = float(np.abs(r.state()[0] - p0).max())
+    assert moved > 1.0   # the pendulum swings, the arms turn, the slider slides
+
+
+@pytest.mark.ref
+def test_joint_chains_and_ignore_flags_bit_exact():
+    """384 ball joints in hanging chains with neighbour ignore pairs, plus bodies flagged
+    setIgnoreCollision in a falling cluster: the oracle's pairs (as a set: the reference's
+    multi-threaded sort orders ties its own way) and, on the reference's pair order, its state
+    after every step equal the compiled reference's."""
+    from pe_amd import scenes
+    chains = scenes.joint_chain_scene()
+    rain = small_rain(200, seed=3)
+    flags = np.zeros(rain.n, np.int32)
+    flags[np.random.default_rng(2).choice(np.arange(1, rain.n), 10, replace=False)] = 1
+    rain.ignore_collision = flags
+    rain.ignore_pairs = [(5, 6), (7, 9), (10, 11), (3, 12)]
+    for sc, steps in ((chains, 40), (rain, 40)):
+        r = RefWorld(sc, 32, threads=2)
+        o = OracleWorld(sc, 32, color_order=False)
+        flagged = set(np.nonzero(sc.ignore_collision)[0].tolist()) if sc.ignore_collision is not None else set()
+        ign = {(min(a, b), max(a, b)) for a, b in sc.ignore_pairs}
+        for step in range(steps):
+            r.integrate(); o.integrate()
+            r.broadphase(); o.broadphase()
+            rp = pair_set(r.pairs())
+            assert pair_set(o.pairs()) == rp, f"{sc.name} pairs step {step}"
+            assert not (rp & ign) and not any(a in flagged or b in flagged for a, b in rp)
+            o.set_pairs(r.pairs())
+            r.narrowphase(); o.narrowphase()
+            r.solve(); o.solve()
+            rs = r.state()
+            for a, b in zip(rs, o.state()):
+                assert np.array_equal(a, b), f"{sc.name} step {step}"
+            o.set_state(*rs)
+
+
+def test_batches_refuse_joints():
+    from pe_amd import scenes
+    from pe_amd.batch import Batch
+    with pytest.raises(ValueError, match="joints"):
+        Batch([scenes.joint_demo_scene()])
+
+
+def test_joint_constraints_hold_in_the_oracle():
+    """What the mathematics fixes, step after step under gravity: the ball joints' anchors coincide
+    (Baumgarte-stabilised: drift of millimetres), the slider's body stays on its rail's axis, and the
+    limited hinge's angle (measured as HingeJointConstraint measures it, hinge_joint_constraint.cpp:
+    56-60) reaches its limit and stays within it up to the stabilisation slack; without the joints the
+    same bodies fall apart."""
+    from pe_amd import scenes
+    sc = scenes.joint_demo_scene()
+    o = OracleWorld(sc, 64, color_order=True)
+    h = sc.joints[3]
+    worst_gap, worst_off, angles = 0.0, 0.0, []
+    for step in range(300):
+        o.step(1)
+        pos, rot, _, _ = o.state()
+        for j in sc.joints[:3]:
+            pa = rot[j.body_a] @ np.array(j.anchor_a) + pos[j.body_a]
+            pb = rot[j.body_b] @ np.array(j.anchor_b) + pos[j.body_b]
+            worst_gap = max(worst_gap, float(np.linalg.norm(pa - pb)))
+        rail, sl = sc.joints[5].body_a, sc.joints[5].body_b
+        axis = rot[rail] @ np.array([0.0, 1.0, 0.0])
+        d = pos[sl] - pos[rail]
+        worst_off = max(worst_off, float(np.linalg.norm(d - (d @ axis) * axis)))
+        ta = rot[h.body_a] @ ortho_units(h.axis_a)[0]
+        tb = rot[h.body_b] @ ortho_units(h.axis_b)[0]
+        wa = rot[h.body_a] @ np.array(h.axis_a)
+        angles.append(float(np.arctan2(np.cross(ta, tb) @ wa, ta @ tb)))
+    print(f"[joints] anchor gap {worst_gap:.2e} m, slider off-axis {worst_off:.2e} m, hinge angle "
+          f"[{min(angles):.3f}, {max(angles):.3f}] rad", flush=True)
+    assert worst_gap < 2e-2 and worst_off < 2e-2
+    lim = np.pi / 3
+    assert max(abs(a) for a in angles) > 0.9 * lim           # the arm swings up to its limit
+    assert all(-lim - 0.1 <= a <= lim + 0.1 for a in angles)  # and no further than the slack
+    free = scenes.joint_demo_scene()
+    free.joints = []
+    f = OracleWorld(free, 64)
+    f.step(300)
+    fp = f.state()[0]
+    pa = f.state()[1][1] @ np.zeros(3) + fp[1]
+    pb = f.state()[1][2] @ np.array([0, -3.3, 0]) + fp[2]
+    assert np.linalg.norm(pa - pb) > 0.5
