@@ -1,5 +1,5 @@
 """Benchmark: body-steps/s of the batched+served rigid-body step vs the plain single-world
-GPU path, p50/p99 step latency of the served loop, roofline of the fused step kernel, and
+GPU path, p50/p99 step latency of the served loop, roofline of the dominant launch group, and
 the reference engine's multithreaded CPU step on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps 120] [--warmup 10]
@@ -7,7 +7,14 @@ the reference engine's multithreaded CPU step on a bounded sample of the same wo
 Prints one JSON line.  Workload (value): 4096 independent worlds, body counts log-uniform in
 8..512 (boxes with U(0.2,1.0) m edges and spheres with r U(0.1,0.6) m, p = 1/2, ~496k bodies),
 each on its own kinematic ground box, dt = 1/60, 10 solver iterations, FP32, one
-hipGraph-replayed batched step per timed step.
+hipGraph-replayed batched step per timed step, steps 0..120 from the drop; launch schedule from
+the tuning table (or --tune).  Beside it: the same worlds settled (steps 150..270), the plain
+path (10k box rain, the same worlds one at a time, the 50k shape mix), the served loop at two
+arrival rates, and the reference's CPU step in FP32 and FP64.
+
+Profiling: --profile batched|box_rain_10k|shape_mix runs one leg's timed window alone;
+--traffic-summary FETCH WRITE turns two rocprofv3 --pmc passes over `--profile batched
+--calibrate-copy` into profiles/batched_traffic.json.
 """
 from __future__ import annotations
 
@@ -65,38 +72,126 @@ def step_bytes(n, n_dyn, pairs, points, hull_bytes=0):
     return sum(stage_bytes(n, n_dyn, pairs, points, hull_bytes))
 
 
-def run_batched(args, worlds):
+def batched_schedule(args, worlds):
+    """The launch schedule of the batched leg: the tuning table's entry for this workload
+    (pe_amd/schedules.json, chosen by Batch.autotune on an MI355X), or with --tune the autotuner's
+    choice made now on a separate copy of the worlds after args.settle settling steps."""
+    from pe_amd.batch import Batch, load_schedule
+    if not args.tune:
+        return load_schedule("batched_4096_worlds"), "tuning table pe_amd/schedules.json: batched_4096_worlds"
+    tb = Batch(worlds)
+    tb.step(args.settle)
+    tuned = tb.autotune(steps=2)
+    sched = tb.schedule()
+    tb.close()
+    log(f"autotuned (block, solver group bodies) per class: {tuned}")
+    if args.write_schedule:
+        with open(args.write_schedule, "w") as f:
+            json.dump(sched, f)
+    return sched, f"autotuned in this run (Batch.autotune(steps=2) after {args.settle} settling steps)"
+
+
+def run_batched(args, worlds, sched):
+    """The timed window is the workload's own: steps 0..K from the drop.  The graph is captured and
+    warmed up on a snapshot of step 0, which is restored before the timed replays.  The stage
+    breakdown (hipEvents between the three launch groups) is measured over the same window, again
+    from the snapshot, with the pair and contact counts of every step for its algorithmic bytes.
+    Then the worlds run on to step args.settle and K more steps are timed there (settled piles)."""
     from pe_amd.batch import Batch
     t0 = time.perf_counter()
     b = Batch(worlds)
     log(f"batch created: {len(worlds)} worlds, {b.n_bodies} bodies in {time.perf_counter() - t0:.1f}s")
-    # the worlds settle for args.settle steps before tuning, so the launch configuration is
-    # chosen for the contact load the timed steps see; the tuner times every candidate from a
-    # snapshot of that state and restores it, so the timed steps follow the settled state
-    b.step(args.settle)
-    tuned = b.autotune(steps=2) if not args.no_tune else {}
-    log(f"autotuned (block, solver group bodies) per class: {tuned}; config {b.config()}")
+    b.apply_schedule(sched)
+    b.snapshot()
     b.capture_graph()
     b.step(args.warmup)
     b.sync()
-    bt_start = b.steps_done
-    # the worlds' states at the first timed step, for the CPU baseline
-    start_states = [b.world_state(i) for i in range(len(worlds))] if not args.no_cpu else []
-    c0 = b.counts()
+    b.restore()
+    K = args.steps
     b.sync()
     w0 = time.perf_counter()
-    dev_ms = b.step_timed(args.steps)
+    dev_ms = b.step_timed(K)
     b.sync()
     wall = time.perf_counter() - w0
     c1 = b.counts()
-    pairs = 0.5 * (c0["pairs"] + c1["pairs"])
-    points = 0.5 * (c0["points"] + c1["points"])
-    nst = 10
-    st_ms = [x / nst for x in b.stage_times(nst)]
-    c2 = b.counts()
-    return dict(wall=wall, dev_ms=dev_ms, counts0=c0, counts1=c1, pairs=pairs, points=points, start_step=bt_start,
-                start_states=start_states,
-                n=b.n_bodies, n_dyn=b.n_dynamic, tuned=tuned, batch=b, stage_ms=st_ms, stage_counts=c2)
+    if c1["overflow_worlds"]:
+        raise SystemExit(f"bench.py: {c1['overflow_worlds']} worlds exceeded a capacity; the step is not valid")
+    out = dict(wall=wall, dev_ms=dev_ms, counts=c1, n=b.n_bodies, n_dyn=b.n_dynamic, batch=b)
+    if args.profile == "batched":
+        return out
+    b.restore()
+    st_ms, st_b = [0.0] * 3, [0.0] * 3
+    pairs = points = 0
+    for _ in range(K):
+        ms = b.stage_times(1)
+        c = b.counts()
+        sb = stage_bytes(c["bodies"], c["dynamic"], c["pairs"], c["points"])
+        for i in range(3):
+            st_ms[i] += ms[i]
+            st_b[i] += sb[i]
+        pairs += c["pairs"]
+        points += c["points"]
+    out.update(stage_ms=[x / K for x in st_ms], stage_bytes=[x / K for x in st_b], pairs=pairs / K, points=points / K)
+    if args.settle > K:
+        b.step(args.settle - K)
+    b.sync()
+    s0 = b.steps_done
+    t1 = time.perf_counter()
+    b.step_timed(K)
+    b.sync()
+    sw = time.perf_counter() - t1
+    cs = b.counts()
+    if cs["overflow_worlds"]:
+        raise SystemExit(f"bench.py: {cs['overflow_worlds']} worlds exceeded a capacity; the step is not valid")
+    out["settled"] = {"window": f"steps {s0}..{s0 + K}", "body_steps_per_s": b.n_dynamic * K / sw,
+                      "ms_per_step": 1000 * sw / K, "pairs": cs["pairs"], "points": cs["points"]}
+    return out
+
+
+def plain_stage_roofline(scene, start, steps):
+    """Stage breakdown of the plain single-world path over steps start..start+steps of the scene
+    (a fresh World; hipEvents between its four stages, every step), with each step's algorithmic
+    bytes from its pair and contact counts; roofline of the costliest stage."""
+    import pe_amd
+    w = pe_amd.World(scene)
+    if start:
+        w.step(start)
+    w.sync()
+    w.set_timing(True)
+    names = ["integrate", "broadphase (sweep-and-prune)", "narrowphase", "solve (colouring + Gauss-Seidel)"]
+    t_prev = w.stage_times()
+    ms, by = [0.0] * 4, [0.0] * 4
+    hull = scene_mesh_bytes(scene)
+    for _ in range(steps):
+        w.step(1)
+        t = w.stage_times()
+        c = w.counts()
+        for i in range(4):
+            ms[i] += t[i] - t_prev[i]
+        t_prev = t
+        k1, k2, k3 = stage_bytes(c["bodies"], c["dynamic"], c["pairs"], c["points"], hull)
+        integ = 148 * c["dynamic"]
+        for i, v in enumerate((integ, k1 - integ, k2, k3)):
+            by[i] += v
+    w.close()
+    k = max(range(4), key=lambda i: ms[i])
+    ach = by[k] / (ms[k] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": names[k], "algorithmic_bytes_per_launch": by[k] / steps,
+            "launch_ms": ms[k] / steps, "stage_ms": [x / steps for x in ms],
+            "stage_algorithmic_bytes": [x / steps for x in by], "window": f"steps {start}..{start + steps}",
+            "whole_step": {"algorithmic_bytes": sum(by) / steps, "ms": sum(ms) / steps,
+                           "achieved_gbs": sum(by) / (sum(ms) * 1e-3) / 1e9}}
+
+
+def scene_mesh_bytes(scene):
+    """Device bytes of every mesh of a scene (FP32 vertices and face normals, int32 face starts and
+    indices): the hull and triangle data the narrowphase reads, counted once per step."""
+    tot = 0
+    for m in scene.meshes or []:
+        v, st, ix, fn = m.flat()
+        tot += 4 * (v.size + st.size + ix.size + fn.size)
+    return tot
 
 
 def run_plain(scene, steps, warmup=0):
@@ -133,30 +228,34 @@ def run_plain_sample(worlds, steps, settle=0):
     return bodies * steps / tot
 
 
-def run_served(args):
+def served_pool(args):
     from pe_amd import scenes
-    from pe_amd.server import Server
     rng = np.random.default_rng(1004)
-    lam, T = args.serve_rate, args.serve_seconds
-    n_req = int(lam * T * 1.2) + 8
-    gaps = rng.exponential(1.0 / lam, n_req)
-    arrivals = np.cumsum(gaps)
-    sizes = scenes.log_uniform_counts(rng, n_req, 16, 4096)
-    horizons = rng.integers(30, 601, n_req)
+    sizes = scenes.log_uniform_counts(rng, args.serve_pool, 16, 4096)
     t0 = time.perf_counter()
     # mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a heightfield); a pool of
     # distinct request scenes, sized from the same distribution, is cycled through
-    pool = [scenes.served_shape_world(int(s), scenes.world_seed(1004, i))
-            for i, s in enumerate(sizes[:args.serve_pool])]
+    pool = [scenes.served_shape_world(int(s), scenes.world_seed(1004, i)) for i, s in enumerate(sizes)]
+    log(f"served: {len(pool)} distinct mixed-shape request worlds generated in {time.perf_counter() - t0:.1f}s")
+    return pool
+
+
+def run_served(args, pool, lam, seed):
+    """Poisson arrivals at lam requests/s for args.serve_seconds; horizons U(30, 600); the request
+    loop submits what has arrived, ticks (one step of every live world), and collects results.
+    Reports throughput, per-tick latency and the per-request figures (admission wait, submission to
+    result, queue depth), and the offered load: lam x the mean body-steps a request asks for."""
+    from pe_amd.server import Server
+    rng = np.random.default_rng(seed)
+    T = args.serve_seconds
+    n_req = int(lam * T * 1.5) + 8
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    horizons = rng.integers(30, 601, n_req)
     reqs = [pool[i % len(pool)] for i in range(n_req)]
-    sizes = np.array([r.n - 1 for r in reqs])
-    log(f"served: {len(pool)} distinct mixed-shape request worlds generated in {time.perf_counter() - t0:.1f}s for "
-        f"{n_req} arrivals (mean {sizes.mean():.0f} bodies, mean horizon {horizons.mean():.0f})")
+    dyn = np.array([r.n_dynamic() for r in reqs], float)
     srv = Server(hbm_budget_bytes=int(args.serve_budget_gb * (1 << 30)), max_worlds=16384, max_world_bodies=4096)
-    nxt = 0
-    done = 0
+    nxt = done = ticks = 0
     start = time.perf_counter()
-    ticks = 0
     while True:
         now = time.perf_counter() - start
         if now >= T:
@@ -173,18 +272,19 @@ def run_served(args):
             time.sleep(max(0.0, min(0.01, arrivals[nxt] - (time.perf_counter() - start))))
     elapsed = time.perf_counter() - start
     st = srv.stats()
+    rq = srv.request_stats()
     srv.close()
-    # the plain single-world path on the same request worlds, one at a time from their first step
-    plain_same = run_plain_sample(pool[:32], 20, settle=150)
-    return dict(plain_same_worlds_body_steps_per_s=plain_same,
-                plain_same_worlds_sample="first 32 request worlds, 150 untimed then 20 timed steps each",
-                served_over_plain_same_worlds=st["body_steps"] / elapsed / plain_same,
-                workload=f"Poisson arrivals at {lam:g}/s for {T:g} s; body counts log-uniform 16-4096, horizons "
-                         f"U(30, 600); mixed shapes as the 50k shape mix (boxes, spheres, cylinders, hulls over a "
-                         f"16x16 heightfield), {len(pool)} distinct request worlds cycled",
-                rate=lam, seconds=elapsed, submitted=nxt, completed=done, ticks=ticks,
-                body_steps=st["body_steps"], body_steps_per_s=st["body_steps"] / elapsed,
-                p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"], budget_gb=args.serve_budget_gb)
+    offered = lam * float(np.mean(dyn * horizons))
+    achieved = st["body_steps"] / elapsed
+    return dict(rate=lam, seconds=elapsed, submitted=nxt, completed=done, ticks=ticks,
+                body_steps=st["body_steps"], body_steps_per_s=achieved,
+                offered_body_steps_per_s=offered, offered_over_achieved=offered / achieved,
+                p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], mean_ms=st["mean_ms"],
+                queued_at_end=st["queued"], live_at_end=st["live"],
+                admission_wait_p50_ms=rq["wait_p50_ms"], admission_wait_p99_ms=rq["wait_p99_ms"],
+                submit_to_result_p50_ms=rq["latency_p50_ms"], submit_to_result_p99_ms=rq["latency_p99_ms"],
+                queue_depth_mean=rq["queue_mean"], queue_depth_max=rq["queue_max"],
+                capacity_errors=rq["capacity_errors"])
 
 
 def run_shape_mix(args):
@@ -204,50 +304,60 @@ def run_shape_mix(args):
     w.step(args.mix_steps)
     w.sync()
     dt = time.perf_counter() - t0
-    w.set_timing(True)       # stage breakdown over 10 further steps (events between stages)
-    w.step(10)
-    w.sync()
-    st = [x / 10 for x in w.stage_times()]
     out = {"workload": f"{sc.n - 1} mixed bodies (40% boxes, 25% spheres, 15% cylinders, 20% hulls of 32 Gaussian "
                        f"points) over a 64x64 heightfield, steps {args.mix_warm}..{args.mix_warm + args.mix_steps}",
            "body_steps_per_s": sc.n_dynamic() * args.mix_steps / dt, "ms_per_step": 1000 * dt / args.mix_steps,
-           "stage_ms": st, "counts": w.counts()}
+           "counts": w.counts()}
+    if args.profile is None:
+        out["roofline"] = plain_stage_roofline(sc, args.mix_warm, args.mix_steps)
     if not args.no_cpu:
         from pyoracle import RefWorld, ref_available
         if ref_available(32):
             with StdoutToStderr():
-                r = RefWorld(sc, 32, threads=args.cpu_threads)
+                r = RefWorld(sc, 32, threads=cpu_threads(args))
                 r.set_state(*[np.asarray(x, np.float64) for x in w.state()])
                 cdt = r.step_timed(args.mix_cpu_steps)
                 del r
             out["cpu_reference"] = {"body_steps_per_s": sc.n_dynamic() * args.mix_cpu_steps / cdt,
-                                    "ms_per_step": 1000 * cdt / args.mix_cpu_steps, "cores": args.cpu_threads,
-                                    "sample": f"{args.mix_cpu_steps} steps from the GPU state at step {args.mix_warm + args.mix_steps + 10}"}
+                                    "ms_per_step": 1000 * cdt / args.mix_cpu_steps, "cores": cpu_threads(args),
+                                    "sample": f"{args.mix_cpu_steps} steps from the GPU state at step "
+                                              f"{args.mix_warm + args.mix_steps}"}
     w.close()
     return out
 
 
-def cpu_baseline(worlds, states, start_step, threads, budget_s):
-    """The reference engine itself (oracle/_ref/libperef32.so, compiled from its sources) when
-    present, else the oracle restatement; worlds stepped one after another, each with the
-    reference's batched-loop thread pool, until about budget_s seconds of CPU work.  Each world
-    starts from its state at the batched leg's first timed step (its contact caches empty), so
-    the CPU sees the contact load the timed GPU steps see."""
+def cpu_threads(args):
+    """Threads the reference's pool gets: --cpu-threads, at most the cores this process may run on."""
+    return max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(worlds, steps, threads, budget_s, bits):
+    """The reference engine itself (oracle/_ref/libperef{32,64}.so, compiled from its sources) when
+    present, else the oracle restatement: the batched workload's worlds one after another, each
+    for the same steps 0..steps from the drop that the timed GPU window covers, each with the
+    reference's thread pool, until about budget_s seconds of CPU work."""
     from pyoracle import OracleWorld, RefWorld, ref_available
-    kind = "reference" if ref_available(32) else "port"
-    steps = 10
+    kind = "reference" if ref_available(bits) else "port"
     tot_t, body_steps, nw = 0.0, 0, 0
-    for sc, st in zip(worlds, states):
-        st64 = [np.asarray(x, np.float64) for x in st]
+    for sc in worlds:
         if kind == "reference":
             with StdoutToStderr():
-                w = RefWorld(sc, 32, threads=threads)
-                w.set_state(*st64)
+                w = RefWorld(sc, bits, threads=threads)
                 dt = w.step_timed(steps)
                 del w
         else:
-            w = OracleWorld(sc, 32)
-            w.set_state(*st64)
+            w = OracleWorld(sc, bits)
             t0 = time.perf_counter()
             w.step(steps)
             dt = time.perf_counter() - t0
@@ -257,122 +367,193 @@ def cpu_baseline(worlds, states, start_step, threads, budget_s):
         if tot_t >= budget_s:
             break
     return dict(value=body_steps / tot_t, unit="body-steps/s", cores=threads if kind == "reference" else 1,
-                kind=kind, sample=f"first {nw} of the 4096 worlds, {steps} steps each from their state at step "
-                                  f"{start_step}, sequential, FP32, "
-                                  f"{'ThreadPool of ' + str(threads) + ' threads' if kind == 'reference' else 'single thread'}")
+                kind=kind, sample=f"first {nw} of the 4096 worlds ({sum(w.n_dynamic() for w in worlds[:nw])} dynamic "
+                                  f"bodies), steps 0..{steps} from the drop each, one world after another, "
+                                  f"FP{bits}, " + (f"the reference's ThreadPool of {threads} threads"
+                                                   if kind == "reference" else "single thread"))
+
+
+def copy_calibration():
+    """Three device-to-device copies of 1 GiB (hipMemcpy): a kernel of known traffic, ahead of
+    everything else, for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE readings."""
+    from pe_amd.hip import DeviceArray, synchronize
+    x = DeviceArray((1 << 28,), np.float32)
+    y = DeviceArray((1 << 28,), np.float32)
+    for _ in range(3):
+        y.copy_from(x)
+    synchronize()
+    x.close()
+    y.close()
+    log("calibration copy: 3 x (1 GiB read + 1 GiB written)")
+
+
+def _pmc_rows(path, counter):
+    import csv
+    import glob
+    files = [path] if os.path.isfile(path) else glob.glob(os.path.join(path, "**", "*counter_collection.csv"),
+                                                              recursive=True)
+    rows = []
+    for fn in files:
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def traffic_summary(fetch_path, write_path, steps, out_path):
+    """HBM bytes per step of each launch group from two rocprofv3 --pmc passes (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) over `python bench.py --profile batched --calibrate-copy`: each
+    counter is scaled by its reading for the three 1 GiB calibration copies (the first three copy
+    dispatches), and summed over the dispatches of the last `steps` steps (one k_list_reset opens a
+    step; the profiled run ends with the timed window)."""
+    stages = ("k_world_broad", "k_narrow_list", "k_group_solve")
+    res = {}
+    for name, path in (("fetch", fetch_path), ("write", write_path)):
+        rows = _pmc_rows(path, "FETCH_SIZE" if name == "fetch" else "WRITE_SIZE")
+        copies = [r for r in rows if "copyBuffer" in r[1]][:3]
+        kib = sum(r[2] for r in copies) / len(copies)
+        scale = (1 << 30) / (kib * 1024)
+        resets = [i for i, r in enumerate(rows) if "k_list_reset" in r[1]]
+        window = rows[resets[-steps]:]
+        per = [0.0, 0.0, 0.0]
+        for _, kname, v in window:
+            for k, sname in enumerate(stages):
+                if sname in kname:
+                    per[k] += v * 1024 * scale
+        res[name] = dict(kib_per_gib_copy=kib, scale=scale, stage_bytes_per_step=[x / steps for x in per])
+    tot = [res["fetch"]["stage_bytes_per_step"][k] + res["write"]["stage_bytes_per_step"][k] for k in range(3)]
+    from pe_amd.batch import load_schedule
+    out = {"steps_counted": steps, "window": f"steps 0..{steps}",
+           "schedule": load_schedule("batched_4096_worlds"), "stages": list(stages),
+           "stage_bytes_per_launch": tot, "fetch": res["fetch"], "write": res["write"]}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=120)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=120, help="timed steps: the batched workload's window from the drop")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed graph replays, from a snapshot of step 0")
     ap.add_argument("--settle", type=int, default=150,
-                    help="batched worlds are stepped this many steps before tuning and the warmup")
-    ap.add_argument("--no-tune", action="store_true")
+                    help="the settled-pile figure times steps settle..settle+steps; --tune tunes after settle steps")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner now instead of using the tuning table")
+    ap.add_argument("--write-schedule", default=None, help="with --tune: write the chosen schedule (JSON) here")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=150.0)
-    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-rates", default="150,40", help="Poisson arrival rates (requests/s), one served run each")
+    ap.add_argument("--serve-seconds", type=float, default=12.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=192, help="distinct request worlds the arrivals cycle through")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU work per precision of the CPU baseline")
     ap.add_argument("--plain-steps", type=int, default=600)
     ap.add_argument("--no-mix", action="store_true")
     ap.add_argument("--mix-bodies", type=int, default=50000)
     ap.add_argument("--mix-warm", type=int, default=240)
     ap.add_argument("--mix-steps", type=int, default=60)
     ap.add_argument("--mix-cpu-steps", type=int, default=2)
-    ap.add_argument("--profile-leg", choices=["box_rain_10k", "shape_mix"], default=None,
-                    help="run only this plain-path leg on the GPU (for a profiler) and print its own line")
+    ap.add_argument("--profile", choices=["batched", "box_rain_10k", "shape_mix"], default=None,
+                    help="run only this leg's timed window (for a profiler) and print its own line")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first copy 1 GiB device-to-device with torch (a kernel of known traffic, "
+                    help="first copy 1 GiB device-to-device three times (a kernel of known traffic, "
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH", "WRITE"), default=None,
+                    help="summarise two rocprofv3 --pmc outputs of --profile batched into profiles/batched_traffic.json")
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary, args.steps, os.path.join(ROOT, "profiles", "batched_traffic.json"))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build steps one GPU; --gpus {args.gpus} is not supported "
               f"(worlds shard across GPUs with no step-path traffic; see the design notes)", file=sys.stderr)
         sys.exit(2)
-
     if args.calibrate_copy:
-        import torch
-        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        log("calibration copy: 3 x 1 GiB read + 1 GiB written")
-        del x, y
+        copy_calibration()
 
     from pe_amd import scenes
-    if args.profile_leg == "box_rain_10k":
+    if args.profile == "box_rain_10k":
         c1 = scenes.box_rain_scene(10000, seed=1001)
         dt1, cnt1 = run_plain(c1, args.plain_steps)
-        print(json.dumps({"profile_leg": args.profile_leg, "steps": args.plain_steps,
+        print(json.dumps({"profile": args.profile, "steps": args.plain_steps,
                           "ms_per_step": 1000 * dt1 / args.plain_steps, "final_counts": cnt1}), flush=True)
         return
-    if args.profile_leg == "shape_mix":
+    if args.profile == "shape_mix":
         args.no_cpu = True
-        print(json.dumps({"profile_leg": args.profile_leg, **run_shape_mix(args)}), flush=True)
+        print(json.dumps({"profile": args.profile, **run_shape_mix(args)}), flush=True)
         return
     t0 = time.perf_counter()
     worlds = scenes.batch_worlds(4096, 8, 512, seed=1003)
     log(f"generated 4096 worlds in {time.perf_counter() - t0:.1f}s")
-
-    bt = run_batched(args, worlds)
+    sched, sched_src = batched_schedule(args, worlds)
+    bt = run_batched(args, worlds, sched)
     K = args.steps
     value = bt["n_dyn"] * K / bt["wall"]
     ms_step = 1000.0 * bt["wall"] / K
+    if args.profile == "batched":
+        bt["batch"].close()
+        print(json.dumps({"profile": "batched", "steps": K, "ms_per_step": ms_step,
+                          "device_ms_per_step": bt["dev_ms"] / K, "body_steps_per_s": value}), flush=True)
+        return
     dev_step_ms = bt["dev_ms"] / K
-    B = step_bytes(bt["n"], bt["n_dyn"], bt["pairs"], bt["points"])
-    step_gbs = B / (dev_step_ms * 1e-3) / 1e9
-    sc = bt["stage_counts"]
-    sb = stage_bytes(sc["bodies"], sc["dynamic"], sc["pairs"], sc["points"])
+    B = sum(bt["stage_bytes"])
     names = ["k_world_broad (integrate + sweep-and-prune, all size classes)", "k_narrow_list (narrowphase)",
              "k_group_solve (solver setup + colouring + Gauss-Seidel, all size classes)"]
     k = max(range(3), key=lambda i: bt["stage_ms"][i])
-    achieved = sb[k] / (bt["stage_ms"][k] * 1e-3) / 1e9
+    achieved = bt["stage_bytes"][k] / (bt["stage_ms"][k] * 1e-3) / 1e9
     log(f"batched: {value:.3e} body-steps/s, {ms_step:.3f} ms/step wall, {dev_step_ms:.3f} ms/step device, "
-        f"pairs {bt['pairs']:.0f} points {bt['points']:.0f} bytes/step {B / 1e6:.1f} MB -> {step_gbs:.1f} GB/s; "
-        f"stage ms {['%.3f' % x for x in bt['stage_ms']]}, dominant {names[k]} {achieved:.1f} GB/s")
+        f"mean pairs {bt['pairs']:.0f} points {bt['points']:.0f}; stage ms {['%.3f' % x for x in bt['stage_ms']]}, "
+        f"dominant {names[k]} {achieved:.1f} GB/s; settled {bt['settled']}")
     bt["batch"].close()
 
-    # HBM bytes per launch group measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this same
-    # command, scaled by the counters' reading for a 1 GiB copy (profiles/batched_traffic.json).
-    traffic = None
+    # HBM bytes per launch group from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+    # `--profile batched` (profiles/batched_traffic.json), when they ran this schedule and window
+    traffic, traffic_note = None, "no counter passes recorded"
     tp = os.path.join(ROOT, "profiles", "batched_traffic.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("stage_bytes_per_launch", [None, None, None])[k]
+            tj = json.load(f)
+        same = {kk: list(v) for kk, v in tj.get("schedule", {}).items() if kk in sched} == \
+            {kk: list(v) for kk, v in sched.items() if kk in ("class_upper_bodies", "block", "solver_group_bodies")}
+        if same and tj.get("window") == f"steps 0..{K}":
+            traffic, traffic_note = tj["stage_bytes_per_launch"][k], "counter passes over this schedule and window"
+        else:
+            traffic_note = "counter passes recorded for another schedule or window: not used"
 
     out = {
         "metric": "body-steps/s batched+served vs plain GPU at matched accuracy; p50/p99 step latency",
         "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched: 4096 worlds, log-uniform 8-512 bodies (boxes + spheres), "
-                               f"~496k bodies, dt 1/60, 10 iterations, hipGraph step, timed from step "
-                               f"{bt['start_step']}",
+        "config": {"workload": f"batched: 4096 worlds, log-uniform 8-512 bodies (boxes + spheres), "
+                               f"~496k bodies, dt 1/60, 10 iterations, hipGraph step, steps 0..{K} from the drop",
                    "worlds": len(worlds), "bodies": bt["n"], "dynamic_bodies": bt["n_dyn"],
-                   "autotuned_block_and_solver_group": {str(k): list(v) for k, v in bt["tuned"].items()}},
+                   "schedule": sched, "schedule_source": sched_src, "overflow_worlds": bt["counts"]["overflow_worlds"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": names[k], "algorithmic_bytes_per_launch": sb[k], "launch_ms": bt["stage_ms"][k],
-                     "stage_ms": bt["stage_ms"], "stage_algorithmic_bytes": sb,
-                     "whole_step": {"algorithmic_bytes": B, "ms": dev_step_ms, "achieved_gbs": step_gbs}},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
+                     "kernel": names[k], "algorithmic_bytes_per_launch": bt["stage_bytes"][k],
+                     "launch_ms": bt["stage_ms"][k], "window": f"steps 0..{K}, eager steps with hipEvents "
+                                                               f"between the launch groups",
+                     "stage_ms": bt["stage_ms"], "stage_algorithmic_bytes": bt["stage_bytes"],
+                     "mean_pairs": bt["pairs"], "mean_points": bt["points"],
+                     "whole_step": {"algorithmic_bytes": B, "ms": dev_step_ms,
+                                    "achieved_gbs": B / (dev_step_ms * 1e-3) / 1e9}},
+        "settled": bt["settled"],
     }
     if not args.no_plain:
         c1 = scenes.box_rain_scene(10000, seed=1001)
         dt1, cnt1 = run_plain(c1, args.plain_steps)
         plain_c1 = c1.n_dynamic() * args.plain_steps / dt1
-        plain_c3 = run_plain_sample(worlds[:64], 20, settle=args.settle)
+        plain_c3 = run_plain_sample(worlds[:64], K)
         out["plain"] = {"box_rain_10k_body_steps_per_s": plain_c1, "box_rain_10k_steps": args.plain_steps,
                         "box_rain_10k_ms_per_step": 1000 * dt1 / args.plain_steps, "box_rain_10k_final_counts": cnt1,
+                        "box_rain_10k_roofline": plain_stage_roofline(c1, 0, args.plain_steps),
                         "batch_worlds_one_at_a_time_body_steps_per_s": plain_c3,
-                        "batch_worlds_one_at_a_time_sample": f"first 64 worlds, {args.settle} untimed then 20 timed "
-                                                             f"steps each",
+                        "batch_worlds_one_at_a_time_sample": f"first 64 worlds, steps 0..{K} each",
                         "batched_over_plain_same_worlds": value / plain_c3,
                         "batched_over_plain_box_rain_10k": value / plain_c1}
         log(f"plain: c1 {plain_c1:.3e} body-steps/s ({1000 * dt1 / args.plain_steps:.2f} ms/step), "
@@ -382,11 +563,25 @@ def main():
         out["shape_mix"] = mx
         log(f"shape mix: {mx}")
     if not args.no_serve:
-        sv = run_served(args)
-        out["served"] = sv
-        log(f"served: {sv}")
+        pool = served_pool(args)
+        runs = [run_served(args, pool, float(r), 1004 + i) for i, r in enumerate(args.serve_rates.split(","))]
+        plain_same = run_plain_sample(pool[:32], 20, settle=150)
+        out["served"] = {"workload": f"Poisson arrivals for {args.serve_seconds:g} s at each rate; body counts "
+                                     f"log-uniform 16-4096, horizons U(30, 600); mixed shapes as the 50k shape mix "
+                                     f"(boxes, spheres, cylinders, hulls over a 16x16 heightfield), {len(pool)} "
+                                     f"distinct request worlds cycled; HBM budget {args.serve_budget_gb:g} GB",
+                         "runs": runs, "plain_same_worlds_body_steps_per_s": plain_same,
+                         "plain_same_worlds_sample": "first 32 request worlds, 150 untimed then 20 timed steps each",
+                         "served_over_plain_same_worlds": runs[0]["body_steps_per_s"] / plain_same}
+        log(f"served: {out['served']}")
     if not args.no_cpu:
-        cb = cpu_baseline(worlds, bt["start_states"], bt["start_step"], args.cpu_threads, args.cpu_seconds)
+        th = cpu_threads(args)
+        cb = cpu_baseline(worlds, K, th, args.cpu_seconds, 32)
+        cb["cpu_model"] = cpu_model()
+        cb["host_cpus_available"] = len(os.sched_getaffinity(0))
+        from pyoracle import ref_available
+        if ref_available(64):
+            cb["fp64"] = cpu_baseline(worlds, K, th, args.cpu_seconds, 64)
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb}")
     print(json.dumps(out), flush=True)

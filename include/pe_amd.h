@@ -79,6 +79,8 @@ typedef struct pe_scene_desc {
     const pe_mesh_desc* meshes;       /* [n_meshes] */
     const int32_t* mesh_index;        /* [n] mesh of body i if it is convex / concave; NULL if n_meshes = 0 */
     const int32_t* ignore_collision;  /* [n] RigidBody::setIgnoreCollision (0/1: the body is in no pair), or NULL */
+    const double* lifetime;           /* [n] RigidBody::setLifeTime, seconds (the body leaves the world once it
+                                         has been stepped that long; >= 1e30 never), or NULL (never) */
 } pe_scene_desc;
 
 /* Joint constraints (src/rigid/constraint/constraint/{ball,hinge,slider}_joint_constraint.h), as a
@@ -167,6 +169,43 @@ int pe_world_add_joints(pe_world* w, int32_t n, const pe_joint_desc* joints);
 /* RigidBody::addIgnoreCollisionId for n body pairs [2n]: the two bodies of a pair never form a
  * broadphase pair (broad_phase_base.cpp:12-13), as for the bodies a joint connects. */
 int pe_world_add_ignore_pairs(pe_world* w, int32_t n, const int32_t* pairs);
+
+/* ---- sleeping, lifetimes, removal, collision reports (single world) ----
+ * World::updateObjectStatus (world.cpp:35-119) with the sequential loop's rules: a body below the
+ * sleep thresholds (scaled by (static + 1) / (static + dynamic + 1) neighbour counts of the last
+ * narrowphase, simple_narrow_phase.cpp:43-62) for `time` seconds falls asleep (velocities zeroed) and
+ * is neither moved nor damped until a velocity reaches its scaled threshold; a body leaves the world
+ * once it has lived its lifetime (pe_scene_desc.lifetime; RigidBody::step, rigidbody.cpp:215-218).
+ * The batched and served paths keep the reference's defaults (no sleeping, no lifetimes). */
+
+/* World::setSleepLinVel2Threshold, setSleepAngVel2Threshold, setSleepTimeThreshold (world.h:26-28);
+ * all zero (the reference's default) never puts a body to sleep. */
+int pe_world_set_sleep(pe_world* w, double lin_vel2, double ang_vel2, double time);
+/* World::removeRigidBody (world.cpp:223-231): the body leaves the sweep and is no longer stepped
+ * (its last state stays readable).  Not for a body with joints. */
+int pe_world_remove_body(pe_world* w, int32_t body);
+/* Per body [n] (each pointer may be NULL): flags (bit 0 RigidBody::isSleep, bit 1 removed: its
+ * lifetime ended or pe_world_remove_body), the static and dynamic neighbour counts of the last
+ * narrowphase (cleared by the next integrate), RigidBody::getSleepTime.  Synchronises. */
+int pe_world_get_body_status(pe_world* w, int32_t* flags, int32_t* static_count, int32_t* dynamic_count,
+                             float* sleep_time);
+/* RigidBody::addCollisionCallback (on = 1) / clearCollisionCallbacks (on = 0) for n bodies: from the
+ * next narrowphase, each contact result with points of which one of them is a body reports one
+ * pe_collision_event (World::execCollisionCallbacks, world.cpp:129-192). */
+int pe_world_set_collision_reports(pe_world* w, int32_t n, const int32_t* bodies, int32_t on);
+/* One contact result's report: the host calls body_a's callbacks with (body_a, body_b,
+ * position + normal * depth, -normal, -velocity) and body_b's with (body_b, body_a, position, normal,
+ * velocity), as World::execCollisionCallbacks does. */
+typedef struct pe_collision_event {
+    int32_t body_a, body_b;   /* the result's objects A and B (the pair's order) */
+    float position[3];        /* mean contact point (world, on B) */
+    float normal[3];          /* normalised sum of the contact normals (B -> A) */
+    float depth;              /* mean contact distance */
+    float velocity[3];        /* mean of A's minus B's velocity at the contact points' local positions */
+} pe_collision_event;
+/* The reports of the last narrowphase, in pair order: at most cap into out, *count = all of them.
+ * Synchronises. */
+int pe_world_get_collision_events(pe_world* w, int32_t cap, pe_collision_event* out, int32_t* count);
 
 /* Synchronises the world's stream, then reports capacity errors raised by the steps since the
  * last report (PE_ERR_CAPACITY, the causes in pe_last_error): more broadphase pairs than max_pairs

@@ -70,6 +70,7 @@ template <class R> Handle<R>* create(const pe_scene_desc* sc, const pe_world_par
         b.ang_damp = sc->angular_damping ? R(sc->angular_damping[i]) : R(0);
         b.kinematic = sc->kinematic[i] != 0;
         b.ignore = sc->ignore_collision && sc->ignore_collision[i] != 0;
+        if (sc->lifetime) b.life_time = R(sc->lifetime[i]);
     }
     return h;
 }
@@ -163,6 +164,32 @@ template <class R> int get_poly(Handle<R>* h, int body, int32_t* counts, double*
         return (int)w.joint_order.size();                                                                \
     }                                                                                                    \
     extern "C" void P##_set_color_order(void* h, int on) { ((Handle<R>*)h)->w.use_color_order = on != 0; } \
+    extern "C" void P##_set_sleep(void* h, double lin2, double ang2, double t) {                         \
+        auto& w = ((Handle<R>*)h)->w;                                                                    \
+        w.sleep_lin2 = R(lin2); w.sleep_ang2 = R(ang2); w.sleep_time = R(t);                             \
+    }                                                                                                    \
+    extern "C" void P##_get_status(void* h, int32_t* status, int32_t* scount, int32_t* dcount, double* st) { \
+        auto& w = ((Handle<R>*)h)->w;                                                                    \
+        for (size_t i = 0; i < w.bodies.size(); i++) {                                                   \
+            const Body<R>& b = w.bodies[i];                                                              \
+            if (status) status[i] = (b.sleep ? 1 : 0) | (b.removed ? 2 : 0);                             \
+            if (scount) scount[i] = b.static_count;                                                      \
+            if (dcount) dcount[i] = b.dynamic_count;                                                     \
+            if (st) st[i] = b.sleep_time;                                                                \
+        }                                                                                                \
+    }                                                                                                    \
+    extern "C" void P##_set_report(void* h, int body, int on) { ((Handle<R>*)h)->w.bodies[body].report = on != 0; } \
+    extern "C" void P##_remove_body(void* h, int body) { ((Handle<R>*)h)->w.bodies[body].removed = true; } \
+    extern "C" int P##_num_events(void* h) { return (int)((Handle<R>*)h)->w.events.size(); }            \
+    extern "C" void P##_get_events(void* h, int32_t* ab, double* data) {                                 \
+        auto& ev = ((Handle<R>*)h)->w.events;                                                            \
+        for (size_t i = 0; i < ev.size(); i++) {                                                         \
+            ab[2 * i] = ev[i].a; ab[2 * i + 1] = ev[i].b;                                                \
+            double* o = data + 10 * i;                                                                   \
+            for (int k = 0; k < 3; k++) { o[k] = ev[i].pos[k]; o[3 + k] = ev[i].nor[k]; o[7 + k] = ev[i].vel[k]; } \
+            o[6] = ev[i].depth;                                                                          \
+        }                                                                                                \
+    }                                                                                                    \
     extern "C" void P##_step(void* h, int nsteps) {                                                      \
         auto* hh = (Handle<R>*)h;                                                                        \
         for (int i = 0; i < nsteps; i++) world_step(hh->w, hh->s);                                       \

@@ -211,6 +211,13 @@ template <class R> struct Body {
     R mu = R(0.5), e = R(0.5), lin_damp = 0, ang_damp = 0;
     bool kinematic = false, ignore = false;
     const Poly<R>* poly = nullptr;   // mesh of a box / cylinder / convex / concave body
+    // World::updateObjectStatus bookkeeping (rigidbody.h:36, 72-83): asleep, seconds below the
+    // sleep thresholds, lifetime and time lived, contacts with static / dynamic neighbours in the
+    // last narrowphase; removed once its lifetime is over (or World::removeRigidBody)
+    bool sleep = false, removed = false;
+    R sleep_time = 0, life_time = Lim<R>::max(), last_time = 0;
+    int static_count = 0, dynamic_count = 0;
+    bool report = false;             // has collision callbacks (RigidBody::addCollisionCallback)
 };
 
 template <class R> M3<R> poly_inertia(const Poly<R>& p);
@@ -357,6 +364,13 @@ template <class R> struct ContactResult {
 
 // ---------------------------------------------------------------- world
 template <class R> struct Joint;
+// What World::execCollisionCallbacks (world.cpp:129-192) hands the callbacks of a contact result with
+// points: A's callbacks get (A, B, pos + nor * depth, -nor, -vel), B's get (B, A, pos, nor, vel).
+template <class R> struct CollisionEvent {
+    int a, b;
+    V3<R> pos, nor, vel;
+    R depth;
+};
 template <class R> struct World {
     std::vector<Body<R>> bodies;
     std::vector<Joint<R>> joints;             // World::addConstraint order
@@ -373,6 +387,8 @@ template <class R> struct World {
     int ncolors = 0;
     std::vector<int> color_of;                // per result
     std::vector<std::shared_ptr<Poly<R>>> polys;   // meshes the bodies point at
+    R sleep_lin2 = 0, sleep_ang2 = 0, sleep_time = 0;   // World::setSleep*Threshold (world.h:26-28)
+    std::vector<CollisionEvent<R>> events;    // the last narrowphase's collision reports
 };
 
 // ContactResult::addContactPoint (contact_result.cpp:88-142)
@@ -801,15 +817,44 @@ template <class R> inline bool sphere_sphere(const World<R>& w, ContactResult<R>
 namespace pe_oracle {
 
 // ---------------------------------------------------------------- stages
-// World::updateObjectStatus + applyExternalForce (world.cpp:35-127).  Sleep thresholds are
-// zero in every supported scene, so a body never sleeps; the sleep bookkeeping is omitted.
+// World::updateObjectStatus + applyExternalForce (world.cpp:35-127), the sequential loop's
+// semantics (world.cpp:85-117; the threaded loop erases from the array it is iterating, so the body
+// after a removed one skips that step and the last is visited twice -- with one removal, of the last
+// body of the array, the two agree).  A sleeping body wakes when a velocity reaches its threshold
+// scaled by (static + 1) / (static + dynamic + 1) neighbour counts of the last narrowphase, and is not
+// moved or damped; an awake body is stepped (RigidBody::step, which ends its lifetime,
+// rigidbody.cpp:215-218), damped, and falls asleep (velocities zeroed) after sleep_time seconds below
+// the scaled thresholds.  Gravity then acts on every body left, asleep or not.
 template <class R> inline void stage_integrate(World<R>& w) {
     for (auto& b : w.bodies) {
+        if (b.removed) continue;
         if (b.kinematic) { body_step(b, w.dt); continue; }
-        body_step(b, w.dt);
-        body_damping(b, w.dt);
+        const R ratio = R(b.static_count + 1) / R(b.dynamic_count + b.static_count + 1);
+        if (b.sleep) {
+            if (sqnorm(b.v) >= w.sleep_lin2 * ratio || sqnorm(b.w) >= w.sleep_ang2 * ratio) {
+                b.sleep = false;
+                b.sleep_time = 0;
+            }
+        } else {
+            b.last_time += w.dt;
+            if (b.last_time >= b.life_time) { b.removed = true; continue; }
+            body_step(b, w.dt);
+            body_damping(b, w.dt);
+            if (sqnorm(b.v) < w.sleep_lin2 * ratio && sqnorm(b.w) < w.sleep_ang2 * ratio) {
+                b.sleep_time += w.dt;
+                if (b.sleep_time >= w.sleep_time) {
+                    b.sleep = true;
+                    b.v = V3<R>();
+                    b.w = V3<R>();
+                }
+            } else {
+                b.sleep_time = 0;
+            }
+        }
+        b.static_count = 0;
+        b.dynamic_count = 0;
     }
-    for (auto& b : w.bodies) body_gravity(b, w.gravity, w.dt);
+    for (auto& b : w.bodies) if (!b.removed) body_gravity(b, w.gravity, w.dt);
 }
 
 // BroadPhaseBase::validateCollisionPair (broad_phase_base.cpp:5-19)
@@ -824,12 +869,13 @@ template <class R> inline bool validate_pair(const Body<R>& a, const Body<R>& b)
 // BroadPhaseSweepAndPrune::calcCollisionPairs (broad_phase_sweep_and_prune.cpp:6-41).
 // The reference sorts with std::sort (unspecified order among equal keys); this restatement
 // breaks ties by body index, which is the order the device radix sort produces.
+// Removed bodies are not in the world's array; the others keep their order.
 template <class R> inline void stage_broadphase(World<R>& w) {
-    const int n = (int)w.bodies.size();
+    std::vector<int> order;
+    for (int i = 0; i < (int)w.bodies.size(); i++) if (!w.bodies[i].removed) order.push_back(i);
+    const int n = (int)order.size();
     if (n < 2) return;  // pairs are left untouched, as in the reference
     w.pairs.clear();
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
     const int ax = w.axis;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return w.bodies[x].aabb_min[ax] < w.bodies[y].aabb_min[ax];
@@ -853,6 +899,50 @@ template <class R> inline void stage_broadphase(World<R>& w) {
     if (v[1] > v[0]) axis = 1;
     if (v[2] > v[axis]) axis = 2;
     w.axis = axis;
+}
+
+// SimpleNarrowPhase::calcContactResults, its second loop (simple_narrow_phase.cpp:43-62): a result with
+// points counts, for each non-kinematic body of it, a static neighbour when the other body sleeps or
+// is kinematic and a dynamic one otherwise.
+template <class R> inline void count_neighbours(World<R>& w) {
+    for (size_t i = 0; i < w.pairs.size(); i++) {
+        const ContactResult<R>& cr = w.results[i];
+        if (cr.npts == 0) continue;
+        Body<R>& A = w.bodies[cr.a];
+        Body<R>& B = w.bodies[cr.b];
+        if (!B.kinematic) { if (A.sleep || A.kinematic) B.static_count++; else B.dynamic_count++; }
+        if (!A.kinematic) { if (B.sleep || B.kinematic) A.static_count++; else A.dynamic_count++; }
+    }
+}
+// World::execCollisionCallbacks (world.cpp:129-192): for each result with points whose bodies have
+// callbacks, the mean contact position, the normalised sum of the normals, the mean distance and the
+// mean relative velocity of the contact points (RigidBody::getLinearVelocityAtLocalPoint,
+// rigidbody.cpp:138-140), in pair order.
+template <class R> inline void collision_events(World<R>& w) {
+    w.events.clear();
+    for (size_t i = 0; i < w.pairs.size(); i++) {
+        const ContactResult<R>& cr = w.results[i];
+        if (cr.npts == 0) continue;
+        const Body<R>& A = w.bodies[cr.a];
+        const Body<R>& B = w.bodies[cr.b];
+        if (!A.report && !B.report) continue;
+        CollisionEvent<R> e{cr.a, cr.b, V3<R>(), V3<R>(), V3<R>(), R(0)};
+        for (int j = 0; j < cr.npts; j++) {
+            const ContactPoint<R>& c = cr.pts[j];
+            e.pos = e.pos + c.wpos;
+            e.nor = e.nor + c.normal;
+            e.depth += c.dist;
+            e.vel = e.vel + (A.v + cross(A.w, c.la));
+            e.vel = e.vel - (B.v + cross(B.w, c.lb));
+        }
+        const R n = R(cr.npts);
+        e.pos = e.pos / n;
+        const R z = sqnorm(e.nor);
+        if (z > R(0)) e.nor = e.nor / std::sqrt(z);
+        e.depth /= n;
+        e.vel = e.vel / n;
+        w.events.push_back(e);
+    }
 }
 
 // SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41)
@@ -912,6 +1002,8 @@ template <class R> inline void stage_narrowphase(World<R>& w) {
         }
         sort_contact_points(cr);
     }
+    count_neighbours(w);
+    collision_events(w);
 }
 
 // The claim priority of pair p: a fixed bijection of [0, 2^31) (odd multiplications modulo 2^31

@@ -65,6 +65,8 @@ class _Base:
             f("add_ignore")(ctypes.c_void_p(self.h), ctypes.c_int(a), ctypes.c_int(b))
         for j in getattr(scene, "joints", None) or []:
             f("add_joint")(ctypes.c_void_p(self.h), ctypes.byref(j))
+        if getattr(scene, "sleep", None):
+            f("set_sleep")(ctypes.c_void_p(self.h), *[ctypes.c_double(x) for x in scene.sleep])
         for name in ("broadphase", "num_pairs", "get_axis", "num_results"):
             f(name).restype = ctypes.c_int
             f(name).argtypes = [_VP]
@@ -130,6 +132,33 @@ class _Base:
     def set_pairs(self, ab):
         ab = np.ascontiguousarray(ab, dtype=np.int32)
         self._f("set_pairs")(ctypes.c_void_p(self.h), ctypes.c_int(len(ab)), _i(ab))
+
+    def status(self):
+        """Per body: (status bits: 1 asleep, 2 removed; static and dynamic neighbour counts of the last
+        narrowphase; sleep time)."""
+        st, sc, dc, t = (np.zeros(self.n, np.int32), np.zeros(self.n, np.int32), np.zeros(self.n, np.int32),
+                         np.zeros(self.n))
+        self._f("get_status")(ctypes.c_void_p(self.h), _i(st), _i(sc), _i(dc), _d(t))
+        return st, sc, dc, t
+
+    def set_report(self, body, on=True):
+        """RigidBody::addCollisionCallback (on) / clearCollisionCallbacks (off)."""
+        self._f("set_report")(ctypes.c_void_p(self.h), ctypes.c_int(body), ctypes.c_int(1 if on else 0))
+
+    def remove_body(self, body):
+        """World::removeRigidBody."""
+        self._f("remove_body")(ctypes.c_void_p(self.h), ctypes.c_int(body))
+
+    def events(self):
+        """The last narrowphase's collision reports: (bodies [E, 2] (A, B), data [E, 10]: mean position,
+        normalised normal sum, mean depth, mean relative velocity)."""
+        f = self._f("num_events")
+        f.restype = ctypes.c_int
+        n = f(ctypes.c_void_p(self.h))
+        ab, data = np.zeros((n, 2), np.int32), np.zeros((n, 10))
+        if n:
+            self._f("get_events")(ctypes.c_void_p(self.h), _i(ab), _d(data))
+        return ab, data
 
     def contacts(self):
         """(npts [P], swap [P], points [P, 8, 13]) for every contact result slot."""
@@ -221,6 +250,8 @@ class RefWorld(_Base):
             pe_amd.lib()
         lib = _load(os.path.join(HERE, "_ref", "libperef32_amd.so" if amd_stages else f"libperef{bits}.so"))
         lib.pref_init_threads(ctypes.c_int(threads))
+        if threads == 1:
+            lib.pref_set_threads(ctypes.c_int(1))
         lib.pref_step_timed.restype = ctypes.c_double
         lib.pref_step_timed.argtypes = [_VP, ctypes.c_int]
         super().__init__(lib, scene)
@@ -230,6 +261,16 @@ class RefWorld(_Base):
             if lib.pref_use_amd_stages(ctypes.c_void_p(self.h), ctypes.c_int(amd_stages),
                                        ctypes.c_int(scene.iterations)) != 0:
                 raise RuntimeError("the adapter could not replace the stages")
+
+    def callbacks(self):
+        """The collision callback invocations of the last narrowphase: (ids [C, 2] (self, other),
+        data [C, 9] (position, normal, velocity as the callback received them)), in the order the
+        reference's threads made them."""
+        n = self.lib.pref_num_callbacks(ctypes.c_void_p(self.h))
+        ids, data = np.zeros((n, 2), np.int32), np.zeros((n, 9))
+        if n:
+            self.lib.pref_get_callbacks(ctypes.c_void_p(self.h), _i(ids), _d(data))
+        return ids, data
 
     def step_checked(self, n=1):
         """World::step through the installed stages; raises when an adapter stage throws."""

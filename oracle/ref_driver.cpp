@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <mutex>
 #include <unordered_map>
 #include <chrono>
 
@@ -51,6 +52,11 @@ struct RefWorld : public pe_intf::World {
 #endif
     std::unordered_map<const pe_phys_object::RigidBody*, int> index;
     std::vector<pe_phys_object::RigidBody*> owned;
+    // collision callback invocations of the last narrowphase: (self, other) and position, normal,
+    // velocity as the callback received them (the threaded loop calls them in any order)
+    std::mutex cb_mtx;
+    std::vector<int> cb_ids;
+    std::vector<double> cb_data;
     void integrate() {
         updateObjectStatus();
         calcDamageEffects();
@@ -62,7 +68,13 @@ struct RefWorld : public pe_intf::World {
     }
     void narrowphase() {
         _narrow_phase->calcContactResults(_collision_pairs, _contact_results);
+        cb_ids.clear();
+        cb_data.clear();
         execCollisionCallbacks();
+    }
+    bool alive(const pe_phys_object::RigidBody* rb) const {
+        for (auto* x : _rigid_objects) if (x == rb) return true;
+        return false;
     }
     void solve() {
         _constraint_solver->setupSolver(getDt(), getGravity(), _rigid_objects, _contact_results, _constraints);
@@ -137,6 +149,14 @@ REF_API void pref_init_threads(int n) {
     utils::ThreadPool::init((uint32_t)(n > 0 ? n : 0));
     if (!g_hook) { g_hook = true; std::atexit(retire_pool_workers); }
 }
+// A pool of exactly n threads from now on (the current workers are retired first); for
+// comparisons that need the reference's loops to run on one thread.
+REF_API void pref_set_threads(int n) {
+    if (n <= 0) return;
+    if (utils::ThreadPool::getInstance()._size == (uint32_t)n) return;
+    retire_pool_workers();
+    pref_init_threads(n);
+}
 
 REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* prm) {
     auto* w = new RefWorld();
@@ -195,6 +215,7 @@ REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* 
         rb->setAngularDamping(sc->angular_damping ? pe::Real(sc->angular_damping[i]) : pe::Real(0));
         rb->setKinematic(sc->kinematic[i] != 0);
         rb->setIgnoreCollision(sc->ignore_collision && sc->ignore_collision[i] != 0);
+        if (sc->lifetime) rb->setLifeTime(pe::Real(sc->lifetime[i]));
         w->addRigidBody(rb);
         w->index[rb] = i;
         w->owned.push_back(rb);
@@ -240,6 +261,51 @@ REF_API int pref_add_joint(void* h, const pe_joint_desc* d) {
 REF_API void pref_add_ignore(void* h, int a, int b) {
     auto* w = (RefWorld*)h;
     w->owned[a]->addIgnoreCollisionId(w->owned[b]->getGlobalId());
+}
+
+// World::setSleepLinVel2Threshold / setSleepAngVel2Threshold / setSleepTimeThreshold
+REF_API void pref_set_sleep(void* h, double lin2, double ang2, double t) {
+    auto* w = (RefWorld*)h;
+    w->setSleepLinVel2Threshold(pe::Real(lin2));
+    w->setSleepAngVel2Threshold(pe::Real(ang2));
+    w->setSleepTimeThreshold(pe::Real(t));
+}
+// Per body: bit 0 isSleep, bit 1 no longer in the world; neighbour counts; sleep time.
+REF_API void pref_get_status(void* h, int32_t* status, int32_t* scount, int32_t* dcount, double* st) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->owned.size(); i++) {
+        const auto* rb = w->owned[i];
+        if (status) status[i] = (rb->isSleep() ? 1 : 0) | (w->alive(rb) ? 0 : 2);
+        if (scount) scount[i] = rb->getStaticCount();
+        if (dcount) dcount[i] = rb->getDynamicCount();
+        if (st) st[i] = (double)rb->getSleepTime();
+    }
+}
+// RigidBody::addCollisionCallback with a callback that records its arguments (on), or
+// clearCollisionCallbacks (off).
+REF_API void pref_set_report(void* h, int body, int on) {
+    auto* w = (RefWorld*)h;
+    auto* rb = w->owned[body];
+    rb->clearCollisionCallbacks();
+    if (!on) return;
+    rb->addCollisionCallback([w](pe_phys_object::RigidBody* self, pe_phys_object::RigidBody* other, const pe::Vector3& pos,
+                                 const pe::Vector3& nor, const pe::Vector3& vel) {
+        std::lock_guard<std::mutex> lk(w->cb_mtx);
+        w->cb_ids.push_back(w->index[self]);
+        w->cb_ids.push_back(w->index[other]);
+        for (const pe::Vector3* x : {&pos, &nor, &vel})
+            for (int k = 0; k < 3; k++) w->cb_data.push_back((double)(*x)[k]);
+    });
+}
+REF_API void pref_remove_body(void* h, int body) {
+    auto* w = (RefWorld*)h;
+    w->removeRigidBody(w->owned[body]);
+}
+REF_API int pref_num_callbacks(void* h) { return (int)((RefWorld*)h)->cb_ids.size() / 2; }
+REF_API void pref_get_callbacks(void* h, int32_t* ids, double* data) {
+    auto* w = (RefWorld*)h;
+    for (size_t i = 0; i < w->cb_ids.size(); i++) ids[i] = w->cb_ids[i];
+    for (size_t i = 0; i < w->cb_data.size(); i++) data[i] = w->cb_data[i];
 }
 
 REF_API void pref_world_destroy(void* h) {

@@ -1070,10 +1070,25 @@ int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes) {
     return PE_OK;
 }
 
+namespace {
+// Batched and served worlds step with the reference's default (zero) sleep thresholds and bodies
+// that live forever: a finite lifetime needs the single-world path.
+int check_batched_scene(const pe_scene_desc* sc) {
+    if (int rc = check_scene(sc)) return rc;
+    if (sc->lifetime)
+        for (int i = 0; i < sc->n; i++)
+            if (!(sc->lifetime[i] >= 1e30)) {
+                pe_set_error("body " + std::to_string(i) + " has a finite lifetime: lifetimes need a pe_world");
+                return PE_ERR_INVALID;
+            }
+    return PE_OK;
+}
+}  // namespace
+
 int pe_batch_create(int32_t nworlds, const pe_scene_desc* scenes, const pe_world_params* prm, pe_batch** out) {
     if (nworlds <= 0 || !scenes || !prm || !out) { pe_set_error("pe_batch_create: invalid argument"); return PE_ERR_INVALID; }
     for (int w = 0; w < nworlds; w++) {
-        if (int rc = check_scene(&scenes[w])) return rc;
+        if (int rc = check_batched_scene(&scenes[w])) return rc;
         if (scenes[w].n > PE_BATCH_MAX_WORLD_BODIES) { pe_set_error("world too large for the batched kernel"); return PE_ERR_INVALID; }
     }
     set_lds_limits();
@@ -1887,7 +1902,7 @@ int pe_server_destroy(pe_server* S) {
 
 int pe_server_submit(pe_server* S, const pe_scene_desc* scene, int32_t horizon, int64_t* id) {
     if (!S || !scene || horizon <= 0) { pe_set_error("invalid argument"); return PE_ERR_INVALID; }
-    if (int rc = check_scene(scene)) return rc;
+    if (int rc = check_batched_scene(scene)) return rc;
     if (scene->n > S->cfg.max_world_bodies) {
         pe_set_error("request larger than the server admits");
         return PE_ERR_BUSY;

@@ -34,7 +34,7 @@ struct NoAllocLayout {
         skey = take(4 * (size_t)NA_MAX_BODIES);
         cand = take(4 * ((size_t)NA_MAX_BODIES + 1));
         hist = take(4 * 256 * (size_t)((NA_MAX_BODIES + RS_TILE - 1) / RS_TILE));
-        red = take(8 * 6 * 256);
+        red = take(8 * SAP_RED_VALUES * 256);
         chunk = take(4 * (size_t)NA_CHUNK_CAP);
         chunk_hit = take(4 * (size_t)NA_CHUNK_CAP);
         scan_part = take(4 * (size_t)(((NA_CHUNK_CAP > NA_MAX_BODIES + 1 ? NA_CHUNK_CAP : NA_MAX_BODIES + 1) + SCAN_TILE - 1) /

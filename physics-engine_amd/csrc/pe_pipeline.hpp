@@ -75,12 +75,50 @@ __device__ __forceinline__ void write_records(const BodyView& b, int i, int type
 }
 
 // ------------------------------------------------------------------ stage 0
-// World::updateObjectStatus (world.cpp:35-119) with zero sleep thresholds (a body never
-// sleeps), World::applyExternalForce (world.cpp:121-127), and the solver's clearTempVelocity
-// (sequential_impulse_solver.cpp:17-20) for body i.
-__device__ __forceinline__ void integrate_body(const BodyView& b, int i, float dt, const float g[3]) {
+// Sleeping, lifetimes and neighbour counts of a single world (World::updateObjectStatus,
+// world.cpp:35-119; RigidBody::step's lifetime, rigidbody.cpp:215-218; the narrowphase's neighbour
+// counts, simple_narrow_phase.cpp:43-62): per body, structure of arrays.
+constexpr int ST_SLEEP = 1, ST_REMOVED = 2;
+struct StatusView {
+    int32_t* flags;      // ST_SLEEP | ST_REMOVED
+    float* sleep_time;   // seconds below the sleep thresholds
+    float* life;         // lifetime (s)
+    float* last;         // time lived (s)
+    int32_t* scnt;       // static neighbours of the last narrowphase
+    int32_t* dcnt;       // dynamic neighbours of the last narrowphase
+    int32_t* report;     // 1: contacts are reported (collision callbacks)
+    float lin2, ang2, tsleep;   // World::setSleepLinVel2Threshold / AngVel2 / Time
+};
+__device__ __forceinline__ float sqnorm3(f3 a) { return a.x * a.x + (a.y * a.y + a.z * a.z); }
+
+// A body leaves the sweep: empty AABB (min +inf, max -inf; sorts last, no candidates, not in the
+// axis variance) and the ignore-collision flag.
+__device__ __forceinline__ void retire_body(const BodyView& b, int i) {
+    const float inf = __builtin_inff();
+    st3(b, F_AMIN, i, f3{inf, inf, inf});
+    st3(b, F_AMAX, i, f3{-inf, -inf, -inf});
+    b.at(F_SP + 3, i) = COLL_IGNORE;
+    const float a[8] = {inf, inf, inf, 0.0f, -inf, -inf, -inf, COLL_IGNORE};
+    st_rec<2>(b.abr(i), a);
+}
+
+// World::updateObjectStatus (world.cpp:35-119), World::applyExternalForce (world.cpp:121-127),
+// and the solver's clearTempVelocity (sequential_impulse_solver.cpp:17-20) for body i.  Without
+// STATUS no body sleeps (the reference's default thresholds are zero) or ends its lifetime; with it,
+// the sequential loop's rules (world.cpp:85-117): a sleeping body wakes when a velocity reaches its
+// threshold scaled by (static + 1) / (static + dynamic + 1), and is neither moved nor damped; an
+// awake body lives dt longer (and leaves the world at its lifetime), is moved and damped, and falls
+// asleep, velocities zeroed, after tsleep seconds below the scaled thresholds.
+template <bool STATUS = false>
+__device__ __forceinline__ void integrate_body(const BodyView& b, int i, float dt, const float g[3],
+                                               const StatusView* st = nullptr) {
     const int type = b.iat(I_TYPE, i);
     const bool kin = b.iat(I_KIN, i) != 0;
+    int flags = 0;
+    if constexpr (STATUS) {
+        flags = st->flags[i];
+        if (flags & ST_REMOVED) return;
+    }
     f3 pos = ld3(b, F_POS, i);
     m33 R = ld9(b, F_ROT, i);
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
@@ -89,17 +127,60 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
     float im = 0.0f;
     f3 tv{0, 0, 0}, tw{0, 0, 0};   // kinematic: temp velocities stay zero (rigidbody.cpp:72-80)
     if (!kin) {
-        integrate_pose(pos, R, v, w, dt);
+        bool awake = true;
+        float ratio = 0.0f;
+        if constexpr (STATUS) {
+            const int sc = st->scnt[i], dc = st->dcnt[i];
+            ratio = (float)(sc + 1) / (float)(dc + sc + 1);
+            if (flags & ST_SLEEP) {
+                awake = false;
+                if (sqnorm3(v) >= st->lin2 * ratio || sqnorm3(w) >= st->ang2 * ratio) {
+                    flags &= ~ST_SLEEP;
+                    st->sleep_time[i] = 0.0f;
+                }
+            } else {
+                const float lived = st->last[i] + dt;
+                st->last[i] = lived;
+                if (lived >= st->life[i]) {
+                    st->flags[i] = flags | ST_REMOVED;
+                    retire_body(b, i);
+                    return;
+                }
+            }
+        }
+        if (awake) {
+            integrate_pose(pos, R, v, w, dt);
+            st3(b, F_POS, i, pos);
+            st9(b, F_ROT, i, R);
+        }
         const m33 L = ld9(b, F_LINV, i);
         Iw = mmt(mm(R, L), R);
-        st3(b, F_POS, i, pos);
-        st9(b, F_ROT, i, R);
-        // RigidBody::applyDamping (rigidbody.cpp:201-205).  With zero damping the factor is
-        // pow(1, dt) = 1 exactly and the product leaves every component as it is, so the power
-        // is only evaluated for a damped body.
-        const float ld = b.at(F_LD, i), ad = b.at(F_AD, i);
-        if (ld != 0.0f) v = mul(v, powf(1.0f - ld, dt));
-        if (ad != 0.0f) w = mul(w, powf(1.0f - ad, dt));
+        if (awake) {
+            // RigidBody::applyDamping (rigidbody.cpp:201-205).  With zero damping the factor is
+            // pow(1, dt) = 1 exactly and the product leaves every component as it is, so the power
+            // is only evaluated for a damped body.
+            const float ld = b.at(F_LD, i), ad = b.at(F_AD, i);
+            if (ld != 0.0f) v = mul(v, powf(1.0f - ld, dt));
+            if (ad != 0.0f) w = mul(w, powf(1.0f - ad, dt));
+        }
+        if constexpr (STATUS) {
+            if (awake) {
+                if (sqnorm3(v) < st->lin2 * ratio && sqnorm3(w) < st->ang2 * ratio) {
+                    const float t = st->sleep_time[i] + dt;
+                    st->sleep_time[i] = t;
+                    if (t >= st->tsleep) {
+                        flags |= ST_SLEEP;
+                        v = f3{0.0f, 0.0f, 0.0f};
+                        w = f3{0.0f, 0.0f, 0.0f};
+                    }
+                } else {
+                    st->sleep_time[i] = 0.0f;
+                }
+            }
+            st->flags[i] = flags;
+            st->scnt[i] = 0;
+            st->dcnt[i] = 0;
+        }
         // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
         const float m = b.at(F_MASS, i);
         im = b.at(F_IM, i);

@@ -39,6 +39,7 @@ constexpr int OVF_CLIP = 8;          // a clipped polygon outgrew its buffer (co
 constexpr int SAP_CHUNK = 32;        // candidates per thread in the sweep
 static_assert(SAP_CHUNK <= 32, "a chunk's test outcomes are one 32-bit mask");
 constexpr int SAP_RED_THREADS = 256;
+constexpr int SAP_RED_VALUES = 7;   // axis reduction: centre sums (3), squared sums (3), bodies
 
 // The AABBs and the working memory of one sweep.  AABB minimum k of body i is
 // amin[k * fs + i * bs] (field-major SoA: fs = stride, bs = 1; records: fs = 1, bs = 8); rec holds
@@ -225,37 +226,43 @@ static __global__ void k_sap_finish(SapView s) {
 
 // Next SAP axis (broad_phase_sweep_and_prune.cpp:37-40): variance of AABB centres, reduced in
 // FP64 over fixed blocks (deterministic), argmax with the reference's tie rule.
+// A body whose AABB is empty (min > max) has left the world (World::removeRigidBody, an ended
+// lifetime): it sorts last with an infinite key, has no candidates and is not in the variance; with
+// fewer than two bodies left the axis stays (the reference's sweep returns early).
 static __global__ void __launch_bounds__(SAP_RED_THREADS) k_sap_axis_partial(SapView s) {
-    __shared__ double sh[6][SAP_RED_THREADS];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    __shared__ double sh[SAP_RED_VALUES][SAP_RED_THREADS];
+    double acc[SAP_RED_VALUES] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x) {
+        if (!(s.mn(0, i) <= s.mx(0, i))) continue;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const float c = (s.mn(k, i) + s.mx(k, i)) * 0.5f;
             acc[k] += (double)c;
             acc[3 + k] += (double)(c * c);
         }
+        acc[6] += 1.0;
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = acc[k];
+    for (int k = 0; k < SAP_RED_VALUES; k++) sh[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int o = SAP_RED_THREADS / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o)
 #pragma unroll
-            for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+            for (int k = 0; k < SAP_RED_VALUES; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
         __syncthreads();
     }
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 6; k++) s.red[k * gridDim.x + blockIdx.x] = sh[k][0];
+        for (int k = 0; k < SAP_RED_VALUES; k++) s.red[k * gridDim.x + blockIdx.x] = sh[k][0];
 }
 static __global__ void k_sap_axis_final(SapView s, int nblocks) {
     if (threadIdx.x != 0) return;
-    double t[6] = {0, 0, 0, 0, 0, 0};
+    double t[SAP_RED_VALUES] = {0, 0, 0, 0, 0, 0, 0};
     for (int b = 0; b < nblocks; b++)
-        for (int k = 0; k < 6; k++) t[k] += s.red[k * nblocks + b];
+        for (int k = 0; k < SAP_RED_VALUES; k++) t[k] += s.red[k * nblocks + b];
+    if (t[6] < 2.0) return;
     double v[3];
-    for (int k = 0; k < 3; k++) v[k] = t[3 + k] - t[k] * t[k] / (double)s.n;
+    for (int k = 0; k < 3; k++) v[k] = t[3 + k] - t[k] * t[k] / t[6];
     int axis = 0;
     if (v[1] > v[0]) axis = 1;
     if (v[2] > v[axis]) axis = 2;

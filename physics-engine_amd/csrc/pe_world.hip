@@ -56,7 +56,17 @@ struct Dev {
     const int32_t* jorder;   // [njoints] joint indices, colour by colour
     const int32_t* jcs;      // [njcol + 1] colour starts in jorder
     int njoints, njcol;
+    // sleeping, lifetimes, removals (World::updateObjectStatus) and collision reports
+    StatusView st;
+    int status_on;           // some body may sleep, end its lifetime or has been removed
+    int report_on;           // some body reports its contacts
+    float* events;           // [pcap][EVW] the last narrowphase's collision reports, by pair
+    int32_t* ev_flag;        // [pcap] 1: pair p has a report
 };
+
+// Collision report of a pair (World::execCollisionCallbacks, world.cpp:129-192): bodies A and B (int
+// bits), mean contact position, normalised sum of the normals, mean distance, mean relative velocity.
+constexpr int EVW = 12;
 
 // ------------------------------------------------------------------ stage 0: integrate
 // World::updateObjectStatus (world.cpp:35-119) with zero sleep thresholds, then
@@ -65,10 +75,82 @@ struct Dev {
 __global__ void k_integrate(Dev d) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) integrate_body(d.b, i, d.dt, d.g);
 }
+// The same with sleeping and lifetimes (world.cpp:85-117).
+__global__ void k_integrate_status(Dev d) {
+    const StatusView st = d.st;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x)
+        integrate_body<true>(d.b, i, d.dt, d.g, &st);
+}
 
-// Recomputes AABBs and scales only (after pe_world_set_state).
+// Recomputes AABBs and scales only (after pe_world_set_state); a removed body stays out of the sweep.
 __global__ void k_aabb_only(Dev d) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) refresh_body(d.b, i);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x) {
+        if (d.status_on && (d.st.flags[i] & ST_REMOVED)) retire_body(d.b, i);
+        else refresh_body(d.b, i);
+    }
+}
+
+// World::removeRigidBody (world.cpp:223-231) of body i.
+__global__ void k_remove_body(Dev d, int i) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    d.st.flags[i] |= ST_REMOVED;
+    retire_body(d.b, i);
+}
+
+// SimpleNarrowPhase's neighbour counts (simple_narrow_phase.cpp:43-62): for a pair with contact
+// points, each non-kinematic body counts a static neighbour when the other sleeps or is kinematic,
+// a dynamic one otherwise (integer atomics: the counts do not depend on the order).
+__global__ void k_neighbours(Dev d) {
+    const int np = d.cnt->npairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        if (d.ps.npts[p] == 0) continue;
+        const int a = d.ps.pa[p], b = d.ps.pb[p];
+        const bool ka = d.b.iat(I_KIN, a) != 0, kb = d.b.iat(I_KIN, b) != 0;
+        const bool sa = (d.st.flags[a] & ST_SLEEP) != 0, sb = (d.st.flags[b] & ST_SLEEP) != 0;
+        if (!kb) atomicAdd((sa || ka) ? &d.st.scnt[b] : &d.st.dcnt[b], 1);
+        if (!ka) atomicAdd((sb || kb) ? &d.st.scnt[a] : &d.st.dcnt[a], 1);
+    }
+}
+
+// World::execCollisionCallbacks' arithmetic (world.cpp:131-151) for every pair with contact points
+// whose bodies report: sums over the points in depth order, then the means and the normalised normal
+// (Eigen's normalize: divide by the square root of the squared norm when it is positive).  Velocities
+// are RigidBody::getLinearVelocityAtLocalPoint of the points' local positions (rigidbody.cpp:138-140)
+// after gravity, as the callbacks see them.
+__global__ void k_events(Dev d) {
+    const int np = d.cnt->npairs;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const int n = d.ps.npts[p];
+        const int a = d.ps.pa[p], b = d.ps.pb[p];
+        int flag = 0;
+        if (n > 0 && (d.st.report[a] || d.st.report[b])) {
+            flag = 1;
+            f3 pos{0.0f, 0.0f, 0.0f}, nor{0.0f, 0.0f, 0.0f}, vel{0.0f, 0.0f, 0.0f};
+            float depth = 0.0f;
+            const f3 va = ld3(d.b, F_V, a), wa = ld3(d.b, F_W, a), vb = ld3(d.b, F_V, b), wb = ld3(d.b, F_W, b);
+            for (int k = 0; k < n; k++) {
+                const float* c = d.ps.cp + ((size_t)p * 8 + k) * CW;
+                pos = add(pos, f3{c[C_WPOS], c[C_WPOS + 1], c[C_WPOS + 2]});
+                nor = add(nor, f3{c[C_N], c[C_N + 1], c[C_N + 2]});
+                depth += d.ps.cd[(size_t)p * 8 + k];
+                vel = add(vel, add(va, cross(wa, f3{c[C_LA], c[C_LA + 1], c[C_LA + 2]})));
+                vel = sub(vel, add(vb, cross(wb, f3{c[C_LB], c[C_LB + 1], c[C_LB + 2]})));
+            }
+            const float fn = (float)n;
+            pos = f3{pos.x / fn, pos.y / fn, pos.z / fn};
+            const float z = sqnorm3(nor);
+            if (z > 0.0f) {
+                const float r = sqrtf(z);
+                nor = f3{nor.x / r, nor.y / r, nor.z / r};
+            }
+            depth = depth / fn;
+            vel = f3{vel.x / fn, vel.y / fn, vel.z / fn};
+            const float e[EVW] = {__int_as_float(a), __int_as_float(b), pos.x, pos.y, pos.z, nor.x, nor.y, nor.z,
+                                  depth, vel.x, vel.y, vel.z};
+            st_rec<EVW / 4>(d.events + (size_t)p * EVW, e);
+        }
+        d.ev_flag[p] = flag;
+    }
 }
 
 // ------------------------------------------------------------------ stage 1: broadphase (pe_sap.hpp)
@@ -242,6 +324,7 @@ struct pe_world {
     std::vector<JointDef> joints;            // host copies of the joint definitions
     std::vector<uint64_t> ignore_keys;       // sorted ignore-pair keys
     std::vector<char> kinematic;             // per body (joint colouring)
+    std::vector<char> finite_life, has_joint, removed;   // per body (joints and removals exclude each other)
     void* joint_mem[4] = {};                 // device: definitions, rows, order, colour starts
     void* ignore_mem = nullptr;
 
@@ -266,7 +349,8 @@ struct pe_world {
     int grid(int work, int block) const { return std::max(1, std::min((work + block - 1) / block, 4096)); }
 
     int launch_integrate() {
-        hipLaunchKernelGGL(k_integrate, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        if (d.status_on) hipLaunchKernelGGL(k_integrate_status, dim3(grid(n, 256)), dim3(256), 0, stream, d);
+        else hipLaunchKernelGGL(k_integrate, dim3(grid(n, 256)), dim3(256), 0, stream, d);
         return PE_OK;
     }
     int launch_broadphase() {
@@ -281,6 +365,8 @@ struct pe_world {
         }
         hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
         hipLaunchKernelGGL(k_count_points, dim3(grid(d.pcap, 1024)), dim3(256), 0, stream, d);
+        if (d.status_on) hipLaunchKernelGGL(k_neighbours, dim3(grid(d.pcap, 256)), dim3(256), 0, stream, d);
+        if (d.report_on) hipLaunchKernelGGL(k_events, dim3(grid(d.pcap, 256)), dim3(256), 0, stream, d);
         return PE_OK;
     }
     int launch_solve() {
@@ -387,6 +473,15 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&d.color_start, pcap + 1);
     rc |= w->alloc(&d.claim, n);
     rc |= w->alloc(&d.cnt, 1);
+    rc |= w->alloc(&d.st.flags, n);
+    rc |= w->alloc(&d.st.sleep_time, n);
+    rc |= w->alloc(&d.st.life, n);
+    rc |= w->alloc(&d.st.last, n);
+    rc |= w->alloc(&d.st.scnt, n);
+    rc |= w->alloc(&d.st.dcnt, n);
+    rc |= w->alloc(&d.st.report, n);
+    rc |= w->alloc(&d.events, (size_t)EVW * pcap);
+    rc |= w->alloc(&d.ev_flag, pcap);
     SapView& sp = d.sap;
     sp.n = n;
     sp.amin = d.b.f + (size_t)F_AMIN * w->stride;
@@ -405,7 +500,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&sp.skey, n);
     rc |= w->alloc(&sp.cand, n + 1);
     rc |= w->alloc(&sp.hist, 256 * (size_t)sap_radix_blocks(n));
-    rc |= w->alloc(&sp.red, 6 * (size_t)sap_red_blocks(n));
+    rc |= w->alloc(&sp.red, SAP_RED_VALUES * (size_t)sap_red_blocks(n));
     // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
     sp.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / SAP_CHUNK + 1, (int64_t)1 << 26);
     rc |= w->alloc(&sp.chunk, sp.chunk_cap);
@@ -428,11 +523,25 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
         w->has_shapes |= sc->shape_type[i] > PE_SHAPE_SPHERE;
     }
     w->n_dynamic = ndyn;
+    // RigidBody::_life_time (rigidbody.cpp:97): FP32 PE_REAL_MAX unless the scene sets one
+    std::vector<float> life(n, 1e30f);
+    w->finite_life.assign(n, 0);
+    for (int i = 0; sc->lifetime && i < n; i++) {
+        life[i] = (float)sc->lifetime[i];
+        if (!(sc->lifetime[i] >= 1e30)) { w->finite_life[i] = 1; d.status_on = 1; }
+    }
+    w->has_joint.assign(n, 0);
+    w->removed.assign(n, 0);
     if (hipMemcpyAsync(d.b.f, hf.data(), hf.size() * sizeof(float), hipMemcpyHostToDevice, w->stream) != hipSuccess ||
-        hipMemcpyAsync(d.b.i, hi.data(), hi.size() * sizeof(int32_t), hipMemcpyHostToDevice, w->stream) != hipSuccess) {
+        hipMemcpyAsync(d.b.i, hi.data(), hi.size() * sizeof(int32_t), hipMemcpyHostToDevice, w->stream) != hipSuccess ||
+        hipMemcpyAsync(d.st.life, life.data(), n * sizeof(float), hipMemcpyHostToDevice, w->stream) != hipSuccess) {
         pe_set_error("upload failed");
         return fail(PE_ERR_DEVICE);
     }
+    for (void* p : {(void*)d.st.flags, (void*)d.st.sleep_time, (void*)d.st.last, (void*)d.st.scnt, (void*)d.st.dcnt,
+                    (void*)d.st.report})
+        if (hipMemsetAsync(p, 0, n * sizeof(int32_t), w->stream) != hipSuccess) { pe_set_error("upload failed"); return fail(PE_ERR_DEVICE); }
+    if (hipMemsetAsync(d.ev_flag, 0, pcap * sizeof(int32_t), w->stream) != hipSuccess) { pe_set_error("upload failed"); return fail(PE_ERR_DEVICE); }
     hipLaunchKernelGGL(k_aabb_only, dim3(w->grid(n, 256)), dim3(256), 0, w->stream, d);
     for (auto& e : w->ev) hipEventCreate(&e);
     for (auto& e : w->order_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
@@ -461,6 +570,15 @@ int pe_world_add_joints(pe_world* w, int32_t n, const pe_joint_desc* jd) {
             pe_set_error("pe_world_add_joints: joint " + std::to_string(k) + " has an unknown type or invalid bodies");
             return PE_ERR_INVALID;
         }
+        for (int x : {s.body_a, s.body_b})
+            if (w->finite_life[x] || w->removed[x]) {
+                pe_set_error("pe_world_add_joints: body " + std::to_string(x) + " has a finite lifetime or was removed");
+                return PE_ERR_INVALID;
+            }
+    }
+    for (int k = 0; k < n; k++) {
+        const pe_joint_desc& s = jd[k];
+        w->has_joint[s.body_a] = w->has_joint[s.body_b] = 1;
         JointDef j{};
         j.type = s.type; j.a = s.body_a; j.b = s.body_b;
         j.flags = (s.use_limits ? JF_LIMITS : 0) | (s.use_motor ? JF_MOTOR : 0);
@@ -546,6 +664,85 @@ int pe_world_add_ignore_pairs(pe_world* w, int32_t n, const int32_t* pairs) {
                            (const uint64_t*)w->ignore_mem, (int)w->ignore_keys.size());
     PE_HIP(hipGetLastError());
     PE_HIP(hipStreamSynchronize(w->stream));
+    return PE_OK;
+}
+
+static int read_counts(pe_world* w, DevCounts& c);
+
+int pe_world_set_sleep(pe_world* w, double lin_vel2, double ang_vel2, double time) {
+    if (int rc = check_world(w)) return rc;
+    w->d.st.lin2 = (float)lin_vel2;
+    w->d.st.ang2 = (float)ang_vel2;
+    w->d.st.tsleep = (float)time;
+    if (lin_vel2 != 0.0 || ang_vel2 != 0.0) w->d.status_on = 1;
+    return PE_OK;
+}
+
+int pe_world_remove_body(pe_world* w, int32_t body) {
+    if (int rc = check_world(w)) return rc;
+    if (body < 0 || body >= w->n) { pe_set_error("pe_world_remove_body: body index out of range"); return PE_ERR_INVALID; }
+    if (w->has_joint[body]) { pe_set_error("pe_world_remove_body: the body has joints"); return PE_ERR_INVALID; }
+    w->removed[body] = 1;
+    w->d.status_on = 1;
+    pe_clear_stale_error();
+    hipLaunchKernelGGL(k_remove_body, dim3(1), dim3(1), 0, w->stream, w->d, (int)body);
+    PE_HIP(hipGetLastError());
+    return PE_OK;
+}
+
+int pe_world_get_body_status(pe_world* w, int32_t* flags, int32_t* static_count, int32_t* dynamic_count,
+                             float* sleep_time) {
+    if (int rc = check_world(w)) return rc;
+    PE_HIP(hipStreamSynchronize(w->stream));
+    const size_t bytes = (size_t)w->n * 4;
+    if (flags) PE_HIP(hipMemcpy(flags, w->d.st.flags, bytes, hipMemcpyDeviceToHost));
+    if (static_count) PE_HIP(hipMemcpy(static_count, w->d.st.scnt, bytes, hipMemcpyDeviceToHost));
+    if (dynamic_count) PE_HIP(hipMemcpy(dynamic_count, w->d.st.dcnt, bytes, hipMemcpyDeviceToHost));
+    if (sleep_time) PE_HIP(hipMemcpy(sleep_time, w->d.st.sleep_time, bytes, hipMemcpyDeviceToHost));
+    return PE_OK;
+}
+
+int pe_world_set_collision_reports(pe_world* w, int32_t n, const int32_t* bodies, int32_t on) {
+    if (int rc = check_world(w)) return rc;
+    if (n < 0 || (n > 0 && !bodies)) { pe_set_error("pe_world_set_collision_reports: invalid argument"); return PE_ERR_INVALID; }
+    for (int k = 0; k < n; k++)
+        if (bodies[k] < 0 || bodies[k] >= w->n) { pe_set_error("pe_world_set_collision_reports: body index out of range"); return PE_ERR_INVALID; }
+    PE_HIP(hipStreamSynchronize(w->stream));
+    std::vector<int32_t> rep(w->n);
+    PE_HIP(hipMemcpy(rep.data(), w->d.st.report, (size_t)w->n * 4, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) rep[bodies[k]] = on ? 1 : 0;
+    PE_HIP(hipMemcpy(w->d.st.report, rep.data(), (size_t)w->n * 4, hipMemcpyHostToDevice));
+    w->d.report_on = 0;
+    for (int32_t r : rep) w->d.report_on |= r;
+    return PE_OK;
+}
+
+int pe_world_get_collision_events(pe_world* w, int32_t cap, pe_collision_event* out, int32_t* count) {
+    if (int rc = check_world(w)) return rc;
+    if (cap < 0 || (cap > 0 && !out) || !count) { pe_set_error("pe_world_get_collision_events: invalid argument"); return PE_ERR_INVALID; }
+    DevCounts c;
+    if (int rc = read_counts(w, c)) return rc;
+    const int np = std::min(c.npairs, w->d.pcap);
+    int total = 0;
+    if (w->d.report_on && np > 0) {
+        std::vector<int32_t> flag(np);
+        std::vector<float> ev((size_t)np * EVW);
+        PE_HIP(hipMemcpy(flag.data(), w->d.ev_flag, (size_t)np * 4, hipMemcpyDeviceToHost));
+        PE_HIP(hipMemcpy(ev.data(), w->d.events, ev.size() * 4, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; p++) {
+            if (!flag[p]) continue;
+            if (total < cap) {
+                const float* e = &ev[(size_t)p * EVW];
+                pe_collision_event& o = out[total];
+                std::memcpy(&o.body_a, &e[0], 4);
+                std::memcpy(&o.body_b, &e[1], 4);
+                for (int k = 0; k < 3; k++) { o.position[k] = e[2 + k]; o.normal[k] = e[5 + k]; o.velocity[k] = e[9 + k]; }
+                o.depth = e[8];
+            }
+            total++;
+        }
+    }
+    *count = total;
     return PE_OK;
 }
 

@@ -17,11 +17,25 @@ _F32P = ctypes.POINTER(ctypes.c_float)
 _I32P = ctypes.POINTER(ctypes.c_int32)
 
 
+def load_schedule(name: str) -> dict:
+    """A launch schedule from the tuning table shipped beside this module (schedules.json): the
+    size-class upper bounds, block size and solver group size per class that Batch.autotune chose
+    for a named workload, applied with Batch.apply_schedule."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "schedules.json")) as f:
+        return json.load(f)[name]
+
+
 def _plain_scene(scene):
-    """Batched worlds carry no joints and no ignore-pair lists (pe_world_add_joints and
-    pe_world_add_ignore_pairs are single-world entry points)."""
+    """Batched worlds carry no joints, ignore-pair lists, sleep thresholds or lifetimes (their entry
+    points are single-world ones)."""
     if getattr(scene, "joints", None) or getattr(scene, "ignore_pairs", None):
         raise ValueError(f"scene {scene.name!r}: joints and ignore pairs need a World, not a batch")
+    sleep = getattr(scene, "sleep", None)
+    if (sleep and (sleep[0] != 0.0 or sleep[1] != 0.0)) or (
+            getattr(scene, "lifetime", None) is not None and bool(np.any(np.asarray(scene.lifetime) < 1e30))):
+        raise ValueError(f"scene {scene.name!r}: sleeping and lifetimes need a World, not a batch")
 
 
 class Batch:
@@ -102,6 +116,17 @@ class Batch:
         r = (ctypes.c_int32 * 32)()
         check(lib().pe_batch_get_solver_groups(self.h, ctypes.byref(nc), r, 32))
         return [r[i] for i in range(nc.value)]
+
+    def apply_schedule(self, sched: dict):
+        """Sets the size classes, block sizes and solver group sizes of a schedule (load_schedule, or
+        {"class_upper_bodies", "block", "solver_group_bodies"} from autotune's result)."""
+        self.set_config(list(sched["class_upper_bodies"]), list(sched["block"]))
+        self.set_solver_groups(list(sched["solver_group_bodies"]))
+
+    def schedule(self) -> dict:
+        cfg = self.config()
+        return {"class_upper_bodies": [c[0] for c in cfg], "block": [c[1] for c in cfg],
+                "solver_group_bodies": self.solver_groups()}
 
     def world_state(self, w: int):
         n = self.scenes[w].n

@@ -90,6 +90,11 @@ class DeviceArray:
         assert a.nbytes == self.nbytes
         _check(hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(self.nbytes), H2D), "hipMemcpy")
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device hipMemcpy of other's bytes into this array (same size)."""
+        assert other.nbytes == self.nbytes
+        _check(hip().hipMemcpy(self.ptr, other.ptr, ctypes.c_size_t(self.nbytes), 3), "hipMemcpy")
+
     def numpy(self):
         out = np.empty(self.shape, self.dtype)
         _check(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes), D2H), "hipMemcpy")

@@ -7,19 +7,20 @@ and a scene built here opens in the reference (src/intf/simulator.cpp: saveScene
 
 What the file holds and how it maps onto a Scene:
 
-* ``Configuration.gravity`` -> ``Scene.gravity``.  The sleep thresholds are read into
-  ``Scene.sleep`` (linear velocity^2, angular velocity^2, time) and written back; the device step
-  does no sleep bookkeeping (bodies never sleep), which is what the reference does with zero
-  thresholds.
+* ``Configuration.gravity`` -> ``Scene.gravity``.  The sleep thresholds -> ``Scene.sleep``
+  (linear velocity^2, angular velocity^2, time), which a World applies (World::setSleep*Threshold;
+  batched and served worlds take none).
 * ``RigidBodies[]``: ``type`` box (``scale`` = full size), sphere (``scale[0]`` = radius),
   cylinder (``scale`` = radius, height), convex / concave (``mesh`` = a Wavefront .obj path,
   ``scale``); ``position``; ``rotation`` as 9 row-major basis entries or 4 quaternion entries
   (w, x, y, z); ``mass``, ``friction``, ``restitution`` (defaults 1, 0.5, 0.5 as a RigidBody's,
   rigidbody.cpp:91-100); ``kinematic``; for dynamic bodies ``linear_velocity``,
-  ``angular_velocity``, ``linear_damping``, ``angular_damping``.
+  ``angular_velocity``, ``linear_damping``, ``angular_damping``; ``ignore_collision`` ->
+  ``Scene.ignore_collision``; ``lifetime`` -> ``Scene.lifetime`` (negative, or PE_REAL_MAX as saveScene
+  writes it, means unlimited, as loadScene takes it).
 * What the step does not model is refused with ValueError rather than dropped: compound shapes,
-  fracturable bodies, ``ignore_collision``, a finite ``lifetime``, damage sources.  ``name`` and
-  ``tag`` are kept on the Scene (``Scene.names``, ``Scene.tags``) for saving.
+  fracturable bodies, damage sources.  ``name`` and ``tag`` are kept on the Scene (``Scene.names``,
+  ``Scene.tags``) for saving.
 
 Mesh files are read as the reference's Mesh::loadFromObj reads them
 (SimpleViewer/include/common/mesh.cpp:36-67): ``v`` lines scaled by ``scale[0]`` on every axis
@@ -105,18 +106,16 @@ def load_scene(path: str, dt: float = 1.0 / 60.0, iterations: int = 10) -> Scene
         raise ValueError("damage sources (fracture) are not part of the device step")
     cfg = data.get("Configuration", {})
     b = SceneBuilder()
-    lin, ang, names, tags = [], [], [], []
+    lin, ang, names, tags, ign, lives = [], [], [], [], [], []
     mesh_ids = {}
     for k, rb in enumerate(data.get("RigidBodies", [])):
         if rb.get("fracturable", False):
             raise ValueError(f"body {k}: fracturable bodies are not part of the device step")
-        if rb.get("ignore_collision", False):
-            raise ValueError(f"body {k}: ignore_collision is not part of the device step")
+        ign.append(1 if rb.get("ignore_collision", False) else 0)
         life = float(rb.get("lifetime", 1e100))
         # saveScene writes PE_REAL_MAX for an unlimited lifetime: 1e30 in the FP32 build, 1e100 in
         # the FP64 one (phys_general.h:36-42); loadScene takes a negative value as unlimited too
-        if 0.0 <= life < 1e30:
-            raise ValueError(f"body {k}: a finite lifetime is not part of the device step")
+        lives.append(life if 0.0 <= life < 1e30 else 1e100)
         typ = rb.get("type")
         if typ not in _TYPES:
             raise ValueError(f"body {k}: shape type {typ!r} is not part of the device step")
@@ -165,9 +164,14 @@ def load_scene(path: str, dt: float = 1.0 / 60.0, iterations: int = 10) -> Scene
     sc.angular_damping = np.asarray(ang, float)
     sc = sc.contiguous()
     sc.names, sc.tags = names, tags
-    sc.sleep = (float(cfg.get("sleep_linear_velocity2_threshold", 0.0)),
-                float(cfg.get("sleep_angular_velocity2_threshold", 0.0)),
-                float(cfg.get("sleep_time_threshold", 0.0)))
+    sleep = (float(cfg.get("sleep_linear_velocity2_threshold", 0.0)),
+             float(cfg.get("sleep_angular_velocity2_threshold", 0.0)),
+             float(cfg.get("sleep_time_threshold", 0.0)))
+    sc.sleep = sleep if (sleep[0] != 0.0 or sleep[1] != 0.0) else None
+    if any(ign):
+        sc.ignore_collision = np.asarray(ign, np.int32)
+    if any(x < 1e30 for x in lives):
+        sc.lifetime = np.asarray(lives, float)
     return sc
 
 
@@ -183,20 +187,23 @@ def save_scene(sc: Scene, path: str, mesh_paths=None, extra_config=None, extra=N
         if k not in mesh_paths:
             mesh_paths[k] = f"{base}_mesh{k}.obj"
             save_obj(m, mesh_paths[k])
-    sleep = getattr(sc, "sleep", (0.0, 0.0, 0.0))
+    sleep = getattr(sc, "sleep", None) or (0.0, 0.0, 0.0)
+    ign = getattr(sc, "ignore_collision", None)
+    life = getattr(sc, "lifetime", None)
     names = getattr(sc, "names", None) or [""] * sc.n
     tags = getattr(sc, "tags", None) or [""] * sc.n
     rbs = []
     for i in range(sc.n):
         t = int(sc.shape_type[i])
         kin = bool(sc.kinematic[i])
-        rb = {"name": names[i], "tag": tags[i], "ignore_collision": False, "kinematic": kin}
+        rb = {"name": names[i], "tag": tags[i], "ignore_collision": bool(ign is not None and ign[i]),
+              "kinematic": kin}
         if not kin:
             rb["linear_damping"] = float(sc.linear_damping[i])
             rb["angular_damping"] = float(sc.angular_damping[i])
             rb["linear_velocity"] = [float(x) for x in sc.linear_velocity[i]]
             rb["angular_velocity"] = [float(x) for x in sc.angular_velocity[i]]
-        rb["lifetime"] = 1e100
+        rb["lifetime"] = float(life[i]) if life is not None and life[i] < 1e30 else 1e100
         rb["fracturable"] = False
         rb["position"] = [float(x) for x in sc.position[i]]
         rb["rotation"] = [float(x) for x in np.asarray(sc.rotation[i]).ravel()]

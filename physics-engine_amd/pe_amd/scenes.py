@@ -73,7 +73,14 @@ class SceneDesc(ctypes.Structure):
         ("meshes", ctypes.POINTER(MeshDesc)),
         ("mesh_index", ctypes.POINTER(ctypes.c_int32)),
         ("ignore_collision", ctypes.POINTER(ctypes.c_int32)),
+        ("lifetime", ctypes.POINTER(ctypes.c_double)),
     ]
+
+
+class CollisionEvent(ctypes.Structure):
+    """ctypes mirror of pe_collision_event."""
+    _fields_ = [("body_a", ctypes.c_int32), ("body_b", ctypes.c_int32), ("position", ctypes.c_float * 3),
+                ("normal", ctypes.c_float * 3), ("depth", ctypes.c_float), ("velocity", ctypes.c_float * 3)]
 
 
 class JointDesc(ctypes.Structure):
@@ -139,6 +146,8 @@ class Scene:
     joints: list = field(default_factory=list)          # [JointDesc] in World::addConstraint order
     ignore_pairs: list = field(default_factory=list)    # [(a, b)] bodies that never collide (addIgnoreCollisionId)
     ignore_collision: np.ndarray = None                 # int32 [n] 0/1 (setIgnoreCollision), or None
+    lifetime: np.ndarray = None                         # f64 [n] seconds (setLifeTime; >= 1e30 never), or None
+    sleep: tuple = None                                 # (lin_vel2, ang_vel2, time) World::setSleep*Threshold, or None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -186,6 +195,9 @@ class Scene:
         if self.ignore_collision is not None:
             self.ignore_collision = np.ascontiguousarray(self.ignore_collision, dtype=np.int32)
             d.ignore_collision = _iptr(self.ignore_collision)
+        if self.lifetime is not None:
+            self.lifetime = np.ascontiguousarray(self.lifetime, dtype=np.float64)
+            d.lifetime = _dptr(self.lifetime)
         return d
 
     def params(self) -> WorldParams:
@@ -201,7 +213,8 @@ class Scene:
                      None if self.angular_damping is None else self.angular_damping[idx],
                      self.dt, self.gravity, self.iterations, self.name,
                      list(self.meshes), self.mesh_index[idx],
-                     ignore_collision=None if self.ignore_collision is None else self.ignore_collision[idx]).contiguous()
+                     ignore_collision=None if self.ignore_collision is None else self.ignore_collision[idx],
+                     lifetime=None if self.lifetime is None else self.lifetime[idx], sleep=self.sleep).contiguous()
 
     def footprint_bodies(self) -> int:
         return self.n

@@ -47,6 +47,9 @@ class World:
             self.add_ignore_pairs(scene.ignore_pairs)
         if scene.joints:
             self.add_joints(scene.joints)
+        if getattr(scene, "sleep", None):
+            self.set_sleep(*scene.sleep)
+        self._callbacks = {}   # body -> [callable]
 
     def add_joints(self, joints):
         """World::addConstraint for each JointDesc, in order (pe_world_add_joints)."""
@@ -72,7 +75,69 @@ class World:
 
     # ---- World::step and its stages
     def step(self, n: int = 1):
-        check(lib().pe_world_step(self.h, ctypes.c_int32(n)))
+        """World::step n times.  With collision callbacks registered the steps run one at a time and
+        each step's callbacks are called after it (World::execCollisionCallbacks, world.cpp:129-192)."""
+        if not self._callbacks:
+            check(lib().pe_world_step(self.h, ctypes.c_int32(n)))
+            return
+        for _ in range(n):
+            check(lib().pe_world_step(self.h, ctypes.c_int32(1)))
+            self.dispatch_callbacks()
+
+    # ---- sleeping, lifetimes, removal, collision callbacks
+    def set_sleep(self, lin_vel2: float, ang_vel2: float, time: float):
+        """World::setSleepLinVel2Threshold / setSleepAngVel2Threshold / setSleepTimeThreshold."""
+        check(lib().pe_world_set_sleep(self.h, ctypes.c_double(lin_vel2), ctypes.c_double(ang_vel2),
+                                       ctypes.c_double(time)))
+
+    def remove_body(self, body: int):
+        """World::removeRigidBody."""
+        check(lib().pe_world_remove_body(self.h, ctypes.c_int32(body)))
+
+    def body_status(self):
+        """(flags [n]: 1 asleep, 2 removed; static and dynamic neighbour counts [n] of the last
+        narrowphase; sleep times [n])."""
+        fl, sc, dc = (np.zeros(self.n, np.int32) for _ in range(3))
+        t = np.zeros(self.n, np.float32)
+        check(lib().pe_world_get_body_status(self.h, _i(fl), _i(sc), _i(dc), _f(t)))
+        return fl, sc, dc, t
+
+    def set_collision_reports(self, bodies, on: bool = True):
+        b = np.ascontiguousarray(np.asarray(bodies, np.int32).ravel())
+        check(lib().pe_world_set_collision_reports(self.h, ctypes.c_int32(b.size), _i(b), ctypes.c_int32(1 if on else 0)))
+
+    def collision_events(self):
+        """The last narrowphase's reports: (bodies [E, 2] (A, B), data [E, 10] float32: position,
+        normal, depth, velocity)."""
+        from .scenes import CollisionEvent
+        cnt = ctypes.c_int32()
+        check(lib().pe_world_get_collision_events(self.h, ctypes.c_int32(0), None, ctypes.byref(cnt)))
+        arr = (CollisionEvent * max(cnt.value, 1))()
+        check(lib().pe_world_get_collision_events(self.h, ctypes.c_int32(cnt.value), arr, ctypes.byref(cnt)))
+        raw = np.frombuffer(arr, dtype=np.float32, count=12 * cnt.value).reshape(-1, 12)
+        return raw[:, :2].copy().view(np.int32), raw[:, 2:].copy()
+
+    def add_collision_callback(self, body: int, fn):
+        """RigidBody::addCollisionCallback: fn(self_body, other_body, position, normal, velocity) is
+        called after each step for each contact result of the body with points."""
+        self._callbacks.setdefault(int(body), []).append(fn)
+        self.set_collision_reports([body], True)
+
+    def clear_collision_callbacks(self, body: int):
+        self._callbacks.pop(int(body), None)
+        self.set_collision_reports([body], False)
+
+    def dispatch_callbacks(self):
+        """Calls the registered callbacks with the last step's reports, as World::execCollisionCallbacks
+        calls them: A's with (A, B, position + normal * depth, -normal, -velocity), B's with
+        (B, A, position, normal, velocity) (FP32)."""
+        ab, ev = self.collision_events()
+        for (a, b), e in zip(ab, ev):
+            pos, nor, dep, vel = e[0:3], e[3:6], e[6], e[7:10]
+            for fn in self._callbacks.get(int(a), ()):
+                fn(int(a), int(b), pos + nor * dep, -nor, -vel)
+            for fn in self._callbacks.get(int(b), ()):
+                fn(int(b), int(a), pos, nor, vel)
 
     def step_async(self, n: int, stream):
         """n steps enqueued on `stream` (a hipStream_t handle as an int, e.g.

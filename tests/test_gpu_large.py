@@ -16,11 +16,13 @@ pytestmark = pytest.mark.gpu
 
 REL = 1e-5
 ABS = 1e-6
-# the configuration the 4096-world benchmark's tuner picked (class upper bounds, block sizes,
-# solver group bodies)
-BENCH_UPPER = [24, 64, 160, 384, 1024, 4096]
-BENCH_BLOCK = [256, 64, 64, 256, 512, 64]
-BENCH_GROUP = [128, 128, 128, 0, 128, 0]
+# the schedule the 4096-world benchmark runs (class upper bounds, block sizes, solver group bodies)
+from pe_amd.batch import load_schedule  # noqa: E402
+
+_SCHED = load_schedule("batched_4096_worlds")
+BENCH_UPPER = _SCHED["class_upper_bodies"]
+BENCH_BLOCK = _SCHED["block"]
+BENCH_GROUP = _SCHED["solver_group_bodies"]
 
 
 def _close(a, b, rel=REL, abs_=ABS):

@@ -215,6 +215,75 @@ def test_joint_chains_and_ignore_flags_bit_exact():
             o.set_state(*rs)
 
 
+def callback_invocations(ab, ev, reporting):
+    """The collision callback calls a step's events make (World::execCollisionCallbacks,
+    world.cpp:153-158): A's callbacks get (A, B, pos + nor * depth, -nor, -vel), B's (B, A, pos, nor,
+    vel), in FP32 as the FP32 reference computes pos + nor * depth."""
+    out = []
+    for (a, b), e in zip(ab, ev):
+        pos, nor, vel = e[0:3].astype(np.float32), e[3:6].astype(np.float32), e[7:10].astype(np.float32)
+        dep = np.float32(e[6])
+        if a in reporting:
+            out.append((int(a), int(b)) + tuple((pos + nor * dep).tolist()) + tuple((-nor).tolist()) + tuple((-vel).tolist()))
+        if b in reporting:
+            out.append((int(b), int(a)) + tuple(pos.tolist()) + tuple(nor.tolist()) + tuple(vel.tolist()))
+    return sorted(out)
+
+
+@pytest.mark.ref
+def test_sleep_lifetime_and_callbacks_bit_exact():
+    """World::updateObjectStatus with sleep thresholds (0.01 m^2/s^2, 0.01 rad^2/s^2, 1 s, as the
+    reference's compound demo sets them), a body whose lifetime ends, World::removeRigidBody of another,
+    the narrowphase's static / dynamic neighbour counts, and collision callbacks on three bodies: the
+    oracle's status bits, counts, sleep times, pairs, callback arguments and states equal the compiled
+    reference's after every stage for 420 steps.  One reference thread: the reference counts
+    neighbours with unsynchronised increments from its pool threads (simple_narrow_phase.cpp:43-62),
+    which lose counts when two threads meet on a body; its threaded status loop erases the expiring
+    body while iterating (world.cpp:59-62), which matches the sequential loop when that body is the
+    last of the array, as here."""
+    sc = small_rain(120, seed=5)
+    sc.sleep = (0.01, 0.01, 1.0)
+    lt = np.full(sc.n, 1e30)
+    lt[-1] = 2.0
+    sc.lifetime = lt
+    r = RefWorld(sc, 32, threads=1)
+    o = OracleWorld(sc, 32, color_order=False)
+    reporting = {3, 7, 20}
+    for b in reporting:
+        r.set_report(b)
+        o.set_report(b)
+    slept, woke, calls, prev = 0, 0, 0, np.zeros(sc.n, np.int32)
+    for step in range(420):
+        if step == 200:
+            r.remove_body(40)
+            o.remove_body(40)
+        r.integrate(); o.integrate()
+        for a, b in zip(r.status(), o.status()):
+            assert np.array_equal(a, b), f"status after integrate, step {step}"
+        r.broadphase(); o.broadphase()
+        assert pair_set(r.pairs()) == pair_set(o.pairs()), f"pairs step {step}"
+        o.set_pairs(r.pairs())
+        r.narrowphase(); o.narrowphase()
+        for a, b in zip(r.status(), o.status()):
+            assert np.array_equal(a, b), f"neighbour counts step {step}"
+        ids, data = r.callbacks()
+        got = sorted(tuple(i.tolist()) + tuple(d.tolist()) for i, d in zip(ids, data))
+        assert got == callback_invocations(*o.events(), reporting), f"callbacks step {step}"
+        calls += len(got)
+        r.solve(); o.solve()
+        for a, b in zip(r.state(), o.state()):
+            assert np.array_equal(a, b), f"state step {step}"
+        o.set_state(*r.state())
+        st = r.status()[0] & 1
+        slept += int(((st == 1) & (prev == 0)).sum())
+        woke += int(((st == 0) & (prev == 1)).sum())
+        prev = st
+    status = r.status()[0]
+    assert status[-1] & 2 and status[40] & 2 and int((status & 2).sum()) == 4
+    assert slept > 0 and calls > 0
+    print(f"[sleep] fell asleep {slept} times, woke {woke} times, {calls} callback calls", flush=True)
+
+
 def test_batches_refuse_joints():
     from pe_amd import scenes
     from pe_amd.batch import Batch

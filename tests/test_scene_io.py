@@ -75,8 +75,31 @@ def test_mesh_body_scaled_by_size_x(tmp_path):
     assert len(sc.meshes[0].faces) == 42
 
 
-@pytest.mark.parametrize("extra", [{"fracturable": True}, {"ignore_collision": True}, {"lifetime": 5.0},
-                                   {"type": "compound"}])
+def test_ignore_collision_lifetime_and_sleep_round_trip(tmp_path):
+    """ignore_collision, a finite lifetime (negative and PE_REAL_MAX of either precision mean
+    unlimited, simulator.cpp:126, 358-361) and the sleep thresholds load into the Scene and save back."""
+    from pe_amd.scene_io import load_scene, save_scene
+    rbs = [{"type": "box", "scale": [1, 1, 1], "ignore_collision": True},
+           {"type": "box", "scale": [1, 1, 1], "lifetime": 5.0},
+           {"type": "box", "scale": [1, 1, 1], "lifetime": -1.0},
+           {"type": "box", "scale": [1, 1, 1], "lifetime": 1e30},
+           {"type": "box", "scale": [1, 1, 1], "lifetime": 1e100}]
+    cfg = {"sleep_linear_velocity2_threshold": 0.01, "sleep_angular_velocity2_threshold": 0.02,
+           "sleep_time_threshold": 1.0}
+    p = tmp_path / "s.json"
+    p.write_text(json.dumps({"Configuration": cfg, "RigidBodies": rbs}))
+    sc = load_scene(str(p))
+    assert sc.ignore_collision.tolist() == [1, 0, 0, 0, 0]
+    assert sc.lifetime.tolist() == [1e100, 5.0, 1e100, 1e100, 1e100]
+    assert sc.sleep == (0.01, 0.02, 1.0)
+    out = str(tmp_path / "back.json")
+    save_scene(sc, out)
+    back = load_scene(out)
+    assert back.ignore_collision.tolist() == sc.ignore_collision.tolist()
+    assert back.lifetime.tolist() == sc.lifetime.tolist() and back.sleep == sc.sleep
+
+
+@pytest.mark.parametrize("extra", [{"fracturable": True}, {"type": "compound"}])
 def test_unmodelled_features_are_refused(tmp_path, extra):
     from pe_amd.scene_io import load_scene
     rb = {"type": "box", "scale": [1, 1, 1]}
