@@ -117,8 +117,6 @@ def run_batched(args, worlds, sched):
     if c1["overflow_worlds"]:
         raise SystemExit(f"bench.py: {c1['overflow_worlds']} worlds exceeded a capacity; the step is not valid")
     out = dict(wall=wall, dev_ms=dev_ms, counts=c1, n=b.n_bodies, n_dyn=b.n_dynamic, batch=b)
-    if args.profile == "batched":
-        return out
     b.restore()
     st_ms, st_b = [0.0] * 3, [0.0] * 3
     pairs = points = 0
@@ -132,6 +130,8 @@ def run_batched(args, worlds, sched):
         pairs += c["pairs"]
         points += c["points"]
     out.update(stage_ms=[x / K for x in st_ms], stage_bytes=[x / K for x in st_b], pairs=pairs / K, points=points / K)
+    if args.profile == "batched":
+        return out
     if args.settle > K:
         b.step(args.settle - K)
     b.sync()
@@ -204,7 +204,9 @@ def run_plain(scene, steps, warmup=0):
     w.step(steps)
     w.sync()
     dt = time.perf_counter() - t0
-    return dt, w.counts()
+    counts = w.counts()
+    w.close()
+    return dt, counts
 
 
 def run_plain_sample(worlds, steps, settle=0):
@@ -407,7 +409,7 @@ def traffic_summary(fetch_path, write_path, steps, out_path):
     WRITE_SIZE, KiB per dispatch) over `python bench.py --profile batched --calibrate-copy`: each
     counter is scaled by its reading for the three 1 GiB calibration copies (the first three copy
     dispatches), and summed over the dispatches of the last `steps` steps (one k_list_reset opens a
-    step; the profiled run ends with the timed window)."""
+    step; the profiled run ends with the staged pass over steps 0..K, the timed window's work)."""
     stages = ("k_world_broad", "k_narrow_list", "k_group_solve")
     res = {}
     for name, path in (("fetch", fetch_path), ("write", write_path)):
@@ -431,6 +433,52 @@ def traffic_summary(fetch_path, write_path, steps, out_path):
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
+
+
+def trace_summary(path, steps):
+    """Per-step spans of the three launch groups (first start to last end of their kernels) and
+    per-kernel mean durations in a rocprofv3 --kernel-trace of `python bench.py --profile batched`
+    (one k_list_reset opens a step): its timed graph replays (steps 0..K), then the same K steps
+    again, eager, with events between the launch groups (what the bench's stage figures time)."""
+    import csv
+    import glob
+    files = [path] if os.path.isfile(path) else glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True)
+    rows = []
+    for fn in files:
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"],
+                             int(r["Workgroup_Size_X"]), int(r["Grid_Size_X"]) // max(1, int(r["Workgroup_Size_X"]))))
+    rows.sort()
+    resets = [i for i, r in enumerate(rows) if "k_list_reset" in r[2]]
+    out = {"steps": steps}
+    for label, first in (("graph_replays", len(resets) - 2 * steps), ("eager_staged", len(resets) - steps)):
+        out[label] = _trace_window(rows, resets, first, steps)
+    print(json.dumps(out, indent=1))
+
+
+def _trace_window(rows, resets, first, steps):
+    stages = ("k_world_broad", "k_narrow_list", "k_group_solve")
+    spans = [[] for _ in stages]
+    whole = []
+    per = {}
+    for k in range(first, first + steps):
+        lo = resets[k]
+        hi = resets[k + 1] if k + 1 < len(resets) else len(rows)
+        step = rows[lo:hi]
+        whole.append((max(r[1] for r in step) - min(r[0] for r in step)) / 1e6)
+        for si, sname in enumerate(stages):
+            ks = [r for r in step if sname in r[2]]
+            if ks:
+                spans[si].append((max(r[1] for r in ks) - min(r[0] for r in ks)) / 1e6)
+        for r in step:
+            key = (r[2].replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0], r[3], r[4])
+            per.setdefault(key, []).append((r[1] - r[0]) / 1e3)
+    return {"median_span_ms": {s_: float(np.median(v)) for s_, v in zip(stages, spans)},
+            "mean_span_ms": {s_: float(np.mean(v)) for s_, v in zip(stages, spans)},
+            "median_step_ms": float(np.median(whole)), "mean_step_ms": float(np.mean(whole)),
+            "kernels_mean_us": sorted([[k[0], k[1], k[2], float(np.mean(v)), len(v)] for k, v in per.items()],
+                                      key=lambda x: -x[3])}
 
 
 def main():
@@ -464,7 +512,12 @@ def main():
                          "for scaling rocprof FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH", "WRITE"), default=None,
                     help="summarise two rocprofv3 --pmc outputs of --profile batched into profiles/batched_traffic.json")
+    ap.add_argument("--trace-summary", default=None, metavar="TRACE",
+                    help="per-step launch-group spans and kernel means of a rocprofv3 --kernel-trace of --profile batched")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.steps)
+        return
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary, args.steps, os.path.join(ROOT, "profiles", "batched_traffic.json"))
         return
@@ -497,7 +550,8 @@ def main():
     if args.profile == "batched":
         bt["batch"].close()
         print(json.dumps({"profile": "batched", "steps": K, "ms_per_step": ms_step,
-                          "device_ms_per_step": bt["dev_ms"] / K, "body_steps_per_s": value}), flush=True)
+                          "device_ms_per_step": bt["dev_ms"] / K, "body_steps_per_s": value,
+                          "stage_ms": bt["stage_ms"], "stage_algorithmic_bytes": bt["stage_bytes"]}), flush=True)
         return
     dev_step_ms = bt["dev_ms"] / K
     B = sum(bt["stage_bytes"])

@@ -207,6 +207,25 @@ typedef struct pe_collision_event {
  * Synchronises. */
 int pe_world_get_collision_events(pe_world* w, int32_t cap, pe_collision_event* out, int32_t* count);
 
+/* ---- raycast ----
+ * DefaultRaycastSolver::performRaycast (default_raycast_solver.cpp:11-86) for each of nrays rays
+ * against the world's bodies as they are now (boxes, spheres, cylinders, convex and concave meshes:
+ * raycast_box.cpp, raycast_sphere.cpp, raycast_cylinder.cpp, raycast_mesh.cpp).  rays is host memory,
+ * [nrays][7]: start (3), direction (3), length; ignore (host, nignore body indices) is the call's
+ * ignore set for every ray (the reference's ignores); ignore-collision and removed bodies are skipped.
+ * As in the reference a query ends with hit = 0 at the first mesh body (in body order) whose AABB
+ * the unbounded ray misses, the result then holding the best hit of the bodies before it.  Results
+ * to host memory out[nrays].  Synchronises. */
+typedef struct pe_ray_hit {
+    int32_t hit;          /* performRaycast's return value */
+    int32_t body;         /* RaycastResult::m_collisionObject's index, -1 if none */
+    float distance;       /* m_distance (PE_REAL_MAX = 1e30 if none; may be negative: a sphere behind the start) */
+    float point[3];       /* m_hitPoint */
+    float normal[3];      /* m_hitNormal */
+} pe_ray_hit;
+int pe_world_raycast(pe_world* w, int32_t nrays, const float* rays, int32_t nignore, const int32_t* ignore,
+                     pe_ray_hit* out);
+
 /* Synchronises the world's stream, then reports capacity errors raised by the steps since the
  * last report (PE_ERR_CAPACITY, the causes in pe_last_error): more broadphase pairs than max_pairs
  * (the extra pairs are dropped), more sweep candidates than the sweep's chunk capacity, or a

@@ -180,6 +180,14 @@ template <class R> int get_poly(Handle<R>* h, int body, int32_t* counts, double*
     }                                                                                                    \
     extern "C" void P##_set_report(void* h, int body, int on) { ((Handle<R>*)h)->w.bodies[body].report = on != 0; } \
     extern "C" void P##_remove_body(void* h, int body) { ((Handle<R>*)h)->w.bodies[body].removed = true; } \
+    extern "C" int P##_raycast(void* h, const double* ray, int nignore, const int32_t* ignore, double* out) { \
+        std::set<int> ig(ignore, ignore + nignore);                                                      \
+        const RayHit<R> r = raycast(((Handle<R>*)h)->w, V3<R>(R(ray[0]), R(ray[1]), R(ray[2])),          \
+                                    V3<R>(R(ray[3]), R(ray[4]), R(ray[5])), R(ray[6]), ig);              \
+        out[0] = r.body; out[1] = r.distance;                                                            \
+        for (int k = 0; k < 3; k++) { out[2 + k] = r.point[k]; out[5 + k] = r.normal[k]; }                \
+        return r.hit ? 1 : 0;                                                                            \
+    }                                                                                                    \
     extern "C" int P##_num_events(void* h) { return (int)((Handle<R>*)h)->w.events.size(); }            \
     extern "C" void P##_get_events(void* h, int32_t* ab, double* data) {                                 \
         auto& ev = ((Handle<R>*)h)->w.events;                                                            \

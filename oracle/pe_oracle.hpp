@@ -1386,6 +1386,240 @@ template <class R> inline void world_step(World<R>& w, Solver<R>& s) {
     stage_solve(w, s);
 }
 
+// ---------------------------------------------------------------- raycast (rigid/raycast)
+// RaycastResult (raycast_solver_base.h:7-13) as DefaultRaycastSolver::performRaycast leaves it,
+// with its return value.
+template <class R> struct RayHit {
+    bool hit = false;             // performRaycast's return value
+    int body = -1;                // m_collisionObject
+    R distance = Lim<R>::max();
+    V3<R> point, normal;
+};
+// RaycastBox::rayHitBox (raycast_box.cpp:31-60): slab test with the direction's tiny components
+// raised to PE_EPS; the normal is the entering slab's, x before y before z on ties.
+template <class R> inline bool ray_hit_box(const V3<R>& start, const V3<R>& direction, const V3<R>& bmin,
+                                           const V3<R>& bmax, R& distance, V3<R>& hp, V3<R>& hn) {
+    const R eps = R(0.00001);
+    V3<R> dir;
+    for (int k = 0; k < 3; k++) dir[k] = std::abs(direction[k]) < eps ? eps : direction[k];
+    V3<R> m, n, tmax, tmin;
+    for (int k = 0; k < 3; k++) {
+        m[k] = (bmax[k] - start[k]) / dir[k];
+        n[k] = (bmin[k] - start[k]) / dir[k];
+        tmax[k] = m[k] > n[k] ? m[k] : n[k];
+        tmin[k] = m[k] < n[k] ? m[k] : n[k];
+    }
+    R t_enter = tmin[0] > tmin[1] ? tmin[0] : tmin[1];
+    t_enter = t_enter > tmin[2] ? t_enter : tmin[2];
+    R t_exit = tmax[0] < tmax[1] ? tmax[0] : tmax[1];
+    t_exit = t_exit < tmax[2] ? t_exit : tmax[2];
+    if (t_enter > t_exit || t_exit < 0) return false;
+    distance = t_enter;
+    hp = start + dir * t_enter;
+    hn = V3<R>();
+    if (t_enter == tmin[0]) hn[0] = dir[0] > 0 ? R(-1.0) : R(1.0);
+    else if (t_enter == tmin[1]) hn[1] = dir[1] > 0 ? R(-1.0) : R(1.0);
+    else hn[2] = dir[2] > 0 ? R(-1.0) : R(1.0);
+    return true;
+}
+// RaycastBox::processRaycast (raycast_box.cpp:6-29)
+template <class R> inline bool ray_box(const V3<R>& start, const V3<R>& dir, R maxd, const Body<R>& b, R& d,
+                                       V3<R>& hp, V3<R>& hn) {
+    const V3<R> sl = b.xf.inv_apply(start), dl = mtv(b.xf.basis, dir);
+    const V3<R> sz(b.size[0], b.size[1], b.size[2]);
+    if (ray_hit_box(sl, dl, (-sz) / R(2.0), sz / R(2.0), d, hp, hn)) {
+        if (d <= maxd) {
+            hp = b.xf.apply(hp);
+            hn = mv(b.xf.basis, hn);
+            return true;
+        }
+        return false;
+    }
+    hp = hn = V3<R>();
+    d = Lim<R>::max();
+    return false;
+}
+// RaycastSphere::processRaycast (raycast_sphere.cpp:6-29); a sphere behind the start, or around it,
+// is hit at a negative distance.
+template <class R> inline bool ray_sphere(const V3<R>& start, const V3<R>& dir, R maxd, const Body<R>& b, R& d,
+                                          V3<R>& hp, V3<R>& hn) {
+    const R r = b.radius;
+    const V3<R> oc = b.xf.origin - start;
+    const R c2s = norm(oc);
+    const R s2p = dot(oc, dir);
+    const R q = c2s * c2s - s2p * s2p;
+    const R p2c = std::sqrt(q > R(0) ? q : R(0));
+    if (p2c > r) {
+        hp = hn = V3<R>();
+        d = Lim<R>::max();
+        return false;
+    }
+    const R p2h = std::sqrt(r * r - p2c * p2c);
+    d = s2p - p2h;
+    if (d <= maxd) {
+        hp = start + dir * d;
+        hn = normalized(hp - b.xf.origin);
+        return true;
+    }
+    return false;
+}
+// RaycastCylinder::processRaycast (raycast_cylinder.cpp:6-77), its branches and its cap points
+// (s + (h d - d * s) / d.y, with the top cap's y then set to h).
+template <class R> inline bool ray_cylinder(const V3<R>& start, const V3<R>& dir, R maxd, const Body<R>& b, R& d,
+                                            V3<R>& hp, V3<R>& hn) {
+    const R radius = b.radius, height = b.height / R(2.0);
+    const V3<R> sl = b.xf.inv_apply(start), dl = mtv(b.xf.basis, dir);
+    const R eps = R(0.00001);
+    auto cap = [&](R h) {
+        V3<R> p;
+        for (int k = 0; k < 3; k++) p[k] = sl[k] + (h * dl[k] - dl[k] * sl[k]) / dl[1];
+        return p;
+    };
+    bool hit = false;
+    if (std::abs(dl[0] - R(0)) < eps && std::abs(dl[2] - R(0)) < eps) {
+        const R d2a = std::sqrt(sl[0] * sl[0] + sl[2] * sl[2]);
+        if (d2a <= radius) {
+            const R h = dl[1] < 0 ? height : -height;
+            hp = cap(h);
+            hn = V3<R>(0, 1, 0);
+            if (dl[1] > 0) hn = -hn;
+            d = norm(hp - sl);
+            hit = true;
+        }
+    } else {
+        const R d2a = std::abs(dot(sl, normalized(cross(dl, V3<R>(0, 1, 0)))));
+        if (d2a <= radius) {
+            const R den = dl[0] * dl[0] + dl[2] * dl[2];
+            const R t = -(dl[0] * sl[0] + dl[2] * sl[2]) / den;
+            const R dt = std::sqrt(radius * radius - d2a * d2a) / std::sqrt(den);
+            const V3<R> in = sl + dl * (t - dt);
+            const V3<R> out = sl + dl * (t + dt);
+            if ((in[1] < -height && out[1] < -height) || (in[1] > height && out[1] > height)) {
+                hit = false;
+            } else if (in[1] >= -height && in[1] <= height) {
+                hp = in;
+                hn = normalized(V3<R>(in[0], R(0), in[2]));
+                d = t - dt;
+                hit = true;
+            } else if (in[1] > height) {
+                hp = cap(height);
+                hp[1] = height;
+                hn = V3<R>(0, 1, 0);
+                d = norm(hp - sl);
+                hit = true;
+            } else {
+                hp = cap(-height);
+                hn = -V3<R>(0, 1, 0);
+                d = norm(hp - sl);
+                hit = true;
+            }
+        }
+    }
+    if (!hit) {
+        hp = hn = V3<R>();
+        d = Lim<R>::max();
+        return false;
+    }
+    if (d <= maxd) {
+        hp = b.xf.apply(hp);
+        hn = mv(b.xf.basis, hn);
+        return true;
+    }
+    return false;
+}
+// RaycastMesh::rayHitTriangle (raycast_mesh.cpp:52-77), Moller-Trumbore; distances behind the start
+// are not rejected.
+template <class R> inline bool ray_triangle(const V3<R>& s, const V3<R>& dir, const V3<R>& v0, const V3<R>& v1,
+                                            const V3<R>& v2, R& d, V3<R>& hit) {
+    const R eps = R(0.00001);
+    const V3<R> e1 = v1 - v0, e2 = v2 - v0;
+    const V3<R> p = cross(dir, e2);
+    const R det = dot(e1, p);
+    if (det > -eps && det < eps) return false;
+    const R inv = R(1.0) / det;
+    const V3<R> t = s - v0;
+    const R u = dot(t, p) * inv;
+    if (u < 0 || u > R(1.0)) return false;
+    const V3<R> q = cross(t, e1);
+    const R v = dot(dir, q) * inv;
+    if (v < 0 || u + v > R(1.0)) return false;
+    d = dot(e2, q) * inv;
+    hit = s + dir * d;
+    return true;
+}
+// RaycastMesh::processRaycast (raycast_mesh.cpp:9-50): the faces the BVH finds in the ray's box
+// (start to start + length, grown by 0.1), each fanned from its first vertex, the first fan triangle
+// hit counting for the face; the nearest face wins (strictly nearer replaces).
+template <class R> inline bool ray_mesh(const V3<R>& start, const V3<R>& dir, R maxd, const Body<R>& b, R& d,
+                                        V3<R>& hp, V3<R>& hn) {
+    const Poly<R>& P = *b.poly;
+    const V3<R> sl = b.xf.inv_apply(start), dl = mtv(b.xf.basis, dir);
+    const V3<R> el = sl + dl * maxd;
+    V3<R> bmin, bmax;
+    for (int k = 0; k < 3; k++) {
+        bmin[k] = (sl[k] < el[k] ? sl[k] : el[k]) - R(0.1);
+        bmax[k] = (sl[k] > el[k] ? sl[k] : el[k]) + R(0.1);
+    }
+    std::vector<int> faces;
+    bvh_search(P, bmin, bmax, faces);
+    d = Lim<R>::max();
+    hp = hn = V3<R>();
+    for (int fi : faces) {
+        const auto& f = P.f[fi];
+        for (int i = 0; i < (int)f.size() - 2; i++) {
+            R t;
+            V3<R> hit;
+            if (ray_triangle(sl, dl, P.v[f[0]], P.v[f[i + 1]], P.v[f[i + 2]], t, hit)) {
+                if (t < d) {
+                    d = t;
+                    hp = hit;
+                    hn = P.fn[fi];
+                }
+                break;
+            }
+        }
+    }
+    if (d < maxd) {
+        hp = b.xf.apply(hp);
+        hn = mv(b.xf.basis, hn);
+        return true;
+    }
+    return false;
+}
+// DefaultRaycastSolver::performRaycast (default_raycast_solver.cpp:11-86) over the world's bodies in
+// order: ignore-collision bodies and the call's ignore set are skipped; a mesh body whose AABB the
+// (unbounded) ray misses ends the whole query with no hit, the result keeping what earlier bodies
+// put there; otherwise the strictly nearest hit within length wins.
+template <class R> inline RayHit<R> raycast(const World<R>& w, const V3<R>& start, const V3<R>& dir, R length,
+                                            const std::set<int>& ignore) {
+    RayHit<R> res;
+    for (int i = 0; i < (int)w.bodies.size(); i++) {
+        const Body<R>& b = w.bodies[i];
+        if (b.removed || b.ignore || ignore.count(i)) continue;
+        R d = 0;
+        V3<R> hp, hn;
+        bool ret = false;
+        if (b.type == ST_BOX) ret = ray_box(start, dir, length, b, d, hp, hn);
+        else if (b.type == ST_SPHERE) ret = ray_sphere(start, dir, length, b, d, hp, hn);
+        else if (b.type == ST_CYLINDER) ret = ray_cylinder(start, dir, length, b, d, hp, hn);
+        else {
+            if (!ray_hit_box(start, dir, b.aabb_min, b.aabb_max, d, hp, hn)) {
+                res.hit = false;
+                return res;
+            }
+            ret = ray_mesh(start, dir, length, b, d, hp, hn);
+        }
+        if (ret && d <= length && d < res.distance) {
+            res.distance = d;
+            res.point = hp;
+            res.normal = hn;
+            res.body = i;
+        }
+    }
+    res.hit = res.body >= 0;
+    return res;
+}
+
 // Parity table (what pins each function; the tests named here live in tests/):
 //   quat / mat helpers ........ pinned: bit-exact vs compiled reference (test_oracle_vs_ref)
 //   body_step / aabb / gravity  pinned: bit-exact vs compiled reference; closed-form KATs
@@ -1400,5 +1634,10 @@ template <class R> inline void world_step(World<R>& w, Solver<R>& s) {
 //                               convex hull, heightfield) bit-exact per stage vs the compiled reference
 //                               in FP32 and FP64 (test_oracle_vs_ref), and vs golden fixtures
 //                               (test_golden)
+//   joints / ignore pairs ..... pinned: bit-exact vs compiled reference (constraints demo, chains)
+//   sleep / lifetime / removal / neighbour counts / collision reports ... pinned: bit-exact vs the
+//                               compiled reference on one pool thread
+//   raycast ................... pinned: bit-exact vs the compiled reference's DefaultRaycastSolver
+//                               on every shape kind, and closed forms (sphere, box) on tiny cases
 
 }  // namespace pe_oracle

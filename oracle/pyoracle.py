@@ -141,6 +141,17 @@ class _Base:
         self._f("get_status")(ctypes.c_void_p(self.h), _i(st), _i(sc), _i(dc), _d(t))
         return st, sc, dc, t
 
+    def raycast(self, start, direction, length, ignore=()):
+        """DefaultRaycastSolver::performRaycast: (hit, body (-1: none), distance, point [3], normal [3]) as
+        the RaycastResult is left."""
+        f = self._f("raycast")
+        f.restype = ctypes.c_int
+        ray = np.ascontiguousarray(list(start) + list(direction) + [length], np.float64)
+        ig = np.ascontiguousarray(sorted(set(int(x) for x in ignore)) or [0], np.int32)
+        out = np.zeros(8)
+        hit = f(ctypes.c_void_p(self.h), _d(ray), ctypes.c_int(len(set(ignore))), _i(ig), _d(out))
+        return bool(hit), int(out[0]), out[1], out[2:5].copy(), out[5:8].copy()
+
     def set_report(self, body, on=True):
         """RigidBody::addCollisionCallback (on) / clearCollisionCallbacks (off)."""
         self._f("set_report")(ctypes.c_void_p(self.h), ctypes.c_int(body), ctypes.c_int(1 if on else 0))

@@ -17,6 +17,7 @@
 #include "rigid/constraint/constraint/ball_joint_constraint.h"
 #include "rigid/constraint/constraint/hinge_joint_constraint.h"
 #include "rigid/constraint/constraint/slider_joint_constraint.h"
+#include "rigid/raycast/raycast_solver/default_raycast_solver.h"
 // The pool's members are private; this test driver needs them to retire the workers at exit.
 #define private public
 #include "utils/thread_pool.h"
@@ -82,6 +83,7 @@ struct RefWorld : public pe_intf::World {
     }
     pe::Array<pe_phys_collision::CollisionPair>& pairs() { return _collision_pairs; }
     pe::Array<pe_phys_collision::ContactResult*>& results() { return _contact_results; }
+    const pe::Array<pe_phys_object::RigidBody*>& objects() const { return _rigid_objects; }
 };
 
 // pe::Mesh of a pe_mesh_desc (vertex normals are not used by the rigid pipeline: zero)
@@ -300,6 +302,22 @@ REF_API void pref_set_report(void* h, int body, int on) {
 REF_API void pref_remove_body(void* h, int body) {
     auto* w = (RefWorld*)h;
     w->removeRigidBody(w->owned[body]);
+}
+// DefaultRaycastSolver::performRaycast over the world's bodies, the ignore set by body index;
+// out = body (-1: none), distance, hit point, hit normal, as the RaycastResult is left.
+REF_API int pref_raycast(void* h, const double* ray, int nignore, const int32_t* ignore, double* out) {
+    auto* w = (RefWorld*)h;
+    pe::Uint32HashList ig;
+    for (int k = 0; k < nignore; k++) ig.push_back(w->owned[ignore[k]]->getGlobalId());
+    pe_phys_raycast::DefaultRaycastSolver solver;
+    pe_phys_raycast::RaycastResult res;
+    const bool hit = solver.performRaycast(pe::Vector3(pe::Real(ray[0]), pe::Real(ray[1]), pe::Real(ray[2])),
+                                           pe::Vector3(pe::Real(ray[3]), pe::Real(ray[4]), pe::Real(ray[5])),
+                                           pe::Real(ray[6]), w->objects(), ig, res);
+    out[0] = res.m_collisionObject ? w->index[res.m_collisionObject] : -1;
+    out[1] = (double)res.m_distance;
+    for (int k = 0; k < 3; k++) { out[2 + k] = (double)res.m_hitPoint[k]; out[5 + k] = (double)res.m_hitNormal[k]; }
+    return hit ? 1 : 0;
 }
 REF_API int pref_num_callbacks(void* h) { return (int)((RefWorld*)h)->cb_ids.size() / 2; }
 REF_API void pref_get_callbacks(void* h, int32_t* ids, double* data) {

@@ -16,6 +16,7 @@
 #include "pe_pipeline.hpp"
 #include "pe_sap.hpp"
 #include "pe_joints.hpp"
+#include "pe_raycast.hpp"
 #include "pe_host.hpp"
 #include "../../include/pe_amd.h"
 
@@ -80,6 +81,83 @@ __global__ void k_integrate_status(Dev d) {
     const StatusView st = d.st;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.b.n; i += gridDim.x * blockDim.x)
         integrate_body<true>(d.b, i, d.dt, d.g, &st);
+}
+
+// DefaultRaycastSolver::performRaycast (default_raycast_solver.cpp:11-86), one 64-lane workgroup per
+// ray over the world's bodies.  Skipped: removed bodies, ignore-collision bodies, the call's ignore
+// set (sorted).  The reference walks the bodies in order and ends the query, reporting no hit, at the
+// first mesh body whose AABB the unbounded ray misses, its result keeping the best hit of the bodies
+// before it; so the first pass finds that body (the lowest such index), the second takes the nearest
+// hit among the bodies before it, ties to the lowest index (the reference replaces a hit only by a
+// strictly nearer one), as a minimum over (distance key, index).
+constexpr int RAY_IN = 7;    // start (3), direction (3), length
+constexpr int RAY_OUT = 12;  // hit, body (int bits), distance, point (3), normal (3), pad
+__global__ void __launch_bounds__(64) k_raycast(Dev d, const float* rays, const int32_t* ignore, int nignore, float* out) {
+    __shared__ int s_abort[64];
+    __shared__ uint64_t s_best[64];
+    const int lane = threadIdx.x;
+    const float* r = rays + (size_t)blockIdx.x * RAY_IN;
+    const f3 start{r[0], r[1], r[2]}, dir{r[3], r[4], r[5]};
+    const float len = r[6];
+    const int n = d.b.n;
+    auto skipped = [&](int i) {
+        if (d.status_on && (d.st.flags[i] & ST_REMOVED)) return true;
+        if (d.b.at(F_SP + 3, i) == COLL_IGNORE) return true;
+        int lo = 0, hi = nignore;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ignore[mid] < i) lo = mid + 1; else hi = mid;
+        }
+        return lo < nignore && ignore[lo] == i;
+    };
+    int abort = 0x7fffffff;
+    for (int i = lane; i < n; i += 64) {
+        if (d.b.iat(I_TYPE, i) < SHAPE_CONVEX || skipped(i)) continue;
+        float t;
+        f3 hp, hn;
+        if (!ray_hit_box(start, dir, ld3(d.b, F_AMIN, i), ld3(d.b, F_AMAX, i), t, hp, hn)) { abort = i; break; }
+    }
+    s_abort[lane] = abort;
+    __syncthreads();
+    for (int k = 0; k < 64; k++) abort = s_abort[k] < abort ? s_abort[k] : abort;
+    const int lim = n < abort ? n : abort;
+    uint64_t best = ~0ull;
+    float bd = REAL_MAX;
+    f3 bp{0.0f, 0.0f, 0.0f}, bn{0.0f, 0.0f, 0.0f};
+    for (int i = lane; i < lim; i += 64) {
+        if (skipped(i)) continue;
+        const int type = d.b.iat(I_TYPE, i);
+        const f3 o = ld3(d.b, F_POS, i);
+        const m33 R = ld9(d.b, F_ROT, i);
+        const float sp[3] = {d.b.at(F_SP, i), d.b.at(F_SP + 1, i), d.b.at(F_SP + 2, i)};
+        float t = 0.0f;
+        f3 hp, hn;
+        bool ret;
+        if (type == SHAPE_BOX) ret = ray_box(start, dir, len, sp, R, o, t, hp, hn);
+        else if (type == SHAPE_SPHERE) ret = ray_sphere(start, dir, len, sp[0], o, t, hp, hn);
+        else if (type == SHAPE_CYLINDER) ret = ray_cylinder(start, dir, len, sp[0], sp[1], R, o, t, hp, hn);
+        else ret = ray_mesh(d.b.ms, __float_as_int(sp[0]), start, dir, len, R, o, t, hp, hn);
+        if (ret && t <= len && t < REAL_MAX) {
+            const uint64_t key = ((uint64_t)float_key(t + 0.0f) << 32) | (uint32_t)i;
+            if (key < best) { best = key; bd = t; bp = hp; bn = hn; }
+        }
+    }
+    __syncthreads();
+    s_best[lane] = best;
+    __syncthreads();
+    uint64_t win = ~0ull;
+    for (int k = 0; k < 64; k++) win = s_best[k] < win ? s_best[k] : win;
+    float* o = out + (size_t)blockIdx.x * RAY_OUT;
+    if (win == ~0ull) {
+        if (lane == 0) {
+            const float e[RAY_OUT] = {0.0f, __int_as_float(-1), REAL_MAX, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            st_rec<RAY_OUT / 4>(o, e);
+        }
+    } else if (best == win) {
+        const float e[RAY_OUT] = {abort == 0x7fffffff ? 1.0f : 0.0f, __int_as_float((int)(uint32_t)win), bd,
+                                  bp.x, bp.y, bp.z, bn.x, bn.y, bn.z, 0.0f, 0.0f, 0.0f};
+        st_rec<RAY_OUT / 4>(o, e);
+    }
 }
 
 // Recomputes AABBs and scales only (after pe_world_set_state); a removed body stays out of the sweep.
@@ -327,12 +405,15 @@ struct pe_world {
     std::vector<char> finite_life, has_joint, removed;   // per body (joints and removals exclude each other)
     void* joint_mem[4] = {};                 // device: definitions, rows, order, colour starts
     void* ignore_mem = nullptr;
+    void* ray_mem = nullptr;                 // raycast inputs, ignore set and results
+    size_t ray_bytes = 0;
 
     ~pe_world() {
         if (stream) hipStreamSynchronize(stream);
         for (void* p : allocs) hipFree(p);
         for (void* p : joint_mem) if (p) hipFree(p);
         if (ignore_mem) hipFree(ignore_mem);
+        if (ray_mem) hipFree(ray_mem);
         meshes.release();
         for (auto& e : ev) if (e) hipEventDestroy(e);
         for (auto& e : order_ev) if (e) hipEventDestroy(e);
@@ -501,8 +582,12 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&sp.cand, n + 1);
     rc |= w->alloc(&sp.hist, 256 * (size_t)sap_radix_blocks(n));
     rc |= w->alloc(&sp.red, SAP_RED_VALUES * (size_t)sap_red_blocks(n));
-    // sweep chunks: candidates are bounded by n * (n - 1) / 2; size for pcap * 64 candidates
-    sp.chunk_cap = (int)std::min<int64_t>((int64_t)pcap * 64 / SAP_CHUNK + 1, (int64_t)1 << 26);
+    // sweep chunks: the candidates (bodies overlapping on the sweep axis) are at most n (n - 1) / 2,
+    // and a pile that lands while the axis is still the fall direction comes close to that (the
+    // reference's sweep too: the axis is chosen after the sweep); 8 bytes per 32 candidates, so the
+    // bound itself is allocated, up to 2^31 candidates (the counters are 32-bit)
+    const int64_t max_cand = std::min<int64_t>((int64_t)n * (n - 1) / 2, (int64_t)1 << 31);
+    sp.chunk_cap = (int)std::max<int64_t>(max_cand / SAP_CHUNK + 1, 1024);
     rc |= w->alloc(&sp.chunk, sp.chunk_cap);
     rc |= w->alloc(&sp.chunk_hit, sp.chunk_cap);
     rc |= w->alloc(&sp.scan_part, sap_scan_parts(n, sp.chunk_cap));
@@ -668,6 +753,49 @@ int pe_world_add_ignore_pairs(pe_world* w, int32_t n, const int32_t* pairs) {
 }
 
 static int read_counts(pe_world* w, DevCounts& c);
+
+int pe_world_raycast(pe_world* w, int32_t nrays, const float* rays, int32_t nignore, const int32_t* ignore,
+                     pe_ray_hit* out) {
+    if (int rc = check_world(w)) return rc;
+    if (nrays < 0 || nignore < 0 || (nrays > 0 && (!rays || !out)) || (nignore > 0 && !ignore) || nrays > (1 << 24)) {
+        pe_set_error("pe_world_raycast: invalid argument");
+        return PE_ERR_INVALID;
+    }
+    if (nrays == 0) return PE_OK;
+    std::vector<int32_t> ig(ignore, ignore + nignore);
+    std::sort(ig.begin(), ig.end());
+    ig.erase(std::unique(ig.begin(), ig.end()), ig.end());
+    pe_clear_stale_error();
+    const size_t in_b = (size_t)nrays * RAY_IN * 4, ig_b = std::max<size_t>(ig.size(), 1) * 4, out_b = (size_t)nrays * RAY_OUT * 4;
+    if (w->ray_bytes < in_b + ig_b + out_b + 512) {
+        if (w->ray_mem) hipFree(w->ray_mem);
+        w->ray_mem = nullptr;
+        w->ray_bytes = 0;
+        PE_HIP(hipMalloc(&w->ray_mem, in_b + ig_b + out_b + 512));
+        w->ray_bytes = in_b + ig_b + out_b + 512;
+    }
+    char* base = (char*)w->ray_mem;
+    float* d_rays = (float*)base;
+    int32_t* d_ig = (int32_t*)(base + ((in_b + 255) & ~(size_t)255));
+    float* d_out = (float*)(base + ((in_b + 255) & ~(size_t)255) + ((ig_b + 255) & ~(size_t)255));
+    PE_HIP(hipMemcpyAsync(d_rays, rays, in_b, hipMemcpyHostToDevice, w->stream));
+    if (!ig.empty()) PE_HIP(hipMemcpyAsync(d_ig, ig.data(), ig.size() * 4, hipMemcpyHostToDevice, w->stream));
+    hipLaunchKernelGGL(k_raycast, dim3(nrays), dim3(64), 0, w->stream, w->d, (const float*)d_rays, (const int32_t*)d_ig,
+                       (int)ig.size(), d_out);
+    PE_HIP(hipGetLastError());
+    std::vector<float> res((size_t)nrays * RAY_OUT);
+    PE_HIP(hipMemcpyAsync(res.data(), d_out, out_b, hipMemcpyDeviceToHost, w->stream));
+    PE_HIP(hipStreamSynchronize(w->stream));
+    for (int k = 0; k < nrays; k++) {
+        const float* e = &res[(size_t)k * RAY_OUT];
+        pe_ray_hit& h = out[k];
+        h.hit = e[0] != 0.0f ? 1 : 0;
+        std::memcpy(&h.body, &e[1], 4);
+        h.distance = e[2];
+        for (int c = 0; c < 3; c++) { h.point[c] = e[3 + c]; h.normal[c] = e[6 + c]; }
+    }
+    return PE_OK;
+}
 
 int pe_world_set_sleep(pe_world* w, double lin_vel2, double ang_vel2, double time) {
     if (int rc = check_world(w)) return rc;

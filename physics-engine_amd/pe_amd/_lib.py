@@ -25,7 +25,7 @@ EXPORTS = [
     "pe_world_get_counts", "pe_world_set_solver_blocks", "pe_world_get_records", "pe_world_step_async",
     "pe_world_get_cache", "pe_world_set_cache", "pe_world_add_joints", "pe_world_add_ignore_pairs",
     "pe_world_set_sleep", "pe_world_remove_body", "pe_world_get_body_status", "pe_world_set_collision_reports",
-    "pe_world_get_collision_events",
+    "pe_world_get_collision_events", "pe_world_raycast",
     "pe_stage_create", "pe_stage_destroy", "pe_stage_set_meshes", "pe_stage_broadphase", "pe_stage_narrowphase",
     "pe_stage_solve", "pe_stage_broadphase_noalloc",
     "pe_batch_create", "pe_batch_destroy", "pe_batch_step", "pe_batch_sync", "pe_batch_get_world_state",

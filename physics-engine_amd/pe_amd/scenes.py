@@ -83,6 +83,12 @@ class CollisionEvent(ctypes.Structure):
                 ("normal", ctypes.c_float * 3), ("depth", ctypes.c_float), ("velocity", ctypes.c_float * 3)]
 
 
+class RayHit(ctypes.Structure):
+    """ctypes mirror of pe_ray_hit."""
+    _fields_ = [("hit", ctypes.c_int32), ("body", ctypes.c_int32), ("distance", ctypes.c_float),
+                ("point", ctypes.c_float * 3), ("normal", ctypes.c_float * 3)]
+
+
 class JointDesc(ctypes.Structure):
     """ctypes mirror of pe_joint_desc (a ball / hinge / slider joint between two bodies)."""
     _fields_ = [("type", ctypes.c_int32), ("body_a", ctypes.c_int32), ("body_b", ctypes.c_int32),

@@ -84,6 +84,22 @@ class World:
             check(lib().pe_world_step(self.h, ctypes.c_int32(1)))
             self.dispatch_callbacks()
 
+    # ---- raycast
+    def raycast(self, starts, directions, lengths, ignore=()):
+        """DefaultRaycastSolver::performRaycast for each ray (pe_world_raycast): starts, directions [R, 3],
+        lengths [R]; ignore: body indices skipped by every ray.  Returns (hit [R] bool, body [R] int32,
+        distance [R], point [R, 3], normal [R, 3]) as each RaycastResult is left."""
+        from .scenes import RayHit
+        st = np.asarray(starts, np.float32).reshape(-1, 3)
+        rays = np.ascontiguousarray(np.concatenate([st, np.asarray(directions, np.float32).reshape(-1, 3),
+                                                    np.asarray(lengths, np.float32).reshape(-1, 1)], axis=1))
+        ig = np.ascontiguousarray(np.asarray(list(ignore) or [0], np.int32))
+        out = (RayHit * max(len(rays), 1))()
+        check(lib().pe_world_raycast(self.h, ctypes.c_int32(len(rays)), _f(rays), ctypes.c_int32(len(ignore)), _i(ig), out))
+        raw = np.frombuffer(out, dtype=np.float32, count=9 * len(rays)).reshape(-1, 9)
+        ints = raw[:, :2].copy().view(np.int32)
+        return ints[:, 0] != 0, ints[:, 1].copy(), raw[:, 2].copy(), raw[:, 3:6].copy(), raw[:, 6:9].copy()
+
     # ---- sleeping, lifetimes, removal, collision callbacks
     def set_sleep(self, lin_vel2: float, ang_vel2: float, time: float):
         """World::setSleepLinVel2Threshold / setSleepAngVel2Threshold / setSleepTimeThreshold."""

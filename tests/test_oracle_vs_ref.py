@@ -284,6 +284,94 @@ def test_sleep_lifetime_and_callbacks_bit_exact():
     print(f"[sleep] fell asleep {slept} times, woke {woke} times, {calls} callback calls", flush=True)
 
 
+def _ray_lockstep(sc, steps, bits):
+    r, o = RefWorld(sc, bits, threads=1), OracleWorld(sc, bits, color_order=False)
+    for _ in range(steps):
+        r.integrate(); o.integrate()
+        r.broadphase()
+        o.set_pairs(r.pairs())
+        r.narrowphase(); o.narrowphase()
+        r.solve(); o.solve()
+        o.set_state(*r.state())
+    return r, o
+
+
+def _same_ray(a, b):
+    return (a[0] == b[0] and a[1] == b[1] and a[2] == b[2] and np.array_equal(a[3], b[3])
+            and np.array_equal(a[4], b[4]))
+
+
+@pytest.mark.ref
+@pytest.mark.parametrize("bits", [32, 64])
+def test_raycast_bit_exact(bits):
+    """DefaultRaycastSolver::performRaycast (default_raycast_solver.cpp:11-86) after 30 steps: the
+    oracle's hit flag, body, distance, point and normal equal the compiled reference's for rays in
+    every direction (some with ignore sets, some starting inside bodies) over a box rain, over the
+    mixed-shape scene (whose many hulls end most queries early: a mesh body whose AABB the ray misses
+    makes the query report no hit, the result keeping the earlier bodies' best), and over one hull on
+    the heightfield with rays aimed down at the hull (mesh hits)."""
+    from pe_amd import scenes
+    rng = np.random.default_rng(1)
+    mix = scenes.shape_mix_scene(160, seed=23, volume=(10, 6, 10), grid=12, ground_size=(14, 14))
+    conv = np.nonzero(mix.shape_type == scenes.SHAPE_CONVEX)[0]
+    keep = sorted([i for i in range(mix.n) if mix.shape_type[i] != scenes.SHAPE_CONVEX] + [int(conv[0])])
+    one_hull = mix.subset(keep)
+    counts = {}
+    for name, sc in (("rain", small_rain(150, seed=2)), ("mix", mix), ("hull", one_hull)):
+        r, o = _ray_lockstep(sc, 30, bits)
+        pos = r.state()[0]
+        lo, hi = pos.min(0) - 1, pos.max(0) + 1
+        hull = keep.index(int(conv[0]))
+        hits = partial = mesh_hits = 0
+        for k in range(300):
+            if name == "hull":
+                tgt = pos[hull] + rng.uniform(-0.4, 0.4, 3) if k % 2 == 0 else rng.uniform(lo, hi)
+                start, d = tgt + np.array([0.0, 8.0, 0.0]), np.array([0.0, -1.0, 0.0]) + rng.normal(size=3) * 0.01
+                length, ignore = 20.0, []
+            else:
+                start = rng.uniform(lo, hi)
+                d = rng.normal(size=3) if k % 3 else np.array([rng.normal() * 0.2, -1.0, rng.normal() * 0.2])
+                if k % 7 == 0:
+                    d = np.array([0.0, -1.0, 0.0])
+                length = float(rng.uniform(0.5, 40.0))
+                ignore = rng.choice(sc.n, 3, replace=False).tolist() if k % 5 == 0 else []
+            d = d / np.linalg.norm(d)
+            a, b = r.raycast(start, d, length, ignore), o.raycast(start, d, length, ignore)
+            assert _same_ray(a, b), f"{name} ray {k}: {a} vs {b}"
+            hits += a[0]
+            partial += (not a[0]) and a[1] >= 0
+            mesh_hits += a[0] and sc.shape_type[a[1]] >= scenes.SHAPE_CONVEX
+        counts[name] = (hits, partial, mesh_hits)
+    print(f"[raycast FP{bits}] (hits, ended early with a partial result, mesh hits): {counts}", flush=True)
+    assert counts["rain"][0] > 50 and counts["mix"][1] > 10 and counts["hull"][2] > 5
+
+
+def test_raycast_closed_forms():
+    """What the geometry fixes, on one unit box and one sphere: a ray down onto the box's top face
+    hits at (start height - 0.5) with normal +y (the point displaced along the EPS-raised direction); a ray through a sphere's centre hits at
+    (distance to centre - r) with the normal pointing back at the start; a ray that starts past a
+    sphere hits it at a negative distance (RaycastSphere keeps it, raycast_sphere.cpp:21-27); a ray
+    pointing away from the box misses."""
+    from pe_amd.scenes import SHAPE_BOX, SHAPE_SPHERE, SceneBuilder
+    b = SceneBuilder()
+    b.add(SHAPE_BOX, (1.0, 1.0, 1.0), 1.0, (0.0, 0.0, 0.0), kin=1)
+    b.add(SHAPE_SPHERE, (0.5,), 1.0, (5.0, 0.0, 0.0), kin=1)
+    sc = b.scene(name="ray_kat")
+    o = OracleWorld(sc, 64)
+    o.integrate()
+    hit, body, d, p, n = o.raycast((0.25, 3.0, 0.1), (0.0, -1.0, 0.0), 10.0)
+    # the hit point follows the direction with its zero components raised to PE_EPS = 1e-5
+    # (raycast_box.cpp:35-38, 49): x and z move by 2.5e-5 over the 2.5 m
+    assert hit and body == 0 and d == 2.5 and np.allclose(p, (0.25 + 2.5e-5, 0.5, 0.1 + 2.5e-5), rtol=0, atol=1e-12)
+    assert np.array_equal(n, (0, 1, 0))
+    hit, body, d, p, n = o.raycast((5.0, 4.0, 0.0), (0.0, -1.0, 0.0), 10.0, ignore=[0])
+    assert hit and body == 1 and d == 3.5 and np.allclose(n, (0, 1, 0))
+    hit, body, d, p, n = o.raycast((5.0, 4.0, 0.0), (0.0, 1.0, 0.0), 10.0, ignore=[0])
+    assert hit and body == 1 and d == -4.5
+    hit, body, d, p, n = o.raycast((0.0, 3.0, 0.0), (0.0, 1.0, 0.0), 10.0, ignore=[1])
+    assert not hit and body == -1
+
+
 def test_batches_refuse_joints():
     from pe_amd import scenes
     from pe_amd.batch import Batch
