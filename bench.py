@@ -571,8 +571,8 @@ def main():
     if os.path.exists(tp):
         with open(tp) as f:
             tj = json.load(f)
-        same = {kk: list(v) for kk, v in tj.get("schedule", {}).items() if kk in sched} == \
-            {kk: list(v) for kk, v in sched.items() if kk in ("class_upper_bodies", "block", "solver_group_bodies")}
+        keys = ("class_upper_bodies", "block", "solver_group_bodies")
+        same = all(list(tj.get("schedule", {}).get(kk, [])) == list(sched[kk]) for kk in keys)
         if same and tj.get("window") == f"steps 0..{K}":
             traffic, traffic_note = tj["stage_bytes_per_launch"][k], "counter passes over this schedule and window"
         else:

@@ -36,6 +36,7 @@ extern "C" {
 #define PE_SHAPE_CYLINDER  2
 #define PE_SHAPE_CONVEX    3
 #define PE_SHAPE_CONCAVE   4
+#define PE_SHAPE_COMPOUND  5   /* children of box / sphere / cylinder / convex shapes (single world) */
 
 /* A polygon mesh as a reference user hands it to ConvexMeshShape::setMesh or
  * ConcaveMeshShape::setMesh (pe::Mesh, common/mesh.h: vertex positions, faces as vertex index
@@ -81,7 +82,26 @@ typedef struct pe_scene_desc {
     const int32_t* ignore_collision;  /* [n] RigidBody::setIgnoreCollision (0/1: the body is in no pair), or NULL */
     const double* lifetime;           /* [n] RigidBody::setLifeTime, seconds (the body leaves the world once it
                                          has been stepped that long; >= 1e30 never), or NULL (never) */
+    const int32_t* compound_start;    /* [n + 1] or NULL: the children of compound body i are
+                                         compound_children[compound_start[i] .. compound_start[i + 1]) */
+    const struct pe_compound_child* compound_children;
 } pe_scene_desc;
+
+/* A child of a compound body (CompoundShape::addShape, compound_shape.cpp:7-20): a box, sphere,
+ * cylinder or convex mesh at a transform relative to the body, with its mass ratio (> 1e-5).  The
+ * library applies CompoundShape::init (compound_shape.cpp:24-47) as pe_world_create runs: the
+ * children move so that their mass-ratio-weighted centroid is the body's origin, and the body's
+ * inertia is the children's (each about its own centre, shifted, weighted by its ratio) times the
+ * body's mass.  The body's transform is taken as given (the reference's compound demo sets the
+ * transform's origin minus the centroid init returns, examples/compound_demo/main.cpp:77-78). */
+typedef struct pe_compound_child {
+    int32_t shape_type;               /* PE_SHAPE_BOX / SPHERE / CYLINDER / CONVEX */
+    int32_t mesh_index;               /* convex: the scene mesh (pe_scene_desc.meshes) */
+    double shape_param[3];            /* as pe_scene_desc.shape_param */
+    double position[3];               /* local transform origin */
+    double rotation[9];               /* local transform basis, row-major */
+    double mass_ratio;
+} pe_compound_child;
 
 /* Joint constraints (src/rigid/constraint/constraint/{ball,hinge,slider}_joint_constraint.h), as a
  * reference user configures them before World::addConstraint (world.h:90): the two bodies

@@ -53,6 +53,31 @@ template <class R> Handle<R>* create(const pe_scene_desc* sc, const pe_world_par
             b.radius = R(sp[0]); b.height = R(sp[1]);
             w.polys.push_back(std::make_shared<Poly<R>>(cylinder_poly(b)));
             b.poly = w.polys.back().get();
+        } else if (b.type == ST_COMPOUND) {
+            // CompoundShape::addShape for each child, then CompoundShape::init
+            for (int k = sc->compound_start[i]; k < sc->compound_start[i + 1]; k++) {
+                const pe_compound_child& ch = sc->compound_children[k];
+                Body<R> kid;
+                kid.type = ch.shape_type;
+                if (kid.type == ST_BOX) {
+                    for (int q = 0; q < 3; q++) { kid.size[q] = R(ch.shape_param[q]); kid.half[q] = kid.size[q] / R(2); }
+                    w.polys.push_back(std::make_shared<Poly<R>>(box_poly(kid)));
+                    kid.poly = w.polys.back().get();
+                } else if (kid.type == ST_SPHERE) {
+                    kid.radius = R(ch.shape_param[0]);
+                } else if (kid.type == ST_CYLINDER) {
+                    kid.radius = R(ch.shape_param[0]); kid.height = R(ch.shape_param[1]);
+                    w.polys.push_back(std::make_shared<Poly<R>>(cylinder_poly(kid)));
+                    kid.poly = w.polys.back().get();
+                } else {
+                    kid.poly = mesh_poly(ch.mesh_index, true);
+                }
+                for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) kid.xf.basis(r, c) = R(ch.rotation[3 * r + c]);
+                for (int q = 0; q < 3; q++) kid.xf.origin[q] = R(ch.position[q]);
+                b.kids.push_back(kid);
+                b.ratios.push_back(R(ch.mass_ratio));
+            }
+            compound_init(b);
         } else {
             b.poly = mesh_poly(sc->mesh_index[i], b.type == ST_CONVEX);
         }

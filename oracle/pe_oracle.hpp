@@ -31,7 +31,7 @@
 
 namespace pe_oracle {
 
-enum ShapeType { ST_BOX = 0, ST_SPHERE = 1, ST_CYLINDER = 2, ST_CONVEX = 3, ST_CONCAVE = 4 };
+enum ShapeType { ST_BOX = 0, ST_SPHERE = 1, ST_CYLINDER = 2, ST_CONVEX = 3, ST_CONCAVE = 4, ST_COMPOUND = 5 };
 
 template <class R> struct Lim;
 template <> struct Lim<float>  { static float  max() { return 1e30f; }  static float  min() { return -1e30f; } };
@@ -218,6 +218,11 @@ template <class R> struct Body {
     R sleep_time = 0, life_time = Lim<R>::max(), last_time = 0;
     int static_count = 0, dynamic_count = 0;
     bool report = false;             // has collision callbacks (RigidBody::addCollisionCallback)
+    // a compound body (CompoundShape): its children as bodies whose xf is the local transform, their
+    // mass ratios, and the unit-mass inertia CompoundShape::init computed
+    std::vector<Body> kids;
+    std::vector<R> ratios;
+    M3<R> compound_inertia;
 };
 
 template <class R> M3<R> poly_inertia(const Poly<R>& p);
@@ -235,10 +240,44 @@ template <class R> inline M3<R> shape_inertia(const Body<R>& b) {
         const R r_2 = b.radius * b.radius, h_2 = b.height * b.height;
         const R axis = r_2 * R(0.5), diag = r_2 / R(4.0) + h_2 / R(12.0);
         I(0, 0) = diag; I(1, 1) = axis; I(2, 2) = diag;
+    } else if (b.type == ST_COMPOUND) {
+        I = b.compound_inertia;      // CompoundShape::init
     } else {
         I = poly_inertia(*b.poly);   // convex_mesh_shape.cpp:93-100 / concave_mesh_shape.cpp:9-10
     }
     return I;
+}
+// CompoundShape::init (compound_shape.cpp:24-47): the mass-ratio-weighted centroid of the children's
+// origins; every child moved by minus it; the unit-mass inertia sum over the children of
+// (R I_child R^T + |p|^2 1 - p p^T) * ratio / total.  Returns the centroid.
+template <class R> inline V3<R> compound_init(Body<R>& b) {
+    R total = 0;
+    for (R r : b.ratios) total += r;
+    V3<R> c;
+    for (size_t k = 0; k < b.kids.size(); k++) c = c + b.kids[k].xf.origin * b.ratios[k];
+    c = c / total;
+    M3<R> I;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) I(i, j) = 0;
+    for (size_t k = 0; k < b.kids.size(); k++) {
+        Body<R>& kid = b.kids[k];
+        const M3<R> si = shape_inertia(kid);
+        const V3<R> p = kid.xf.origin - c;
+        kid.xf.origin = p;
+        const M3<R>& rot = kid.xf.basis;
+        const R pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        M3<R> t;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) t(i, j) = (i == j ? R(1) : R(0)) * pp;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) t(i, j) -= p[i] * p[j];
+        const M3<R> T = mm(rot, si);
+        const R q = b.ratios[k] / total;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const R a = T(i, 0) * rot(j, 0) + (T(i, 1) * rot(j, 1) + T(i, 2) * rot(j, 2));
+                I(i, j) += (a + t(i, j)) * q;
+            }
+    }
+    b.compound_inertia = I;
+    return c;
 }
 
 // RigidBody::setMass then setCollisionShape then setTransform (rigidbody.cpp:11-46)
@@ -288,6 +327,18 @@ template <class R> inline void body_aabb(Body<R>& b) {
         ext[0] += b.radius * std::sqrt(axis[1] * axis[1] + axis[2] * axis[2]);
         ext[2] += b.radius * std::sqrt(axis[0] * axis[0] + axis[1] * axis[1]);
         b.aabb_min = pos - ext; b.aabb_max = pos + ext;
+    } else if (b.type == ST_COMPOUND) {  // compound_shape.cpp:49-58: the union of the children's
+        V3<R> mn(Lim<R>::max(), Lim<R>::max(), Lim<R>::max()), mx(Lim<R>::min(), Lim<R>::min(), Lim<R>::min());
+        for (const Body<R>& kid : b.kids) {
+            Body<R> t = kid;
+            t.xf = Xf<R>{mm(b.xf.basis, kid.xf.basis), mv(b.xf.basis, kid.xf.origin) + b.xf.origin};
+            body_aabb(t);
+            for (int k = 0; k < 3; k++) {
+                mn[k] = mn[k] < t.aabb_min[k] ? mn[k] : t.aabb_min[k];
+                mx[k] = mx[k] > t.aabb_max[k] ? mx[k] : t.aabb_max[k];
+            }
+        }
+        b.aabb_min = mn; b.aabb_max = mx;
     } else {  // convex_mesh_shape.cpp:138-150 (the concave mesh inherits it)
         poly_aabb(*b.poly, rot, pos, b.aabb_min, b.aabb_max);
     }
@@ -945,6 +996,73 @@ template <class R> inline void collision_events(World<R>& w) {
     }
 }
 
+// NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) and each algorithm's processCollision
+// for shapes A and B (bodies, or compound children posed in the world), adding to the result cr whose
+// objects are cr.a / cr.b: which shape plays which role, and the swap flag it sets.
+template <class R> inline void dispatch_pair(World<R>& w, ContactResult<R>& cr, const Body<R>& A, const Body<R>& B,
+                                             R ref_scale) {
+    const int ta = A.type, tb = B.type;
+    auto is = [&](int x, int y) { return (ta == x && tb == y) || (ta == y && tb == x); };
+    auto pick = [&](int t) -> const Body<R>& { return ta == t ? A : B; };
+    auto other = [&](int t) -> const Body<R>& { return ta == t ? B : A; };
+    auto part = [](const Body<R>& b) { return Part<R>{b.type == ST_CONCAVE ? ST_CONVEX : b.type, &b, b.poly}; };
+    if (ta == ST_COMPOUND || tb == ST_COMPOUND) {
+        // compound_compound_collision_algorithm.cpp:8-48: a compound B trades places with A (the result's
+        // objects too); each child of A, posed by A's transform, against B or each child of B
+        const Body<R>* pa = &A;
+        const Body<R>* pb = &B;
+        if (tb == ST_COMPOUND) {
+            std::swap(pa, pb);
+            std::swap(cr.a, cr.b);
+        }
+        for (const Body<R>& ka : pa->kids) {
+            Body<R> ca = ka;
+            ca.xf = xf_mul(pa->xf, ka.xf);
+            if (pb->type == ST_COMPOUND) {
+                for (const Body<R>& kb : pb->kids) {
+                    Body<R> cb = kb;
+                    cb.xf = xf_mul(pb->xf, kb.xf);
+                    dispatch_pair(w, cr, ca, cb, ref_scale);
+                }
+            } else {
+                dispatch_pair(w, cr, ca, *pb, ref_scale);
+            }
+        }
+    } else if (ta == ST_BOX && tb == ST_BOX) box_box(w, cr, A, B);
+    else if (is(ST_BOX, ST_SPHERE)) box_sphere(w, cr, A, B);
+    else if (ta == ST_SPHERE && tb == ST_SPHERE) sphere_sphere(w, cr, A, B);
+    else if (is(ST_BOX, ST_CYLINDER)) {            // box_cylinder_collision_algorithm.cpp:13-47
+        cr.swap = ta == ST_BOX;
+        box_cylinder(w, cr, pick(ST_BOX), pick(ST_CYLINDER));
+        cr.swap = false;
+    } else if (is(ST_SPHERE, ST_CYLINDER)) {       // sphere_cylinder_collision_algorithm.cpp:8-30
+        cr.swap = ta == ST_SPHERE;
+        sphere_cylinder(w, cr, pick(ST_SPHERE), pick(ST_CYLINDER));
+        cr.swap = false;
+    } else if ((ta == ST_CYLINDER && tb == ST_CYLINDER) || is(ST_BOX, ST_CONVEX) || (ta == ST_CONVEX && tb == ST_CONVEX)) {
+        // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
+        convex_convex(w, cr, part(A), part(B), A.xf, B.xf, ref_scale);
+    } else if (is(ST_SPHERE, ST_CONVEX)) {         // sphere_convex_collision_algorithm.cpp:11-34
+        cr.swap = ta == ST_CONVEX;
+        sphere_mesh(w, cr, pick(ST_SPHERE), pick(ST_CONVEX), *pick(ST_CONVEX).poly);
+        cr.swap = false;
+    } else if (is(ST_CYLINDER, ST_CONVEX)) {       // cylinder_convex_collision_algorithm.cpp:11-51
+        cr.swap = ta == ST_CONVEX;
+        cylinder_mesh(w, cr, pick(ST_CONVEX), *pick(ST_CONVEX).poly, pick(ST_CYLINDER));
+        cr.swap = false;
+    } else if ((ta == ST_CONCAVE) != (tb == ST_CONCAVE)) {
+        // concave_{box,sphere,cylinder,convex}_collision_algorithm.cpp: the other body is A
+        const Body<R>& mb = pick(ST_CONCAVE);
+        const Body<R>& ob = other(ST_CONCAVE);
+        cr.swap = ta == ST_CONCAVE;
+        if (ob.type == ST_SPHERE) sphere_mesh(w, cr, ob, mb, *mb.poly);
+        else if (ob.type == ST_CYLINDER) cylinder_mesh(w, cr, mb, *mb.poly, ob);
+        else concave_poly(w, cr, mb, *mb.poly, part(ob), ob, ref_scale);
+        cr.swap = false;
+    }
+    // concave-concave: no algorithm, the result stays cleared
+}
+
 // SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41)
 template <class R> inline void stage_narrowphase(World<R>& w) {
     const size_t np = w.pairs.size();
@@ -958,48 +1076,9 @@ template <class R> inline void stage_narrowphase(World<R>& w) {
         cr.a = ia; cr.b = ib;
         cr.mu = std::sqrt(A.mu * B.mu);
         cr.e = std::sqrt(A.e * B.e);
-        const int ta = A.type, tb = B.type;
         const R ref_scale = (body_aabb_scale(A) + body_aabb_scale(B)) * R(0.05);
-        // NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) and each algorithm's
-        // processCollision: which body plays which role, and the swap flag it sets.
-        auto is = [&](int x, int y) { return (ta == x && tb == y) || (ta == y && tb == x); };
-        auto pick = [&](int t) -> const Body<R>& { return ta == t ? A : B; };
-        auto other = [&](int t) -> const Body<R>& { return ta == t ? B : A; };
-        auto part = [](const Body<R>& b) { return Part<R>{b.type == ST_CONCAVE ? ST_CONVEX : b.type, &b, b.poly}; };
-        if (ta == ST_BOX && tb == ST_BOX) box_box(w, cr, A, B);
-        else if (is(ST_BOX, ST_SPHERE)) box_sphere(w, cr, A, B);
-        else if (ta == ST_SPHERE && tb == ST_SPHERE) sphere_sphere(w, cr, A, B);
-        else if (is(ST_BOX, ST_CYLINDER)) {            // box_cylinder_collision_algorithm.cpp:13-47
-            cr.swap = ta == ST_BOX;
-            box_cylinder(w, cr, pick(ST_BOX), pick(ST_CYLINDER));
-            cr.swap = false;
-        } else if (is(ST_SPHERE, ST_CYLINDER)) {       // sphere_cylinder_collision_algorithm.cpp:8-30
-            cr.swap = ta == ST_SPHERE;
-            sphere_cylinder(w, cr, pick(ST_SPHERE), pick(ST_CYLINDER));
-            cr.swap = false;
-        } else if ((ta == ST_CYLINDER && tb == ST_CYLINDER) || is(ST_BOX, ST_CONVEX) || (ta == ST_CONVEX && tb == ST_CONVEX)) {
-            // cylinder_cylinder / box_convex / convex_convex: the SAT routine, no swap
-            convex_convex(w, cr, part(A), part(B), A.xf, B.xf, ref_scale);
-        } else if (is(ST_SPHERE, ST_CONVEX)) {         // sphere_convex_collision_algorithm.cpp:11-34
-            cr.swap = ta == ST_CONVEX;
-            sphere_mesh(w, cr, pick(ST_SPHERE), pick(ST_CONVEX), *pick(ST_CONVEX).poly);
-            cr.swap = false;
-        } else if (is(ST_CYLINDER, ST_CONVEX)) {       // cylinder_convex_collision_algorithm.cpp:11-51
-            cr.swap = ta == ST_CONVEX;
-            cylinder_mesh(w, cr, pick(ST_CONVEX), *pick(ST_CONVEX).poly, pick(ST_CYLINDER));
-            cr.swap = false;
-        } else if ((ta == ST_CONCAVE) != (tb == ST_CONCAVE)) {
-            // concave_{box,sphere,cylinder,convex}_collision_algorithm.cpp: the other body is A
-            const Body<R>& mb = pick(ST_CONCAVE);
-            const Body<R>& ob = other(ST_CONCAVE);
-            cr.swap = ta == ST_CONCAVE;
-            if (ob.type == ST_SPHERE) sphere_mesh(w, cr, ob, mb, *mb.poly);
-            else if (ob.type == ST_CYLINDER) cylinder_mesh(w, cr, mb, *mb.poly, ob);
-            else concave_poly(w, cr, mb, *mb.poly, part(ob), ob, ref_scale);
-            cr.swap = false;
-        } else {
-            continue;  // concave-concave: no algorithm, the result stays cleared
-        }
+        if (A.type == ST_CONCAVE && B.type == ST_CONCAVE) continue;
+        dispatch_pair(w, cr, A, B, ref_scale);
         sort_contact_points(cr);
     }
     count_neighbours(w);
@@ -1607,7 +1686,20 @@ template <class R> inline RayHit<R> raycast(const World<R>& w, const V3<R>& star
                 res.hit = false;
                 return res;
             }
-            ret = ray_mesh(start, dir, length, b, d, hp, hn);
+            if (b.type != ST_COMPOUND) {
+                ret = ray_mesh(start, dir, length, b, d, hp, hn);
+            } else {
+                // each child in turn writes the same outputs, its hit or-ed into ret: the last child's
+                // distance and point decide (default_raycast_solver.cpp:50-72)
+                for (const Body<R>& kid : b.kids) {
+                    Body<R> t = kid;
+                    t.xf = xf_mul(b.xf, kid.xf);
+                    if (kid.type == ST_BOX) ret |= ray_box(start, dir, length, t, d, hp, hn);
+                    else if (kid.type == ST_SPHERE) ret |= ray_sphere(start, dir, length, t, d, hp, hn);
+                    else if (kid.type == ST_CYLINDER) ret |= ray_cylinder(start, dir, length, t, d, hp, hn);
+                    else if (kid.type == ST_CONVEX) ret |= ray_mesh(start, dir, length, t, d, hp, hn);
+                }
+            }
         }
         if (ret && d <= length && d < res.distance) {
             res.distance = d;

@@ -12,6 +12,7 @@
 #include "rigid/shape/cylinder_shape.h"
 #include "rigid/shape/convex_mesh_shape.h"
 #include "rigid/shape/concave_mesh_shape.h"
+#include "rigid/shape/compound_shape.h"
 #include "rigid/collision/broad_phase/broad_phase_sweep_and_prune.h"
 #include "rigid/constraint/constraint_solver/sequential_impulse_solver.h"
 #include "rigid/constraint/constraint/ball_joint_constraint.h"
@@ -200,6 +201,32 @@ REF_API void* pref_world_create(const pe_scene_desc* sc, const pe_world_params* 
                     concave[m] = cs;
                 }
                 shape = concave[m];
+                break;
+            }
+            case PE_SHAPE_COMPOUND: {
+                // CompoundShape::addShape for each child, then init (as the reference's compound demo)
+                auto* cs = new pe_phys_shape::CompoundShape();
+                for (int k = sc->compound_start[i]; k < sc->compound_start[i + 1]; k++) {
+                    const pe_compound_child& ch = sc->compound_children[k];
+                    pe_phys_shape::Shape* kid = nullptr;
+                    const double* cp = ch.shape_param;
+                    if (ch.shape_type == PE_SHAPE_BOX) kid = new pe_phys_shape::BoxShape(pe::Vector3(pe::Real(cp[0]), pe::Real(cp[1]), pe::Real(cp[2])));
+                    else if (ch.shape_type == PE_SHAPE_SPHERE) kid = new pe_phys_shape::SphereShape(pe::Real(cp[0]));
+                    else if (ch.shape_type == PE_SHAPE_CYLINDER) kid = new pe_phys_shape::CylinderShape(pe::Real(cp[0]), pe::Real(cp[1]));
+                    else {
+                        const int m = ch.mesh_index;
+                        if (!convex[m]) {
+                            auto* ms = new pe_phys_shape::ConvexMeshShape();
+                            ms->setMesh(mesh_of(sc->meshes[m]));
+                            convex[m] = ms;
+                        }
+                        kid = convex[m];
+                    }
+                    const pe::Vector3 lo(pe::Real(ch.position[0]), pe::Real(ch.position[1]), pe::Real(ch.position[2]));
+                    cs->addShape(pe::Transform(basis_of(ch.rotation), lo), pe::Real(ch.mass_ratio), kid);
+                }
+                cs->init();
+                shape = cs;
                 break;
             }
             default: delete rb; delete w; return nullptr;

@@ -1071,10 +1071,15 @@ int pe_batch_world_footprint(const pe_scene_desc* scene, int64_t* bytes) {
 }
 
 namespace {
-// Batched and served worlds step with the reference's default (zero) sleep thresholds and bodies
-// that live forever: a finite lifetime needs the single-world path.
+// Batched and served worlds step with the reference's default (zero) sleep thresholds, bodies that
+// live forever and no compound shapes: those need the single-world path.
 int check_batched_scene(const pe_scene_desc* sc) {
     if (int rc = check_scene(sc)) return rc;
+    for (int i = 0; i < sc->n; i++)
+        if (sc->shape_type[i] == PE_SHAPE_COMPOUND) {
+            pe_set_error("body " + std::to_string(i) + " is a compound: compounds need a pe_world");
+            return PE_ERR_UNSUPPORTED;
+        }
     if (sc->lifetime)
         for (int i = 0; i < sc->n; i++)
             if (!(sc->lifetime[i] >= 1e30)) {

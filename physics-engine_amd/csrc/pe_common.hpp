@@ -65,7 +65,13 @@ struct MeshSet {
     const int32_t* rec;   // [MRW per mesh]
     const float4* arena;
     const float* cyl;     // unit cylinder: top ring x[20] z[20], bottom ring x[20] z[20], side normals [40][3]
+    const float4* kids = nullptr;    // compound children, CHW / 4 float4 each (a single world's compounds)
+    const int2* krange = nullptr;    // per compound: first child, child count
 };
+// Compound child record (CompoundShape::ShapeEle after CompoundShape::init): shape type (int bits),
+// shape parameters (a convex child's mesh id as int bits in the first), local basis (row-major), local
+// origin (moved by the children's centroid).
+enum ChildField : int { CH_TYPE = 0, CH_SP = 1, CH_R = 4, CH_P = 13, CHW = 16 };
 // Mesh record: arena offsets (in float4 units) and counts.
 //   vertices: float4 xyz;  face normals: float4 xyz;  faces: int2 (first index, count) per face;
 //   face indices: int32, relative to the mesh's first vertex;  unique edges: 2 float4 (ends);

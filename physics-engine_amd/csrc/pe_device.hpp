@@ -18,7 +18,7 @@ constexpr float REAL_MIN = -1e30f;
 constexpr float EPS = 0.00001f;      // PE_EPS
 constexpr float PI_R = 3.141592653589f;
 
-enum : int { SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2, SHAPE_CONVEX = 3, SHAPE_CONCAVE = 4 };
+enum : int { SHAPE_BOX = 0, SHAPE_SPHERE = 1, SHAPE_CYLINDER = 2, SHAPE_CONVEX = 3, SHAPE_CONCAVE = 4, SHAPE_COMPOUND = 5 };
 
 struct f3 { float x, y, z; };
 PE_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
@@ -195,6 +195,9 @@ struct Pose {
 PE_HD f3 xf_apply(const Pose& P, f3 v) { return add(mv(P.R, v), P.p); }
 PE_HD f3 xf_inv(const Pose& P, f3 v) { return mtv(P.R, sub(v, P.p)); }
 
+// The contact result's objects (their poses turn contact points into local positions) and the
+// result's swap flag; the shapes' own poses are passed to each algorithm (a compound's children are
+// posed apart from their body).
 struct PairCtx {
     Pose A, B;
     bool swap;
@@ -423,11 +426,12 @@ PE_HD void cull_points2(int n, const RArr<8>& px, const RArr<8>& py, int i0, int
     }
 }
 
-PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
+// PA, PB: the two boxes' poses (the bodies', or compound children's); x carries the result's objects.
+PE_HD void box_box(Cache& c, PairCtx& x, const Pose& PA, const Pose& PB, const float sa[3], const float sb[3]) {
     const float fudge = 1.05f;
-    const f3 p1 = x.A.p, p2 = x.B.p;
-    const m33& R1 = x.A.R;
-    const m33& R2 = x.B.R;
+    const f3 p1 = PA.p, p2 = PB.p;
+    const m33& R1 = PA.R;
+    const m33& R2 = PB.R;
     const f3 p = sub(p2, p1);
     const f3 pp = mtv(R1, p);
     const float A[3] = {sa[0] * 0.5f, sa[1] * 0.5f, sa[2] * 0.5f};
@@ -626,9 +630,10 @@ PE_HD void box_box(Cache& c, PairCtx& x, const float sa[3], const float sb[3]) {
 // ------------------------------------------------------------------ sphere pairs
 // BoxSphereCollisionAlgorithm (box_sphere_collision_algorithm.cpp:6-140).  The +y face of the
 // inside case uses half_extent.x (:51), as in the reference.
-PE_HD void box_sphere(Cache& c, PairCtx& x, bool a_is_box, const float box_size[3], float radius) {
-    const Pose& box = a_is_box ? x.A : x.B;
-    const Pose& sph = a_is_box ? x.B : x.A;
+PE_HD void box_sphere(Cache& c, PairCtx& x, bool a_is_box, const Pose& PA, const Pose& PB, const float box_size[3],
+                      float radius) {
+    const Pose& box = a_is_box ? PA : PB;
+    const Pose& sph = a_is_box ? PB : PA;
     x.swap = a_is_box;
     const f3 he{box_size[0] / 2.0f, box_size[1] / 2.0f, box_size[2] / 2.0f};
     const f3 ps = xf_inv(box, sph.p);
@@ -667,12 +672,12 @@ PE_HD void box_sphere(Cache& c, PairCtx& x, bool a_is_box, const float box_size[
 }
 
 // SphereSphereCollisionAlgorithm (sphere_sphere_collision_algorithm.cpp:23-37)
-PE_HD void sphere_sphere(Cache& c, PairCtx& x, float ra, float rb) {
-    const f3 rel = sub(x.A.p, x.B.p);
+PE_HD void sphere_sphere(Cache& c, PairCtx& x, const Pose& PA, const Pose& PB, float ra, float rb) {
+    const f3 rel = sub(PA.p, PB.p);
     const float dist = norm(rel);
     if (dist > ra + rb || fabsf(dist - 0.0f) < EPS) return;
     const f3 normal = divs(rel, dist);
-    const f3 pt = add(x.B.p, mul(normal, rb));
+    const f3 pt = add(PB.p, mul(normal, rb));
     const float depth = dist - ra - rb;
     add_point(c, x, normal, sub(pt, mul(normal, 0.0f)), depth + 2 * 0.0f);
 }

@@ -318,9 +318,27 @@ int check_scene(const pe_scene_desc* sc) {
     if (!sc || sc->n <= 0) { pe_set_error("empty scene"); return PE_ERR_INVALID; }
     for (int i = 0; i < sc->n; i++) {
         const int t = sc->shape_type[i];
-        if (t < PE_SHAPE_BOX || t > PE_SHAPE_CONCAVE) {
+        if (t < PE_SHAPE_BOX || t > PE_SHAPE_COMPOUND) {
             pe_set_error("body " + std::to_string(i) + ": unknown shape type " + std::to_string(t));
             return PE_ERR_UNSUPPORTED;
+        }
+        if (t == PE_SHAPE_COMPOUND) {
+            // CompoundShape::addShape (compound_shape.cpp:7-20): no compound or concave children, mass
+            // ratios above PE_EPS
+            if (!sc->compound_start || !sc->compound_children || sc->compound_start[i + 1] <= sc->compound_start[i]) {
+                pe_set_error("body " + std::to_string(i) + ": compound body without children");
+                return PE_ERR_INVALID;
+            }
+            for (int k = sc->compound_start[i]; k < sc->compound_start[i + 1]; k++) {
+                const pe_compound_child& ch = sc->compound_children[k];
+                const bool mesh_ok = ch.shape_type != PE_SHAPE_CONVEX ||
+                                     (sc->meshes && ch.mesh_index >= 0 && ch.mesh_index < sc->n_meshes);
+                if (ch.shape_type < PE_SHAPE_BOX || ch.shape_type > PE_SHAPE_CONVEX || !mesh_ok || !(ch.mass_ratio >= 1e-5)) {
+                    pe_set_error("body " + std::to_string(i) + ": compound child " + std::to_string(k) +
+                                 " is not a box, sphere, cylinder or convex mesh with a mass ratio above 1e-5");
+                    return PE_ERR_INVALID;
+                }
+            }
         }
         if (t == PE_SHAPE_CONVEX || t == PE_SHAPE_CONCAVE) {
             if (!sc->mesh_index || !sc->meshes || sc->mesh_index[i] < 0 || sc->mesh_index[i] >= sc->n_meshes) {
@@ -335,6 +353,90 @@ int check_scene(const pe_scene_desc* sc) {
     }
     return PE_OK;
 }
+
+namespace {
+// Unit-mass inertia of a box / sphere / cylinder (box_shape.cpp:16-22, sphere_shape.cpp:5-12,
+// cylinder_shape.cpp:23-31), diagonal.
+void primitive_inertia(int t, const float sp[3], float I[9]) {
+    for (int k = 0; k < 9; k++) I[k] = 0.0f;
+    if (t == PE_SHAPE_BOX) {
+        const float x2 = sp[0] * sp[0], y2 = sp[1] * sp[1], z2 = sp[2] * sp[2];
+        I[0] = (y2 + z2) / 12.0f; I[4] = (x2 + z2) / 12.0f; I[8] = (x2 + y2) / 12.0f;
+    } else if (t == PE_SHAPE_SPHERE) {
+        I[0] = I[4] = I[8] = (float)(2.0 / 5.0) * sp[0] * sp[0];
+    } else {
+        const float r_2 = sp[0] * sp[0], h_2 = sp[1] * sp[1];
+        I[4] = r_2 * 0.5f;
+        I[0] = I[8] = r_2 / 4.0f + h_2 / 12.0f;
+    }
+}
+float sum3h(float a, float b, float c) { return a + (b + c); }   // Eigen's FP32 3-term sum
+
+// CompoundShape::addShape for each child of body i, then CompoundShape::init (compound_shape.cpp:
+// 24-47) in FP32 and the reference's evaluation order: the children's origins move by minus their
+// mass-ratio-weighted centroid, and the unit-mass inertia is the sum over the children of
+// ((R I_child) R^T + |p|^2 1 - p p^T) ratio / total.  Appends the children to the compound table
+// and returns the compound id (or -1).
+int stage_compound(const pe_scene_desc* sc, int i, HostMeshes& meshes, std::map<std::pair<int, int>, int>& ids,
+                   float I[9]) {
+    const int k0 = sc->compound_start[i], k1 = sc->compound_start[i + 1];
+    float total = 0.0f;
+    for (int k = k0; k < k1; k++) total += (float)sc->compound_children[k].mass_ratio;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (int k = k0; k < k1; k++) {
+        const pe_compound_child& ch = sc->compound_children[k];
+        const float r = (float)ch.mass_ratio;
+        for (int q = 0; q < 3; q++) c[q] = c[q] + (float)ch.position[q] * r;
+    }
+    for (int q = 0; q < 3; q++) c[q] = c[q] / total;
+    for (int q = 0; q < 9; q++) I[q] = 0.0f;
+    const int first = (int)meshes.kids.size() / (CHW / 4);
+    for (int k = k0; k < k1; k++) {
+        const pe_compound_child& ch = sc->compound_children[k];
+        float rec[CHW] = {};
+        float sp[3] = {(float)ch.shape_param[0], (float)ch.shape_param[1], (float)ch.shape_param[2]};
+        float si[9];
+        if (ch.shape_type == PE_SHAPE_CONVEX) {
+            const auto key = std::make_pair(ch.mesh_index, 0);
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                const int id = meshes.add(sc->meshes[ch.mesh_index], false);
+                if (id < 0) return -1;
+                it = ids.emplace(key, id).first;
+            }
+            const int id = it->second;
+            for (int q = 0; q < 9; q++) si[q] = meshes.props[10 * id + 1 + q];
+            int32_t bits = id;
+            std::memcpy(&sp[0], &bits, 4);
+            sp[1] = sp[2] = 0.0f;
+        } else {
+            primitive_inertia(ch.shape_type, sp, si);
+        }
+        float R[9], p[3];
+        for (int q = 0; q < 9; q++) R[q] = (float)ch.rotation[q];
+        for (int q = 0; q < 3; q++) p[q] = (float)ch.position[q] - c[q];
+        const float pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        float T[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) T[3 * r + q] = sum3h(R[3 * r] * si[q], R[3 * r + 1] * si[3 + q], R[3 * r + 2] * si[6 + q]);
+        const float w = (float)ch.mass_ratio / total;
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                const float a = sum3h(T[3 * r] * R[3 * q], T[3 * r + 1] * R[3 * q + 1], T[3 * r + 2] * R[3 * q + 2]);
+                const float t = (r == q ? 1.0f : 0.0f) * pp - p[r] * p[q];
+                I[3 * r + q] += (a + t) * w;
+            }
+        int32_t tb = ch.shape_type;
+        std::memcpy(&rec[CH_TYPE], &tb, 4);
+        for (int q = 0; q < 3; q++) rec[CH_SP + q] = sp[q];
+        for (int q = 0; q < 9; q++) rec[CH_R + q] = R[q];
+        for (int q = 0; q < 3; q++) rec[CH_P + q] = p[q];
+        for (int q = 0; q < CHW / 4; q++) meshes.kids.push_back(make_float4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]));
+    }
+    meshes.krange.push_back(make_int2(first, k1 - k0));
+    return (int)meshes.krange.size() - 1;
+}
+}  // namespace
 
 int stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<int32_t>& hi, size_t stride, int off,
                  HostMeshes& meshes) {
@@ -371,6 +473,12 @@ int stage_bodies(const pe_scene_desc* sc, std::vector<float>& hf, std::vector<in
             const float r_2 = r * r, h_2 = h * h;
             I[4] = r_2 * 0.5f;
             I[0] = I[8] = r_2 / 4.0f + h_2 / 12.0f;
+        } else if (t == PE_SHAPE_COMPOUND) {
+            const int id = stage_compound(sc, i, meshes, ids, I);
+            if (id < 0) return PE_ERR_INVALID;
+            int32_t bits = id;
+            std::memcpy(&F(F_SP), &bits, 4);
+            F(F_SP + 1) = F(F_SP + 2) = 0.0f;
         } else {
             const bool concave = t == PE_SHAPE_CONCAVE;
             const auto key = std::make_pair(sc->mesh_index[i], concave ? 1 : 0);
@@ -414,7 +522,20 @@ int DevMeshes::upload(const HostMeshes& h, hipStream_t s) {
     }
     if ((!h.rec.empty() && hipMemcpyAsync(rec, h.rec.data(), h.rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) ||
         (!h.arena.empty() && hipMemcpyAsync(arena, h.arena.data(), h.arena.size() * sizeof(float4), hipMemcpyHostToDevice, s) != hipSuccess) ||
-        hipMemcpyAsync(cyl, unit, sizeof(unit), hipMemcpyHostToDevice, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        hipMemcpyAsync(cyl, unit, sizeof(unit), hipMemcpyHostToDevice, s) != hipSuccess) {
+        pe_set_error("mesh arena upload failed");
+        return PE_ERR_DEVICE;
+    }
+    if (!h.krange.empty()) {
+        if (hipMalloc(&kids, h.kids.size() * sizeof(float4)) != hipSuccess ||
+            hipMalloc(&krange, h.krange.size() * sizeof(int2)) != hipSuccess ||
+            hipMemcpyAsync(kids, h.kids.data(), h.kids.size() * sizeof(float4), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(krange, h.krange.data(), h.krange.size() * sizeof(int2), hipMemcpyHostToDevice, s) != hipSuccess) {
+            pe_set_error("compound table upload failed");
+            return PE_ERR_DEVICE;
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
         pe_set_error("mesh arena upload failed");
         return PE_ERR_DEVICE;
     }
@@ -425,9 +546,13 @@ void DevMeshes::release() {
     if (rec) hipFree(rec);
     if (arena) hipFree(arena);
     if (cyl) hipFree(cyl);
+    if (kids) hipFree(kids);
+    if (krange) hipFree(krange);
     rec = nullptr;
     arena = nullptr;
     cyl = nullptr;
+    kids = nullptr;
+    krange = nullptr;
 }
 
 }  // namespace pe

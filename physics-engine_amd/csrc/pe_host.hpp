@@ -17,6 +17,8 @@ struct HostMeshes {
     std::vector<int32_t> rec;      // [MRW per mesh]
     std::vector<float4> arena;
     std::vector<float> props;      // [10 per mesh] volume, unit-mass inertia (row-major)
+    std::vector<float4> kids;      // compound children (CHW floats each)
+    std::vector<int2> krange;      // per compound: first child, count
     int count() const { return (int)rec.size() / MRW; }
     // ConvexMeshShape::setMesh (concave = false) or ConcaveMeshShape::setMesh (concave = true)
     // of a mesh; returns its mesh id, or -1 (with pe_last_error set) if the mesh is malformed.
@@ -50,9 +52,11 @@ struct DevMeshes {
     int32_t* rec = nullptr;
     float4* arena = nullptr;
     float* cyl = nullptr;
+    float4* kids = nullptr;
+    int2* krange = nullptr;
     int upload(const HostMeshes& h, hipStream_t s);
     void release();
-    MeshSet view() const { return MeshSet{rec, arena, cyl}; }
+    MeshSet view() const { return MeshSet{rec, arena, cyl, kids, krange}; }
 };
 
 }  // namespace pe

@@ -201,6 +201,8 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
         // a kinematic body's pose does not change in a step, so neither does its mesh AABB
         mn = ld3(b, F_AMIN, i);
         mx = ld3(b, F_AMAX, i);
+    } else if (type == SHAPE_COMPOUND) {
+        compound_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     } else {
         mesh_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     }
@@ -218,6 +220,7 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
     f3 mn, mx;
     if (type <= SHAPE_CYLINDER) shape_aabb(type, sp, R, pos, mn, mx);
+    else if (type == SHAPE_COMPOUND) compound_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     else mesh_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
@@ -348,6 +351,70 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
     x.swap = false;
 }
 
+// A compound's child k as a narrowphase participant: the child's shape at its world pose; the
+// body's material, kinematic flag and AABB scale.
+__device__ __forceinline__ CollBody child_coll(const MeshSet& ms, const CollBody& body, int k) {
+    CollBody c = body;
+    float sp[3];
+    m33 Rc;
+    f3 pc;
+    c.type = child_pose(ms, k, body.P.R, body.P.p, sp, Rc, pc);
+    c.P.R = Rc;
+    c.P.p = pc;
+    c.sp[0] = sp[0]; c.sp[1] = sp[1]; c.sp[2] = sp[2];
+    return c;
+}
+// NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) and processCollision for two shapes
+// of which neither is a compound, every algorithm on one thread (compound children: rare, so the
+// separating-axis routine runs here on a lane of its own rather than a wave).
+static __device__ __noinline__ void child_dispatch(const MeshSet& ms, Cache& c, PairCtx& x, const CollBody& A,
+                                                   const CollBody& B, float ref_scale) {
+    const int ta = A.type, tb = B.type;
+    const int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+    if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.P, B.P, A.sp, B.sp);
+    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.P, B.P, A.sp, B.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, A.P, B.P, B.sp, A.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.P, B.P, A.sp[0], B.sp[0]);
+    else if ((lo == SHAPE_CYLINDER && hi == SHAPE_CYLINDER) || (lo == SHAPE_BOX && hi == SHAPE_CONVEX) ||
+             (lo == SHAPE_CONVEX && hi == SHAPE_CONVEX)) {
+        convex_convex<1>(c, x, body_poly(ms, A), body_poly(ms, B), A.P, B.P, ref_scale);
+    } else if ((lo == SHAPE_BOX || lo == SHAPE_CONVEX) && hi == SHAPE_CONCAVE) {
+        // concave_convex_collision_algorithm.cpp: the convex shape is A
+        const CollBody& mb = ta == SHAPE_CONCAVE ? A : B;
+        const CollBody& ob = ta == SHAPE_CONCAVE ? B : A;
+        x.swap = ta == SHAPE_CONCAVE;
+        concave_poly<1>(c, x, ms, __float_as_int(mb.sp[0]), mb.P, body_poly(ms, ob), ob.P, ref_scale);
+        x.swap = false;
+    } else {
+        narrow_shapes(ms, c, x, A, B);
+    }
+}
+// CompoundCompoundCollisionAlgorithm::processCollision (compound_compound_collision_algorithm.cpp:
+// 8-48): a compound B trades places with A, the result's objects too (returns true then); each child
+// of A, posed by A's transform, meets B or each child of B.
+static __device__ __noinline__ bool narrow_compound(const MeshSet& ms, Cache& c, PairCtx& x, const CollBody& A0,
+                                                    const CollBody& B0, float ref_scale) {
+    const bool sw = B0.type == SHAPE_COMPOUND;
+    const CollBody& A = sw ? B0 : A0;
+    const CollBody& B = sw ? A0 : B0;
+    if (sw) {
+        const Pose t = x.A;
+        x.A = x.B;
+        x.B = t;
+    }
+    const int2 ka = ms.krange[__float_as_int(A.sp[0])];
+    for (int k = 0; k < ka.y; k++) {
+        const CollBody ca = child_coll(ms, A, ka.x + k);
+        if (B.type == SHAPE_COMPOUND) {
+            const int2 kb = ms.krange[__float_as_int(B.sp[0])];
+            for (int q = 0; q < kb.y; q++) child_dispatch(ms, c, x, ca, child_coll(ms, B, kb.x + q), ref_scale);
+        } else {
+            child_dispatch(ms, c, x, ca, B, ref_scale);
+        }
+    }
+    return sw;
+}
+
 // SimpleNarrowPhase::calcContactResults (simple_narrow_phase.cpp:7-41) for pair slot p.
 // Slots that existed in the previous step (p < prev_np) keep their distances, because the
 // reference reuses its ContactResult objects and clearContactPoints only invalidates
@@ -356,7 +423,8 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
 // code and keeps its register budget); SHAPES = true handles every pair type.
 // G = 64 (a whole wave on one pair, every lane holding the same values) is for the pairs
 // pair_is_sat selects: the separating-axis loops are split over the lanes and lane 0 writes.
-template <bool SHAPES, int G = 1>
+// COMPOUND = true is for the pairs with a compound body (one thread each).
+template <bool SHAPES, int G = 1, bool COMPOUND = false>
 static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
     const int ia = ps.pa[p], ib = ps.pb[p];
     // The shape kernels keep the contact cache in LDS: their algorithms take it by reference across
@@ -390,7 +458,10 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
         x.same_th2 = th * th;
     }
     const int ta = A.type, tb = B.type;
-    if (G > 1) {
+    bool swapped = false;
+    if (COMPOUND) {
+        swapped = narrow_compound(b.ms, c, x, A, B, (x.A.scale + x.B.scale) * 0.05f);
+    } else if (G > 1) {
         const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
         if (ta == SHAPE_CONCAVE || tb == SHAPE_CONCAVE) {
             // concave_convex_collision_algorithm.cpp: the convex body is A (narrow_shapes)
@@ -402,15 +473,19 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
         } else {
             convex_convex<G>(c, x, body_poly(b.ms, A), body_poly(b.ms, B), A.P, B.P, ref_scale);
         }
-    } else if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.sp, B.sp);
-    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.sp, B.sp[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, B.sp, A.sp[0]);
-    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.sp[0], B.sp[0]);
+    } else if (ta == SHAPE_BOX && tb == SHAPE_BOX) box_box(c, x, A.P, B.P, A.sp, B.sp);
+    else if (ta == SHAPE_BOX && tb == SHAPE_SPHERE) box_sphere(c, x, true, A.P, B.P, A.sp, B.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_BOX) box_sphere(c, x, false, A.P, B.P, B.sp, A.sp[0]);
+    else if (ta == SHAPE_SPHERE && tb == SHAPE_SPHERE) sphere_sphere(c, x, A.P, B.P, A.sp[0], B.sp[0]);
     else if (SHAPES) narrow_shapes(b.ms, c, x, A, B);
     int rank[8];
     const int np = sort_rank(c, rank);
     if (G > 1 && Grp<G>::rank() != 0) return;
     ps.npts[p] = np;
+    if (COMPOUND && swapped) {   // the result's objects: the compound first
+        ps.pa[p] = ib;
+        ps.pb[p] = ia;
+    }
     // bit 0: the swap flag; bit 1: contact points were lost to a full clip buffer (a capacity error)
     ps.swap[p] = (x.swap ? 1 : 0) | (c.trunc ? 2 : 0);
     float dsorted[8];

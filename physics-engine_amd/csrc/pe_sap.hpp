@@ -27,7 +27,8 @@ struct DevCounts {          // device-side counters of one step
     uint32_t bar_gen;       // grid barrier: generation
     int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh ...
     int32_t nsat;           // ... of which these run the polyhedral separating-axis routine
-    int32_t pad[2];
+    int32_t ncomp;          // ... and these involve a compound body
+    int32_t pad;
 };
 // Capacity errors, OR-ed into DevCounts::overflow by the step that hits them and kept until the
 // host reads and reports them: a result that lost pairs or contact points is never returned as a

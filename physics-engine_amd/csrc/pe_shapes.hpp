@@ -153,6 +153,40 @@ PE_HD Pose pose_mul(const Pose& a, const Pose& b) {
 
 // ------------------------------------------------------------------ shape queries
 // ConvexMeshShape::getAABB (convex_mesh_shape.cpp:138-150)
+__device__ inline void mesh_aabb(const MeshSet& ms, int mesh, const m33& R, f3 pos, f3& mn, f3& mx);
+
+// Compound child k of the table, posed by its body's basis R and origin o (Transform::operator*,
+// transform.cpp:7-9): type, shape parameters, world basis and origin.
+__device__ __forceinline__ int child_pose(const MeshSet& ms, int k, const m33& R, f3 o, float sp[3], m33& Rc, f3& pc) {
+    const float4* r = ms.kids + (size_t)k * (CHW / 4);
+    float c[CHW];
+#pragma unroll
+    for (int q = 0; q < CHW / 4; q++) { const float4 v = r[q]; c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w; }
+    m33 L;
+#pragma unroll
+    for (int q = 0; q < 9; q++) L.m[q] = c[CH_R + q];
+    Rc = mm(R, L);
+    pc = add(mv(R, f3{c[CH_P], c[CH_P + 1], c[CH_P + 2]}), o);
+    sp[0] = c[CH_SP]; sp[1] = c[CH_SP + 1]; sp[2] = c[CH_SP + 2];
+    return __float_as_int(c[CH_TYPE]);
+}
+// CompoundShape::getAABB (compound_shape.cpp:49-58): the union of the children's AABBs.
+__device__ inline void compound_aabb(const MeshSet& ms, int cid, const m33& R, f3 pos, f3& mn, f3& mx) {
+    const int2 kr = ms.krange[cid];
+    mn = f3{REAL_MAX, REAL_MAX, REAL_MAX};
+    mx = f3{REAL_MIN, REAL_MIN, REAL_MIN};
+    for (int k = 0; k < kr.y; k++) {
+        float sp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        m33 Rc;
+        f3 pc, a, b;
+        const int t = child_pose(ms, kr.x + k, R, pos, sp, Rc, pc);
+        if (t <= SHAPE_CYLINDER) shape_aabb(t, sp, Rc, pc, a, b);
+        else mesh_aabb(ms, __float_as_int(sp[0]), Rc, pc, a, b);
+        mn = f3{mn.x < a.x ? mn.x : a.x, mn.y < a.y ? mn.y : a.y, mn.z < a.z ? mn.z : a.z};
+        mx = f3{mx.x > b.x ? mx.x : b.x, mx.y > b.y ? mx.y : b.y, mx.z > b.z ? mx.z : b.z};
+    }
+}
+
 __device__ inline void mesh_aabb(const MeshSet& ms, int mesh, const m33& R, f3 pos, f3& mn, f3& mx) {
     const int32_t* r = ms.rec + MRW * mesh;
     const float4* vv = ms.arena + r[MR_V];

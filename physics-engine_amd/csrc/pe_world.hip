@@ -41,6 +41,7 @@ struct Dev {
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
     SapView sap;            // the broadphase over the body store's AABBs (pairs to ps.pa / ps.pb)
     int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes (from the front)
+    int32_t* comp_list;     // [pcap] pairs with a compound body (worlds with compounds; else null)
                             // and to k_narrow_sat (from the back)
     int32_t* order;         // [pcap]
     int32_t* act;           // [3*pcap] colouring work lists
@@ -136,7 +137,23 @@ __global__ void __launch_bounds__(64) k_raycast(Dev d, const float* rays, const 
         if (type == SHAPE_BOX) ret = ray_box(start, dir, len, sp, R, o, t, hp, hn);
         else if (type == SHAPE_SPHERE) ret = ray_sphere(start, dir, len, sp[0], o, t, hp, hn);
         else if (type == SHAPE_CYLINDER) ret = ray_cylinder(start, dir, len, sp[0], sp[1], R, o, t, hp, hn);
-        else ret = ray_mesh(d.b.ms, __float_as_int(sp[0]), start, dir, len, R, o, t, hp, hn);
+        else if (type != SHAPE_COMPOUND) ret = ray_mesh(d.b.ms, __float_as_int(sp[0]), start, dir, len, R, o, t, hp, hn);
+        else {
+            // each child in turn writes the same outputs, its hit or-ed in: the last child's distance and
+            // point decide (default_raycast_solver.cpp:50-78)
+            ret = false;
+            const int2 kr = d.b.ms.krange[__float_as_int(sp[0])];
+            for (int k = 0; k < kr.y; k++) {
+                float csp[3];
+                m33 Rc;
+                f3 pc;
+                const int ct = child_pose(d.b.ms, kr.x + k, R, o, csp, Rc, pc);
+                if (ct == SHAPE_BOX) ret |= ray_box(start, dir, len, csp, Rc, pc, t, hp, hn);
+                else if (ct == SHAPE_SPHERE) ret |= ray_sphere(start, dir, len, csp[0], pc, t, hp, hn);
+                else if (ct == SHAPE_CYLINDER) ret |= ray_cylinder(start, dir, len, csp[0], csp[1], Rc, pc, t, hp, hn);
+                else ret |= ray_mesh(d.b.ms, __float_as_int(csp[0]), start, dir, len, Rc, pc, t, hp, hn);
+            }
+        }
         if (ret && t <= len && t < REAL_MAX) {
             const uint64_t key = ((uint64_t)float_key(t + 0.0f) << 32) | (uint32_t)i;
             if (key < best) { best = key; bd = t; bp = hp; bn = hn; }
@@ -240,12 +257,18 @@ __global__ void __launch_bounds__(128) k_narrow(Dev d) {
     const int prev = d.cnt->prev_npairs;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         const bool simple = pair_is_simple(d.b, d.ps, p);
-        const bool sat = !simple && pair_is_sat(d.b, d.ps, p);
-        const int slot = wave_append(!simple && !sat, &d.cnt->nshape);
+        const bool comp = !simple && (d.b.iat(I_TYPE, d.ps.pa[p]) == SHAPE_COMPOUND ||
+                                      d.b.iat(I_TYPE, d.ps.pb[p]) == SHAPE_COMPOUND);
+        const bool sat = !simple && !comp && pair_is_sat(d.b, d.ps, p);
+        const int slot = wave_append(!simple && !sat && !comp, &d.cnt->nshape);
         const int sslot = wave_append(sat, &d.cnt->nsat);
+        const int cslot = wave_append(comp && d.comp_list, &d.cnt->ncomp);
         if (simple) narrow_pair<false>(d.b, d.ps, p, prev);
         else if (sat) d.shape_list[d.pcap - 1 - sslot] = p;
-        else d.shape_list[slot] = p;
+        else if (comp) {
+            if (d.comp_list) d.comp_list[cslot] = p;
+            else { d.ps.npts[p] = 0; d.ps.swap[p] = 0; }   // no compound table: no algorithm
+        } else d.shape_list[slot] = p;
     }
 }
 __global__ void __launch_bounds__(64) k_narrow_shapes(Dev d) {
@@ -262,10 +285,19 @@ __global__ void __launch_bounds__(256) k_narrow_sat(Dev d) {
     for (int q = blockIdx.x * (blockDim.x / WAVE) + (int)(threadIdx.x / WAVE); q < ns; q += nw)
         narrow_pair<true, WAVE>(d.b, d.ps, d.shape_list[d.pcap - 1 - q], prev);
 }
+// The pairs with a compound body, one thread each (64-thread workgroups: the contact cache and the
+// BVH stacks of the separating-axis and mesh routines live in LDS rows of one lane each).
+__global__ void __launch_bounds__(64) k_narrow_compound(Dev d) {
+    const int nc = d.cnt->ncomp;
+    const int prev = d.cnt->prev_npairs;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += gridDim.x * blockDim.x)
+        narrow_pair<true, 1, true>(d.b, d.ps, d.comp_list[q], prev);
+}
 __global__ void k_narrow_finish(Dev d) {
     d.cnt->prev_npairs = d.cnt->npairs;
     d.cnt->nshape = 0;
     d.cnt->nsat = 0;
+    d.cnt->ncomp = 0;
     d.cnt->npoints = 0;   // summed by k_count_points next
 }
 
@@ -444,6 +476,7 @@ struct pe_world {
             hipLaunchKernelGGL(k_narrow_shapes, dim3(grid(d.pcap, 64)), dim3(64), 0, stream, d);
             hipLaunchKernelGGL(k_narrow_sat, dim3(std::min(grid(d.pcap, 4), 4096)), dim3(256), 0, stream, d);
         }
+        if (d.comp_list) hipLaunchKernelGGL(k_narrow_compound, dim3(grid(d.pcap, 64)), dim3(64), 0, stream, d);
         hipLaunchKernelGGL(k_narrow_finish, dim3(1), dim3(1), 0, stream, d);
         hipLaunchKernelGGL(k_count_points, dim3(grid(d.pcap, 1024)), dim3(256), 0, stream, d);
         if (d.status_on) hipLaunchKernelGGL(k_neighbours, dim3(grid(d.pcap, 256)), dim3(256), 0, stream, d);
@@ -601,12 +634,15 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     if (int rc2 = w->meshes.upload(hm, w->stream)) return fail(rc2);
     d.b.ms = w->meshes.view();
     int ndyn = 0;
+    bool has_compound = false;
     w->kinematic.resize(n);
     for (int i = 0; i < n; i++) {
         ndyn += sc->kinematic[i] ? 0 : 1;
         w->kinematic[i] = sc->kinematic[i] ? 1 : 0;
         w->has_shapes |= sc->shape_type[i] > PE_SHAPE_SPHERE;
+        has_compound |= sc->shape_type[i] == PE_SHAPE_COMPOUND;
     }
+    if (has_compound && w->alloc(&d.comp_list, pcap) != PE_OK) return fail(PE_ERR_DEVICE);
     w->n_dynamic = ndyn;
     // RigidBody::_life_time (rigidbody.cpp:97): FP32 PE_REAL_MAX unless the scene sets one
     std::vector<float> life(n, 1e30f);

@@ -36,6 +36,8 @@ def _plain_scene(scene):
     if (sleep and (sleep[0] != 0.0 or sleep[1] != 0.0)) or (
             getattr(scene, "lifetime", None) is not None and bool(np.any(np.asarray(scene.lifetime) < 1e30))):
         raise ValueError(f"scene {scene.name!r}: sleeping and lifetimes need a World, not a batch")
+    if getattr(scene, "compounds", None):
+        raise ValueError(f"scene {scene.name!r}: compound bodies need a World, not a batch")
 
 
 class Batch:

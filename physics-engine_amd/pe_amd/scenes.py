@@ -24,6 +24,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SHAPE_BOX, SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_CONVEX, SHAPE_CONCAVE = 0, 1, 2, 3, 4
+SHAPE_COMPOUND = 5
 
 
 class MeshDesc(ctypes.Structure):
@@ -53,6 +54,25 @@ class Mesh:
                 np.ascontiguousarray(self.normals, np.float64))
 
 
+class CompoundChild(ctypes.Structure):
+    """ctypes mirror of pe_compound_child."""
+    _fields_ = [("shape_type", ctypes.c_int32), ("mesh_index", ctypes.c_int32), ("shape_param", ctypes.c_double * 3),
+                ("position", ctypes.c_double * 3), ("rotation", ctypes.c_double * 9), ("mass_ratio", ctypes.c_double)]
+
+
+def compound_child(shape, param, pos=(0.0, 0.0, 0.0), rot=None, ratio=1.0, mesh=-1) -> CompoundChild:
+    """CompoundShape::addShape's arguments: a child shape (SHAPE_BOX / SPHERE / CYLINDER / CONVEX with its
+    parameters or scene mesh) at a local transform, with a mass ratio."""
+    c = CompoundChild()
+    c.shape_type, c.mesh_index = int(shape), int(mesh)
+    p = list(param) + [0.0] * (3 - len(param))
+    c.shape_param[:] = [float(x) for x in p[:3]]
+    c.position[:] = [float(x) for x in pos]
+    c.rotation[:] = [float(x) for x in (np.eye(3) if rot is None else np.asarray(rot, float)).ravel()]
+    c.mass_ratio = float(ratio)
+    return c
+
+
 class SceneDesc(ctypes.Structure):
     """ctypes mirror of pe_scene_desc."""
     _fields_ = [
@@ -74,6 +94,8 @@ class SceneDesc(ctypes.Structure):
         ("mesh_index", ctypes.POINTER(ctypes.c_int32)),
         ("ignore_collision", ctypes.POINTER(ctypes.c_int32)),
         ("lifetime", ctypes.POINTER(ctypes.c_double)),
+        ("compound_start", ctypes.POINTER(ctypes.c_int32)),
+        ("compound_children", ctypes.POINTER(CompoundChild)),
     ]
 
 
@@ -154,6 +176,7 @@ class Scene:
     ignore_collision: np.ndarray = None                 # int32 [n] 0/1 (setIgnoreCollision), or None
     lifetime: np.ndarray = None                         # f64 [n] seconds (setLifeTime; >= 1e30 never), or None
     sleep: tuple = None                                 # (lin_vel2, ang_vel2, time) World::setSleep*Threshold, or None
+    compounds: dict = field(default_factory=dict)       # body -> [CompoundChild] of SHAPE_COMPOUND bodies
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -204,6 +227,16 @@ class Scene:
         if self.lifetime is not None:
             self.lifetime = np.ascontiguousarray(self.lifetime, dtype=np.float64)
             d.lifetime = _dptr(self.lifetime)
+        if self.compounds:
+            start = np.zeros(self.n + 1, np.int32)
+            kids = []
+            for i in range(self.n):
+                kids += list(self.compounds.get(i, []))
+                start[i + 1] = len(kids)
+            arr = (CompoundChild * max(len(kids), 1))(*kids)
+            self._keep += [start, arr]
+            d.compound_start = _iptr(start)
+            d.compound_children = arr
         return d
 
     def params(self) -> WorldParams:
@@ -220,7 +253,9 @@ class Scene:
                      self.dt, self.gravity, self.iterations, self.name,
                      list(self.meshes), self.mesh_index[idx],
                      ignore_collision=None if self.ignore_collision is None else self.ignore_collision[idx],
-                     lifetime=None if self.lifetime is None else self.lifetime[idx], sleep=self.sleep).contiguous()
+                     lifetime=None if self.lifetime is None else self.lifetime[idx], sleep=self.sleep,
+                     compounds={k: list(self.compounds[int(i)]) for k, i in enumerate(idx) if int(i) in self.compounds}
+                     ).contiguous()
 
     def footprint_bodies(self) -> int:
         return self.n
@@ -668,4 +703,65 @@ def joint_chain_scene(chains=48, links=8, seed=5, spacing=2.5) -> Scene:
     sc = b.scene(name=f"chains{chains}x{links}")
     sc.joints = joints
     sc.ignore_pairs = ignore
+    return sc
+
+
+def compound_demo_scene(count=10, sleep=True) -> Scene:
+    """The reference's compound demo (examples/compound_demo/main.cpp:10-86): a kinematic ground box,
+    a box, a sphere and a cylinder, and `count` compound bodies stacked 5 m apart -- a unit box with a
+    cylinder above and one below and four horizontal cylinders around it (mass ratios 0.3, 0.2, 0.1,
+    0.1 x 4) -- each placed at its stack position minus the children's centroid, as the demo does
+    with CompoundShape::init's result; friction 0.5, restitution 0.5, angular damping 0.8; the demo's
+    sleep thresholds (0.01, 0.01, 1 s)."""
+    pi = math.pi
+    kids = [compound_child(SHAPE_BOX, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), ratio=0.3),
+            compound_child(SHAPE_CYLINDER, (0.2, 2.0), (0.0, 1.5, 0.0), ratio=0.2),
+            compound_child(SHAPE_CYLINDER, (0.2, 1.0), (0.0, -1.0, 0.0), ratio=0.1),
+            compound_child(SHAPE_CYLINDER, (0.2, 1.0), (1.0, 0.0, 0.0), _axis_angle((0, 0, 1), pi / 2), 0.1),
+            compound_child(SHAPE_CYLINDER, (0.2, 1.0), (-1.0, 0.0, 0.0), _axis_angle((0, 0, 1), pi / 2), 0.1),
+            compound_child(SHAPE_CYLINDER, (0.2, 1.0), (0.0, 0.0, 1.0), _axis_angle((1, 0, 0), pi / 2), 0.1),
+            compound_child(SHAPE_CYLINDER, (0.2, 1.0), (0.0, 0.0, -1.0), _axis_angle((1, 0, 0), pi / 2), 0.1)]
+    ratios = np.array([k.mass_ratio for k in kids])
+    centroid = sum(np.array(k.position[:]) * k.mass_ratio for k in kids) / ratios.sum()
+    b = SceneBuilder()
+    b.add(SHAPE_BOX, (30.0, 10.0, 30.0), 10000.0, (0, -5, 0), kin=1, e=0.5)
+    b.add(SHAPE_BOX, (1.0, 1.0, 1.0), 1.0, (0, 2, 0), e=0.5)
+    b.add(SHAPE_SPHERE, (0.5,), 1.0, (0.1, 4, 0.1), e=0.5)
+    b.add(SHAPE_CYLINDER, (0.4, 1.0), 1.0, (0, 6, 0), e=0.5)
+    first = len(b.rows)
+    for i in range(count):
+        b.add(SHAPE_COMPOUND, (), 1.0, tuple(np.array([0.0, 10.0 + 5.0 * i, 0.0]) - centroid), e=0.5)
+    sc = b.scene(name=f"compound_demo{count}")
+    sc.angular_damping = np.full(sc.n, 0.8)
+    sc.angular_damping[0] = 0.0
+    sc.compounds = {first + i: list(kids) for i in range(count)}
+    if sleep:
+        sc.sleep = (0.01, 0.01, 1.0)
+    return sc
+
+
+def compound_pile_scene(n=24, seed=9, grid=10, size=(12.0, 12.0)) -> Scene:
+    """Compound bodies over a heightfield: each a box with a sphere, a cylinder and a convex hull
+    child at random offsets and orientations (mass ratios 0.4, 0.2, 0.2, 0.2), dropped in a column so
+    they land on the terrain and on each other, with a few plain boxes and spheres among them."""
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    hf = b.add_mesh(heightfield_mesh(rng, grid, grid, size, 0.5))
+    b.add(SHAPE_CONCAVE, (), 1.0, (0.0, 0.0, 0.0), kin=1, mesh=hf)
+    hull = b.add_mesh(gaussian_hull(rng, 16, 0.3))
+    compounds = {}
+    for i in range(n):
+        pos = (rng.uniform(-2, 2), 2.5 + 2.2 * i, rng.uniform(-2, 2))
+        rot = quat_to_matrix(random_quaternions(rng, 1))[0]
+        if i % 6 == 5:
+            b.add(SHAPE_SPHERE if i % 12 == 5 else SHAPE_BOX, (0.4,) if i % 12 == 5 else (0.7, 0.5, 0.6), 1.0, pos, rot)
+            continue
+        k = b.add(SHAPE_COMPOUND, (), 2.0, pos, rot, e=0.2)
+        r3 = quat_to_matrix(random_quaternions(rng, 3))
+        compounds[k] = [compound_child(SHAPE_BOX, (0.8, 0.4, 0.6), (0.0, 0.0, 0.0), ratio=0.4),
+                        compound_child(SHAPE_SPHERE, (0.3,), tuple(rng.uniform(-0.6, 0.6, 3)), ratio=0.2),
+                        compound_child(SHAPE_CYLINDER, (0.15, 0.8), tuple(rng.uniform(-0.6, 0.6, 3)), r3[0], 0.2),
+                        compound_child(SHAPE_CONVEX, (), tuple(rng.uniform(-0.6, 0.6, 3)), r3[1], 0.2, mesh=hull)]
+    sc = b.scene(name=f"compound_pile{n}")
+    sc.compounds = compounds
     return sc
