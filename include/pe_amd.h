@@ -294,8 +294,10 @@ int pe_world_set_solver_blocks(pe_world* w, int32_t blocks);
  * as the world's last integrate stage wrote them: AABB records [n][8], collision records [n][20],
  * solver records [n][24] (velocities after gravity, before the solve).  Any pointer may be NULL. */
 int pe_world_get_records(pe_world* w, float* aabb, float* body, float* solver);
-/* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours. */
-int pe_world_get_counts(pe_world* w, int64_t* counts /*5*/);
+/* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours, rounds of the
+ * colouring (worlds of more than 12 288 bodies are coloured in rounds over the GPU; 0 for smaller
+ * worlds, which one workgroup colours). */
+int pe_world_get_counts(pe_world* w, int64_t* counts /*6*/);
 
 /* ------------------------------------------------------------------ stage entry points
  * The reference's World calls three replaceable stages (World::setBroadPhase / setNarrowPhase /

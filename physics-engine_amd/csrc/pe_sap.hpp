@@ -28,7 +28,11 @@ struct DevCounts {          // device-side counters of one step
     int32_t nshape;         // pairs of this narrowphase that involve a cylinder or a mesh ...
     int32_t nsat;           // ... of which these run the polyhedral separating-axis routine
     int32_t ncomp;          // ... and these involve a compound body
-    int32_t pad;
+    int32_t nact;           // pairs the colouring orders (contact points, not both bodies kinematic)
+    uint32_t jp_rem[3];     // colouring rounds: pairs left uncoloured, round r in slot r % 3 (a slot is
+                            // cleared two rounds before it is counted, one round after it is read)
+    uint32_t adj_total;     // colouring: adjacency entries
+    int32_t jp_rounds;      // colouring: rounds the last colouring took
 };
 // Capacity errors, OR-ed into DevCounts::overflow by the step that hits them and kept until the
 // host reads and reports them: a result that lost pairs or contact points is never returned as a

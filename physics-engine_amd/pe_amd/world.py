@@ -274,6 +274,6 @@ class World:
         return a, b, c
 
     def counts(self):
-        c = (ctypes.c_int64 * 5)()
+        c = (ctypes.c_int64 * 6)()
         check(lib().pe_world_get_counts(self.h, c))
-        return dict(zip(("bodies", "dynamic", "pairs", "points", "colors"), list(c)))
+        return dict(zip(("bodies", "dynamic", "pairs", "points", "colors", "colour_rounds"), list(c)))
