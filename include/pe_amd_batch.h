@@ -60,6 +60,10 @@ int pe_batch_step_timed(pe_batch* b, int32_t nsteps, double* ms);
  * duration in microseconds: integrate, sort, sweep, narrowphase, solver setup, colouring,
  * Gauss-Seidel sweeps.  Drops a captured graph. */
 int pe_batch_phase_profile(pe_batch* b, double* us /*8*/);
+/* The same step's raw timestamps, ts[world][8] in ticks of 10 ns (0 where not recorded): start,
+ * integrate, sort, sweep done (K1); solver start, rows set up, coloured, swept (K3; stamped on the
+ * first world of each solver group only).  No counterpart in the reference; for tuning. */
+int pe_batch_phase_trace(pe_batch* b, uint64_t* ts /*8 nworlds*/);
 
 /* Runs nsteps eagerly, staged for measurement (every class's integrate + broadphase, then every
  * class's narrowphase, then every class's solver), with a hipEvent pair around every stage;

@@ -1085,23 +1085,26 @@ template <class R> inline void stage_narrowphase(World<R>& w) {
     collision_events(w);
 }
 
-// The claim priority of pair p: a fixed bijection of [0, 2^31) (odd multiplications modulo 2^31
-// and right xor-shifts), the device library's rule.
-inline int color_prio(int p) {
-    uint32_t x = (uint32_t)p & 0x7fffffffu;
-    x = (x * 0x9E3779B1u) & 0x7fffffffu;
+// The claim priority of pair p with npts contact points (the device library's rule): the point
+// count first, more points lower (earlier), then a fixed bijection of the pair index onto [0, 2^28)
+// (odd multiplications modulo 2^28 and right xor-shifts).
+inline int color_prio(int p, int npts) {
+    const uint32_t m = (1u << 28) - 1u;
+    uint32_t x = (uint32_t)p & m;
+    x = (x * 0x9E3779B1u) & m;
     x ^= x >> 15;
-    x = (x * 0x2C1B3C6Du) & 0x7fffffffu;
+    x = (x * 0x2C1B3C6Du) & m;
     x ^= x >> 12;
-    x = (x * 0x297A2D39u) & 0x7fffffffu;
+    x = (x * 0x297A2D39u) & m;
     x ^= x >> 15;
-    return (int)x;
+    const int k = npts < 1 ? 1 : (npts > 8 ? 8 : npts);
+    return (int)((uint32_t)(8 - k) << 28 | x);
 }
 
 // Constraint-graph colouring that fixes the solve order (the device library's rule; the
 // reference itself sweeps in plain pair order).  Colour c is a maximal independent set built in
 // sub-rounds: every remaining candidate claims each of its untaken dynamic bodies with its
-// priority color_prio(pair index) (lowest priority wins); a candidate holding all its claims
+// priority color_prio(pair index, point count) (lowest priority wins); a candidate holding all its claims
 // joins colour c and takes its bodies; a candidate touching a body taken earlier in this colour
 // waits for the next colour.  Kinematic bodies never change (rigidbody.cpp:167), so they impose
 // no order.  Pairs of one colour share no dynamic body, so any order inside a colour gives the
@@ -1126,14 +1129,14 @@ template <class R> inline void compute_color_order(World<R>& w) {
         while (!cand.empty()) {
             for (int p : cand) {
                 const int a = w.results[p].a, b = w.results[p].b;
-                if (dyn(a) && !taken[a]) claim[a] = std::min(claim[a], color_prio(p));
-                if (dyn(b) && !taken[b]) claim[b] = std::min(claim[b], color_prio(p));
+                if (dyn(a) && !taken[a]) claim[a] = std::min(claim[a], color_prio(p, w.results[p].npts));
+                if (dyn(b) && !taken[b]) claim[b] = std::min(claim[b], color_prio(p, w.results[p].npts));
             }
             std::vector<int> next, winners;
             for (int p : cand) {
                 const int a = w.results[p].a, b = w.results[p].b;
                 const bool blk = (dyn(a) && taken[a]) || (dyn(b) && taken[b]);
-                const bool win = !blk && (!dyn(a) || claim[a] == color_prio(p)) && (!dyn(b) || claim[b] == color_prio(p));
+                const bool win = !blk && (!dyn(a) || claim[a] == color_prio(p, w.results[p].npts)) && (!dyn(b) || claim[b] == color_prio(p, w.results[p].npts));
                 if (blk) deferred.push_back(p);
                 else if (win) winners.push_back(p);
                 else next.push_back(p);

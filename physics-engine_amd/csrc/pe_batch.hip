@@ -528,7 +528,7 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
                 atomicOr(d.ovf_any, WOVF_CLIP);
             }
             gidx[lp] = gs;
-            pri[lp] = colour::prio(lp - s_wb[k]);
+            pri[lp] = colour::prio(lp - s_wb[k], np_);
             pab[lp] = (pa - B0) | ((pb - B0) << 16);
             if (np_ > 0) {
                 flag = !(imk[pa - B0] < 0.0f && imk[pb - B0] < 0.0f);
@@ -575,28 +575,28 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
 
 // The colour-ordered passes of a group: thread tid solves the pairs at order slots
 // cs[c] + tid + k * BLOCK of every colour c in every iteration, so it alone reads and writes their
-// rows.  pair(lp, true, cur) loads the first row of local pair lp into cur, pair(lp, false, cur)
-// solves the pair from it; the first row of a thread's first pair of the next colour is loaded
-// right after its pairs of this colour are done, ahead of the barrier.
+// rows.  pair(lp, true, cur, nxt) loads the first two rows of local pair lp, pair(lp, false, cur,
+// nxt) solves the pair from them; the first rows of a thread's first pair of the next colour are
+// loaded right after its pairs of this colour are done, ahead of the barrier.
 template <int BLOCK, class PairFn>
 __device__ __forceinline__ void sweep_passes(int iterations, int ncol, const int* cs, const int* order, int tid,
                                              PairFn pair) {
-    float cur[RW];
-    int pend = -1;   // local pair whose first row cur holds
+    float cur[RW], nxt[RW];
+    int pend = -1;   // local pair whose first rows cur and nxt hold
     auto fetch = [&](int c) {
         const int q = cs[c] + tid;
         pend = q < cs[c + 1] ? order[q] : -1;
-        if (pend >= 0) pair(pend, true, cur);
+        if (pend >= 0) pair(pend, true, cur, nxt);
     };
     if (ncol > 0) fetch(0);
     for (int it = 0; it < iterations; it++) {
         for (int c = 0; c < ncol; c++) {
             if (pend >= 0) {
-                pair(pend, false, cur);
+                pair(pend, false, cur, nxt);
                 for (int q = cs[c] + tid + BLOCK; q < cs[c + 1]; q += BLOCK) {
                     const int lp = order[q];
-                    pair(lp, true, cur);
-                    pair(lp, false, cur);
+                    pair(lp, true, cur, nxt);
+                    pair(lp, false, cur, nxt);
                 }
             }
             const bool last = it + 1 == iterations && c + 1 == ncol;
@@ -655,10 +655,10 @@ __device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __
     int* cs = order + nord;   // colour starts (ncol + 1 <= GP + 1 entries fit the layout's bound)
     for (int q = tid; q <= ncol; q += BLOCK) cs[q] = d.cstart[w0.cs_off + q];
     __syncthreads();
-    sweep_passes<BLOCK>(d.iterations, ncol, cs, order, tid, [&](int lp, bool first, float (&cur)[RW]) {
+    sweep_passes<BLOCK>(d.iterations, ncol, cs, order, tid, [&](int lp, bool first, float (&cur)[RW], float (&nxt)[RW]) {
         float* rows = d.ps.rows + (size_t)gidx[lp] * 8 * RW;
-        if (first) prefetch_pair_row(L, lp, rows, cur);
-        else iterate_pair_lds(L, lp, rows, cur);
+        if (first) prefetch_pair_rows(L, lp, rows, cur, nxt);
+        else iterate_pair_lds(L, lp, rows, cur, nxt);
     });
     PE_PHASE(7);
     for (int i = tid; i < GB; i += BLOCK) {
@@ -1330,25 +1330,44 @@ int pe_batch_get_counts(pe_batch* B, int64_t* counts) {
 }  // extern "C"
 
 // ====================================================================== phase profile
-// Runs one step with per-world phase timestamps and returns, per phase, the sum over worlds of
-// the phase's duration in microseconds (wall_clock64 runs at 100 MHz on gfx9).
-extern "C" int pe_batch_phase_profile(pe_batch* B, double* us /*8*/) {
-    if (!B || !us) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+// Runs one step with per-world phase timestamps (wall_clock64, 100 MHz on gfx9) into ts
+// [nworlds][PHASES], zero where a phase was not recorded: the solver phases of a group of worlds
+// are stamped on its first world only.
+static int phase_step(pe_batch* B, uint64_t* h) {
     uint64_t* ts = nullptr;
-    PE_HIP(hipMalloc(&ts, (size_t)B->nworlds * PHASES * 8));
+    const size_t bytes = (size_t)B->nworlds * PHASES * 8;
+    PE_HIP(hipMalloc(&ts, bytes));
+    int rc = hipMemsetAsync(ts, 0, bytes, B->stream) == hipSuccess ? PE_OK : PE_ERR_DEVICE;
     B->d.phase_ts = ts;
     B->drop_graph();
-    int rc = B->launch_step();
+    if (rc == PE_OK) rc = B->launch_step();
     B->d.phase_ts = nullptr;
-    std::vector<uint64_t> h((size_t)B->nworlds * PHASES);
-    if (rc == PE_OK) {
-        PE_HIP(hipMemcpyAsync(h.data(), ts, h.size() * 8, hipMemcpyDeviceToHost, B->stream));
-        PE_HIP(hipStreamSynchronize(B->stream));
+    if (rc == PE_OK && (hipMemcpyAsync(h, ts, bytes, hipMemcpyDeviceToHost, B->stream) != hipSuccess ||
+                        hipStreamSynchronize(B->stream) != hipSuccess)) {
+        pe_set_error("phase profile: copy failed");
+        rc = PE_ERR_DEVICE;
     }
     hipFree(ts);
+    return rc;
+}
+
+extern "C" int pe_batch_phase_trace(pe_batch* B, uint64_t* ts) {
+    if (!B || !ts) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    return phase_step(B, ts);
+}
+
+// The sum over worlds of each phase's duration in microseconds (phases whose two stamps exist).
+extern "C" int pe_batch_phase_profile(pe_batch* B, double* us /*8*/) {
+    if (!B || !us) { pe_set_error("null argument"); return PE_ERR_INVALID; }
+    std::vector<uint64_t> h((size_t)B->nworlds * PHASES);
+    const int rc = phase_step(B, h.data());
     for (int k = 0; k < PHASES; k++) us[k] = 0;
+    if (rc != PE_OK) return rc;
     for (int w = 0; w < B->nworlds; w++)
-        for (int k = 1; k < PHASES; k++) us[k - 1] += (double)(h[(size_t)w * PHASES + k] - h[(size_t)w * PHASES + k - 1]) / 100.0;
+        for (int k = 1; k < PHASES; k++) {
+            const uint64_t a = h[(size_t)w * PHASES + k - 1], b = h[(size_t)w * PHASES + k];
+            if (a && b) us[k - 1] += (double)(b - a) / 100.0;
+        }
     return rc;
 }
 

@@ -654,15 +654,21 @@ __device__ __forceinline__ void lds_load_body(const LdsSolve& L, const BodyView&
 }
 // iterate_pair on the LDS copy of the bodies (same arithmetic, same order); the pair's rows are
 // at rows (the row store), RW floats apart.
-// Loads the first solver row of local pair pl (nothing for a pair without rows) into cur, ahead
-// of iterate_pair_lds: the sweep issues it one colour early so its latency overlaps the barrier.
-__device__ __forceinline__ void prefetch_pair_row(const LdsSolve& L, int pl, const float* rows, float (&cur)[RW]) {
-    if (L.rs[pl + 1] > L.rs[pl]) ld_rec<RW / 4>(rows, cur);
+// Loads the first two solver rows of local pair pl (those it has) into cur and nxt, ahead of
+// iterate_pair_lds: the sweep issues them one colour early so their latency overlaps the barrier,
+// and a pair of one or two rows then solves without waiting on memory.
+__device__ __forceinline__ void prefetch_pair_rows(const LdsSolve& L, int pl, const float* rows, float (&cur)[RW],
+                                                   float (&nxt)[RW]) {
+    const int nr = L.rs[pl + 1] - L.rs[pl];
+    if (nr > 0) ld_rec<RW / 4>(rows, cur);
+    if (nr > 1) ld_rec<RW / 4>(rows + RW, nxt);
 }
 
-// One Gauss-Seidel pass over the rows of local pair pl on the LDS body state; cur holds its first
-// row (prefetch_pair_row); each later row is loaded while the one before it is solved.
-__device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, float* rows, float (&cur)[RW]) {
+// One Gauss-Seidel pass over the rows of local pair pl on the LDS body state; cur and nxt hold
+// its first two rows (prefetch_pair_rows); row k + 2 is loaded into the buffer row k leaves, while
+// row k + 1 is solved.
+__device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, float* rows, float (&cur)[RW],
+                                                 float (&nxt)[RW]) {
     const int ab = L.pab[pl];
     const int a = ab & 0xffff, c = ab >> 16;
     const int nr = L.rs[pl + 1] - L.rs[pl];
@@ -688,14 +694,13 @@ __device__ __forceinline__ void iterate_pair_lds(const LdsSolve& L, int pl, floa
         if (imb >= 0.0f) apply_temp(tvb, twb, imb, Ib, r.rb, neg(J));
         reinterpret_cast<float4*>(rows + (size_t)k * RW)[5] = make_float4(ln, lt0, lt1, 0.0f);
     };
-    // Rows alternate between cur and nxt, each loaded while the other is solved (no copies).
-    float nxt[RW];
+    // Rows alternate between cur and nxt (no copies).
     for (int k = 0; k < nr; k += 2) {
-        if (k + 1 < nr) ld_rec<RW / 4>(rows + (size_t)(k + 1) * RW, nxt);
         row(cur, k);
-        if (k + 1 >= nr) break;
         if (k + 2 < nr) ld_rec<RW / 4>(rows + (size_t)(k + 2) * RW, cur);
+        if (k + 1 >= nr) break;
         row(nxt, k + 1);
+        if (k + 3 < nr) ld_rec<RW / 4>(rows + (size_t)(k + 3) * RW, nxt);
     }
     if (ima >= 0.0f) {
         L.bs[4 * a] = make_float4(tva.x, tva.y, tva.z, twa.x);
@@ -727,22 +732,27 @@ __device__ __forceinline__ void sync_body(const BodyView& b, int i) {
 namespace colour {
 constexpr int FREE = 0x7fffffff;
 constexpr int TAKEN = -1;
-// A pair's claim priority: a fixed bijection of its index in its world onto [0, 2^31) (odd
-// multiplications modulo 2^31 and right xor-shifts, each invertible there).  Claims by plain
-// index make long chains (pairs come out of the sweep sorted by body, so neighbours in the
-// constraint graph have neighbouring indices and each sub-round frees only the next link);
-// mixed priorities settle a colour in a few sub-rounds (70 instead of 738 for the 10k box rain
-// at rest).  The lowest priority still wins, so the colours stay a deterministic function of
-// the pair list.
-__host__ __device__ inline int prio(int p) {
-    uint32_t x = (uint32_t)p & 0x7fffffffu;
-    x = (x * 0x9E3779B1u) & 0x7fffffffu;
+// A pair's claim priority (the lowest wins), unique within its world: its contact-point count
+// first, more points colouring earlier, then a fixed bijection of its index in the world onto
+// [0, 2^28) (odd multiplications modulo 2^28 and right xor-shifts, each invertible there).  A
+// colour's pass lasts as long as its pair with the most rows (points), so gathering those pairs in
+// the first colours shortens the sum of the passes (about a fifth for settled box piles).  Claims
+// by plain index would make long chains (pairs come out of the sweep sorted by body, so neighbours
+// in the constraint graph have neighbouring indices and each sub-round frees only the next link);
+// mixed indices settle a colour in a few sub-rounds (70 instead of 738 for the 10k box rain at
+// rest).  Pair indices are below 2^28 (a world's pair capacity is capped there).
+constexpr int PRIO_INDEX_BITS = 28;
+__host__ __device__ inline int prio(int p, int npts) {
+    const uint32_t m = (1u << PRIO_INDEX_BITS) - 1u;
+    uint32_t x = (uint32_t)p & m;
+    x = (x * 0x9E3779B1u) & m;
     x ^= x >> 15;
-    x = (x * 0x2C1B3C6Du) & 0x7fffffffu;
+    x = (x * 0x2C1B3C6Du) & m;
     x ^= x >> 12;
-    x = (x * 0x297A2D39u) & 0x7fffffffu;
+    x = (x * 0x297A2D39u) & m;
     x ^= x >> 15;
-    return (int)x;
+    const int k = npts < 1 ? 1 : (npts > 8 ? 8 : npts);
+    return (int)((uint32_t)(8 - k) << PRIO_INDEX_BITS | x);
 }
 }
 
@@ -779,7 +789,7 @@ __device__ int colour_pairs(const BodyView& b, const PairStore& ps, int* bufs[3]
             for (int q = tid; q < ncand; q += B) {
                 const int p = cand[q];
                 const int a = ps.pa[p], c = ps.pb[p];
-                const int pr = prio(p - pbase);
+                const int pr = prio(p - pbase, ps.npts[p]);
                 if (!b.iat(I_KIN, a)) atomicMin(&claim[a - off], pr);
                 if (!b.iat(I_KIN, c)) atomicMin(&claim[c - off], pr);
             }
@@ -796,7 +806,7 @@ __device__ int colour_pairs(const BodyView& b, const PairStore& ps, int* bufs[3]
                     const bool ka = b.iat(I_KIN, a) != 0, kc = b.iat(I_KIN, c) != 0;
                     const int ca = ka ? 0 : claim_get<LDS>(claim, a - off);
                     const int cc = kc ? 0 : claim_get<LDS>(claim, c - off);
-                    const int pr = prio(p - pbase);
+                    const int pr = prio(p - pbase, ps.npts[p]);
                     blk = (!ka && ca == TAKEN) || (!kc && cc == TAKEN);
                     win = !blk && (ka || ca == pr) && (kc || cc == pr);
                     stay = !blk && !win;

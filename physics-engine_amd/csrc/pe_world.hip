@@ -46,7 +46,7 @@ struct Dev {
     // colour per pair [pcap], pairs per colour [pcap + 1]
     uint32_t* jp_deg;
     int32_t* jp_fill;
-    int32_t* jp_adj;
+    int2* jp_adj;        // per dynamic body, its active pairs: (pair, claim priority)
     int32_t* jp_col;
     int32_t* jp_ccount;
                             // and to k_narrow_sat (from the back)
@@ -422,8 +422,9 @@ __global__ void k_jp_fill(Dev d) {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nact; k += gridDim.x * blockDim.x) {
         const int p = d.act[k];
         const int a = d.ps.pa[p], c = d.ps.pb[p];
-        if (!d.b.iat(I_KIN, a)) d.jp_adj[d.jp_deg[a] + atomicAdd(&d.jp_fill[a], 1)] = p;
-        if (!d.b.iat(I_KIN, c)) d.jp_adj[d.jp_deg[c] + atomicAdd(&d.jp_fill[c], 1)] = p;
+        const int2 e = make_int2(p, colour::prio(p, d.ps.npts[p]));
+        if (!d.b.iat(I_KIN, a)) d.jp_adj[d.jp_deg[a] + atomicAdd(&d.jp_fill[a], 1)] = e;
+        if (!d.b.iat(I_KIN, c)) d.jp_adj[d.jp_deg[c] + atomicAdd(&d.jp_fill[c], 1)] = e;
     }
 }
 constexpr int JP_THREADS = 256;
@@ -435,7 +436,7 @@ __device__ __forceinline__ int jp_col_load(const int32_t* col, int q) {
 // neighbour scan is a chain of L2 round trips otherwise).
 constexpr int JP_BATCH = 8;
 __device__ int jp_try(const Dev& d, int p) {
-    const int pr = colour::prio(p);
+    const int pr = colour::prio(p, d.ps.npts[p]);
     const int body[2] = {d.ps.pa[p], d.ps.pb[p]};
     uint64_t used = 0;
     bool big = false;
@@ -444,13 +445,14 @@ __device__ int jp_try(const Dev& d, int p) {
         if (d.b.iat(I_KIN, x)) continue;
         const int k0 = (int)d.jp_deg[x], k1 = (int)d.jp_deg[x + 1];
         for (int k = k0; k < k1; k += JP_BATCH) {
-            int q[JP_BATCH], cq[JP_BATCH];
+            int2 q[JP_BATCH];
+            int cq[JP_BATCH];
 #pragma unroll
-            for (int j = 0; j < JP_BATCH; j++) q[j] = k + j < k1 ? d.jp_adj[k + j] : p;
+            for (int j = 0; j < JP_BATCH; j++) q[j] = k + j < k1 ? d.jp_adj[k + j] : make_int2(p, pr);
 #pragma unroll
             for (int j = 0; j < JP_BATCH; j++) {
-                const bool lower = q[j] != p && colour::prio(q[j]) < pr;
-                cq[j] = lower ? jp_col_load(d.jp_col, q[j]) : -2;
+                const bool lower = q[j].x != p && q[j].y < pr;
+                cq[j] = lower ? jp_col_load(d.jp_col, q[j].x) : -2;
             }
             int wait = 0;
 #pragma unroll
@@ -471,8 +473,8 @@ __device__ int jp_try(const Dev& d, int p) {
             const int x = body[s];
             if (d.b.iat(I_KIN, x)) continue;
             for (int k = (int)d.jp_deg[x]; k < (int)d.jp_deg[x + 1] && !taken; k++) {
-                const int q = d.jp_adj[k];
-                if (q != p && colour::prio(q) < pr && jp_col_load(d.jp_col, q) == c) taken = true;
+                const int2 q = d.jp_adj[k];
+                if (q.x != p && q.y < pr && jp_col_load(d.jp_col, q.x) == c) taken = true;
             }
         }
         if (!taken) return c;
@@ -762,7 +764,13 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     const int n = sc->n;
     w->n = n;
     w->stride = (n + 63) & ~63;
-    const int pcap = max_pairs > 0 ? max_pairs : std::max(1024, 16 * n);
+    const int64_t want = max_pairs > 0 ? (int64_t)max_pairs : std::max<int64_t>(1024, 16 * (int64_t)n);
+    if (want > ((int64_t)1 << colour::PRIO_INDEX_BITS)) {
+        delete w;
+        pe_set_error("pe_world_create: at most 2^28 pairs per world");
+        return PE_ERR_INVALID;
+    }
+    const int pcap = (int)want;
     auto fail = [&](int rc) { delete w; return rc; };
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) { pe_set_error("stream create failed"); return fail(PE_ERR_DEVICE); }
     Dev& d = w->d;

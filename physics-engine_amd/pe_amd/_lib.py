@@ -30,7 +30,7 @@ EXPORTS = [
     "pe_stage_solve", "pe_stage_broadphase_noalloc",
     "pe_batch_create", "pe_batch_destroy", "pe_batch_step", "pe_batch_sync", "pe_batch_get_world_state",
     "pe_batch_get_counts", "pe_batch_set_config", "pe_batch_get_config", "pe_batch_capture_graph",
-    "pe_batch_world_footprint", "pe_batch_step_timed", "pe_batch_set_solver_groups", "pe_batch_get_solver_groups", "pe_batch_phase_profile", "pe_batch_stage_times",
+    "pe_batch_world_footprint", "pe_batch_step_timed", "pe_batch_set_solver_groups", "pe_batch_get_solver_groups", "pe_batch_phase_profile", "pe_batch_phase_trace", "pe_batch_stage_times",
     "pe_batch_class_times", "pe_batch_snapshot", "pe_batch_restore",
     "pe_server_create", "pe_server_destroy", "pe_server_footprint", "pe_server_submit", "pe_server_tick",
     "pe_server_poll", "pe_server_result", "pe_server_stats", "pe_server_cancel", "pe_server_request_stats",

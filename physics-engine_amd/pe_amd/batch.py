@@ -95,6 +95,15 @@ class Batch:
         names = ("integrate", "sort", "sweep", "narrowphase", "setup", "colouring", "gauss_seidel")
         return dict(zip(names, list(us)[:7]))
 
+    def phase_trace(self):
+        """One step with in-kernel phase timestamps: [nworlds, 8] array of 10 ns ticks, 0 where not
+        recorded (K1: start, integrate, sort, sweep; K3: start, rows, colouring, sweeps, on the first
+        world of each solver group)."""
+        ts = np.zeros((len(self.scenes), 8), np.uint64)
+        check(lib().pe_batch_phase_trace(self.h, ts.ctypes.data_as(ctypes.c_void_p)))
+        self.steps_done += 1
+        return ts
+
     def capture_graph(self):
         check(lib().pe_batch_capture_graph(self.h))
 
