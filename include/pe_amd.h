@@ -290,13 +290,16 @@ int pe_world_get_stage_times(pe_world* w, double* ms /*4*/);
  * colour's pair per thread; clamped to the co-resident limit).  No counterpart in the reference; results
  * do not depend on it. */
 int pe_world_set_solver_blocks(pe_world* w, int32_t blocks);
+/* How the solver's colouring runs (0 = by body count: one workgroup's claims in LDS up to 7 500
+ * bodies, rounds over the GPU beyond; 1 = one workgroup, at most 12 288 bodies; 2 = rounds).  Both
+ * give the same colour classes.  No counterpart in the reference; results do not depend on it. */
+int pe_world_set_colouring(pe_world* w, int32_t mode);
 /* The per-body records the stage entry points take (layouts under "stage entry points" below),
  * as the world's last integrate stage wrote them: AABB records [n][8], collision records [n][20],
  * solver records [n][24] (velocities after gravity, before the solve).  Any pointer may be NULL. */
 int pe_world_get_records(pe_world* w, float* aabb, float* body, float* solver);
 /* Counts of the last step: bodies, dynamic bodies, pairs, contact points, colours, rounds of the
- * colouring (worlds of more than 12 288 bodies are coloured in rounds over the GPU; 0 for smaller
- * worlds, which one workgroup colours). */
+ * colouring (0 when one workgroup coloured: see pe_world_set_colouring). */
 int pe_world_get_counts(pe_world* w, int64_t* counts /*6*/);
 
 /* ------------------------------------------------------------------ stage entry points

@@ -555,7 +555,7 @@ __device__ __forceinline__ void group_prep(const BatchDev& d, const int32_t* __r
             if (rs[mid] <= r) lo = mid;
             else hi = mid - 1;
         }
-        setup_row(b, d.ps, gidx[lo], r - rs[lo], d.dt);
+        setup_row(b, d.ps, gidx[lo], B0 + (pab[lo] & 0xffff), B0 + (pab[lo] >> 16), r - rs[lo], d.dt);
     }
     PE_PHASE(5);
     const int ncol = colour_group(pab, imk, pri, bufs, nact, order, cs, claim);

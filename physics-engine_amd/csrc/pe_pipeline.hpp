@@ -568,11 +568,14 @@ __device__ __forceinline__ void setup_pair(const BodyView& b, const PairStore& p
 }
 // Point k of pair p alone, with the same arithmetic as setup_pair; the rows of a pair can so be
 // set up by different threads.
-__device__ __forceinline__ void setup_row(const BodyView& b, const PairStore& ps, int p, int k, float dt) {
-    const int ia = ps.pa[p], ib = ps.pb[p];
+// ia, ib: the pair's bodies, when the caller holds them (one dependent load fewer).
+__device__ __forceinline__ void setup_row(const BodyView& b, const PairStore& ps, int p, int ia, int ib, int k, float dt) {
     const CollBody CA = load_coll(b, ia), CB = load_coll(b, ib);
     const SolvBody A = load_solv(b, ia), B = load_solv(b, ib);
     setup_point(ps, p, k, CA, CB, A, B, pair_coef(CA, CB, A, B), dt, ps.rows + ((size_t)p * 8 + k) * RW);
+}
+__device__ __forceinline__ void setup_row(const BodyView& b, const PairStore& ps, int p, int k, float dt) {
+    setup_row(b, ps, p, ps.pa[p], ps.pb[p], k, dt);
 }
 
 // RigidBody::applyTempImpulse (rigidbody.cpp:166-170), with the body's inverse mass and world
@@ -629,6 +632,78 @@ __device__ __forceinline__ void iterate_pair(const BodyView& b, const PairStore&
         float4* d = reinterpret_cast<float4*>(b.sbr(ib));
         d[0] = make_float4(B.tv.x, B.tv.y, B.tv.z, B.tw.x);
         d[1].x = B.tw.y; d[1].y = B.tw.z;
+    }
+}
+
+// iterate_pair split around the barrier before the pair's colour: gs_prefetch loads what no other
+// thread changes during the sweeps -- the pair's bodies, their world inverse inertias and its
+// first four rows (a pair's rows, accumulated impulses included, are only written by the thread
+// that solves it, every iteration the same thread) -- and gs_solve, after the barrier, loads the
+// bodies' temp velocities, which earlier colours changed, and solves the pair with the same
+// arithmetic in the same order.
+struct GsPair {
+    int p, ia, ib, n;
+    m33 Ia, Ib;
+    float rr[4][RW];
+};
+__device__ __forceinline__ void gs_prefetch(const BodyView& b, const PairStore& ps, int p, GsPair& G) {
+    G.p = p;
+    G.ia = ps.pa[p];
+    G.ib = ps.pb[p];
+    G.n = ps.npts[p];
+    const float* sa = b.sbr(G.ia);
+    const float* sb = b.sbr(G.ib);
+#pragma unroll
+    for (int k = 0; k < 9; k++) { G.Ia.m[k] = sa[SB_IW + k]; G.Ib.m[k] = sb[SB_IW + k]; }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        if (u < G.n) ld_rec<RW / 4>(ps.rows + ((size_t)p * 8 + u) * RW, G.rr[u]);
+}
+__device__ __forceinline__ void gs_solve(const BodyView& b, const PairStore& ps, GsPair& G) {
+    const int p = G.p, n = G.n;
+    float* R = ps.rows;
+    const float4* qa = reinterpret_cast<const float4*>(b.sbr(G.ia));
+    const float4* qb = reinterpret_cast<const float4*>(b.sbr(G.ib));
+    const float4 a0 = qa[0], a1 = qa[1], b0 = qb[0], b1 = qb[1];
+    f3 tva{a0.x, a0.y, a0.z}, twa{a0.w, a1.x, a1.y}, tvb{b0.x, b0.y, b0.z}, twb{b0.w, b1.x, b1.y};
+    const float ima = a1.z, imb = b1.z;
+    const bool ka = a1.w != 0.0f, kb = b1.w != 0.0f;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        if (k0 > 0) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (k0 + u < n) ld_rec<RW / 4>(R + ((size_t)p * 8 + k0 + u) * RW, G.rr[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (k0 + u < n) {
+                const float* o = G.rr[u];
+                Row r;
+                r.ra = f3{o[R_RA], o[R_RA + 1], o[R_RA + 2]};
+                r.rb = f3{o[R_RB], o[R_RB + 1], o[R_RB + 2]};
+                r.n = f3{o[R_N], o[R_N + 1], o[R_N + 2]};
+                r.t0 = f3{o[R_T0], o[R_T0 + 1], o[R_T0 + 2]};
+                r.t1 = f3{o[R_T1], o[R_T1 + 1], o[R_T1 + 2]};
+                r.rhs = o[R_RHS]; r.mn = o[R_MN]; r.mt0 = o[R_MT0]; r.mt1 = o[R_MT1]; r.mu = o[R_MU];
+                float ln = o[R_LN], lt0 = o[R_LT0], lt1 = o[R_LT1];
+                const f3 J = iterate_row(r, ln, lt0, lt1, tva, twa, tvb, twb);
+                if (!ka) apply_temp(tva, twa, ima, G.Ia, r.ra, J);
+                if (!kb) apply_temp(tvb, twb, imb, G.Ib, r.rb, neg(J));
+                G.rr[u][R_LN] = ln; G.rr[u][R_LT0] = lt0; G.rr[u][R_LT1] = lt1;
+                float4* dst = reinterpret_cast<float4*>(R + ((size_t)p * 8 + k0 + u) * RW) + 5;   // floats 20..23
+                *dst = make_float4(ln, lt0, lt1, 0.0f);
+            }
+        }
+    }
+    if (!ka) {
+        float4* d = reinterpret_cast<float4*>(b.sbr(G.ia));
+        d[0] = make_float4(tva.x, tva.y, tva.z, twa.x);
+        d[1].x = twa.y; d[1].y = twa.z;
+    }
+    if (!kb) {
+        float4* d = reinterpret_cast<float4*>(b.sbr(G.ib));
+        d[0] = make_float4(tvb.x, tvb.y, tvb.z, twb.x);
+        d[1].x = twb.y; d[1].y = twb.z;
     }
 }
 

@@ -355,16 +355,23 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_colour(Dev d) {
 
 // Sense-reversing barrier over all workgroups of a cooperative launch (every workgroup is
 // resident, so every wait ends).  The last arrival resets the count and bumps the generation.
+// It also orders memory across the grid: the workgroup's writes before the barrier are visible to
+// every workgroup after it.  One thread per workgroup fences (a release fence before arriving, an
+// acquire fence after the wait: the workgroup barriers around them carry the ordering to its other
+// threads), and the arrival and the polling are relaxed atomics, so the caches are written back and
+// invalidated once per workgroup and barrier, not once per wave or poll.
 __device__ __forceinline__ void grid_barrier(uint32_t* count, uint32_t* gen, uint32_t nblocks) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (__hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
+        const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // also keeps the load of g before the arrival
+        if (__hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nblocks - 1) {
             __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);   // after the reset
         } else {
-            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -508,9 +515,7 @@ __global__ void __launch_bounds__(JP_THREADS) k_jp_colour(Dev d) {
         if (left) s_any = 1;
         __syncthreads();
         if (threadIdx.x == 0 && s_any) atomicAdd(&d.cnt->jp_rem[r % 3], 1u);
-        __threadfence();
         grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-        __threadfence();
         if (__hip_atomic_load(&d.cnt->jp_rem[r % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
             if (gtid == 0) d.cnt->jp_rounds = r + 1;
             break;
@@ -527,9 +532,7 @@ __global__ void __launch_bounds__(JP_THREADS) k_jp_colour(Dev d) {
     __syncthreads();
     for (int c = threadIdx.x; c < JP_BINS; c += blockDim.x)
         if (s_hist[c]) atomicAdd(&d.jp_ccount[c], s_hist[c]);
-    __threadfence();
     grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-    __threadfence();
     if (gtid == 0) {
         int run = 0, ncol = 0;
         while (ncol <= d.pcap && d.jp_ccount[ncol] > 0) {
@@ -542,9 +545,7 @@ __global__ void __launch_bounds__(JP_THREADS) k_jp_colour(Dev d) {
         d.cnt->ncolors = ncol;
         d.cnt->nordered = run;
     }
-    __threadfence();
     grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-    __threadfence();
     // the solve order: each workgroup reserves its range of every colour once, then places its pairs
     for (int c = threadIdx.x; c < JP_BINS; c += blockDim.x) {
         s_base[c] = s_hist[c] ? atomicAdd(&d.jp_ccount[c], s_hist[c]) : 0;
@@ -568,13 +569,29 @@ constexpr int GS_THREADS = 256;
 __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
     const int ncol = d.cnt->ncolors;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
+    // A thread's first pair of each colour is loaded before the barrier ahead of it (gs_prefetch):
+    // only the bodies' temp velocities are read after the barrier.
+    GsPair G;
+    bool have = false;
+    auto fetch = [&](int c) {
+        const int q = d.color_start[c] + gtid;
+        have = q < d.color_start[c + 1];
+        if (have) gs_prefetch(d.b, d.ps, d.order[q], G);
+    };
+    if (ncol > 0) fetch(0);
     for (int it = 0; it < d.iterations; it++) {
         for (int c = 0; c < ncol; c++) {
             const int c0 = d.color_start[c], c1 = d.color_start[c + 1];
-            for (int q = c0 + gtid; q < c1; q += gsz) iterate_pair(d.b, d.ps, d.order[q]);
-            __threadfence();   // release this colour's velocity updates
-            grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-            __threadfence();   // acquire: later colours read velocities other CUs wrote
+            if (have) {
+                gs_solve(d.b, d.ps, G);
+                for (int q = c0 + gtid + gsz; q < c1; q += gsz) {
+                    gs_prefetch(d.b, d.ps, d.order[q], G);
+                    gs_solve(d.b, d.ps, G);
+                }
+            }
+            if (c + 1 < ncol) fetch(c + 1);
+            else if (it + 1 < d.iterations) fetch(0);
+            grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);   // later colours read these velocities
         }
         // the joints after the contacts, colour by colour (sequential_impulse_solver.cpp:66-68)
         for (int c = 0; c < d.njcol; c++) {
@@ -582,9 +599,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
                 const int k = d.jorder[q];
                 joint_iterate(d.b, d.joints[k], d.jrows[k]);
             }
-            __threadfence();
             grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-            __threadfence();
         }
     }
     for (int i = gtid; i < d.b.n; i += gsz) sync_body(d.b, i);
@@ -611,9 +626,14 @@ __global__ void __launch_bounds__(256) k_count_points(Dev d) {
 // ====================================================================== host side
 // Workgroups of the cooperative Gauss-Seidel sweep: enough that a colour's pairs run about one
 // per thread, few enough that the grid barrier between colours stays cheap.  Measured at rest
-// (ms per step, 4 / 8 / 16 / 32 / 64 workgroups of 256): 10k box rain 3.01 / 2.52 / 2.47 / 2.71 /
-// 3.62; 50k shape mix 9.27 / 8.31 / 5.99 / 6.49 / 7.36.
-static int default_gs_blocks(int n, int max_blocks) { return std::min(max_blocks, std::max(4, std::min(16, n / 600))); }
+// (ms per step, 4 / 8 / 16 / 32 / 64 workgroups of 256): box rain of 1 000 bodies 0.45 / 0.47 / 0.49 /
+// 0.49, of 5 000 bodies 1.29 / 1.14 / 1.10 / 1.14, of 10 000 bodies (colouring rounds) - / 1.94 / 1.76 /
+// 1.72 / 1.82; 50k shape mix at step 280: 16 / 32 / 64 workgroups 7.29 / 7.06 / 7.08.
+static int default_gs_blocks(int n, int max_blocks) { return std::min(max_blocks, std::max(4, std::min(32, n / 300))); }
+// Bodies beyond which the colouring runs in rounds over the GPU rather than in one workgroup (box
+// rain at rest, ms per step with one workgroup / rounds: 5 000 bodies 1.10 / 1.22, 10 000 bodies
+// 1.97 / 1.72).
+constexpr int COLOUR_ROUNDS_FROM = 7500;
 
 struct pe_world {
     Dev d{};
@@ -623,6 +643,7 @@ struct pe_world {
     int gs_blocks = 0;      // workgroups of the cooperative Gauss-Seidel kernel
     int gs_max_blocks = 0;
     int jp_blocks = 1;                       // workgroups of the cooperative colouring  // co-resident limit
+    int colouring = 0;                       // 0: by body count, 1: one workgroup's claims, 2: rounds
     bool timing = false;
     hipEvent_t ev[5] = {};
     hipEvent_t order_ev[2] = {};   // orders work on a caller's stream with the world's own stream
@@ -685,9 +706,8 @@ struct pe_world {
     int launch_solve() {
         hipLaunchKernelGGL(k_setup, dim3(grid(d.pcap, 128)), dim3(128), 0, stream, d);
         if (d.njoints > 0) hipLaunchKernelGGL(k_joint_setup, dim3(grid(d.njoints, 64)), dim3(64), 0, stream, d);
-        if (n <= CLAIM_LDS) {
-            // claims in one workgroup's LDS: for up to CLAIM_LDS bodies faster than rounds over the GPU
-            // (10k box rain at rest: 0.53 ms against 1.23 ms for the rounds and their grid barriers)
+        if (colouring == 1 || (colouring == 0 && n <= COLOUR_ROUNDS_FROM)) {
+            // claims in one workgroup's LDS (at most CLAIM_LDS bodies)
             hipLaunchKernelGGL(k_colour<true>, dim3(1), dim3(SOLVE_THREADS), n * sizeof(int), stream, d);
         } else {
             // colouring by rounds: the pairs and their bodies' adjacency, then the rounds (cooperative)
@@ -1380,6 +1400,16 @@ int pe_world_set_solver_blocks(pe_world* w, int32_t blocks) {
     if (int rc = check_world(w)) return rc;
     if (blocks < 0) { pe_set_error("blocks must be >= 0"); return PE_ERR_INVALID; }
     w->gs_blocks = blocks == 0 ? default_gs_blocks(w->n, w->gs_max_blocks) : std::min(blocks, w->gs_max_blocks);
+    return PE_OK;
+}
+
+int pe_world_set_colouring(pe_world* w, int32_t mode) {
+    if (int rc = check_world(w)) return rc;
+    if (mode < 0 || mode > 2 || (mode == 1 && w->n > CLAIM_LDS)) {
+        pe_set_error("pe_world_set_colouring: mode 0, 1 (at most 12288 bodies) or 2");
+        return PE_ERR_INVALID;
+    }
+    w->colouring = mode;
     return PE_OK;
 }
 

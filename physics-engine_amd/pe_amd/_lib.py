@@ -22,7 +22,7 @@ EXPORTS = [
     "pe_world_narrowphase", "pe_world_solve", "pe_world_sync", "pe_world_get_state", "pe_world_set_state",
     "pe_world_get_aabbs", "pe_world_get_pairs", "pe_world_get_contacts", "pe_world_get_impulses", "pe_world_get_solve_order",
     "pe_world_get_axis", "pe_world_set_axis", "pe_world_set_timing", "pe_world_get_stage_times",
-    "pe_world_get_counts", "pe_world_set_solver_blocks", "pe_world_get_records", "pe_world_step_async",
+    "pe_world_get_counts", "pe_world_set_solver_blocks", "pe_world_set_colouring", "pe_world_get_records", "pe_world_step_async",
     "pe_world_get_cache", "pe_world_set_cache", "pe_world_add_joints", "pe_world_add_ignore_pairs",
     "pe_world_set_sleep", "pe_world_remove_body", "pe_world_get_body_status", "pe_world_set_collision_reports",
     "pe_world_get_collision_events", "pe_world_raycast",

@@ -264,6 +264,11 @@ class World:
     def set_solver_blocks(self, blocks: int):
         check(lib().pe_world_set_solver_blocks(self.h, ctypes.c_int32(blocks)))
 
+    def set_colouring(self, mode: int):
+        """0: by body count, 1: one workgroup's claims (at most 12 288 bodies), 2: rounds over the GPU;
+        the colour classes are the same."""
+        check(lib().pe_world_set_colouring(self.h, ctypes.c_int32(mode)))
+
     def records(self):
         """(AABB records [n, 8], collision records [n, 20], solver records [n, 24]) as the last
         integrate stage wrote them: the inputs of the stage entry points (pe_amd.stage)."""

@@ -203,16 +203,19 @@ def test_reference_sample_scene_stages():
     assert contacts > 0
 
 
-def test_large_world_colouring_matches_oracle():
-    """A world beyond the bodies whose colouring claims fit LDS (12 288) is coloured with its
-    claims in global memory: its colour classes equal the oracle's (bit-exact) and the solved
-    velocities stay within 1e-5, step after step in lock-step."""
+@pytest.mark.parametrize("n,mode", [(13000, 0), (6000, 1), (6000, 2)])
+def test_large_world_colouring_matches_oracle(n, mode):
+    """The solver's colouring in one workgroup (claims in LDS) and in rounds over the GPU: a world
+    beyond the one-workgroup limit (12 288 bodies, rounds) and a smaller one both ways give colour
+    classes equal to the oracle's (bit-exact), and the solved velocities stay within 1e-5, step after
+    step in lock-step."""
     import pe_amd
     from pe_amd import scenes
     from pyoracle import OracleWorld
-    sc = scenes.grid_world(13000, seed=77)
+    sc = scenes.grid_world(n, seed=77)
     g, o = pe_amd.World(sc), OracleWorld(sc, 32)
-    print(f"[large colouring] n={sc.n}", flush=True)
+    g.set_colouring(mode)
+    print(f"[large colouring] n={sc.n}, colouring mode {mode}", flush=True)
     checked = 0
     for step in range(95):
         g.integrate(); o.integrate()
@@ -233,5 +236,8 @@ def test_large_world_colouring_matches_oracle():
             assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
             checked += 1
         o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
-    print(f"[large colouring] pairs {len(g.pairs())}, colours {len(cs) - 1}, steps checked {checked}", flush=True)
+    rounds = g.counts()["colour_rounds"]
+    print(f"[large colouring] pairs {len(g.pairs())}, colours {len(cs) - 1}, rounds {rounds}, steps checked {checked}",
+          flush=True)
     assert len(cs) - 1 >= 3
+    assert (rounds > 0) == (mode != 1)
