@@ -61,6 +61,7 @@ struct SapView {
     uint32_t* keys2;
     uint32_t* vals2;
     float* skey;            // [n] sorted keys as floats
+    float* srec;            // [n][8] the AABB records in sorted order (or null: read through vals)
     uint32_t* cand;         // [n + 1] candidate counts -> offsets
     uint32_t* chunk;        // [chunk_cap] pairs per chunk -> offsets
     uint32_t* chunk_hit;    // [chunk_cap] per chunk, bit k: its candidate k is a pair
@@ -112,6 +113,20 @@ __device__ __forceinline__ void sap_ld_rec(const float* src, float (&dst)[8]) {
     dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
     dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
 }
+// The record of the body at sorted position q: from the sorted copy (contiguous loads along a run
+// of candidates) when there is one.
+__device__ __forceinline__ void sap_ld_sorted(const SapView& s, int q, float (&dst)[8]) {
+    if (s.srec) sap_ld_rec(s.srec + (size_t)q * 8, dst);
+    else sap_ld_rec(s.rec + (size_t)s.vals[q] * 8, dst);
+}
+// sap_pair_allowed for the bodies at sorted positions qa and qb (their indices are loaded only
+// for the ignore-pair search).
+__device__ __forceinline__ bool sap_pair_allowed_sorted(const SapView& s, const float (&A)[8], const float (&B)[8],
+                                                        int qa, int qb) {
+    if (!sap_pair_ok(A, B)) return false;
+    if (A[7] != COLL_PARTNERS || B[7] != COLL_PARTNERS) return true;
+    return sap_pair_allowed(s, A, B, (int)s.vals[qa], (int)s.vals[qb]);
+}
 
 // Sets the axis of this sweep and clears the overflow bits a previous stage call left (the
 // single-world pipeline keeps both in its counters and does not launch this).
@@ -137,6 +152,12 @@ static __global__ void k_sap_ranges(SapView s) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < s.n; q += gridDim.x * blockDim.x) {
         const int i = (int)s.vals[q];
         s.skey[q] = s.mn(ax, i) + 0.0f;
+        if (s.srec) {
+            const float4* src = reinterpret_cast<const float4*>(s.rec + (size_t)i * 8);
+            float4* dst = reinterpret_cast<float4*>(s.srec + (size_t)q * 8);
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
     }
 }
 static __global__ void k_sap_cand(SapView s) {
@@ -194,18 +215,17 @@ static __global__ void k_sap_chunks(SapView s) {
         } else {
             uint32_t cntv = 0, hit = 0;
             const uint32_t t0 = t;
-            int cur = q, ci = (int)s.vals[q];
+            int cur = q;
             float A[8], B[8];
-            sap_ld_rec(s.rec + (size_t)ci * 8, A);   // the run body's record stays in registers
+            sap_ld_sorted(s, cur, A);   // the run body's record stays in registers
             for (; t < tend; t++) {
                 while (s.cand[cur + 1] <= t) {
                     cur++;
-                    ci = (int)s.vals[cur];
-                    sap_ld_rec(s.rec + (size_t)ci * 8, A);
+                    sap_ld_sorted(s, cur, A);
                 }
-                const int j = (int)s.vals[cur + 1 + (int)(t - s.cand[cur])];
-                sap_ld_rec(s.rec + (size_t)j * 8, B);
-                if (sap_pair_allowed(s, A, B, ci, j)) {
+                const int qj = cur + 1 + (int)(t - s.cand[cur]);
+                sap_ld_sorted(s, qj, B);
+                if (sap_pair_allowed_sorted(s, A, B, cur, qj)) {
                     cntv++;
                     hit |= 1u << (t - t0);
                 }

@@ -851,6 +851,7 @@ int pe_world_create(const pe_scene_desc* sc, const pe_world_params* prm, int32_t
     rc |= w->alloc(&sp.keys2, n);
     rc |= w->alloc(&sp.vals2, n);
     rc |= w->alloc(&sp.skey, n);
+    rc |= w->alloc(&sp.srec, (size_t)8 * n);
     rc |= w->alloc(&sp.cand, n + 1);
     rc |= w->alloc(&sp.hist, 256 * (size_t)sap_radix_blocks(n));
     rc |= w->alloc(&sp.red, SAP_RED_VALUES * (size_t)sap_red_blocks(n));
