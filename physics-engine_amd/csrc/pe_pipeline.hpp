@@ -47,7 +47,8 @@ template <int N4> __device__ __forceinline__ void st_rec(float* dst, const float
 
 // Writes the collision-body, solver-body and AABB records of body i.
 __device__ __forceinline__ void write_records(const BodyView& b, int i, int type, bool kin, const float sp[4], const m33& R, f3 pos,
-                                              float scale, f3 mn, f3 mx, const m33& Iw, float im, f3 v, f3 w, f3 tv, f3 tw) {
+                                              float scale, f3 mn, f3 mx, const m33& Iw, float im, f3 v, f3 w, f3 tv, f3 tw,
+                                              float mu, float e) {
     float c[CBW];
 #pragma unroll
     for (int k = 0; k < 9; k++) c[CB_R + k] = R.m[k];
@@ -56,8 +57,8 @@ __device__ __forceinline__ void write_records(const BodyView& b, int i, int type
     c[CB_SP] = sp[0]; c[CB_SP + 1] = sp[1]; c[CB_SP + 2] = sp[2];
     c[CB_TYPE] = __int_as_float(type);
     c[CB_KIN] = __int_as_float(kin ? 1 : 0);
-    c[CB_MU] = b.at(F_MU, i);
-    c[CB_E] = b.at(F_E, i);
+    c[CB_MU] = mu;
+    c[CB_E] = e;
     st_rec<CBW / 4>(b.cbr(i), c);
     float s[SBW];
     s[SB_TV] = tv.x; s[SB_TV + 1] = tv.y; s[SB_TV + 2] = tv.z;
@@ -119,10 +120,15 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
         flags = st->flags[i];
         if (flags & ST_REMOVED) return;
     }
+    // Every field the step reads, in one round of loads (the mass properties of a kinematic body
+    // are loaded and not used): the loads are not left behind the branches that use them.
     f3 pos = ld3(b, F_POS, i);
     m33 R = ld9(b, F_ROT, i);
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
     f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
+    const m33 L = ld9(b, F_LINV, i);
+    const float ldamp = b.at(F_LD, i), adamp = b.at(F_AD, i), mass = b.at(F_MASS, i), imass = b.at(F_IM, i);
+    const float mu = b.at(F_MU, i), e = b.at(F_E, i);
     m33 Iw;
     float im = 0.0f;
     f3 tv{0, 0, 0}, tw{0, 0, 0};   // kinematic: temp velocities stay zero (rigidbody.cpp:72-80)
@@ -153,15 +159,13 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
             st3(b, F_POS, i, pos);
             st9(b, F_ROT, i, R);
         }
-        const m33 L = ld9(b, F_LINV, i);
         Iw = mmt(mm(R, L), R);
         if (awake) {
             // RigidBody::applyDamping (rigidbody.cpp:201-205).  With zero damping the factor is
             // pow(1, dt) = 1 exactly and the product leaves every component as it is, so the power
             // is only evaluated for a damped body.
-            const float ld = b.at(F_LD, i), ad = b.at(F_AD, i);
-            if (ld != 0.0f) v = mul(v, powf(1.0f - ld, dt));
-            if (ad != 0.0f) w = mul(w, powf(1.0f - ad, dt));
+            if (ldamp != 0.0f) v = mul(v, powf(1.0f - ldamp, dt));
+            if (adamp != 0.0f) w = mul(w, powf(1.0f - adamp, dt));
         }
         if constexpr (STATUS) {
             if (awake) {
@@ -182,9 +186,8 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
             st->dcnt[i] = 0;
         }
         // RigidBody::applyForce with force = gravity * mass (rigidbody.cpp:193-199)
-        const float m = b.at(F_MASS, i);
-        im = b.at(F_IM, i);
-        const f3 f{g[0] * m, g[1] * m, g[2] * m};
+        im = imass;
+        const f3 f{g[0] * mass, g[1] * mass, g[2] * mass};
         v = add(v, mul(mul(f, im), dt));
         st3(b, F_V, i, v);
         st3(b, F_W, i, w);
@@ -208,7 +211,7 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
     }
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
-    write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw);
+    write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw, mu, e);
 }
 
 // AABB, scale, world inverse inertia and records of the current pose (RigidBody::setTransform).
@@ -229,7 +232,7 @@ __device__ __forceinline__ void refresh_body(const BodyView& b, int i) {
     else Iw = mmt(mm(R, ld9(b, F_LINV, i)), R);
     const f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
     write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, kin ? 0.0f : b.at(F_IM, i), v, w,
-                  kin ? f3{0, 0, 0} : v, kin ? f3{0, 0, 0} : w);
+                  kin ? f3{0, 0, 0} : v, kin ? f3{0, 0, 0} : w, b.at(F_MU, i), b.at(F_E, i));
 }
 
 // ------------------------------------------------------------------ stage 1
