@@ -109,7 +109,7 @@ constexpr int PHASES = 8;   // start, integrate, sort, sweep, narrow, setup, col
 __host__ __device__ inline int np2(int n) { int p = 1; while (p < n) p <<= 1; return p; }
 __host__ inline size_t broad_lds_bytes(int nmax) {
     const int P = np2(std::max(nmax, 2));
-    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/;
+    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/ + (size_t)nmax * 4 /*smax*/;
 }
 // LDS of K3 for a group of GB bodies and GP pairs.  The preparation uses per body its claim and
 // inverse mass (8 B) and per pair its local bodies, global slot, claim priority, order slot,
@@ -160,19 +160,24 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
     uint64_t* key = (uint64_t*)lds;
     float* skey = (float*)(key + P2);
     uint32_t* cand = (uint32_t*)(skey + nmax);
+    float* smax = (float*)(cand + nmax + 1);   // per body (unsorted): aabb_max[axis]
     const int axis = d.st[w].axis;
     const int prev_np = d.st[w].prev_npairs;
 
     PE_PHASE(0);
     // ---- stage 0: integrate (world.cpp:35-127)
-    for (int i = tid; i < n; i += BLOCK) integrate_body(b, off + i, d.dt, d.g);
+    // with each body's sort key (aabb_min[axis] + 0, index) into LDS, from the AABB it just made
+    for (int i = tid; i < n; i += BLOCK) {
+        const Box3 a = integrate_body(b, off + i, d.dt, d.g);
+        const float m = (axis == 0 ? a.mn.x : (axis == 1 ? a.mn.y : a.mn.z)) + 0.0f;
+        key[i] = ((uint64_t)float_key(m) << 32) | (uint32_t)i;
+        smax[i] = axis == 0 ? a.mx.x : (axis == 1 ? a.mx.y : a.mx.z);
+    }
+    for (int i = n + tid; i < P2; i += BLOCK) key[i] = KEY_PAD;
     __syncthreads();
     PE_PHASE(1);
 
     // ---- stage 1: sweep and prune (broad_phase_sweep_and_prune.cpp:6-41)
-    for (int i = tid; i < P2; i += BLOCK)
-        key[i] = i < n ? (((uint64_t)float_key(b.at(F_AMIN + axis, off + i) + 0.0f) << 32) | (uint32_t)i) : KEY_PAD;
-    __syncthreads();
     for (int k = 2; k <= P2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = tid; i < P2; i += BLOCK) {
@@ -186,7 +191,7 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
             __syncthreads();
         }
     }
-    for (int s = tid; s < n; s += BLOCK) skey[s] = b.at(F_AMIN + axis, off + (int)(uint32_t)key[s]) + 0.0f;
+    for (int s = tid; s < n; s += BLOCK) skey[s] = key_float((uint32_t)(key[s] >> 32));
     __syncthreads();
     PE_PHASE(2);
     // candidate run of sorted position s: j in (s, first skey > max_i)
@@ -195,8 +200,7 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
         const int s = base + tid;
         int c = 0;
         if (s < n) {
-            const int i = (int)(uint32_t)key[s];
-            const float mx = b.at(F_AMAX + axis, off + i);
+            const float mx = smax[(int)(uint32_t)key[s]];
             int lo = s + 1, hi = n;
             while (lo < hi) { const int mid = (lo + hi) >> 1; if (skey[mid] > mx) hi = mid; else lo = mid + 1; }
             c = lo - s - 1;

@@ -110,15 +110,21 @@ __device__ __forceinline__ void retire_body(const BodyView& b, int i) {
 // threshold scaled by (static + 1) / (static + dynamic + 1), and is neither moved nor damped; an
 // awake body lives dt longer (and leaves the world at its lifetime), is moved and damped, and falls
 // asleep, velocities zeroed, after tsleep seconds below the scaled thresholds.
+// Returns the body's new AABB (min +inf, max -inf for a body that has left the world).
+struct Box3 {
+    f3 mn, mx;
+};
 template <bool STATUS = false>
-__device__ __forceinline__ void integrate_body(const BodyView& b, int i, float dt, const float g[3],
+__device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float dt, const float g[3],
                                                const StatusView* st = nullptr) {
+    const float inf = __int_as_float(0x7f800000);
+    const Box3 gone{f3{inf, inf, inf}, f3{-inf, -inf, -inf}};
     const int type = b.iat(I_TYPE, i);
     const bool kin = b.iat(I_KIN, i) != 0;
     int flags = 0;
     if constexpr (STATUS) {
         flags = st->flags[i];
-        if (flags & ST_REMOVED) return;
+        if (flags & ST_REMOVED) return gone;
     }
     // Every field the step reads, in one round of loads (the mass properties of a kinematic body
     // are loaded and not used): the loads are not left behind the branches that use them.
@@ -150,7 +156,7 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
                 if (lived >= st->life[i]) {
                     st->flags[i] = flags | ST_REMOVED;
                     retire_body(b, i);
-                    return;
+                    return gone;
                 }
             }
         }
@@ -212,6 +218,7 @@ __device__ __forceinline__ void integrate_body(const BodyView& b, int i, float d
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
     write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw, mu, e);
+    return Box3{mn, mx};
 }
 
 // AABB, scale, world inverse inertia and records of the current pose (RigidBody::setTransform).
