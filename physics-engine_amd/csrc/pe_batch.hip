@@ -107,9 +107,16 @@ constexpr int PHASES = 8;   // start, integrate, sort, sweep, narrow, setup, col
 
 // LDS layout for a class whose largest world has nmax bodies.
 __host__ __device__ inline int np2(int n) { int p = 1; while (p < n) p <<= 1; return p; }
+// K1's LDS: sort keys, sorted keys as floats, candidate counts, axis maxima, then (for worlds
+// small enough: up to 64 KiB in all) the bodies' AABB records.
+__host__ __device__ inline size_t broad_lds_head(int nmax) {
+    const int P = np2(nmax > 2 ? nmax : 2);
+    const size_t head = (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/ + (size_t)nmax * 4 /*smax*/;
+    return (head + 15) & ~(size_t)15;
+}
+__host__ __device__ inline bool broad_rec_in_lds(int nmax) { return broad_lds_head(nmax) + (size_t)nmax * 32 <= 64 * 1024; }
 __host__ inline size_t broad_lds_bytes(int nmax) {
-    const int P = np2(std::max(nmax, 2));
-    return (size_t)P * 8 + (size_t)nmax * 4 /*skey*/ + (size_t)(nmax + 1) * 4 /*cand*/ + (size_t)nmax * 4 /*smax*/;
+    return broad_lds_head(nmax) + (broad_rec_in_lds(nmax) ? (size_t)nmax * 32 : 0);
 }
 // LDS of K3 for a group of GB bodies and GP pairs.  The preparation uses per body its claim and
 // inverse mass (8 B) and per pair its local bodies, global slot, claim priority, order slot,
@@ -143,6 +150,16 @@ __host__ inline size_t solve_lds_bytes(int nmax, int group_bodies) {
     const int g = std::max(std::max(nmax, group_bodies), 1);
     return std::min(SOLVE_LDS_CAP, PrepLayout(g, 2 * g).end);
 }
+// The AABB record of local body i: from LDS when the world's records are there.
+__device__ __forceinline__ void lds_rec(const float4* lrec, const BodyView& b, int off, int i, float (&r)[8]) {
+    if (!lrec) {
+        ld_rec<2>(b.abr(off + i), r);
+        return;
+    }
+    const float4 a = lrec[2 * i], c = lrec[2 * i + 1];
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+    r[4] = c.x; r[5] = c.y; r[6] = c.z; r[7] = c.w;
+}
 // K1: integrate + sweep-and-prune for one world per workgroup; appends the world's pairs to
 // the narrowphase work list.
 template <int BLOCK>
@@ -161,6 +178,9 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
     float* skey = (float*)(key + P2);
     uint32_t* cand = (uint32_t*)(skey + nmax);
     float* smax = (float*)(cand + nmax + 1);   // per body (unsorted): aabb_max[axis]
+    // per body (unsorted): its AABB record, as integrate writes it to the body's record (null: the
+    // sweep reads the records from the body store)
+    float4* lrec = broad_rec_in_lds(nmax) ? (float4*)(lds + broad_lds_head(nmax)) : nullptr;
     const int axis = d.st[w].axis;
     const int prev_np = d.st[w].prev_npairs;
 
@@ -172,6 +192,10 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
         const float m = (axis == 0 ? a.mn.x : (axis == 1 ? a.mn.y : a.mn.z)) + 0.0f;
         key[i] = ((uint64_t)float_key(m) << 32) | (uint32_t)i;
         smax[i] = axis == 0 ? a.mx.x : (axis == 1 ? a.mx.y : a.mx.z);
+        if (lrec) {
+            lrec[2 * i] = make_float4(a.mn.x, a.mn.y, a.mn.z, a.kin);
+            lrec[2 * i + 1] = make_float4(a.mx.x, a.mx.y, a.mx.z, a.coll);
+        }
     }
     for (int i = n + tid; i < P2; i += BLOCK) key[i] = KEY_PAD;
     __syncthreads();
@@ -236,10 +260,10 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
             if (s != si) {
                 si = s;
                 i = (int)(uint32_t)key[s];
-                ld_rec<2>(b.abr(off + i), A);
+                lds_rec(lrec, b, off, i, A);
             }
             const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
-            ld_rec<2>(b.abr(off + j), Bj);
+            lds_rec(lrec, b, off, j, Bj);
             if (bp_valid_rec(A, Bj)) {
                 cnt++;
                 if (masked) hit |= 1ull << (t - t0);
@@ -271,10 +295,10 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
                 if (s != si) {
                     si = s;
                     i = (int)(uint32_t)key[s];
-                    ld_rec<2>(b.abr(off + i), A);
+                    lds_rec(lrec, b, off, i, A);
                 }
                 const int j = (int)(uint32_t)key[s + 1 + (int)(t - cand[s])];
-                ld_rec<2>(b.abr(off + j), Bj);
+                lds_rec(lrec, b, off, j, Bj);
                 if (bp_valid_rec(A, Bj)) {
                     if (out < wd.pair_cap) { d.ps.pa[poff + out] = off + i; d.ps.pb[poff + out] = off + j; }
                     out++;

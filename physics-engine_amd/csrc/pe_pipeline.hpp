@@ -110,15 +110,17 @@ __device__ __forceinline__ void retire_body(const BodyView& b, int i) {
 // threshold scaled by (static + 1) / (static + dynamic + 1), and is neither moved nor damped; an
 // awake body lives dt longer (and leaves the world at its lifetime), is moved and damped, and falls
 // asleep, velocities zeroed, after tsleep seconds below the scaled thresholds.
-// Returns the body's new AABB (min +inf, max -inf for a body that has left the world).
+// Returns the body's new AABB record (min, kinematic flag, max, collision flag; min +inf and max
+// -inf for a body that has left the world), as written to the body's AABB record.
 struct Box3 {
     f3 mn, mx;
+    float kin, coll;
 };
 template <bool STATUS = false>
 __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float dt, const float g[3],
                                                const StatusView* st = nullptr) {
     const float inf = __int_as_float(0x7f800000);
-    const Box3 gone{f3{inf, inf, inf}, f3{-inf, -inf, -inf}};
+    const Box3 gone{f3{inf, inf, inf}, f3{-inf, -inf, -inf}, 0.0f, COLL_IGNORE};
     const int type = b.iat(I_TYPE, i);
     const bool kin = b.iat(I_KIN, i) != 0;
     int flags = 0;
@@ -218,7 +220,7 @@ __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float d
     st3(b, F_AMIN, i, mn);
     st3(b, F_AMAX, i, mx);
     write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw, mu, e);
-    return Box3{mn, mx};
+    return Box3{mn, mx, kin ? 1.0f : 0.0f, sp[3]};
 }
 
 // AABB, scale, world inverse inertia and records of the current pose (RigidBody::setTransform).
