@@ -579,6 +579,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
         if (have) gs_prefetch(d.b, d.ps, d.order[q], G);
     };
     if (ncol > 0) fetch(0);
+    // The last colours of at most GS_THREADS pairs each: every pair of such a colour falls to a thread
+    // of workgroup 0, so between two of them a workgroup barrier orders the updates (the grid barrier
+    // after the last one publishes them to the next iteration).
+    int ctail = ncol;
+    while (ctail > 0 && d.color_start[ctail] - d.color_start[ctail - 1] <= GS_THREADS) ctail--;
     for (int it = 0; it < d.iterations; it++) {
         for (int c = 0; c < ncol; c++) {
             const int c0 = d.color_start[c], c1 = d.color_start[c + 1];
@@ -591,7 +596,8 @@ __global__ void __launch_bounds__(GS_THREADS) k_gs_grid(Dev d) {
             }
             if (c + 1 < ncol) fetch(c + 1);
             else if (it + 1 < d.iterations) fetch(0);
-            grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);   // later colours read these velocities
+            if (c >= ctail && c + 1 < ncol) __syncthreads();
+            else grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);   // later colours read these velocities
         }
         // the joints after the contacts, colour by colour (sequential_impulse_solver.cpp:66-68)
         for (int c = 0; c < d.njcol; c++) {
