@@ -490,37 +490,63 @@ __device__ int jp_try(const Dev& d, int p) {
 // Rounds until every pair has its colour (each round colours at least the uncoloured pair of lowest
 // priority, so the loop ends), then the colour starts and the solve order, colour by colour.  First-fit
 // colours are dense (a pair of colour c has lower neighbours of every colour below c), so the colour
-// count is the first empty bin.  Counters are touched once per workgroup (per-workgroup flags and
-// LDS histograms), not per thread.  Launched cooperatively (all workgroups resident).
+// count is the first empty bin.  The colour counts go through LDS histograms, one global add per
+// workgroup and bin.  Launched cooperatively (all workgroups resident).
 constexpr int JP_BINS = 64;
 __global__ void __launch_bounds__(JP_THREADS) k_jp_colour(Dev d) {
     __shared__ int s_any;
     __shared__ int s_hist[JP_BINS], s_base[JP_BINS];
     const int nact = d.cnt->nact;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gsz = gridDim.x * blockDim.x;
-    for (int r = 0;; r++) {
-        // slot (r + 1) % 3 was last read after round r - 2's barrier, which every thread passed
-        // before arriving at round r - 1's
-        if (gtid == 0) d.cnt->jp_rem[(r + 1) % 3] = 0;
-        if (threadIdx.x == 0) s_any = 0;
-        __syncthreads();
-        bool left = false;
-        for (int k = gtid; k < nact; k += gsz) {
-            const int p = d.act[k];
-            if (jp_col_load(d.jp_col, p) >= 0) continue;
-            const int c = jp_try(d, p);
-            if (c >= 0) __hip_atomic_store(&d.jp_col[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else left = true;
+    // Round 0 takes every active pair, each later round the pairs the round before left uncoloured
+    // (lists alternating in act + pcap and act + 2 pcap; the count of round r's list in
+    // jp_rem[r % 3], cleared two rounds ahead, after every thread has read it).  Once at most
+    // JP_THREADS pairs are left, workgroup 0 finishes them alone, with workgroup barriers.
+    const int32_t* cur = d.act;
+    int ncur = nact;
+    int r = 0;
+    for (;; r++) {
+        int32_t* nxt = d.act + (size_t)(1 + (r & 1)) * d.pcap;
+        if (gtid == 0) d.cnt->jp_rem[(r + 2) % 3] = 0;
+        const int n_iter = (ncur + gsz - 1) / gsz;
+        for (int it = 0; it < n_iter; it++) {
+            const int k = it * gsz + gtid;
+            int p = -1, c = 0;
+            if (k < ncur) {
+                p = cur[k];
+                c = jp_try(d, p);
+                if (c >= 0) __hip_atomic_store(&d.jp_col[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const int slot = wave_append(p >= 0 && c < 0, (int*)&d.cnt->jp_rem[(r + 1) % 3]);
+            if (p >= 0 && c < 0) nxt[slot] = p;
         }
-        if (left) s_any = 1;
-        __syncthreads();
-        if (threadIdx.x == 0 && s_any) atomicAdd(&d.cnt->jp_rem[r % 3], 1u);
         grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);
-        if (__hip_atomic_load(&d.cnt->jp_rem[r % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            if (gtid == 0) d.cnt->jp_rounds = r + 1;
-            break;
-        }
+        ncur = (int)__hip_atomic_load(&d.cnt->jp_rem[(r + 1) % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cur = nxt;
+        if (ncur <= JP_THREADS) break;
     }
+    r++;
+    if (blockIdx.x == 0) {
+        const int p = (int)threadIdx.x < ncur ? cur[threadIdx.x] : -1;
+        bool done = p < 0;
+        for (;; r++) {
+            if (threadIdx.x == 0) s_any = 0;
+            __syncthreads();
+            if (!done) {
+                const int c = jp_try(d, p);
+                if (c >= 0) {
+                    __hip_atomic_store(&d.jp_col[p], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    done = true;
+                } else {
+                    s_any = 1;
+                }
+            }
+            __syncthreads();
+            if (!s_any) break;
+        }
+        if (threadIdx.x == 0) d.cnt->jp_rounds = ncur > 0 ? r + 1 : r;
+    }
+    grid_barrier(&d.cnt->bar_count, &d.cnt->bar_gen, gridDim.x);   // every colour is set
     // pairs per colour: LDS bins per workgroup, one global add per workgroup and bin
     for (int c = threadIdx.x; c < JP_BINS; c += blockDim.x) s_hist[c] = 0;
     __syncthreads();
