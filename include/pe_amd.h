@@ -290,7 +290,7 @@ int pe_world_get_stage_times(pe_world* w, double* ms /*4*/);
  * colour's pair per thread; clamped to the co-resident limit).  No counterpart in the reference; results
  * do not depend on it. */
 int pe_world_set_solver_blocks(pe_world* w, int32_t blocks);
-/* How the solver's colouring runs (0 = by body count: one workgroup's claims in LDS up to 7 500
+/* How the solver's colouring runs (0 = by body count: one workgroup's claims in LDS up to 3 000
  * bodies, rounds over the GPU beyond; 1 = one workgroup, at most 12 288 bodies; 2 = rounds).  Both
  * give the same colour classes.  No counterpart in the reference; results do not depend on it. */
 int pe_world_set_colouring(pe_world* w, int32_t mode);

@@ -663,9 +663,9 @@ __global__ void __launch_bounds__(256) k_count_points(Dev d) {
 // 1.72 / 1.82; 50k shape mix at step 280: 16 / 32 / 64 workgroups 7.29 / 7.06 / 7.08.
 static int default_gs_blocks(int n, int max_blocks) { return std::min(max_blocks, std::max(4, std::min(32, n / 300))); }
 // Bodies beyond which the colouring runs in rounds over the GPU rather than in one workgroup (box
-// rain at rest, ms per step with one workgroup / rounds: 5 000 bodies 1.10 / 1.22, 10 000 bodies
-// 1.97 / 1.72).
-constexpr int COLOUR_ROUNDS_FROM = 7500;
+// rain at rest, ms per step with one workgroup / rounds: 1 000 bodies 0.42 / 0.46, 2 500 bodies
+// 0.67 / 0.69, 4 000 bodies 0.89 / 0.87, 6 000 bodies 1.26 / 1.14, 8 000 bodies 1.61 / 1.33).
+constexpr int COLOUR_ROUNDS_FROM = 3000;
 
 struct pe_world {
     Dev d{};
