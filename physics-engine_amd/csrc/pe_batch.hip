@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
     // ---- stage 0: integrate (world.cpp:35-127)
     // with each body's sort key (aabb_min[axis] + 0, index) into LDS, from the AABB it just made
     for (int i = tid; i < n; i += BLOCK) {
-        const Box3 a = integrate_body(b, off + i, d.dt, d.g);
+        const Box3 a = integrate_body<false, true>(b, off + i, d.dt, d.g);
         const float m = (axis == 0 ? a.mn.x : (axis == 1 ? a.mn.y : a.mn.z)) + 0.0f;
         key[i] = ((uint64_t)float_key(m) << 32) | (uint32_t)i;
         smax[i] = axis == 0 ? a.mx.x : (axis == 1 ? a.mx.y : a.mx.z);
@@ -310,9 +310,11 @@ __global__ void __launch_bounds__(BLOCK) k_world_broad(BatchDev d, const int32_t
     {
         double acc[6] = {0, 0, 0, 0, 0, 0};
         for (int i = tid; i < n; i += BLOCK) {
+            float r[8];
+            lds_rec(lrec, b, off, i, r);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float c = (b.at(F_AMIN + k, off + i) + b.at(F_AMAX + k, off + i)) * 0.5f;
+                const float c = (r[k] + r[4 + k]) * 0.5f;
                 acc[k] += (double)c;
                 acc[3 + k] += (double)(c * c);
             }
@@ -465,7 +467,7 @@ __device__ void solve_world_global(const BatchDev& d, int w, int* claim, int* ws
             __syncthreads();
         }
     }
-    for (int i = tid; i < n; i += BLOCK) sync_body(b, off + i);
+    for (int i = tid; i < n; i += BLOCK) sync_body_rec(b, off + i);
     __syncthreads();
 }
 
@@ -689,11 +691,14 @@ __device__ __forceinline__ void group_sweep(const BatchDev& d, const int32_t* __
         else iterate_pair_lds(L, lp, rows, cur, nxt);
     });
     PE_PHASE(7);
+    // the solved velocities into the solver records' v, w (floats 17..22; float 16 is the last
+    // inverse-inertia entry, kept as it is)
     for (int i = tid; i < GB; i += BLOCK) {
-        const float4 q0 = L.bs[4 * i], q1 = L.bs[4 * i + 1];
+        const float4 q0 = L.bs[4 * i], q1 = L.bs[4 * i + 1], q3 = L.bs[4 * i + 3];
         if (q1.z < 0.0f) continue;
-        st3(b, F_V, B0 + i, f3{q0.x, q0.y, q0.z});
-        st3(b, F_W, B0 + i, f3{q0.w, q1.x, q1.y});
+        float4* sr = reinterpret_cast<float4*>(b.sbr(B0 + i));
+        sr[4] = make_float4(q3.w, q0.x, q0.y, q0.z);
+        sr[5] = make_float4(q0.w, q1.x, q1.y, 0.0f);
     }
 }
 
@@ -1324,17 +1329,20 @@ int pe_batch_get_world_state(pe_batch* B, int32_t w, float* pos, float* rot, flo
     if (!B || w < 0 || w >= B->nworlds) { pe_set_error("world index out of range"); return PE_ERR_INVALID; }
     PE_HIP(hipStreamSynchronize(B->stream));
     const WorldDesc& wd = B->hdesc[w];
-    std::vector<float> tmp(wd.n);
-    auto get = [&](int f0, int nf, float* out) -> int {
-        if (!out) return PE_OK;
-        for (int k = 0; k < nf; k++) {
-            PE_HIP(hipMemcpy(tmp.data(), B->d.b.f + (size_t)(f0 + k) * B->stride + wd.body_off, wd.n * 4, hipMemcpyDeviceToHost));
-            for (int i = 0; i < wd.n; i++) out[(size_t)i * nf + k] = tmp[i];
+    // a batched world's pose and velocities live in its collision and solver records
+    std::vector<float> cb((size_t)wd.n * CBW), sb((size_t)wd.n * SBW);
+    PE_HIP(hipMemcpy(cb.data(), B->d.b.cb + (size_t)wd.body_off * CBW, cb.size() * 4, hipMemcpyDeviceToHost));
+    PE_HIP(hipMemcpy(sb.data(), B->d.b.sb + (size_t)wd.body_off * SBW, sb.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < wd.n; i++) {
+        const float* c = cb.data() + (size_t)i * CBW;
+        const float* s = sb.data() + (size_t)i * SBW;
+        for (int k = 0; k < 3; k++) {
+            if (pos) pos[(size_t)i * 3 + k] = c[CB_P + k];
+            if (v) v[(size_t)i * 3 + k] = s[SB_V + k];
+            if (av) av[(size_t)i * 3 + k] = s[SB_W + k];
         }
-        return PE_OK;
-    };
-    int rc;
-    if ((rc = get(F_POS, 3, pos)) || (rc = get(F_ROT, 9, rot)) || (rc = get(F_V, 3, v)) || (rc = get(F_W, 3, av))) return rc;
+        if (rot) for (int k = 0; k < 9; k++) rot[(size_t)i * 9 + k] = c[CB_R + k];
+    }
     return PE_OK;
 }
 
@@ -1487,15 +1495,17 @@ __global__ void k_admit(BodyView b, const float* __restrict__ stage, int off, in
         refresh_body(b, off + i);
     }
 }
-// Packs pos/rot/v/w of bodies [off, off+n) into out[NRESULT][n], then the world's capacity-error
-// bits (int) into out[NRESULT * n].
+// Packs pos/rot/v/w of bodies [off, off+n) (from their records) into out[NRESULT][n], then the
+// world's capacity-error bits (int) into out[NRESULT * n].
 __global__ void k_gather(BodyView b, const WorldState* __restrict__ st, int slot, float* __restrict__ out, int off, int n) {
     if (blockIdx.x == 0 && threadIdx.x == 0) out[(size_t)NRESULT * n] = __int_as_float(st[slot].overflow);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        for (int k = 0; k < 3; k++) out[(size_t)k * n + i] = b.at(F_POS + k, off + i);
-        for (int k = 0; k < 9; k++) out[(size_t)(3 + k) * n + i] = b.at(F_ROT + k, off + i);
-        for (int k = 0; k < 3; k++) out[(size_t)(12 + k) * n + i] = b.at(F_V + k, off + i);
-        for (int k = 0; k < 3; k++) out[(size_t)(15 + k) * n + i] = b.at(F_W + k, off + i);
+        const float* c = b.cbr(off + i);   // the records hold a served world's pose and velocities
+        const float* s = b.sbr(off + i);
+        for (int k = 0; k < 3; k++) out[(size_t)k * n + i] = c[CB_P + k];
+        for (int k = 0; k < 9; k++) out[(size_t)(3 + k) * n + i] = c[CB_R + k];
+        for (int k = 0; k < 3; k++) out[(size_t)(12 + k) * n + i] = s[SB_V + k];
+        for (int k = 0; k < 3; k++) out[(size_t)(15 + k) * n + i] = s[SB_W + k];
     }
 }
 __global__ void k_reset_state(WorldState* st, int w) {

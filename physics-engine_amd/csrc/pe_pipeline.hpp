@@ -112,11 +112,14 @@ __device__ __forceinline__ void retire_body(const BodyView& b, int i) {
 // asleep, velocities zeroed, after tsleep seconds below the scaled thresholds.
 // Returns the body's new AABB record (min, kinematic flag, max, collision flag; min +inf and max
 // -inf for a body that has left the world), as written to the body's AABB record.
+// REC: the records hold the body's pose, velocities and AABB between steps (the batched and served
+// worlds): they are read from and written to the records only, not to the field-major store, whose
+// copies of those fields then keep the values the world was created with.
 struct Box3 {
     f3 mn, mx;
     float kin, coll;
 };
-template <bool STATUS = false>
+template <bool STATUS = false, bool REC = false>
 __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float dt, const float g[3],
                                                const StatusView* st = nullptr) {
     const float inf = __int_as_float(0x7f800000);
@@ -130,10 +133,24 @@ __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float d
     }
     // Every field the step reads, in one round of loads (the mass properties of a kinematic body
     // are loaded and not used): the loads are not left behind the branches that use them.
-    f3 pos = ld3(b, F_POS, i);
-    m33 R = ld9(b, F_ROT, i);
+    f3 pos, v, w;
+    m33 R;
+    if constexpr (REC) {
+        const float4* c = reinterpret_cast<const float4*>(b.cbr(i));   // basis, origin: floats 0..11
+        const float4 c0 = c[0], c1 = c[1], c2 = c[2];
+        R = m33{{c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x}};
+        pos = f3{c2.y, c2.z, c2.w};
+        const float4* q = reinterpret_cast<const float4*>(b.sbr(i));   // v, w: floats 17..22
+        const float4 q4 = q[4], q5 = q[5];
+        v = f3{q4.y, q4.z, q4.w};
+        w = f3{q5.x, q5.y, q5.z};
+    } else {
+        pos = ld3(b, F_POS, i);
+        R = ld9(b, F_ROT, i);
+        v = ld3(b, F_V, i);
+        w = ld3(b, F_W, i);
+    }
     const float sp[4] = {b.at(F_SP, i), b.at(F_SP + 1, i), b.at(F_SP + 2, i), b.at(F_SP + 3, i)};
-    f3 v = ld3(b, F_V, i), w = ld3(b, F_W, i);
     const m33 L = ld9(b, F_LINV, i);
     const float ldamp = b.at(F_LD, i), adamp = b.at(F_AD, i), mass = b.at(F_MASS, i), imass = b.at(F_IM, i);
     const float mu = b.at(F_MU, i), e = b.at(F_E, i);
@@ -164,8 +181,10 @@ __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float d
         }
         if (awake) {
             integrate_pose(pos, R, v, w, dt);
-            st3(b, F_POS, i, pos);
-            st9(b, F_ROT, i, R);
+            if constexpr (!REC) {
+                st3(b, F_POS, i, pos);
+                st9(b, F_ROT, i, R);
+            }
         }
         Iw = mmt(mm(R, L), R);
         if (awake) {
@@ -197,8 +216,10 @@ __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float d
         im = imass;
         const f3 f{g[0] * mass, g[1] * mass, g[2] * mass};
         v = add(v, mul(mul(f, im), dt));
-        st3(b, F_V, i, v);
-        st3(b, F_W, i, w);
+        if constexpr (!REC) {
+            st3(b, F_V, i, v);
+            st3(b, F_W, i, w);
+        }
         tv = v;
         tw = w;
     } else {
@@ -210,15 +231,24 @@ __device__ __forceinline__ Box3 integrate_body(const BodyView& b, int i, float d
         shape_aabb(type, sp, R, pos, mn, mx);
     } else if (kin) {
         // a kinematic body's pose does not change in a step, so neither does its mesh AABB
-        mn = ld3(b, F_AMIN, i);
-        mx = ld3(b, F_AMAX, i);
+        if constexpr (REC) {
+            const float4* a = reinterpret_cast<const float4*>(b.abr(i));
+            const float4 a0 = a[0], a1 = a[1];
+            mn = f3{a0.x, a0.y, a0.z};
+            mx = f3{a1.x, a1.y, a1.z};
+        } else {
+            mn = ld3(b, F_AMIN, i);
+            mx = ld3(b, F_AMAX, i);
+        }
     } else if (type == SHAPE_COMPOUND) {
         compound_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     } else {
         mesh_aabb(b.ms, __float_as_int(sp[0]), R, pos, mn, mx);
     }
-    st3(b, F_AMIN, i, mn);
-    st3(b, F_AMAX, i, mx);
+    if constexpr (!REC) {
+        st3(b, F_AMIN, i, mn);
+        st3(b, F_AMAX, i, mx);
+    }
     write_records(b, i, type, kin, sp, R, pos, aabb_scale(mn, mx, pos), mn, mx, Iw, im, v, w, tv, tw, mu, e);
     return Box3{mn, mx, kin ? 1.0f : 0.0f, sp[3]};
 }
@@ -805,6 +835,15 @@ __device__ __forceinline__ void sync_body(const BodyView& b, int i) {
     const float* s = b.sbr(i);
     st3(b, F_V, i, f3{s[SB_TV], s[SB_TV + 1], s[SB_TV + 2]});
     st3(b, F_W, i, f3{s[SB_TW], s[SB_TW + 1], s[SB_TW + 2]});
+}
+// The same for a world whose records hold its velocities (integrate_body<.., true>): into the
+// solver record's v, w.
+__device__ __forceinline__ void sync_body_rec(const BodyView& b, int i) {
+    if (b.iat(I_KIN, i)) return;
+    float* s = b.sbr(i);
+    const f3 tv{s[SB_TV], s[SB_TV + 1], s[SB_TV + 2]}, tw{s[SB_TW], s[SB_TW + 1], s[SB_TW + 2]};
+    s[SB_V] = tv.x; s[SB_V + 1] = tv.y; s[SB_V + 2] = tv.z;
+    s[SB_W] = tw.x; s[SB_W + 1] = tw.y; s[SB_W + 2] = tw.z;
 }
 
 // ------------------------------------------------------------------ constraint-graph colouring
