@@ -78,8 +78,8 @@ struct BatchDev {
     int32_t* nl_limit;   // [total pair cap] per listed pair: first slot of its world that starts fresh
     int32_t* nl_shape;   // [total pair cap] per segment, list entries whose pair involves a cylinder or a
                          // mesh: from the front those of k_narrow_shape_list, from the back those of
-                         // k_narrow_sat_list
-    int32_t* nl_count;   // [3 * MAX_CLASSES] per class: listed pairs, shape entries, SAT entries
+                         // k_narrow_sat_list (pair_on_wave)
+    int32_t* nl_count;   // [3 * MAX_CLASSES] per class: listed pairs, shape entries, wave entries
     int32_t* ovf_any;    // [1] OR of the capacity-error bits raised since the host last read it
 };
 constexpr int MAX_CLASSES = 32;
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(128) k_narrow_list(BatchDev d, NarrowSeg g) {
         const int q = g.off + k;
         const int p = d.nl_pair[q];
         const bool simple = pair_is_simple(d.b, d.ps, p);
-        const bool sat = !simple && pair_is_sat(d.b, d.ps, p);
+        const bool sat = !simple && pair_on_wave(d.b, d.ps, p);
         const int slot = wave_append(!simple && !sat, &d.nl_count[3 * g.cls + 1]);
         const int sslot = wave_append(sat, &d.nl_count[3 * g.cls + 2]);
         if (simple) narrow_pair<false>(d.b, d.ps, p, d.nl_limit[q]);
@@ -386,7 +386,8 @@ __device__ __forceinline__ void seg_prefix(const BatchDev& d, const NarrowSegs& 
     }
     __syncthreads();
 }
-// K2b: the listed pairs of every class that involve a cylinder, a convex mesh or a concave mesh.
+// K2b: the listed pairs of every class that involve a cylinder, a convex mesh or a concave mesh
+// and are not wave pairs (a box or sphere with a cylinder).
 // One launch for all classes: these kernels carry large scratch frames, and launching them per
 // class multiplied their launch cost (measured: K2 of 96 shape worlds 2.6 -> 4.7 ms).
 __global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d, NarrowSegs g) {
@@ -399,7 +400,8 @@ __global__ void __launch_bounds__(64) k_narrow_shape_list(BatchDev d, NarrowSegs
     }
 }
 
-// K2c: the listed separating-axis pairs (pair_is_sat) of every class, one wave each.
+// K2c: the listed wave pairs (pair_on_wave: separating-axis pairs, a sphere or cylinder with a
+// mesh) of every class, one wave each.
 __global__ void __launch_bounds__(256) k_narrow_sat_list(BatchDev d, NarrowSegs g) {
     __shared__ int pre[MAX_CLASSES + 1];
     seg_prefix(d, g, 2, pre);

@@ -333,16 +333,18 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter) {
 __device__ __forceinline__ bool pair_is_simple(const BodyView& b, const PairStore& ps, int p) {
     return b.iat(I_TYPE, ps.pa[p]) <= SHAPE_SPHERE && b.iat(I_TYPE, ps.pb[p]) <= SHAPE_SPHERE;
 }
-// True when pair p runs the convex separating-axis routine (and its clipper): cylinder and
-// cylinder, box and convex mesh, convex mesh and convex mesh (edge-pair axes in the thousands),
-// box or convex mesh and concave mesh (one test per overlapped triangle).  Such pairs are solved
-// by a whole wave each.
-__device__ __forceinline__ bool pair_is_sat(const BodyView& b, const PairStore& ps, int p) {
+// True when pair p is solved by a whole wave (narrow_pair's G > 1 branch): the pairs of the
+// convex separating-axis routine (and its clipper) -- cylinder and cylinder, box and convex mesh,
+// convex mesh and convex mesh (edge-pair axes in the thousands), box or convex mesh and concave
+// mesh (one test per overlapped triangle) -- and a sphere or cylinder with a convex or concave
+// mesh (the faces its BVH walk finds are tested on the wave's lanes).
+__device__ __forceinline__ bool pair_on_wave(const BodyView& b, const PairStore& ps, int p) {
     const int ta = b.iat(I_TYPE, ps.pa[p]), tb = b.iat(I_TYPE, ps.pb[p]);
     const int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
     return (lo == SHAPE_CYLINDER && hi == SHAPE_CYLINDER) || (lo == SHAPE_BOX && hi == SHAPE_CONVEX) ||
            (lo == SHAPE_CONVEX && hi == SHAPE_CONVEX) ||
-           ((lo == SHAPE_BOX || lo == SHAPE_CONVEX) && hi == SHAPE_CONCAVE);
+           ((lo == SHAPE_BOX || lo == SHAPE_CONVEX) && hi == SHAPE_CONCAVE) ||
+           ((lo == SHAPE_SPHERE || lo == SHAPE_CYLINDER) && (hi == SHAPE_CONVEX || hi == SHAPE_CONCAVE));
 }
 
 // The polyhedral view of a box, cylinder or convex-mesh body.
@@ -352,11 +354,12 @@ __device__ __forceinline__ Poly body_poly(const MeshSet& ms, const CollBody& B) 
     return poly_mesh(&ms, __float_as_int(B.sp[0]));
 }
 // NarrowPhaseBase::getAlgorithm (narrow_phase_base.cpp:20-30) for the pairs that involve a
-// cylinder, a convex mesh or a concave mesh and are not separating-axis pairs (pair_is_sat:
-// those run a wave each, in narrow_pair's G > 1 branch, and never reach this thread-per-pair
-// routine, which so carries neither the separating-axis routine's registers nor its clip
-// buffers), with the role and swap flag each algorithm's processCollision gives the two bodies;
-// the swap flag is cleared again afterwards, as processCollision does.
+// cylinder, a convex mesh or a concave mesh and are not separating-axis pairs, with the role and
+// swap flag each algorithm's processCollision gives the two bodies; the swap flag is cleared again
+// afterwards, as processCollision does.  G = 1: one lane (the thread-per-pair kernels, where only
+// box or sphere with cylinder arrive, and compound children); G = WAVE: the sphere and cylinder
+// pairs with a mesh that pair_on_wave sends to a whole wave.
+template <int G = 1>
 static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, PairCtx& x, const CollBody& A,
                                                   const CollBody& B) {
     const int ta = A.type, tb = B.type;
@@ -375,20 +378,20 @@ static __device__ __noinline__ void narrow_shapes(const MeshSet& ms, Cache& c, P
         const CollBody& sp = ta == SHAPE_SPHERE ? A : B;
         const CollBody& mb = ta == SHAPE_SPHERE ? B : A;
         x.swap = ta == SHAPE_CONVEX;
-        sphere_mesh(c, x, ms, __float_as_int(mb.sp[0]), mb.P, sp.P, sp.sp[0]);
+        sphere_mesh<G>(c, x, ms, __float_as_int(mb.sp[0]), mb.P, sp.P, sp.sp[0]);
     } else if (is(SHAPE_CYLINDER, SHAPE_CONVEX)) {           // cylinder_convex_collision_algorithm.cpp:11-51
         const CollBody& cy = ta == SHAPE_CYLINDER ? A : B;
         const CollBody& mb = ta == SHAPE_CYLINDER ? B : A;
         x.swap = ta == SHAPE_CONVEX;
-        cylinder_mesh(c, x, ms, __float_as_int(mb.sp[0]), mb.P, cy.P, cy.sp[0], cy.sp[1]);
+        cylinder_mesh<G>(c, x, ms, __float_as_int(mb.sp[0]), mb.P, cy.P, cy.sp[0], cy.sp[1]);
     } else if ((ta == SHAPE_CONCAVE) != (tb == SHAPE_CONCAVE)) {
         // concave_{box,sphere,cylinder,convex}_collision_algorithm.cpp: the other body is A
         const CollBody& mb = ta == SHAPE_CONCAVE ? A : B;
         const CollBody& ob = ta == SHAPE_CONCAVE ? B : A;
         const int mesh = __float_as_int(mb.sp[0]);
         x.swap = ta == SHAPE_CONCAVE;
-        if (ob.type == SHAPE_SPHERE) sphere_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0]);
-        else if (ob.type == SHAPE_CYLINDER) cylinder_mesh(c, x, ms, mesh, mb.P, ob.P, ob.sp[0], ob.sp[1]);
+        if (ob.type == SHAPE_SPHERE) sphere_mesh<G>(c, x, ms, mesh, mb.P, ob.P, ob.sp[0]);
+        else if (ob.type == SHAPE_CYLINDER) cylinder_mesh<G>(c, x, ms, mesh, mb.P, ob.P, ob.sp[0], ob.sp[1]);
     }
     x.swap = false;
 }
@@ -464,7 +467,8 @@ static __device__ __noinline__ bool narrow_compound(const MeshSet& ms, Cache& c,
 // SHAPES = false handles box and sphere pairs only (the kernel then carries no polyhedral
 // code and keeps its register budget); SHAPES = true handles every pair type.
 // G = 64 (a whole wave on one pair, every lane holding the same values) is for the pairs
-// pair_is_sat selects: the separating-axis loops are split over the lanes and lane 0 writes.
+// pair_on_wave selects: the separating-axis loops, or the mesh faces a BVH walk finds, are split
+// over the lanes, and lane 0 writes.
 // COMPOUND = true is for the pairs with a compound body (one thread each).
 template <bool SHAPES, int G = 1, bool COMPOUND = false>
 static __device__ __forceinline__ void narrow_pair(const BodyView& b, const PairStore& ps, int p, int prev_np) {
@@ -505,7 +509,9 @@ static __device__ __forceinline__ void narrow_pair(const BodyView& b, const Pair
         swapped = narrow_compound(b.ms, c, x, A, B, (x.A.scale + x.B.scale) * 0.05f);
     } else if (G > 1) {
         const float ref_scale = (x.A.scale + x.B.scale) * 0.05f;
-        if (ta == SHAPE_CONCAVE || tb == SHAPE_CONCAVE) {
+        if (ta == SHAPE_SPHERE || tb == SHAPE_SPHERE || (ta == SHAPE_CYLINDER) != (tb == SHAPE_CYLINDER)) {
+            narrow_shapes<G>(b.ms, c, x, A, B);   // a sphere or a cylinder with a mesh
+        } else if (ta == SHAPE_CONCAVE || tb == SHAPE_CONCAVE) {
             // concave_convex_collision_algorithm.cpp: the convex body is A (narrow_shapes)
             const CollBody& mb = ta == SHAPE_CONCAVE ? A : B;
             const CollBody& ob = ta == SHAPE_CONCAVE ? B : A;

@@ -743,6 +743,40 @@ __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& 
     }
 }
 
+// The faces bvh_visit<WAVE> finds, tested by a wave's lanes in batches of WAVE and added in the
+// walk's order.  test(f), on the lane that holds face f, returns the bit mask of f's items (the
+// routine's own units: triangles of a fan, segments, edges) that give contact points; commit(f, i)
+// then runs on every lane with the same values for each such item, face by face in walk order and
+// item by item in ascending order -- the order in which a sequential walk adds the points, so the
+// contact cache ends as it would.  Most faces yield no point, so the tests (most of the work) run
+// in parallel and only the hits are computed again, serially, to be added.
+template <class Test, class Commit>
+__device__ inline void bvh_faces_wave(const MeshSet& ms, int mesh, f3 aa, f3 bb, Test&& test, Commit&& commit) {
+    const int r = (int)(threadIdx.x & (WAVE - 1));
+    int mine = -1, cnt = 0;
+    auto flush = [&]() {
+        const uint32_t m = r < cnt ? test(mine) : 0u;
+        uint64_t hit = __ballot(m != 0u);
+        while (hit) {
+            const int src = __ffsll((unsigned long long)hit) - 1;
+            hit &= hit - 1;
+            const int f = __shfl(mine, src);
+            uint32_t mk = (uint32_t)__shfl((int)m, src);
+            while (mk) {
+                const int i = __ffs(mk) - 1;
+                mk &= mk - 1;
+                commit(f, i);
+            }
+        }
+        cnt = 0;
+    };
+    bvh_visit<WAVE>(ms, mesh, aa, bb, [&](int f) {
+        if (r == cnt) mine = f;
+        if (++cnt == WAVE) flush();
+    });
+    if (cnt > 0) flush();
+}
+
 // ------------------------------------------------------------------ cylinder-mesh
 // intersectSegmentFace (cylinder_convex_collision_algorithm.cpp:53-107)
 PE_HD bool seg_face(const Poly& M, int f, f3 ps, f3 sd, float l, float& t1, float& t2, float& d1, float& d2) {
@@ -779,76 +813,139 @@ PE_HD bool seg_face(const Poly& M, int f, f3 ps, f3 sd, float l, float& t1, floa
     if (-d1 > mind + 0.0f || -d2 > mind + 0.0f) return false;
     return true;
 }
-// CylinderConvexCollisionAlgorithm::getClosestPoints (:109-203) for the mesh of a convex or
-// concave body (pm: mesh body pose).
+// CylinderConvexCollisionAlgorithm::getClosestPoints (:109-203), split into its items.  The
+// cylinder in the mesh's frame: axis ax, centre pc, half height h.
+struct CylInMesh {
+    f3 ax, pc, aa, bb;
+    float rad, h;
+};
+__device__ __forceinline__ CylInMesh cyl_in_mesh(const Pose& pm, const Pose& pcy, float rad, float height) {
+    CylInMesh q;
+    q.h = height / 2.0f;
+    q.rad = rad;
+    const Pose rel = pose_mul(pose_inv(pm), pcy);
+    q.ax = col(rel.R, 1);
+    q.pc = rel.p;
+    const float sp[4] = {rad, height, 0.0f, 0.0f};
+    shape_aabb(SHAPE_CYLINDER, sp, rel.R, rel.p, q.aa, q.bb);
+    return q;
+}
+// First walk, face f: the cylinder's section in the plane of the axis and the face normal (ux
+// across, the axis up; uz the second section when the axis is along the normal).  Item i < 4 is
+// side i of the section, items 4 and 5 the sides of the second section along the axis.
+struct CylFace {
+    f3 ux, uz;
+    bool vert;
+};
+PE_HD CylFace cyl_face_frame(const Poly& M, int f, f3 ax) {
+    const f3 fn = face_normal(M, f);
+    const float adc = dot(ax, fn);
+    CylFace F;
+    F.uz = f3{0, 0, 0};
+    F.vert = false;
+    if (fabsf(fabsf(adc) - 1) < EPS) {
+        F.uz = normalized(cross(ax, f3{1.0f, 0.0f, 0.0f}));
+        F.ux = normalized(cross(ax, F.uz));
+        F.vert = true;
+    } else {
+        F.ux = normalized(cross(cross(fn, ax), ax));
+    }
+    return F;
+}
+PE_HD int cyl_face_items(const CylFace& F) { return F.vert ? 6 : 4; }
+PE_HD void cyl_face_seg(const CylFace& F, const CylInMesh& q, int i, f3& ps, f3& sd, float& l) {
+    constexpr int dx[4] = {1, -1, -1, 1};
+    constexpr int dy[4] = {1, 1, -1, -1};
+    const f3 uy = q.ax;
+    if (i < 4) {
+        ps = add(add(q.pc, mul(mul(F.ux, (float)dx[i]), q.rad)), mul(mul(uy, (float)dy[i]), q.h));
+        sd = mul(dx[i] == dy[i] ? uy : F.ux, (float)-dx[i]);
+        l = (dx[i] == dy[i] ? q.h : q.rad) * 2;
+    } else {
+        const int j = (i - 4) * 2;
+        ps = add(add(q.pc, mul(mul(F.uz, (float)dx[j]), q.rad)), mul(mul(uy, (float)dy[j]), q.h));
+        sd = mul(uy, (float)-dx[j]);
+        l = q.h * 2;
+    }
+}
+PE_HD bool cyl_face_hit(const Poly& M, int f, const CylFace& F, const CylInMesh& q, int i) {
+    f3 ps, sd;
+    float l, t1, t2, d1, d2;
+    cyl_face_seg(F, q, i, ps, sd, l);
+    return seg_face(M, f, ps, sd, l, t1, t2, d1, d2);
+}
+PE_HD void cyl_face_add(Cache& c, const PairCtx& x, const Poly& M, int f, const CylFace& F, const CylInMesh& q,
+                        int i, const Pose& pm) {
+    f3 ps, sd;
+    float l, t1, t2, d1, d2;
+    cyl_face_seg(F, q, i, ps, sd, l);
+    if (seg_face(M, f, ps, sd, l, t1, t2, d1, d2)) {
+        const f3 fn = face_normal(M, f);
+        const f3 p1 = xf_apply(pm, sub(add(ps, mul(sd, t1)), mul(fn, d1)));
+        const f3 p2 = xf_apply(pm, sub(add(ps, mul(sd, t2)), mul(fn, d2)));
+        const f3 n = mv(pm.R, fn);
+        add_point(c, x, n, sub(p1, mul(n, 0.0f)), d1 + 0.0f * 2);
+        add_point(c, x, n, sub(p2, mul(n, 0.0f)), d2 + 0.0f * 2);
+    }
+}
+// Second walk, face f: item i is the face's edge from vertex i to vertex i + 1.
+PE_HD bool cyl_edge_seg(const Poly& M, int f, int i, const CylInMesh& q, f3& ss, f3& sd, float& t1, float& t2) {
+    const int nf = face_len(M, f);
+    const f3 v1 = face_vertex(M, f, i);
+    const f3 v2 = face_vertex(M, f, (i + 1) % nf);
+    ss = divs(add(v1, v2), 2.0f);
+    const float l = norm(sub(v2, v1)) / 2.0f;
+    sd = divs(sub(v2, v1), l * 2);
+    return seg_cyl(ss, sd, l, q.pc, q.ax, q.h, q.rad, 0.0f, t1, t2);
+}
+PE_HD void cyl_edge_add(Cache& c, const PairCtx& x, const Poly& M, int f, int i, const CylInMesh& q, const Pose& pm) {
+    f3 ss, sd;
+    float t1, t2;
+    if (cyl_edge_seg(M, f, i, q, ss, sd, t1, t2)) {
+        const f3 p1 = add(ss, mul(sd, t1));
+        const f3 p2 = add(ss, mul(sd, t2));
+        const f3 pmid = divs(add(p1, p2), 2.0f);
+        point_on_cyl(c, x, pmid, q.pc, q.ax, q.rad, q.h, pm);
+    }
+}
+// The whole routine for the mesh of a convex or concave body (pm: mesh body pose): G = 1 on one
+// lane; G = WAVE on a wave's lanes, the faces of each walk split over them (bvh_faces_wave).
+template <int G = 1>
 __device__ __noinline__ void cylinder_mesh(Cache& c, const PairCtx& x, const MeshSet& ms, int mesh, const Pose& pm,
                                            const Pose& pcy, float rad, float height) {
     const Poly M = poly_mesh(&ms, mesh);
-    const float h = height / 2.0f;
-    const Pose rel = pose_mul(pose_inv(pm), pcy);
-    const f3 ax = col(rel.R, 1);
-    const f3 pc = rel.p;
-    f3 aa, bb;
-    {
-        const float sp[4] = {rad, height, 0.0f, 0.0f};
-        shape_aabb(SHAPE_CYLINDER, sp, rel.R, rel.p, aa, bb);
+    const CylInMesh q = cyl_in_mesh(pm, pcy, rad, height);
+    if (G == 1) {
+        bvh_visit(ms, mesh, q.aa, q.bb, [&](int f) {
+            const CylFace F = cyl_face_frame(M, f, q.ax);
+            for (int i = 0; i < cyl_face_items(F); i++) cyl_face_add(c, x, M, f, F, q, i, pm);
+        });
+        bvh_visit(ms, mesh, q.aa, q.bb, [&](int f) {
+            const int nf = face_len(M, f);
+            for (int i = 0; i < nf; i++) cyl_edge_add(c, x, M, f, i, q, pm);
+        });
+    } else {
+        bvh_faces_wave(ms, mesh, q.aa, q.bb,
+            [&](int f) {
+                const CylFace F = cyl_face_frame(M, f, q.ax);
+                uint32_t m = 0;
+                for (int i = 0; i < cyl_face_items(F); i++) m |= (cyl_face_hit(M, f, F, q, i) ? 1u : 0u) << i;
+                return m;
+            },
+            [&](int f, int i) { cyl_face_add(c, x, M, f, cyl_face_frame(M, f, q.ax), q, i, pm); });
+        bvh_faces_wave(ms, mesh, q.aa, q.bb,
+            [&](int f) {
+                const int nf = face_len(M, f);
+                uint32_t m = 0;
+                for (int i = 0; i < nf; i++) {
+                    f3 ss, sd;
+                    float t1, t2;
+                    m |= (cyl_edge_seg(M, f, i, q, ss, sd, t1, t2) ? 1u : 0u) << i;
+                }
+                return m;
+            },
+            [&](int f, int i) { cyl_edge_add(c, x, M, f, i, q, pm); });
     }
-    const int dx[4] = {1, -1, -1, 1};
-    const int dy[4] = {1, 1, -1, -1};
-    auto add_seg = [&](int f, f3 ps, f3 sd, float l) {
-        float t1, t2, d1, d2;
-        if (seg_face(M, f, ps, sd, l, t1, t2, d1, d2)) {
-            const f3 fn = face_normal(M, f);
-            const f3 p1 = xf_apply(pm, sub(add(ps, mul(sd, t1)), mul(fn, d1)));
-            const f3 p2 = xf_apply(pm, sub(add(ps, mul(sd, t2)), mul(fn, d2)));
-            const f3 n = mv(pm.R, fn);
-            add_point(c, x, n, sub(p1, mul(n, 0.0f)), d1 + 0.0f * 2);
-            add_point(c, x, n, sub(p2, mul(n, 0.0f)), d2 + 0.0f * 2);
-        }
-    };
-    bvh_visit(ms, mesh, aa, bb, [&](int f) {
-        const f3 fn = face_normal(M, f);
-        const float adc = dot(ax, fn);
-        f3 ux, uy = ax, uz{0, 0, 0};
-        bool vert = false;
-        if (fabsf(fabsf(adc) - 1) < EPS) {
-            uz = normalized(cross(ax, f3{1.0f, 0.0f, 0.0f}));
-            ux = normalized(cross(ax, uz));
-            vert = true;
-        } else {
-            ux = normalized(cross(cross(fn, ax), ax));
-        }
-        for (int i = 0; i < 4; i++) {
-            const f3 ps = add(add(pc, mul(mul(ux, (float)dx[i]), rad)), mul(mul(uy, (float)dy[i]), h));
-            const f3 sd = mul(dx[i] == dy[i] ? uy : ux, (float)-dx[i]);
-            const float l = (dx[i] == dy[i] ? h : rad) * 2;
-            add_seg(f, ps, sd, l);
-        }
-        if (vert) {
-            for (int i = 0; i < 3; i += 2) {
-                const f3 ps = add(add(pc, mul(mul(uz, (float)dx[i]), rad)), mul(mul(uy, (float)dy[i]), h));
-                const f3 sd = mul(uy, (float)-dx[i]);
-                add_seg(f, ps, sd, h * 2);
-            }
-        }
-    });
-    bvh_visit(ms, mesh, aa, bb, [&](int f) {
-        const int nf = face_len(M, f);
-        for (int i = 0; i < nf; i++) {
-            const f3 v1 = face_vertex(M, f, i);
-            const f3 v2 = face_vertex(M, f, (i + 1) % nf);
-            float t1, t2;
-            const f3 ss = divs(add(v1, v2), 2.0f);
-            const float l = norm(sub(v2, v1)) / 2.0f;
-            const f3 sd = divs(sub(v2, v1), l * 2);
-            if (seg_cyl(ss, sd, l, pc, ax, h, rad, 0.0f, t1, t2)) {
-                const f3 p1 = add(ss, mul(sd, t1));
-                const f3 p2 = add(ss, mul(sd, t2));
-                const f3 pmid = divs(add(p1, p2), 2.0f);
-                point_on_cyl(c, x, pmid, pc, ax, rad, h, pm);
-            }
-        }
-    });
 }
 
 // ------------------------------------------------------------------ sphere-mesh
@@ -920,25 +1017,47 @@ PE_HD bool sphere_tri(f3 cc, float rad, const f3* vt, f3& point, f3& rn, float& 
     return false;
 }
 // SphereConvexCollisionAlgorithm::getClosestPoints (:165-211) for the mesh of a convex or
-// concave body: every face of every BVH leaf overlapping the sphere's box, fan-triangulated.
+// concave body: every face of every BVH leaf overlapping the sphere's box, fan-triangulated; item
+// i of face f is the fan's triangle (0, i + 1, i + 2).  G as for cylinder_mesh.
+PE_HD bool sphere_fan_tri(const Poly& M, int f, int i, f3 cc, float rad, f3& p, f3& n, float& depth) {
+    const f3 vt[3] = {face_vertex(M, f, 0), face_vertex(M, f, i + 1), face_vertex(M, f, i + 2)};
+    depth = 0;
+    return sphere_tri(cc, rad, vt, p, n, depth);
+}
+PE_HD void sphere_fan_add(Cache& c, const PairCtx& x, const Poly& M, int f, int i, f3 cc, float rad, const Pose& pm) {
+    f3 p, n;
+    float depth;
+    if (sphere_fan_tri(M, f, i, cc, rad, p, n, depth)) {
+        n = mv(pm.R, n);
+        p = xf_apply(pm, p);
+        add_point(c, x, n, sub(p, mul(n, 0.0f)), depth + 2 * 0.0f);
+    }
+}
+template <int G = 1>
 __device__ __noinline__ void sphere_mesh(Cache& c, const PairCtx& x, const MeshSet& ms, int mesh, const Pose& pm,
                                          const Pose& psph, float rad) {
     const Poly M = poly_mesh(&ms, mesh);
     const f3 cc = xf_inv(pm, psph.p);
     const f3 aa = sub(cc, f3{rad, rad, rad}), bb = add(cc, f3{rad, rad, rad});
-    bvh_visit(ms, mesh, aa, bb, [&](int f) {
-        const int nf = face_len(M, f);
-        for (int i = 0; i < nf - 2; i++) {
-            const f3 vt[3] = {face_vertex(M, f, 0), face_vertex(M, f, i + 1), face_vertex(M, f, i + 2)};
-            f3 p, n;
-            float depth = 0;
-            if (sphere_tri(cc, rad, vt, p, n, depth)) {
-                n = mv(pm.R, n);
-                p = xf_apply(pm, p);
-                add_point(c, x, n, sub(p, mul(n, 0.0f)), depth + 2 * 0.0f);
-            }
-        }
-    });
+    if (G == 1) {
+        bvh_visit(ms, mesh, aa, bb, [&](int f) {
+            const int nf = face_len(M, f);
+            for (int i = 0; i < nf - 2; i++) sphere_fan_add(c, x, M, f, i, cc, rad, pm);
+        });
+    } else {
+        bvh_faces_wave(ms, mesh, aa, bb,
+            [&](int f) {
+                const int nf = face_len(M, f);
+                uint32_t m = 0;
+                for (int i = 0; i < nf - 2; i++) {
+                    f3 p, n;
+                    float depth;
+                    m |= (sphere_fan_tri(M, f, i, cc, rad, p, n, depth) ? 1u : 0u) << i;
+                }
+                return m;
+            },
+            [&](int f, int i) { sphere_fan_add(c, x, M, f, i, cc, rad, pm); });
+    }
 }
 
 // ------------------------------------------------------------------ concave mesh + polyhedron

@@ -41,6 +41,7 @@ struct Dev {
     PairStore ps;           // pairs [pcap], contact slots and rows [pcap*8]
     SapView sap;            // the broadphase over the body store's AABBs (pairs to ps.pa / ps.pb)
     int32_t* shape_list;    // [pcap] pairs handed from k_narrow to k_narrow_shapes (from the front)
+                            // and to k_narrow_sat (from the back)
     int32_t* comp_list;     // [pcap] pairs with a compound body (worlds with compounds; else null)
     // colouring by rounds: per-body degree offsets [n + 1] and fill cursors [n], adjacency [2 pcap],
     // colour per pair [pcap], pairs per colour [pcap + 1]
@@ -49,7 +50,6 @@ struct Dev {
     int2* jp_adj;        // per dynamic body, its active pairs: (pair, claim priority)
     int32_t* jp_col;
     int32_t* jp_ccount;
-                            // and to k_narrow_sat (from the back)
     int32_t* order;         // [pcap]
     int32_t* act;           // [3*pcap] colouring work lists
     int32_t* color_start;   // [pcap+1]
@@ -258,7 +258,8 @@ __global__ void k_events(Dev d) {
 // ------------------------------------------------------------------ stage 1: broadphase (pe_sap.hpp)
 
 // ------------------------------------------------------------------ stage 2: narrowphase
-// Box and sphere pairs here; pairs with a cylinder or a mesh are listed for k_narrow_shapes.
+// Box and sphere pairs here; pairs with a cylinder or a mesh are listed for k_narrow_sat (a wave
+// each, pair_on_wave) or k_narrow_shapes (a thread each: a box or sphere with a cylinder).
 __global__ void __launch_bounds__(128) k_narrow(Dev d) {
     const int np = d.cnt->npairs;
     const int prev = d.cnt->prev_npairs;
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(128) k_narrow(Dev d) {
         const bool simple = pair_is_simple(d.b, d.ps, p);
         const bool comp = !simple && (d.b.iat(I_TYPE, d.ps.pa[p]) == SHAPE_COMPOUND ||
                                       d.b.iat(I_TYPE, d.ps.pb[p]) == SHAPE_COMPOUND);
-        const bool sat = !simple && !comp && pair_is_sat(d.b, d.ps, p);
+        const bool sat = !simple && !comp && pair_on_wave(d.b, d.ps, p);
         const int slot = wave_append(!simple && !sat && !comp, &d.cnt->nshape);
         const int sslot = wave_append(sat, &d.cnt->nsat);
         const int cslot = wave_append(comp && d.comp_list, &d.cnt->ncomp);
@@ -284,7 +285,8 @@ __global__ void __launch_bounds__(64) k_narrow_shapes(Dev d) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < ns; q += gridDim.x * blockDim.x)
         narrow_pair<true>(d.b, d.ps, d.shape_list[q], prev);
 }
-// One wave per listed separating-axis pair.
+// One wave per listed wave pair (pair_on_wave: separating-axis pairs, a sphere or cylinder with a
+// mesh).
 __global__ void __launch_bounds__(256) k_narrow_sat(Dev d) {
     const int ns = d.cnt->nsat;
     const int prev = d.cnt->prev_npairs;

@@ -83,9 +83,10 @@ def test_shape_stages_match_oracle():
     # the heightfield against every other shape, and the polyhedral pairs among bodies
     for t in (0, 1, 2, 3):
         assert (t, 4) in seen
-    # the pairs solved one wave each by the split separating-axis test: cylinder-cylinder,
-    # box-hull, hull-hull (the scene gives all 14 pair types contacts within these steps)
-    for t in ((2, 2), (0, 3), (3, 3)):
+    # the pairs solved one wave each: by the split separating-axis test (cylinder-cylinder,
+    # box-hull, hull-hull) and by the mesh faces tested over the lanes (sphere or cylinder with a
+    # hull or the heightfield); the scene gives all 14 pair types contacts within these steps
+    for t in ((2, 2), (0, 3), (3, 3), (1, 3), (2, 3), (1, 4), (2, 4)):
         assert t in seen
     assert len(seen) == 14
 
