@@ -242,6 +242,21 @@ PE_HD void project(const Poly& S, const Pose& t, f3 axis, float& mn, float& mx, 
         pmn = xf_apply(t, lmn); pmx = xf_apply(t, lmx);
     }
 }
+// The mesh branch of project over nv vertices at v (a wave's copy of a mesh's vertices in LDS).
+PE_HD void project_verts(const float4* v, int nv, const Pose& t, f3 axis, float& mn, float& mx, f3& pmn, f3& pmx) {
+    const f3 la = mtv(t.R, axis);
+    const float off = dot(t.p, axis);
+    mn = REAL_MAX; mx = REAL_MIN;
+    f3 lmn{0, 0, 0}, lmx{0, 0, 0};
+    for (int i = 0; i < nv; i++) {
+        const f3 q = ld4(v[i]);
+        const float x = dot(q, la);
+        if (x < mn) { mn = x; lmn = q; }
+        else if (x > mx) { mx = x; lmx = q; }
+    }
+    mn += off; mx += off;
+    pmn = xf_apply(t, lmn); pmx = xf_apply(t, lmx);
+}
 // Shape::localIsInside -- box_shape.cpp:45-49, cylinder_shape.cpp:45-48, convex_mesh_shape.cpp:152-158
 PE_HD bool inside(const Poly& S, f3 q) {
     if (S.kind == PK_BOX) return fabsf(q.x) <= S.h.x && fabsf(q.y) <= S.h.y && fabsf(q.z) <= S.h.z;
@@ -369,17 +384,35 @@ __device__ __noinline__ void clip_hull_hull(Cache& c, const PairCtx& x, f3 sep, 
     }
 }
 // testSepAxis (:216-243)
-PE_HD bool test_axis(const Poly& A, const Poly& B, const Pose& ta, const Pose& tb, f3 ax, float& depth, f3& wa, f3& wb) {
+// projA(ax, mn, mx, pmn, pmx) and projB: Shape::project of the two shapes.
+template <class PA, class PB>
+PE_HD bool test_axis_with(PA&& projA, PB&& projB, f3 ax, float& depth, f3& wa, f3& wb) {
     float min0, max0, min1, max1;
     f3 pmna, pmxa, pmnb, pmxb;
-    project(A, ta, ax, min0, max0, pmna, pmxa);
-    project(B, tb, ax, min1, max1, pmnb, pmxb);
+    projA(ax, min0, max0, pmna, pmxa);
+    projB(ax, min1, max1, pmnb, pmxb);
     if (max0 < min1 || max1 < min0) return false;
     const float d0 = max0 - min1, d1 = max1 - min0;
     if (d0 < d1) { depth = d0; wa = pmxa; wb = pmnb; }
     else { depth = d1; wa = pmna; wb = pmxb; }
     return true;
 }
+PE_HD bool test_axis(const Poly& A, const Poly& B, const Pose& ta, const Pose& tb, f3 ax, float& depth, f3& wa, f3& wb) {
+    return test_axis_with([&](f3 a, float& mn, float& mx, f3& pn, f3& px) { project(A, ta, a, mn, mx, pn, px); },
+                          [&](f3 a, float& mn, float& mx, f3& pn, f3& px) { project(B, tb, a, mn, mx, pn, px); },
+                          ax, depth, wa, wb);
+}
+// Orders a wave's LDS stores before its later LDS loads of other lanes' words (a wave's LDS
+// instructions execute in order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// A wave's copies of two convex meshes' vertices in LDS (up to MESH_LDS_VERTS each), for the
+// projections of the separating-axis test: every axis projects every vertex of both shapes, and
+// LDS serves a wave's same-address loads faster than the vector memory path.
+constexpr int MESH_LDS_VERTS = 64;
 // isOnLine (:245-249)
 PE_HD bool on_line(f3 v0, f3 v1, f3 v2) {
     const f3 n = cross(sub(v1, v0), sub(v2, v0));
@@ -401,6 +434,31 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
     f3 dmin_pt{0, 0, 0}, sep{0, 0, 0};
     bool pt_on_a = false;
     const int nfa = n_faces(A), nfb = n_faces(B);
+    // with a wave: the vertices of a convex mesh A or B staged in LDS (same values, same order)
+    __shared__ float4 s_vtx[G == 1 ? 1 : 4][2][G == 1 ? 1 : MESH_LDS_VERTS];
+    float4* va = s_vtx[G == 1 ? 0 : (threadIdx.x / WAVE) % 4][0];
+    float4* vb = s_vtx[G == 1 ? 0 : (threadIdx.x / WAVE) % 4][1];
+    const bool sa = G > 1 && A.kind == PK_MESH && A.nv <= MESH_LDS_VERTS;
+    const bool sb = G > 1 && B.kind == PK_MESH && B.nv <= MESH_LDS_VERTS;
+    if (G > 1 && (sa || sb)) {
+        wave_lds_sync();
+        if (sa) for (int i = L::rank(); i < A.nv; i += G) va[i] = A.v[i];
+        if (sb) for (int i = L::rank(); i < B.nv; i += G) vb[i] = B.v[i];
+        wave_lds_sync();
+    }
+    auto test_axis = [&](const Poly& A, const Poly& B, const Pose& ta, const Pose& tb, f3 ax, float& depth, f3& wa,
+                         f3& wb) {
+        return test_axis_with(
+            [&](f3 a, float& mn, float& mx, f3& pn, f3& px) {
+                if (sa) project_verts(va, A.nv, ta, a, mn, mx, pn, px);
+                else project(A, ta, a, mn, mx, pn, px);
+            },
+            [&](f3 a, float& mn, float& mx, f3& pn, f3& px) {
+                if (sb) project_verts(vb, B.nv, tb, a, mn, mx, pn, px);
+                else project(B, tb, a, mn, mx, pn, px);
+            },
+            ax, depth, wa, wb);
+    };
     {
         float bd = REAL_MAX;
         int bi = 0x7fffffff;
