@@ -124,3 +124,63 @@ def test_shape_worlds_batched_equal_alone():
         bb = b.world_state(k)
         for x, y in zip(a, bb):
             assert np.array_equal(x, y), f"world {k}"
+
+
+def long_walk_scene():
+    """Spheres and cylinders whose BVH walks find more faces than a wave has lanes: a 1 m sphere
+    and a 0.8 m cylinder on a fine heightfield (0.15 m cells: hundreds of triangles under each), a
+    lying cylinder, a box and a sphere on a kinematic hull of more than 64 vertices (its
+    separating-axis projections read the mesh from global memory rather than a wave's LDS copy)."""
+    from pe_amd import scenes
+    rng = np.random.default_rng(77)
+    b = scenes.SceneBuilder()
+    hf = b.add_mesh(scenes.heightfield_mesh(rng, 40, 40, (6.0, 6.0), 0.05))
+    b.add(scenes.SHAPE_CONCAVE, (), 1.0, (0.0, 0.0, 0.0), kin=1, mesh=hf)
+    pts = rng.standard_normal((120, 3))
+    big = scenes.hull_mesh(1.2 * pts / np.linalg.norm(pts, axis=1)[:, None])   # every point a vertex
+    assert len(big.vertices) > 64
+    vol = abs(sum(np.dot(big.vertices[f[0]], np.cross(big.vertices[f[1]], big.vertices[f[2]])) for f in big.faces)) / 6.0
+    b.add(scenes.SHAPE_CONVEX, (), 1000.0 * vol, (-1.6, 1.25, 0.0), kin=1, mesh=b.add_mesh(big))
+    b.add(scenes.SHAPE_SPHERE, (1.0,), 4188.8, (1.5, 1.1, 1.5))
+    b.add(scenes.SHAPE_CYLINDER, (0.8, 0.8), 1608.5, (1.5, 0.5, -1.5))
+    lying = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])   # axis along -x
+    b.add(scenes.SHAPE_CYLINDER, (0.5, 1.6), 1256.6, (-1.6, 3.0, 0.0), lying)
+    b.add(scenes.SHAPE_BOX, (0.6, 0.6, 0.6), 216.0, (-1.0, 3.2, 0.8))
+    b.add(scenes.SHAPE_SPHERE, (0.5,), 523.6, (-1.6, 3.2, -1.05))
+    return b.scene(name="long_walks")
+
+
+def test_mesh_walks_longer_than_a_wave_match_oracle():
+    """The wave routines hand a walk's faces to the lanes 64 at a time; walks of hundreds of faces
+    take several batches and must add their points in the sequential walk's order."""
+    import pe_amd
+    from pyoracle import OracleWorld
+    sc = long_walk_scene()
+    g, o = pe_amd.World(sc), OracleWorld(sc, 32)
+    T = sc.shape_type
+    seen = set()
+    print(f"[long walks] n={sc.n} meshes={len(sc.meshes)}", flush=True)
+    for step in range(90):
+        g.integrate(); o.integrate()
+        g.broadphase(); o.broadphase()
+        gp, op = g.pairs(), o.pairs()
+        assert np.array_equal(gp, op), f"pair order step {step}"
+        g.narrowphase(); o.narrowphase()
+        gn, gpts = g.contacts()
+        on, _, opts = o.contacts()
+        bad = np.nonzero(gn != on)[0]
+        assert len(bad) == 0, f"point counts step {step}: pairs {bad[:5]} types {[tuple(T[gp[k]]) for k in bad[:5]]}"
+        vm = np.arange(8)[None, :] < on[:, None]
+        assert _close(gpts[vm], opts[vm]), f"contact points step {step}"
+        for k in np.nonzero(on)[0]:
+            seen.add(tuple(sorted((int(T[gp[k][0]]), int(T[gp[k][1]])))))
+        g.solve(); o.solve()
+        g.sync()
+        _, _, gv, gw = g.state()
+        _, _, ov, ow = o.state()
+        assert _close(gv, ov) and _close(gw, ow), f"velocities step {step}"
+        o.set_state(*[np.asarray(x, np.float64) for x in g.state()])
+    print(f"[long walks] contact pair types seen: {sorted(seen)}", flush=True)
+    for t in ((1, 4), (2, 4), (0, 3), (1, 3), (2, 3)):
+        assert t in seen
+
