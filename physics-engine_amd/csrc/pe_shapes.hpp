@@ -767,14 +767,48 @@ __device__ __noinline__ void box_cylinder(Cache& c, const PairCtx& x, const Pose
 // The stack lives in LDS (a private array would live in scratch memory, whose every push and
 // pop is a round trip beyond L1): one row per lane when each lane walks its own pair (G = 1,
 // kernels of at most BVH_ROWS threads per workgroup), one row per wave when the wave's lanes walk
-// one pair together (G = WAVE: every lane writes the same value to the same word; kernels of at
-// most BVH_ROWS waves per workgroup).
+// one pair together (G = WAVE, bvh_visit_wave: every lane writes the same value to the same word;
+// kernels of at most 4 waves per workgroup).
 constexpr int BVH_ROWS = 64;
-template <int G>
 __device__ __forceinline__ int* bvh_stack_row() {
-    __shared__ int s_stack[(G == 1 ? BVH_ROWS : 4) * (BVH_STACK + 1)];   // rows padded to an odd length
-    const int row = G == 1 ? (int)(threadIdx.x % BVH_ROWS) : (int)(threadIdx.x / WAVE) % 4;
-    return s_stack + row * (BVH_STACK + 1);
+    __shared__ int s_stack[BVH_ROWS * (BVH_STACK + 1)];   // rows padded to an odd length
+    return s_stack + (int)(threadIdx.x % BVH_ROWS) * (BVH_STACK + 1);
+}
+// The walk of a wave (G = WAVE): a stack entry carries what its node's record and leaf entry hold
+// (a leaf's face range; an inner node's children), read when the node was tested as its parent's
+// child.  A node then costs one round of dependent loads -- its children's records, when it is an
+// inner node -- rather than one for its own record and one more for its leaf entry, and a child
+// that misses the box is never pushed.  Children are tested and pushed left then right, so the
+// nodes that pass are popped, and the faces visited, in the same order as by the walk below.
+template <class F>
+__device__ inline void bvh_visit_wave(const float4* node, const int2* leaf, f3 aa, f3 bb, F&& visit) {
+    __shared__ int4 s_st[4][BVH_STACK];   // a row per wave (every lane writes the same value)
+    int4* st = s_st[(threadIdx.x / WAVE) % 4];
+    auto hit = [&](float4 lo, float4 hi) {
+        return !(lo.x > bb.x || lo.y > bb.y || lo.z > bb.z) && !(hi.x < aa.x || hi.y < aa.y || hi.z < aa.z);
+    };
+    int sp = 0;
+    {
+        const float4 lo = node[0], hi = node[1];
+        const int2 lf = leaf[0];
+        if (hit(lo, hi)) st[sp++] = make_int4(lf.x, lf.y, __float_as_int(lo.w), __float_as_int(hi.w));
+    }
+    while (sp > 0) {
+        const int4 e = st[--sp];
+        if (e.x > 0) {
+            for (int i = 0; i < e.x; i++) visit(e.y + i);
+            continue;
+        }
+        const int cl = e.z, cr = e.w;
+        float4 llo{}, lhi{}, rlo{}, rhi{};
+        int2 llf{}, rlf{};
+        if (cl >= 0) { llo = node[2 * cl]; lhi = node[2 * cl + 1]; llf = leaf[cl]; }
+        if (cr >= 0) { rlo = node[2 * cr]; rhi = node[2 * cr + 1]; rlf = leaf[cr]; }
+        if (cl >= 0 && hit(llo, lhi) && sp < BVH_STACK)
+            st[sp++] = make_int4(llf.x, llf.y, __float_as_int(llo.w), __float_as_int(lhi.w));
+        if (cr >= 0 && hit(rlo, rhi) && sp < BVH_STACK)
+            st[sp++] = make_int4(rlf.x, rlf.y, __float_as_int(rlo.w), __float_as_int(rhi.w));
+    }
 }
 template <int G = 1, class F>
 __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& visit) {
@@ -782,7 +816,11 @@ __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& 
     if (r[MR_NB] == 0) return;
     const float4* node = ms.arena + r[MR_NODE];
     const int2* leaf = reinterpret_cast<const int2*>(ms.arena + r[MR_LEAF]);
-    int* st = bvh_stack_row<G>();
+    if (G > 1) {
+        bvh_visit_wave(node, leaf, aa, bb, visit);
+        return;
+    }
+    int* st = bvh_stack_row();
     int sp = 0;
     st[sp++] = 0;
     while (sp > 0) {
