@@ -413,6 +413,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // projections of the separating-axis test: every axis projects every vertex of both shapes, and
 // LDS serves a wave's same-address loads faster than the vector memory path.
 constexpr int MESH_LDS_VERTS = 64;
+// The same for the world directions of B's edges (up to EDGE_LDS_DIRS), the operand every
+// edge-pair axis reads: computed once per pair instead of once per axis.
+constexpr int EDGE_LDS_DIRS = 128;
 // isOnLine (:245-249)
 PE_HD bool on_line(f3 v0, f3 v1, f3 v2) {
     const f3 n = cross(sub(v1, v0), sub(v2, v0));
@@ -494,6 +497,20 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
         f3 bsep{0, 0, 0}, bwea{0, 0, 0}, bweb{0, 0, 0}, bwpa{0, 0, 0}, bwpb{0, 0, 0};
         bool separated = false;
         const int ne = nea * neb;
+        // with a wave: B's world edge directions (the inner loop's operand), computed once into LDS
+        __shared__ float4 s_edir[G == 1 ? 1 : 4][G == 1 ? 1 : EDGE_LDS_DIRS];
+        float4* edir = s_edir[G == 1 ? 0 : (threadIdx.x / WAVE) % 4];
+        const bool se = G > 1 && neb <= EDGE_LDS_DIRS;
+        if (G > 1 && se) {
+            wave_lds_sync();
+            for (int e = L::rank(); e < neb; e += G) {
+                f3 b0, b1;
+                edge_of(B, e, b0, b1);
+                const f3 d = mv(tb.R, normalized(sub(b0, b1)));
+                edir[e] = make_float4(d.x, d.y, d.z, 0.0f);
+            }
+            wave_lds_sync();
+        }
         int t = L::rank();
         int e0 = t / neb, e1 = t - e0 * neb;
         int cur0 = -1;
@@ -504,9 +521,14 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
                 we0 = mv(ta.R, normalized(sub(a0, a1)));
                 cur0 = e0;
             }
-            f3 b0, b1;
-            edge_of(B, e1, b0, b1);
-            const f3 we1 = mv(tb.R, normalized(sub(b0, b1)));
+            f3 we1;
+            if (se) {
+                we1 = ld4(edir[e1]);
+            } else {
+                f3 b0, b1;
+                edge_of(B, e1, b0, b1);
+                we1 = mv(tb.R, normalized(sub(b0, b1)));
+            }
             f3 cr = cross(we0, we1);
             if (!(fabsf(norm(cr) - 0.0f) < EPS)) {
                 cr = normalized(cr);
@@ -514,8 +536,12 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
                 float dist;
                 f3 wa, wb;
                 if (!test_axis(A, B, ta, tb, cr, dist, wa, wb)) { separated = true; break; }
-                if (dist < bd && on_line(a0, a1, xf_inv(ta, wa)) && on_line(b0, b1, xf_inv(tb, wb))) {
-                    bd = dist; bi = t; bsep = cr; bwea = we0; bweb = we1; bwpa = wa; bwpb = wb;
+                if (dist < bd && on_line(a0, a1, xf_inv(ta, wa))) {
+                    f3 b0, b1;
+                    edge_of(B, e1, b0, b1);
+                    if (on_line(b0, b1, xf_inv(tb, wb))) {
+                        bd = dist; bi = t; bsep = cr; bwea = we0; bweb = we1; bwpa = wa; bwpb = wb;
+                    }
                 }
             }
             e1 += G;
@@ -780,10 +806,13 @@ __device__ __forceinline__ int* bvh_stack_row() {
 // inner node -- rather than one for its own record and one more for its leaf entry, and a child
 // that misses the box is never pushed.  Children are tested and pushed left then right, so the
 // nodes that pass are popped, and the faces visited, in the same order as by the walk below.
+__device__ __forceinline__ int4* bvh_wave_stack_row() {
+    __shared__ int4 s_st[4][BVH_STACK];   // a row per wave (every lane writes the same value)
+    return s_st[(threadIdx.x / WAVE) % 4];
+}
 template <class F>
 __device__ inline void bvh_visit_wave(const float4* node, const int2* leaf, f3 aa, f3 bb, F&& visit) {
-    __shared__ int4 s_st[4][BVH_STACK];   // a row per wave (every lane writes the same value)
-    int4* st = s_st[(threadIdx.x / WAVE) % 4];
+    int4* st = bvh_wave_stack_row();
     auto hit = [&](float4 lo, float4 hi) {
         return !(lo.x > bb.x || lo.y > bb.y || lo.z > bb.z) && !(hi.x < aa.x || hi.y < aa.y || hi.z < aa.z);
     };
