@@ -413,8 +413,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // projections of the separating-axis test: every axis projects every vertex of both shapes, and
 // LDS serves a wave's same-address loads faster than the vector memory path.
 constexpr int MESH_LDS_VERTS = 64;
-// The same for the world directions of B's edges (up to EDGE_LDS_DIRS), the operand every
-// edge-pair axis reads: computed once per pair instead of once per axis.
+// The same for the world directions of both shapes' edges (up to EDGE_LDS_DIRS each), the operands
+// every edge-pair axis reads: computed once per pair instead of once per axis.
 constexpr int EDGE_LDS_DIRS = 128;
 // isOnLine (:245-249)
 PE_HD bool on_line(f3 v0, f3 v1, f3 v2) {
@@ -498,16 +498,28 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
         bool separated = false;
         const int ne = nea * neb;
         // with a wave: B's world edge directions (the inner loop's operand), computed once into LDS
-        __shared__ float4 s_edir[G == 1 ? 1 : 4][G == 1 ? 1 : EDGE_LDS_DIRS];
-        float4* edir = s_edir[G == 1 ? 0 : (threadIdx.x / WAVE) % 4];
+        __shared__ float4 s_edir[G == 1 ? 1 : 4][2][G == 1 ? 1 : EDGE_LDS_DIRS];
+        float4* adir = s_edir[G == 1 ? 0 : (threadIdx.x / WAVE) % 4][0];
+        float4* edir = s_edir[G == 1 ? 0 : (threadIdx.x / WAVE) % 4][1];
+        const bool sea = G > 1 && nea <= EDGE_LDS_DIRS;
         const bool se = G > 1 && neb <= EDGE_LDS_DIRS;
-        if (G > 1 && se) {
+        if (G > 1 && (sea || se)) {
             wave_lds_sync();
-            for (int e = L::rank(); e < neb; e += G) {
-                f3 b0, b1;
-                edge_of(B, e, b0, b1);
-                const f3 d = mv(tb.R, normalized(sub(b0, b1)));
-                edir[e] = make_float4(d.x, d.y, d.z, 0.0f);
+            if (sea) {
+                for (int e = L::rank(); e < nea; e += G) {
+                    f3 p0, p1;
+                    edge_of(A, e, p0, p1);
+                    const f3 d = mv(ta.R, normalized(sub(p0, p1)));
+                    adir[e] = make_float4(d.x, d.y, d.z, 0.0f);
+                }
+            }
+            if (se) {
+                for (int e = L::rank(); e < neb; e += G) {
+                    f3 b0, b1;
+                    edge_of(B, e, b0, b1);
+                    const f3 d = mv(tb.R, normalized(sub(b0, b1)));
+                    edir[e] = make_float4(d.x, d.y, d.z, 0.0f);
+                }
             }
             wave_lds_sync();
         }
@@ -517,8 +529,12 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
         f3 a0, a1, we0;
         for (; t < ne; t += G) {
             if (e0 != cur0) {
-                edge_of(A, e0, a0, a1);
-                we0 = mv(ta.R, normalized(sub(a0, a1)));
+                if (sea) {
+                    we0 = ld4(adir[e0]);
+                } else {
+                    edge_of(A, e0, a0, a1);
+                    we0 = mv(ta.R, normalized(sub(a0, a1)));
+                }
                 cur0 = e0;
             }
             f3 we1;
@@ -536,6 +552,9 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
                 float dist;
                 f3 wa, wb;
                 if (!test_axis(A, B, ta, tb, cr, dist, wa, wb)) { separated = true; break; }
+                if (dist < bd) {
+                    if (sea) edge_of(A, e0, a0, a1);
+                }
                 if (dist < bd && on_line(a0, a1, xf_inv(ta, wa))) {
                     f3 b0, b1;
                     edge_of(B, e1, b0, b1);
