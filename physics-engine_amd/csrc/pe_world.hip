@@ -286,8 +286,9 @@ __global__ void __launch_bounds__(64) k_narrow_shapes(Dev d) {
         narrow_pair<true>(d.b, d.ps, d.shape_list[q], prev);
 }
 // One wave per listed wave pair (pair_on_wave: separating-axis pairs, a sphere or cylinder with a
-// mesh).
-__global__ void __launch_bounds__(256) k_narrow_sat(Dev d) {
+// mesh).  Built for three waves per SIMD (at most 168 VGPRs; a little spilling): the 50k shape mix
+// took 4.16 ms per step at two waves (248 VGPRs), 4.12 at three, 4.26 at four.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_narrow_sat(Dev d) {
     const int ns = d.cnt->nsat;
     const int prev = d.cnt->prev_npairs;
     const int nw = gridDim.x * (blockDim.x / WAVE);
