@@ -894,10 +894,19 @@ __device__ inline void bvh_visit(const MeshSet& ms, int mesh, f3 aa, f3 bb, F&& 
 // item by item in ascending order -- the order in which a sequential walk adds the points, so the
 // contact cache ends as it would.  Most faces yield no point, so the tests (most of the work) run
 // in parallel and only the hits are computed again, serially, to be added.
+// With rec, the walk's faces are also kept in rec (a wave's LDS row: every lane writes the same
+// value) while there are at most FACE_REC of them, and *nrec is their count: a second walk over the
+// same box then runs over the list (faces_list_wave) instead of the tree.
+constexpr int FACE_REC = 128;
+__device__ __forceinline__ int* face_rec_row() {
+    __shared__ int s_rec[4][FACE_REC];
+    return s_rec[(threadIdx.x / WAVE) % 4];
+}
 template <class Test, class Commit>
-__device__ inline void bvh_faces_wave(const MeshSet& ms, int mesh, f3 aa, f3 bb, Test&& test, Commit&& commit) {
+__device__ inline void bvh_faces_wave(const MeshSet& ms, int mesh, f3 aa, f3 bb, Test&& test, Commit&& commit,
+                                      int* rec = nullptr, int* nrec = nullptr) {
     const int r = (int)(threadIdx.x & (WAVE - 1));
-    int mine = -1, cnt = 0;
+    int mine = -1, cnt = 0, all = 0;
     auto flush = [&]() {
         const uint32_t m = r < cnt ? test(mine) : 0u;
         uint64_t hit = __ballot(m != 0u);
@@ -915,10 +924,35 @@ __device__ inline void bvh_faces_wave(const MeshSet& ms, int mesh, f3 aa, f3 bb,
         cnt = 0;
     };
     bvh_visit<WAVE>(ms, mesh, aa, bb, [&](int f) {
+        if (rec && all < FACE_REC) rec[all] = f;
+        all++;
         if (r == cnt) mine = f;
         if (++cnt == WAVE) flush();
     });
     if (cnt > 0) flush();
+    if (nrec) *nrec = all;
+}
+// The same tests and commits over faces already found: rec[0 .. n), in order.
+template <class Test, class Commit>
+__device__ inline void faces_list_wave(const int* rec, int n, Test&& test, Commit&& commit) {
+    const int r = (int)(threadIdx.x & (WAVE - 1));
+    for (int base = 0; base < n; base += WAVE) {
+        const int cnt = n - base < WAVE ? n - base : WAVE;
+        const int mine = r < cnt ? rec[base + r] : -1;
+        const uint32_t m = r < cnt ? test(mine) : 0u;
+        uint64_t hit = __ballot(m != 0u);
+        while (hit) {
+            const int src = __ffsll((unsigned long long)hit) - 1;
+            hit &= hit - 1;
+            const int f = __shfl(mine, src);
+            uint32_t mk = (uint32_t)__shfl((int)m, src);
+            while (mk) {
+                const int i = __ffs(mk) - 1;
+                mk &= mk - 1;
+                commit(f, i);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ cylinder-mesh
@@ -1069,6 +1103,10 @@ __device__ __noinline__ void cylinder_mesh(Cache& c, const PairCtx& x, const Mes
             for (int i = 0; i < nf; i++) cyl_edge_add(c, x, M, f, i, q, pm);
         });
     } else {
+        // the second walk is over the same box: it visits the faces the first one found, in the same
+        // order, so it runs over their list when the first walk could keep it
+        int* rec = face_rec_row();
+        int nrec = 0;
         bvh_faces_wave(ms, mesh, q.aa, q.bb,
             [&](int f) {
                 const CylFace F = cyl_face_frame(M, f, q.ax);
@@ -1076,19 +1114,20 @@ __device__ __noinline__ void cylinder_mesh(Cache& c, const PairCtx& x, const Mes
                 for (int i = 0; i < cyl_face_items(F); i++) m |= (cyl_face_hit(M, f, F, q, i) ? 1u : 0u) << i;
                 return m;
             },
-            [&](int f, int i) { cyl_face_add(c, x, M, f, cyl_face_frame(M, f, q.ax), q, i, pm); });
-        bvh_faces_wave(ms, mesh, q.aa, q.bb,
-            [&](int f) {
-                const int nf = face_len(M, f);
-                uint32_t m = 0;
-                for (int i = 0; i < nf; i++) {
-                    f3 ss, sd;
-                    float t1, t2;
-                    m |= (cyl_edge_seg(M, f, i, q, ss, sd, t1, t2) ? 1u : 0u) << i;
-                }
-                return m;
-            },
-            [&](int f, int i) { cyl_edge_add(c, x, M, f, i, q, pm); });
+            [&](int f, int i) { cyl_face_add(c, x, M, f, cyl_face_frame(M, f, q.ax), q, i, pm); }, rec, &nrec);
+        auto edge_test = [&](int f) {
+            const int nf = face_len(M, f);
+            uint32_t m = 0;
+            for (int i = 0; i < nf; i++) {
+                f3 ss, sd;
+                float t1, t2;
+                m |= (cyl_edge_seg(M, f, i, q, ss, sd, t1, t2) ? 1u : 0u) << i;
+            }
+            return m;
+        };
+        auto edge_commit = [&](int f, int i) { cyl_edge_add(c, x, M, f, i, q, pm); };
+        if (nrec <= FACE_REC) faces_list_wave(rec, nrec, edge_test, edge_commit);
+        else bvh_faces_wave(ms, mesh, q.aa, q.bb, edge_test, edge_commit);
     }
 }
 
