@@ -202,13 +202,18 @@ static __global__ void k_sap_chunks(SapView s) {
             uint32_t hit = s.chunk_hit[c];
             const uint32_t t0 = t;
             int cur = q;
+            uint32_t cb = s.cand[cur], nb = s.cand[cur + 1];
             while (hit) {
                 const uint32_t tt = t0 + (uint32_t)(__ffs(hit) - 1);
                 hit &= hit - 1;
-                while (s.cand[cur + 1] <= tt) cur++;
+                while (nb <= tt) {
+                    cur++;
+                    cb = nb;
+                    nb = s.cand[cur + 1];
+                }
                 if ((int)out < s.pcap) {
                     s.pa[out] = (int)s.vals[cur];
-                    s.pb[out] = (int)s.vals[cur + 1 + (int)(tt - s.cand[cur])];
+                    s.pb[out] = (int)s.vals[cur + 1 + (int)(tt - cb)];
                 }
                 out++;
             }
@@ -216,14 +221,19 @@ static __global__ void k_sap_chunks(SapView s) {
             uint32_t cntv = 0, hit = 0;
             const uint32_t t0 = t;
             int cur = q;
+            // the run's first candidate and the next run's, in registers: a candidate's record is then
+            // addressed without a load of the offsets (only a change of run reads them)
+            uint32_t cb = s.cand[cur], nb = s.cand[cur + 1];
             float A[8], B[8];
             sap_ld_sorted(s, cur, A);   // the run body's record stays in registers
             for (; t < tend; t++) {
-                while (s.cand[cur + 1] <= t) {
+                while (nb <= t) {
                     cur++;
+                    cb = nb;
+                    nb = s.cand[cur + 1];
                     sap_ld_sorted(s, cur, A);
                 }
-                const int qj = cur + 1 + (int)(t - s.cand[cur]);
+                const int qj = cur + 1 + (int)(t - cb);
                 sap_ld_sorted(s, qj, B);
                 if (sap_pair_allowed_sorted(s, A, B, cur, qj)) {
                     cntv++;

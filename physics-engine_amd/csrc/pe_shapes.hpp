@@ -397,11 +397,6 @@ PE_HD bool test_axis_with(PA&& projA, PB&& projB, f3 ax, float& depth, f3& wa, f
     else { depth = d1; wa = pmna; wb = pmxb; }
     return true;
 }
-PE_HD bool test_axis(const Poly& A, const Poly& B, const Pose& ta, const Pose& tb, f3 ax, float& depth, f3& wa, f3& wb) {
-    return test_axis_with([&](f3 a, float& mn, float& mx, f3& pn, f3& px) { project(A, ta, a, mn, mx, pn, px); },
-                          [&](f3 a, float& mn, float& mx, f3& pn, f3& px) { project(B, tb, a, mn, mx, pn, px); },
-                          ax, depth, wa, wb);
-}
 // Orders a wave's LDS stores before its later LDS loads of other lanes' words (a wave's LDS
 // instructions execute in order; this keeps the compiler from moving them across).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -449,6 +444,7 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
         if (sb) for (int i = L::rank(); i < B.nv; i += G) vb[i] = B.v[i];
         wave_lds_sync();
     }
+    // testSepAxis, with a mesh's vertices from the wave's LDS copy when there is one
     auto test_axis = [&](const Poly& A, const Poly& B, const Pose& ta, const Pose& tb, f3 ax, float& depth, f3& wa,
                          f3& wb) {
         return test_axis_with(
