@@ -424,8 +424,10 @@ PE_HD bool on_line(f3 v0, f3 v1, f3 v2) {
 // those the scan would accept (an edge axis only below the face depth and with both witness
 // points on their edges), which is the one the sequential scan ends with.
 template <int G>
+// keep_a (a wave only): the wave's LDS already holds A's vertex copy, staged by its previous call
+// with the same A (the triangles of one concave pair), so it is not staged again.
 __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Poly& A, const Poly& B, const Pose& ta,
-                                           const Pose& tb, float ref_scale) {
+                                           const Pose& tb, float ref_scale, bool keep_a = false) {
     using L = Grp<G>;
     const f3 dc = sub(ta.p, tb.p);
     float dmin = REAL_MAX;
@@ -438,9 +440,9 @@ __device__ __noinline__ void convex_convex(Cache& c, const PairCtx& x, const Pol
     float4* vb = s_vtx[G == 1 ? 0 : (threadIdx.x / WAVE) % 4][1];
     const bool sa = G > 1 && A.kind == PK_MESH && A.nv <= MESH_LDS_VERTS;
     const bool sb = G > 1 && B.kind == PK_MESH && B.nv <= MESH_LDS_VERTS;
-    if (G > 1 && (sa || sb)) {
+    if (G > 1 && ((sa && !keep_a) || sb)) {
         wave_lds_sync();
-        if (sa) for (int i = L::rank(); i < A.nv; i += G) va[i] = A.v[i];
+        if (sa && !keep_a) for (int i = L::rank(); i < A.nv; i += G) va[i] = A.v[i];
         if (sb) for (int i = L::rank(); i < B.nv; i += G) vb[i] = B.v[i];
         wave_lds_sync();
     }
@@ -1263,6 +1265,7 @@ __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const Mesh
         bb = add(bb, rel.p);
     }
     const Poly M = poly_mesh(&ms, mesh);
+    bool staged = false;   // a wave copies A's vertices to LDS once per pair (convex_convex's keep_a)
     bvh_visit<G>(ms, mesh, aa, bb, [&](int f) {
         Poly T;
         T.kind = PK_TRI;
@@ -1271,7 +1274,8 @@ __device__ __noinline__ void concave_poly(Cache& c, const PairCtx& x, const Mesh
         T.t[1] = face_vertex(M, f, 1);
         T.t[2] = face_vertex(M, f, 2);
         T.tn = face_normal(M, f);
-        convex_convex<G>(c, x, A, T, pa, pm, ref_scale);
+        convex_convex<G>(c, x, A, T, pa, pm, ref_scale, staged);
+        staged = G > 1;
     });
 }
 
